@@ -1,0 +1,216 @@
+#!/usr/bin/env python3
+"""World-step benchmark: grid cell-steps per second on one MI355X.
+
+Workload (the headline configuration): a 2048 x 2048 grid world with a
+1024 x 1024 fluid, FP32 fluid and multigrid pressure, exact u32 monomer
+transport.  Inputs are synthetic and seeded: 16-mode sinusoidal velocity
+at 0.5 fluid cells per step, ~15 % Gaussian-blob rock plus a rubble shell,
+Poisson(4) monomers in 8 channels, a 32-grid-row (16 fluid rows) vent band.
+Parameters are the reference defaults with the parity profile (fluid and
+transport every step, no UV, no solar heating).  A step is one full world
+step: fluid (temperature, forces, viscosity, advection, vorticity,
+divergence, multigrid pressure, projection, net-flux, walls, thermal
+activation) then transport (aggregation, scatter, commit) with the exact
+per-channel ledger checked on the device every step.
+
+Prints one JSON line.  The timed region covers exactly --steps steps with the
+world's stream drained on both sides; inputs are resident in HBM before it.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "ribossome-v2_amd"))
+
+import numpy as np  # noqa: E402
+
+import ribossome_amd as ra  # noqa: E402
+
+GRID, FLUID = 2048, 1024
+SEED = 2
+METRIC = "world cell-steps/s at 2048^2 (FP32, HBM %)"
+
+# Algorithmic bytes per unit of work of each kernel (reads + writes of the
+# arrays it must touch once; stencil halos and gathers counted once), per
+# fluid cell (F) or grid cell (G).  These are the per-kernel terms of the
+# byte model of the world step.
+BYTES_PER_UNIT = {
+    "update_temperature": (56, "F"),
+    "buoyancy_gather": (12, "F"),
+    "add_forces": (52, "F"),
+    "diffuse_velocity": (20, "F"),
+    "advect_velocity": (20, "F"),
+    "vorticity": (20, "F"),
+    "divergence": (16, "F"),
+    "subtract_gradient": (24, "F"),
+    "net_flux": (12, "F"),
+    "enforce_boundaries": (20, "F"),
+    "thermal_activation": (17, "G"),
+    "agg": (20, "G"),
+    "clear_next": (16, "G"),
+    "transport_scatter": (64, "G"),
+    "ledger": (16, "G"),
+}
+
+
+def world_inputs(cfg):
+    gamma = ra.gen_gamma(cfg, SEED, 0.15, True)
+    vel = ra.gen_velocity(cfg, SEED, 0.5 / 0.017, gamma)
+    chem = ra.gen_chem(cfg, SEED, 4.0, gamma)
+    heat = ra.gen_vent_heat(cfg, 16, 5000.0 * 0.3)
+    return gamma, vel, chem, heat
+
+
+def configure(p, s, args):
+    p.transport_every = 1
+    p.uv_strength = 0.0
+    p.sun_heat = 0.0
+    s.fluid_substep = 1
+    s.mg_cycles = args.mg_cycles
+    s.ledger_check = 1
+    return p, s
+
+
+def build_world(cfg, inputs, args):
+    gamma, vel, chem, heat = inputs
+    w = ra.World(cfg, SEED)
+    p, s = configure(ra.SimParams.default(SEED), ra.WorldSettings.default(), args)
+    w.params = p
+    w.upload(ra.GAMMA, gamma)
+    w.upload(ra.VELOCITY, vel)
+    w.upload(ra.CHEM, chem)
+    w.upload(ra.HEAT_SRC, heat)
+    w.settings = s
+    return w
+
+
+def cpu_baseline(cfg, inputs, args):
+    """The CPU oracle (C++ restatement of the reference WGSL, OpenMP over all
+    threads it is given) on a bounded sample of the same workload: the same
+    2048^2 world, `--cpu-steps` steps from the same inputs."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as oc
+
+    gamma, vel, chem, heat = inputs
+    ow = oc.OracleWorld(cfg, SEED)
+    p, s = configure(ow.params, ra.WorldSettings.default(), args)
+    ow.params = p
+    ow.set_settings(s)
+    ow.upload(ra.GAMMA, gamma)
+    ow.upload(ra.VELOCITY, vel)
+    ow.upload(ra.CHEM, chem)
+    ow.upload(ra.HEAT_SRC, heat)
+    ow.encode_steps(1)  # first-touch warm-up
+    t0 = time.perf_counter()
+    ow.run(args.cpu_steps)
+    dt = time.perf_counter() - t0
+    cores = oc.lib().oracle_get_threads()
+    return {
+        "value": cfg.cells() * args.cpu_steps / dt,
+        "unit": "cell-steps/s",
+        "cores": int(cores),
+        "kind": "port",
+        "sample": f"{args.cpu_steps} steps of the same 2048^2 world after 1 warm-up step, "
+                  f"C++ oracle, {cores} OpenMP threads, {dt:.2f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=64)
+    ap.add_argument("--mg-cycles", type=int, default=1)
+    ap.add_argument("--profile-steps", type=int, default=32)
+    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the world step is a single-device path here; --gpus {args.gpus} "
+              "is not supported (independent worlds would shard one per GPU).", file=sys.stderr)
+        sys.exit(2)
+
+    cfg = ra.WorldConfig(GRID, FLUID)
+    inputs = world_inputs(cfg)
+    w = build_world(cfg, inputs, args)
+    seeded = ra.Ledger.from_cells(inputs[2])
+
+    w.run(args.warmup)
+    w.sync()
+    t0 = time.perf_counter()
+    w.run(args.steps)
+    w.sync()
+    dt = time.perf_counter() - t0
+    bad = w.ledger_status()
+    if bad is not None:
+        raise SystemExit(f"per-channel monomer totals changed at epoch {bad}")
+    if w.ledger_blocking().channels() != seeded.channels():
+        raise SystemExit("ledger differs from the seeded totals")
+    value = cfg.cells() * args.steps / dt
+    ms = 1000.0 * dt / args.steps
+
+    # Per-kernel HIP-event timing pass (events on the world's stream).
+    w.set_timing(True)
+    w.reset_kernel_times()
+    w.run(args.profile_steps)
+    times = w.kernel_times()
+    w.set_timing(False)
+    dom = max(times, key=lambda k: times[k][0])
+    tot_ms, launches = times[dom]
+    avg_s = tot_ms / launches / 1000.0
+    per, unit = BYTES_PER_UNIT.get(dom, (0, "G"))
+    units = cfg.cells() if unit == "G" else cfg.fluid_cells()
+    achieved = per * units / avg_s / 1e9 if per else None
+    roofline = {
+        "kernel": dom,
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": 8000.0,
+        "unit": "GB/s",
+        "frac": (achieved / 8000.0) if achieved else None,
+        "traffic": None,
+        "avg_launch_us": avg_s * 1e6,
+        "bytes_per_unit": per,
+        "per": unit,
+    }
+    kernels = {k: round(v[0] / max(v[1], 1) * 1000.0, 2) for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(cfg, inputs, args)
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": "world step, 2048^2 grid / 1024^2 fluid, FP32 multigrid (reference V-cycle schedule), "
+                        "15% rock + rubble, 16-row vent, Poisson(4) monomers, exact ledger every step",
+            "grid": GRID,
+            "fluid": FLUID,
+            "mg_levels": len(w.mg_levels()),
+            "mg_cycles": args.mg_cycles,
+        },
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernel_avg_us": kernels,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

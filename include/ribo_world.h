@@ -1,0 +1,359 @@
+/*
+ * ribo_world.h - C ABI of the MI355X world-step library (libribo_world.so).
+ *
+ * The library owns one world per handle: the monomer grid, the terrain, the
+ * fluid fields and the multigrid hierarchy, all resident in HBM, and advances
+ * it with hand-written HIP kernels for gfx950 on one HIP stream per world.
+ *
+ * Every entry point returns an int status (RIBO_OK = 0, negative on error)
+ * and never aborts; ribo_last_error() returns a thread-local message for the
+ * last failing call on the calling thread.  Host pointers are borrowed for
+ * the duration of the call only (copy in / copy out).  A handle is not
+ * thread-safe: use one world from one thread at a time.
+ *
+ * Reference interface this boundary replaces (manalokos/Ribossome-v2):
+ *   World::new / seed_matter / encode_steps / encode_ledger / ledger_blocking /
+ *   read_cells_blocking / read_f32_blocking      src/world/mod.rs:416,969,1406,1661,1834,1807,1802
+ *   SimParams (repr(C), 116 x 4 B = 464 B)        src/params.rs:86-487, defaults :553-690
+ *   MgLevel                                        src/params.rs:727-740
+ *   WorldConfig                                    src/params.rs:1003-1034
+ *   WorldSettings                                  src/world/mod.rs:91-134
+ *   buffer layout of groups 1 / 2 / 4              shaders/world/bindings.wgsl:4-62,
+ *                                                  shaders/world/multigrid.wgsl:19-23
+ */
+#ifndef RIBO_WORLD_H
+#define RIBO_WORLD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define RIBO_API __attribute__((visibility("default")))
+#else
+#define RIBO_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RIBO_ABI_VERSION 1u
+
+/* Opaque world handle. */
+typedef struct RiboWorld RiboWorld;
+
+enum {
+    RIBO_OK = 0,
+    RIBO_ERR_ARG = -1,          /* bad handle, size or enum value */
+    RIBO_ERR_HIP = -2,          /* a HIP runtime call failed */
+    RIBO_ERR_NO_DEVICE = -3,    /* no gfx950 device visible */
+    RIBO_ERR_UNSUPPORTED = -4,  /* a setting outside the world-step path */
+    RIBO_ERR_LEDGER = -5,       /* the exact per-channel ledger check failed */
+    RIBO_ERR_OOM = -6           /* device allocation failed */
+};
+
+/* WorldConfig (src/params.rs:1003-1034).  grid_size must be a multiple of
+ * fluid_size (the reference runs 2048/1024 and 256/128). */
+typedef struct RiboWorldConfig {
+    uint32_t grid_size;
+    uint32_t fluid_size;
+    uint32_t world_units_per_cell;
+    uint32_t max_agents;        /* accepted for layout parity; agents are not simulated */
+} RiboWorldConfig;
+
+/* SimParams, byte-identical to the reference's repr(C) struct
+ * (src/params.rs:86-487): same 116 fields in the same order, 464 bytes.
+ * Only the fields the world step reads are used; the rest are carried. */
+typedef struct RiboSimParams {
+    uint32_t epoch;
+    uint32_t seed;
+    float dt;
+    float diffusion;
+    float settle;
+    float fluid_dt;
+    float fluid_decay;
+    float fluid_vorticity;
+    float fluid_viscosity;
+    uint32_t fumarole_count;
+    float uv_strength;
+    float uv_depth;
+    uint32_t fluid_enabled;
+    float fluid_obstacle_strength;
+    float slope_steer_rate;
+    float cohesion;
+    uint32_t spawn_count;
+    uint32_t max_agents;
+    float death_probability;
+    float spawn_energy;
+    float food_power;
+    float maintenance_cost;
+    float pairing_rate;
+    float mutation_rate;
+    float uv_damage;
+    float uptake_rate;
+    float brownian;
+    float phoretic_gain;
+    float chain_stiffness;
+    float thermal_kt;
+    float motor_amplitude;
+    uint32_t rft_enabled;
+    uint32_t require_start;
+    float joint_coupling;
+    uint32_t hunger_regulation;
+    float pairing_cost;
+    float reactivation_rate;
+    float swim_gain;
+    float bioturbation;
+    float bioturbation_cost;
+    float monomer_uv_absorb;
+    float sedimentation;
+    float direct_photoactivation;
+    float activation_decay;
+    float monomer_pressure;
+    float agent_fluid_push;
+    float swim_wobble;
+    uint32_t fluid_swim_only;
+    float motion_cost;
+    float bond_rate;
+    float bond_energy_share;
+    float bond_signal;
+    float aggregation;
+    float photo_yield;
+    float heat_kill;
+    float flow_coupling;
+    float inertia;
+    float death_energy_cap;
+    float chemo_yield;
+    float redox_decay;
+    float thermal_activation;
+    float denature_temp;
+    float day_period;
+    float sun_slope;
+    float sun_now;
+    float sun_angle;
+    float day_fraction;
+    float metabolic_q10;
+    float metabolic_span;
+    float metabolic_ref;
+    float sun_heat;
+    float sediment_transport;
+    float sediment_threshold;
+    float sediment_settle;
+    float signal_mode;
+    float clock_mute;
+    float paint_x;
+    float paint_y;
+    float paint_radius;
+    float paint_grains;
+    float rest_angle_mult;
+    float protease_power;
+    float lysis_yield;
+    float protease_direct;
+    float joint_load;
+    float swim_grip;
+    float swim_memory;
+    float mouth_leak;
+    float leak_base;
+    float skin_uptake;
+    float death_metab;
+    float salvage;
+    float bond_matter_share;
+    float brownian_rot;
+    uint32_t transport_every;
+    float signal_crosstalk;
+    float flow_mass;
+    float chemo_take;
+    float vent_cycle_period;
+    float vent_cycle_amp;
+    float sediment_cohesion;
+    float sediment_compaction;
+    float rock_crumble;
+    float lysis_matter;
+    uint32_t scramble_conduction;
+    uint32_t scramble_angles;
+    uint32_t scramble_response;
+    float scramble_amount;
+    float pairing_energy_half;
+    float pairing_min_energy;
+    float uv_mutation;
+    float heat_mutation;
+    float birth_crowding;
+    float photo_damage;
+    float naked_decay;
+    float store_loss;
+} RiboSimParams;
+
+/* MgLevel (src/params.rs:727-740): one multigrid level of the pressure
+ * hierarchy; sizes halve from fluid_size while > mg_min_size and even. */
+typedef struct RiboMgLevel {
+    uint32_t n, off, n_c, off_c;
+    float eps;
+    uint32_t _pad0, _pad1, _pad2;
+} RiboMgLevel;
+
+/* WorldSettings (src/world/mod.rs:91-134) plus the solver extensions.
+ * terrain_enabled must be 0: grain dynamics are not part of the world-step
+ * path, the terrain (gamma) is a frozen input and its slope is derived from
+ * it on upload.  Light is a frozen input (RIBO_BUF_LIGHT); light_* settings
+ * and contact_enabled are accepted and ignored. */
+typedef struct RiboWorldSettings {
+    uint32_t fluid_enabled;       /* reference default 1 */
+    uint32_t fluid_substep;       /* reference default 2, clamped to 1..4 */
+    uint32_t multigrid;           /* 1 = multigrid, 0 = Jacobi ping-pong */
+    uint32_t mg_cycles;           /* V-cycles per fluid step (default 1) */
+    uint32_t jacobi_iters;        /* rounded up to even, clamped 2..128 */
+    uint32_t light_rows_per_step; /* ignored */
+    uint32_t light_interval;      /* ignored */
+    uint32_t terrain_enabled;     /* must be 0 */
+    uint32_t contact_enabled;     /* ignored */
+    /* --- extensions (not in the reference) --- */
+    uint32_t mg_min_size;         /* coarsest level bound, reference MG_MIN_SIZE = 8 */
+    float    mg_tol;              /* 0: run exactly mg_cycles (reference behaviour);
+                                     > 0: run V-cycles until ||r||2/||rhs||2 <= mg_tol
+                                     over fluid cells of level 0, at most mg_max_cycles */
+    uint32_t mg_max_cycles;
+    uint32_t precision;           /* RIBO_PRECISION_FP32 (drop-in) or _MIXED */
+    uint32_t ledger_check;        /* 1: exact per-channel totals checked on the
+                                     device after every transport pass */
+    uint32_t _pad0, _pad1;
+} RiboWorldSettings;
+
+enum { RIBO_PRECISION_FP32 = 0, RIBO_PRECISION_MIXED = 1 };
+
+/* Buffers a caller can move in or out.  Sizes in bytes (G = grid_size^2,
+ * F = fluid_size^2, L = (grid_size/2)^2):
+ *   CHEM        u32[4G]  chem_grid, (y*grid+x)*4+ch, low 16 bits activated,
+ *                        high 16 spent (bindings.wgsl:5-7, chem.wgsl:6-7)
+ *   GAMMA       u32[G]   grains per grid cell, >= 3 is rock (bindings.wgsl:10-11)
+ *   LIGHT       f32[2L]  UV light, 2 bands (bindings.wgsl:12-14); frozen input
+ *   SLOPE       f32[2G]  terrain slope, derived from GAMMA (download only)
+ *   VELOCITY    f32[2F]  velocity_a, fluid cells per second (bindings.wgsl:21)
+ *   PRESSURE    f32[F]   pressure_a (warm start of the solve)
+ *   TEMP        f32[F]   temperature (temp_in)
+ *   REDOX       f32[F]   reductant (redox_in)
+ *   HEAT_SRC    f32[2F]  vent (heat, reductant) source per fluid cell
+ *   DIVERGENCE  f32[F]   last divergence (download only)
+ *   SOLID_MASK  u32[F]   fluid wall mask, 1 = solid (download only)
+ *   NET_FLUX    f32[2*fluid_size] row means of v_y then column means of v_x
+ *   REDOX_EATEN u32[F]   fixed-point reductant eaten since the last fluid step
+ *   AGG         u32[G]   activated count in each 3x3 block (agg_nb + agg_act;
+ *                        download only, valid after a transport pass) */
+typedef enum RiboBuffer {
+    RIBO_BUF_CHEM = 0,
+    RIBO_BUF_GAMMA = 1,
+    RIBO_BUF_LIGHT = 2,
+    RIBO_BUF_SLOPE = 3,
+    RIBO_BUF_VELOCITY = 4,
+    RIBO_BUF_PRESSURE = 5,
+    RIBO_BUF_TEMP = 6,
+    RIBO_BUF_REDOX = 7,
+    RIBO_BUF_HEAT_SRC = 8,
+    RIBO_BUF_DIVERGENCE = 9,
+    RIBO_BUF_SOLID_MASK = 10,
+    RIBO_BUF_NET_FLUX = 11,
+    RIBO_BUF_REDOX_EATEN = 12,
+    RIBO_BUF_AGG = 13,
+    RIBO_BUF_COUNT = 14
+} RiboBuffer;
+
+/* Maximum steps one ribo_world_encode_steps call records
+ * (MAX_STEPS_PER_FRAME, src/world/mod.rs:16). */
+#define RIBO_MAX_STEPS_PER_CALL 64u
+
+/* ---- defaults ---------------------------------------------------------- */
+
+/* SimParams::default() (src/params.rs:553-690), with seed = 1. */
+RIBO_API void ribo_sim_params_default(RiboSimParams* out);
+/* WorldSettings::default() (src/world/mod.rs:121-134) except terrain_enabled
+ * = 0, plus mg_min_size 8, mg_tol 0, mg_max_cycles 1, precision FP32. */
+RIBO_API void ribo_world_settings_default(RiboWorldSettings* out);
+/* Byte size of a buffer for a config (0 for an unknown buffer). */
+RIBO_API size_t ribo_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which);
+
+/* ---- lifetime (World::new, src/world/mod.rs:416-932) --------------------- */
+
+/* Creates a world on the current HIP device: every hot-path buffer, the
+ * multigrid level table and the stream.  All state starts at zero; params =
+ * SimParams::default() with the given seed; settings = defaults above. */
+RIBO_API int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out);
+RIBO_API int ribo_world_destroy(RiboWorld* w);
+
+/* World.params / World.settings (pub fields, src/world/mod.rs:205-207). */
+RIBO_API int ribo_world_set_params(RiboWorld* w, const RiboSimParams* p);
+RIBO_API int ribo_world_get_params(const RiboWorld* w, RiboSimParams* p);
+RIBO_API int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s);
+RIBO_API int ribo_world_get_settings(const RiboWorld* w, RiboWorldSettings* s);
+/* Multigrid level table (src/world/mod.rs:635-671): writes up to cap levels,
+ * returns the level count in *count. */
+RIBO_API int ribo_world_mg_levels(const RiboWorld* w, RiboMgLevel* out, uint32_t cap, uint32_t* count);
+
+/* ---- data movement (queue.write_buffer / read_*_blocking) ---------------- */
+
+/* Copies host bytes into a world buffer (synchronous). bytes must equal
+ * ribo_buffer_bytes().  Uploading GAMMA rebuilds the derived terrain data
+ * (slope, wall mask, fill, drag, wall normals, multigrid masks). */
+RIBO_API int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t bytes);
+/* Copies a world buffer out to host memory after the world's queued work. */
+RIBO_API int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes);
+
+/* ---- stepping (World::encode_steps, src/world/mod.rs:1406-1632) --------- */
+
+/* Records min(steps, RIBO_MAX_STEPS_PER_CALL) world steps on the world's
+ * stream, asynchronously, with the reference's per-step epoch, fluid_substep
+ * and transport_every semantics, and advances params.epoch by that count.
+ * Returns the number of steps recorded (>= 0) or a negative status. */
+RIBO_API int ribo_world_encode_steps(RiboWorld* w, uint32_t steps);
+/* Convenience: records `steps` steps in chunks of RIBO_MAX_STEPS_PER_CALL
+ * (the tests' run() loop, tests/conservation.rs:14-24). */
+RIBO_API int ribo_world_run(RiboWorld* w, uint32_t steps);
+/* Waits for the world's stream (gpu.wait_idle, src/gpu/mod.rs:50-52). */
+RIBO_API int ribo_world_sync(RiboWorld* w);
+
+/* ---- ledger (Ledger, src/world/mod.rs:39-82; ledger.wgsl:5-27) ---------- */
+
+/* Exact free-matter ledger of the current grid: out[0..3] activated A U G C,
+ * out[4..7] spent, out[8..11] held in agents (always 0 here).  u32 wrap. */
+RIBO_API int ribo_world_ledger(RiboWorld* w, uint32_t out[12]);
+/* With settings.ledger_check = 1: returns RIBO_OK if every checked step kept
+ * the per-channel totals seen when checking was armed, else RIBO_ERR_LEDGER
+ * and the first failing epoch in *epoch (0xFFFFFFFF when none). */
+RIBO_API int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch);
+
+/* ---- measurement (KernelTiming / profile_kernels.rs) -------------------- */
+
+/* Enables per-kernel HIP-event timing on the world's stream (off by default;
+ * it adds an event pair per launch). */
+RIBO_API int ribo_world_set_timing(RiboWorld* w, int enable);
+/* Per-kernel totals since the last reset: names[i] (static strings), total
+ * milliseconds, launch counts.  *count returns the number of kernels. */
+RIBO_API int ribo_world_kernel_times(RiboWorld* w, const char** names, double* total_ms,
+                            uint64_t* launches, uint32_t cap, uint32_t* count);
+RIBO_API int ribo_world_reset_kernel_times(RiboWorld* w);
+/* Last multigrid statistics of the world: V-cycles run in the last fluid
+ * step and the final relative residual (tolerance mode only, else -1). */
+RIBO_API int ribo_world_mg_stats(RiboWorld* w, uint32_t* cycles, double* rel_residual);
+
+/* ---- errors -------------------------------------------------------------- */
+RIBO_API const char* ribo_last_error(void);
+RIBO_API uint32_t ribo_abi_version(void);
+
+/* ---- synthetic inputs (host only; no device needed) ---------------------- */
+
+/* Deterministic generators for the benchmark worlds (SplitMix64 seeded):
+ *   velocity: sum of `modes` sinusoid modes scaled so max |v| = vmax (fluid
+ *             cells per second), zero in wall cells.  out f32[2F]
+ *   gamma:    Gaussian blobs thresholded to rock_frac rock cells (6 grains),
+ *             plus a rubble shell (1-2 grains, 1-3 cells) when rubble != 0.
+ *   chem:     8 Poisson(lambda) counts per cell (4 bases x act/spent),
+ *             0 in rock, each clamped to 0xFFFF.  out u32[4G]
+ *   heat:     vent band: (h, h) on fluid rows [0, rows).  out f32[2F] */
+RIBO_API int ribo_gen_gamma(const RiboWorldConfig* cfg, uint64_t seed, float rock_frac, int rubble, uint32_t* out);
+RIBO_API int ribo_gen_velocity(const RiboWorldConfig* cfg, uint64_t seed, uint32_t modes, float vmax,
+                      const uint32_t* gamma, float* out);
+RIBO_API int ribo_gen_chem(const RiboWorldConfig* cfg, uint64_t seed, float lambda, const uint32_t* gamma, uint32_t* out);
+RIBO_API int ribo_gen_vent_heat(const RiboWorldConfig* cfg, uint32_t rows, float h, float* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

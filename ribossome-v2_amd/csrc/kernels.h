@@ -1,0 +1,104 @@
+// kernels.h - device data layout and kernel launchers of the world step.
+#ifndef RIBO_KERNELS_H
+#define RIBO_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/ribo_world.h"
+
+namespace ribo {
+
+// Bits of the per-fluid-cell flag byte derived from the terrain.
+enum : uint8_t {
+    CF_SOLID = 1,   // fluid wall (solid_mask = 1)
+    CF_NORMAL = 2,  // solid_normal_from_neighbors is non-zero
+};
+
+// Terrain-derived, per-world constant data (rebuilt when gamma changes) and
+// the sizes every kernel needs.  Passed by value.
+struct Geo {
+    uint32_t G, F, Lsz, wupc;
+    float sim_size;
+    const uint32_t* gamma;    // u32[G]
+    const float* light;       // f32[2 * Lsz^2]
+    const float2* slope;      // f32x2[G], compute_gamma_slope
+    const float2* slope_f;    // slope at env_cell_for_fluid, f32x2[F]
+    const uint8_t* cflags;    // CF_* per fluid cell
+    const float2* normal;     // solid_normal_from_neighbors per fluid cell
+    const float* fill;        // gamma_fill_at_fluid_cell per fluid cell
+    const float* dragg;       // rubble grain fraction g of rubble_drag per fluid cell
+    const float2* heat;       // heat_src, f32x2[F]
+};
+
+// One multigrid level resident in HBM.
+struct MgLevelDev {
+    uint32_t n;
+    float eps;
+    float* p;
+    float* rhs;
+    const uint8_t* fluid;
+};
+
+// Device scratch of the multigrid residual tracking and ledger check.
+struct Counters {
+    double rr;          // sum of squared residuals (level 0)
+    double bb;          // sum of squared rhs (level 0)
+    uint32_t blocks_done;
+    uint32_t cycles;    // V-cycles run in the current fluid step
+    uint32_t done;      // tolerance reached
+    uint32_t pad;
+    double last_rel;    // last relative residual
+};
+
+struct LedgerDev {
+    uint32_t acc[8];        // act[4], spent[4] of the current reduction
+    uint32_t blocks_done;
+    uint32_t armed;         // 1 once expected[] is set
+    uint32_t expected[4];   // per-channel act + spent
+    uint32_t first_bad;     // first epoch whose totals differed (0xFFFFFFFF none)
+    uint32_t last[8];       // last completed reduction
+    uint32_t pad;
+};
+
+// ---- launchers (kernels.hip) ----
+// terrain-derived data
+void launch_slope(hipStream_t s, const Geo& g, float2* slope);
+void launch_fluid_static(hipStream_t s, const Geo& g, uint8_t* cflags, float* fill, float* dragg, float2* slope_f);
+void launch_fluid_normals(hipStream_t s, const Geo& g, uint8_t* cflags, float2* normal);
+void launch_mg_mask0(hipStream_t s, const Geo& g, uint8_t* fluid0);
+void launch_mg_mask_coarsen(hipStream_t s, uint32_t n, const uint8_t* fine, uint8_t* coarse);
+void launch_mask_u32(hipStream_t s, const Geo& g, uint32_t* out);
+
+// fluid step
+void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vel,
+                        const float* t_in, const float* r_in, float* t_out, float* r_out,
+                        uint32_t* eaten, int eaten_active, const uint32_t* chem);
+void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, float2* forces);
+void launch_add_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* forces,
+                       const float2* vin, float2* vout);
+void launch_diffuse(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vin, float2* vout);
+void launch_advect(hipStream_t s, const Geo& g, const RiboSimParams& P, float decay, const float2* vin, float2* vout);
+void launch_vorticity(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vin, float2* vout);
+void launch_divergence(hipStream_t s, const Geo& g, const float2* vin, float* div, float* p0);
+void launch_jacobi(hipStream_t s, const Geo& g, const float* pin, float* pout, const float* div);
+void launch_mg_smooth(hipStream_t s, const MgLevelDev& L, uint32_t color, const uint32_t* done);
+void launch_mg_restrict(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
+void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
+void launch_mg_rhs_norm(hipStream_t s, const MgLevelDev& L0, Counters* c);
+void launch_mg_residual(hipStream_t s, const MgLevelDev& L0, Counters* c, float tol, uint32_t max_cycles, int count_cycle);
+void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
+void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
+void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);
+void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, uint32_t* chem);
+
+// transport
+void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e);
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
+void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t* next,
+                    const float2* vel, const float* temp, const uint32_t* agg_e);
+void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, int check);
+
+}  // namespace ribo
+
+#endif

@@ -1,0 +1,1195 @@
+// kernels.hip - HIP kernels of the world step for gfx950 (MI355X).
+//
+// Every kernel computes each output value with the same sequence of
+// binary32 operations as the reference WGSL (the file:line next to each
+// kernel), compiled without contraction and with correctly rounded divide
+// and sqrt, so the results are bit-identical to a faithful restatement.
+// What differs from the reference is the data path: terrain-derived values
+// (wall mask, wall normals, slope at the fluid sample, grain fill, rubble
+// fraction, multigrid masks) are computed once per terrain instead of once
+// per use; the temperature ping-pongs instead of being copied; the level-0
+// multigrid arrays alias the pressure and divergence fields; force
+// quantisation and gathering fuse into one pass.  All buffers are
+// HBM-resident.  The work is stencils, gathers, integer scatter and
+// reductions: nothing here is a dense contraction, so no MFMA.
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "kernels.h"
+#include "wgsl_sem.h"
+
+namespace ribo {
+namespace {
+
+// ---- constants (fluid.wgsl:15-56, transport.wgsl:20-51, rng.wgsl:25-33,
+// chem.wgsl:6-7, common/terrain.wgsl:4, light.wgsl:18-40,181) ----------------
+constexpr float MAX_DT = 0.07f;
+constexpr float MAX_FORCE = 200000.0f;
+constexpr float MAX_VEL = 2000.0f;
+constexpr float FORCE_SMOOTH_MIX = 0.25f;
+constexpr float SOLID_PERM_THRESHOLD = 0.02f;
+constexpr float TEMP_HEAT_RATE = 0.01f;
+constexpr float TEMP_COOL_RATE = 0.12f;
+constexpr float TEMP_MAX = 12.0f;
+constexpr float SUN_HEAT_RATE = 0.15f;
+constexpr float SUN_IR = 1.2f;
+constexpr float SUN_IR_DEPTH = 0.08f;
+constexpr float TEMP_BUOYANCY = 12.0f;
+constexpr float TEMP_AMBIENT_SURFACE = 0.0f;
+constexpr float TEMP_AMBIENT_ATTEN = 4.0f;
+constexpr float TEMP_DIFFUSE = 0.08f;
+constexpr float TEMP_ROCK_CONDUCT = 4.0f;
+constexpr float REDOX_RATE = 0.01f;
+constexpr float VENT_WAVES = 1.5f;
+constexpr float REDOX_FP = 10000.0f;
+constexpr float REDOX_MAX = 50.0f;
+constexpr float RUBBLE_DRAG = 300.0f;
+constexpr float FORCE_FP = 256.0f;
+constexpr float DIFF_HOP_P = 0.01f;
+constexpr float DIFF_HOP_FLOOR = 0.15f;
+constexpr float DIFF_AGITATION_SPEED = 0.5f;
+constexpr float BURIED_DIFF_FACTOR = 1.0f;
+constexpr float GAMMA_POROSITY_K = 0.3f;
+constexpr float PRESSURE_HOP_P = 0.01f;
+constexpr float RUBBLE_DISPERSION = 1.0f;
+constexpr float CHEM_SQUEEZE_P = 0.25f;
+constexpr float MONOMER_SETTLE_P = 0.002f;
+constexpr uint32_t MAX_MOVERS_PER_CH = 256u;
+constexpr float LIGHT_ACT_P = 0.006f;
+constexpr float CHEM_SENSITIZE = 0.5f;
+constexpr float CHEM_SHIELD = 0.5f;
+constexpr float TEMP_ACT_THRESHOLD = 2.0f;
+constexpr float FUMAROLE_ACT_P = 0.15f;
+constexpr uint32_t SMOOTH_CELL = 16u;
+constexpr uint32_t S_PHOTO = 1u;
+constexpr uint32_t S_DECAY = 2u;
+constexpr uint32_t S_THERMAL = 3u;
+constexpr uint32_t S_BUOYANCY = 4u;
+constexpr uint32_t S_MOVE = 1u << 16;
+constexpr uint32_t S_DISPERSE = 6u << 16;
+constexpr uint32_t CHEM_STATE_MASK = 0xFFFFu;
+constexpr uint32_t CHEM_SPENT_ONE = 0x10000u;
+constexpr uint32_t CHEM_CELL_CAP = 48u;
+constexpr uint32_t GAMMA_SOLID = 3u;
+constexpr float UV_SHADOW_ABSORB = 0.6f;
+constexpr int LIGHT_TAPS = 7;
+constexpr float SUN_WATER_ABSORB = 0.05f;
+constexpr uint32_t LIGHT_DIV = 2u;
+constexpr float GAMMA_QUANTUM = 0.25f;
+
+// ---- PCG4D counter RNG (rng.wgsl:10-43; Jarzynski & Olano, JCGT 2020) --------
+struct U4 { uint32_t x, y, z, w; };
+__device__ __forceinline__ U4 pcg4d(U4 v) {
+    v.x = v.x * 1664525u + 1013904223u;
+    v.y = v.y * 1664525u + 1013904223u;
+    v.z = v.z * 1664525u + 1013904223u;
+    v.w = v.w * 1664525u + 1013904223u;
+    v.x += v.y * v.w;
+    v.y += v.z * v.x;
+    v.z += v.x * v.y;
+    v.w += v.y * v.z;
+    v.x ^= v.x >> 16; v.y ^= v.y >> 16; v.z ^= v.z >> 16; v.w ^= v.w >> 16;
+    v.x += v.y * v.w;
+    v.y += v.z * v.x;
+    v.z += v.x * v.y;
+    v.w += v.y * v.z;
+    return v;
+}
+struct F4 { float x, y, z, w; };
+__device__ __forceinline__ F4 u4_to_f4(U4 v) {
+    const float s = 1.0f / 16777216.0f;
+    return {(float)(v.x >> 8) * s, (float)(v.y >> 8) * s, (float)(v.z >> 8) * s, (float)(v.w >> 8) * s};
+}
+__device__ __forceinline__ F4 rng_f4(uint32_t key, uint32_t epoch, uint32_t stream, uint32_t seed) {
+    return u4_to_f4(pcg4d({key, epoch, stream, seed}));
+}
+
+// ---- small vector helpers -------------------------------------------------------
+__device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
+__device__ __forceinline__ float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 sub2(float2 a, float2 b) { return f2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 mul2(float2 a, float s) { return f2(a.x * s, a.y * s); }
+__device__ __forceinline__ float2 mix2(float2 a, float2 b, float t) { return f2(rb_mix(a.x, b.x, t), rb_mix(a.y, b.y, t)); }
+__device__ __forceinline__ float len2(float2 v) { return rb_len2(v.x, v.y); }
+__device__ __forceinline__ float dot2(float2 a, float2 b) { return a.x * b.x + a.y * b.y; }
+
+// fluid.wgsl:70-83
+__device__ __forceinline__ bool is_bad(float x) { return (x != x) || (rb_abs(x) > 1e20f); }
+__device__ __forceinline__ float2 sanitize(float2 v) {
+    if (is_bad(v.x) || is_bad(v.y)) return f2(0.0f, 0.0f);
+    return v;
+}
+__device__ __forceinline__ float2 clamp_len(float2 v, float m) {
+    float len = len2(v);
+    if (len > m) return mul2(v, m / rb_max(len, 1e-12f));
+    return v;
+}
+__device__ __forceinline__ float fluid_dt(const RiboSimParams& P) { return rb_clamp(P.fluid_dt, 0.0f, MAX_DT); }
+__device__ __forceinline__ float temp_ambient_at(uint32_t y, uint32_t F) {  // fluid.wgsl:62-64
+    return TEMP_AMBIENT_SURFACE * rb_expf(-TEMP_AMBIENT_ATTEN * (1.0f - ((float)y + 0.5f) / (float)F));
+}
+__device__ __forceinline__ bool solid(const Geo& g, uint32_t i) { return (g.cflags[i] & CF_SOLID) != 0; }
+// reflect_if_into_solid (fluid.wgsl:219-224) with the precomputed normal.
+__device__ __forceinline__ float2 reflect(const Geo& g, uint32_t i, float2 v) {
+    if (g.cflags[i] & CF_NORMAL) {
+        float2 n = g.normal[i];
+        float d = dot2(v, n);
+        if (len2(n) > 1e-6f && d < 0.0f) return sub2(v, mul2(n, d));
+    }
+    return v;
+}
+__device__ __forceinline__ float2 raw_vel(const Geo& g, const float2* vin, uint32_t i) {  // fluid.wgsl:191-194
+    if (solid(g, i)) return f2(0.0f, 0.0f);
+    return sanitize(vin[i]);
+}
+__device__ __forceinline__ uint32_t env_cell_for_fluid(const Geo& g, uint32_t x, uint32_t y) {  // fluid.wgsl:87-92
+    float s = (float)g.G / (float)g.F;
+    uint32_t gx = rb_f2u(rb_clamp(((float)x + 0.5f) * s, 0.0f, (float)(g.G - 1u)));
+    uint32_t gy = rb_f2u(rb_clamp(((float)y + 0.5f) * s, 0.0f, (float)(g.G - 1u)));
+    return gy * g.G + gx;
+}
+__device__ __forceinline__ uint32_t cell_total(const uint32_t* chem, uint32_t cell) {  // chem.wgsl:17-24
+    uint4 v = *reinterpret_cast<const uint4*>(chem + 4u * cell);
+    return (v.x & CHEM_STATE_MASK) + (v.x >> 16) + (v.y & CHEM_STATE_MASK) + (v.y >> 16) + (v.z & CHEM_STATE_MASK) +
+           (v.z >> 16) + (v.w & CHEM_STATE_MASK) + (v.w >> 16);
+}
+__device__ __forceinline__ uint32_t capacity(uint32_t gam) {  // chem.wgsl:29-33
+    if (gam >= GAMMA_SOLID) return 0u;
+    return CHEM_CELL_CAP * (GAMMA_SOLID - gam) / GAMMA_SOLID;
+}
+
+// ---- terrain-derived data (on gamma upload) ---------------------------------------
+// compute_gamma_slope (terrain.wgsl:104-133)
+__global__ __launch_bounds__(256) void k_slope(Geo g, float2* slope) {
+    const int32_t G = (int32_t)g.G;
+    const int32_t ix = (int32_t)(blockIdx.x * 16 + threadIdx.x), iy = (int32_t)(blockIdx.y * 16 + threadIdx.y);
+    if (ix >= G || iy >= G) return;
+    auto h = [&](int32_t a, int32_t b) {
+        uint32_t x = (uint32_t)rb_clampi(a, 0, G - 1), y = (uint32_t)rb_clampi(b, 0, G - 1);
+        return (float)g.gamma[y * (uint32_t)G + x] * GAMMA_QUANTUM;
+    };
+    float l = h(ix - 1, iy), r = h(ix + 1, iy), b = h(ix, iy - 1), t = h(ix, iy + 1);
+    float bl = h(ix - 1, iy - 1), br = h(ix + 1, iy - 1), tl = h(ix - 1, iy + 1), tr = h(ix + 1, iy + 1);
+    const float k = (float)(1.0 / (4.0 * 1.41421356237));
+    float dx = (r - l) * 0.5f + (tr + br - tl - bl) * k;
+    float dy = (t - b) * 0.5f + (tl + tr - bl - br) * k;
+    slope[(uint32_t)iy * (uint32_t)G + (uint32_t)ix] = f2(dx / (float)g.wupc, dy / (float)g.wupc);
+}
+
+// build_solid_mask (fluid.wgsl:185-189) with permeability and
+// gamma_solidity_at_fluid_cell (:95-109,138-142); gamma_fill_at_fluid_cell
+// (:114-128); the grain fraction of rubble_drag (:150-161); the slope sample
+// of slope_steer_velocity (:166).
+__global__ __launch_bounds__(256) void k_fluid_static(Geo g, uint8_t* cflags, float* fill, float* dragg, float2* slope_f) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= g.F || y >= g.F) return;
+    const uint32_t i = y * g.F + x;
+    const float scale = (float)g.G / (float)g.F;
+    const float gx = ((float)x + 0.5f) * scale, gy = ((float)y + 0.5f) * scale;
+    const float h = scale * 0.25f;
+    uint32_t n_solid = 0;
+    float fl = 0.0f;
+    for (uint32_t s = 0; s < 4; ++s) {
+        float ox = (s & 1u) == 1u ? h : -h;
+        float oy = (s & 2u) == 2u ? h : -h;
+        uint32_t sx = rb_f2u(rb_clamp(gx + ox, 0.0f, (float)(g.G - 1u)));
+        uint32_t sy = rb_f2u(rb_clamp(gy + oy, 0.0f, (float)(g.G - 1u)));
+        uint32_t c = g.gamma[sy * g.G + sx];
+        if (c >= GAMMA_SOLID) n_solid += 1u;
+        fl += (float)rb_minu(c, GAMMA_SOLID) / (float)GAMMA_SOLID;
+    }
+    const float solidity = (float)n_solid * 0.25f;
+    const float perm = solidity >= 0.5f ? 0.0f : rb_clamp(1.0f - solidity * 2.0f, 0.0f, 1.0f);
+    cflags[i] = perm < SOLID_PERM_THRESHOLD ? CF_SOLID : 0;
+    fill[i] = fl * 0.25f;
+    const uint32_t sc = g.G / g.F;
+    float gr = 0.0f;
+    for (uint32_t dy = 0; dy < sc; ++dy)
+        for (uint32_t dx = 0; dx < sc; ++dx) {
+            uint32_t c = g.gamma[(y * sc + dy) * g.G + x * sc + dx];
+            if (c < GAMMA_SOLID) gr += (float)c;
+        }
+    gr /= (float)(sc * sc) * (float)GAMMA_SOLID;
+    dragg[i] = gr;
+    slope_f[i] = g.slope[env_cell_for_fluid(g, x, y)];
+}
+
+// solid_normal_from_neighbors (fluid.wgsl:197-216)
+__global__ __launch_bounds__(256) void k_fluid_normals(Geo g, uint8_t* cflags, float2* normal) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const uint32_t F = g.F;
+    if (x >= F || y >= F) return;
+    auto so = [&](uint32_t a, uint32_t b) { return (cflags[b * F + a] & CF_SOLID) != 0; };
+    const uint32_t xm = x == 0u ? x : x - 1u, xp = x + 1u >= F ? x : x + 1u;
+    const uint32_t ym = y == 0u ? y : y - 1u, yp = y + 1u >= F ? y : y + 1u;
+    float s_l = (x > 0u && so(xm, y)) ? 1.0f : 0.0f;
+    float s_r = (x + 1u < F && so(xp, y)) ? 1.0f : 0.0f;
+    float s_b = (y > 0u && so(x, ym)) ? 1.0f : 0.0f;
+    float s_t = (y + 1u < F && so(x, yp)) ? 1.0f : 0.0f;
+    float s_bl = (x > 0u && y > 0u && so(xm, ym)) ? 1.0f : 0.0f;
+    float s_tl = (x > 0u && y + 1u < F && so(xm, yp)) ? 1.0f : 0.0f;
+    float s_br = (x + 1u < F && y > 0u && so(xp, ym)) ? 1.0f : 0.0f;
+    float s_tr = (x + 1u < F && y + 1u < F && so(xp, yp)) ? 1.0f : 0.0f;
+    const float diag = 0.70710678f;
+    float nx = (s_l - s_r) + diag * ((s_bl + s_tl) - (s_br + s_tr));
+    float ny = (s_b - s_t) + diag * ((s_bl + s_br) - (s_tl + s_tr));
+    float n_len = rb_len2(nx, ny);
+    float2 n = n_len > 1e-6f ? f2(nx / n_len, ny / n_len) : f2(0.0f, 0.0f);
+    normal[y * F + x] = n;
+    // Only this thread writes its own flag byte, and the reads of the other
+    // threads test bit 0 (CF_SOLID), which no thread changes here.
+    if (n_len > 1e-6f) cflags[y * F + x] = (uint8_t)(cflags[y * F + x] | CF_NORMAL);
+}
+
+__global__ __launch_bounds__(256) void k_mg_mask0(Geo g, uint8_t* fluid0) {  // mg_init (multigrid.wgsl:54-55)
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.F * g.F) return;
+    fluid0[i] = (g.cflags[i] & CF_SOLID) ? 0 : 1;
+}
+__global__ __launch_bounds__(256) void k_mg_mask_coarsen(uint32_t nc, const uint8_t* fine, uint8_t* coarse) {  // multigrid.wgsl:98
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= nc * nc) return;
+    const uint32_t gx = i % nc, gy = i / nc, n = 2 * nc;
+    uint32_t nf = 0;
+    for (uint32_t c = 0; c < 4; ++c) nf += fine[(gy * 2u + (c >> 1)) * n + gx * 2u + (c & 1u)] ? 1u : 0u;
+    coarse[i] = nf > 0u ? 1 : 0;
+}
+__global__ __launch_bounds__(256) void k_mask_u32(Geo g, uint32_t* out) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= g.F * g.F) return;
+    out[i] = (g.cflags[i] & CF_SOLID) ? 1u : 0u;
+}
+
+// ---- light on the frozen light field (light.wgsl:46-102,182-189) -------------------
+__device__ float light_above(const Geo& g, const RiboSimParams& P, uint32_t x, uint32_t row, uint32_t band) {
+    const uint32_t LS = g.Lsz, LC = LS * LS;
+    float sh = -P.sun_slope;
+    int32_t base = rb_f2i(rb_floor(sh));
+    float f = sh - (float)base;
+    float s = 0.0f;
+    for (int d = 0; d < LIGHT_TAPS; ++d) {
+        int32_t x0 = rb_clampi((int32_t)x + d - LIGHT_TAPS / 2 + base, 0, (int32_t)LS - 1);
+        int32_t x1 = rb_clampi(x0 + 1, 0, (int32_t)LS - 1);
+        s += rb_mix(g.light[band * LC + row * LS + (uint32_t)x0], g.light[band * LC + row * LS + (uint32_t)x1], f);
+    }
+    float cpos = (float)x + sh;
+    float inside = rb_clamp(cpos + 1.0f, 0.0f, 1.0f) * rb_clamp((float)LS - cpos, 0.0f, 1.0f);
+    return s / (float)LIGHT_TAPS * inside;
+}
+__device__ float light_transmit0(const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t lx, uint32_t ly) {
+    float gg = 0.0f;
+    uint32_t m = 0;
+    for (uint32_t dy = 0; dy < LIGHT_DIV; ++dy)
+        for (uint32_t dx = 0; dx < LIGHT_DIV; ++dx) {
+            uint32_t c = (ly * LIGHT_DIV + dy) * g.G + lx * LIGHT_DIV + dx;
+            gg += (float)g.gamma[c];
+            m += cell_total(chem, c);
+        }
+    gg /= (float)LIGHT_DIV;
+    float dens = (float)m / (float)(LIGHT_DIV * LIGHT_DIV);
+    float tau_m = rb_max(P.monomer_uv_absorb, 0.0f) * dens * (float)LIGHT_DIV / (float)g.G * 1.0f;
+    float shade = 0.0f;  // agents are not part of the world step
+    return rb_expf(-UV_SHADOW_ABSORB * gg - 0.15f * shade - tau_m);
+}
+__device__ float light_absorbed_at(const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t env) {
+    uint32_t lx = (env % g.G) / LIGHT_DIV, ly = (env / g.G) / LIGHT_DIV;
+    float incoming = P.sun_now;
+    if (ly + 1u < g.Lsz) incoming = light_above(g, P, lx, ly + 1u, 0u);
+    incoming = rb_max(incoming, 0.0f);
+    return incoming * ((1.0f - light_transmit0(g, P, chem, lx, ly)) + SUN_WATER_ABSORB);
+}
+__device__ __forceinline__ float vent_cycle(const RiboSimParams& P, uint32_t x, uint32_t F) {  // fluid.wgsl:46-54
+    float m = 1.0f;
+    if (P.vent_cycle_amp > 0.0f && P.vent_cycle_period >= 1.0f) {
+        uint32_t period = rb_f2u(P.vent_cycle_period);
+        float t = (float)(P.epoch % period) / (float)period;
+        m = rb_max(1.0f + P.vent_cycle_amp * rb_sinf(6.2831853f * (t + VENT_WAVES * (float)x / (float)F)), 0.0f);
+    }
+    return m;
+}
+
+// ---- fluid step kernels -------------------------------------------------------------
+#define FLUID_XY                                                              \
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y; \
+    const uint32_t F = g.F;                                                   \
+    if (x >= F || y >= F) return;                                             \
+    const uint32_t idx = y * F + x;
+
+// update_temperature (fluid.wgsl:300-363); the copy back (copy_temperature,
+// :365-372) is a pointer swap on the host, and redox_eaten is cleared here.
+__global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, const float2* __restrict__ vel,
+                                                     const float* __restrict__ T, const float* __restrict__ Q,
+                                                     float* __restrict__ t_out, float* __restrict__ r_out,
+                                                     uint32_t* eaten, int eaten_active, const uint32_t* chem) {
+    FLUID_XY
+    const float dt = fluid_dt(P);
+    const float2 v = raw_vel(g, vel, idx);
+    const float2 src = sub2(f2((float)x + 0.5f, (float)y + 0.5f), mul2(v, dt));
+    const float sxf = rb_clamp(src.x - 0.5f, 0.0f, (float)(F - 1u));
+    const float syf = rb_clamp(src.y - 0.5f, 0.0f, (float)(F - 1u));
+    const uint32_t x0 = rb_f2u(rb_floor(sxf)), y0 = rb_f2u(rb_floor(syf));
+    const uint32_t x1 = rb_minu(x0 + 1u, F - 1u), y1 = rb_minu(y0 + 1u, F - 1u);
+    const float fx = rb_fract(sxf), fy = rb_fract(syf);
+    const uint32_t xl = x == 0u ? 0u : x - 1u, xr = rb_minu(x + 1u, F - 1u);
+    const uint32_t yb = y == 0u ? 0u : y - 1u, yt = rb_minu(y + 1u, F - 1u);
+    const float fill = g.fill[idx];
+    const float2 hs = g.heat[idx];
+    const float vc = vent_cycle(P, x, F);
+    float t = rb_mix(rb_mix(T[y0 * F + x0], T[y0 * F + x1], fx), rb_mix(T[y1 * F + x0], T[y1 * F + x1], fx), fy);
+    const float tl = T[y * F + xl], tr = T[y * F + xr], tb = T[yb * F + x], tt = T[yt * F + x];
+    t = rb_mix(t, (tl + tr + tb + tt) * 0.25f, TEMP_DIFFUSE * rb_mix(1.0f, TEMP_ROCK_CONDUCT, fill));
+    t += hs.x * vc * TEMP_HEAT_RATE * dt;
+    if (P.sun_heat > 0.0f) {
+        const uint32_t env = env_cell_for_fluid(g, x, y);
+        const float depth = 1.0f - ((float)y + 0.5f) / (float)F;
+        const float ir = P.sun_now * SUN_IR * rb_expf(-depth / SUN_IR_DEPTH);
+        t += rb_max(P.sun_heat, 0.0f) * SUN_HEAT_RATE * (light_absorbed_at(g, P, chem, env) + ir) * dt;
+    }
+    const float amb = temp_ambient_at(y, F);
+    t = amb + (t - amb) * rb_expf(-TEMP_COOL_RATE * (1.0f - fill) * dt);
+    t_out[idx] = rb_clamp(t, 0.0f, TEMP_MAX);
+
+    float r = rb_mix(rb_mix(Q[y0 * F + x0], Q[y0 * F + x1], fx), rb_mix(Q[y1 * F + x0], Q[y1 * F + x1], fx), fy);
+    const float rl = Q[y * F + xl], rr = Q[y * F + xr], rbm = Q[yb * F + x], rt = Q[yt * F + x];
+    r = rb_mix(r, (rl + rr + rbm + rt) * 0.25f, TEMP_DIFFUSE);
+    r += hs.y * vc * REDOX_RATE * dt;
+    uint32_t e = 0u;
+    if (eaten_active) {
+        e = eaten[idx];
+        eaten[idx] = 0u;
+    }
+    r = rb_max(r - (float)e / REDOX_FP, 0.0f);
+    r *= rb_expf(-rb_max(P.redox_decay, 0.0f) * dt);
+    r_out[idx] = rb_clamp(r, 0.0f, REDOX_MAX);
+}
+
+// buoyancy (fluid.wgsl:376-389) into the fixed-point force words, then
+// gather_forces (:392-398).  Without agents each word only ever receives its
+// own cell's buoyancy, so quantise -> dequantise -> clamp happens in place.
+__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const float* __restrict__ T, float2* forces) {
+    FLUID_XY
+    int32_t qx = 0, qy = 0;
+    if (!(x == 0u || y == 0u || x >= F - 1u || y >= F - 1u)) {
+        const float dev = T[idx] - temp_ambient_at(y, F);
+        if (!(rb_abs(dev) <= 1e-3f)) {
+            const uint32_t time_bucket = P.epoch / 16u;
+            const float ad = rb_abs(dev);
+            const float side = (rng_f4(idx, time_bucket, S_BUOYANCY, P.seed).x * 2.0f - 1.0f) * 0.25f * ad / rb_max(ad, 1.0f);
+            const float fx = side * TEMP_BUOYANCY * ad;
+            const float fy = TEMP_BUOYANCY * dev;
+            qx = rb_f2i(rb_rint(rb_clamp(fx, -1.0e5f, 1.0e5f) * FORCE_FP));
+            qy = rb_f2i(rb_rint(rb_clamp(fy, -1.0e5f, 1.0e5f) * FORCE_FP));
+        }
+    }
+    const float2 f = f2((float)qx / FORCE_FP, (float)qy / FORCE_FP);
+    forces[idx] = clamp_len(sanitize(f), MAX_FORCE);
+}
+
+// slope_steer_velocity (fluid.wgsl:165-178)
+__device__ __forceinline__ float2 slope_steer(const Geo& g, const RiboSimParams& P, uint32_t idx, float2 v_in, float dt) {
+    const float2 sr = sanitize(g.slope_f[idx]);
+    const float2 s = f2(-sr.x, -sr.y);
+    const float v_len = len2(v_in);
+    const float s_len = len2(s);
+    if (v_len < 1e-5f || s_len < 1e-5f) return v_in;
+    const float2 v_dir = f2(v_in.x / v_len, v_in.y / v_len);
+    const float2 s_dir = f2(s.x / s_len, s.y / s_len);
+    const float misalign = rb_clamp(1.0f - dot2(v_dir, s_dir), 0.0f, 2.0f);
+    const float t = rb_clamp(1.0f - rb_expf(-rb_max(P.slope_steer_rate, 0.0f) * misalign * dt), 0.0f, 1.0f);
+    const float2 dir_raw = add2(v_dir, mul2(sub2(s_dir, v_dir), t));
+    const float dir_len = len2(dir_raw);
+    if (dir_len < 1e-5f) return v_in;
+    return mul2(f2(dir_raw.x / dir_len, dir_raw.y / dir_len), v_len);
+}
+
+// add_forces (fluid.wgsl:410-432) with rubble_drag (:150-161)
+__global__ __launch_bounds__(256) void k_add_forces(Geo g, RiboSimParams P, const float2* __restrict__ forces,
+                                                    const float2* __restrict__ vin, float2* __restrict__ vout) {
+    FLUID_XY
+    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    const float dt = fluid_dt(P);
+    const uint32_t xm = x > 0u ? x - 1u : x, xp = x + 1u < F ? x + 1u : x;
+    const uint32_t ym = y > 0u ? y - 1u : y, yp = y + 1u < F ? y + 1u : y;
+    auto fa = [&](uint32_t a, uint32_t b) { return sanitize(forces[b * F + a]); };
+    const float2 f_c = fa(x, y);
+    const float2 axis = add2(add2(add2(fa(xm, y), fa(xp, y)), fa(x, ym)), fa(x, yp));
+    const float2 diag = add2(add2(add2(fa(xm, ym), fa(xp, ym)), fa(xm, yp)), fa(xp, yp));
+    const float2 f_avg = mul2(add2(add2(mul2(axis, 4.0f), diag), mul2(f_c, 20.0f)), (float)(1.0 / 36.0));
+    const float2 f_user = clamp_len(mix2(f_c, f_avg, FORCE_SMOOTH_MIX), MAX_FORCE);
+    float2 v = sanitize(add2(sanitize(vin[idx]), mul2(f_user, dt)));
+    v = slope_steer(g, P, idx, v, dt);
+    v = mul2(v, 1.0f / (1.0f + RUBBLE_DRAG * g.dragg[idx] * dt));
+    v = reflect(g, idx, v);
+    vout[idx] = clamp_len(v, MAX_VEL);
+}
+
+// diffuse_velocity (fluid.wgsl:435-456)
+__global__ __launch_bounds__(256) void k_diffuse(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
+    FLUID_XY
+    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    if (x == 0u || x == F - 1u || y == 0u || y == F - 1u) { vout[idx] = vin[idx]; return; }
+    auto rv = [&](uint32_t a, uint32_t b) { return raw_vel(g, vin, b * F + a); };
+    const float2 v_c = rv(x, y);
+    const float2 axis_sum = add2(add2(add2(rv(x - 1, y), rv(x + 1, y)), rv(x, y - 1)), rv(x, y + 1));
+    const float2 diag_sum = add2(add2(add2(rv(x - 1, y - 1), rv(x + 1, y - 1)), rv(x - 1, y + 1)), rv(x + 1, y + 1));
+    const float2 lap = mul2(sub2(add2(mul2(axis_sum, 4.0f), diag_sum), mul2(v_c, 20.0f)), (float)(1.0 / 6.0));
+    const float a = rb_max(P.fluid_viscosity, 0.0f) * fluid_dt(P);
+    float2 v = sanitize(add2(v_c, mul2(lap, a)));
+    v = reflect(g, idx, v);
+    vout[idx] = clamp_len(v, MAX_VEL);
+}
+
+// is_solid_at_pos and sample_velocity (fluid.wgsl:226-255)
+__device__ __forceinline__ bool is_solid_at_pos(const Geo& g, float2 pos) {
+    const int32_t ix = rb_f2i(rb_floor(pos.x - 0.5f)), iy = rb_f2i(rb_floor(pos.y - 0.5f));
+    if (ix < 0 || ix >= (int32_t)g.F || iy < 0 || iy >= (int32_t)g.F) return true;
+    return solid(g, (uint32_t)iy * g.F + (uint32_t)ix);
+}
+__device__ __forceinline__ float2 sample_velocity(const Geo& g, const float2* vin, float2 pos) {
+    const float min_pos = 0.5f, max_pos = (float)g.F - 0.5f;
+    if (pos.x < min_pos || pos.x > max_pos || pos.y < min_pos || pos.y > max_pos) return f2(0.0f, 0.0f);
+    const float xx = pos.x - 0.5f, yy = pos.y - 0.5f;
+    const int32_t x0 = rb_f2i(rb_floor(xx)), y0 = rb_f2i(rb_floor(yy));
+    const float fx = rb_fract(xx), fy = rb_fract(yy);
+    const int32_t Fm = (int32_t)g.F - 1;
+    const uint32_t cx0 = (uint32_t)rb_clampi(x0, 0, Fm), cx1 = (uint32_t)rb_clampi(x0 + 1, 0, Fm);
+    const uint32_t cy0 = (uint32_t)rb_clampi(y0, 0, Fm), cy1 = (uint32_t)rb_clampi(y0 + 1, 0, Fm);
+    const float2 v0 = mix2(raw_vel(g, vin, cy0 * g.F + cx0), raw_vel(g, vin, cy0 * g.F + cx1), fx);
+    const float2 v1 = mix2(raw_vel(g, vin, cy1 * g.F + cx0), raw_vel(g, vin, cy1 * g.F + cx1), fx);
+    return mix2(v0, v1, fy);
+}
+
+// advect_velocity (fluid.wgsl:458-477); decay = pow(clamp(fluid_decay,0,1),
+// dt*60), uniform over the step, comes from the host.
+__global__ __launch_bounds__(256) void k_advect(Geo g, RiboSimParams P, float decay, const float2* __restrict__ vin,
+                                                float2* __restrict__ vout) {
+    FLUID_XY
+    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    const float2 pos = f2((float)x + 0.5f, (float)y + 0.5f);
+    const float dt = fluid_dt(P);
+    const float2 vel = reflect(g, idx, clamp_len(raw_vel(g, vin, idx), MAX_VEL));
+    float2 trace = sub2(pos, mul2(vel, dt));
+    if (is_solid_at_pos(g, trace)) trace = sub2(pos, mul2(reflect(g, idx, vel), dt));
+    float2 out = sanitize(mul2(sample_velocity(g, vin, trace), decay));
+    out = reflect(g, idx, out);
+    vout[idx] = clamp_len(out, MAX_VEL);
+}
+
+// curl_at (fluid.wgsl:479-489)
+__device__ __forceinline__ float curl_at(const Geo& g, const float2* vin, uint32_t x, uint32_t y) {
+    const int32_t Fm = (int32_t)g.F - 1;
+    const uint32_t xm = (uint32_t)rb_clampi((int32_t)x - 1, 0, Fm), xp = (uint32_t)rb_clampi((int32_t)x + 1, 0, Fm);
+    const uint32_t ym = (uint32_t)rb_clampi((int32_t)y - 1, 0, Fm), yp = (uint32_t)rb_clampi((int32_t)y + 1, 0, Fm);
+    const uint32_t F = g.F;
+    const float2 v_l = reflect(g, y * F + xm, raw_vel(g, vin, y * F + xm));
+    const float2 v_r = reflect(g, y * F + xp, raw_vel(g, vin, y * F + xp));
+    const float2 v_b = reflect(g, ym * F + x, raw_vel(g, vin, ym * F + x));
+    const float2 v_t = reflect(g, yp * F + x, raw_vel(g, vin, yp * F + x));
+    return 0.5f * (v_r.y - v_l.y) - 0.5f * (v_t.x - v_b.x);
+}
+
+// vorticity_confinement (fluid.wgsl:491-518)
+__global__ __launch_bounds__(256) void k_vorticity(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
+    FLUID_XY
+    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    const uint32_t border = 2u;
+    const float eps = rb_clamp(P.fluid_vorticity, 0.0f, 10.0f);
+    if (x < border || x >= F - border || y < border || y >= F - border || eps < 1e-6f) { vout[idx] = vin[idx]; return; }
+    const float w = curl_at(g, vin, x, y);
+    if (rb_abs(w) < 5e-4f) { vout[idx] = vin[idx]; return; }
+    const float2 grad = mul2(f2(rb_abs(curl_at(g, vin, x + 1, y)) - rb_abs(curl_at(g, vin, x - 1, y)),
+                                rb_abs(curl_at(g, vin, x, y + 1)) - rb_abs(curl_at(g, vin, x, y - 1))),
+                             0.5f);
+    const float gl = rb_max(len2(grad), 1e-5f);
+    const float2 n = f2(grad.x / gl, grad.y / gl);
+    const float2 f = mul2(f2(n.y, -n.x), w * eps);
+    const float2 dv = clamp_len(mul2(f, fluid_dt(P)), 2.0f);
+    float2 v = sanitize(add2(vin[idx], dv));
+    v = reflect(g, idx, v);
+    vout[idx] = clamp_len(v, MAX_VEL);
+}
+
+// compute_divergence (fluid.wgsl:525-551).  Also zeroes the pressure of wall
+// cells, which is what mg_init (multigrid.wgsl:57) does to the level-0
+// multigrid unknowns that alias the pressure field.
+__global__ __launch_bounds__(256) void k_divergence(Geo g, const float2* __restrict__ vin, float* __restrict__ div, float* p0) {
+    FLUID_XY
+    if (solid(g, idx)) {
+        div[idx] = 0.0f;
+        p0[idx] = 0.0f;
+        return;
+    }
+    const float2 v_c = raw_vel(g, vin, idx);
+    auto nbv = [&](int32_t nx, int32_t ny, bool axis_x) -> float2 {
+        const bool outside = nx < 0 || ny < 0 || nx >= (int32_t)F || ny >= (int32_t)F;
+        if (outside || solid(g, (uint32_t)ny * F + (uint32_t)nx)) return axis_x ? f2(-v_c.x, v_c.y) : f2(v_c.x, -v_c.y);
+        return raw_vel(g, vin, (uint32_t)ny * F + (uint32_t)nx);
+    };
+    const int32_t xi = (int32_t)x, yi = (int32_t)y;
+    const float2 v_l = nbv(xi - 1, yi, true), v_r = nbv(xi + 1, yi, true);
+    const float2 v_b = nbv(xi, yi - 1, false), v_t = nbv(xi, yi + 1, false);
+    div[idx] = 0.5f * ((v_r.x - v_l.x) + (v_t.y - v_b.y));
+}
+
+// jacobi_pressure (fluid.wgsl:553-608): Neumann by mirroring p_c.
+__global__ __launch_bounds__(256) void k_jacobi(Geo g, const float* __restrict__ pin, float* __restrict__ pout,
+                                                const float* __restrict__ div) {
+    FLUID_XY
+    if (solid(g, idx)) { pout[idx] = 0.0f; return; }
+    const float p_c = pin[idx];
+    auto nb = [&](int32_t a, int32_t b) -> float {
+        if (a < 0 || b < 0 || a >= (int32_t)F || b >= (int32_t)F) return p_c;
+        const uint32_t j = (uint32_t)b * F + (uint32_t)a;
+        return solid(g, j) ? p_c : pin[j];
+    };
+    const float p_l = nb((int32_t)x - 1, (int32_t)y), p_r = nb((int32_t)x + 1, (int32_t)y);
+    const float p_b = nb((int32_t)x, (int32_t)y - 1), p_t = nb((int32_t)x, (int32_t)y + 1);
+    const float d = div[idx];
+    if (is_bad(d) || is_bad(p_c)) { pout[idx] = 0.0f; return; }
+    const float p_new = (p_l + p_r + p_b + p_t - d) * 0.25f;
+    pout[idx] = is_bad(p_new) ? 0.0f : p_new;
+}
+
+// ---- multigrid, one dispatch per operation (multigrid.wgsl:25-114) ----------------
+__device__ __forceinline__ void mg_neighbors(const MgLevelDev& L, uint32_t x, uint32_t y, float* s, float* k) {
+    const uint32_t n = L.n;
+    float ss = 0.0f, kk = 0.0f;
+    if (x > 0u && L.fluid[y * n + x - 1u]) { ss += L.p[y * n + x - 1u]; kk += 1.0f; }
+    if (x + 1u < n && L.fluid[y * n + x + 1u]) { ss += L.p[y * n + x + 1u]; kk += 1.0f; }
+    if (y > 0u && L.fluid[(y - 1u) * n + x]) { ss += L.p[(y - 1u) * n + x]; kk += 1.0f; }
+    if (y + 1u < n && L.fluid[(y + 1u) * n + x]) { ss += L.p[(y + 1u) * n + x]; kk += 1.0f; }
+    *s = ss;
+    *k = kk;
+}
+__global__ __launch_bounds__(256) void k_mg_smooth(MgLevelDev L, uint32_t color, const uint32_t* done) {  // :60-77
+    if (done && *done) return;
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= L.n || y >= L.n) return;
+    if (((x + y) & 1u) != color) return;
+    const uint32_t i = y * L.n + x;
+    if (!L.fluid[i]) return;
+    float s, k;
+    mg_neighbors(L, x, y, &s, &k);
+    L.p[i] = (s - L.rhs[i]) / (k + L.eps);
+}
+__global__ __launch_bounds__(256) void k_mg_restrict(MgLevelDev L, MgLevelDev C, const uint32_t* done) {  // :82-105
+    if (done && *done) return;
+    const uint32_t gx = blockIdx.x * 16 + threadIdx.x, gy = blockIdx.y * 16 + threadIdx.y;
+    if (gx >= C.n || gy >= C.n) return;
+    float sum = 0.0f;
+    uint32_t nf = 0;
+    for (uint32_t c = 0; c < 4; ++c) {
+        const uint32_t fx = gx * 2u + (c & 1u), fy = gy * 2u + (c >> 1);
+        const uint32_t i = fy * L.n + fx;
+        if (!L.fluid[i]) continue;
+        float s, k;
+        mg_neighbors(L, fx, fy, &s, &k);
+        sum += L.rhs[i] - (s - (k + L.eps) * L.p[i]);
+        nf += 1u;
+    }
+    const uint32_t ci = gy * C.n + gx;
+    C.rhs[ci] = nf > 0u ? 2.0f * sum / (float)rb_maxu(nf, 1u) : 0.0f;
+    C.p[ci] = 0.0f;
+}
+__global__ __launch_bounds__(256) void k_mg_prolong(MgLevelDev L, MgLevelDev C, const uint32_t* done) {  // :108-114
+    if (done && *done) return;
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= L.n || y >= L.n) return;
+    const uint32_t i = y * L.n + x;
+    if (!L.fluid[i]) return;
+    L.p[i] += C.p[(y / 2u) * C.n + x / 2u];
+}
+
+// Residual tracking for the tolerance mode (binary64 sums, one block-level
+// reduction and a last-block finaliser; no host round trip).
+__device__ __forceinline__ double block_sum_f64(double v, double* sh) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
+    return t;
+}
+__global__ void k_counters_reset(Counters* c) {
+    c->rr = 0.0;
+    c->bb = 0.0;
+    c->blocks_done = 0u;
+    c->cycles = 0u;
+    c->done = 0u;
+}
+__global__ __launch_bounds__(256) void k_mg_rhs_norm(MgLevelDev L, Counters* c) {
+    __shared__ double sh[4];
+    const uint32_t n2 = L.n * L.n;
+    double v = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n2; i += gridDim.x * 256)
+        if (L.fluid[i]) v += (double)L.rhs[i] * (double)L.rhs[i];
+    double t = block_sum_f64(v, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&c->bb, t);
+        __threadfence();
+        uint32_t d = atomicAdd(&c->blocks_done, 1u);
+        if (d + 1u == gridDim.x) c->blocks_done = 0u;
+    }
+}
+__global__ __launch_bounds__(256) void k_mg_residual(MgLevelDev L, Counters* c, float tol, uint32_t max_cycles, int count_cycle) {
+    if (c->done) return;
+    __shared__ double sh[4];
+    const uint32_t n = L.n, n2 = n * n;
+    double v = 0.0;
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n2; i += gridDim.x * 256) {
+        if (!L.fluid[i]) continue;
+        float s, k;
+        mg_neighbors(L, i % n, i / n, &s, &k);
+        double r = (double)L.rhs[i] - ((double)s - ((double)k + (double)L.eps) * (double)L.p[i]);
+        v += r * r;
+    }
+    double t = block_sum_f64(v, sh);
+    if (threadIdx.x == 0) {
+        atomicAdd(&c->rr, t);
+        __threadfence();
+        uint32_t d = atomicAdd(&c->blocks_done, 1u);
+        if (d + 1u == gridDim.x) {
+            __threadfence();
+            const double rr = atomicAdd(&c->rr, 0.0);
+            const double bb = atomicAdd(&c->bb, 0.0);
+            const double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+            const uint32_t cyc = c->cycles + (count_cycle ? 1u : 0u);
+            c->cycles = cyc;
+            c->last_rel = rel;
+            if (rel <= (double)tol || cyc >= max_cycles) c->done = 1u;
+            c->rr = 0.0;
+            c->blocks_done = 0u;
+        }
+    }
+}
+
+// subtract_gradient (fluid.wgsl:610-630)
+__global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
+                                                 const float* __restrict__ P) {
+    FLUID_XY
+    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    const float p_c = P[idx];
+    float p_l = p_c, p_r = p_c, p_b = p_c, p_t = p_c;
+    if (x > 0u && !solid(g, idx - 1u)) p_l = P[idx - 1u];
+    if (x + 1u < F && !solid(g, idx + 1u)) p_r = P[idx + 1u];
+    if (y > 0u && !solid(g, idx - F)) p_b = P[idx - F];
+    if (y + 1u < F && !solid(g, idx + F)) p_t = P[idx + F];
+    const float2 grad = mul2(f2(p_r - p_l, p_t - p_b), 0.5f);
+    float2 v = sub2(sanitize(vin[idx]), grad);
+    v = reflect(g, idx, v);
+    vout[idx] = clamp_len(sanitize(v), MAX_VEL);
+}
+
+// net_flux (fluid.wgsl:640-657), in the reference's sequential order per row
+// and per column so the means are bit-identical to it.  Blocks [0, F/64)
+// walk columns (one thread per column; each step reads one coalesced row
+// segment); the rest walk rows through 64x64 tiles staged in LDS, so the
+// per-row walk reads LDS instead of strided HBM.
+constexpr uint32_t NF_T = 64;
+__global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
+    const uint32_t F = g.F;
+    const uint32_t nb = (F + NF_T - 1) / NF_T;
+    const uint32_t t = threadIdx.x;
+    if (blockIdx.x < nb) {
+        const uint32_t col = blockIdx.x * NF_T + t;
+        if (col >= F) return;
+        float sum = 0.0f, n = 0.0f;
+        for (uint32_t k = 0; k < F; ++k) {
+            const uint32_t q = k * F + col;
+            if (solid(g, q)) continue;
+            sum += sanitize(vin[q]).x;
+            n += 1.0f;
+        }
+        nf[F + col] = n > 0.0f ? sum / n : 0.0f;
+        return;
+    }
+    __shared__ float tile[NF_T][NF_T + 1];
+    const uint32_t row0 = (blockIdx.x - nb) * NF_T;
+    float sum = 0.0f, n = 0.0f;
+    for (uint32_t c0 = 0; c0 < F; c0 += NF_T) {
+        __syncthreads();
+        for (uint32_t r = 0; r < NF_T; ++r) {
+            const uint32_t row = row0 + r, col = c0 + t;
+            float val = __builtin_nanf("");  // marks a wall or out-of-range cell
+            if (row < F && col < F) {
+                const uint32_t q = row * F + col;
+                if (!solid(g, q)) val = sanitize(vin[q]).y;
+            }
+            tile[r][t] = val;
+        }
+        __syncthreads();
+        const uint32_t lim = rb_minu(NF_T, F - c0);
+        for (uint32_t c = 0; c < lim; ++c) {
+            const float val = tile[t][c];
+            if (val != val) continue;
+            sum += val;
+            n += 1.0f;
+        }
+    }
+    if (row0 + t < F) nf[row0 + t] = n > 0.0f ? sum / n : 0.0f;
+}
+
+// enforce_boundaries (fluid.wgsl:663-677)
+__global__ __launch_bounds__(256) void k_boundaries(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
+                                                    const float* __restrict__ nf) {
+    FLUID_XY
+    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    float2 v = sanitize(vin[idx]);
+    v = sub2(v, f2(nf[F + x], nf[y]));
+    if (x == 0u || x == F - 1u) v.x = 0.0f;
+    if (y == 0u || y == F - 1u) v.y = 0.0f;
+    v = reflect(g, idx, v);
+    vout[idx] = clamp_len(sanitize(v), MAX_VEL);
+}
+
+// thermal_activation (transport.wgsl:419-438) with chem_activate_one
+// (chem.wgsl:90-110).  Each thread owns its cell's four words, so the
+// compare-and-swap of the reference reduces to a plain read-modify-write.
+__global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const float* __restrict__ T, uint32_t* chem) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const uint32_t G = g.G, F = g.F;
+    if (x >= G || y >= G) return;
+    const uint32_t fx = rb_minu((x * F) / G, F - 1u), fy = rb_minu((y * F) / G, F - 1u);
+    const float t_loc = T[fy * F + fx];
+    if (t_loc <= TEMP_ACT_THRESHOLD) return;
+    const float p_act = rb_clamp(FUMAROLE_ACT_P * rb_max(P.thermal_activation, 0.0f) * (t_loc - TEMP_ACT_THRESHOLD) / TEMP_ACT_THRESHOLD, 0.0f, 0.9f);
+    const uint32_t idx = y * G + x;
+    uint4 v = *reinterpret_cast<uint4*>(chem + 4u * idx);
+    uint32_t w[4] = {v.x, v.y, v.z, v.w};
+    bool changed = false;
+    for (uint32_t ch = 0; ch < 4; ++ch) {
+        const uint32_t slot = idx * 4u + ch;
+        if ((w[ch] >> 16) > 0u && rng_f4(slot, P.epoch, S_THERMAL, P.seed).x < p_act) {
+            w[ch] += 0xFFFF0001u;
+            changed = true;
+        }
+    }
+    if (changed) *reinterpret_cast<uint4*>(chem + 4u * idx) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// ---- transport ------------------------------------------------------------------------
+// agg_count + agg_neighbours (transport.wgsl:61-87) in one pass: activated
+// counts of the tile and its 1-cell ring go to LDS, then each cell sums its
+// 3x3 block; the scatter only ever reads agg_nb + agg_act, stored as one word.
+__global__ __launch_bounds__(256) void k_agg(Geo g, const uint32_t* __restrict__ chem, uint32_t* __restrict__ agg_e) {
+    __shared__ uint32_t act[18][18];
+    const uint32_t G = g.G;
+    const int32_t bx = (int32_t)(blockIdx.x * 16), by = (int32_t)(blockIdx.y * 16);
+    const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
+    for (uint32_t k = tid; k < 18u * 18u; k += 256u) {
+        const int32_t lx = (int32_t)(k % 18u), ly = (int32_t)(k / 18u);
+        const int32_t gx = bx + lx - 1, gy = by + ly - 1;
+        uint32_t a = 0u;
+        if (gx >= 0 && gy >= 0 && gx < (int32_t)G && gy < (int32_t)G) {
+            uint4 v = *reinterpret_cast<const uint4*>(chem + 4u * ((uint32_t)gy * G + (uint32_t)gx));
+            a = (v.x & CHEM_STATE_MASK) + (v.y & CHEM_STATE_MASK) + (v.z & CHEM_STATE_MASK) + (v.w & CHEM_STATE_MASK);
+        }
+        act[ly][lx] = a;
+    }
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= G || y >= G) return;
+    uint32_t s = 0u;
+    for (uint32_t dy = 0; dy < 3; ++dy)
+        for (uint32_t dx = 0; dx < 3; ++dx) s += act[threadIdx.y + dy][threadIdx.x + dx];
+    agg_e[y * G + x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x * 4u;
+    if (i + 3 < n) {
+        *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
+    } else {
+        for (; i < n; ++i) p[i] = v;
+    }
+}
+
+// fluid_velocity_at_world (fluid.wgsl:260-273)
+__device__ __forceinline__ float2 fluid_velocity_at_world(const Geo& g, const float2* vin, float2 pw) {
+    const float gf = (float)g.F;
+    const float gx = rb_clamp(pw.x / g.sim_size * gf, 0.5f, gf - 0.5f) - 0.5f;
+    const float gy = rb_clamp(pw.y / g.sim_size * gf, 0.5f, gf - 0.5f) - 0.5f;
+    const uint32_t x0 = rb_f2u(rb_floor(gx)), y0 = rb_f2u(rb_floor(gy));
+    const uint32_t x1 = rb_minu(x0 + 1u, g.F - 1u), y1 = rb_minu(y0 + 1u, g.F - 1u);
+    const float tx = rb_fract(gx), ty = rb_fract(gy);
+    const float2 v0 = mix2(sanitize(vin[y0 * g.F + x0]), sanitize(vin[y0 * g.F + x1]), tx);
+    const float2 v1 = mix2(sanitize(vin[y1 * g.F + x0]), sanitize(vin[y1 * g.F + x1]), tx);
+    return mix2(v0, v1, ty);
+}
+
+// lattice_rand / smooth_start (transport.wgsl:95-117)
+__device__ __forceinline__ float2 lattice_rand(const RiboSimParams& P, uint32_t ix, uint32_t iy) {
+    F4 r = rng_f4((ix * 73856093u) ^ (iy * 19349663u), P.epoch, S_MOVE + MAX_MOVERS_PER_CH + 1u, P.seed);
+    return f2(r.x, r.y);
+}
+__device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0) {
+    const F4 jit = rng_f4(0x85EBCA6Bu, P.epoch, S_MOVE + MAX_MOVERS_PER_CH + 2u, P.seed);
+    const uint32_t x = x0 + rb_f2u(jit.x * (float)SMOOTH_CELL);
+    const uint32_t y = y0 + rb_f2u(jit.y * (float)SMOOTH_CELL);
+    const uint32_t gx = x / SMOOTH_CELL, gy = y / SMOOTH_CELL;
+    const float tx = rb_smoothstep01(((float)(x % SMOOTH_CELL) + 0.5f) / (float)SMOOTH_CELL);
+    const float ty = rb_smoothstep01(((float)(y % SMOOTH_CELL) + 0.5f) / (float)SMOOTH_CELL);
+    const float2 n = mix2(mix2(lattice_rand(P, gx, gy), lattice_rand(P, gx + 1u, gy), tx),
+                          mix2(lattice_rand(P, gx, gy + 1u), lattice_rand(P, gx + 1u, gy + 1u), tx), ty);
+    const F4 gg = rng_f4(0x9E3779B9u, P.epoch, S_MOVE + MAX_MOVERS_PER_CH, P.seed);
+    const float vx = rb_fract(n.x + gg.x), vy = rb_fract(n.y + gg.y);
+    return f2(1.0f - rb_abs(2.0f * vx - 1.0f), 1.0f - rb_abs(2.0f * vy - 1.0f));
+}
+
+// transport_scatter (transport.wgsl:119-406).  One thread per grid cell; the
+// 16-bin destination histogram of each channel lives in LDS (column per
+// thread), the emit is one global atomic add per non-empty bin.
+__global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* __restrict__ chem,
+                                                 uint32_t* __restrict__ next, const float2* __restrict__ vel,
+                                                 const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
+    __shared__ uint32_t s_bins[16 * 256];
+    const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const uint32_t G = g.G, F = g.F;
+    if (x >= G || y >= G) return;
+    const uint32_t idx = y * G + x;
+    const uint4 cw = *reinterpret_cast<const uint4*>(chem + 4u * idx);
+    const uint32_t cwv[4] = {cw.x, cw.y, cw.z, cw.w};
+    const uint32_t src_total = (cw.x & CHEM_STATE_MASK) + (cw.x >> 16) + (cw.y & CHEM_STATE_MASK) + (cw.y >> 16) +
+                               (cw.z & CHEM_STATE_MASK) + (cw.z >> 16) + (cw.w & CHEM_STATE_MASK) + (cw.w >> 16);
+    if (src_total == 0u) return;
+    const float cell_w = (float)g.wupc;
+    const uint32_t g_src = g.gamma[idx];
+    float2 v = f2(0.0f, 0.0f);
+    if (P.fluid_enabled != 0u && g_src < GAMMA_SOLID) {
+        const float2 c = f2(((float)x + 0.5f) * cell_w, ((float)y + 0.5f) * cell_w);
+        const float2 v0 = fluid_velocity_at_world(g, vel, c);
+        const float world_per_fluid = g.sim_size / (float)F;
+        const float k = rb_max(P.dt, 0.0f);
+        v = fluid_velocity_at_world(g, vel, f2(c.x + v0.x * world_per_fluid * k * 0.5f, c.y + v0.y * world_per_fluid * k * 0.5f));
+    }
+    if (v.x != 0.0f || v.y != 0.0f) {
+        const bool rock_r = x + 1u < G && g.gamma[idx + 1u] >= GAMMA_SOLID;
+        const bool rock_l = x > 0u && g.gamma[idx - 1u] >= GAMMA_SOLID;
+        const bool rock_u = y + 1u < G && g.gamma[idx + G] >= GAMMA_SOLID;
+        const bool rock_d = y > 0u && g.gamma[idx - G] >= GAMMA_SOLID;
+        if ((v.x > 0.0f && rock_r) || (v.x < 0.0f && rock_l)) v.x = 0.0f;
+        else if ((v.x > 0.0f && rock_l) || (v.x < 0.0f && rock_r)) v.x *= 0.5f;
+        if ((v.y > 0.0f && rock_u) || (v.y < 0.0f && rock_d)) v.y = 0.0f;
+        else if ((v.y > 0.0f && rock_d) || (v.y < 0.0f && rock_u)) v.y *= 0.5f;
+    }
+    const float tn = (float)rb_maxu(P.transport_every, 1u);
+    const float ratio = (float)G / (float)F;
+    const float dtp = rb_max(P.dt, 0.0f);
+    const float2 disp = f2(v.x * ratio * dtp * tn, v.y * ratio * dtp * tn);
+    const float agitation = rb_clamp(len2(v) / DIFF_AGITATION_SPEED, 0.0f, 1.0f);
+    float p_diff = rb_clamp(DIFF_HOP_P * rb_mix(DIFF_HOP_FLOOR, 1.0f, agitation) * rb_max(P.diffusion, 0.0f) * tn, 0.0f, 0.5f);
+    if (g_src > 0u) p_diff = rb_min(DIFF_HOP_P * BURIED_DIFF_FACTOR / (1.0f + GAMMA_POROSITY_K * (float)g_src) * tn, 0.5f);
+    const uint32_t cap_self = capacity(g_src);
+    if (src_total > cap_self) p_diff = rb_min(CHEM_SQUEEZE_P * tn, 0.5f);
+    float p_settle = 0.0f;
+    if (g_src == 0u) p_settle = rb_min(MONOMER_SETTLE_P * rb_max(P.settle, 0.0f) * tn, 0.5f);
+    const int32_t base_x = rb_addi(rb_f2i(rb_floor(disp.x)), -1);
+    const int32_t base_y = rb_addi(rb_f2i(rb_floor(disp.y)), -1);
+
+    float press[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float max_drop = 0.0f;
+    if (P.monomer_pressure > 0.0f) {
+        const float f_self = (float)src_total / (float)rb_maxu(cap_self, 1u);
+        const int32_t nbo[4] = {1, -1, (int32_t)G, -(int32_t)G};
+        const bool inside[4] = {x + 1u < G, x > 0u, y + 1u < G, y > 0u};
+        for (int d = 0; d < 4; ++d) {
+            if (!inside[d]) continue;
+            const uint32_t n = (uint32_t)((int32_t)idx + nbo[d]);
+            const uint32_t cap_n = capacity(g.gamma[n]);
+            if (cap_n == 0u) continue;
+            const float f_n = (float)cell_total(chem, n) / (float)cap_n;
+            press[d] = rb_max(0.05f, 1.0f + P.monomer_pressure * (f_self - f_n));
+            max_drop = rb_max(max_drop, f_self - f_n);
+        }
+    }
+    p_diff = rb_clamp(p_diff + PRESSURE_HOP_P * P.monomer_pressure * max_drop * tn, 0.0f, 0.5f);
+    const uint32_t lxi = rb_minu(x, G - 1u) / LIGHT_DIV, lyi = rb_minu(y, G - 1u) / LIGHT_DIV;
+    const float light_t = g.light[lyi * g.Lsz + lxi];
+
+    float p_bound = 0.0f;
+    float accept[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    float accept_spent[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+    if (P.aggregation > 0.0f && src_total <= cap_self) {
+        float e[5] = {(float)agg_e[idx], 0.0f, 0.0f, 0.0f, 0.0f};
+        if (x + 1u < G) e[1] = (float)agg_e[idx + 1u];
+        if (x > 0u) e[2] = (float)agg_e[idx - 1u];
+        if (y + 1u < G) e[3] = (float)agg_e[idx + G];
+        if (y > 0u) e[4] = (float)agg_e[idx - G];
+        const uint32_t fx = rb_minu((x * F) / G, F - 1u), fy = rb_minu((y * F) / G, F - 1u);
+        const float t_rel = 1.0f + rb_max(temp[fy * F + fx], 0.0f) / TEMP_ACT_THRESHOLD;
+        const float room = rb_clamp(5.0f * (1.0f - (float)src_total / (float)rb_maxu(cap_self, 1u)), 0.0f, 1.0f);
+        const float k_e = P.aggregation * room / t_rel;
+        for (int d = 0; d < 4; ++d) {
+            const float drop = e[0] - e[d + 1];
+            if (drop > 0.0f) accept[d] = rb_expf(-k_e * drop);
+            const float rise = e[d + 1] - e[0];
+            if (rise > 0.0f) accept_spent[d] = rb_expf(-k_e * rise);
+        }
+        p_bound = 1.0f - rb_expf(-k_e * rb_min(rb_max(e[0] - 1.0f, 0.0f), 48.0f));
+    }
+    const float dlen = len2(disp);
+    const bool jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
+    float2 bound_start = f2(0.0f, 0.0f);
+    bool have_bound_start = false;
+    uint32_t* bins = s_bins + tid;
+
+    for (uint32_t ch = 0; ch < 4; ++ch) {
+        const uint32_t slot = idx * 4u + ch;
+        const uint32_t v_ch = cwv[ch];
+        uint32_t act_n = v_ch & CHEM_STATE_MASK;
+        uint32_t spent_n = v_ch >> 16;
+        if (act_n + spent_n == 0u) continue;
+        if (spent_n > 0u) {
+            const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
+            const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
+                                  (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
+                                   rb_max(P.reactivation_rate, 0.0f));
+            uint32_t na = rb_f2u(rb_floor(exp_act));
+            const float frac = exp_act - (float)na;
+            if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
+            na = rb_minu(na, rb_minu(spent_n, 8u));
+            act_n += na;
+            spent_n -= na;
+        }
+        if (act_n > 0u) {
+            const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
+            const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
+            if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
+                act_n -= 1u;
+                spent_n += 1u;
+            }
+        }
+        float coh[4] = {1.0f, 1.0f, 1.0f, 1.0f};
+        if (act_n > 0u && P.cohesion > 0.0f) {
+            auto ac = [&](uint32_t cell) { return rb_minu(chem[cell * 4u + ch] & CHEM_STATE_MASK, 8u); };
+            if (x + 1u < G) coh[0] = 1.0f + P.cohesion * (float)ac(idx + 1u);
+            if (x > 0u) coh[1] = 1.0f + P.cohesion * (float)ac(idx - 1u);
+            if (y + 1u < G) coh[2] = 1.0f + P.cohesion * (float)ac(idx + G);
+            if (y > 0u) coh[3] = 1.0f + P.cohesion * (float)ac(idx - G);
+        }
+        for (uint32_t b = 0; b < 16; ++b) bins[b * 256u] = 0u;
+        uint32_t stay = 0u;
+        const uint32_t n = act_n + spent_n;
+        const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
+        const uint32_t act_stay = act_n - rb_minu(act_n, movers);
+        stay += act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
+        const float wa[4] = {press[0] * coh[0], press[1] * coh[1], press[2] * coh[2], press[3] * coh[3]};
+        for (uint32_t k = 0; k < movers; ++k) {
+            const bool is_act = k < act_n;
+            const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
+            const F4 rf = rng_f4(slot, P.epoch, S_MOVE + k, P.seed);
+            const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
+            if (bound && !have_bound_start) {
+                bound_start = smooth_start(P, x, y);
+                have_bound_start = true;
+            }
+            float px = bound ? bound_start.x : rf.x;
+            float py = bound ? bound_start.y : rf.y;
+            px += disp.x;
+            py += disp.y;
+            if (jitter_on) {
+                const F4 rj = rng_f4(slot, P.epoch, S_DISPERSE + k, P.seed);
+                const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                px += rb_clamp(jx, -0.9f, 0.9f);
+                py += rb_clamp(jy, -0.9f, 0.9f);
+            }
+            const float r = rf.z;
+            if (r < p_diff) {
+                const float w0 = is_act ? wa[0] : press[0], w1 = is_act ? wa[1] : press[1];
+                const float w2 = is_act ? wa[2] : press[2], w3 = is_act ? wa[3] : press[3];
+                float u = rf.w * (w0 + w1 + w2 + w3);
+                uint32_t d;
+                if (u < w0) d = 0u;
+                else if ((u -= w0) < w1) d = 1u;
+                else if ((u -= w1) < w2) d = 2u;
+                else d = 3u;
+                const float acc_d = is_act ? accept[d] : accept_spent[d];
+                const bool stay_put = rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d;
+                if (!stay_put) {
+                    if (d == 0u) px += 1.0f;
+                    else if (d == 1u) px -= 1.0f;
+                    else if (d == 2u) py += 1.0f;
+                    else py -= 1.0f;
+                }
+            } else if (r < p_diff + p_settle) {
+                py -= 1.0f;
+            }
+            const int32_t ox = rb_addi(rb_f2i(rb_floor(px)), -base_x);
+            const int32_t oy = rb_addi(rb_f2i(rb_floor(py)), -base_y);
+            const uint32_t bi = (uint32_t)rb_clampi(oy, 0, 3) * 4u + (uint32_t)rb_clampi(ox, 0, 3);
+            bins[bi * 256u] += unit;
+        }
+        for (uint32_t b = 0; b < 16; ++b) {
+            const uint32_t cnt = bins[b * 256u];
+            if (cnt == 0u) continue;
+            const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
+            const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
+            const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
+            bool blocked = false;
+            if (t_idx != idx) {
+                const uint32_t g_tgt = g.gamma[t_idx];
+                if (g_tgt >= GAMMA_SOLID) blocked = g_src < GAMMA_SOLID;
+                else if (cell_total(chem, t_idx) > capacity(g_tgt)) blocked = true;
+            }
+            if (blocked) stay += cnt;
+            else atomicAdd(next + t_idx * 4u + ch, cnt);
+        }
+        if (stay > 0u) atomicAdd(next + slot, stay);
+    }
+}
+
+// ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state.
+// Wave reductions, one LDS slot per wave, one global atomic per counter per
+// block; the last block publishes the totals, optionally checks the
+// per-channel act + spent totals against the armed values, and clears the
+// accumulators for the next use.
+__global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led,
+                                                uint32_t epoch, int check) {
+    __shared__ uint32_t sh[4][8];
+    uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
+        const uint4 v = *reinterpret_cast<const uint4*>(chem + 4u * i);
+        a[0] += v.x & CHEM_STATE_MASK; a[4] += v.x >> 16;
+        a[1] += v.y & CHEM_STATE_MASK; a[5] += v.y >> 16;
+        a[2] += v.z & CHEM_STATE_MASK; a[6] += v.z >> 16;
+        a[3] += v.w & CHEM_STATE_MASK; a[7] += v.w >> 16;
+    }
+    for (int k = 0; k < 8; ++k)
+        for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_down(a[k], o, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0)
+        for (int k = 0; k < 8; ++k) sh[wid][k] = a[k];
+    __syncthreads();
+    if (threadIdx.x < 8) {
+        uint32_t s = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+        atomicAdd(&led->acc[threadIdx.x], s);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const uint32_t d = atomicAdd(&led->blocks_done, 1u);
+        if (d + 1u == gridDim.x) {
+            __threadfence();
+            uint32_t tot[8];
+            for (int k = 0; k < 8; ++k) {
+                tot[k] = atomicExch(&led->acc[k], 0u);
+                led->last[k] = tot[k];
+            }
+            if (check) {
+                if (!led->armed) {
+                    for (int c = 0; c < 4; ++c) led->expected[c] = tot[c] + tot[4 + c];
+                    led->armed = 1u;
+                } else {
+                    bool ok = true;
+                    for (int c = 0; c < 4; ++c) ok = ok && (led->expected[c] == tot[c] + tot[4 + c]);
+                    if (!ok) atomicMin(&led->first_bad, epoch);
+                }
+            }
+            led->blocks_done = 0u;
+        }
+    }
+}
+
+inline dim3 grid2(uint32_t n) { return dim3((n + 15) / 16, (n + 15) / 16); }
+const dim3 B16(16, 16);
+inline uint32_t blocks1(size_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+
+}  // namespace
+
+// ---- launchers -------------------------------------------------------------------------------
+void launch_slope(hipStream_t s, const Geo& g, float2* slope) { k_slope<<<grid2(g.G), B16, 0, s>>>(g, slope); }
+void launch_fluid_static(hipStream_t s, const Geo& g, uint8_t* cflags, float* fill, float* dragg, float2* slope_f) {
+    k_fluid_static<<<grid2(g.F), B16, 0, s>>>(g, cflags, fill, dragg, slope_f);
+}
+void launch_fluid_normals(hipStream_t s, const Geo& g, uint8_t* cflags, float2* normal) {
+    k_fluid_normals<<<grid2(g.F), B16, 0, s>>>(g, cflags, normal);
+}
+void launch_mg_mask0(hipStream_t s, const Geo& g, uint8_t* fluid0) {
+    k_mg_mask0<<<blocks1((size_t)g.F * g.F, 256), 256, 0, s>>>(g, fluid0);
+}
+void launch_mg_mask_coarsen(hipStream_t s, uint32_t nc, const uint8_t* fine, uint8_t* coarse) {
+    k_mg_mask_coarsen<<<blocks1((size_t)nc * nc, 256), 256, 0, s>>>(nc, fine, coarse);
+}
+void launch_mask_u32(hipStream_t s, const Geo& g, uint32_t* out) {
+    k_mask_u32<<<blocks1((size_t)g.F * g.F, 256), 256, 0, s>>>(g, out);
+}
+void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vel, const float* t_in,
+                        const float* r_in, float* t_out, float* r_out, uint32_t* eaten, int eaten_active,
+                        const uint32_t* chem) {
+    k_temperature<<<grid2(g.F), B16, 0, s>>>(g, P, vel, t_in, r_in, t_out, r_out, eaten, eaten_active, chem);
+}
+void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, float2* forces) {
+    k_forces<<<grid2(g.F), B16, 0, s>>>(g, P, temp, forces);
+}
+void launch_add_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* forces, const float2* vin,
+                       float2* vout) {
+    k_add_forces<<<grid2(g.F), B16, 0, s>>>(g, P, forces, vin, vout);
+}
+void launch_diffuse(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vin, float2* vout) {
+    k_diffuse<<<grid2(g.F), B16, 0, s>>>(g, P, vin, vout);
+}
+void launch_advect(hipStream_t s, const Geo& g, const RiboSimParams& P, float decay, const float2* vin, float2* vout) {
+    k_advect<<<grid2(g.F), B16, 0, s>>>(g, P, decay, vin, vout);
+}
+void launch_vorticity(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vin, float2* vout) {
+    k_vorticity<<<grid2(g.F), B16, 0, s>>>(g, P, vin, vout);
+}
+void launch_divergence(hipStream_t s, const Geo& g, const float2* vin, float* div, float* p0) {
+    k_divergence<<<grid2(g.F), B16, 0, s>>>(g, vin, div, p0);
+}
+void launch_jacobi(hipStream_t s, const Geo& g, const float* pin, float* pout, const float* div) {
+    k_jacobi<<<grid2(g.F), B16, 0, s>>>(g, pin, pout, div);
+}
+void launch_mg_smooth(hipStream_t s, const MgLevelDev& L, uint32_t color, const uint32_t* done) {
+    k_mg_smooth<<<grid2(L.n), B16, 0, s>>>(L, color, done);
+}
+void launch_mg_restrict(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
+    k_mg_restrict<<<grid2(C.n), B16, 0, s>>>(L, C, done);
+}
+void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
+    k_mg_prolong<<<grid2(L.n), B16, 0, s>>>(L, C, done);
+}
+void launch_mg_rhs_norm(hipStream_t s, const MgLevelDev& L0, Counters* c) {
+    k_counters_reset<<<1, 1, 0, s>>>(c);
+    uint32_t nb = rb_minu(blocks1((size_t)L0.n * L0.n, 256), 1024u);
+    k_mg_rhs_norm<<<nb, 256, 0, s>>>(L0, c);
+}
+void launch_mg_residual(hipStream_t s, const MgLevelDev& L0, Counters* c, float tol, uint32_t max_cycles, int count_cycle) {
+    uint32_t nb = rb_minu(blocks1((size_t)L0.n * L0.n, 256), 1024u);
+    k_mg_residual<<<nb, 256, 0, s>>>(L0, c, tol, max_cycles, count_cycle);
+}
+void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p) {
+    k_subgrad<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, p);
+}
+void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf) {
+    uint32_t nb = (g.F + NF_T - 1) / NF_T;
+    k_net_flux<<<2 * nb, NF_T, 0, s>>>(g, vin, nf);
+}
+void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf) {
+    k_boundaries<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, nf);
+}
+void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, uint32_t* chem) {
+    k_thermal<<<grid2(g.G), B16, 0, s>>>(g, P, temp, chem);
+}
+void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e) {
+    k_agg<<<grid2(g.G), B16, 0, s>>>(g, chem, agg_e);
+}
+void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
+    k_fill_u32<<<blocks1(n, 1024), 256, 0, s>>>(p, n, v);
+}
+void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t* next,
+                    const float2* vel, const float* temp, const uint32_t* agg_e) {
+    k_scatter<<<grid2(g.G), B16, 0, s>>>(g, P, chem, next, vel, temp, agg_e);
+}
+void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, int check) {
+    uint32_t cells = g.G * g.G;
+    uint32_t nb = rb_minu(blocks1(cells, 256), 2048u);
+    k_ledger<<<nb, 256, 0, s>>>(chem, cells, led, epoch, check);
+}
+
+}  // namespace ribo

@@ -1,0 +1,693 @@
+// world.hip - the world object, the per-step schedule and the C ABI of
+// libribo_world.so.
+//
+// The host side mirrors World (src/world/mod.rs): it owns every hot-path
+// buffer in HBM, keeps params/settings, builds the multigrid level table
+// (:635-671) and records the steps of encode_steps (:1406-1632) on one HIP
+// stream per world, with the reference's epoch, fluid_substep and
+// transport_every semantics.  Buffers that the reference copies between
+// steps ping-pong instead (chem/chem_next, temperature, reductant).
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/ribo_params_default.h"
+#include "../../include/ribo_world.h"
+#include "kernels.h"
+
+using namespace ribo;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+#define HIPCHK(expr)                                                                              \
+    do {                                                                                          \
+        hipError_t e_ = (expr);                                                                   \
+        if (e_ != hipSuccess) return fail(RIBO_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+constexpr uint32_t LIGHT_DIV = 2u;  // src/shaders.rs:132
+constexpr float MG_EPS = 1e-4f;     // src/world/mod.rs:202
+constexpr uint32_t MG_PRE = 2, MG_POST = 2, MG_COARSE = 30;  // :196-198
+constexpr float MAX_DT = 0.07f;     // fluid.wgsl:15
+
+// Host-side per-step values (src/params.rs:505-537), binary64 as the reference.
+void day_phase(const RiboSimParams& p, uint32_t epoch, bool* day, double* phase) {
+    double ph = (double)epoch / (double)p.day_period;
+    ph = ph - floor(ph);
+    double frac = std::min(std::max((double)p.day_fraction, 0.05), 1.0);
+    *day = ph < frac;
+    *phase = ph / frac;
+}
+float day_light(const RiboSimParams& p, uint32_t epoch) {
+    if (p.day_period < 1.0f) return 1.0f;
+    bool day;
+    double ph;
+    day_phase(p, epoch, &day, &ph);
+    if (!day) return 0.0f;
+    return (float)std::max(sin(M_PI * ph), 0.0);
+}
+float sun_slope_at(const RiboSimParams& p, uint32_t epoch) {
+    if (p.day_period < 1.0f) return 0.0f;
+    bool day;
+    double ph;
+    day_phase(p, epoch, &day, &ph);
+    double max_angle = std::min(std::max((double)p.sun_angle, 0.0), 85.0) * (M_PI / 180.0);
+    if (!day) ph = 1.0;
+    double angle = (ph - 0.5) * 2.0 * max_angle;
+    return (float)tan(angle);
+}
+// pow(clamp(fluid_decay, 0, 1), dt * 60) of advect_velocity (fluid.wgsl:473),
+// uniform over a step, evaluated once on the host in binary64.
+float decay_factor(float decay, float fluid_dt_param) {
+    float dt = std::min(std::max(fluid_dt_param, 0.0f), MAX_DT);
+    float d = std::min(std::max(decay, 0.0f), 1.0f);
+    return (float)pow((double)d, (double)(dt * 60.0f));
+}
+
+struct Timer {
+    const char* name;
+    double total_ms = 0;
+    uint64_t launches = 0;
+};
+
+}  // namespace
+
+struct RiboWorld {
+    RiboWorldConfig cfg{};
+    uint32_t G = 0, F = 0, Lsz = 0;
+    float sim_size = 0;
+    RiboSimParams params{};
+    RiboWorldSettings st{};
+    bool fluid_was_on = true;  // src/world/mod.rs:929
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // world-resolution buffers (bindings.wgsl group 1)
+    uint32_t* chem[2] = {nullptr, nullptr};  // chem_grid / chem_next, ping-pong
+    int cur = 0;
+    uint32_t* gamma = nullptr;
+    float* light = nullptr;
+    float2* slope = nullptr;
+    uint32_t* agg_e = nullptr;
+    // fluid-resolution buffers (group 2)
+    float2* vel_a = nullptr;
+    float2* vel_b = nullptr;
+    float* p_a = nullptr;
+    float* p_b = nullptr;
+    float* div = nullptr;
+    float* temp[2] = {nullptr, nullptr};
+    int tcur = 0;
+    float* redox[2] = {nullptr, nullptr};
+    uint32_t* eaten = nullptr;
+    bool eaten_active = false;
+    float2* forces = nullptr;
+    float2* heat = nullptr;
+    float* net_flux = nullptr;
+    // terrain-derived
+    uint8_t* cflags = nullptr;
+    float2* normal = nullptr;
+    float* fill = nullptr;
+    float* dragg = nullptr;
+    float2* slope_f = nullptr;
+    // multigrid (group 4); level 0 aliases p_a / div
+    std::vector<RiboMgLevel> levels;
+    std::vector<MgLevelDev> mg;
+    float* mg_p = nullptr;
+    float* mg_rhs = nullptr;
+    uint8_t* mg_fluid = nullptr;
+    Counters* counters = nullptr;
+    LedgerDev* ledger = nullptr;
+    uint32_t* scratch_u32 = nullptr;  // F words for downloads of derived masks
+
+    // timing
+    bool timing = false;
+    std::vector<Timer> timers;
+    struct Pending { size_t timer; hipEvent_t a, b; };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> event_pool;
+
+    Geo geo() const {
+        Geo g;
+        g.G = G; g.F = F; g.Lsz = Lsz; g.wupc = cfg.world_units_per_cell; g.sim_size = sim_size;
+        g.gamma = gamma; g.light = light; g.slope = slope; g.slope_f = slope_f; g.cflags = cflags;
+        g.normal = normal; g.fill = fill; g.dragg = dragg; g.heat = heat;
+        return g;
+    }
+};
+
+namespace {
+
+template <class T> int dalloc(T** p, size_t count) {
+    if (count == 0) count = 1;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) return fail(RIBO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    e = hipMemset(*p, 0, count * sizeof(T));
+    if (e != hipSuccess) return fail(RIBO_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
+    return RIBO_OK;
+}
+template <class T> void dfree(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+hipEvent_t get_event(RiboWorld* w) {
+    if (!w->event_pool.empty()) {
+        hipEvent_t e = w->event_pool.back();
+        w->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    (void)hipEventCreate(&e);
+    return e;
+}
+
+void resolve_timers(RiboWorld* w) {
+    for (auto& p : w->pending) {
+        float ms = 0;
+        if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+            w->timers[p.timer].total_ms += ms;
+            w->timers[p.timer].launches += 1;
+        }
+        w->event_pool.push_back(p.a);
+        w->event_pool.push_back(p.b);
+    }
+    w->pending.clear();
+}
+
+// Records `fn` (one kernel launch) and, when timing is on, an event pair
+// around it on the world's stream.
+template <class Fn> void run(RiboWorld* w, const char* name, Fn&& fn) {
+    if (!w->timing) {
+        fn();
+        return;
+    }
+    size_t t = 0;
+    for (; t < w->timers.size(); ++t)
+        if (w->timers[t].name == name) break;
+    if (t == w->timers.size()) w->timers.push_back(Timer{name});
+    hipEvent_t a = get_event(w), b = get_event(w);
+    (void)hipEventRecord(a, w->stream);
+    fn();
+    (void)hipEventRecord(b, w->stream);
+    w->pending.push_back({t, a, b});
+    if (w->pending.size() > 4096) resolve_timers(w);
+}
+
+// Multigrid level table (src/world/mod.rs:637-671): sizes halve while > min
+// and even; eps_l = MG_EPS * 2^l.
+std::vector<RiboMgLevel> make_levels(uint32_t n0, uint32_t min_size) {
+    std::vector<uint32_t> sizes{n0};
+    while (sizes.back() > min_size && sizes.back() % 2 == 0) sizes.push_back(sizes.back() / 2);
+    std::vector<uint32_t> offs{0};
+    for (uint32_t n : sizes) offs.push_back(offs.back() + n * n);
+    std::vector<RiboMgLevel> lv;
+    for (size_t l = 0; l < sizes.size(); ++l) {
+        RiboMgLevel L{};
+        L.n = sizes[l];
+        L.off = offs[l];
+        L.n_c = l + 1 < sizes.size() ? sizes[l + 1] : 0;
+        L.off_c = offs[l + 1];
+        L.eps = MG_EPS * (float)(1u << l);
+        lv.push_back(L);
+    }
+    return lv;
+}
+
+int build_mg(RiboWorld* w) {
+    dfree(w->mg_p);
+    dfree(w->mg_rhs);
+    dfree(w->mg_fluid);
+    w->levels = make_levels(w->F, std::max(w->st.mg_min_size, 1u));
+    size_t total = w->levels.back().off_c;
+    size_t fine = (size_t)w->F * w->F;
+    size_t coarse = total - fine;
+    int rc;
+    if ((rc = dalloc(&w->mg_p, coarse)) || (rc = dalloc(&w->mg_rhs, coarse)) || (rc = dalloc(&w->mg_fluid, total))) return rc;
+    w->mg.clear();
+    for (size_t l = 0; l < w->levels.size(); ++l) {
+        MgLevelDev d;
+        d.n = w->levels[l].n;
+        d.eps = w->levels[l].eps;
+        if (l == 0) {
+            d.p = w->p_a;
+            d.rhs = w->div;
+        } else {
+            d.p = w->mg_p + (w->levels[l].off - fine);
+            d.rhs = w->mg_rhs + (w->levels[l].off - fine);
+        }
+        d.fluid = w->mg_fluid + w->levels[l].off;
+        w->mg.push_back(d);
+    }
+    return RIBO_OK;
+}
+
+// Terrain-derived data, rebuilt whenever gamma or the level table changes.
+int rebuild_terrain(RiboWorld* w) {
+    Geo g = w->geo();
+    launch_slope(w->stream, g, w->slope);
+    launch_fluid_static(w->stream, g, w->cflags, w->fill, w->dragg, w->slope_f);
+    launch_fluid_normals(w->stream, g, w->cflags, w->normal);
+    launch_mg_mask0(w->stream, g, w->mg_fluid);
+    for (size_t l = 1; l < w->levels.size(); ++l)
+        launch_mg_mask_coarsen(w->stream, w->levels[l].n, w->mg_fluid + w->levels[l - 1].off, w->mg_fluid + w->levels[l].off);
+    HIPCHK(hipGetLastError());
+    return RIBO_OK;
+}
+
+void vcycle(RiboWorld* w, const uint32_t* done) {  // src/world/mod.rs:1555-1575
+    const size_t L = w->mg.size();
+    hipStream_t s = w->stream;
+    for (size_t l = 0; l + 1 < L; ++l) {
+        for (uint32_t k = 0; k < MG_PRE; ++k) {
+            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 0, done); });
+            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 1, done); });
+        }
+        run(w, "mg_restrict", [&] { launch_mg_restrict(s, w->mg[l], w->mg[l + 1], done); });
+    }
+    for (uint32_t k = 0; k < MG_COARSE; ++k) {
+        run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 0, done); });
+        run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 1, done); });
+    }
+    for (size_t l = L - 1; l-- > 0;) {
+        run(w, "mg_prolong", [&] { launch_mg_prolong(s, w->mg[l], w->mg[l + 1], done); });
+        for (uint32_t k = 0; k < MG_POST; ++k) {
+            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 0, done); });
+            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 1, done); });
+        }
+    }
+}
+
+void fluid_step(RiboWorld* w, const RiboSimParams& P) {  // src/world/mod.rs:1527-1588
+    hipStream_t s = w->stream;
+    Geo g = w->geo();
+    const int tc = w->tcur;
+    run(w, "update_temperature", [&] {
+        launch_temperature(s, g, P, w->vel_a, w->temp[tc], w->redox[tc], w->temp[1 - tc], w->redox[1 - tc], w->eaten,
+                           w->eaten_active ? 1 : 0, w->chem[w->cur]);
+    });
+    w->eaten_active = false;
+    w->tcur = 1 - tc;  // copy_temperature
+    const float* T = w->temp[w->tcur];
+    run(w, "buoyancy_gather", [&] { launch_forces(s, g, P, T, w->forces); });
+    run(w, "add_forces", [&] { launch_add_forces(s, g, P, w->forces, w->vel_a, w->vel_b); });
+    run(w, "diffuse_velocity", [&] { launch_diffuse(s, g, P, w->vel_b, w->vel_a); });
+    const float decay = decay_factor(P.fluid_decay, P.fluid_dt);
+    run(w, "advect_velocity", [&] { launch_advect(s, g, P, decay, w->vel_a, w->vel_b); });
+    run(w, "vorticity", [&] { launch_vorticity(s, g, P, w->vel_b, w->vel_a); });
+    run(w, "divergence", [&] { launch_divergence(s, g, w->vel_a, w->div, w->p_a); });
+    if (w->st.multigrid) {
+        if (w->st.mg_tol > 0.0f) {
+            const uint32_t maxc = std::max(w->st.mg_max_cycles, 1u);
+            uint32_t* done = &w->counters->done;
+            run(w, "mg_residual", [&] { launch_mg_rhs_norm(s, w->mg[0], w->counters); });
+            run(w, "mg_residual", [&] { launch_mg_residual(s, w->mg[0], w->counters, w->st.mg_tol, maxc, 0); });
+            for (uint32_t c = 0; c < maxc; ++c) {
+                vcycle(w, done);
+                run(w, "mg_residual", [&] { launch_mg_residual(s, w->mg[0], w->counters, w->st.mg_tol, maxc, 1); });
+            }
+        } else {
+            const uint32_t cycles = std::max(w->st.mg_cycles, 1u);
+            for (uint32_t c = 0; c < cycles; ++c) vcycle(w, nullptr);
+        }
+    } else {
+        const uint32_t iters = (std::min(std::max(w->st.jacobi_iters, 2u), 128u) + 1u) & ~1u;  // :1442
+        for (uint32_t k = 0; k < iters; ++k) {
+            if (k % 2 == 0) run(w, "jacobi", [&] { launch_jacobi(s, g, w->p_a, w->p_b, w->div); });
+            else run(w, "jacobi", [&] { launch_jacobi(s, g, w->p_b, w->p_a, w->div); });
+        }
+    }
+    run(w, "subtract_gradient", [&] { launch_subgrad(s, g, w->vel_a, w->vel_b, w->p_a); });
+    run(w, "net_flux", [&] { launch_net_flux(s, g, w->vel_b, w->net_flux); });
+    run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux); });
+    run(w, "thermal_activation", [&] { launch_thermal(s, g, P, T, w->chem[w->cur]); });
+}
+
+void transport_step(RiboWorld* w, const RiboSimParams& P) {  // src/world/mod.rs:1594-1601
+    hipStream_t s = w->stream;
+    Geo g = w->geo();
+    uint32_t* cur = w->chem[w->cur];
+    uint32_t* nxt = w->chem[1 - w->cur];
+    if (P.aggregation > 0.0f) run(w, "agg", [&] { launch_agg(s, g, cur, w->agg_e); });
+    run(w, "clear_next", [&] { launch_fill_u32(s, nxt, (size_t)4 * w->G * w->G, 0u); });
+    run(w, "transport_scatter", [&] { launch_scatter(s, g, P, cur, nxt, w->vel_a, w->temp[w->tcur], w->agg_e); });
+    w->cur = 1 - w->cur;  // transport_commit
+    if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, w->chem[w->cur], w->ledger, P.epoch, 1); });
+}
+
+int validate_settings(const RiboWorldSettings* s) {
+    if (s->terrain_enabled) return fail(RIBO_ERR_UNSUPPORTED, "terrain physics is not part of the world-step path (terrain_enabled must be 0)");
+    if (s->precision != RIBO_PRECISION_FP32 && s->precision != RIBO_PRECISION_MIXED)
+        return fail(RIBO_ERR_ARG, "unknown precision");
+    if (s->precision == RIBO_PRECISION_MIXED) return fail(RIBO_ERR_UNSUPPORTED, "mixed precision multigrid is not built yet");
+    if (!(s->mg_tol >= 0.0f)) return fail(RIBO_ERR_ARG, "mg_tol must be >= 0");
+    return RIBO_OK;
+}
+
+bool buffer_size_ok(RiboWorld* w, RiboBuffer which, size_t bytes) { return bytes == ribo_buffer_bytes(&w->cfg, which) && bytes > 0; }
+
+}  // namespace
+
+// ============================================================================
+extern "C" {
+
+const char* ribo_last_error(void) { return g_err.c_str(); }
+uint32_t ribo_abi_version(void) { return RIBO_ABI_VERSION; }
+
+void ribo_sim_params_default(RiboSimParams* out) {
+    if (out) ribo_sim_params_fill_default(out);
+}
+
+void ribo_world_settings_default(RiboWorldSettings* s) {  // src/world/mod.rs:121-134
+    if (!s) return;
+    memset(s, 0, sizeof(*s));
+    s->fluid_enabled = 1;
+    s->fluid_substep = 2;
+    s->multigrid = 1;
+    s->mg_cycles = 1;
+    s->jacobi_iters = 128;
+    s->light_interval = 10;
+    s->light_rows_per_step = 1;
+    s->terrain_enabled = 0;
+    s->contact_enabled = 1;
+    s->mg_min_size = 8;
+    s->mg_tol = 0.0f;
+    s->mg_max_cycles = 1;
+    s->precision = RIBO_PRECISION_FP32;
+    s->ledger_check = 0;
+}
+
+size_t ribo_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which) {
+    if (!cfg) return 0;
+    size_t G = (size_t)cfg->grid_size * cfg->grid_size, F = (size_t)cfg->fluid_size * cfg->fluid_size;
+    size_t L = (size_t)(cfg->grid_size / LIGHT_DIV) * (cfg->grid_size / LIGHT_DIV);
+    switch (which) {
+        case RIBO_BUF_CHEM: return 16 * G;
+        case RIBO_BUF_GAMMA: return 4 * G;
+        case RIBO_BUF_LIGHT: return 8 * L;
+        case RIBO_BUF_SLOPE: return 8 * G;
+        case RIBO_BUF_VELOCITY: return 8 * F;
+        case RIBO_BUF_PRESSURE: return 4 * F;
+        case RIBO_BUF_TEMP: return 4 * F;
+        case RIBO_BUF_REDOX: return 4 * F;
+        case RIBO_BUF_HEAT_SRC: return 8 * F;
+        case RIBO_BUF_DIVERGENCE: return 4 * F;
+        case RIBO_BUF_SOLID_MASK: return 4 * F;
+        case RIBO_BUF_NET_FLUX: return 8 * (size_t)cfg->fluid_size;
+        case RIBO_BUF_REDOX_EATEN: return 4 * F;
+        case RIBO_BUF_AGG: return 4 * G;
+        default: return 0;
+    }
+}
+
+int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out) {
+    if (!cfg || !out) return fail(RIBO_ERR_ARG, "null argument");
+    *out = nullptr;
+    if (cfg->grid_size == 0 || cfg->fluid_size == 0 || cfg->grid_size % cfg->fluid_size != 0 || cfg->grid_size < 2 ||
+        cfg->world_units_per_cell == 0 || cfg->grid_size > 32768)
+        return fail(RIBO_ERR_ARG, "grid_size must be a non-zero multiple of fluid_size (<= 32768) and world_units_per_cell > 0");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(RIBO_ERR_NO_DEVICE, "no HIP device visible");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return fail(RIBO_ERR_NO_DEVICE, std::string("this library is built for gfx950; device is ") + prop.gcnArchName);
+
+    RiboWorld* w = new RiboWorld();
+    w->cfg = *cfg;
+    w->G = cfg->grid_size;
+    w->F = cfg->fluid_size;
+    w->Lsz = cfg->grid_size / LIGHT_DIV;
+    w->sim_size = (float)(cfg->grid_size * cfg->world_units_per_cell);
+    w->device = dev;
+    ribo_sim_params_fill_default(&w->params);
+    w->params.seed = seed;
+    ribo_world_settings_default(&w->st);
+    size_t G = (size_t)w->G * w->G, F = (size_t)w->F * w->F;
+    int rc;
+    hipError_t he = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+    if (he != hipSuccess) {
+        delete w;
+        return fail(RIBO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
+    }
+    if ((rc = dalloc(&w->chem[0], 4 * G)) || (rc = dalloc(&w->chem[1], 4 * G)) || (rc = dalloc(&w->gamma, G)) ||
+        (rc = dalloc(&w->light, 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(&w->slope, G)) || (rc = dalloc(&w->agg_e, G)) ||
+        (rc = dalloc(&w->vel_a, F)) || (rc = dalloc(&w->vel_b, F)) || (rc = dalloc(&w->p_a, F)) || (rc = dalloc(&w->p_b, F)) ||
+        (rc = dalloc(&w->div, F)) || (rc = dalloc(&w->temp[0], F)) || (rc = dalloc(&w->temp[1], F)) ||
+        (rc = dalloc(&w->redox[0], F)) || (rc = dalloc(&w->redox[1], F)) || (rc = dalloc(&w->eaten, F)) ||
+        (rc = dalloc(&w->forces, F)) || (rc = dalloc(&w->heat, F)) || (rc = dalloc(&w->net_flux, 2 * (size_t)w->F)) ||
+        (rc = dalloc(&w->cflags, F)) || (rc = dalloc(&w->normal, F)) || (rc = dalloc(&w->fill, F)) ||
+        (rc = dalloc(&w->dragg, F)) || (rc = dalloc(&w->slope_f, F)) || (rc = dalloc(&w->counters, 1)) ||
+        (rc = dalloc(&w->ledger, 1)) || (rc = dalloc(&w->scratch_u32, F)) || (rc = build_mg(w))) {
+        ribo_world_destroy(w);
+        return rc;
+    }
+    LedgerDev init{};
+    init.first_bad = 0xFFFFFFFFu;
+    he = hipMemcpy(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice);
+    if (he != hipSuccess || (rc = rebuild_terrain(w)) != RIBO_OK || hipStreamSynchronize(w->stream) != hipSuccess) {
+        ribo_world_destroy(w);
+        return fail(RIBO_ERR_HIP, "world initialisation failed");
+    }
+    *out = w;
+    return RIBO_OK;
+}
+
+int ribo_world_destroy(RiboWorld* w) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    if (w->stream) (void)hipStreamSynchronize(w->stream);
+    resolve_timers(w);
+    for (auto e : w->event_pool) (void)hipEventDestroy(e);
+    dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light); dfree(w->slope); dfree(w->agg_e);
+    dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
+    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
+    dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
+    dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->counters); dfree(w->ledger); dfree(w->scratch_u32);
+    if (w->stream) (void)hipStreamDestroy(w->stream);
+    delete w;
+    return RIBO_OK;
+}
+
+int ribo_world_set_params(RiboWorld* w, const RiboSimParams* p) {
+    if (!w || !p) return fail(RIBO_ERR_ARG, "null argument");
+    w->params = *p;
+    return RIBO_OK;
+}
+int ribo_world_get_params(const RiboWorld* w, RiboSimParams* p) {
+    if (!w || !p) return fail(RIBO_ERR_ARG, "null argument");
+    *p = w->params;
+    return RIBO_OK;
+}
+int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s) {
+    if (!w || !s) return fail(RIBO_ERR_ARG, "null argument");
+    int rc = validate_settings(s);
+    if (rc) return rc;
+    const bool relevel = std::max(s->mg_min_size, 1u) != std::max(w->st.mg_min_size, 1u);
+    const bool arm = s->ledger_check && !w->st.ledger_check;
+    w->st = *s;
+    if (relevel) {
+        HIPCHK(hipStreamSynchronize(w->stream));
+        if ((rc = build_mg(w)) || (rc = rebuild_terrain(w))) return rc;
+    }
+    if (arm) {
+        LedgerDev init{};
+        init.first_bad = 0xFFFFFFFFu;
+        HIPCHK(hipMemcpyAsync(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice, w->stream));
+        // arm the expected per-channel totals with the current grid
+        launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, 1);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(w->stream));
+    }
+    return RIBO_OK;
+}
+int ribo_world_get_settings(const RiboWorld* w, RiboWorldSettings* s) {
+    if (!w || !s) return fail(RIBO_ERR_ARG, "null argument");
+    *s = w->st;
+    return RIBO_OK;
+}
+int ribo_world_mg_levels(const RiboWorld* w, RiboMgLevel* out, uint32_t cap, uint32_t* count) {
+    if (!w || !count) return fail(RIBO_ERR_ARG, "null argument");
+    *count = (uint32_t)w->levels.size();
+    for (uint32_t i = 0; i < cap && i < *count; ++i) out[i] = w->levels[i];
+    return RIBO_OK;
+}
+
+int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t bytes) {
+    if (!w || !src) return fail(RIBO_ERR_ARG, "null argument");
+    if (!buffer_size_ok(w, which, bytes)) return fail(RIBO_ERR_ARG, "buffer size does not match the world configuration");
+    void* dst = nullptr;
+    switch (which) {
+        case RIBO_BUF_CHEM: dst = w->chem[w->cur]; break;
+        case RIBO_BUF_GAMMA: dst = w->gamma; break;
+        case RIBO_BUF_LIGHT: dst = w->light; break;
+        case RIBO_BUF_VELOCITY: dst = w->vel_a; break;
+        case RIBO_BUF_PRESSURE: dst = w->p_a; break;
+        case RIBO_BUF_TEMP: dst = w->temp[w->tcur]; break;
+        case RIBO_BUF_REDOX: dst = w->redox[w->tcur]; break;
+        case RIBO_BUF_HEAT_SRC: dst = w->heat; break;
+        case RIBO_BUF_REDOX_EATEN: dst = w->eaten; w->eaten_active = true; break;
+        default: return fail(RIBO_ERR_ARG, "buffer is derived and cannot be uploaded");
+    }
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w->stream));
+    if (which == RIBO_BUF_GAMMA) {
+        int rc = rebuild_terrain(w);
+        if (rc) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return RIBO_OK;
+}
+
+int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes) {
+    if (!w || !dst) return fail(RIBO_ERR_ARG, "null argument");
+    if (!buffer_size_ok(w, which, bytes)) return fail(RIBO_ERR_ARG, "buffer size does not match the world configuration");
+    const void* src = nullptr;
+    switch (which) {
+        case RIBO_BUF_CHEM: src = w->chem[w->cur]; break;
+        case RIBO_BUF_GAMMA: src = w->gamma; break;
+        case RIBO_BUF_LIGHT: src = w->light; break;
+        case RIBO_BUF_SLOPE: src = w->slope; break;
+        case RIBO_BUF_VELOCITY: src = w->vel_a; break;
+        case RIBO_BUF_PRESSURE: src = w->p_a; break;
+        case RIBO_BUF_TEMP: src = w->temp[w->tcur]; break;
+        case RIBO_BUF_REDOX: src = w->redox[w->tcur]; break;
+        case RIBO_BUF_HEAT_SRC: src = w->heat; break;
+        case RIBO_BUF_DIVERGENCE: src = w->div; break;
+        case RIBO_BUF_NET_FLUX: src = w->net_flux; break;
+        case RIBO_BUF_REDOX_EATEN: src = w->eaten; break;
+        case RIBO_BUF_AGG: src = w->agg_e; break;
+        case RIBO_BUF_SOLID_MASK:
+            launch_mask_u32(w->stream, w->geo(), w->scratch_u32);
+            HIPCHK(hipGetLastError());
+            src = w->scratch_u32;
+            break;
+        default: return fail(RIBO_ERR_ARG, "unknown buffer");
+    }
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return RIBO_OK;
+}
+
+int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs:1406-1446,1622,1631
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    int rc = validate_settings(&w->st);
+    if (rc) return rc;
+    steps = std::min(steps, RIBO_MAX_STEPS_PER_CALL);
+    if (steps == 0) return 0;
+    const RiboWorldSettings st = w->st;
+    const uint32_t substep = std::min(std::max(st.fluid_substep, 1u), 4u);
+    if (!st.fluid_enabled && w->fluid_was_on) {
+        const size_t F = (size_t)w->F * w->F;
+        launch_fill_u32(w->stream, (uint32_t*)w->vel_a, 2 * F, 0u);
+        launch_fill_u32(w->stream, (uint32_t*)w->temp[w->tcur], F, 0u);
+        launch_fill_u32(w->stream, (uint32_t*)w->redox[w->tcur], F, 0u);
+    }
+    w->fluid_was_on = st.fluid_enabled != 0;
+    w->params.fluid_dt = w->params.dt * (float)substep;
+    w->params.fluid_enabled = st.fluid_enabled ? 1u : 0u;
+    for (uint32_t i = 0; i < steps; ++i) {
+        RiboSimParams P = w->params;
+        P.epoch = w->params.epoch + i;
+        P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, P.epoch) : 0.0f;
+        P.sun_slope = sun_slope_at(w->params, P.epoch);
+        if (st.fluid_enabled && P.epoch % substep == 0) fluid_step(w, P);
+        if (P.epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P);
+    }
+    HIPCHK(hipGetLastError());
+    w->params.epoch += steps;
+    return (int)steps;
+}
+
+int ribo_world_run(RiboWorld* w, uint32_t steps) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    uint32_t done = 0;
+    while (done < steps) {
+        int n = ribo_world_encode_steps(w, std::min(steps - done, RIBO_MAX_STEPS_PER_CALL));
+        if (n < 0) return n;
+        done += (uint32_t)n;
+    }
+    return RIBO_OK;
+}
+
+int ribo_world_sync(RiboWorld* w) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return RIBO_OK;
+}
+
+int ribo_world_ledger(RiboWorld* w, uint32_t out[12]) {  // encode_ledger + ledger_blocking (src/world/mod.rs:1661-1674,1834-1839)
+    if (!w || !out) return fail(RIBO_ERR_ARG, "null argument");
+    launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, 0);
+    HIPCHK(hipGetLastError());
+    uint32_t last[8];
+    HIPCHK(hipMemcpyAsync(last, w->ledger->last, sizeof(last), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    for (int i = 0; i < 8; ++i) out[i] = last[i];
+    for (int i = 8; i < 12; ++i) out[i] = 0u;
+    return RIBO_OK;
+}
+
+int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch) {
+    if (!w || !epoch) return fail(RIBO_ERR_ARG, "null argument");
+    uint32_t fb = 0;
+    HIPCHK(hipMemcpyAsync(&fb, &w->ledger->first_bad, sizeof(fb), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    *epoch = fb;
+    if (fb != 0xFFFFFFFFu) return fail(RIBO_ERR_LEDGER, "per-channel monomer totals changed at epoch " + std::to_string(fb));
+    return RIBO_OK;
+}
+
+int ribo_world_set_timing(RiboWorld* w, int enable) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    w->timing = enable != 0;
+    return RIBO_OK;
+}
+int ribo_world_kernel_times(RiboWorld* w, const char** names, double* total_ms, uint64_t* launches, uint32_t cap,
+                            uint32_t* count) {
+    if (!w || !count) return fail(RIBO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(w->stream));
+    resolve_timers(w);
+    *count = (uint32_t)w->timers.size();
+    for (uint32_t i = 0; i < cap && i < *count; ++i) {
+        if (names) names[i] = w->timers[i].name;
+        if (total_ms) total_ms[i] = w->timers[i].total_ms;
+        if (launches) launches[i] = w->timers[i].launches;
+    }
+    return RIBO_OK;
+}
+int ribo_world_reset_kernel_times(RiboWorld* w) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    HIPCHK(hipStreamSynchronize(w->stream));
+    resolve_timers(w);
+    w->timers.clear();
+    return RIBO_OK;
+}
+int ribo_world_mg_stats(RiboWorld* w, uint32_t* cycles, double* rel_residual) {
+    if (!w || !cycles || !rel_residual) return fail(RIBO_ERR_ARG, "null argument");
+    Counters c;
+    HIPCHK(hipMemcpyAsync(&c, w->counters, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    if (w->st.mg_tol > 0.0f) {
+        *cycles = c.cycles;
+        *rel_residual = c.last_rel;
+    } else {
+        *cycles = std::max(w->st.mg_cycles, 1u);
+        *rel_residual = -1.0;
+    }
+    return RIBO_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,520 @@
+"""ribossome_amd - Python mirror of the reference World API over libribo_world.so.
+
+Names follow the reference (src/world/mod.rs, src/params.rs): ``WorldConfig``,
+``SimParams``, ``WorldSettings``, ``Ledger``, ``World`` with ``encode_steps``,
+``ledger_blocking``, ``read_cells_blocking`` and ``read_f32_blocking``.  Every
+call goes through the C ABI declared in include/ribo_world.h; there is no
+fallback: if the HIP library is missing or no gfx950 device is visible, the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libribo_world.so")
+
+_SIMPARAMS_FIELDS = (
+    ('epoch', 'u32'),
+    ('seed', 'u32'),
+    ('dt', 'f32'),
+    ('diffusion', 'f32'),
+    ('settle', 'f32'),
+    ('fluid_dt', 'f32'),
+    ('fluid_decay', 'f32'),
+    ('fluid_vorticity', 'f32'),
+    ('fluid_viscosity', 'f32'),
+    ('fumarole_count', 'u32'),
+    ('uv_strength', 'f32'),
+    ('uv_depth', 'f32'),
+    ('fluid_enabled', 'u32'),
+    ('fluid_obstacle_strength', 'f32'),
+    ('slope_steer_rate', 'f32'),
+    ('cohesion', 'f32'),
+    ('spawn_count', 'u32'),
+    ('max_agents', 'u32'),
+    ('death_probability', 'f32'),
+    ('spawn_energy', 'f32'),
+    ('food_power', 'f32'),
+    ('maintenance_cost', 'f32'),
+    ('pairing_rate', 'f32'),
+    ('mutation_rate', 'f32'),
+    ('uv_damage', 'f32'),
+    ('uptake_rate', 'f32'),
+    ('brownian', 'f32'),
+    ('phoretic_gain', 'f32'),
+    ('chain_stiffness', 'f32'),
+    ('thermal_kt', 'f32'),
+    ('motor_amplitude', 'f32'),
+    ('rft_enabled', 'u32'),
+    ('require_start', 'u32'),
+    ('joint_coupling', 'f32'),
+    ('hunger_regulation', 'u32'),
+    ('pairing_cost', 'f32'),
+    ('reactivation_rate', 'f32'),
+    ('swim_gain', 'f32'),
+    ('bioturbation', 'f32'),
+    ('bioturbation_cost', 'f32'),
+    ('monomer_uv_absorb', 'f32'),
+    ('sedimentation', 'f32'),
+    ('direct_photoactivation', 'f32'),
+    ('activation_decay', 'f32'),
+    ('monomer_pressure', 'f32'),
+    ('agent_fluid_push', 'f32'),
+    ('swim_wobble', 'f32'),
+    ('fluid_swim_only', 'u32'),
+    ('motion_cost', 'f32'),
+    ('bond_rate', 'f32'),
+    ('bond_energy_share', 'f32'),
+    ('bond_signal', 'f32'),
+    ('aggregation', 'f32'),
+    ('photo_yield', 'f32'),
+    ('heat_kill', 'f32'),
+    ('flow_coupling', 'f32'),
+    ('inertia', 'f32'),
+    ('death_energy_cap', 'f32'),
+    ('chemo_yield', 'f32'),
+    ('redox_decay', 'f32'),
+    ('thermal_activation', 'f32'),
+    ('denature_temp', 'f32'),
+    ('day_period', 'f32'),
+    ('sun_slope', 'f32'),
+    ('sun_now', 'f32'),
+    ('sun_angle', 'f32'),
+    ('day_fraction', 'f32'),
+    ('metabolic_q10', 'f32'),
+    ('metabolic_span', 'f32'),
+    ('metabolic_ref', 'f32'),
+    ('sun_heat', 'f32'),
+    ('sediment_transport', 'f32'),
+    ('sediment_threshold', 'f32'),
+    ('sediment_settle', 'f32'),
+    ('signal_mode', 'f32'),
+    ('clock_mute', 'f32'),
+    ('paint_x', 'f32'),
+    ('paint_y', 'f32'),
+    ('paint_radius', 'f32'),
+    ('paint_grains', 'f32'),
+    ('rest_angle_mult', 'f32'),
+    ('protease_power', 'f32'),
+    ('lysis_yield', 'f32'),
+    ('protease_direct', 'f32'),
+    ('joint_load', 'f32'),
+    ('swim_grip', 'f32'),
+    ('swim_memory', 'f32'),
+    ('mouth_leak', 'f32'),
+    ('leak_base', 'f32'),
+    ('skin_uptake', 'f32'),
+    ('death_metab', 'f32'),
+    ('salvage', 'f32'),
+    ('bond_matter_share', 'f32'),
+    ('brownian_rot', 'f32'),
+    ('transport_every', 'u32'),
+    ('signal_crosstalk', 'f32'),
+    ('flow_mass', 'f32'),
+    ('chemo_take', 'f32'),
+    ('vent_cycle_period', 'f32'),
+    ('vent_cycle_amp', 'f32'),
+    ('sediment_cohesion', 'f32'),
+    ('sediment_compaction', 'f32'),
+    ('rock_crumble', 'f32'),
+    ('lysis_matter', 'f32'),
+    ('scramble_conduction', 'u32'),
+    ('scramble_angles', 'u32'),
+    ('scramble_response', 'u32'),
+    ('scramble_amount', 'f32'),
+    ('pairing_energy_half', 'f32'),
+    ('pairing_min_energy', 'f32'),
+    ('uv_mutation', 'f32'),
+    ('heat_mutation', 'f32'),
+    ('birth_crowding', 'f32'),
+    ('photo_damage', 'f32'),
+    ('naked_decay', 'f32'),
+    ('store_loss', 'f32'),
+)
+
+
+_CT = {"u32": ctypes.c_uint32, "f32": ctypes.c_float}
+
+
+class SimParams(ctypes.Structure):
+    """SimParams (src/params.rs:86-487): 116 four-byte fields, 464 bytes."""
+
+    _fields_ = [(n, _CT[t]) for n, t in _SIMPARAMS_FIELDS]
+
+    @classmethod
+    def default(cls, seed: int = 1) -> "SimParams":
+        p = cls()
+        lib().ribo_sim_params_default(ctypes.byref(p))
+        p.seed = seed
+        return p
+
+    def copy(self) -> "SimParams":
+        q = SimParams()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(self), ctypes.sizeof(self))
+        return q
+
+
+class WorldSettings(ctypes.Structure):
+    """WorldSettings (src/world/mod.rs:91-134) plus the solver extensions."""
+
+    _fields_ = [
+        ("fluid_enabled", ctypes.c_uint32),
+        ("fluid_substep", ctypes.c_uint32),
+        ("multigrid", ctypes.c_uint32),
+        ("mg_cycles", ctypes.c_uint32),
+        ("jacobi_iters", ctypes.c_uint32),
+        ("light_rows_per_step", ctypes.c_uint32),
+        ("light_interval", ctypes.c_uint32),
+        ("terrain_enabled", ctypes.c_uint32),
+        ("contact_enabled", ctypes.c_uint32),
+        ("mg_min_size", ctypes.c_uint32),
+        ("mg_tol", ctypes.c_float),
+        ("mg_max_cycles", ctypes.c_uint32),
+        ("precision", ctypes.c_uint32),
+        ("ledger_check", ctypes.c_uint32),
+        ("_pad0", ctypes.c_uint32),
+        ("_pad1", ctypes.c_uint32),
+    ]
+
+    @classmethod
+    def default(cls) -> "WorldSettings":
+        s = cls()
+        lib().ribo_world_settings_default(ctypes.byref(s))
+        return s
+
+    def copy(self) -> "WorldSettings":
+        q = WorldSettings()
+        ctypes.memmove(ctypes.byref(q), ctypes.byref(self), ctypes.sizeof(self))
+        return q
+
+
+class _CWorldConfig(ctypes.Structure):
+    _fields_ = [("grid_size", ctypes.c_uint32), ("fluid_size", ctypes.c_uint32),
+                ("world_units_per_cell", ctypes.c_uint32), ("max_agents", ctypes.c_uint32)]
+
+
+class MgLevel(ctypes.Structure):
+    """MgLevel (src/params.rs:727-740)."""
+
+    _fields_ = [("n", ctypes.c_uint32), ("off", ctypes.c_uint32), ("n_c", ctypes.c_uint32),
+                ("off_c", ctypes.c_uint32), ("eps", ctypes.c_float), ("_pad0", ctypes.c_uint32),
+                ("_pad1", ctypes.c_uint32), ("_pad2", ctypes.c_uint32)]
+
+
+@dataclass(frozen=True)
+class WorldConfig:
+    """WorldConfig (src/params.rs:1003-1034)."""
+
+    grid_size: int
+    fluid_size: int
+    world_units_per_cell: int = 30
+    max_agents: int = 0
+
+    def c(self) -> _CWorldConfig:
+        return _CWorldConfig(self.grid_size, self.fluid_size, self.world_units_per_cell, self.max_agents)
+
+    def cells(self) -> int:
+        return self.grid_size * self.grid_size
+
+    def fluid_cells(self) -> int:
+        return self.fluid_size * self.fluid_size
+
+    def sim_size(self) -> float:
+        return float(self.grid_size * self.world_units_per_cell)
+
+
+WorldConfig.DEFAULT = WorldConfig(2048, 1024, 30, 400_000)
+WorldConfig.TEST = WorldConfig(256, 128, 30, 4096)
+
+# Buffer ids (RiboBuffer)
+CHEM, GAMMA, LIGHT, SLOPE, VELOCITY, PRESSURE, TEMP, REDOX, HEAT_SRC, DIVERGENCE, SOLID_MASK, NET_FLUX, \
+    REDOX_EATEN, AGG = range(14)
+_BUF_DTYPE = {CHEM: np.uint32, GAMMA: np.uint32, LIGHT: np.float32, SLOPE: np.float32, VELOCITY: np.float32,
+              PRESSURE: np.float32, TEMP: np.float32, REDOX: np.float32, HEAT_SRC: np.float32,
+              DIVERGENCE: np.float32, SOLID_MASK: np.uint32, NET_FLUX: np.float32, REDOX_EATEN: np.uint32,
+              AGG: np.uint32}
+BUFFER_NAMES = {CHEM: "chem", GAMMA: "gamma", LIGHT: "light", SLOPE: "slope", VELOCITY: "velocity",
+                PRESSURE: "pressure", TEMP: "temp", REDOX: "redox", HEAT_SRC: "heat_src",
+                DIVERGENCE: "divergence", SOLID_MASK: "solid_mask", NET_FLUX: "net_flux",
+                REDOX_EATEN: "redox_eaten", AGG: "agg"}
+MAX_STEPS_PER_FRAME = 64
+
+
+class RiboError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libribo_world.so (built in-tree by ``make -C ribossome-v2_amd``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RiboError(f"{LIB_PATH} is missing: build it with `make -C ribossome-v2_amd`")
+    L = ctypes.CDLL(LIB_PATH)
+    P, U32, I, SZ = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_size_t
+    sig = {
+        "ribo_sim_params_default": (None, [P]),
+        "ribo_world_settings_default": (None, [P]),
+        "ribo_buffer_bytes": (SZ, [P, I]),
+        "ribo_world_create": (I, [P, U32, ctypes.POINTER(P)]),
+        "ribo_world_destroy": (I, [P]),
+        "ribo_world_set_params": (I, [P, P]),
+        "ribo_world_get_params": (I, [P, P]),
+        "ribo_world_set_settings": (I, [P, P]),
+        "ribo_world_get_settings": (I, [P, P]),
+        "ribo_world_mg_levels": (I, [P, P, U32, ctypes.POINTER(U32)]),
+        "ribo_world_upload": (I, [P, I, P, SZ]),
+        "ribo_world_download": (I, [P, I, P, SZ]),
+        "ribo_world_encode_steps": (I, [P, U32]),
+        "ribo_world_run": (I, [P, U32]),
+        "ribo_world_sync": (I, [P]),
+        "ribo_world_ledger": (I, [P, P]),
+        "ribo_world_ledger_status": (I, [P, ctypes.POINTER(U32)]),
+        "ribo_world_set_timing": (I, [P, I]),
+        "ribo_world_kernel_times": (I, [P, P, P, P, U32, ctypes.POINTER(U32)]),
+        "ribo_world_reset_kernel_times": (I, [P]),
+        "ribo_world_mg_stats": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
+        "ribo_last_error": (ctypes.c_char_p, []),
+        "ribo_abi_version": (U32, []),
+        "ribo_gen_gamma": (I, [P, ctypes.c_uint64, ctypes.c_float, I, P]),
+        "ribo_gen_velocity": (I, [P, ctypes.c_uint64, U32, ctypes.c_float, P, P]),
+        "ribo_gen_chem": (I, [P, ctypes.c_uint64, ctypes.c_float, P, P]),
+        "ribo_gen_vent_heat": (I, [P, U32, ctypes.c_float, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        raise RiboError(f"{what} failed ({rc}): {lib().ribo_last_error().decode()}")
+    return rc
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def buffer_bytes(cfg: WorldConfig, which: int) -> int:
+    c = cfg.c()
+    return int(lib().ribo_buffer_bytes(ctypes.byref(c), which))
+
+
+def buffer_shape(cfg: WorldConfig, which: int) -> tuple:
+    n = buffer_bytes(cfg, which) // 4
+    return (n,)
+
+
+@dataclass
+class Ledger:
+    """Ledger (src/world/mod.rs:39-82): exact matter per channel and state."""
+
+    act: tuple
+    spent: tuple
+    held: tuple = (0, 0, 0, 0)
+
+    @classmethod
+    def from_cells(cls, cells: np.ndarray) -> "Ledger":
+        c = np.asarray(cells, dtype=np.uint32).reshape(-1, 4)
+        act = tuple(int((c[:, k] & 0xFFFF).astype(np.uint64).sum() & 0xFFFFFFFF) for k in range(4))
+        spent = tuple(int((c[:, k] >> 16).astype(np.uint64).sum() & 0xFFFFFFFF) for k in range(4))
+        return cls(act, spent, (0, 0, 0, 0))
+
+    @classmethod
+    def from_words(cls, w) -> "Ledger":
+        w = [int(x) for x in w]
+        return cls(tuple(w[0:4]), tuple(w[4:8]), tuple(w[8:12]))
+
+    def total(self) -> int:
+        return sum(self.act) + sum(self.spent) + sum(self.held)
+
+    def channel(self, ch: int) -> int:
+        return self.act[ch] + self.spent[ch] + self.held[ch]
+
+    def channels(self) -> tuple:
+        return tuple(self.channel(c) for c in range(4))
+
+
+class World:
+    """World (src/world/mod.rs): one world resident on the current HIP device."""
+
+    def __init__(self, cfg: WorldConfig, seed: int = 1):
+        self.cfg = cfg
+        self._h = ctypes.c_void_p()
+        c = cfg.c()
+        _check(lib().ribo_world_create(ctypes.byref(c), seed, ctypes.byref(self._h)), "ribo_world_create")
+
+    def close(self):
+        if self._h:
+            lib().ribo_world_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- params / settings (pub fields of World) --
+    @property
+    def params(self) -> SimParams:
+        p = SimParams()
+        _check(lib().ribo_world_get_params(self._h, ctypes.byref(p)), "get_params")
+        return p
+
+    @params.setter
+    def params(self, p: SimParams):
+        _check(lib().ribo_world_set_params(self._h, ctypes.byref(p)), "set_params")
+
+    def set_param(self, **kw):
+        p = self.params
+        for k, v in kw.items():
+            setattr(p, k, v)
+        self.params = p
+
+    @property
+    def settings(self) -> WorldSettings:
+        s = WorldSettings()
+        _check(lib().ribo_world_get_settings(self._h, ctypes.byref(s)), "get_settings")
+        return s
+
+    @settings.setter
+    def settings(self, s: WorldSettings):
+        _check(lib().ribo_world_set_settings(self._h, ctypes.byref(s)), "set_settings")
+
+    def set_setting(self, **kw):
+        s = self.settings
+        for k, v in kw.items():
+            setattr(s, k, v)
+        self.settings = s
+
+    def mg_levels(self) -> list:
+        arr = (MgLevel * 32)()
+        n = ctypes.c_uint32()
+        _check(lib().ribo_world_mg_levels(self._h, arr, 32, ctypes.byref(n)), "mg_levels")
+        return [(arr[i].n, arr[i].off, arr[i].n_c, arr[i].off_c, arr[i].eps) for i in range(n.value)]
+
+    # -- data movement --
+    def upload(self, which: int, data: np.ndarray):
+        a = np.ascontiguousarray(data, dtype=_BUF_DTYPE[which])
+        _check(lib().ribo_world_upload(self._h, which, _ptr(a), a.nbytes), f"upload {BUFFER_NAMES[which]}")
+
+    def download(self, which: int) -> np.ndarray:
+        a = np.empty(buffer_shape(self.cfg, which), dtype=_BUF_DTYPE[which])
+        _check(lib().ribo_world_download(self._h, which, _ptr(a), a.nbytes), f"download {BUFFER_NAMES[which]}")
+        return a
+
+    # -- stepping --
+    def encode_steps(self, steps: int) -> int:
+        return _check(lib().ribo_world_encode_steps(self._h, steps), "encode_steps")
+
+    def run(self, steps: int):
+        _check(lib().ribo_world_run(self._h, steps), "run")
+
+    def sync(self):
+        _check(lib().ribo_world_sync(self._h), "sync")
+
+    # -- readback helpers named like the reference's test API --
+    def read_cells_blocking(self) -> np.ndarray:
+        return self.download(CHEM)
+
+    def read_f32_blocking(self, which: int) -> np.ndarray:
+        return self.download(which)
+
+    def ledger_blocking(self) -> Ledger:
+        out = (ctypes.c_uint32 * 12)()
+        _check(lib().ribo_world_ledger(self._h, out), "ledger")
+        return Ledger.from_words(out)
+
+    def ledger_status(self) -> int | None:
+        e = ctypes.c_uint32()
+        rc = lib().ribo_world_ledger_status(self._h, ctypes.byref(e))
+        return None if rc == 0 else int(e.value)
+
+    # -- measurement --
+    def set_timing(self, on: bool):
+        _check(lib().ribo_world_set_timing(self._h, 1 if on else 0), "set_timing")
+
+    def kernel_times(self) -> dict:
+        cap = 64
+        names = (ctypes.c_char_p * cap)()
+        ms = (ctypes.c_double * cap)()
+        cnt = (ctypes.c_uint64 * cap)()
+        n = ctypes.c_uint32()
+        _check(lib().ribo_world_kernel_times(self._h, names, ms, cnt, cap, ctypes.byref(n)), "kernel_times")
+        return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, cap))}
+
+    def reset_kernel_times(self):
+        _check(lib().ribo_world_reset_kernel_times(self._h), "reset_kernel_times")
+
+    def mg_stats(self) -> tuple:
+        c = ctypes.c_uint32()
+        r = ctypes.c_double()
+        _check(lib().ribo_world_mg_stats(self._h, ctypes.byref(c), ctypes.byref(r)), "mg_stats")
+        return int(c.value), float(r.value)
+
+
+# ---- synthetic inputs (host generators in the library) ----
+def gen_gamma(cfg: WorldConfig, seed: int, rock_frac: float, rubble: bool) -> np.ndarray:
+    out = np.empty(cfg.cells(), dtype=np.uint32)
+    c = cfg.c()
+    _check(lib().ribo_gen_gamma(ctypes.byref(c), seed, rock_frac, 1 if rubble else 0, _ptr(out)), "gen_gamma")
+    return out
+
+
+def gen_velocity(cfg: WorldConfig, seed: int, vmax: float, gamma: np.ndarray | None, modes: int = 16) -> np.ndarray:
+    out = np.empty(2 * cfg.fluid_cells(), dtype=np.float32)
+    c = cfg.c()
+    g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.uint32)
+    _check(lib().ribo_gen_velocity(ctypes.byref(c), seed, modes, vmax, None if g is None else _ptr(g), _ptr(out)),
+           "gen_velocity")
+    return out
+
+
+def gen_chem(cfg: WorldConfig, seed: int, lam: float, gamma: np.ndarray | None) -> np.ndarray:
+    out = np.empty(4 * cfg.cells(), dtype=np.uint32)
+    c = cfg.c()
+    g = None if gamma is None else np.ascontiguousarray(gamma, dtype=np.uint32)
+    _check(lib().ribo_gen_chem(ctypes.byref(c), seed, lam, None if g is None else _ptr(g), _ptr(out)), "gen_chem")
+    return out
+
+
+def gen_vent_heat(cfg: WorldConfig, rows: int, h: float) -> np.ndarray:
+    out = np.empty(2 * cfg.fluid_cells(), dtype=np.float32)
+    c = cfg.c()
+    _check(lib().ribo_gen_vent_heat(ctypes.byref(c), rows, h, _ptr(out)), "gen_vent_heat")
+    return out
+
+
+def device_available() -> bool:
+    """True when a gfx950 device is visible (without creating a world)."""
+    try:
+        import subprocess  # noqa: F401
+        h = ctypes.c_void_p()
+        c = WorldConfig(16, 8).c()
+        rc = lib().ribo_world_create(ctypes.byref(c), 1, ctypes.byref(h))
+        if rc == 0:
+            lib().ribo_world_destroy(h)
+            return True
+        return False
+    except Exception:
+        return False

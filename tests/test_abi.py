@@ -1,0 +1,90 @@
+"""The C-ABI library loads without a GPU, exports every declared symbol and
+keeps the reference struct layouts."""
+import ctypes
+import os
+import re
+
+import numpy as np
+
+import ribossome_amd as ra
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, "include")
+    for f in os.listdir(inc):
+        if f.endswith(".h"):
+            names |= set(re.findall(r"RIBO_API\s+[\w\s\*]+?\b(ribo_\w+)\s*\(", open(os.path.join(inc, f)).read()))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ra.lib()
+    syms = declared_symbols()
+    assert len(syms) >= 25
+    for s in sorted(syms):
+        assert hasattr(L, s), s
+    assert L.ribo_abi_version() == 1
+
+
+def test_struct_layouts_match_reference():
+    # SimParams: 116 four-byte fields = 464 bytes (src/params.rs:86-487,
+    # size checked against WGSL by tests/shaders.rs:59-77)
+    assert ctypes.sizeof(ra.SimParams) == 464
+    assert len(ra.SimParams._fields_) == 116
+    assert ctypes.sizeof(ra.MgLevel) == 32
+    p = ra.SimParams.default(5)
+    # spot values of SimParams::default() (src/params.rs:553-690)
+    assert p.seed == 5 and p.epoch == 0 and p.transport_every == 3
+    assert np.float32(p.dt) == np.float32(0.017) and np.float32(p.fluid_dt) == np.float32(0.034)
+    assert np.float32(p.fluid_decay) == np.float32(0.99) and np.float32(p.fluid_viscosity) == np.float32(3.7)
+    assert np.float32(p.aggregation) == np.float32(0.03) and np.float32(p.reactivation_rate) == np.float32(1e-5)
+    assert np.float32(p.vent_cycle_amp) == np.float32(0.91) and p.vent_cycle_period == 7000.0
+    s = ra.WorldSettings.default()
+    assert (s.fluid_enabled, s.fluid_substep, s.multigrid, s.mg_cycles, s.jacobi_iters) == (1, 2, 1, 1, 128)
+    assert s.mg_min_size == 8 and s.terrain_enabled == 0
+
+
+def test_buffer_sizes():
+    cfg = ra.WorldConfig.DEFAULT
+    G, F = 2048 * 2048, 1024 * 1024
+    assert ra.buffer_bytes(cfg, ra.CHEM) == 16 * G
+    assert ra.buffer_bytes(cfg, ra.GAMMA) == 4 * G
+    assert ra.buffer_bytes(cfg, ra.VELOCITY) == 8 * F
+    assert ra.buffer_bytes(cfg, ra.LIGHT) == 8 * 1024 * 1024
+    assert ra.buffer_bytes(cfg, ra.NET_FLUX) == 8 * 1024
+    assert ra.buffer_bytes(cfg, 99) == 0
+
+
+def test_world_create_fails_cleanly_without_device():
+    if ra.device_available():
+        return
+    h = ctypes.c_void_p()
+    c = ra.WorldConfig(64, 32).c()
+    rc = ra.lib().ribo_world_create(ctypes.byref(c), 1, ctypes.byref(h))
+    assert rc == -3 and not h.value
+    assert b"device" in ra.lib().ribo_last_error()
+
+
+def test_bad_config_rejected():
+    h = ctypes.c_void_p()
+    c = ra.WorldConfig(100, 30).c()
+    assert ra.lib().ribo_world_create(ctypes.byref(c), 1, ctypes.byref(h)) == -1
+
+
+def test_generators_deterministic_and_shaped():
+    cfg = ra.WorldConfig(128, 64)
+    g1, g2 = ra.gen_gamma(cfg, 3, 0.15, True), ra.gen_gamma(cfg, 3, 0.15, True)
+    assert np.array_equal(g1, g2)
+    assert abs((g1 >= 3).mean() - 0.15) < 1e-3
+    assert set(np.unique(g1)) <= {0, 1, 2, 6}
+    ch = ra.gen_chem(cfg, 3, 4.0, g1)
+    c = ch.reshape(-1, 4)
+    assert np.all(c[g1 >= 3] == 0)
+    open_ = g1 < 3
+    counts = np.concatenate([(c[open_] & 0xFFFF).ravel(), (c[open_] >> 16).ravel()])
+    assert abs(counts.mean() - 4.0) < 0.05 and abs(counts.var() - 4.0) < 0.15
+    v = ra.gen_velocity(cfg, 3, 10.0, g1).reshape(-1, 2)
+    assert abs(np.sqrt((v.astype(np.float64) ** 2).sum(1)).max() - 10.0) < 1e-4

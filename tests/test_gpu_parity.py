@@ -1,0 +1,174 @@
+"""GPU parity: the HIP world against the CPU oracle on the same seeded inputs.
+
+Every kernel computes each value with the oracle's operation sequence, so
+the expected result is bit-identical buffers: integer monomer grids always,
+and the binary32 fluid fields too.  A mismatch prints, per buffer, how many
+elements differ and by how much before the assertion fails.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import ribossome_amd as ra
+import oracle_ctypes as oc
+from worlds import FIELDS, bit_diff, load, make_inputs, parity_params, parity_settings
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    # 256^2 / 128^2, ~10 % rock, Poisson(4), 2 V-cycles to a 16^2 coarsest level
+    "cfg1": dict(cfg=ra.WorldConfig(256, 128), seed=1, inputs=dict(rock=0.10),
+                 params={}, settings=dict(mg_cycles=2, mg_min_size=16)),
+    # rubble, vent band, strong diffusion, settle, osmotic pressure, cohesion,
+    # thermal activation: every branch of transport and of the fluid forcing
+    "rubble_vent": dict(cfg=ra.WorldConfig(128, 64), seed=3, inputs=dict(rock=0.15, rubble=True, vent_rows=6),
+                        params=dict(diffusion=30.0, settle=1.0, thermal_activation=1.0, monomer_pressure=0.5,
+                                    activation_decay=0.01, reactivation_rate=0.05), settings={}),
+    # reference schedule defaults (substep 2, transport every 3, 1 V-cycle)
+    "ref_schedule": dict(cfg=ra.WorldConfig(128, 64), seed=5, inputs=dict(rock=0.12, rubble=True, vent_rows=4),
+                         params=dict(transport_every=3), settings=dict(fluid_substep=2)),
+    # Jacobi pressure instead of multigrid
+    "jacobi": dict(cfg=ra.WorldConfig(64, 32), seed=7, inputs=dict(rock=0.08, vent_rows=3),
+                   params={}, settings=dict(multigrid=0, jacobi_iters=16)),
+    # grid == fluid resolution, levels 96 48 24 12 6 on a 3*2^k side
+    "odd_levels": dict(cfg=ra.WorldConfig(96, 96), seed=11, inputs=dict(rock=0.05, vent_rows=5),
+                       params={}, settings=dict(mg_min_size=8)),
+    # UV light (frozen field) and solar heating on
+    "light": dict(cfg=ra.WorldConfig(64, 32), seed=13, inputs=dict(rock=0.1, vent_rows=2),
+                  params=dict(uv_strength=3.0, sun_heat=1.0, direct_photoactivation=0.5), settings={}, light=True),
+}
+
+
+def build(case, steps_light_seed=99):
+    c = CASES[case]
+    cfg = c["cfg"]
+    inp = make_inputs(cfg, c["seed"], **c["inputs"])
+    if c.get("light"):
+        rng = np.random.default_rng(steps_light_seed)
+        inp[ra.LIGHT] = rng.random(ra.buffer_bytes(cfg, ra.LIGHT) // 4).astype(np.float32)
+    p = parity_params(c["seed"], **c["params"])
+    s = parity_settings(**c["settings"])
+    gw = ra.World(cfg, c["seed"])
+    load(gw, inp, p, s)
+    ow = oc.OracleWorld(cfg, c["seed"])
+    load(ow, inp, p, s)
+    return gw, ow
+
+
+def compare(gw, ow, tag):
+    report = []
+    ok = True
+    a, b = gw.download(ra.CHEM), ow.download(ra.CHEM)
+    nchem = int(np.count_nonzero(a != b))
+    report.append(f"chem words differing: {nchem}")
+    ok &= nchem == 0
+    for which in FIELDS + (ra.SOLID_MASK, ra.SLOPE):
+        a, b = gw.download(which), ow.download(which)
+        nd = bit_diff(a, b)
+        if nd:
+            af, bf = a.astype(np.float64), b.astype(np.float64)
+            rel = np.linalg.norm(af - bf) / max(np.linalg.norm(bf), 1e-30)
+            report.append(f"{ra.BUFFER_NAMES[which]}: {nd} differ, max|d|={np.abs(af - bf).max():.3e}, relL2={rel:.3e}")
+            ok = False
+    print(f"[{tag}] " + "; ".join(report), flush=True)
+    return ok
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_world_step_bit_exact(case):
+    print(f"case {case}", flush=True)
+    gw, ow = build(case)
+    assert compare(gw, ow, f"{case} initial")
+    for k, n in enumerate((1, 1, 3, 7)):
+        assert gw.encode_steps(n) == n and ow.encode_steps(n) == n
+        assert compare(gw, ow, f"{case} after chunk {k}"), f"{case}: GPU differs from the oracle"
+    assert gw.params.epoch == ow.params.epoch
+
+
+def test_transport_only_bit_exact():
+    """Transport with a frozen uploaded velocity: fluid_substep 4 and odd
+    epochs, so the steps run transport but not the fluid."""
+    print("case transport_only", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 21, rock=0.12, rubble=True, vcells_per_step=1.7)
+    p = parity_params(21, epoch=1, diffusion=20.0, settle=1.0, monomer_pressure=0.3)
+    s = parity_settings(fluid_substep=4)
+    gw, ow = ra.World(cfg, 21), oc.OracleWorld(cfg, 21)
+    load(gw, inp, p, s)
+    load(ow, inp, p, s)
+    gw.encode_steps(3)
+    ow.encode_steps(3)
+    a, b = gw.download(ra.CHEM), ow.download(ra.CHEM)
+    assert np.array_equal(a, b), f"{np.count_nonzero(a != b)} chem words differ"
+    assert ra.Ledger.from_cells(a).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
+
+
+def test_conservation_and_gpu_ledger():
+    """tests/conservation.rs:30-69 on the HIP world: 600 steps with strong
+    diffusion, settle, thermal activation and aggregation; exact per-channel
+    totals, device-side check every step, GPU ledger == CPU count."""
+    print("case conservation", flush=True)
+    cfg = ra.WorldConfig(256, 128)
+    inp = make_inputs(cfg, 7, rock=0.1, rubble=True, vent_rows=8)
+    p = parity_params(7, diffusion=30.0, settle=1.0, thermal_activation=1.0, aggregation=0.02)
+    s = parity_settings(ledger_check=1)
+    gw = ra.World(cfg, 7)
+    gw.params = p
+    gw.upload(ra.GAMMA, inp[ra.GAMMA])
+    for k in (ra.VELOCITY, ra.CHEM, ra.HEAT_SRC):
+        gw.upload(k, inp[k])
+    gw.settings = s
+    seeded = ra.Ledger.from_cells(inp[ra.CHEM])
+    assert gw.ledger_blocking() == seeded
+    gw.run(600)
+    cells = gw.read_cells_blocking()
+    after = ra.Ledger.from_cells(cells)
+    moved = int(np.count_nonzero(cells != inp[ra.CHEM]))
+    print(f"moved {moved} of {cells.size}; act {sum(seeded.act)} -> {sum(after.act)}", flush=True)
+    assert moved > cells.size // 10
+    assert sum(seeded.act) != sum(after.act)
+    assert after.channels() == seeded.channels()
+    assert gw.ledger_blocking() == after
+    assert gw.ledger_status() is None
+
+
+def test_batch_invariance():
+    """tests/conservation.rs:120-141: bit-identical results for 64-step and
+    1-step batches and for two identical runs."""
+    print("case batch_invariance", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 9, rock=0.1, rubble=True, vent_rows=4)
+
+    def go(batch):
+        w = ra.World(cfg, 9)
+        load(w, inp, parity_params(9), parity_settings())
+        done = 0
+        while done < 128:
+            done += w.encode_steps(min(batch, 128 - done))
+        w.sync()
+        return w.read_cells_blocking(), w.download(ra.VELOCITY)
+
+    a = go(64)
+    b = go(64)
+    c = go(1)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[0], c[0])
+    assert bit_diff(a[1], b[1]) == 0 and bit_diff(a[1], c[1]) == 0
+
+
+def test_tolerance_mode_reaches_residual():
+    print("case tolerance", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 4, rock=0.1, vent_rows=4)
+    s = parity_settings(mg_tol=1e-4, mg_max_cycles=40)
+    gw = ra.World(cfg, 4)
+    load(gw, inp, parity_params(4), s)
+    ow = oc.OracleWorld(cfg, 4)
+    load(ow, inp, parity_params(4), s)
+    for _ in range(3):
+        gw.encode_steps(1)
+        ow.encode_steps(1)
+        cyc, rel = gw.mg_stats()
+        print(f"cycles {cyc} rel {rel:.3e} oracle rel {ow.mg_residual():.3e}", flush=True)
+        assert rel <= 1e-4 or cyc == 40
+    assert compare(gw, ow, "tolerance")

@@ -127,7 +127,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--mg-cycles", type=int, default=1)
     ap.add_argument("--profile-steps", type=int, default=32)
-    ap.add_argument("--cpu-steps", type=int, default=8)
+    ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -140,7 +140,12 @@ def main():
     w = build_world(cfg, inputs, args)
     seeded = ra.Ledger.from_cells(inputs[2])
 
+    # warm-up, including one replay of every (chunk length, buffer parity)
+    # graph the timed loop will use, so no capture falls in the timed region
     w.run(args.warmup)
+    for n in sorted({64 if args.steps >= 64 else 0, args.steps % 64} - {0}):
+        w.encode_steps(n)
+        w.encode_steps(n)
     w.sync()
     t0 = time.perf_counter()
     w.run(args.steps)

@@ -215,7 +215,9 @@ typedef struct RiboWorldSettings {
     uint32_t precision;           /* RIBO_PRECISION_FP32 (drop-in) or _MIXED */
     uint32_t ledger_check;        /* 1: exact per-channel totals checked on the
                                      device after every transport pass */
-    uint32_t _pad0, _pad1;
+    uint32_t no_graphs;           /* 1: record every step eagerly instead of
+                                     replaying captured HIP graphs */
+    uint32_t _pad1;
 } RiboWorldSettings;
 
 enum { RIBO_PRECISION_FP32 = 0, RIBO_PRECISION_MIXED = 1 };

@@ -71,10 +71,13 @@ void launch_mg_mask_coarsen(hipStream_t s, uint32_t n, const uint8_t* fine, uint
 void launch_mask_u32(hipStream_t s, const Geo& g, uint32_t* out);
 
 // fluid step
-void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vel,
+// Kernels that read the epoch take `ep`: null, or a device word added to
+// P.epoch (graph replays keep the epoch base on the device).
+void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float2* vel,
                         const float* t_in, const float* r_in, float* t_out, float* r_out,
                         uint32_t* eaten, int eaten_active, const uint32_t* chem);
-void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, float2* forces);
+void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
+                   float2* forces);
 void launch_add_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* forces,
                        const float2* vin, float2* vout);
 void launch_diffuse(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vin, float2* vout);
@@ -85,19 +88,28 @@ void launch_jacobi(hipStream_t s, const Geo& g, const float* pin, float* pout, c
 void launch_mg_smooth(hipStream_t s, const MgLevelDev& L, uint32_t color, const uint32_t* done);
 void launch_mg_restrict(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
 void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
+// fused multigrid: pre-smooth + restrict, prolong + post-smooth per level, and
+// the coarse end of the cycle (at most 8 levels) in one workgroup
+void launch_mg_down(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
+void launch_mg_up(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
+size_t mg_block_lds_bytes(const MgLevelDev* lv, int count);
+void launch_mg_block(hipStream_t s, const MgLevelDev* lv, int count, const uint32_t* done);
 void launch_mg_rhs_norm(hipStream_t s, const MgLevelDev& L0, Counters* c);
 void launch_mg_residual(hipStream_t s, const MgLevelDev& L0, Counters* c, float tol, uint32_t max_cycles, int count_cycle);
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);
-void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, uint32_t* chem);
+void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
+                    uint32_t* chem);
 
 // transport
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e);
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
-void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t* next,
-                    const float2* vel, const float* temp, const uint32_t* agg_e);
-void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, int check);
+void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const uint32_t* chem,
+                    uint32_t* next, const float2* vel, const float* temp, const uint32_t* agg_e);
+void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
+                   int check);
+void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n);
 
 }  // namespace ribo
 

@@ -105,6 +105,37 @@ __device__ __forceinline__ F4 rng_f4(uint32_t key, uint32_t epoch, uint32_t stre
     return u4_to_f4(pcg4d({key, epoch, stream, seed}));
 }
 
+// PCG4D for a run of streams s0, s0+1, ... with (key, epoch, seed) fixed:
+// the LCG images of key, epoch and seed and the first cross-lane update of x
+// do not depend on the stream, so they are computed once and each draw costs
+// 7 of the 12 32-bit multiplies.  Bit-identical to pcg4d({key, epoch, s, seed}).
+struct PcgRun {
+    uint32_t y0, w0, x1, x1s, z0;  // z0: LCG image of the current stream
+    __device__ __forceinline__ PcgRun(uint32_t key, uint32_t epoch, uint32_t stream0, uint32_t seed) {
+        const uint32_t x0 = key * 1664525u + 1013904223u;
+        y0 = epoch * 1664525u + 1013904223u;
+        w0 = seed * 1664525u + 1013904223u;
+        x1 = x0 + y0 * w0;
+        x1s = x1 ^ (x1 >> 16);
+        z0 = stream0 * 1664525u + 1013904223u;
+    }
+    __device__ __forceinline__ U4 at_current() const {
+        uint32_t y1 = y0 + z0 * x1;
+        uint32_t z1 = z0 + x1 * y1;
+        uint32_t w1 = w0 + y1 * z1;
+        y1 ^= y1 >> 16;
+        z1 ^= z1 >> 16;
+        w1 ^= w1 >> 16;
+        U4 r;
+        r.x = x1s + y1 * w1;
+        r.y = y1 + z1 * r.x;
+        r.z = z1 + r.x * r.y;
+        r.w = w1 + r.y * r.z;
+        return r;
+    }
+    __device__ __forceinline__ void advance() { z0 += 1664525u; }
+};
+
 // ---- small vector helpers -------------------------------------------------------
 __device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
 __device__ __forceinline__ float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
@@ -318,11 +349,12 @@ __device__ __forceinline__ float vent_cycle(const RiboSimParams& P, uint32_t x, 
 
 // update_temperature (fluid.wgsl:300-363); the copy back (copy_temperature,
 // :365-372) is a pointer swap on the host, and redox_eaten is cleared here.
-__global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, const float2* __restrict__ vel,
+__global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, const uint32_t* ep, const float2* __restrict__ vel,
                                                      const float* __restrict__ T, const float* __restrict__ Q,
                                                      float* __restrict__ t_out, float* __restrict__ r_out,
                                                      uint32_t* eaten, int eaten_active, const uint32_t* chem) {
     FLUID_XY
+    if (ep) P.epoch += *ep;
     const float dt = fluid_dt(P);
     const float2 v = raw_vel(g, vel, idx);
     const float2 src = sub2(f2((float)x + 0.5f, (float)y + 0.5f), mul2(v, dt));
@@ -367,8 +399,9 @@ __global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, con
 // buoyancy (fluid.wgsl:376-389) into the fixed-point force words, then
 // gather_forces (:392-398).  Without agents each word only ever receives its
 // own cell's buoyancy, so quantise -> dequantise -> clamp happens in place.
-__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const float* __restrict__ T, float2* forces) {
+__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T, float2* forces) {
     FLUID_XY
+    if (ep) P.epoch += *ep;
     int32_t qx = 0, qy = 0;
     if (!(x == 0u || y == 0u || x >= F - 1u || y >= F - 1u)) {
         const float dev = T[idx] - temp_ambient_at(y, F);
@@ -601,6 +634,247 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgLevelDev L, MgLevelDev C, 
     L.p[i] += C.p[(y / 2u) * C.n + x / 2u];
 }
 
+// ---- multigrid, fused: one launch per level per direction -------------------
+// A red-black half-sweep updates every cell of one colour from the current
+// values of its 4 neighbours (the other colour), so the order inside a
+// half-sweep is irrelevant and a tile with an h-cell halo can run h
+// half-sweeps from its own copy and still hold exact values in its core:
+// after half-sweep k every cell at distance >= k from the loaded edge is
+// exact.  The tiles below therefore reproduce the per-dispatch schedule of
+// the reference bit for bit while reading each level once per direction.
+constexpr int MG_T = 32;  // output tile (even, so tile origins keep (x+y) parity)
+
+template <int H>
+struct MgTile {
+    static constexpr int R = MG_T + 2 * H;
+    float p[R][R + 1];
+    float rhs[R][R + 1];
+    uint8_t fl[R][R + 1];
+};
+
+// p_c <- (sum of fluid neighbours - rhs) / (k + eps), on the cells of one
+// colour inside [lo, hi) of the tile (multigrid.wgsl:35-45,60-67).
+template <int H>
+__device__ __forceinline__ void tile_half_sweep(MgTile<H>& t, int color, float eps, int lo, int hi) {
+    const int w = hi - lo;
+    const int half = (w + 1) / 2;
+    for (int k = threadIdx.x; k < w * half; k += blockDim.x) {
+        const int ly = lo + k / half;
+        int lx = lo + 2 * (k % half);
+        if (((lx + ly) & 1) != color) ++lx;
+        if (lx >= hi) continue;
+        if (!t.fl[ly][lx]) continue;
+        float s = 0.0f, kk = 0.0f;
+        if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
+        if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
+        if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
+        if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
+        t.p[ly][lx] = (s - t.rhs[ly][lx]) / (kk + eps);
+    }
+}
+
+template <int H>
+__device__ __forceinline__ void tile_load(MgTile<H>& t, const MgLevelDev& L, int x0, int y0, const MgLevelDev* C) {
+    constexpr int R = MgTile<H>::R;
+    const int n = (int)L.n;
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
+        const int ly = k / R, lx = k % R;
+        const int gx = x0 + lx, gy = y0 + ly;
+        float pv = 0.0f, rv = 0.0f;
+        uint8_t f = 0;
+        if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
+            const int i = gy * n + gx;
+            f = L.fluid[i];
+            if (f) {
+                pv = L.p[i];
+                rv = L.rhs[i];
+                // mg_prolong (multigrid.wgsl:108-114), pointwise, applied on load
+                if (C) pv += C->p[(gy / 2) * (int)C->n + gx / 2];
+            }
+        }
+        t.p[ly][lx] = pv;
+        t.rhs[ly][lx] = rv;
+        t.fl[ly][lx] = f;
+    }
+}
+
+// Pre-smoothing (2 x red, black) and restriction of one level
+// (src/world/mod.rs:1556-1563; multigrid.wgsl:60-105).
+__global__ __launch_bounds__(256) void k_mg_down(MgLevelDev L, MgLevelDev C, const uint32_t* done) {
+    if (done && *done) return;
+    constexpr int H = 5;
+    __shared__ MgTile<H> t;
+    const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
+    tile_load<H>(t, L, x0, y0, nullptr);
+    __syncthreads();
+    constexpr int R = MgTile<H>::R;
+    for (int s = 0; s < 4; ++s) {
+        tile_half_sweep<H>(t, s & 1, L.eps, 1 + s, R - 1 - s);
+        __syncthreads();
+    }
+    const int n = (int)L.n;
+    for (int k = threadIdx.x; k < MG_T * MG_T; k += blockDim.x) {
+        const int gx = x0 + H + k % MG_T, gy = y0 + H + k / MG_T;
+        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) L.p[gy * n + gx] = t.p[H + k / MG_T][H + k % MG_T];
+    }
+    const int nc = (int)C.n;
+    const int ct = threadIdx.x;  // one thread per coarse cell of the tile
+    if (ct < (MG_T / 2) * (MG_T / 2)) {
+        const int cx = (int)blockIdx.x * (MG_T / 2) + ct % (MG_T / 2), cy = (int)blockIdx.y * (MG_T / 2) + ct / (MG_T / 2);
+        if (cx < nc && cy < nc) {
+            float sum = 0.0f;
+            uint32_t nf = 0;
+            for (int c = 0; c < 4; ++c) {
+                const int lx = 2 * (ct % (MG_T / 2)) + (c & 1) + H, ly = 2 * (ct / (MG_T / 2)) + (c >> 1) + H;
+                if (!t.fl[ly][lx]) continue;
+                float s = 0.0f, kk = 0.0f;
+                if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
+                if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
+                if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
+                if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
+                sum += t.rhs[ly][lx] - (s - (kk + L.eps) * t.p[ly][lx]);
+                nf += 1u;
+            }
+            const int ci = cy * nc + cx;
+            C.rhs[ci] = nf > 0u ? 2.0f * sum / (float)rb_maxu(nf, 1u) : 0.0f;
+            C.p[ci] = 0.0f;
+        }
+    }
+}
+
+// Prolongation and post-smoothing (2 x red, black) of one level
+// (src/world/mod.rs:1569-1575).
+__global__ __launch_bounds__(256) void k_mg_up(MgLevelDev L, MgLevelDev C, const uint32_t* done) {
+    if (done && *done) return;
+    constexpr int H = 4;
+    __shared__ MgTile<H> t;
+    const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
+    tile_load<H>(t, L, x0, y0, &C);
+    __syncthreads();
+    constexpr int R = MgTile<H>::R;
+    for (int s = 0; s < 4; ++s) {
+        tile_half_sweep<H>(t, s & 1, L.eps, 1 + s, R - 1 - s);
+        __syncthreads();
+    }
+    const int n = (int)L.n;
+    for (int k = threadIdx.x; k < MG_T * MG_T; k += blockDim.x) {
+        const int gx = x0 + H + k % MG_T, gy = y0 + H + k / MG_T;
+        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) L.p[gy * n + gx] = t.p[H + k / MG_T][H + k % MG_T];
+    }
+}
+
+// The coarse end of the V-cycle in one workgroup: every level of `lv` (at
+// most MG_BLOCK_LEVELS, each small enough that all of them fit in LDS
+// together) goes through pre-smoothing and restriction, the coarsest gets
+// 30 x (red, black), then prolongation and post-smoothing back up; the top
+// level's unknowns are read from and written back to HBM.
+constexpr int MG_BLOCK_LEVELS = 8;
+struct MgBlockArgs {
+    int levels;
+    int n[MG_BLOCK_LEVELS];
+    float eps[MG_BLOCK_LEVELS];
+    const uint8_t* fluid[MG_BLOCK_LEVELS];
+    int off[MG_BLOCK_LEVELS + 1];  // cell offsets of each level in LDS
+    float* p0;
+    const float* rhs0;
+};
+
+__device__ __forceinline__ void blk_sweep(float* p, const float* rhs, const uint8_t* fl, int n, float eps, int color) {
+    const int half = (n + 1) / 2;
+    for (int k = threadIdx.x; k < n * half; k += blockDim.x) {
+        const int y = k / half;
+        int x = 2 * (k % half);
+        if (((x + y) & 1) != color) ++x;
+        if (x >= n) continue;
+        const int i = y * n + x;
+        if (!fl[i]) continue;
+        float s = 0.0f, kk = 0.0f;
+        if (x > 0 && fl[i - 1]) { s += p[i - 1]; kk += 1.0f; }
+        if (x + 1 < n && fl[i + 1]) { s += p[i + 1]; kk += 1.0f; }
+        if (y > 0 && fl[i - n]) { s += p[i - n]; kk += 1.0f; }
+        if (y + 1 < n && fl[i + n]) { s += p[i + n]; kk += 1.0f; }
+        p[i] = (s - rhs[i]) / (kk + eps);
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs a, const uint32_t* done) {
+    if (done && *done) return;
+    extern __shared__ float sm[];
+    const int total = a.off[a.levels];
+    float* P = sm;
+    float* RHS = sm + total;
+    uint8_t* FL = reinterpret_cast<uint8_t*>(sm + 2 * total);
+    const int n0 = a.n[0];
+    for (int i = threadIdx.x; i < n0 * n0; i += blockDim.x) {
+        const uint8_t f = a.fluid[0][i];
+        FL[i] = f;
+        P[i] = f ? a.p0[i] : 0.0f;
+        RHS[i] = f ? a.rhs0[i] : 0.0f;
+    }
+    for (int l = 1; l < a.levels; ++l)
+        for (int i = threadIdx.x; i < a.n[l] * a.n[l]; i += blockDim.x) FL[a.off[l] + i] = a.fluid[l][i];
+    __syncthreads();
+    const int L = a.levels;
+    for (int l = 0; l + 1 < L; ++l) {
+        float* p = P + a.off[l];
+        const float* rhs = RHS + a.off[l];
+        const uint8_t* fl = FL + a.off[l];
+        const int n = a.n[l];
+        for (int s = 0; s < 4; ++s) {
+            blk_sweep(p, rhs, fl, n, a.eps[l], s & 1);
+            __syncthreads();
+        }
+        const int nc = a.n[l + 1];
+        float* pc = P + a.off[l + 1];
+        float* rc = RHS + a.off[l + 1];
+        for (int ci = threadIdx.x; ci < nc * nc; ci += blockDim.x) {
+            const int gx = ci % nc, gy = ci / nc;
+            float sum = 0.0f;
+            uint32_t nf = 0;
+            for (int c = 0; c < 4; ++c) {
+                const int fx = gx * 2 + (c & 1), fy = gy * 2 + (c >> 1);
+                const int i = fy * n + fx;
+                if (!fl[i]) continue;
+                float s2 = 0.0f, kk = 0.0f;
+                if (fx > 0 && fl[i - 1]) { s2 += p[i - 1]; kk += 1.0f; }
+                if (fx + 1 < n && fl[i + 1]) { s2 += p[i + 1]; kk += 1.0f; }
+                if (fy > 0 && fl[i - n]) { s2 += p[i - n]; kk += 1.0f; }
+                if (fy + 1 < n && fl[i + n]) { s2 += p[i + n]; kk += 1.0f; }
+                sum += rhs[i] - (s2 - (kk + a.eps[l]) * p[i]);
+                nf += 1u;
+            }
+            rc[ci] = nf > 0u ? 2.0f * sum / (float)rb_maxu(nf, 1u) : 0.0f;
+            pc[ci] = 0.0f;
+        }
+        __syncthreads();
+    }
+    {
+        float* p = P + a.off[L - 1];
+        const float* rhs = RHS + a.off[L - 1];
+        const uint8_t* fl = FL + a.off[L - 1];
+        for (int s = 0; s < 2 * 30; ++s) {
+            blk_sweep(p, rhs, fl, a.n[L - 1], a.eps[L - 1], s & 1);
+            __syncthreads();
+        }
+    }
+    for (int l = L - 2; l >= 0; --l) {
+        float* p = P + a.off[l];
+        const float* rhs = RHS + a.off[l];
+        const uint8_t* fl = FL + a.off[l];
+        const int n = a.n[l], nc = a.n[l + 1];
+        const float* pc = P + a.off[l + 1];
+        for (int i = threadIdx.x; i < n * n; i += blockDim.x)
+            if (fl[i]) p[i] += pc[((i / n) / 2) * nc + (i % n) / 2];
+        __syncthreads();
+        for (int s = 0; s < 4; ++s) {
+            blk_sweep(p, rhs, fl, n, a.eps[l], s & 1);
+            __syncthreads();
+        }
+    }
+    for (int i = threadIdx.x; i < n0 * n0; i += blockDim.x)
+        if (FL[i]) a.p0[i] = P[i];
+}
+
 // Residual tracking for the tolerance mode (binary64 sums, one block-level
 // reduction and a last-block finaliser; no host round trip).
 __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
@@ -748,7 +1022,8 @@ __global__ __launch_bounds__(256) void k_boundaries(Geo g, const float2* __restr
 // thermal_activation (transport.wgsl:419-438) with chem_activate_one
 // (chem.wgsl:90-110).  Each thread owns its cell's four words, so the
 // compare-and-swap of the reference reduces to a plain read-modify-write.
-__global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const float* __restrict__ T, uint32_t* chem) {
+__global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T, uint32_t* chem) {
+    if (ep) P.epoch += *ep;
     const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     const uint32_t G = g.G, F = g.F;
     if (x >= G || y >= G) return;
@@ -798,6 +1073,8 @@ __global__ __launch_bounds__(256) void k_agg(Geo g, const uint32_t* __restrict__
     agg_e[y * G + x] = s;
 }
 
+__global__ void k_epoch_add(uint32_t* ep, uint32_t n) { *ep += n; }
+
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x * 4u;
     if (i + 3 < n) {
@@ -842,10 +1119,11 @@ __device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0)
 // transport_scatter (transport.wgsl:119-406).  One thread per grid cell; the
 // 16-bin destination histogram of each channel lives in LDS (column per
 // thread), the emit is one global atomic add per non-empty bin.
-__global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* __restrict__ chem,
+__global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
     __shared__ uint32_t s_bins[16 * 256];
+    if (ep) P.epoch += *ep;
     const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
     const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     const uint32_t G = g.G, F = g.F;
@@ -971,17 +1249,19 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
             if (y + 1u < G) coh[2] = 1.0f + P.cohesion * (float)ac(idx + G);
             if (y > 0u) coh[3] = 1.0f + P.cohesion * (float)ac(idx - G);
         }
-        for (uint32_t b = 0; b < 16; ++b) bins[b * 256u] = 0u;
+        uint32_t used = 0u;  // bit b set when bins[b] is non-zero
         uint32_t stay = 0u;
         const uint32_t n = act_n + spent_n;
         const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
         const uint32_t act_stay = act_n - rb_minu(act_n, movers);
         stay += act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
         const float wa[4] = {press[0] * coh[0], press[1] * coh[1], press[2] * coh[2], press[3] * coh[3]};
-        for (uint32_t k = 0; k < movers; ++k) {
+        PcgRun mv(slot, P.epoch, S_MOVE, P.seed);
+        PcgRun dsp(slot, P.epoch, S_DISPERSE, P.seed);
+        for (uint32_t k = 0; k < movers; ++k, mv.advance(), dsp.advance()) {
             const bool is_act = k < act_n;
             const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
-            const F4 rf = rng_f4(slot, P.epoch, S_MOVE + k, P.seed);
+            const F4 rf = u4_to_f4(mv.at_current());
             const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
             if (bound && !have_bound_start) {
                 bound_start = smooth_start(P, x, y);
@@ -992,7 +1272,7 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
             px += disp.x;
             py += disp.y;
             if (jitter_on) {
-                const F4 rj = rng_f4(slot, P.epoch, S_DISPERSE + k, P.seed);
+                const F4 rj = u4_to_f4(dsp.at_current());
                 const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
                 const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
                 px += rb_clamp(jx, -0.9f, 0.9f);
@@ -1022,11 +1302,13 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
             const int32_t ox = rb_addi(rb_f2i(rb_floor(px)), -base_x);
             const int32_t oy = rb_addi(rb_f2i(rb_floor(py)), -base_y);
             const uint32_t bi = (uint32_t)rb_clampi(oy, 0, 3) * 4u + (uint32_t)rb_clampi(ox, 0, 3);
-            bins[bi * 256u] += unit;
+            bins[bi * 256u] = ((used >> bi) & 1u) ? bins[bi * 256u] + unit : unit;
+            used |= 1u << bi;
         }
-        for (uint32_t b = 0; b < 16; ++b) {
+        while (used) {
+            const uint32_t b = (uint32_t)__builtin_ctz(used);
+            used &= used - 1u;
             const uint32_t cnt = bins[b * 256u];
-            if (cnt == 0u) continue;
             const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
             const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
             const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
@@ -1049,7 +1331,8 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
 // per-channel act + spent totals against the armed values, and clears the
 // accumulators for the next use.
 __global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led,
-                                                uint32_t epoch, int check) {
+                                                uint32_t epoch, const uint32_t* ep, int check) {
+    if (ep) epoch += *ep;
     __shared__ uint32_t sh[4][8];
     uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
@@ -1118,13 +1401,14 @@ void launch_mg_mask_coarsen(hipStream_t s, uint32_t nc, const uint8_t* fine, uin
 void launch_mask_u32(hipStream_t s, const Geo& g, uint32_t* out) {
     k_mask_u32<<<blocks1((size_t)g.F * g.F, 256), 256, 0, s>>>(g, out);
 }
-void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vel, const float* t_in,
-                        const float* r_in, float* t_out, float* r_out, uint32_t* eaten, int eaten_active,
-                        const uint32_t* chem) {
-    k_temperature<<<grid2(g.F), B16, 0, s>>>(g, P, vel, t_in, r_in, t_out, r_out, eaten, eaten_active, chem);
+void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float2* vel,
+                        const float* t_in, const float* r_in, float* t_out, float* r_out, uint32_t* eaten,
+                        int eaten_active, const uint32_t* chem) {
+    k_temperature<<<grid2(g.F), B16, 0, s>>>(g, P, ep, vel, t_in, r_in, t_out, r_out, eaten, eaten_active, chem);
 }
-void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, float2* forces) {
-    k_forces<<<grid2(g.F), B16, 0, s>>>(g, P, temp, forces);
+void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
+                   float2* forces) {
+    k_forces<<<grid2(g.F), B16, 0, s>>>(g, P, ep, temp, forces);
 }
 void launch_add_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* forces, const float2* vin,
                        float2* vout) {
@@ -1154,6 +1438,36 @@ void launch_mg_restrict(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C,
 void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
     k_mg_prolong<<<grid2(L.n), B16, 0, s>>>(L, C, done);
 }
+void launch_mg_down(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
+    const uint32_t nt = (L.n + MG_T - 1) / MG_T;
+    k_mg_down<<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+}
+void launch_mg_up(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
+    const uint32_t nt = (L.n + MG_T - 1) / MG_T;
+    k_mg_up<<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+}
+size_t mg_block_lds_bytes(const MgLevelDev* lv, int count) {
+    size_t cells = 0;
+    for (int i = 0; i < count; ++i) cells += (size_t)lv[i].n * lv[i].n;
+    return cells * 9;
+}
+void launch_mg_block(hipStream_t s, const MgLevelDev* lv, int count, const uint32_t* done) {
+    MgBlockArgs a{};
+    a.levels = count;
+    int off = 0;
+    for (int i = 0; i < count; ++i) {
+        a.n[i] = (int)lv[i].n;
+        a.eps[i] = lv[i].eps;
+        a.fluid[i] = lv[i].fluid;
+        a.off[i] = off;
+        off += (int)(lv[i].n * lv[i].n);
+    }
+    a.off[count] = off;
+    a.p0 = lv[0].p;
+    a.rhs0 = lv[0].rhs;
+    const size_t lds = (size_t)off * 9;
+    k_mg_block<<<1, 1024, lds, s>>>(a, done);
+}
 void launch_mg_rhs_norm(hipStream_t s, const MgLevelDev& L0, Counters* c) {
     k_counters_reset<<<1, 1, 0, s>>>(c);
     uint32_t nb = rb_minu(blocks1((size_t)L0.n * L0.n, 256), 1024u);
@@ -1173,8 +1487,9 @@ void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf) 
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf) {
     k_boundaries<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, nf);
 }
-void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const float* temp, uint32_t* chem) {
-    k_thermal<<<grid2(g.G), B16, 0, s>>>(g, P, temp, chem);
+void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
+                    uint32_t* chem) {
+    k_thermal<<<grid2(g.G), B16, 0, s>>>(g, P, ep, temp, chem);
 }
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e) {
     k_agg<<<grid2(g.G), B16, 0, s>>>(g, chem, agg_e);
@@ -1182,14 +1497,18 @@ void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v) {
     k_fill_u32<<<blocks1(n, 1024), 256, 0, s>>>(p, n, v);
 }
-void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t* next,
-                    const float2* vel, const float* temp, const uint32_t* agg_e) {
-    k_scatter<<<grid2(g.G), B16, 0, s>>>(g, P, chem, next, vel, temp, agg_e);
+void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const uint32_t* chem,
+                    uint32_t* next, const float2* vel, const float* temp, const uint32_t* agg_e) {
+    k_scatter<<<grid2(g.G), B16, 0, s>>>(g, P, ep, chem, next, vel, temp, agg_e);
 }
-void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, int check) {
+void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
+                   int check) {
     uint32_t cells = g.G * g.G;
     uint32_t nb = rb_minu(blocks1(cells, 256), 2048u);
-    k_ledger<<<nb, 256, 0, s>>>(chem, cells, led, epoch, check);
+    k_ledger<<<nb, 256, 0, s>>>(chem, cells, led, epoch, ep, check);
+}
+void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n) {
+    k_epoch_add<<<1, 1, 0, s>>>(ep, n);
 }
 
 }  // namespace ribo

@@ -40,7 +40,7 @@ int fail(int code, const std::string& msg) {
 
 constexpr uint32_t LIGHT_DIV = 2u;  // src/shaders.rs:132
 constexpr float MG_EPS = 1e-4f;     // src/world/mod.rs:202
-constexpr uint32_t MG_PRE = 2, MG_POST = 2, MG_COARSE = 30;  // :196-198
+constexpr uint32_t MG_COARSE = 30;  // src/world/mod.rs:198
 constexpr float MAX_DT = 0.07f;     // fluid.wgsl:15
 
 // Host-side per-step values (src/params.rs:505-537), binary64 as the reference.
@@ -128,9 +128,20 @@ struct RiboWorld {
     float* mg_p = nullptr;
     float* mg_rhs = nullptr;
     uint8_t* mg_fluid = nullptr;
+    size_t mg_block_start = 0;  // first level solved inside one workgroup
+    size_t lds_limit = 65536;
     Counters* counters = nullptr;
     LedgerDev* ledger = nullptr;
     uint32_t* scratch_u32 = nullptr;  // F words for downloads of derived masks
+    // graph replay: the epoch base lives on the device; one executable graph
+    // per (schedule phase, step count, ping-pong parities)
+    uint32_t* epoch_dev = nullptr;
+    uint32_t epoch_dev_value = 0;
+    bool epoch_dev_known = false;
+    struct GraphEntry { uint32_t phase, n; int cur, tcur, cur_end, tcur_end; hipGraphExec_t exec; };
+    std::vector<GraphEntry> graphs;
+    RiboSimParams graph_params{};
+    RiboWorldSettings graph_settings{};
 
     // timing
     bool timing = false;
@@ -226,7 +237,13 @@ std::vector<RiboMgLevel> make_levels(uint32_t n0, uint32_t min_size) {
     return lv;
 }
 
+void clear_graphs(RiboWorld* w) {
+    for (auto& e : w->graphs) (void)hipGraphExecDestroy(e.exec);
+    w->graphs.clear();
+}
+
 int build_mg(RiboWorld* w) {
+    clear_graphs(w);
     dfree(w->mg_p);
     dfree(w->mg_rhs);
     dfree(w->mg_fluid);
@@ -251,6 +268,13 @@ int build_mg(RiboWorld* w) {
         d.fluid = w->mg_fluid + w->levels[l].off;
         w->mg.push_back(d);
     }
+    // the longest suffix of levels (at most 8) whose unknowns, right-hand
+    // sides and masks fit in one workgroup's LDS together
+    size_t b = w->mg.size();
+    while (b > 0 && w->mg.size() - (b - 1) <= 8 &&
+           mg_block_lds_bytes(w->mg.data() + (b - 1), (int)(w->mg.size() - (b - 1))) <= w->lds_limit)
+        --b;
+    w->mg_block_start = b;
     return RIBO_OK;
 }
 
@@ -267,41 +291,40 @@ int rebuild_terrain(RiboWorld* w) {
     return RIBO_OK;
 }
 
-void vcycle(RiboWorld* w, const uint32_t* done) {  // src/world/mod.rs:1555-1575
+// One V-cycle (src/world/mod.rs:1555-1575).  Levels [0, block) run as one
+// fused pre-smooth + restrict launch and one prolong + post-smooth launch
+// each; levels [block, L) run inside a single workgroup.
+void vcycle(RiboWorld* w, const uint32_t* done) {
     const size_t L = w->mg.size();
     hipStream_t s = w->stream;
-    for (size_t l = 0; l + 1 < L; ++l) {
-        for (uint32_t k = 0; k < MG_PRE; ++k) {
-            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 0, done); });
-            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 1, done); });
-        }
-        run(w, "mg_restrict", [&] { launch_mg_restrict(s, w->mg[l], w->mg[l + 1], done); });
+    const size_t b = w->mg_block_start;
+    if (b < L) {
+        for (size_t l = 0; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down(s, w->mg[l], w->mg[l + 1], done); });
+        run(w, "mg_block", [&] { launch_mg_block(s, w->mg.data() + b, (int)(L - b), done); });
+        for (size_t l = b; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up(s, w->mg[l], w->mg[l + 1], done); });
+        return;
     }
+    // coarsest level too large for one workgroup: per-operation dispatches
+    for (size_t l = 0; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down(s, w->mg[l], w->mg[l + 1], done); });
     for (uint32_t k = 0; k < MG_COARSE; ++k) {
         run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 0, done); });
         run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 1, done); });
     }
-    for (size_t l = L - 1; l-- > 0;) {
-        run(w, "mg_prolong", [&] { launch_mg_prolong(s, w->mg[l], w->mg[l + 1], done); });
-        for (uint32_t k = 0; k < MG_POST; ++k) {
-            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 0, done); });
-            run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[l], 1, done); });
-        }
-    }
+    for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up(s, w->mg[l], w->mg[l + 1], done); });
 }
 
-void fluid_step(RiboWorld* w, const RiboSimParams& P) {  // src/world/mod.rs:1527-1588
+void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588
     hipStream_t s = w->stream;
     Geo g = w->geo();
     const int tc = w->tcur;
     run(w, "update_temperature", [&] {
-        launch_temperature(s, g, P, w->vel_a, w->temp[tc], w->redox[tc], w->temp[1 - tc], w->redox[1 - tc], w->eaten,
+        launch_temperature(s, g, P, ep, w->vel_a, w->temp[tc], w->redox[tc], w->temp[1 - tc], w->redox[1 - tc], w->eaten,
                            w->eaten_active ? 1 : 0, w->chem[w->cur]);
     });
     w->eaten_active = false;
     w->tcur = 1 - tc;  // copy_temperature
     const float* T = w->temp[w->tcur];
-    run(w, "buoyancy_gather", [&] { launch_forces(s, g, P, T, w->forces); });
+    run(w, "buoyancy_gather", [&] { launch_forces(s, g, P, ep, T, w->forces); });
     run(w, "add_forces", [&] { launch_add_forces(s, g, P, w->forces, w->vel_a, w->vel_b); });
     run(w, "diffuse_velocity", [&] { launch_diffuse(s, g, P, w->vel_b, w->vel_a); });
     const float decay = decay_factor(P.fluid_decay, P.fluid_dt);
@@ -332,19 +355,41 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P) {  // src/world/mod.rs:152
     run(w, "subtract_gradient", [&] { launch_subgrad(s, g, w->vel_a, w->vel_b, w->p_a); });
     run(w, "net_flux", [&] { launch_net_flux(s, g, w->vel_b, w->net_flux); });
     run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux); });
-    run(w, "thermal_activation", [&] { launch_thermal(s, g, P, T, w->chem[w->cur]); });
+    run(w, "thermal_activation", [&] { launch_thermal(s, g, P, ep, T, w->chem[w->cur]); });
 }
 
-void transport_step(RiboWorld* w, const RiboSimParams& P) {  // src/world/mod.rs:1594-1601
+void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1594-1601
     hipStream_t s = w->stream;
     Geo g = w->geo();
     uint32_t* cur = w->chem[w->cur];
     uint32_t* nxt = w->chem[1 - w->cur];
     if (P.aggregation > 0.0f) run(w, "agg", [&] { launch_agg(s, g, cur, w->agg_e); });
     run(w, "clear_next", [&] { launch_fill_u32(s, nxt, (size_t)4 * w->G * w->G, 0u); });
-    run(w, "transport_scatter", [&] { launch_scatter(s, g, P, cur, nxt, w->vel_a, w->temp[w->tcur], w->agg_e); });
+    run(w, "transport_scatter", [&] { launch_scatter(s, g, P, ep, cur, nxt, w->vel_a, w->temp[w->tcur], w->agg_e); });
     w->cur = 1 - w->cur;  // transport_commit
-    if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, w->chem[w->cur], w->ledger, P.epoch, 1); });
+    if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, w->chem[w->cur], w->ledger, P.epoch, ep, 1); });
+}
+
+// Steps [0, steps) of one encode_steps call, epochs P.epoch + i (eager) or
+// i on top of the device epoch base `ep` (graph capture).
+void record_steps(RiboWorld* w, uint32_t steps, uint32_t substep, const uint32_t* ep) {
+    for (uint32_t i = 0; i < steps; ++i) {
+        const uint32_t epoch = w->params.epoch + i;
+        RiboSimParams P = w->params;
+        P.epoch = ep ? i : epoch;
+        P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, epoch) : 0.0f;
+        P.sun_slope = sun_slope_at(w->params, epoch);
+        if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
+        if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep);
+    }
+}
+
+// Graph replay is used when nothing in the recorded work depends on host
+// values that change from step to step: the light-dependent solar heating
+// uses the host-computed sun of each step, and a pending redox_eaten upload
+// is consumed once.
+bool graph_eligible(const RiboWorld* w) {
+    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !(w->params.sun_heat > 0.0f);
 }
 
 int validate_settings(const RiboWorldSettings* s) {
@@ -434,6 +479,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     w->Lsz = cfg->grid_size / LIGHT_DIV;
     w->sim_size = (float)(cfg->grid_size * cfg->world_units_per_cell);
     w->device = dev;
+    w->lds_limit = std::min<size_t>(65536, (size_t)prop.sharedMemPerBlock);
     ribo_sim_params_fill_default(&w->params);
     w->params.seed = seed;
     ribo_world_settings_default(&w->st);
@@ -452,7 +498,8 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(&w->forces, F)) || (rc = dalloc(&w->heat, F)) || (rc = dalloc(&w->net_flux, 2 * (size_t)w->F)) ||
         (rc = dalloc(&w->cflags, F)) || (rc = dalloc(&w->normal, F)) || (rc = dalloc(&w->fill, F)) ||
         (rc = dalloc(&w->dragg, F)) || (rc = dalloc(&w->slope_f, F)) || (rc = dalloc(&w->counters, 1)) ||
-        (rc = dalloc(&w->ledger, 1)) || (rc = dalloc(&w->scratch_u32, F)) || (rc = build_mg(w))) {
+        (rc = dalloc(&w->ledger, 1)) || (rc = dalloc(&w->scratch_u32, F)) || (rc = dalloc(&w->epoch_dev, 1)) ||
+        (rc = build_mg(w))) {
         ribo_world_destroy(w);
         return rc;
     }
@@ -476,7 +523,8 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
-    dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->counters); dfree(w->ledger); dfree(w->scratch_u32);
+    dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->counters); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
+    clear_graphs(w);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
     return RIBO_OK;
@@ -508,7 +556,7 @@ int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s) {
         init.first_bad = 0xFFFFFFFFu;
         HIPCHK(hipMemcpyAsync(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice, w->stream));
         // arm the expected per-channel totals with the current grid
-        launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, 1);
+        launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, nullptr, 1);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(w->stream));
     }
@@ -598,13 +646,51 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
     w->fluid_was_on = st.fluid_enabled != 0;
     w->params.fluid_dt = w->params.dt * (float)substep;
     w->params.fluid_enabled = st.fluid_enabled ? 1u : 0u;
-    for (uint32_t i = 0; i < steps; ++i) {
-        RiboSimParams P = w->params;
-        P.epoch = w->params.epoch + i;
-        P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, P.epoch) : 0.0f;
-        P.sun_slope = sun_slope_at(w->params, P.epoch);
-        if (st.fluid_enabled && P.epoch % substep == 0) fluid_step(w, P);
-        if (P.epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P);
+    if (graph_eligible(w)) {
+        // any change of the by-value kernel arguments invalidates the graphs
+        RiboSimParams a = w->params, b = w->graph_params;
+        a.epoch = b.epoch = 0;
+        if (memcmp(&a, &b, sizeof(a)) != 0 || memcmp(&w->st, &w->graph_settings, sizeof(w->st)) != 0) {
+            clear_graphs(w);
+            w->graph_params = w->params;
+            w->graph_settings = w->st;
+        }
+        if (!w->epoch_dev_known || w->epoch_dev_value != w->params.epoch) {
+            HIPCHK(hipMemcpyAsync(w->epoch_dev, &w->params.epoch, sizeof(uint32_t), hipMemcpyHostToDevice, w->stream));
+            w->epoch_dev_value = w->params.epoch;
+            w->epoch_dev_known = true;
+        }
+        const uint32_t te = std::max(w->params.transport_every, 1u);
+        uint32_t period = substep;
+        while (period % te != 0) period += substep;
+        const uint32_t phase = w->params.epoch % period;
+        RiboWorld::GraphEntry* ge = nullptr;
+        for (auto& e : w->graphs)
+            if (e.phase == phase && e.n == steps && e.cur == w->cur && e.tcur == w->tcur) ge = &e;
+        if (!ge) {
+            RiboWorld::GraphEntry e{phase, steps, w->cur, w->tcur, 0, 0, nullptr};
+            hipGraph_t graph = nullptr;
+            HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+            record_steps(w, steps, substep, w->epoch_dev);
+            launch_epoch_add(w->stream, w->epoch_dev, steps);
+            hipError_t ce = hipStreamEndCapture(w->stream, &graph);
+            e.cur_end = w->cur;
+            e.tcur_end = w->tcur;
+            w->cur = e.cur;
+            w->tcur = e.tcur;
+            if (ce != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+            hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+            w->graphs.push_back(e);
+            ge = &w->graphs.back();
+        }
+        HIPCHK(hipGraphLaunch(ge->exec, w->stream));
+        w->cur = ge->cur_end;
+        w->tcur = ge->tcur_end;
+        w->epoch_dev_value += steps;
+    } else {
+        record_steps(w, steps, substep, nullptr);
     }
     HIPCHK(hipGetLastError());
     w->params.epoch += steps;
@@ -630,7 +716,7 @@ int ribo_world_sync(RiboWorld* w) {
 
 int ribo_world_ledger(RiboWorld* w, uint32_t out[12]) {  // encode_ledger + ledger_blocking (src/world/mod.rs:1661-1674,1834-1839)
     if (!w || !out) return fail(RIBO_ERR_ARG, "null argument");
-    launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, 0);
+    launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, nullptr, 0);
     HIPCHK(hipGetLastError());
     uint32_t last[8];
     HIPCHK(hipMemcpyAsync(last, w->ledger->last, sizeof(last), hipMemcpyDeviceToHost, w->stream));

@@ -177,7 +177,7 @@ class WorldSettings(ctypes.Structure):
         ("mg_max_cycles", ctypes.c_uint32),
         ("precision", ctypes.c_uint32),
         ("ledger_check", ctypes.c_uint32),
-        ("_pad0", ctypes.c_uint32),
+        ("no_graphs", ctypes.c_uint32),
         ("_pad1", ctypes.c_uint32),
     ]
 

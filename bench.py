@@ -33,7 +33,7 @@ import ribossome_amd as ra  # noqa: E402
 
 GRID, FLUID = 2048, 1024
 SEED = 2
-METRIC = "world cell-steps/s at 2048^2 (FP32, HBM %)"
+METRIC = "world cell-steps/s at 2048^2 (mixed vs FP32, HBM %)"
 
 # Algorithmic bytes per unit of work of each kernel (reads + writes of the
 # arrays it must touch once; stencil halos and gathers counted once), per
@@ -66,20 +66,33 @@ def world_inputs(cfg):
     return gamma, vel, chem, heat
 
 
-def configure(p, s, args):
+MODES = {
+    # the headline: FP32 multigrid-preconditioned CG to relative residual 1e-5
+    "fp32": dict(mg_tol=1e-5, precision=0),
+    # mixed precision: the preconditioner stored in bfloat16, same tolerance
+    "mixed": dict(mg_tol=1e-5, precision=1),
+    # the reference's own schedule: one V-cycle per step, no residual target
+    "reference": dict(mg_tol=0.0, precision=0),
+}
+
+
+def configure(p, s, args, mode="fp32"):
     p.transport_every = 1
     p.uv_strength = 0.0
     p.sun_heat = 0.0
     s.fluid_substep = 1
     s.mg_cycles = args.mg_cycles
+    s.mg_max_cycles = args.mg_max_cycles
+    s.mg_tol = MODES[mode]["mg_tol"]
+    s.precision = MODES[mode]["precision"]
     s.ledger_check = 1
     return p, s
 
 
-def build_world(cfg, inputs, args):
+def build_world(cfg, inputs, args, mode="fp32"):
     gamma, vel, chem, heat = inputs
     w = ra.World(cfg, SEED)
-    p, s = configure(ra.SimParams.default(SEED), ra.WorldSettings.default(), args)
+    p, s = configure(ra.SimParams.default(SEED), ra.WorldSettings.default(), args, mode)
     w.params = p
     w.upload(ra.GAMMA, gamma)
     w.upload(ra.VELOCITY, vel)
@@ -87,6 +100,30 @@ def build_world(cfg, inputs, args):
     w.upload(ra.HEAT_SRC, heat)
     w.settings = s
     return w
+
+
+def timed_run(cfg, inputs, args, mode):
+    """Warm-up, then exactly args.steps steps with the stream drained on both
+    sides; returns (seconds, world)."""
+    w = build_world(cfg, inputs, args, mode)
+    seeded = ra.Ledger.from_cells(inputs[2])
+    # warm-up, including one replay of every (chunk length, buffer parity)
+    # graph the timed loop will use, so no capture falls in the timed region
+    w.run(args.warmup)
+    for n in sorted({64 if args.steps >= 64 else 0, args.steps % 64} - {0}):
+        w.encode_steps(n)
+        w.encode_steps(n)
+    w.sync()
+    t0 = time.perf_counter()
+    w.run(args.steps)
+    w.sync()
+    dt = time.perf_counter() - t0
+    bad = w.ledger_status()
+    if bad is not None:
+        raise SystemExit(f"{mode}: per-channel monomer totals changed at epoch {bad}")
+    if w.ledger_blocking().channels() != seeded.channels():
+        raise SystemExit(f"{mode}: ledger differs from the seeded totals")
+    return dt, w
 
 
 def cpu_baseline(cfg, inputs, args):
@@ -98,7 +135,7 @@ def cpu_baseline(cfg, inputs, args):
 
     gamma, vel, chem, heat = inputs
     ow = oc.OracleWorld(cfg, SEED)
-    p, s = configure(ow.params, ra.WorldSettings.default(), args)
+    p, s = configure(ow.params, ra.WorldSettings.default(), args, args.cpu_mode)
     ow.params = p
     ow.set_settings(s)
     ow.upload(ra.GAMMA, gamma)
@@ -126,6 +163,10 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=64)
     ap.add_argument("--mg-cycles", type=int, default=1)
+    ap.add_argument("--mg-max-cycles", type=int, default=80, help="iteration cap of the tolerance-mode solve")
+    ap.add_argument("--modes", default="fp32,mixed,reference",
+                    help="comma list of fp32, mixed, reference; the first is the headline value")
+    ap.add_argument("--cpu-mode", default="fp32")
     ap.add_argument("--profile-steps", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
@@ -137,27 +178,22 @@ def main():
 
     cfg = ra.WorldConfig(GRID, FLUID)
     inputs = world_inputs(cfg)
-    w = build_world(cfg, inputs, args)
-    seeded = ra.Ledger.from_cells(inputs[2])
-
-    # warm-up, including one replay of every (chunk length, buffer parity)
-    # graph the timed loop will use, so no capture falls in the timed region
-    w.run(args.warmup)
-    for n in sorted({64 if args.steps >= 64 else 0, args.steps % 64} - {0}):
-        w.encode_steps(n)
-        w.encode_steps(n)
-    w.sync()
-    t0 = time.perf_counter()
-    w.run(args.steps)
-    w.sync()
-    dt = time.perf_counter() - t0
-    bad = w.ledger_status()
-    if bad is not None:
-        raise SystemExit(f"per-channel monomer totals changed at epoch {bad}")
-    if w.ledger_blocking().channels() != seeded.channels():
-        raise SystemExit("ledger differs from the seeded totals")
-    value = cfg.cells() * args.steps / dt
-    ms = 1000.0 * dt / args.steps
+    results = {}
+    for mode in args.modes.split(","):
+        dt, w = timed_run(cfg, inputs, args, mode)
+        it, rel = w.mg_stats()
+        solves, iters, missed = w.solver_stats()
+        results[mode] = {"value": cfg.cells() * args.steps / dt, "ms_per_step": 1000.0 * dt / args.steps,
+                         "last_solve_iterations": it, "last_true_rel_residual": rel,
+                         "mean_iterations": (iters / solves) if solves else None, "solves_missing_tol": missed}
+        if mode == args.modes.split(",")[0]:
+            head_world = w
+        else:
+            w.close()
+    head = args.modes.split(",")[0]
+    value = results[head]["value"]
+    ms = results[head]["ms_per_step"]
+    w = head_world
 
     # Per-kernel HIP-event timing pass (events on the world's stream).
     w.set_timing(True)
@@ -183,7 +219,7 @@ def main():
         "bytes_per_unit": per,
         "per": unit,
     }
-    kernels = {k: round(v[0] / max(v[1], 1) * 1000.0, 2) for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
+    kernels = {k: [round(v[0] / max(v[1], 1) * 1000.0, 2), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
 
     cpu = None
     if not args.no_cpu:
@@ -203,13 +239,17 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "world step, 2048^2 grid / 1024^2 fluid, FP32 multigrid (reference V-cycle schedule), "
-                        "15% rock + rubble, 16-row vent, Poisson(4) monomers, exact ledger every step",
+            "workload": "world step, 2048^2 grid / 1024^2 fluid, FP32 fluid, 8-level multigrid-preconditioned "
+                        "CG to relative residual 1e-5, 15% rock + rubble, 16-row vent, Poisson(4) monomers in 8 "
+                        "channels, exact u32 transport with the ledger checked every step",
             "grid": GRID,
             "fluid": FLUID,
             "mg_levels": len(w.mg_levels()),
-            "mg_cycles": args.mg_cycles,
+            "mode": head,
+            "mg_tol": MODES[head]["mg_tol"],
         },
+        "modes": results,
+        "mixed_vs_fp32": (results["mixed"]["value"] / results["fp32"]["value"]) if "mixed" in results and "fp32" in results else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,

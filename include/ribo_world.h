@@ -208,10 +208,18 @@ typedef struct RiboWorldSettings {
     uint32_t contact_enabled;     /* ignored */
     /* --- extensions (not in the reference) --- */
     uint32_t mg_min_size;         /* coarsest level bound, reference MG_MIN_SIZE = 8 */
-    float    mg_tol;              /* 0: run exactly mg_cycles (reference behaviour);
-                                     > 0: run V-cycles until ||r||2/||rhs||2 <= mg_tol
-                                     over fluid cells of level 0, at most mg_max_cycles */
-    uint32_t mg_max_cycles;
+    float    mg_tol;              /* 0: run exactly mg_cycles V-cycles (reference
+                                     behaviour); > 0: tolerance mode, conjugate
+                                     gradients preconditioned by a symmetric
+                                     V-cycle until ||r||2/||rhs||2 <= mg_tol over
+                                     fluid cells of level 0 */
+    uint32_t mg_max_cycles;       /* iteration cap of the tolerance mode.  Each
+                                     ribo_world_encode_steps call records, per
+                                     solve, 1.3 x the previous solve's iterations
+                                     + 4 (multiple of 4, at most the cap); a solve
+                                     that runs out is counted (see
+                                     ribo_world_solver_stats) and the next call
+                                     records the full cap again. */
     uint32_t precision;           /* RIBO_PRECISION_FP32 (drop-in) or _MIXED */
     uint32_t ledger_check;        /* 1: exact per-channel totals checked on the
                                      device after every transport pass */
@@ -317,8 +325,10 @@ RIBO_API int ribo_world_sync(RiboWorld* w);
  * out[4..7] spent, out[8..11] held in agents (always 0 here).  u32 wrap. */
 RIBO_API int ribo_world_ledger(RiboWorld* w, uint32_t out[12]);
 /* With settings.ledger_check = 1: returns RIBO_OK if every checked step kept
- * the per-channel totals seen when checking was armed, else RIBO_ERR_LEDGER
- * and the first failing epoch in *epoch (0xFFFFFFFF when none). */
+ * the per-channel totals seen when checking was armed (when ledger_check was
+ * switched on, and again at every upload of RIBO_BUF_CHEM), else
+ * RIBO_ERR_LEDGER and the first failing epoch in *epoch (0xFFFFFFFF when
+ * none). */
 RIBO_API int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch);
 
 /* ---- measurement (KernelTiming / profile_kernels.rs) -------------------- */
@@ -334,6 +344,11 @@ RIBO_API int ribo_world_reset_kernel_times(RiboWorld* w);
 /* Last multigrid statistics of the world: V-cycles run in the last fluid
  * step and the final relative residual (tolerance mode only, else -1). */
 RIBO_API int ribo_world_mg_stats(RiboWorld* w, uint32_t* cycles, double* rel_residual);
+
+/* Tolerance-mode totals since the world was created: solves, iterations,
+ * and solves that stopped on their recorded iteration slots before reaching
+ * mg_tol. */
+RIBO_API int ribo_world_solver_stats(RiboWorld* w, uint64_t* solves, uint64_t* iterations, uint64_t* missed);
 
 /* ---- errors -------------------------------------------------------------- */
 RIBO_API const char* ribo_last_error(void);

@@ -37,6 +37,7 @@ def lib() -> ctypes.CDLL:
             "oracle_ledger": (None, [P, P]),
             "oracle_run_kernel": (I, [P, ctypes.c_char_p]),
             "oracle_mg_residual": (ctypes.c_double, [P]),
+            "oracle_mg_stats": (None, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
             "oracle_set_threads": (None, [I]),
             "oracle_get_threads": (I, []),
             "oracle_pcg4d": (None, [P, P, SZ]),
@@ -146,6 +147,11 @@ class OracleWorld:
     def run_kernel(self, name: str):
         if lib().oracle_run_kernel(self._h, name.encode()) != 0:
             raise RuntimeError(f"unknown oracle kernel {name}")
+
+    def mg_stats(self):
+        c, r = ctypes.c_uint32(), ctypes.c_double()
+        lib().oracle_mg_stats(self._h, ctypes.byref(c), ctypes.byref(r))
+        return int(c.value), float(r.value)
 
     def mg_residual(self) -> float:
         return float(lib().oracle_mg_residual(self._h))

@@ -188,6 +188,7 @@ struct OracleWorld {
     Fluid<float> f32;
     Fluid<double> f64;
     double last_rel_res = -1.0;
+    uint32_t last_cycles = 0;
 };
 
 namespace {
@@ -784,11 +785,69 @@ template <class R> void enforce_boundaries(const Ctx& c, const R* vin, R* vout, 
 // ---- multigrid (multigrid.wgsl:25-121; schedule src/world/mod.rs:1545-1577) ----
 struct Level { uint32_t n, off, n_c, off_c; float eps; };
 
+// bfloat16 storage of the mixed-precision preconditioner: round to nearest
+// even on the 16 high bits.
+inline float round_bf16(float x) {
+    uint32_t u = rb_as_u32(x);
+    if ((u & 0x7F800000u) == 0x7F800000u) return x;
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return rb_as_f32(u & 0xFFFF0000u);
+}
+
+// Exact solve of the coarsest level of the tolerance mode: the dense inverse
+// of its eps-anchored operator over fluid cells, in binary64 (Gauss-Jordan
+// with partial pivoting; -A is symmetric positive definite).
+struct CoarseInv {
+    uint32_t n = 0;
+    std::vector<int32_t> idx;   // cell -> fluid unknown, -1 for walls
+    std::vector<uint32_t> cell; // unknown -> cell
+    std::vector<double> inv;    // m x m, row major
+    bool build(uint32_t n_, const uint32_t* fluid, float eps) {
+        n = n_;
+        idx.assign((size_t)n * n, -1);
+        cell.clear();
+        for (uint32_t i = 0; i < n * n; ++i)
+            if (fluid[i]) { idx[i] = (int32_t)cell.size(); cell.push_back(i); }
+        const size_t m = cell.size();
+        if (m == 0 || m > 1024) return false;
+        std::vector<double> a(m * m, 0.0);
+        inv.assign(m * m, 0.0);
+        for (size_t r = 0; r < m; ++r) {
+            const uint32_t c = cell[r], x = c % n, y = c / n;
+            int k = 0;
+            const int32_t nbr[4] = {x > 0 ? idx[c - 1] : -1, x + 1 < n ? idx[c + 1] : -1,
+                                    y > 0 ? idx[c - n] : -1, y + 1 < n ? idx[c + n] : -1};
+            for (int d = 0; d < 4; ++d)
+                if (nbr[d] >= 0) { a[r * m + (size_t)nbr[d]] = 1.0; ++k; }
+            a[r * m + r] = -((double)k + (double)eps);
+            inv[r * m + r] = 1.0;
+        }
+        for (size_t c = 0; c < m; ++c) {
+            size_t piv = c;
+            for (size_t r = c + 1; r < m; ++r)
+                if (fabs(a[r * m + c]) > fabs(a[piv * m + c])) piv = r;
+            if (piv != c)
+                for (size_t j = 0; j < m; ++j) { std::swap(a[c * m + j], a[piv * m + j]); std::swap(inv[c * m + j], inv[piv * m + j]); }
+            const double d = a[c * m + c];
+            for (size_t j = 0; j < m; ++j) { a[c * m + j] /= d; inv[c * m + j] /= d; }
+            for (size_t r = 0; r < m; ++r) {
+                if (r == c) continue;
+                const double f = a[r * m + c];
+                if (f == 0.0) continue;
+                for (size_t j = 0; j < m; ++j) { a[r * m + j] -= f * a[c * m + j]; inv[r * m + j] -= f * inv[c * m + j]; }
+            }
+        }
+        return true;
+    }
+};
+
 template <class R> struct MG {
     const std::vector<Level>* lv;
     R* p;
     R* rhs;
     uint32_t* fluid;
+    bool bf16 = false;              // round stores of the preconditioner to bfloat16
+    const CoarseInv* cinv = nullptr;  // exact coarsest-level solve (tolerance mode)
     bool is_fluid(const Level& L, int32_t x, int32_t y) const {  // :29-32
         if (x < 0 || y < 0 || x >= (int32_t)L.n || y >= (int32_t)L.n) return false;
         return fluid[L.off + (uint32_t)y * L.n + (uint32_t)x] != 0u;
@@ -856,6 +915,78 @@ template <class R> struct MG {
             prolong(L[l]);
             for (uint32_t s = 0; s < MG_POST; ++s) { smooth(L[l], 0); smooth(L[l], 1); }
         }
+    }
+    void round_level(const Level& L, R* a) {
+        if (!bf16) return;
+        for (uint32_t i = 0; i < L.n * L.n; ++i) a[L.off + i] = R(round_bf16((float)a[L.off + i]));
+    }
+    // The symmetric V-cycle of the tolerance mode: the reference V-cycle with
+    // the post-smoother in reverse colour order (black, red), restriction
+    // proportional to the transpose of the injection (half the sum over fluid
+    // children instead of twice their mean; equal on all-fluid cells) and an
+    // exact (or colour-symmetric) coarsest solve.  It is a symmetric
+    // preconditioner, as conjugate gradients requires.
+    void vcycle_sym() {
+        const std::vector<Level>& L = *lv;
+        const size_t levels = L.size();
+        for (size_t l = 0; l + 1 < levels; ++l) {
+            for (uint32_t s2 = 0; s2 < MG_PRE; ++s2) { smooth(L[l], 0); smooth(L[l], 1); }
+            round_level(L[l], p);
+            const Level& Lf = L[l];
+            for (uint32_t gy = 0; gy < Lf.n_c; ++gy)
+                for (uint32_t gx = 0; gx < Lf.n_c; ++gx) {
+                    R sum = 0;
+                    uint32_t nf = 0;
+                    for (uint32_t c = 0; c < 4; ++c) {
+                        const uint32_t fx = gx * 2u + (c & 1u), fy = gy * 2u + (c >> 1);
+                        const uint32_t i = Lf.off + fy * Lf.n + fx;
+                        if (fluid[i] == 0u) continue;
+                        R s, k;
+                        neighbors(Lf, fx, fy, &s, &k);
+                        sum += rhs[i] - (s - (k + R(Lf.eps)) * p[i]);
+                        nf += 1;
+                    }
+                    const uint32_t ci = Lf.off_c + gy * Lf.n_c + gx;
+                    fluid[ci] = nf > 0 ? 1u : 0u;
+                    rhs[ci] = nf > 0 ? R(0.5) * sum : R(0);
+                    p[ci] = 0;
+                }
+            round_level(L[l + 1], rhs);
+        }
+        const Level& C = L[levels - 1];
+        if (cinv) {
+            const size_t m = cinv->cell.size();
+            std::vector<double> b(m);
+            for (size_t r = 0; r < m; ++r) b[r] = (double)rhs[C.off + cinv->cell[r]];
+            for (size_t r = 0; r < m; ++r) {
+                double acc = 0.0;
+                for (size_t j = 0; j < m; ++j) acc += cinv->inv[r * m + j] * b[j];
+                p[C.off + cinv->cell[r]] = R(acc);
+            }
+        } else {
+            for (uint32_t s2 = 0; s2 < MG_COARSE / 2; ++s2) { smooth(C, 0); smooth(C, 1); }
+            for (uint32_t s2 = 0; s2 < MG_COARSE / 2; ++s2) { smooth(C, 1); smooth(C, 0); }
+        }
+        for (size_t l = levels - 1; l-- > 0;) {
+            prolong(L[l]);
+            for (uint32_t s2 = 0; s2 < MG_POST; ++s2) { smooth(L[l], 1); smooth(L[l], 0); }
+            round_level(L[l], p);
+        }
+    }
+    // A v on fluid cells of level 0: sum of fluid neighbours - (k + eps) v.
+    void apply(const R* v, R* out) const {
+        const Level& L = (*lv)[0];
+        for (uint32_t y = 0; y < L.n; ++y)
+            for (uint32_t x = 0; x < L.n; ++x) {
+                const uint32_t i = y * L.n + x;
+                if (fluid[L.off + i] == 0u) { out[i] = 0; continue; }
+                R s = 0, k = 0;
+                if (x > 0 && fluid[L.off + i - 1]) { s += v[i - 1]; k += R(1); }
+                if (x + 1 < L.n && fluid[L.off + i + 1]) { s += v[i + 1]; k += R(1); }
+                if (y > 0 && fluid[L.off + i - L.n]) { s += v[i - L.n]; k += R(1); }
+                if (y + 1 < L.n && fluid[L.off + i + L.n]) { s += v[i + L.n]; k += R(1); }
+                out[i] = s - (k + R(L.eps)) * v[i];
+            }
     }
     // ||rhs - A p||2 / ||rhs||2 over fluid cells of level 0 (binary64 sums).
     double rel_residual() const {
@@ -1181,6 +1312,82 @@ template <class R> void transport(OracleWorld& w, const Fluid<R>& fl, const Ribo
     std::fill(w.chem_next.begin(), w.chem_next.end(), 0u);
 }
 
+// Tolerance mode (an extension; the reference runs a fixed number of
+// V-cycles): preconditioned conjugate gradients on the eps-anchored operator
+// A p = div over fluid cells of level 0, warm-started from the previous
+// pressure, preconditioned by the symmetric V-cycle (MG::vcycle_sym), until
+// ||div - A p||2 / ||div||2 <= mg_tol or mg_max_cycles iterations.  Scalars
+// and dot products are binary64; the search direction's image is updated
+// as q = A z + beta q.  precision = MIXED stores the preconditioner's
+// unknowns and right-hand sides in bfloat16.
+template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const std::vector<Level>& lv) {
+    const RiboWorldSettings& st = w.st;
+    const size_t n2 = (size_t)lv[0].n * lv[0].n;
+    mg.bf16 = st.precision == RIBO_PRECISION_MIXED;
+    for (size_t l = 0; l + 1 < lv.size(); ++l)  // coarse masks: any fluid child (multigrid.wgsl:98)
+        for (uint32_t gy = 0; gy < lv[l].n_c; ++gy)
+            for (uint32_t gx = 0; gx < lv[l].n_c; ++gx) {
+                uint32_t nf = 0;
+                for (uint32_t c = 0; c < 4; ++c)
+                    nf += w.mg_fluid[lv[l].off + (gy * 2u + (c >> 1)) * lv[l].n + gx * 2u + (c & 1u)] ? 1u : 0u;
+                w.mg_fluid[lv[l].off_c + gy * lv[l].n_c + gx] = nf > 0 ? 1u : 0u;
+            }
+    CoarseInv ci;
+    const Level& C = lv.back();
+    mg.cinv = ci.build(C.n, w.mg_fluid.data() + C.off, C.eps) ? &ci : nullptr;
+    std::vector<R> r(n2), z(n2), d(n2), q(n2), az(n2);
+    R* p = fl.p_a.data();
+    const R* b = fl.div.data();
+    mg.apply(p, q.data());
+    double bb = 0.0, rr = 0.0;
+    for (size_t i = 0; i < n2; ++i) {
+        r[i] = w.mg_fluid[i] ? b[i] - q[i] : R(0);
+        bb += (double)b[i] * (double)b[i];
+        rr += (double)r[i] * (double)r[i];
+    }
+    auto precond = [&]() {
+        for (size_t i = 0; i < n2; ++i) { fl.mg_rhs[i] = r[i]; fl.mg_p[i] = 0; }
+        mg.vcycle_sym();
+        for (size_t i = 0; i < n2; ++i) z[i] = fl.mg_p[i];
+    };
+    uint32_t it = 0;
+    const uint32_t maxit = std::max(st.mg_max_cycles, 1u);
+    double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+    if (rel > (double)st.mg_tol) {
+        precond();
+        double rho = 0.0;
+        for (size_t i = 0; i < n2; ++i) rho += (double)z[i] * (double)r[i];
+        d = z;
+        mg.apply(d.data(), q.data());
+        while (it < maxit) {
+            double dq = 0.0;
+            for (size_t i = 0; i < n2; ++i) dq += (double)d[i] * (double)q[i];
+            const R alpha = R(dq != 0.0 ? rho / dq : 0.0);
+            rr = 0.0;
+            for (size_t i = 0; i < n2; ++i) {
+                p[i] += alpha * d[i];
+                r[i] -= alpha * q[i];
+                rr += (double)r[i] * (double)r[i];
+            }
+            ++it;
+            rel = bb > 0 ? sqrt(rr / bb) : 0.0;
+            if (rel <= (double)st.mg_tol || it >= maxit) break;
+            precond();
+            double rho2 = 0.0;
+            for (size_t i = 0; i < n2; ++i) rho2 += (double)z[i] * (double)r[i];
+            const R beta = R(rho != 0.0 ? rho2 / rho : 0.0);
+            rho = rho2;
+            mg.apply(z.data(), az.data());
+            for (size_t i = 0; i < n2; ++i) {
+                d[i] = z[i] + beta * d[i];
+                q[i] = az[i] + beta * q[i];
+            }
+        }
+    }
+    w.last_cycles = it;
+    w.last_rel_res = rel;
+}
+
 // ---- one fluid step (src/world/mod.rs:1527-1588) ---------------------------------
 template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimParams& P) {
     Ctx c{&w, w.G, w.F, &P};
@@ -1212,14 +1419,8 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
         }
         uint32_t cycles = std::max(st.mg_cycles, 1u);
         if (st.mg_tol > 0.0f) {
-            cycles = 0;
-            double rr = mg.rel_residual();
-            while (cycles < std::max(st.mg_max_cycles, 1u) && !(rr <= (double)st.mg_tol)) {
-                mg.vcycle();
-                ++cycles;
-                rr = mg.rel_residual();
-            }
-            w.last_rel_res = rr;
+            pcg_solve<R>(w, fl, mg, lv);
+            for (uint32_t i = 0; i < L0.n * L0.n; ++i) fl.mg_p[i] = fl.p_a[i];
         } else {
             for (uint32_t k = 0; k < cycles; ++k) mg.vcycle();
         }
@@ -1506,6 +1707,11 @@ void oracle_ledger(const OracleWorld* w, uint32_t out[12]) {  // Ledger::from_ce
         out[i % 4] += w->chem[i] & 0xFFFFu;
         out[4 + i % 4] += w->chem[i] >> 16;
     }
+}
+
+void oracle_mg_stats(const OracleWorld* w, uint32_t* cycles, double* rel) {
+    *cycles = w->last_cycles;
+    *rel = w->last_rel_res;
 }
 
 double oracle_mg_residual(const OracleWorld* cw) {

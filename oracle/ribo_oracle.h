@@ -62,6 +62,8 @@ int oracle_run_kernel(OracleWorld* w, const char* name);
 /* Relative residual ||r||2 / ||rhs||2 over fluid cells of level 0 for the
  * current pressure and divergence (eps-anchored operator). */
 double oracle_mg_residual(const OracleWorld* w);
+/* Iterations and final relative residual of the last tolerance-mode solve. */
+void oracle_mg_stats(const OracleWorld* w, uint32_t* cycles, double* rel);
 /* Threads used by the parallel loops (OpenMP); 0 = library default. */
 void oracle_set_threads(int n);
 int oracle_get_threads(void);

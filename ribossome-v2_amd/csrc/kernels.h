@@ -31,28 +31,56 @@ struct Geo {
     const float2* heat;       // heat_src, f32x2[F]
 };
 
-// One multigrid level resident in HBM.
-struct MgLevelDev {
+// bfloat16 storage word (round to nearest even on the high 16 bits of the
+// binary32 value; infinities and NaN pass through).
+struct bf16 { uint16_t bits; };
+__host__ __device__ inline uint16_t bf16_bits(float v) {
+    union { float f; uint32_t u; } c;
+    c.f = v;
+    uint32_t u = c.u;
+    if ((u & 0x7F800000u) == 0x7F800000u) return (uint16_t)(u >> 16);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// One multigrid level resident in HBM; S is the storage type of its arrays.
+template <class S> struct MgLevelT {
     uint32_t n;
     float eps;
-    float* p;
-    float* rhs;
+    S* p;
+    S* rhs;
     const uint8_t* fluid;
 };
+using MgLevelDev = MgLevelT<float>;
 
-// Device scratch of the multigrid residual tracking and ledger check.
-struct Counters {
-    double rr;          // sum of squared residuals (level 0)
-    double bb;          // sum of squared rhs (level 0)
-    uint32_t blocks_done;
-    uint32_t cycles;    // V-cycles run in the current fluid step
-    uint32_t done;      // tolerance reached
-    uint32_t pad;
-    double last_rel;    // last relative residual
+// Level-0 vectors of the preconditioned conjugate gradients (tolerance
+// mode).  z is the preconditioner's output and lives in its storage type S;
+// rs is the preconditioner's copy of r (null when S is float: it reads r).
+template <class S> struct PcgVec {
+    uint32_t n;
+    float eps;
+    const uint8_t* fl;
+    float* p;
+    const float* b;
+    float* r;
+    S* rs;
+    S* z;
+    float* d;
+    float* q;
 };
 
+struct PcgState {
+    double bb, rho, beta, alpha, rel, true_rel;
+    uint32_t it, done, count, steps, iters, missed;
+    float tol_seen;
+    uint32_t pad;
+    double* part;       // 2 doubles per block of the level-0 grid
+    uint32_t* host_out; // host-mapped words: last iterations, misses, solves
+};
+
+constexpr uint32_t LEDGER_MAX_BLOCKS = 1024;
 struct LedgerDev {
-    uint32_t acc[8];        // act[4], spent[4] of the current reduction
+    uint32_t part[8 * LEDGER_MAX_BLOCKS];  // per-block act[4], spent[4]
     uint32_t blocks_done;
     uint32_t armed;         // 1 once expected[] is set
     uint32_t expected[4];   // per-channel act + spent
@@ -86,16 +114,22 @@ void launch_vorticity(hipStream_t s, const Geo& g, const RiboSimParams& P, const
 void launch_divergence(hipStream_t s, const Geo& g, const float2* vin, float* div, float* p0);
 void launch_jacobi(hipStream_t s, const Geo& g, const float* pin, float* pout, const float* div);
 void launch_mg_smooth(hipStream_t s, const MgLevelDev& L, uint32_t color, const uint32_t* done);
-void launch_mg_restrict(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
-void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
-// fused multigrid: pre-smooth + restrict, prolong + post-smooth per level, and
-// the coarse end of the cycle (at most 8 levels) in one workgroup
-void launch_mg_down(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
-void launch_mg_up(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done);
-size_t mg_block_lds_bytes(const MgLevelDev* lv, int count);
-void launch_mg_block(hipStream_t s, const MgLevelDev* lv, int count, const uint32_t* done);
-void launch_mg_rhs_norm(hipStream_t s, const MgLevelDev& L0, Counters* c);
-void launch_mg_residual(hipStream_t s, const MgLevelDev& L0, Counters* c, float tol, uint32_t max_cycles, int count_cycle);
+// fused multigrid: pre-smooth + restrict, prolong + post-smooth per level,
+// and the coarse end of the cycle (at most 8 levels) in one workgroup.
+// sym = the symmetric V-cycle of the tolerance mode.
+template <class S> void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym,
+                                       const uint32_t* done);
+template <class S> void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym,
+                                     const uint32_t* done);
+size_t mg_block_lds_bytes(const uint32_t* n, int count);
+template <class S> void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv,
+                                        const int32_t* ccell, int cm, const uint32_t* done);
+// preconditioned conjugate gradients
+template <class S> void launch_pcg_init(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
+template <class S> void launch_pcg_rho(hipStream_t s, const PcgVec<S>& V, PcgState* st);
+template <class S> void launch_pcg_dir(hipStream_t s, const PcgVec<S>& V, PcgState* st);
+template <class S> void launch_pcg_update(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
+template <class S> void launch_pcg_final(hipStream_t s, const PcgVec<S>& V, PcgState* st);
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);

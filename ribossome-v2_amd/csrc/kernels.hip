@@ -642,7 +642,18 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgLevelDev L, MgLevelDev C, 
 // after half-sweep k every cell at distance >= k from the loaded edge is
 // exact.  The tiles below therefore reproduce the per-dispatch schedule of
 // the reference bit for bit while reading each level once per direction.
+//
+// Template parameters: S is the storage type of the level arrays in HBM
+// (float, or bf16 for the mixed-precision preconditioner; arithmetic is
+// always binary32 in LDS); SYM selects the symmetric V-cycle of the
+// tolerance mode (post-smoothing black then red, restriction = half the sum
+// over fluid children, exact coarsest solve) instead of the reference's.
 constexpr int MG_T = 32;  // output tile (even, so tile origins keep (x+y) parity)
+
+__device__ __forceinline__ float ldv(const float* a, int i) { return a[i]; }
+__device__ __forceinline__ float ldv(const bf16* a, int i) { return rb_as_f32((uint32_t)a[i].bits << 16); }
+__device__ __forceinline__ void stv(float* a, int i, float v) { a[i] = v; }
+__device__ __forceinline__ void stv(bf16* a, int i, float v) { a[i].bits = bf16_bits(v); }
 
 template <int H>
 struct MgTile {
@@ -673,8 +684,8 @@ __device__ __forceinline__ void tile_half_sweep(MgTile<H>& t, int color, float e
     }
 }
 
-template <int H>
-__device__ __forceinline__ void tile_load(MgTile<H>& t, const MgLevelDev& L, int x0, int y0, const MgLevelDev* C) {
+template <int H, class S>
+__device__ __forceinline__ void tile_load(MgTile<H>& t, const MgLevelT<S>& L, int x0, int y0, const MgLevelT<S>* C) {
     constexpr int R = MgTile<H>::R;
     const int n = (int)L.n;
     for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
@@ -686,10 +697,10 @@ __device__ __forceinline__ void tile_load(MgTile<H>& t, const MgLevelDev& L, int
             const int i = gy * n + gx;
             f = L.fluid[i];
             if (f) {
-                pv = L.p[i];
-                rv = L.rhs[i];
+                pv = ldv(L.p, i);
+                rv = ldv(L.rhs, i);
                 // mg_prolong (multigrid.wgsl:108-114), pointwise, applied on load
-                if (C) pv += C->p[(gy / 2) * (int)C->n + gx / 2];
+                if (C) pv += ldv(C->p, (gy / 2) * (int)C->n + gx / 2);
             }
         }
         t.p[ly][lx] = pv;
@@ -700,12 +711,13 @@ __device__ __forceinline__ void tile_load(MgTile<H>& t, const MgLevelDev& L, int
 
 // Pre-smoothing (2 x red, black) and restriction of one level
 // (src/world/mod.rs:1556-1563; multigrid.wgsl:60-105).
-__global__ __launch_bounds__(256) void k_mg_down(MgLevelDev L, MgLevelDev C, const uint32_t* done) {
+template <class S, bool SYM>
+__global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
     constexpr int H = 5;
     __shared__ MgTile<H> t;
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H>(t, L, x0, y0, nullptr);
+    tile_load<H, S>(t, L, x0, y0, nullptr);
     __syncthreads();
     constexpr int R = MgTile<H>::R;
     for (int s = 0; s < 4; ++s) {
@@ -715,7 +727,7 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelDev L, MgLevelDev C, con
     const int n = (int)L.n;
     for (int k = threadIdx.x; k < MG_T * MG_T; k += blockDim.x) {
         const int gx = x0 + H + k % MG_T, gy = y0 + H + k / MG_T;
-        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) L.p[gy * n + gx] = t.p[H + k / MG_T][H + k % MG_T];
+        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) stv(L.p, gy * n + gx, t.p[H + k / MG_T][H + k % MG_T]);
     }
     const int nc = (int)C.n;
     const int ct = threadIdx.x;  // one thread per coarse cell of the tile
@@ -736,47 +748,54 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelDev L, MgLevelDev C, con
                 nf += 1u;
             }
             const int ci = cy * nc + cx;
-            C.rhs[ci] = nf > 0u ? 2.0f * sum / (float)rb_maxu(nf, 1u) : 0.0f;
-            C.p[ci] = 0.0f;
+            const float rc = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
+            stv(C.rhs, ci, rc);
+            stv(C.p, ci, 0.0f);
         }
     }
 }
 
-// Prolongation and post-smoothing (2 x red, black) of one level
-// (src/world/mod.rs:1569-1575).
-__global__ __launch_bounds__(256) void k_mg_up(MgLevelDev L, MgLevelDev C, const uint32_t* done) {
+// Prolongation and post-smoothing (2 x red, black; black, red when SYM) of
+// one level (src/world/mod.rs:1569-1575).
+template <class S, bool SYM>
+__global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
     constexpr int H = 4;
     __shared__ MgTile<H> t;
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H>(t, L, x0, y0, &C);
+    tile_load<H, S>(t, L, x0, y0, &C);
     __syncthreads();
     constexpr int R = MgTile<H>::R;
     for (int s = 0; s < 4; ++s) {
-        tile_half_sweep<H>(t, s & 1, L.eps, 1 + s, R - 1 - s);
+        tile_half_sweep<H>(t, SYM ? 1 - (s & 1) : (s & 1), L.eps, 1 + s, R - 1 - s);
         __syncthreads();
     }
     const int n = (int)L.n;
     for (int k = threadIdx.x; k < MG_T * MG_T; k += blockDim.x) {
         const int gx = x0 + H + k % MG_T, gy = y0 + H + k / MG_T;
-        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) L.p[gy * n + gx] = t.p[H + k / MG_T][H + k % MG_T];
+        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) stv(L.p, gy * n + gx, t.p[H + k / MG_T][H + k % MG_T]);
     }
 }
 
 // The coarse end of the V-cycle in one workgroup: every level of `lv` (at
-// most MG_BLOCK_LEVELS, each small enough that all of them fit in LDS
-// together) goes through pre-smoothing and restriction, the coarsest gets
-// 30 x (red, black), then prolongation and post-smoothing back up; the top
-// level's unknowns are read from and written back to HBM.
+// most MG_BLOCK_LEVELS, small enough that all of them fit in LDS together)
+// goes through pre-smoothing and restriction, the coarsest gets 30 x (red,
+// black) -- or, in the tolerance mode, an exact solve with the binary64
+// inverse of its operator -- then prolongation and post-smoothing back up;
+// the top level's unknowns are read from and written back to HBM.
 constexpr int MG_BLOCK_LEVELS = 8;
+template <class S>
 struct MgBlockArgs {
     int levels;
     int n[MG_BLOCK_LEVELS];
     float eps[MG_BLOCK_LEVELS];
     const uint8_t* fluid[MG_BLOCK_LEVELS];
     int off[MG_BLOCK_LEVELS + 1];  // cell offsets of each level in LDS
-    float* p0;
-    const float* rhs0;
+    S* p0;
+    const S* rhs0;
+    const double* cinv;            // m x m inverse of the coarsest operator (SYM only)
+    const int32_t* ccell;          // unknown -> coarsest cell
+    int cm;                        // unknowns of the coarsest level (0: use sweeps)
 };
 
 __device__ __forceinline__ void blk_sweep(float* p, const float* rhs, const uint8_t* fl, int n, float eps, int color) {
@@ -797,7 +816,8 @@ __device__ __forceinline__ void blk_sweep(float* p, const float* rhs, const uint
     }
 }
 
-__global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs a, const uint32_t* done) {
+template <class S, bool SYM>
+__global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint32_t* done) {
     if (done && *done) return;
     extern __shared__ float sm[];
     const int total = a.off[a.levels];
@@ -808,8 +828,8 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs a, const uint32_t
     for (int i = threadIdx.x; i < n0 * n0; i += blockDim.x) {
         const uint8_t f = a.fluid[0][i];
         FL[i] = f;
-        P[i] = f ? a.p0[i] : 0.0f;
-        RHS[i] = f ? a.rhs0[i] : 0.0f;
+        P[i] = f ? ldv(a.p0, i) : 0.0f;
+        RHS[i] = f ? ldv(a.rhs0, i) : 0.0f;
     }
     for (int l = 1; l < a.levels; ++l)
         for (int i = threadIdx.x; i < a.n[l] * a.n[l]; i += blockDim.x) FL[a.off[l] + i] = a.fluid[l][i];
@@ -843,7 +863,7 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs a, const uint32_t
                 sum += rhs[i] - (s2 - (kk + a.eps[l]) * p[i]);
                 nf += 1u;
             }
-            rc[ci] = nf > 0u ? 2.0f * sum / (float)rb_maxu(nf, 1u) : 0.0f;
+            rc[ci] = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
             pc[ci] = 0.0f;
         }
         __syncthreads();
@@ -852,9 +872,53 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs a, const uint32_t
         float* p = P + a.off[L - 1];
         const float* rhs = RHS + a.off[L - 1];
         const uint8_t* fl = FL + a.off[L - 1];
-        for (int s = 0; s < 2 * 30; ++s) {
-            blk_sweep(p, rhs, fl, a.n[L - 1], a.eps[L - 1], s & 1);
+        if (SYM && a.cm > 0) {
+            // exact coarsest solve: p = A^-1 rhs over fluid cells, binary64
+            for (int r = threadIdx.x; r < a.cm; r += blockDim.x) {
+                double acc = 0.0;
+                const double* row = a.cinv + (size_t)r * a.cm;
+                for (int j = 0; j < a.cm; ++j) acc += row[j] * (double)rhs[a.ccell[j]];
+                p[a.ccell[r]] = (float)acc;
+            }
             __syncthreads();
+        } else if (a.n[L - 1] * a.n[L - 1] <= 64) {
+            // coarsest level of at most 64 cells: one wave, one cell per lane,
+            // neighbours exchanged through cross-lane permutes
+            if (threadIdx.x < 64) {
+                const int nn = a.n[L - 1], lane = threadIdx.x;
+                const bool live = lane < nn * nn;
+                const int cx = live ? lane % nn : 0, cy = live ? lane / nn : 0;
+                const bool me = live && fl[lane];
+                const bool fl_l = live && cx > 0 && fl[lane - 1];
+                const bool fl_r = live && cx + 1 < nn && fl[lane + 1];
+                const bool fl_b = live && cy > 0 && fl[lane - nn];
+                const bool fl_t = live && cy + 1 < nn && fl[lane + nn];
+                const float rv = live ? rhs[lane] : 0.0f;
+                const float eps = a.eps[L - 1];
+                const int color = (cx + cy) & 1;
+                float pv = live ? p[lane] : 0.0f;
+                for (int s = 0; s < 2 * 30; ++s) {
+                    const int want = (SYM && s >= 30) ? 1 - (s & 1) : (s & 1);
+                    const float pl = __shfl(pv, lane - 1, 64), pr = __shfl(pv, lane + 1, 64);
+                    const float pb = __shfl(pv, lane - nn, 64), pt = __shfl(pv, lane + nn, 64);
+                    if (me && color == want) {
+                        float sum = 0.0f, k = 0.0f;
+                        if (fl_l) { sum += pl; k += 1.0f; }
+                        if (fl_r) { sum += pr; k += 1.0f; }
+                        if (fl_b) { sum += pb; k += 1.0f; }
+                        if (fl_t) { sum += pt; k += 1.0f; }
+                        pv = (sum - rv) / (k + eps);
+                    }
+                }
+                if (live) p[lane] = pv;
+            }
+            __syncthreads();
+        } else {
+            for (int s = 0; s < 2 * 30; ++s) {
+                const int color = (SYM && s >= 30) ? 1 - (s & 1) : (s & 1);
+                blk_sweep(p, rhs, fl, a.n[L - 1], a.eps[L - 1], color);
+                __syncthreads();
+            }
         }
     }
     for (int l = L - 2; l >= 0; --l) {
@@ -867,75 +931,194 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs a, const uint32_t
             if (fl[i]) p[i] += pc[((i / n) / 2) * nc + (i % n) / 2];
         __syncthreads();
         for (int s = 0; s < 4; ++s) {
-            blk_sweep(p, rhs, fl, n, a.eps[l], s & 1);
+            blk_sweep(p, rhs, fl, n, a.eps[l], SYM ? 1 - (s & 1) : (s & 1));
             __syncthreads();
         }
     }
     for (int i = threadIdx.x; i < n0 * n0; i += blockDim.x)
-        if (FL[i]) a.p0[i] = P[i];
+        if (FL[i]) stv(a.p0, i, P[i]);
 }
 
-// Residual tracking for the tolerance mode (binary64 sums, one block-level
-// reduction and a last-block finaliser; no host round trip).
-__device__ __forceinline__ double block_sum_f64(double v, double* sh) {
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-    if (lane == 0) sh[wid] = v;
+// ---- tolerance mode: preconditioned conjugate gradients ------------------------
+// Level-0 vectors over fluid cells; dot products are binary64, summed per
+// block and then over blocks in block order by the last block to finish, so
+// every run gives the same bits.  A v = sum of fluid neighbours - (k + eps) v.
+__device__ __forceinline__ float apply_A(const uint8_t* fl, const float* v, int n, int x, int y, float eps) {
+    const int i = y * n + x;
+    float s = 0.0f, k = 0.0f;
+    if (x > 0 && fl[i - 1]) { s += v[i - 1]; k += 1.0f; }
+    if (x + 1 < n && fl[i + 1]) { s += v[i + 1]; k += 1.0f; }
+    if (y > 0 && fl[i - n]) { s += v[i - n]; k += 1.0f; }
+    if (y + 1 < n && fl[i + n]) { s += v[i + n]; k += 1.0f; }
+    return s - (k + eps) * v[i];
+}
+template <class S>
+__device__ __forceinline__ float apply_A_s(const uint8_t* fl, const S* v, int n, int x, int y, float eps) {
+    const int i = y * n + x;
+    float s = 0.0f, k = 0.0f;
+    if (x > 0 && fl[i - 1]) { s += ldv(v, i - 1); k += 1.0f; }
+    if (x + 1 < n && fl[i + 1]) { s += ldv(v, i + 1); k += 1.0f; }
+    if (y > 0 && fl[i - n]) { s += ldv(v, i - n); k += 1.0f; }
+    if (y + 1 < n && fl[i + n]) { s += ldv(v, i + n); k += 1.0f; }
+    return s - (k + eps) * ldv(v, i);
+}
+
+// Block sums of up to 2 values into part[block * 2 + j]; returns true in the
+// one thread of the last block to finish, which then owns the totals.
+__device__ __forceinline__ bool block_reduce2(double v0, double v1, double* part, uint32_t* count, double* tot) {
+    __shared__ double sh[2][16];
+    __shared__ bool last;
+    for (int o = 32; o > 0; o >>= 1) {
+        v0 += __shfl_down(v0, o, 64);
+        v1 += __shfl_down(v1, o, 64);
+    }
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6, nw = (blockDim.x * blockDim.y) >> 6;
+    if (lane == 0) { sh[0][wid] = v0; sh[1][wid] = v1; }
     __syncthreads();
-    double t = 0.0;
-    if (threadIdx.x == 0)
-        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) t += sh[i];
-    return t;
-}
-__global__ void k_counters_reset(Counters* c) {
-    c->rr = 0.0;
-    c->bb = 0.0;
-    c->blocks_done = 0u;
-    c->cycles = 0u;
-    c->done = 0u;
-}
-__global__ __launch_bounds__(256) void k_mg_rhs_norm(MgLevelDev L, Counters* c) {
-    __shared__ double sh[4];
-    const uint32_t n2 = L.n * L.n;
-    double v = 0.0;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n2; i += gridDim.x * 256)
-        if (L.fluid[i]) v += (double)L.rhs[i] * (double)L.rhs[i];
-    double t = block_sum_f64(v, sh);
-    if (threadIdx.x == 0) {
-        atomicAdd(&c->bb, t);
+    const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
+    if (tid == 0) {
+        double a0 = 0.0, a1 = 0.0;
+        for (int w = 0; w < nw; ++w) { a0 += sh[0][w]; a1 += sh[1][w]; }
+        part[2 * b] = a0;
+        part[2 * b + 1] = a1;
         __threadfence();
-        uint32_t d = atomicAdd(&c->blocks_done, 1u);
-        if (d + 1u == gridDim.x) c->blocks_done = 0u;
+        last = atomicAdd(count, 1u) + 1u == nb;
+    }
+    __syncthreads();
+    if (!last) return false;
+    if (tid == 0) {
+        __threadfence();
+        double a0 = 0.0, a1 = 0.0;
+        for (uint32_t k = 0; k < nb; ++k) {
+            a0 += __hip_atomic_load(part + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            a1 += __hip_atomic_load(part + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        tot[0] = a0;
+        tot[1] = a1;
+        *count = 0u;
+    }
+    return tid == 0;
+}
+
+#define PCG_XY                                                                         \
+    const int x = (int)(blockIdx.x * 16 + threadIdx.x), y = (int)(blockIdx.y * 16 + threadIdx.y); \
+    const int n = (int)V.n;                                                            \
+    const bool in = x < n && y < n;                                                    \
+    const int i = in ? y * n + x : 0;                                                  \
+    const bool f = in && V.fl[i];
+
+// r = b - A p, z = 0 (and the preconditioner's copy of r); ||b||^2, ||r||^2.
+template <class S>
+__global__ __launch_bounds__(256) void k_pcg_init(PcgVec<S> V, PcgState* st, float tol, uint32_t maxit) {
+    PCG_XY
+    double bb = 0.0, rr = 0.0;
+    if (in) {
+        float r = 0.0f;
+        if (f) {
+            r = V.b[i] - apply_A(V.fl, V.p, n, x, y, V.eps);
+            bb = (double)V.b[i] * (double)V.b[i];
+            rr = (double)r * (double)r;
+        }
+        V.r[i] = r;
+        if (V.rs) stv(V.rs, i, r);
+        stv(V.z, i, 0.0f);
+    }
+    double tot[2];
+    if (block_reduce2(bb, rr, st->part, &st->count, tot)) {
+        st->bb = tot[0];
+        const double rel = tot[0] > 0.0 ? sqrt(tot[1] / tot[0]) : 0.0;
+        st->rel = rel;
+        st->it = 0u;
+        st->done = (rel <= (double)tol || maxit == 0u) ? 1u : 0u;
+        st->rho = 0.0;
+        st->tol_seen = tol;
     }
 }
-__global__ __launch_bounds__(256) void k_mg_residual(MgLevelDev L, Counters* c, float tol, uint32_t max_cycles, int count_cycle) {
-    if (c->done) return;
-    __shared__ double sh[4];
-    const uint32_t n = L.n, n2 = n * n;
-    double v = 0.0;
-    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < n2; i += gridDim.x * 256) {
-        if (!L.fluid[i]) continue;
-        float s, k;
-        mg_neighbors(L, i % n, i / n, &s, &k);
-        double r = (double)L.rhs[i] - ((double)s - ((double)k + (double)L.eps) * (double)L.p[i]);
-        v += r * r;
+
+// rho_new = z . r; beta = rho_new / rho (0 on the first iteration).
+template <class S>
+__global__ __launch_bounds__(256) void k_pcg_rho(PcgVec<S> V, PcgState* st) {
+    if (st->done) return;
+    PCG_XY
+    double zr = 0.0;
+    if (f) zr = (double)ldv(V.z, i) * (double)V.r[i];
+    double tot[2];
+    if (block_reduce2(zr, 0.0, st->part, &st->count, tot)) {
+        st->beta = st->it == 0u ? 0.0 : (st->rho != 0.0 ? tot[0] / st->rho : 0.0);
+        st->rho = tot[0];
     }
-    double t = block_sum_f64(v, sh);
-    if (threadIdx.x == 0) {
-        atomicAdd(&c->rr, t);
-        __threadfence();
-        uint32_t d = atomicAdd(&c->blocks_done, 1u);
-        if (d + 1u == gridDim.x) {
-            __threadfence();
-            const double rr = atomicAdd(&c->rr, 0.0);
-            const double bb = atomicAdd(&c->bb, 0.0);
-            const double rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
-            const uint32_t cyc = c->cycles + (count_cycle ? 1u : 0u);
-            c->cycles = cyc;
-            c->last_rel = rel;
-            if (rel <= (double)tol || cyc >= max_cycles) c->done = 1u;
-            c->rr = 0.0;
-            c->blocks_done = 0u;
+}
+
+// d = z + beta d, q = A z + beta q; d . q.
+template <class S>
+__global__ __launch_bounds__(256) void k_pcg_dir(PcgVec<S> V, PcgState* st) {
+    if (st->done) return;
+    PCG_XY
+    const float beta = (float)st->beta;
+    const bool first = st->it == 0u;
+    double dq = 0.0;
+    if (in) {
+        float d = 0.0f, q = 0.0f;
+        if (f) {
+            const float az = apply_A_s(V.fl, V.z, n, x, y, V.eps);
+            const float z = ldv(V.z, i);
+            d = first ? z : z + beta * V.d[i];
+            q = first ? az : az + beta * V.q[i];
+            dq = (double)d * (double)q;
+        }
+        V.d[i] = d;
+        V.q[i] = q;
+    }
+    double tot[2];
+    if (block_reduce2(dq, 0.0, st->part, &st->count, tot)) st->alpha = tot[0] != 0.0 ? st->rho / tot[0] : 0.0;
+}
+
+// p += alpha d, r -= alpha q, z = 0; ||r||^2 and the convergence test.
+template <class S>
+__global__ __launch_bounds__(256) void k_pcg_update(PcgVec<S> V, PcgState* st, float tol, uint32_t maxit) {
+    if (st->done) return;
+    PCG_XY
+    const float alpha = (float)st->alpha;
+    double rr = 0.0;
+    if (f) {
+        V.p[i] += alpha * V.d[i];
+        const float r = V.r[i] - alpha * V.q[i];
+        V.r[i] = r;
+        if (V.rs) stv(V.rs, i, r);
+        stv(V.z, i, 0.0f);
+        rr = (double)r * (double)r;
+    }
+    double tot[2];
+    if (block_reduce2(rr, 0.0, st->part, &st->count, tot)) {
+        const uint32_t it = st->it + 1u;
+        st->it = it;
+        const double rel = st->bb > 0.0 ? sqrt(tot[0] / st->bb) : 0.0;
+        st->rel = rel;
+        if (rel <= (double)tol || it >= maxit) st->done = 1u;
+    }
+}
+
+// Final true residual ||b - A p|| / ||b|| of the solve, for the statistics.
+template <class S>
+__global__ __launch_bounds__(256) void k_pcg_final(PcgVec<S> V, PcgState* st) {
+    PCG_XY
+    double rr = 0.0;
+    if (f) {
+        const float r = V.b[i] - apply_A(V.fl, V.p, n, x, y, V.eps);
+        rr = (double)r * (double)r;
+    }
+    double tot[2];
+    if (block_reduce2(rr, 0.0, st->part, &st->count, tot)) {
+        st->true_rel = st->bb > 0.0 ? sqrt(tot[0] / st->bb) : 0.0;
+        st->steps += 1u;
+        st->iters += st->it;
+        // a solve that stopped on its iteration slots before converging
+        if (!st->done) st->missed += 1u;
+        if (st->host_out) {
+            st->host_out[0] = st->it;
+            st->host_out[1] = st->missed;
+            st->host_out[2] = st->steps;
         }
     }
 }
@@ -958,52 +1141,56 @@ __global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict
 }
 
 // net_flux (fluid.wgsl:640-657), in the reference's sequential order per row
-// and per column so the means are bit-identical to it.  Blocks [0, F/64)
-// walk columns (one thread per column; each step reads one coalesced row
-// segment); the rest walk rows through 64x64 tiles staged in LDS, so the
-// per-row walk reads LDS instead of strided HBM.
-constexpr uint32_t NF_T = 64;
-__global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
+// and per column so the means are bit-identical to it.  Each block owns
+// NF_C chains (rows or columns): all threads stage NF_K-long stretches of
+// the chains into LDS with coalesced loads, double-buffered, while NF_C
+// threads walk the previous stretch in order.  Walls are marked NaN (a
+// sanitised velocity is never NaN) and skipped.
+constexpr uint32_t NF_C = 16, NF_K = 64;
+__global__ __launch_bounds__(256) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
+    __shared__ float buf[2][NF_C][NF_K + 1];
     const uint32_t F = g.F;
-    const uint32_t nb = (F + NF_T - 1) / NF_T;
+    const uint32_t nb = (F + NF_C - 1) / NF_C;
+    const bool cols = blockIdx.x < nb;
+    const uint32_t c0 = (cols ? blockIdx.x : blockIdx.x - nb) * NF_C;
     const uint32_t t = threadIdx.x;
-    if (blockIdx.x < nb) {
-        const uint32_t col = blockIdx.x * NF_T + t;
-        if (col >= F) return;
-        float sum = 0.0f, n = 0.0f;
-        for (uint32_t k = 0; k < F; ++k) {
-            const uint32_t q = k * F + col;
-            if (solid(g, q)) continue;
-            sum += sanitize(vin[q]).x;
-            n += 1.0f;
-        }
-        nf[F + col] = n > 0.0f ? sum / n : 0.0f;
-        return;
-    }
-    __shared__ float tile[NF_T][NF_T + 1];
-    const uint32_t row0 = (blockIdx.x - nb) * NF_T;
-    float sum = 0.0f, n = 0.0f;
-    for (uint32_t c0 = 0; c0 < F; c0 += NF_T) {
-        __syncthreads();
-        for (uint32_t r = 0; r < NF_T; ++r) {
-            const uint32_t row = row0 + r, col = c0 + t;
-            float val = __builtin_nanf("");  // marks a wall or out-of-range cell
-            if (row < F && col < F) {
-                const uint32_t q = row * F + col;
-                if (!solid(g, q)) val = sanitize(vin[q]).y;
+    const uint32_t ntile = (F + NF_K - 1) / NF_K;
+    auto stage = [&](uint32_t tile, int bsel) {
+        const uint32_t k0 = tile * NF_K;
+        for (uint32_t e = t; e < NF_C * NF_K; e += 256u) {
+            uint32_t c, k;
+            if (cols) { c = e % NF_C; k = e / NF_C; }   // 16 consecutive columns of one row
+            else { k = e % NF_K; c = e / NF_K; }        // 64 consecutive columns of one row
+            const uint32_t chain = c0 + c, pos = k0 + k;
+            float val = __builtin_nanf("");
+            if (chain < F && pos < F) {
+                const uint32_t q = cols ? pos * F + chain : chain * F + pos;
+                if (!solid(g, q)) {
+                    const float2 v = sanitize(vin[q]);
+                    val = cols ? v.x : v.y;
+                }
             }
-            tile[r][t] = val;
+            buf[bsel][c][k] = val;
+        }
+    };
+    float sum = 0.0f, n = 0.0f;
+    stage(0, 0);
+    __syncthreads();
+    for (uint32_t tile = 0; tile < ntile; ++tile) {
+        if (tile + 1 < ntile) stage(tile + 1, (tile + 1) & 1);
+        if (t < NF_C) {
+            const int bsel = tile & 1;
+            const uint32_t lim = rb_minu(NF_K, F - tile * NF_K);
+            for (uint32_t k = 0; k < lim; ++k) {
+                const float val = buf[bsel][t][k];
+                if (val != val) continue;
+                sum += val;
+                n += 1.0f;
+            }
         }
         __syncthreads();
-        const uint32_t lim = rb_minu(NF_T, F - c0);
-        for (uint32_t c = 0; c < lim; ++c) {
-            const float val = tile[t][c];
-            if (val != val) continue;
-            sum += val;
-            n += 1.0f;
-        }
     }
-    if (row0 + t < F) nf[row0 + t] = n > 0.0f ? sum / n : 0.0f;
+    if (t < NF_C && c0 + t < F) nf[(cols ? F : 0u) + c0 + t] = n > 0.0f ? sum / n : 0.0f;
 }
 
 // enforce_boundaries (fluid.wgsl:663-677)
@@ -1116,224 +1303,276 @@ __device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0)
     return f2(1.0f - rb_abs(2.0f * vx - 1.0f), 1.0f - rb_abs(2.0f * vy - 1.0f));
 }
 
-// transport_scatter (transport.wgsl:119-406).  One thread per grid cell; the
-// 16-bin destination histogram of each channel lives in LDS (column per
-// thread), the emit is one global atomic add per non-empty bin.
+// transport_scatter (transport.wgsl:119-406).  One thread per grid cell of
+// a 16 x 16 tile.  Destinations within SC_R cells of the tile (all of them
+// unless the flow moves more than ~2 cells per step) accumulate in LDS and
+// reach HBM as one coalesced atomic add per non-empty (cell, channel) at the
+// end; farther destinations add to HBM directly.  Integer sums do not depend
+// on order, so the result is the reference's exactly.  The blocked test of
+// each window cell (rock, or more monomers than its capacity) is staged
+// once per block.  Each thread's 16-bin destination histogram lives in LDS.
+constexpr int SC_T = 16, SC_R = 3, SC_W = SC_T + 2 * SC_R;
+__device__ __forceinline__ float sel4(int d, float a0, float a1, float a2, float a3) {
+    return d == 0 ? a0 : (d == 1 ? a1 : (d == 2 ? a2 : a3));
+}
 __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
     __shared__ uint32_t s_bins[16 * 256];
+    __shared__ uint32_t s_acc[SC_W * SC_W * 4];
+    __shared__ uint8_t s_tf[SC_W * SC_W];  // bit 0: rock, bit 1: over capacity
     if (ep) P.epoch += *ep;
     const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
-    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     const uint32_t G = g.G, F = g.F;
-    if (x >= G || y >= G) return;
+    const int wx0 = (int)(blockIdx.x * SC_T) - SC_R, wy0 = (int)(blockIdx.y * SC_T) - SC_R;
+    for (uint32_t k = tid; k < SC_W * SC_W * 4; k += 256) s_acc[k] = 0u;
+    for (uint32_t k = tid; k < SC_W * SC_W; k += 256) {
+        const int gx = wx0 + (int)(k % SC_W), gy = wy0 + (int)(k / SC_W);
+        uint8_t f = 0;
+        if (gx >= 0 && gy >= 0 && gx < (int)G && gy < (int)G) {
+            const uint32_t c = (uint32_t)gy * G + (uint32_t)gx;
+            const uint32_t gm = g.gamma[c];
+            if (gm >= GAMMA_SOLID) f = 1;
+            else if (cell_total(chem, c) > capacity(gm)) f = 2;
+        }
+        s_tf[k] = f;
+    }
+    __syncthreads();
+
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     const uint32_t idx = y * G + x;
-    const uint4 cw = *reinterpret_cast<const uint4*>(chem + 4u * idx);
-    const uint32_t cwv[4] = {cw.x, cw.y, cw.z, cw.w};
-    const uint32_t src_total = (cw.x & CHEM_STATE_MASK) + (cw.x >> 16) + (cw.y & CHEM_STATE_MASK) + (cw.y >> 16) +
-                               (cw.z & CHEM_STATE_MASK) + (cw.z >> 16) + (cw.w & CHEM_STATE_MASK) + (cw.w >> 16);
-    if (src_total == 0u) return;
-    const float cell_w = (float)g.wupc;
-    const uint32_t g_src = g.gamma[idx];
-    float2 v = f2(0.0f, 0.0f);
-    if (P.fluid_enabled != 0u && g_src < GAMMA_SOLID) {
-        const float2 c = f2(((float)x + 0.5f) * cell_w, ((float)y + 0.5f) * cell_w);
-        const float2 v0 = fluid_velocity_at_world(g, vel, c);
-        const float world_per_fluid = g.sim_size / (float)F;
-        const float k = rb_max(P.dt, 0.0f);
-        v = fluid_velocity_at_world(g, vel, f2(c.x + v0.x * world_per_fluid * k * 0.5f, c.y + v0.y * world_per_fluid * k * 0.5f));
+    uint32_t src_total = 0u;
+    uint4 cw = make_uint4(0u, 0u, 0u, 0u);
+    if (x < G && y < G) {
+        cw = *reinterpret_cast<const uint4*>(chem + 4u * idx);
+        src_total = (cw.x & CHEM_STATE_MASK) + (cw.x >> 16) + (cw.y & CHEM_STATE_MASK) + (cw.y >> 16) +
+                    (cw.z & CHEM_STATE_MASK) + (cw.z >> 16) + (cw.w & CHEM_STATE_MASK) + (cw.w >> 16);
     }
-    if (v.x != 0.0f || v.y != 0.0f) {
-        const bool rock_r = x + 1u < G && g.gamma[idx + 1u] >= GAMMA_SOLID;
-        const bool rock_l = x > 0u && g.gamma[idx - 1u] >= GAMMA_SOLID;
-        const bool rock_u = y + 1u < G && g.gamma[idx + G] >= GAMMA_SOLID;
-        const bool rock_d = y > 0u && g.gamma[idx - G] >= GAMMA_SOLID;
-        if ((v.x > 0.0f && rock_r) || (v.x < 0.0f && rock_l)) v.x = 0.0f;
-        else if ((v.x > 0.0f && rock_l) || (v.x < 0.0f && rock_r)) v.x *= 0.5f;
-        if ((v.y > 0.0f && rock_u) || (v.y < 0.0f && rock_d)) v.y = 0.0f;
-        else if ((v.y > 0.0f && rock_d) || (v.y < 0.0f && rock_u)) v.y *= 0.5f;
-    }
-    const float tn = (float)rb_maxu(P.transport_every, 1u);
-    const float ratio = (float)G / (float)F;
-    const float dtp = rb_max(P.dt, 0.0f);
-    const float2 disp = f2(v.x * ratio * dtp * tn, v.y * ratio * dtp * tn);
-    const float agitation = rb_clamp(len2(v) / DIFF_AGITATION_SPEED, 0.0f, 1.0f);
-    float p_diff = rb_clamp(DIFF_HOP_P * rb_mix(DIFF_HOP_FLOOR, 1.0f, agitation) * rb_max(P.diffusion, 0.0f) * tn, 0.0f, 0.5f);
-    if (g_src > 0u) p_diff = rb_min(DIFF_HOP_P * BURIED_DIFF_FACTOR / (1.0f + GAMMA_POROSITY_K * (float)g_src) * tn, 0.5f);
-    const uint32_t cap_self = capacity(g_src);
-    if (src_total > cap_self) p_diff = rb_min(CHEM_SQUEEZE_P * tn, 0.5f);
-    float p_settle = 0.0f;
-    if (g_src == 0u) p_settle = rb_min(MONOMER_SETTLE_P * rb_max(P.settle, 0.0f) * tn, 0.5f);
-    const int32_t base_x = rb_addi(rb_f2i(rb_floor(disp.x)), -1);
-    const int32_t base_y = rb_addi(rb_f2i(rb_floor(disp.y)), -1);
+    if (src_total != 0u) {
+        const float cell_w = (float)g.wupc;
+        const uint32_t g_src = g.gamma[idx];
+        float2 v = f2(0.0f, 0.0f);
+        if (P.fluid_enabled != 0u && g_src < GAMMA_SOLID) {
+            const float2 c = f2(((float)x + 0.5f) * cell_w, ((float)y + 0.5f) * cell_w);
+            const float2 v0 = fluid_velocity_at_world(g, vel, c);
+            const float world_per_fluid = g.sim_size / (float)F;
+            const float k = rb_max(P.dt, 0.0f);
+            v = fluid_velocity_at_world(g, vel, f2(c.x + v0.x * world_per_fluid * k * 0.5f, c.y + v0.y * world_per_fluid * k * 0.5f));
+        }
+        if (v.x != 0.0f || v.y != 0.0f) {
+            const bool rock_r = x + 1u < G && g.gamma[idx + 1u] >= GAMMA_SOLID;
+            const bool rock_l = x > 0u && g.gamma[idx - 1u] >= GAMMA_SOLID;
+            const bool rock_u = y + 1u < G && g.gamma[idx + G] >= GAMMA_SOLID;
+            const bool rock_d = y > 0u && g.gamma[idx - G] >= GAMMA_SOLID;
+            if ((v.x > 0.0f && rock_r) || (v.x < 0.0f && rock_l)) v.x = 0.0f;
+            else if ((v.x > 0.0f && rock_l) || (v.x < 0.0f && rock_r)) v.x *= 0.5f;
+            if ((v.y > 0.0f && rock_u) || (v.y < 0.0f && rock_d)) v.y = 0.0f;
+            else if ((v.y > 0.0f && rock_d) || (v.y < 0.0f && rock_u)) v.y *= 0.5f;
+        }
+        const float tn = (float)rb_maxu(P.transport_every, 1u);
+        const float ratio = (float)G / (float)F;
+        const float dtp = rb_max(P.dt, 0.0f);
+        const float2 disp = f2(v.x * ratio * dtp * tn, v.y * ratio * dtp * tn);
+        const float agitation = rb_clamp(len2(v) / DIFF_AGITATION_SPEED, 0.0f, 1.0f);
+        float p_diff = rb_clamp(DIFF_HOP_P * rb_mix(DIFF_HOP_FLOOR, 1.0f, agitation) * rb_max(P.diffusion, 0.0f) * tn, 0.0f, 0.5f);
+        if (g_src > 0u) p_diff = rb_min(DIFF_HOP_P * BURIED_DIFF_FACTOR / (1.0f + GAMMA_POROSITY_K * (float)g_src) * tn, 0.5f);
+        const uint32_t cap_self = capacity(g_src);
+        if (src_total > cap_self) p_diff = rb_min(CHEM_SQUEEZE_P * tn, 0.5f);
+        float p_settle = 0.0f;
+        if (g_src == 0u) p_settle = rb_min(MONOMER_SETTLE_P * rb_max(P.settle, 0.0f) * tn, 0.5f);
+        const int32_t base_x = rb_addi(rb_f2i(rb_floor(disp.x)), -1);
+        const int32_t base_y = rb_addi(rb_f2i(rb_floor(disp.y)), -1);
 
-    float press[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-    float max_drop = 0.0f;
-    if (P.monomer_pressure > 0.0f) {
-        const float f_self = (float)src_total / (float)rb_maxu(cap_self, 1u);
-        const int32_t nbo[4] = {1, -1, (int32_t)G, -(int32_t)G};
-        const bool inside[4] = {x + 1u < G, x > 0u, y + 1u < G, y > 0u};
-        for (int d = 0; d < 4; ++d) {
-            if (!inside[d]) continue;
-            const uint32_t n = (uint32_t)((int32_t)idx + nbo[d]);
-            const uint32_t cap_n = capacity(g.gamma[n]);
-            if (cap_n == 0u) continue;
-            const float f_n = (float)cell_total(chem, n) / (float)cap_n;
-            press[d] = rb_max(0.05f, 1.0f + P.monomer_pressure * (f_self - f_n));
-            max_drop = rb_max(max_drop, f_self - f_n);
+        float pr0 = 1.0f, pr1 = 1.0f, pr2 = 1.0f, pr3 = 1.0f;
+        float max_drop = 0.0f;
+        if (P.monomer_pressure > 0.0f) {  // transport.wgsl:191-206
+            const float f_self = (float)src_total / (float)rb_maxu(cap_self, 1u);
+            auto press_of = [&](bool inside, uint32_t n, float cur) {
+                if (!inside) return cur;
+                const uint32_t cap_n = capacity(g.gamma[n]);
+                if (cap_n == 0u) return cur;
+                const float f_n = (float)cell_total(chem, n) / (float)cap_n;
+                max_drop = rb_max(max_drop, f_self - f_n);
+                return rb_max(0.05f, 1.0f + P.monomer_pressure * (f_self - f_n));
+            };
+            pr0 = press_of(x + 1u < G, idx + 1u, pr0);
+            pr1 = press_of(x > 0u, idx - 1u, pr1);
+            pr2 = press_of(y + 1u < G, idx + G, pr2);
+            pr3 = press_of(y > 0u, idx - G, pr3);
         }
-    }
-    p_diff = rb_clamp(p_diff + PRESSURE_HOP_P * P.monomer_pressure * max_drop * tn, 0.0f, 0.5f);
-    const uint32_t lxi = rb_minu(x, G - 1u) / LIGHT_DIV, lyi = rb_minu(y, G - 1u) / LIGHT_DIV;
-    const float light_t = g.light[lyi * g.Lsz + lxi];
+        p_diff = rb_clamp(p_diff + PRESSURE_HOP_P * P.monomer_pressure * max_drop * tn, 0.0f, 0.5f);
+        const uint32_t lxi = rb_minu(x, G - 1u) / LIGHT_DIV, lyi = rb_minu(y, G - 1u) / LIGHT_DIV;
+        const float light_t = g.light[lyi * g.Lsz + lxi];
 
-    float p_bound = 0.0f;
-    float accept[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-    float accept_spent[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-    if (P.aggregation > 0.0f && src_total <= cap_self) {
-        float e[5] = {(float)agg_e[idx], 0.0f, 0.0f, 0.0f, 0.0f};
-        if (x + 1u < G) e[1] = (float)agg_e[idx + 1u];
-        if (x > 0u) e[2] = (float)agg_e[idx - 1u];
-        if (y + 1u < G) e[3] = (float)agg_e[idx + G];
-        if (y > 0u) e[4] = (float)agg_e[idx - G];
-        const uint32_t fx = rb_minu((x * F) / G, F - 1u), fy = rb_minu((y * F) / G, F - 1u);
-        const float t_rel = 1.0f + rb_max(temp[fy * F + fx], 0.0f) / TEMP_ACT_THRESHOLD;
-        const float room = rb_clamp(5.0f * (1.0f - (float)src_total / (float)rb_maxu(cap_self, 1u)), 0.0f, 1.0f);
-        const float k_e = P.aggregation * room / t_rel;
-        for (int d = 0; d < 4; ++d) {
-            const float drop = e[0] - e[d + 1];
-            if (drop > 0.0f) accept[d] = rb_expf(-k_e * drop);
-            const float rise = e[d + 1] - e[0];
-            if (rise > 0.0f) accept_spent[d] = rb_expf(-k_e * rise);
+        float p_bound = 0.0f;
+        float ac0 = 1.0f, ac1 = 1.0f, ac2 = 1.0f, ac3 = 1.0f;
+        float as0 = 1.0f, as1 = 1.0f, as2 = 1.0f, as3 = 1.0f;
+        if (P.aggregation > 0.0f && src_total <= cap_self) {  // transport.wgsl:228-260
+            const float e0 = (float)agg_e[idx];
+            const float e1 = x + 1u < G ? (float)agg_e[idx + 1u] : 0.0f;
+            const float e2 = x > 0u ? (float)agg_e[idx - 1u] : 0.0f;
+            const float e3 = y + 1u < G ? (float)agg_e[idx + G] : 0.0f;
+            const float e4 = y > 0u ? (float)agg_e[idx - G] : 0.0f;
+            const uint32_t fx = rb_minu((x * F) / G, F - 1u), fy = rb_minu((y * F) / G, F - 1u);
+            const float t_rel = 1.0f + rb_max(temp[fy * F + fx], 0.0f) / TEMP_ACT_THRESHOLD;
+            const float room = rb_clamp(5.0f * (1.0f - (float)src_total / (float)rb_maxu(cap_self, 1u)), 0.0f, 1.0f);
+            const float k_e = P.aggregation * room / t_rel;
+            auto acc_of = [&](float en, float& acc, float& accs) {
+                const float drop = e0 - en;
+                if (drop > 0.0f) acc = rb_expf(-k_e * drop);
+                const float rise = en - e0;
+                if (rise > 0.0f) accs = rb_expf(-k_e * rise);
+            };
+            acc_of(e1, ac0, as0);
+            acc_of(e2, ac1, as1);
+            acc_of(e3, ac2, as2);
+            acc_of(e4, ac3, as3);
+            p_bound = 1.0f - rb_expf(-k_e * rb_min(rb_max(e0 - 1.0f, 0.0f), 48.0f));
         }
-        p_bound = 1.0f - rb_expf(-k_e * rb_min(rb_max(e[0] - 1.0f, 0.0f), 48.0f));
-    }
-    const float dlen = len2(disp);
-    const bool jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
-    float2 bound_start = f2(0.0f, 0.0f);
-    bool have_bound_start = false;
-    uint32_t* bins = s_bins + tid;
+        const float dlen = len2(disp);
+        const bool jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
+        float2 bound_start = f2(0.0f, 0.0f);
+        bool have_bound_start = false;
+        uint32_t* bins = s_bins + tid;
+        const int lself = ((int)y - wy0) * SC_W + ((int)x - wx0);
 
-    for (uint32_t ch = 0; ch < 4; ++ch) {
-        const uint32_t slot = idx * 4u + ch;
-        const uint32_t v_ch = cwv[ch];
-        uint32_t act_n = v_ch & CHEM_STATE_MASK;
-        uint32_t spent_n = v_ch >> 16;
-        if (act_n + spent_n == 0u) continue;
-        if (spent_n > 0u) {
-            const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
-            const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
-                                  (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
-                                   rb_max(P.reactivation_rate, 0.0f));
-            uint32_t na = rb_f2u(rb_floor(exp_act));
-            const float frac = exp_act - (float)na;
-            if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
-            na = rb_minu(na, rb_minu(spent_n, 8u));
-            act_n += na;
-            spent_n -= na;
-        }
-        if (act_n > 0u) {
-            const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
-            const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
-            if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
-                act_n -= 1u;
-                spent_n += 1u;
+        for (uint32_t ch = 0; ch < 4; ++ch) {
+            const uint32_t slot = idx * 4u + ch;
+            const uint32_t v_ch = ch == 0 ? cw.x : (ch == 1 ? cw.y : (ch == 2 ? cw.z : cw.w));
+            uint32_t act_n = v_ch & CHEM_STATE_MASK;
+            uint32_t spent_n = v_ch >> 16;
+            if (act_n + spent_n == 0u) continue;
+            if (spent_n > 0u) {
+                const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
+                const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
+                                      (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
+                                       rb_max(P.reactivation_rate, 0.0f));
+                uint32_t na = rb_f2u(rb_floor(exp_act));
+                const float frac = exp_act - (float)na;
+                if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
+                na = rb_minu(na, rb_minu(spent_n, 8u));
+                act_n += na;
+                spent_n -= na;
             }
-        }
-        float coh[4] = {1.0f, 1.0f, 1.0f, 1.0f};
-        if (act_n > 0u && P.cohesion > 0.0f) {
-            auto ac = [&](uint32_t cell) { return rb_minu(chem[cell * 4u + ch] & CHEM_STATE_MASK, 8u); };
-            if (x + 1u < G) coh[0] = 1.0f + P.cohesion * (float)ac(idx + 1u);
-            if (x > 0u) coh[1] = 1.0f + P.cohesion * (float)ac(idx - 1u);
-            if (y + 1u < G) coh[2] = 1.0f + P.cohesion * (float)ac(idx + G);
-            if (y > 0u) coh[3] = 1.0f + P.cohesion * (float)ac(idx - G);
-        }
-        uint32_t used = 0u;  // bit b set when bins[b] is non-zero
-        uint32_t stay = 0u;
-        const uint32_t n = act_n + spent_n;
-        const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
-        const uint32_t act_stay = act_n - rb_minu(act_n, movers);
-        stay += act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
-        const float wa[4] = {press[0] * coh[0], press[1] * coh[1], press[2] * coh[2], press[3] * coh[3]};
-        PcgRun mv(slot, P.epoch, S_MOVE, P.seed);
-        PcgRun dsp(slot, P.epoch, S_DISPERSE, P.seed);
-        for (uint32_t k = 0; k < movers; ++k, mv.advance(), dsp.advance()) {
-            const bool is_act = k < act_n;
-            const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
-            const F4 rf = u4_to_f4(mv.at_current());
-            const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
-            if (bound && !have_bound_start) {
-                bound_start = smooth_start(P, x, y);
-                have_bound_start = true;
-            }
-            float px = bound ? bound_start.x : rf.x;
-            float py = bound ? bound_start.y : rf.y;
-            px += disp.x;
-            py += disp.y;
-            if (jitter_on) {
-                const F4 rj = u4_to_f4(dsp.at_current());
-                const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
-                const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
-                px += rb_clamp(jx, -0.9f, 0.9f);
-                py += rb_clamp(jy, -0.9f, 0.9f);
-            }
-            const float r = rf.z;
-            if (r < p_diff) {
-                const float w0 = is_act ? wa[0] : press[0], w1 = is_act ? wa[1] : press[1];
-                const float w2 = is_act ? wa[2] : press[2], w3 = is_act ? wa[3] : press[3];
-                float u = rf.w * (w0 + w1 + w2 + w3);
-                uint32_t d;
-                if (u < w0) d = 0u;
-                else if ((u -= w0) < w1) d = 1u;
-                else if ((u -= w1) < w2) d = 2u;
-                else d = 3u;
-                const float acc_d = is_act ? accept[d] : accept_spent[d];
-                const bool stay_put = rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d;
-                if (!stay_put) {
-                    if (d == 0u) px += 1.0f;
-                    else if (d == 1u) px -= 1.0f;
-                    else if (d == 2u) py += 1.0f;
-                    else py -= 1.0f;
+            if (act_n > 0u) {
+                const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
+                const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
+                if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
+                    act_n -= 1u;
+                    spent_n += 1u;
                 }
-            } else if (r < p_diff + p_settle) {
-                py -= 1.0f;
             }
-            const int32_t ox = rb_addi(rb_f2i(rb_floor(px)), -base_x);
-            const int32_t oy = rb_addi(rb_f2i(rb_floor(py)), -base_y);
-            const uint32_t bi = (uint32_t)rb_clampi(oy, 0, 3) * 4u + (uint32_t)rb_clampi(ox, 0, 3);
-            bins[bi * 256u] = ((used >> bi) & 1u) ? bins[bi * 256u] + unit : unit;
-            used |= 1u << bi;
-        }
-        while (used) {
-            const uint32_t b = (uint32_t)__builtin_ctz(used);
-            used &= used - 1u;
-            const uint32_t cnt = bins[b * 256u];
-            const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
-            const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
-            const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
-            bool blocked = false;
-            if (t_idx != idx) {
-                const uint32_t g_tgt = g.gamma[t_idx];
-                if (g_tgt >= GAMMA_SOLID) blocked = g_src < GAMMA_SOLID;
-                else if (cell_total(chem, t_idx) > capacity(g_tgt)) blocked = true;
+            float ch0 = 1.0f, ch1 = 1.0f, ch2 = 1.0f, ch3 = 1.0f;
+            if (act_n > 0u && P.cohesion > 0.0f) {
+                auto ac = [&](uint32_t cell) { return (float)rb_minu(chem[cell * 4u + ch] & CHEM_STATE_MASK, 8u); };
+                if (x + 1u < G) ch0 = 1.0f + P.cohesion * ac(idx + 1u);
+                if (x > 0u) ch1 = 1.0f + P.cohesion * ac(idx - 1u);
+                if (y + 1u < G) ch2 = 1.0f + P.cohesion * ac(idx + G);
+                if (y > 0u) ch3 = 1.0f + P.cohesion * ac(idx - G);
             }
-            if (blocked) stay += cnt;
-            else atomicAdd(next + t_idx * 4u + ch, cnt);
+            for (uint32_t b = 0; b < 16; ++b) bins[b * 256u] = 0u;
+            uint32_t used = 0u;
+            uint32_t stay = 0u;
+            const uint32_t n = act_n + spent_n;
+            const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
+            const uint32_t act_stay = act_n - rb_minu(act_n, movers);
+            stay += act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
+            const float wa0 = pr0 * ch0, wa1 = pr1 * ch1, wa2 = pr2 * ch2, wa3 = pr3 * ch3;
+            PcgRun mv(slot, P.epoch, S_MOVE, P.seed);
+            PcgRun dsp(slot, P.epoch, S_DISPERSE, P.seed);
+            for (uint32_t k = 0; k < movers; ++k, mv.advance(), dsp.advance()) {
+                const bool is_act = k < act_n;
+                const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
+                const F4 rf = u4_to_f4(mv.at_current());
+                const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
+                if (bound && !have_bound_start) {
+                    bound_start = smooth_start(P, x, y);
+                    have_bound_start = true;
+                }
+                float px = bound ? bound_start.x : rf.x;
+                float py = bound ? bound_start.y : rf.y;
+                px += disp.x;
+                py += disp.y;
+                if (jitter_on) {
+                    const F4 rj = u4_to_f4(dsp.at_current());
+                    const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                    const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                    px += rb_clamp(jx, -0.9f, 0.9f);
+                    py += rb_clamp(jy, -0.9f, 0.9f);
+                }
+                const float r = rf.z;
+                if (r < p_diff) {
+                    const float w0 = is_act ? wa0 : pr0, w1 = is_act ? wa1 : pr1;
+                    const float w2 = is_act ? wa2 : pr2, w3 = is_act ? wa3 : pr3;
+                    float u = rf.w * (w0 + w1 + w2 + w3);
+                    int d;
+                    if (u < w0) d = 0;
+                    else if ((u -= w0) < w1) d = 1;
+                    else if ((u -= w1) < w2) d = 2;
+                    else d = 3;
+                    const float acc_d = is_act ? sel4(d, ac0, ac1, ac2, ac3) : sel4(d, as0, as1, as2, as3);
+                    const bool stay_put = rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d;
+                    if (!stay_put) {
+                        if (d == 0) px += 1.0f;
+                        else if (d == 1) px -= 1.0f;
+                        else if (d == 2) py += 1.0f;
+                        else py -= 1.0f;
+                    }
+                } else if (r < p_diff + p_settle) {
+                    py -= 1.0f;
+                }
+                const int32_t ox = rb_addi(rb_f2i(rb_floor(px)), -base_x);
+                const int32_t oy = rb_addi(rb_f2i(rb_floor(py)), -base_y);
+                const uint32_t bi = (uint32_t)rb_clampi(oy, 0, 3) * 4u + (uint32_t)rb_clampi(ox, 0, 3);
+                atomicAdd(bins + bi * 256u, unit);
+                used |= 1u << bi;
+            }
+            while (used) {
+                const uint32_t b = (uint32_t)__builtin_ctz(used);
+                used &= used - 1u;
+                const uint32_t cnt = bins[b * 256u];
+                const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
+                const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
+                const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
+                const int lx = tx - wx0, ly = ty - wy0;
+                const bool in_win = lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W;
+                bool blocked = false;
+                if (t_idx != idx) {
+                    uint32_t tf;
+                    if (in_win) {
+                        tf = s_tf[ly * SC_W + lx];
+                    } else {
+                        const uint32_t g_tgt = g.gamma[t_idx];
+                        tf = g_tgt >= GAMMA_SOLID ? 1u : (cell_total(chem, t_idx) > capacity(g_tgt) ? 2u : 0u);
+                    }
+                    blocked = (tf == 1u) ? (g_src < GAMMA_SOLID) : (tf == 2u);
+                }
+                if (blocked) stay += cnt;
+                else if (in_win) atomicAdd(s_acc + (ly * SC_W + lx) * 4 + ch, cnt);
+                else atomicAdd(next + t_idx * 4u + ch, cnt);
+            }
+            if (stay > 0u) atomicAdd(s_acc + lself * 4 + ch, stay);
         }
-        if (stay > 0u) atomicAdd(next + slot, stay);
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < SC_W * SC_W * 4; k += 256) {
+        const uint32_t v = s_acc[k];
+        if (v == 0u) continue;
+        const int gx = wx0 + (int)((k / 4) % SC_W), gy = wy0 + (int)((k / 4) / SC_W);
+        atomicAdd(next + ((uint32_t)gy * G + (uint32_t)gx) * 4u + (k & 3u), v);
     }
 }
 
-// ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state.
-// Wave reductions, one LDS slot per wave, one global atomic per counter per
-// block; the last block publishes the totals, optionally checks the
-// per-channel act + spent totals against the armed values, and clears the
-// accumulators for the next use.
+// ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state
+// (wrapping, as the reference's atomics).  Wave reductions, one partial per
+// block in HBM, and the last block to finish sums the partials; it
+// publishes the totals, optionally checks the per-channel act + spent
+// totals against the armed values, and resets the block counter.
 __global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led,
                                                 uint32_t epoch, const uint32_t* ep, int check) {
     if (ep) epoch += *ep;
     __shared__ uint32_t sh[4][8];
+    __shared__ bool last;
     uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
         const uint4 v = *reinterpret_cast<const uint4*>(chem + 4u * i);
@@ -1348,33 +1587,36 @@ __global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ che
     if (lane == 0)
         for (int k = 0; k < 8; ++k) sh[wid][k] = a[k];
     __syncthreads();
-    if (threadIdx.x < 8) {
-        uint32_t s = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
-        atomicAdd(&led->acc[threadIdx.x], s);
-    }
+    if (threadIdx.x < 8) led->part[blockIdx.x * 8 + threadIdx.x] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
-        const uint32_t d = atomicAdd(&led->blocks_done, 1u);
-        if (d + 1u == gridDim.x) {
-            __threadfence();
-            uint32_t tot[8];
-            for (int k = 0; k < 8; ++k) {
-                tot[k] = atomicExch(&led->acc[k], 0u);
-                led->last[k] = tot[k];
+        last = atomicAdd(&led->blocks_done, 1u) + 1u == gridDim.x;
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    uint32_t tot = 0u;
+    if (threadIdx.x < 8)
+        for (uint32_t b = 0; b < gridDim.x; ++b)
+            tot += __hip_atomic_load(led->part + b * 8 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 8) {
+        led->last[threadIdx.x] = tot;
+        sh[0][threadIdx.x] = tot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        if (check) {
+            if (!led->armed) {
+                for (int c = 0; c < 4; ++c) led->expected[c] = sh[0][c] + sh[0][4 + c];
+                led->armed = 1u;
+            } else {
+                bool ok = true;
+                for (int c = 0; c < 4; ++c) ok = ok && (led->expected[c] == sh[0][c] + sh[0][4 + c]);
+                if (!ok) atomicMin(&led->first_bad, epoch);
             }
-            if (check) {
-                if (!led->armed) {
-                    for (int c = 0; c < 4; ++c) led->expected[c] = tot[c] + tot[4 + c];
-                    led->armed = 1u;
-                } else {
-                    bool ok = true;
-                    for (int c = 0; c < 4; ++c) ok = ok && (led->expected[c] == tot[c] + tot[4 + c]);
-                    if (!ok) atomicMin(&led->first_bad, epoch);
-                }
-            }
-            led->blocks_done = 0u;
         }
+        led->blocks_done = 0u;
     }
 }
 
@@ -1438,21 +1680,27 @@ void launch_mg_restrict(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C,
 void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
     k_mg_prolong<<<grid2(L.n), B16, 0, s>>>(L, C, done);
 }
-void launch_mg_down(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
+template <class S>
+void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done) {
     const uint32_t nt = (L.n + MG_T - 1) / MG_T;
-    k_mg_down<<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    if (sym) k_mg_down<S, true><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    else k_mg_down<S, false><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
 }
-void launch_mg_up(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, const uint32_t* done) {
+template <class S>
+void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done) {
     const uint32_t nt = (L.n + MG_T - 1) / MG_T;
-    k_mg_up<<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    if (sym) k_mg_up<S, true><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    else k_mg_up<S, false><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
 }
-size_t mg_block_lds_bytes(const MgLevelDev* lv, int count) {
+size_t mg_block_lds_bytes(const uint32_t* n, int count) {
     size_t cells = 0;
-    for (int i = 0; i < count; ++i) cells += (size_t)lv[i].n * lv[i].n;
+    for (int i = 0; i < count; ++i) cells += (size_t)n[i] * n[i];
     return cells * 9;
 }
-void launch_mg_block(hipStream_t s, const MgLevelDev* lv, int count, const uint32_t* done) {
-    MgBlockArgs a{};
+template <class S>
+void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
+                     int cm, const uint32_t* done) {
+    MgBlockArgs<S> a{};
     a.levels = count;
     int off = 0;
     for (int i = 0; i < count; ++i) {
@@ -1465,24 +1713,47 @@ void launch_mg_block(hipStream_t s, const MgLevelDev* lv, int count, const uint3
     a.off[count] = off;
     a.p0 = lv[0].p;
     a.rhs0 = lv[0].rhs;
+    a.cinv = cinv;
+    a.ccell = ccell;
+    a.cm = cm;
     const size_t lds = (size_t)off * 9;
-    k_mg_block<<<1, 1024, lds, s>>>(a, done);
+    if (sym) k_mg_block<S, true><<<1, 1024, lds, s>>>(a, done);
+    else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }
-void launch_mg_rhs_norm(hipStream_t s, const MgLevelDev& L0, Counters* c) {
-    k_counters_reset<<<1, 1, 0, s>>>(c);
-    uint32_t nb = rb_minu(blocks1((size_t)L0.n * L0.n, 256), 1024u);
-    k_mg_rhs_norm<<<nb, 256, 0, s>>>(L0, c);
+template <class S> void launch_pcg_init(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
+    k_pcg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st, tol, maxit);
 }
-void launch_mg_residual(hipStream_t s, const MgLevelDev& L0, Counters* c, float tol, uint32_t max_cycles, int count_cycle) {
-    uint32_t nb = rb_minu(blocks1((size_t)L0.n * L0.n, 256), 1024u);
-    k_mg_residual<<<nb, 256, 0, s>>>(L0, c, tol, max_cycles, count_cycle);
+template <class S> void launch_pcg_rho(hipStream_t s, const PcgVec<S>& V, PcgState* st) {
+    k_pcg_rho<S><<<grid2(V.n), B16, 0, s>>>(V, st);
 }
+template <class S> void launch_pcg_dir(hipStream_t s, const PcgVec<S>& V, PcgState* st) {
+    k_pcg_dir<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+}
+template <class S> void launch_pcg_update(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
+    k_pcg_update<S><<<grid2(V.n), B16, 0, s>>>(V, st, tol, maxit);
+}
+template <class S> void launch_pcg_final(hipStream_t s, const PcgVec<S>& V, PcgState* st) {
+    k_pcg_final<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+}
+#define RIBO_INSTANTIATE(S)                                                                                         \
+    template void launch_mg_down<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);   \
+    template void launch_mg_up<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);     \
+    template void launch_mg_block<S>(hipStream_t, const MgLevelT<S>*, int, bool, const double*, const int32_t*, int, \
+                                     const uint32_t*);                                                              \
+    template void launch_pcg_init<S>(hipStream_t, const PcgVec<S>&, PcgState*, float, uint32_t);                   \
+    template void launch_pcg_rho<S>(hipStream_t, const PcgVec<S>&, PcgState*);                                     \
+    template void launch_pcg_dir<S>(hipStream_t, const PcgVec<S>&, PcgState*);                                     \
+    template void launch_pcg_update<S>(hipStream_t, const PcgVec<S>&, PcgState*, float, uint32_t);                 \
+    template void launch_pcg_final<S>(hipStream_t, const PcgVec<S>&, PcgState*);
+RIBO_INSTANTIATE(float)
+RIBO_INSTANTIATE(bf16)
+#undef RIBO_INSTANTIATE
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p) {
     k_subgrad<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, p);
 }
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf) {
-    uint32_t nb = (g.F + NF_T - 1) / NF_T;
-    k_net_flux<<<2 * nb, NF_T, 0, s>>>(g, vin, nf);
+    uint32_t nb = (g.F + NF_C - 1) / NF_C;
+    k_net_flux<<<2 * nb, 256, 0, s>>>(g, vin, nf);
 }
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf) {
     k_boundaries<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, nf);
@@ -1504,7 +1775,7 @@ void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check) {
     uint32_t cells = g.G * g.G;
-    uint32_t nb = rb_minu(blocks1(cells, 256), 2048u);
+    uint32_t nb = rb_minu(blocks1(cells, 256), LEDGER_MAX_BLOCKS);
     k_ledger<<<nb, 256, 0, s>>>(chem, cells, led, epoch, ep, check);
 }
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n) {

@@ -130,7 +130,26 @@ struct RiboWorld {
     uint8_t* mg_fluid = nullptr;
     size_t mg_block_start = 0;  // first level solved inside one workgroup
     size_t lds_limit = 65536;
-    Counters* counters = nullptr;
+    // tolerance mode (preconditioned conjugate gradients)
+    PcgState* pcg = nullptr;
+    double* pcg_part = nullptr;
+    float* pcg_r = nullptr;
+    float* pcg_z = nullptr;
+    float* pcg_d = nullptr;
+    float* pcg_q = nullptr;
+    bf16* pcg_r16 = nullptr;
+    bf16* pcg_z16 = nullptr;
+    bf16* mg_p16 = nullptr;     // coarse levels of the bfloat16 preconditioner
+    bf16* mg_rhs16 = nullptr;
+    std::vector<MgLevelT<float>> mgz;   // preconditioner levels, binary32 (level 0 = z, r)
+    std::vector<MgLevelT<bf16>> mgh;    // preconditioner levels, bfloat16 (level 0 = z16, r16)
+    double* cinv = nullptr;             // inverse of the coarsest operator (binary64)
+    int32_t* ccell = nullptr;
+    int cm = 0;
+    bool cinv_valid = false;
+    uint32_t* pcg_host = nullptr;   // host-mapped: last iterations, misses, solves
+    uint32_t pcg_slots = 0;         // iteration slots recorded per solve (0: not set)
+    uint32_t missed_seen = 0;
     LedgerDev* ledger = nullptr;
     uint32_t* scratch_u32 = nullptr;  // F words for downloads of derived masks
     // graph replay: the epoch base lives on the device; one executable graph
@@ -138,7 +157,7 @@ struct RiboWorld {
     uint32_t* epoch_dev = nullptr;
     uint32_t epoch_dev_value = 0;
     bool epoch_dev_known = false;
-    struct GraphEntry { uint32_t phase, n; int cur, tcur, cur_end, tcur_end; hipGraphExec_t exec; };
+    struct GraphEntry { uint32_t phase, n, slots; int cur, tcur, cur_end, tcur_end; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
     RiboSimParams graph_params{};
     RiboWorldSettings graph_settings{};
@@ -253,26 +272,49 @@ int build_mg(RiboWorld* w) {
     size_t coarse = total - fine;
     int rc;
     if ((rc = dalloc(&w->mg_p, coarse)) || (rc = dalloc(&w->mg_rhs, coarse)) || (rc = dalloc(&w->mg_fluid, total))) return rc;
+    dfree(w->mg_p16);
+    dfree(w->mg_rhs16);
+    if ((rc = dalloc(&w->mg_p16, coarse)) || (rc = dalloc(&w->mg_rhs16, coarse))) return rc;
     w->mg.clear();
+    w->mgz.clear();
+    w->mgh.clear();
     for (size_t l = 0; l < w->levels.size(); ++l) {
         MgLevelDev d;
         d.n = w->levels[l].n;
         d.eps = w->levels[l].eps;
+        MgLevelT<bf16> h;
+        h.n = d.n;
+        h.eps = d.eps;
         if (l == 0) {
             d.p = w->p_a;
             d.rhs = w->div;
+            h.p = w->pcg_z16;
+            h.rhs = w->pcg_r16;
         } else {
             d.p = w->mg_p + (w->levels[l].off - fine);
             d.rhs = w->mg_rhs + (w->levels[l].off - fine);
+            h.p = w->mg_p16 + (w->levels[l].off - fine);
+            h.rhs = w->mg_rhs16 + (w->levels[l].off - fine);
         }
         d.fluid = w->mg_fluid + w->levels[l].off;
+        h.fluid = d.fluid;
         w->mg.push_back(d);
+        MgLevelDev z = d;
+        if (l == 0) {
+            z.p = w->pcg_z;
+            z.rhs = w->pcg_r;
+        }
+        w->mgz.push_back(z);
+        w->mgh.push_back(h);
     }
+    w->cinv_valid = false;
     // the longest suffix of levels (at most 8) whose unknowns, right-hand
     // sides and masks fit in one workgroup's LDS together
+    std::vector<uint32_t> sizes;
+    for (auto& L : w->levels) sizes.push_back(L.n);
     size_t b = w->mg.size();
     while (b > 0 && w->mg.size() - (b - 1) <= 8 &&
-           mg_block_lds_bytes(w->mg.data() + (b - 1), (int)(w->mg.size() - (b - 1))) <= w->lds_limit)
+           mg_block_lds_bytes(sizes.data() + (b - 1), (int)(w->mg.size() - (b - 1))) <= w->lds_limit)
         --b;
     w->mg_block_start = b;
     return RIBO_OK;
@@ -288,6 +330,62 @@ int rebuild_terrain(RiboWorld* w) {
     for (size_t l = 1; l < w->levels.size(); ++l)
         launch_mg_mask_coarsen(w->stream, w->levels[l].n, w->mg_fluid + w->levels[l - 1].off, w->mg_fluid + w->levels[l].off);
     HIPCHK(hipGetLastError());
+    w->cinv_valid = false;
+    return RIBO_OK;
+}
+
+// Inverse of the coarsest level's eps-anchored operator over its fluid
+// cells, in binary64 (Gauss-Jordan with partial pivoting on the host), for
+// the exact coarse solve of the tolerance mode.  Skipped above 1024 unknowns.
+int build_coarse_inverse(RiboWorld* w) {
+    const RiboMgLevel& C = w->levels.back();
+    const uint32_t n = C.n;
+    std::vector<uint8_t> fl((size_t)n * n);
+    HIPCHK(hipMemcpyAsync(fl.data(), w->mg_fluid + C.off, fl.size(), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    std::vector<int32_t> idx((size_t)n * n, -1), cell;
+    for (uint32_t i = 0; i < n * n; ++i)
+        if (fl[i]) { idx[i] = (int32_t)cell.size(); cell.push_back((int32_t)i); }
+    const size_t m = cell.size();
+    dfree(w->cinv);
+    dfree(w->ccell);
+    w->cm = 0;
+    if (m == 0 || m > 1024) {
+        w->cinv_valid = true;
+        return RIBO_OK;
+    }
+    std::vector<double> a(m * m, 0.0), inv(m * m, 0.0);
+    for (size_t r = 0; r < m; ++r) {
+        const uint32_t c = (uint32_t)cell[r], x = c % n, y = c / n;
+        int k = 0;
+        const int32_t nb[4] = {x > 0 ? idx[c - 1] : -1, x + 1 < n ? idx[c + 1] : -1, y > 0 ? idx[c - n] : -1,
+                               y + 1 < n ? idx[c + n] : -1};
+        for (int d = 0; d < 4; ++d)
+            if (nb[d] >= 0) { a[r * m + (size_t)nb[d]] = 1.0; ++k; }
+        a[r * m + r] = -((double)k + (double)C.eps);
+        inv[r * m + r] = 1.0;
+    }
+    for (size_t c = 0; c < m; ++c) {
+        size_t piv = c;
+        for (size_t r = c + 1; r < m; ++r)
+            if (fabs(a[r * m + c]) > fabs(a[piv * m + c])) piv = r;
+        if (piv != c)
+            for (size_t j = 0; j < m; ++j) { std::swap(a[c * m + j], a[piv * m + j]); std::swap(inv[c * m + j], inv[piv * m + j]); }
+        const double dg = a[c * m + c];
+        for (size_t j = 0; j < m; ++j) { a[c * m + j] /= dg; inv[c * m + j] /= dg; }
+        for (size_t r = 0; r < m; ++r) {
+            if (r == c) continue;
+            const double f = a[r * m + c];
+            if (f == 0.0) continue;
+            for (size_t j = 0; j < m; ++j) { a[r * m + j] -= f * a[c * m + j]; inv[r * m + j] -= f * inv[c * m + j]; }
+        }
+    }
+    int rc;
+    if ((rc = dalloc(&w->cinv, m * m)) || (rc = dalloc(&w->ccell, m))) return rc;
+    HIPCHK(hipMemcpy(w->cinv, inv.data(), m * m * sizeof(double), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(w->ccell, cell.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
+    w->cm = (int)m;
+    w->cinv_valid = true;
     return RIBO_OK;
 }
 
@@ -299,18 +397,68 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
     hipStream_t s = w->stream;
     const size_t b = w->mg_block_start;
     if (b < L) {
-        for (size_t l = 0; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down(s, w->mg[l], w->mg[l + 1], done); });
-        run(w, "mg_block", [&] { launch_mg_block(s, w->mg.data() + b, (int)(L - b), done); });
-        for (size_t l = b; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up(s, w->mg[l], w->mg[l + 1], done); });
+        for (size_t l = 0; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<float>(s, w->mg[l], w->mg[l + 1], false, done); });
+        run(w, "mg_block", [&] { launch_mg_block<float>(s, w->mg.data() + b, (int)(L - b), false, nullptr, nullptr, 0, done); });
+        for (size_t l = b; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done); });
         return;
     }
     // coarsest level too large for one workgroup: per-operation dispatches
-    for (size_t l = 0; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down(s, w->mg[l], w->mg[l + 1], done); });
+    for (size_t l = 0; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<float>(s, w->mg[l], w->mg[l + 1], false, done); });
     for (uint32_t k = 0; k < MG_COARSE; ++k) {
         run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 0, done); });
         run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 1, done); });
     }
-    for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up(s, w->mg[l], w->mg[l + 1], done); });
+    for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done); });
+}
+
+// The symmetric V-cycle of the tolerance mode on the preconditioner levels
+// (storage S), with the exact coarsest solve when its inverse exists.
+template <class S> void vcycle_sym(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done) {
+    const size_t L = lv.size();
+    hipStream_t s = w->stream;
+    const size_t b = w->mg_block_start;
+    if (b < L) {
+        for (size_t l = 0; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+        run(w, "mg_block", [&] { launch_mg_block<S>(s, lv.data() + b, (int)(L - b), true, w->cinv, w->ccell, w->cm, done); });
+        for (size_t l = b; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
+        return;
+    }
+    // coarsest level too large for one workgroup: the tolerance mode needs
+    // the block kernel for its symmetric coarse solve
+    for (size_t l = 0; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+    for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
+}
+
+// Tolerance mode: preconditioned conjugate gradients on A p = div over the
+// fluid cells of level 0, warm-started from the pressure, preconditioned by
+// the symmetric V-cycle stored in S (binary32, or bfloat16 for the mixed
+// precision form); stops on ||div - A p|| / ||div|| <= mg_tol or after
+// mg_max_cycles iterations.  Every iteration's kernels return at once after
+// convergence, so the recorded work does not depend on host round trips.
+template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, S* rs, S* z, uint32_t slots) {
+    hipStream_t s = w->stream;
+    PcgVec<S> V;
+    V.n = w->F;
+    V.eps = w->levels[0].eps;
+    V.fl = w->mg_fluid;
+    V.p = w->p_a;
+    V.b = w->div;
+    V.r = w->pcg_r;
+    V.rs = rs;
+    V.z = z;
+    V.d = w->pcg_d;
+    V.q = w->pcg_q;
+    const float tol = w->st.mg_tol;
+    const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
+    const uint32_t* done = &w->pcg->done;
+    run(w, "pcg_init", [&] { launch_pcg_init<S>(s, V, w->pcg, tol, maxit); });
+    for (uint32_t it = 0; it < std::min(slots, maxit); ++it) {
+        vcycle_sym<S>(w, lv, done);
+        run(w, "pcg_rho", [&] { launch_pcg_rho<S>(s, V, w->pcg); });
+        run(w, "pcg_dir", [&] { launch_pcg_dir<S>(s, V, w->pcg); });
+        run(w, "pcg_update", [&] { launch_pcg_update<S>(s, V, w->pcg, tol, maxit); });
+    }
+    run(w, "pcg_final", [&] { launch_pcg_final<S>(s, V, w->pcg); });
 }
 
 void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588
@@ -333,14 +481,8 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     run(w, "divergence", [&] { launch_divergence(s, g, w->vel_a, w->div, w->p_a); });
     if (w->st.multigrid) {
         if (w->st.mg_tol > 0.0f) {
-            const uint32_t maxc = std::max(w->st.mg_max_cycles, 1u);
-            uint32_t* done = &w->counters->done;
-            run(w, "mg_residual", [&] { launch_mg_rhs_norm(s, w->mg[0], w->counters); });
-            run(w, "mg_residual", [&] { launch_mg_residual(s, w->mg[0], w->counters, w->st.mg_tol, maxc, 0); });
-            for (uint32_t c = 0; c < maxc; ++c) {
-                vcycle(w, done);
-                run(w, "mg_residual", [&] { launch_mg_residual(s, w->mg[0], w->counters, w->st.mg_tol, maxc, 1); });
-            }
+            if (w->st.precision == RIBO_PRECISION_MIXED) pcg_solve<bf16>(w, w->mgh, w->pcg_r16, w->pcg_z16, w->pcg_slots);
+            else pcg_solve<float>(w, w->mgz, nullptr, w->pcg_z, w->pcg_slots);
         } else {
             const uint32_t cycles = std::max(w->st.mg_cycles, 1u);
             for (uint32_t c = 0; c < cycles; ++c) vcycle(w, nullptr);
@@ -392,11 +534,25 @@ bool graph_eligible(const RiboWorld* w) {
     return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !(w->params.sun_heat > 0.0f);
 }
 
+// Arms the per-step ledger check with the totals of the current grid (when
+// checking is switched on, and again after every upload of the grid: a
+// host-side change of matter is not a transport error).
+int arm_ledger(RiboWorld* w) {
+    LedgerDev init{};
+    init.first_bad = 0xFFFFFFFFu;
+    HIPCHK(hipMemcpyAsync(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice, w->stream));
+    launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, nullptr, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return RIBO_OK;
+}
+
 int validate_settings(const RiboWorldSettings* s) {
     if (s->terrain_enabled) return fail(RIBO_ERR_UNSUPPORTED, "terrain physics is not part of the world-step path (terrain_enabled must be 0)");
     if (s->precision != RIBO_PRECISION_FP32 && s->precision != RIBO_PRECISION_MIXED)
         return fail(RIBO_ERR_ARG, "unknown precision");
-    if (s->precision == RIBO_PRECISION_MIXED) return fail(RIBO_ERR_UNSUPPORTED, "mixed precision multigrid is not built yet");
+    if (s->precision == RIBO_PRECISION_MIXED && !(s->mg_tol > 0.0f))
+        return fail(RIBO_ERR_ARG, "the mixed-precision form is a tolerance-mode solver: set mg_tol > 0");
     if (!(s->mg_tol >= 0.0f)) return fail(RIBO_ERR_ARG, "mg_tol must be >= 0");
     return RIBO_OK;
 }
@@ -497,7 +653,10 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(&w->redox[0], F)) || (rc = dalloc(&w->redox[1], F)) || (rc = dalloc(&w->eaten, F)) ||
         (rc = dalloc(&w->forces, F)) || (rc = dalloc(&w->heat, F)) || (rc = dalloc(&w->net_flux, 2 * (size_t)w->F)) ||
         (rc = dalloc(&w->cflags, F)) || (rc = dalloc(&w->normal, F)) || (rc = dalloc(&w->fill, F)) ||
-        (rc = dalloc(&w->dragg, F)) || (rc = dalloc(&w->slope_f, F)) || (rc = dalloc(&w->counters, 1)) ||
+        (rc = dalloc(&w->dragg, F)) || (rc = dalloc(&w->slope_f, F)) || (rc = dalloc(&w->pcg, 1)) ||
+        (rc = dalloc(&w->pcg_part, 2 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(&w->pcg_r, F)) ||
+        (rc = dalloc(&w->pcg_z, F)) || (rc = dalloc(&w->pcg_d, F)) || (rc = dalloc(&w->pcg_q, F)) ||
+        (rc = dalloc(&w->pcg_r16, F)) || (rc = dalloc(&w->pcg_z16, F)) ||
         (rc = dalloc(&w->ledger, 1)) || (rc = dalloc(&w->scratch_u32, F)) || (rc = dalloc(&w->epoch_dev, 1)) ||
         (rc = build_mg(w))) {
         ribo_world_destroy(w);
@@ -506,6 +665,13 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     LedgerDev init{};
     init.first_bad = 0xFFFFFFFFu;
     he = hipMemcpy(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice);
+    PcgState pst{};
+    pst.part = w->pcg_part;
+    if (he == hipSuccess && hipHostMalloc((void**)&w->pcg_host, 4 * sizeof(uint32_t), hipHostMallocMapped) == hipSuccess) {
+        memset(w->pcg_host, 0, 4 * sizeof(uint32_t));
+        if (hipHostGetDevicePointer((void**)&pst.host_out, w->pcg_host, 0) != hipSuccess) pst.host_out = nullptr;
+    }
+    if (he == hipSuccess) he = hipMemcpy(w->pcg, &pst, sizeof(pst), hipMemcpyHostToDevice);
     if (he != hipSuccess || (rc = rebuild_terrain(w)) != RIBO_OK || hipStreamSynchronize(w->stream) != hipSuccess) {
         ribo_world_destroy(w);
         return fail(RIBO_ERR_HIP, "world initialisation failed");
@@ -523,8 +689,11 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
-    dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->counters); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
+    dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
+    dfree(w->pcg_z); dfree(w->pcg_d); dfree(w->pcg_q); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
+    dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
     clear_graphs(w);
+    if (w->pcg_host) (void)hipHostFree(w->pcg_host);
     if (w->stream) (void)hipStreamDestroy(w->stream);
     delete w;
     return RIBO_OK;
@@ -552,13 +721,8 @@ int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s) {
         if ((rc = build_mg(w)) || (rc = rebuild_terrain(w))) return rc;
     }
     if (arm) {
-        LedgerDev init{};
-        init.first_bad = 0xFFFFFFFFu;
-        HIPCHK(hipMemcpyAsync(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice, w->stream));
-        // arm the expected per-channel totals with the current grid
-        launch_ledger(w->stream, w->geo(), w->chem[w->cur], w->ledger, w->params.epoch, nullptr, 1);
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipStreamSynchronize(w->stream));
+        rc = arm_ledger(w);
+        if (rc) return rc;
     }
     return RIBO_OK;
 }
@@ -593,6 +757,10 @@ int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t by
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w->stream));
     if (which == RIBO_BUF_GAMMA) {
         int rc = rebuild_terrain(w);
+        if (rc) return rc;
+    }
+    if (which == RIBO_BUF_CHEM && w->st.ledger_check) {
+        int rc = arm_ledger(w);
         if (rc) return rc;
     }
     HIPCHK(hipStreamSynchronize(w->stream));
@@ -635,6 +803,26 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
     if (rc) return rc;
     steps = std::min(steps, RIBO_MAX_STEPS_PER_CALL);
     if (steps == 0) return 0;
+    if (w->st.multigrid && w->st.mg_tol > 0.0f) {
+        if (!w->cinv_valid) {
+            clear_graphs(w);
+            rc = build_coarse_inverse(w);
+            if (rc) return rc;
+        }
+        // Iteration slots per solve for this chunk: the previous solve's
+        // count plus 30 % and 4, in multiples of 4 (after draining the
+        // stream, so the choice depends only on the sequence of calls); a
+        // solve that ran out of slots (counted on the device) restores the
+        // full cap.
+        const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
+        HIPCHK(hipStreamSynchronize(w->stream));
+        const uint32_t last = w->pcg_host ? __atomic_load_n(&w->pcg_host[0], __ATOMIC_RELAXED) : maxit;
+        const uint32_t missed = w->pcg_host ? __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED) : 0u;
+        uint32_t slots = std::min(maxit, ((last + last * 3 / 10 + 4) + 3) / 4 * 4);
+        if (w->pcg_slots == 0 || missed != w->missed_seen) slots = maxit;
+        w->missed_seen = missed;
+        w->pcg_slots = std::max(slots, 1u);
+    }
     const RiboWorldSettings st = w->st;
     const uint32_t substep = std::min(std::max(st.fluid_substep, 1u), 4u);
     if (!st.fluid_enabled && w->fluid_was_on) {
@@ -666,9 +854,9 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         const uint32_t phase = w->params.epoch % period;
         RiboWorld::GraphEntry* ge = nullptr;
         for (auto& e : w->graphs)
-            if (e.phase == phase && e.n == steps && e.cur == w->cur && e.tcur == w->tcur) ge = &e;
+            if (e.phase == phase && e.n == steps && e.slots == w->pcg_slots && e.cur == w->cur && e.tcur == w->tcur) ge = &e;
         if (!ge) {
-            RiboWorld::GraphEntry e{phase, steps, w->cur, w->tcur, 0, 0, nullptr};
+            RiboWorld::GraphEntry e{phase, steps, w->pcg_slots, w->cur, w->tcur, 0, 0, nullptr};
             hipGraph_t graph = nullptr;
             HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
             record_steps(w, steps, substep, w->epoch_dev);
@@ -763,16 +951,27 @@ int ribo_world_reset_kernel_times(RiboWorld* w) {
 }
 int ribo_world_mg_stats(RiboWorld* w, uint32_t* cycles, double* rel_residual) {
     if (!w || !cycles || !rel_residual) return fail(RIBO_ERR_ARG, "null argument");
-    Counters c;
-    HIPCHK(hipMemcpyAsync(&c, w->counters, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    PcgState c;
+    HIPCHK(hipMemcpyAsync(&c, w->pcg, sizeof(c), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
     if (w->st.mg_tol > 0.0f) {
-        *cycles = c.cycles;
-        *rel_residual = c.last_rel;
+        *cycles = c.it;
+        *rel_residual = c.true_rel;
     } else {
         *cycles = std::max(w->st.mg_cycles, 1u);
         *rel_residual = -1.0;
     }
+    return RIBO_OK;
+}
+
+int ribo_world_solver_stats(RiboWorld* w, uint64_t* solves, uint64_t* iterations, uint64_t* missed) {
+    if (!w || !solves || !iterations || !missed) return fail(RIBO_ERR_ARG, "null argument");
+    PcgState c;
+    HIPCHK(hipMemcpyAsync(&c, w->pcg, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    *solves = c.steps;
+    *iterations = c.iters;
+    *missed = c.missed;
     return RIBO_OK;
 }
 

@@ -283,6 +283,8 @@ def lib() -> ctypes.CDLL:
         "ribo_world_kernel_times": (I, [P, P, P, P, U32, ctypes.POINTER(U32)]),
         "ribo_world_reset_kernel_times": (I, [P]),
         "ribo_world_mg_stats": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
+        "ribo_world_solver_stats": (I, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint64)]),
         "ribo_last_error": (ctypes.c_char_p, []),
         "ribo_abi_version": (U32, []),
         "ribo_gen_gamma": (I, [P, ctypes.c_uint64, ctypes.c_float, I, P]),
@@ -465,6 +467,11 @@ class World:
 
     def reset_kernel_times(self):
         _check(lib().ribo_world_reset_kernel_times(self._h), "reset_kernel_times")
+
+    def solver_stats(self) -> tuple:
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().ribo_world_solver_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "solver_stats")
+        return int(a.value), int(b.value), int(c.value)
 
     def mg_stats(self) -> tuple:
         c = ctypes.c_uint32()

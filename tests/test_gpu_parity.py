@@ -156,22 +156,40 @@ def test_batch_invariance():
     assert bit_diff(a[1], b[1]) == 0 and bit_diff(a[1], c[1]) == 0
 
 
-def test_tolerance_mode_reaches_residual():
-    print("case tolerance", flush=True)
-    cfg = ra.WorldConfig(128, 64)
-    inp = make_inputs(cfg, 4, rock=0.1, vent_rows=4)
-    s = parity_settings(mg_tol=1e-4, mg_max_cycles=40)
+def rel_l2(a, b):
+    a, b = a.astype(np.float64), b.astype(np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tolerance_mode_matches_oracle(precision):
+    """Tolerance mode (preconditioned conjugate gradients with the symmetric
+    V-cycle; precision 1 stores the preconditioner in bfloat16): the solve
+    reaches the relative residual on the GPU, takes about as many iterations
+    as the oracle, and the fields agree with the oracle's to a stated
+    relative L2 (dot products are summed in a different order, so the bits
+    differ; monomer totals stay exact)."""
+    print(f"case tolerance precision={precision}", flush=True)
+    cfg = ra.WorldConfig(256, 128)
+    inp = make_inputs(cfg, 4, rock=0.15, rubble=True, vent_rows=4)
+    s = parity_settings(mg_tol=1e-5, mg_max_cycles=120, precision=precision)
     gw = ra.World(cfg, 4)
     load(gw, inp, parity_params(4), s)
     ow = oc.OracleWorld(cfg, 4)
     load(ow, inp, parity_params(4), s)
-    for _ in range(3):
+    for k in range(4):
         gw.encode_steps(1)
         ow.encode_steps(1)
         cyc, rel = gw.mg_stats()
-        print(f"cycles {cyc} rel {rel:.3e} oracle rel {ow.mg_residual():.3e}", flush=True)
-        assert rel <= 1e-4 or cyc == 40
-    assert compare(gw, ow, "tolerance")
+        ocyc, orel = ow.mg_stats()
+        print(f"step {k}: GPU {cyc} iterations, true rel {rel:.3e}; oracle {ocyc} iterations, rel {orel:.3e}", flush=True)
+        assert rel <= 1.5e-5 and cyc < 120
+        assert abs(cyc - ocyc) <= max(3, ocyc // 5)
+    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-4), (ra.TEMP, 1e-5)):
+        e = rel_l2(gw.download(which), ow.download(which))
+        print(f"{ra.BUFFER_NAMES[which]} rel L2 vs oracle {e:.2e}", flush=True)
+        assert e < tol
+    assert ra.Ledger.from_cells(gw.read_cells_blocking()).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
 
 
 @pytest.mark.parametrize("tol", [0.0, 1e-4])
@@ -182,7 +200,7 @@ def test_graph_replays_match_eager_recording(tol):
     print(f"case graphs tol={tol}", flush=True)
     cfg = ra.WorldConfig(128, 64)
     inp = make_inputs(cfg, 17, rock=0.12, rubble=True, vent_rows=4)
-    extra = dict(mg_tol=tol, mg_max_cycles=6) if tol else {}
+    extra = dict(mg_tol=tol, mg_max_cycles=12) if tol else {}
     worlds = []
     for no_graphs in (1, 0):
         w = ra.World(cfg, 17)

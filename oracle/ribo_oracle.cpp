@@ -988,6 +988,25 @@ template <class R> struct MG {
                 out[i] = s - (k + R(L.eps)) * v[i];
             }
     }
+    // ||b - A p||2 / ||b||2 over fluid cells of level 0 for level-0 vectors.
+    double rel_residual_of(const R* p0, const R* b) const {
+        const Level& L = (*lv)[0];
+        double rr = 0, bb = 0;
+        for (uint32_t y = 0; y < L.n; ++y)
+            for (uint32_t x = 0; x < L.n; ++x) {
+                const uint32_t i = y * L.n + x;
+                if (fluid[L.off + i] == 0u) continue;
+                double s = 0, k = 0;
+                if (x > 0 && fluid[L.off + i - 1]) { s += (double)p0[i - 1]; k += 1; }
+                if (x + 1 < L.n && fluid[L.off + i + 1]) { s += (double)p0[i + 1]; k += 1; }
+                if (y > 0 && fluid[L.off + i - L.n]) { s += (double)p0[i - L.n]; k += 1; }
+                if (y + 1 < L.n && fluid[L.off + i + L.n]) { s += (double)p0[i + L.n]; k += 1; }
+                const double r = (double)b[i] - (s - (k + (double)L.eps) * (double)p0[i]);
+                rr += r * r;
+                bb += (double)b[i] * (double)b[i];
+            }
+        return bb > 0 ? sqrt(rr / bb) : 0.0;
+    }
     // ||rhs - A p||2 / ||rhs||2 over fluid cells of level 0 (binary64 sums).
     double rel_residual() const {
         const Level& L = (*lv)[0];
@@ -1317,8 +1336,11 @@ template <class R> void transport(OracleWorld& w, const Fluid<R>& fl, const Ribo
 // A p = div over fluid cells of level 0, warm-started from the previous
 // pressure, preconditioned by the symmetric V-cycle (MG::vcycle_sym), until
 // ||div - A p||2 / ||div||2 <= mg_tol or mg_max_cycles iterations.  Scalars
-// and dot products are binary64; the search direction's image is updated
-// as q = A z + beta q.  precision = MIXED stores the preconditioner's
+// and dot products are binary64; the search direction's image is updated as
+// q = A z + beta q.  The stop uses the recursively updated residual; the
+// reported residual is recomputed from the final pressure (in binary32 it
+// cannot fall much below ~1e-5 on rock-laden worlds: the pressure's own
+// rounding bounds it).  precision = MIXED stores the preconditioner's
 // unknowns and right-hand sides in bfloat16.
 template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const std::vector<Level>& lv) {
     const RiboWorldSettings& st = w.st;
@@ -1385,7 +1407,8 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
         }
     }
     w.last_cycles = it;
-    w.last_rel_res = rel;
+    (void)rel;
+    w.last_rel_res = mg.rel_residual_of(p, b);
 }
 
 // ---- one fluid step (src/world/mod.rs:1527-1588) ---------------------------------

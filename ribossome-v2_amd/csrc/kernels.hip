@@ -963,17 +963,20 @@ __device__ __forceinline__ float apply_A_s(const uint8_t* fl, const S* v, int n,
     return s - (k + eps) * ldv(v, i);
 }
 
-// Block sums of up to 2 values into part[block * 2 + j]; returns true in the
-// one thread of the last block to finish, which then owns the totals.
+// Block sums of 2 values into part[block * 2 + j]; the last block to finish
+// then sums all partials -- every thread a strided subset in block order,
+// then a fixed tree -- so the totals do not depend on which block finished
+// last.  Returns true in thread 0 of that block, with the totals in tot[].
 __device__ __forceinline__ bool block_reduce2(double v0, double v1, double* part, uint32_t* count, double* tot) {
     __shared__ double sh[2][16];
     __shared__ bool last;
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    const int lane = tid & 63, wid = tid >> 6, nw = (blockDim.x * blockDim.y) >> 6;
+    const int nthreads = blockDim.x * blockDim.y;
     for (int o = 32; o > 0; o >>= 1) {
         v0 += __shfl_down(v0, o, 64);
         v1 += __shfl_down(v1, o, 64);
     }
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    const int lane = tid & 63, wid = tid >> 6, nw = (blockDim.x * blockDim.y) >> 6;
     if (lane == 0) { sh[0][wid] = v0; sh[1][wid] = v1; }
     __syncthreads();
     const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
@@ -987,15 +990,24 @@ __device__ __forceinline__ bool block_reduce2(double v0, double v1, double* part
     }
     __syncthreads();
     if (!last) return false;
+    __threadfence();
+    double a0 = 0.0, a1 = 0.0;
+    for (uint32_t k = (uint32_t)tid; k < nb; k += (uint32_t)nthreads) {
+        a0 += __hip_atomic_load(part + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a1 += __hip_atomic_load(part + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        a0 += __shfl_down(a0, o, 64);
+        a1 += __shfl_down(a1, o, 64);
+    }
+    __syncthreads();
+    if (lane == 0) { sh[0][wid] = a0; sh[1][wid] = a1; }
+    __syncthreads();
     if (tid == 0) {
-        __threadfence();
-        double a0 = 0.0, a1 = 0.0;
-        for (uint32_t k = 0; k < nb; ++k) {
-            a0 += __hip_atomic_load(part + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            a1 += __hip_atomic_load(part + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        tot[0] = a0;
-        tot[1] = a1;
+        double t0 = 0.0, t1 = 0.0;
+        for (int w = 0; w < nw; ++w) { t0 += sh[0][w]; t1 += sh[1][w]; }
+        tot[0] = t0;
+        tot[1] = t1;
         *count = 0u;
     }
     return tid == 0;
@@ -1596,10 +1608,18 @@ __global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ che
     __syncthreads();
     if (!last) return;
     __threadfence();
+    // every thread sums a strided subset of the partials, then a fixed tree
+    uint32_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256u)
+        for (int k = 0; k < 8; ++k) t8[k] += __hip_atomic_load(led->part + b * 8 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (int k = 0; k < 8; ++k)
+        for (int o = 32; o > 0; o >>= 1) t8[k] += __shfl_down(t8[k], o, 64);
+    __syncthreads();
+    if (lane == 0)
+        for (int k = 0; k < 8; ++k) sh[wid][k] = t8[k];
+    __syncthreads();
     uint32_t tot = 0u;
-    if (threadIdx.x < 8)
-        for (uint32_t b = 0; b < gridDim.x; ++b)
-            tot += __hip_atomic_load(led->part + b * 8 + threadIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (threadIdx.x < 8) tot = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
     if (threadIdx.x < 8) {
         led->last[threadIdx.x] = tot;
         sh[0][threadIdx.x] = tot;

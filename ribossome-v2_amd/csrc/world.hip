@@ -512,11 +512,12 @@ void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) { 
     if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, w->chem[w->cur], w->ledger, P.epoch, ep, 1); });
 }
 
-// Steps [0, steps) of one encode_steps call, epochs P.epoch + i (eager) or
-// i on top of the device epoch base `ep` (graph capture).
-void record_steps(RiboWorld* w, uint32_t steps, uint32_t substep, const uint32_t* ep) {
-    for (uint32_t i = 0; i < steps; ++i) {
-        const uint32_t epoch = w->params.epoch + i;
+// Steps [first, first + count) of one encode_steps call: epochs
+// params.epoch + first + i, passed as is (eager) or as i on top of the
+// device epoch base `ep` (graph capture).
+void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep, const uint32_t* ep) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const uint32_t epoch = w->params.epoch + first + i;
         RiboSimParams P = w->params;
         P.epoch = ep ? i : epoch;
         P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, epoch) : 0.0f;
@@ -851,34 +852,41 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         const uint32_t te = std::max(w->params.transport_every, 1u);
         uint32_t period = substep;
         while (period % te != 0) period += substep;
-        const uint32_t phase = w->params.epoch % period;
-        RiboWorld::GraphEntry* ge = nullptr;
-        for (auto& e : w->graphs)
-            if (e.phase == phase && e.n == steps && e.slots == w->pcg_slots && e.cur == w->cur && e.tcur == w->tcur) ge = &e;
-        if (!ge) {
-            RiboWorld::GraphEntry e{phase, steps, w->pcg_slots, w->cur, w->tcur, 0, 0, nullptr};
-            hipGraph_t graph = nullptr;
-            HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
-            record_steps(w, steps, substep, w->epoch_dev);
-            launch_epoch_add(w->stream, w->epoch_dev, steps);
-            hipError_t ce = hipStreamEndCapture(w->stream, &graph);
-            e.cur_end = w->cur;
-            e.tcur_end = w->tcur;
-            w->cur = e.cur;
-            w->tcur = e.tcur;
-            if (ce != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
-            hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (ie != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
-            w->graphs.push_back(e);
-            ge = &w->graphs.back();
+        // tolerance-mode steps carry their solver iterations: shorter graphs
+        const uint32_t max_graph = (w->st.multigrid && w->st.mg_tol > 0.0f) ? 8u : RIBO_MAX_STEPS_PER_CALL;
+        for (uint32_t first = 0; first < steps;) {
+            const uint32_t n = std::min(max_graph, steps - first);
+            const uint32_t phase = (w->params.epoch + first) % period;
+            RiboWorld::GraphEntry* ge = nullptr;
+            for (auto& e : w->graphs)
+                if (e.phase == phase && e.n == n && e.slots == w->pcg_slots && e.cur == w->cur && e.tcur == w->tcur) ge = &e;
+            if (!ge) {
+                if (w->graphs.size() >= 32) clear_graphs(w);
+                RiboWorld::GraphEntry e{phase, n, w->pcg_slots, w->cur, w->tcur, 0, 0, nullptr};
+                hipGraph_t graph = nullptr;
+                HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
+                record_steps(w, first, n, substep, w->epoch_dev);
+                launch_epoch_add(w->stream, w->epoch_dev, n);
+                hipError_t ce = hipStreamEndCapture(w->stream, &graph);
+                e.cur_end = w->cur;
+                e.tcur_end = w->tcur;
+                w->cur = e.cur;
+                w->tcur = e.tcur;
+                if (ce != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
+                hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (ie != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph instantiate: ") + hipGetErrorString(ie));
+                w->graphs.push_back(e);
+                ge = &w->graphs.back();
+            }
+            HIPCHK(hipGraphLaunch(ge->exec, w->stream));
+            w->cur = ge->cur_end;
+            w->tcur = ge->tcur_end;
+            w->epoch_dev_value += n;
+            first += n;
         }
-        HIPCHK(hipGraphLaunch(ge->exec, w->stream));
-        w->cur = ge->cur_end;
-        w->tcur = ge->tcur_end;
-        w->epoch_dev_value += steps;
     } else {
-        record_steps(w, steps, substep, nullptr);
+        record_steps(w, 0, steps, substep, nullptr);
     }
     HIPCHK(hipGetLastError());
     w->params.epoch += steps;

@@ -183,7 +183,10 @@ def test_tolerance_mode_matches_oracle(precision):
         cyc, rel = gw.mg_stats()
         ocyc, orel = ow.mg_stats()
         print(f"step {k}: GPU {cyc} iterations, true rel {rel:.3e}; oracle {ocyc} iterations, rel {orel:.3e}", flush=True)
-        assert rel <= 1.5e-5 and cyc < 120
+        # binary32 pressure bounds the true residual near 1e-5 on this world
+        # (the solve stops on the recursive residual); both report it
+        assert rel <= 3e-5 and cyc < 120
+        assert abs(rel - orel) <= 0.5 * orel
         assert abs(cyc - ocyc) <= max(3, ocyc // 5)
     for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-4), (ra.TEMP, 1e-5)):
         e = rel_l2(gw.download(which), ow.download(which))

@@ -86,6 +86,7 @@ def configure(p, s, args, mode="fp32"):
     s.mg_tol = MODES[mode]["mg_tol"]
     s.precision = MODES[mode]["precision"]
     s.ledger_check = 1
+    s.no_graphs = 1 if getattr(args, "no_graphs", False) else 0
     return p, s
 
 
@@ -170,6 +171,8 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true",
+                    help="launch every kernel eagerly instead of replaying captured HIP graphs (for tracing tools)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the world step is a single-device path here; --gpus {args.gpus} "

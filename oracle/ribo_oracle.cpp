@@ -1335,13 +1335,28 @@ template <class R> void transport(OracleWorld& w, const Fluid<R>& fl, const Ribo
 // V-cycles): preconditioned conjugate gradients on the eps-anchored operator
 // A p = div over fluid cells of level 0, warm-started from the previous
 // pressure, preconditioned by the symmetric V-cycle (MG::vcycle_sym), until
-// ||div - A p||2 / ||div||2 <= mg_tol or mg_max_cycles iterations.  Scalars
-// and dot products are binary64; the search direction's image is updated as
-// q = A z + beta q.  The stop uses the recursively updated residual; the
+// ||div - A p||2 / ||div||2 <= mg_tol or mg_max_cycles iterations.
+//
+// The recurrences put all inner products of an iteration at one point,
+// after the preconditioner (Chronopoulos and Gear, 1989): with u = M r and
+// w = A u,
+//   gamma = (r, u), delta = (w, u), e = (u, s), f = (p, w), rr = (r, r)
+// for the current p and s, then
+//   beta = gamma / gamma_prev (0 on the first application),
+//   pi = delta + beta (e + f) + beta^2 pi_prev   (= (p', s') below),
+//   alpha = gamma / pi,
+//   p' = u + beta p,  s' = w + beta s,  x += alpha p',  r -= alpha s'.
+// pi is the expansion of (p', s') itself rather than Chronopoulos and Gear's
+// shorter form delta - beta gamma / alpha_prev, which holds only for an
+// exactly linear preconditioner: with the bfloat16 preconditioner the short
+// form stalls and this one converges like binary32.
+//
+// Scalars and inner products are binary64; vectors binary32 (or binary64 in
+// the FP64 oracle).  The stop uses the recursively updated residual; the
 // reported residual is recomputed from the final pressure (in binary32 it
 // cannot fall much below ~1e-5 on rock-laden worlds: the pressure's own
 // rounding bounds it).  precision = MIXED stores the preconditioner's
-// unknowns and right-hand sides in bfloat16.
+// unknowns and right-hand sides (level 0 included) in bfloat16.
 template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const std::vector<Level>& lv) {
     const RiboWorldSettings& st = w.st;
     const size_t n2 = (size_t)lv[0].n * lv[0].n;
@@ -1357,58 +1372,64 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
     CoarseInv ci;
     const Level& C = lv.back();
     mg.cinv = ci.build(C.n, w.mg_fluid.data() + C.off, C.eps) ? &ci : nullptr;
-    std::vector<R> r(n2), z(n2), d(n2), q(n2), az(n2);
-    R* p = fl.p_a.data();
+    std::vector<R> r(n2), u(n2), wv(n2), pd(n2, R(0)), sd(n2, R(0));
+    R* x = fl.p_a.data();
     const R* b = fl.div.data();
-    mg.apply(p, q.data());
+    mg.apply(x, wv.data());
     double bb = 0.0, rr = 0.0;
     for (size_t i = 0; i < n2; ++i) {
-        r[i] = w.mg_fluid[i] ? b[i] - q[i] : R(0);
+        r[i] = w.mg_fluid[i] ? b[i] - wv[i] : R(0);
         bb += (double)b[i] * (double)b[i];
         rr += (double)r[i] * (double)r[i];
     }
+    // u = M r, w = A u and the iteration's inner products
+    double gamma = 0.0, delta = 0.0, e = 0.0, f = 0.0;
     auto precond = [&]() {
-        for (size_t i = 0; i < n2; ++i) { fl.mg_rhs[i] = r[i]; fl.mg_p[i] = 0; }
+        for (size_t i = 0; i < n2; ++i) {
+            fl.mg_rhs[i] = mg.bf16 ? R(round_bf16((float)r[i])) : r[i];
+            fl.mg_p[i] = 0;
+        }
         mg.vcycle_sym();
-        for (size_t i = 0; i < n2; ++i) z[i] = fl.mg_p[i];
+        for (size_t i = 0; i < n2; ++i) u[i] = fl.mg_p[i];
+        mg.apply(u.data(), wv.data());
+        gamma = delta = e = f = rr = 0.0;
+        for (size_t i = 0; i < n2; ++i) {
+            if (!w.mg_fluid[i]) continue;
+            gamma += (double)r[i] * (double)u[i];
+            delta += (double)wv[i] * (double)u[i];
+            e += (double)u[i] * (double)sd[i];
+            f += (double)pd[i] * (double)wv[i];
+            rr += (double)r[i] * (double)r[i];
+        }
     };
     uint32_t it = 0;
     const uint32_t maxit = std::max(st.mg_max_cycles, 1u);
     double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
     if (rel > (double)st.mg_tol) {
         precond();
-        double rho = 0.0;
-        for (size_t i = 0; i < n2; ++i) rho += (double)z[i] * (double)r[i];
-        d = z;
-        mg.apply(d.data(), q.data());
+        double beta = 0.0, pi = delta, g_prev = gamma;
+        double alpha = pi != 0.0 ? gamma / pi : 0.0;
         while (it < maxit) {
-            double dq = 0.0;
-            for (size_t i = 0; i < n2; ++i) dq += (double)d[i] * (double)q[i];
-            const R alpha = R(dq != 0.0 ? rho / dq : 0.0);
-            rr = 0.0;
+            const R a = R(alpha), be = R(beta);
             for (size_t i = 0; i < n2; ++i) {
-                p[i] += alpha * d[i];
-                r[i] -= alpha * q[i];
-                rr += (double)r[i] * (double)r[i];
+                if (!w.mg_fluid[i]) continue;
+                pd[i] = u[i] + be * pd[i];
+                sd[i] = wv[i] + be * sd[i];
+                x[i] += a * pd[i];
+                r[i] -= a * sd[i];
             }
             ++it;
+            precond();
             rel = bb > 0 ? sqrt(rr / bb) : 0.0;
             if (rel <= (double)st.mg_tol || it >= maxit) break;
-            precond();
-            double rho2 = 0.0;
-            for (size_t i = 0; i < n2; ++i) rho2 += (double)z[i] * (double)r[i];
-            const R beta = R(rho != 0.0 ? rho2 / rho : 0.0);
-            rho = rho2;
-            mg.apply(z.data(), az.data());
-            for (size_t i = 0; i < n2; ++i) {
-                d[i] = z[i] + beta * d[i];
-                q[i] = az[i] + beta * q[i];
-            }
+            beta = g_prev != 0.0 ? gamma / g_prev : 0.0;
+            pi = delta + beta * (e + f) + beta * beta * pi;
+            alpha = pi != 0.0 ? gamma / pi : 0.0;
+            g_prev = gamma;
         }
     }
     w.last_cycles = it;
-    (void)rel;
-    w.last_rel_res = mg.rel_residual_of(p, b);
+    w.last_rel_res = mg.rel_residual_of(x, b);
 }
 
 // ---- one fluid step (src/world/mod.rs:1527-1588) ---------------------------------

@@ -54,39 +54,39 @@ template <class S> struct MgLevelT {
 using MgLevelDev = MgLevelT<float>;
 
 // Level-0 vectors of the preconditioned conjugate gradients (tolerance
-// mode).  z is the preconditioner's output and lives in its storage type S;
-// rs is the preconditioner's copy of r (null when S is float: it reads r).
-template <class S> struct PcgVec {
+// mode, Chronopoulos-Gear recurrences).  r and s are double-buffered: the
+// kernel that updates them reads their old values with a halo.  u is the
+// preconditioner's output and lives in its storage type S; rs is the
+// preconditioner's copy of r (null when S is float: it reads r itself).
+template <class S> struct CgVec {
     uint32_t n;
     float eps;
     const uint8_t* fl;
-    float* p;
-    const float* b;
-    float* r;
+    float* x;        // the pressure
+    const float* b;  // the divergence
+    float* r[2];
+    float* s[2];
+    float* pd;       // search direction
+    float* w;        // A u
+    S* u;
     S* rs;
-    S* z;
-    float* d;
-    float* q;
 };
 
 struct PcgState {
-    double bb, rho, beta, alpha, rel, true_rel;
-    uint32_t it, done, count, steps, iters, missed;
+    double bb, gamma, pi, beta, alpha, rel, true_rel;
+    uint32_t it, done, steps, iters, missed;
     float tol_seen;
-    uint32_t pad;
-    double* part;       // 2 doubles per block of the level-0 grid
+    double* part;       // per-block partial sums (up to 5 per block of 16 x 16 cells)
     uint32_t* host_out; // host-mapped words: last iterations, misses, solves
 };
 
 constexpr uint32_t LEDGER_MAX_BLOCKS = 1024;
 struct LedgerDev {
     uint32_t part[8 * LEDGER_MAX_BLOCKS];  // per-block act[4], spent[4]
-    uint32_t blocks_done;
     uint32_t armed;         // 1 once expected[] is set
     uint32_t expected[4];   // per-channel act + spent
     uint32_t first_bad;     // first epoch whose totals differed (0xFFFFFFFF none)
     uint32_t last[8];       // last completed reduction
-    uint32_t pad;
 };
 
 // ---- launchers (kernels.hip) ----
@@ -125,11 +125,20 @@ size_t mg_block_lds_bytes(const uint32_t* n, int count);
 template <class S> void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv,
                                         const int32_t* ccell, int cm, const uint32_t* done);
 // preconditioned conjugate gradients
-template <class S> void launch_pcg_init(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
-template <class S> void launch_pcg_rho(hipStream_t s, const PcgVec<S>& V, PcgState* st);
-template <class S> void launch_pcg_dir(hipStream_t s, const PcgVec<S>& V, PcgState* st);
-template <class S> void launch_pcg_update(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
-template <class S> void launch_pcg_final(hipStream_t s, const PcgVec<S>& V, PcgState* st);
+// r = b - A x and the solve's initial state (zeroes the other vectors).
+template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
+// Pre-smoothing and restriction of level 0, fused with the vector updates of
+// the iteration (p, s, x, r; none on the first application, first = 1);
+// parity selects which r and s buffers are read.
+template <class S>
+void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first);
+// Prolongation and post-smoothing of level 0 fused with w = A u and the
+// inner products; then the scalar recurrences (first: the initial ones).
+template <class S>
+void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first,
+                   float tol, uint32_t maxit);
+// True residual ||b - A x|| / ||b|| of the solve, and the statistics.
+template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st);
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);

@@ -648,16 +648,19 @@ __global__ __launch_bounds__(256) void k_mg_prolong(MgLevelDev L, MgLevelDev C, 
 // always binary32 in LDS); SYM selects the symmetric V-cycle of the
 // tolerance mode (post-smoothing black then red, restriction = half the sum
 // over fluid children, exact coarsest solve) instead of the reference's.
-constexpr int MG_T = 32;  // output tile (even, so tile origins keep (x+y) parity)
+// Output tiles are T x T with T even, so tile origins keep (x + y) parity:
+// 32 on the large levels, 16 on levels of at most MG_SMALL cells a side,
+// where there are few tiles and each block's latency is the whole cost.
+constexpr int MG_SMALL = 256;
 
 __device__ __forceinline__ float ldv(const float* a, int i) { return a[i]; }
 __device__ __forceinline__ float ldv(const bf16* a, int i) { return rb_as_f32((uint32_t)a[i].bits << 16); }
 __device__ __forceinline__ void stv(float* a, int i, float v) { a[i] = v; }
 __device__ __forceinline__ void stv(bf16* a, int i, float v) { a[i].bits = bf16_bits(v); }
 
-template <int H>
+template <int H, int T>
 struct MgTile {
-    static constexpr int R = MG_T + 2 * H;
+    static constexpr int R = T + 2 * H;
     float p[R][R + 1];
     float rhs[R][R + 1];
     uint8_t fl[R][R + 1];
@@ -665,8 +668,8 @@ struct MgTile {
 
 // p_c <- (sum of fluid neighbours - rhs) / (k + eps), on the cells of one
 // colour inside [lo, hi) of the tile (multigrid.wgsl:35-45,60-67).
-template <int H>
-__device__ __forceinline__ void tile_half_sweep(MgTile<H>& t, int color, float eps, int lo, int hi) {
+template <int H, int T>
+__device__ __forceinline__ void tile_half_sweep(MgTile<H, T>& t, int color, float eps, int lo, int hi) {
     const int w = hi - lo;
     const int half = (w + 1) / 2;
     for (int k = threadIdx.x; k < w * half; k += blockDim.x) {
@@ -684,44 +687,61 @@ __device__ __forceinline__ void tile_half_sweep(MgTile<H>& t, int color, float e
     }
 }
 
-template <int H, class S>
-__device__ __forceinline__ void tile_load(MgTile<H>& t, const MgLevelT<S>& L, int x0, int y0, const MgLevelT<S>* C) {
-    constexpr int R = MgTile<H>::R;
+// Loads the tile with its halo.  Every thread issues all of its loads
+// before the first LDS store (one memory round trip per tile, not one per
+// loop trip); values of solid or outside cells are replaced by 0 after the
+// load, so the fluid mask is not on the load's critical path.
+template <int H, int T, class S>
+__device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L, int x0, int y0, const MgLevelT<S>* C) {
+    constexpr int R = MgTile<H, T>::R;
+    constexpr int PER = (R * R + 255) / 256;
     const int n = (int)L.n;
-    for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
+    float pv[PER], rv[PER], cv[PER];
+    uint32_t fv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = (int)threadIdx.x + 256 * j;
         const int ly = k / R, lx = k % R;
         const int gx = x0 + lx, gy = y0 + ly;
-        float pv = 0.0f, rv = 0.0f;
-        uint8_t f = 0;
-        if (gx >= 0 && gy >= 0 && gx < n && gy < n) {
+        pv[j] = 0.0f;
+        rv[j] = 0.0f;
+        cv[j] = 0.0f;
+        fv[j] = 0u;
+        if (k < R * R && gx >= 0 && gy >= 0 && gx < n && gy < n) {
             const int i = gy * n + gx;
-            f = L.fluid[i];
-            if (f) {
-                pv = ldv(L.p, i);
-                rv = ldv(L.rhs, i);
-                // mg_prolong (multigrid.wgsl:108-114), pointwise, applied on load
-                if (C) pv += ldv(C->p, (gy / 2) * (int)C->n + gx / 2);
-            }
+            fv[j] = L.fluid[i];
+            pv[j] = ldv(L.p, i);
+            rv[j] = ldv(L.rhs, i);
+            if (C) cv[j] = ldv(C->p, (gy / 2) * (int)C->n + gx / 2);
         }
-        t.p[ly][lx] = pv;
-        t.rhs[ly][lx] = rv;
-        t.fl[ly][lx] = f;
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = (int)threadIdx.x + 256 * j;
+        if (k < R * R) {
+            const int ly = k / R, lx = k % R;
+            const bool f = fv[j] != 0u;
+            // mg_prolong (multigrid.wgsl:108-114), pointwise, applied on load
+            t.p[ly][lx] = f ? (C ? pv[j] + cv[j] : pv[j]) : 0.0f;
+            t.rhs[ly][lx] = f ? rv[j] : 0.0f;
+            t.fl[ly][lx] = (uint8_t)fv[j];
+        }
     }
 }
 
 // Pre-smoothing (2 x red, black) and restriction of one level
 // (src/world/mod.rs:1556-1563; multigrid.wgsl:60-105).
-template <class S, bool SYM>
+template <class S, bool SYM, int MG_T>
 __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
     constexpr int H = 5;
-    __shared__ MgTile<H> t;
+    __shared__ MgTile<H, MG_T> t;
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H, S>(t, L, x0, y0, nullptr);
+    tile_load<H, MG_T, S>(t, L, x0, y0, nullptr);
     __syncthreads();
-    constexpr int R = MgTile<H>::R;
+    constexpr int R = MgTile<H, MG_T>::R;
     for (int s = 0; s < 4; ++s) {
-        tile_half_sweep<H>(t, s & 1, L.eps, 1 + s, R - 1 - s);
+        tile_half_sweep<H, MG_T>(t, s & 1, L.eps, 1 + s, R - 1 - s);
         __syncthreads();
     }
     const int n = (int)L.n;
@@ -757,17 +777,17 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
 
 // Prolongation and post-smoothing (2 x red, black; black, red when SYM) of
 // one level (src/world/mod.rs:1569-1575).
-template <class S, bool SYM>
+template <class S, bool SYM, int MG_T>
 __global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
     constexpr int H = 4;
-    __shared__ MgTile<H> t;
+    __shared__ MgTile<H, MG_T> t;
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H, S>(t, L, x0, y0, &C);
+    tile_load<H, MG_T, S>(t, L, x0, y0, &C);
     __syncthreads();
-    constexpr int R = MgTile<H>::R;
+    constexpr int R = MgTile<H, MG_T>::R;
     for (int s = 0; s < 4; ++s) {
-        tile_half_sweep<H>(t, SYM ? 1 - (s & 1) : (s & 1), L.eps, 1 + s, R - 1 - s);
+        tile_half_sweep<H, MG_T>(t, SYM ? 1 - (s & 1) : (s & 1), L.eps, 1 + s, R - 1 - s);
         __syncthreads();
     }
     const int n = (int)L.n;
@@ -940,9 +960,20 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
 }
 
 // ---- tolerance mode: preconditioned conjugate gradients ------------------------
-// Level-0 vectors over fluid cells; dot products are binary64, summed per
-// block and then over blocks in block order by the last block to finish, so
-// every run gives the same bits.  A v = sum of fluid neighbours - (k + eps) v.
+// All inner products of an iteration are formed at one point, after the
+// preconditioner (Chronopoulos and Gear, 1989): with u = M r (the symmetric
+// V-cycle) and w = A u,
+//   gamma = (r, u), delta = (w, u), e = (u, s), f = (p, w), rr = (r, r)
+// for the current p and s; then beta = gamma / gamma_prev,
+// pi = delta + beta (e + f) + beta^2 pi_prev (the expansion of (p', s')),
+// alpha = gamma / pi, and
+//   p' = u + beta p,  s' = w + beta s,  x += alpha p',  r -= alpha s'.
+// The vector updates ride on the V-cycle's first kernel, w = A u and the
+// inner products on its last: an iteration is the V-cycle's launches plus
+// one one-block kernel for the scalars.  Level-0 vectors cover fluid cells;
+// inner products are binary64, summed per block and then over the blocks in
+// a fixed order, so every run gives the same bits.  A v = sum of fluid
+// neighbours - (k + eps) v.
 __device__ __forceinline__ float apply_A(const uint8_t* fl, const float* v, int n, int x, int y, float eps) {
     const int i = y * n + x;
     float s = 0.0f, k = 0.0f;
@@ -952,65 +983,58 @@ __device__ __forceinline__ float apply_A(const uint8_t* fl, const float* v, int 
     if (y + 1 < n && fl[i + n]) { s += v[i + n]; k += 1.0f; }
     return s - (k + eps) * v[i];
 }
-template <class S>
-__device__ __forceinline__ float apply_A_s(const uint8_t* fl, const S* v, int n, int x, int y, float eps) {
-    const int i = y * n + x;
-    float s = 0.0f, k = 0.0f;
-    if (x > 0 && fl[i - 1]) { s += ldv(v, i - 1); k += 1.0f; }
-    if (x + 1 < n && fl[i + 1]) { s += ldv(v, i + 1); k += 1.0f; }
-    if (y > 0 && fl[i - n]) { s += ldv(v, i - n); k += 1.0f; }
-    if (y + 1 < n && fl[i + n]) { s += ldv(v, i + n); k += 1.0f; }
-    return s - (k + eps) * ldv(v, i);
-}
 
-// Block sums of 2 values into part[block * 2 + j]; the last block to finish
-// then sums all partials -- every thread a strided subset in block order,
-// then a fixed tree -- so the totals do not depend on which block finished
-// last.  Returns true in thread 0 of that block, with the totals in tot[].
-__device__ __forceinline__ bool block_reduce2(double v0, double v1, double* part, uint32_t* count, double* tot) {
-    __shared__ double sh[2][16];
-    __shared__ bool last;
+// Value after a store to S and a load back.
+__device__ __forceinline__ float round_s(float v, const float*) { return v; }
+__device__ __forceinline__ float round_s(float v, const bf16*) { return rb_as_f32((uint32_t)bf16_bits(v) << 16); }
+
+// Per-block sums of N values into part[N * block + j]: wave shuffles, then
+// the block's waves in order.  The totals are formed in a later launch
+// (k_pcg_fin), so no block signals or waits on another and the sums do not
+// depend on block scheduling.
+template <int N>
+__device__ __forceinline__ void block_partial(const double (&v)[N], double* part) {
+    __shared__ double sh[N][16];
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6, nw = (blockDim.x * blockDim.y) >> 6;
-    const int nthreads = blockDim.x * blockDim.y;
-    for (int o = 32; o > 0; o >>= 1) {
-        v0 += __shfl_down(v0, o, 64);
-        v1 += __shfl_down(v1, o, 64);
+    double a[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        a[j] = v[j];
+        for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_down(a[j], o, 64);
     }
-    if (lane == 0) { sh[0][wid] = v0; sh[1][wid] = v1; }
+    if (lane == 0)
+#pragma unroll
+        for (int j = 0; j < N; ++j) sh[j][wid] = a[j];
     __syncthreads();
-    const uint32_t nb = gridDim.x * gridDim.y, b = blockIdx.y * gridDim.x + blockIdx.x;
-    if (tid == 0) {
-        double a0 = 0.0, a1 = 0.0;
-        for (int w = 0; w < nw; ++w) { a0 += sh[0][w]; a1 += sh[1][w]; }
-        part[2 * b] = a0;
-        part[2 * b + 1] = a1;
-        __threadfence();
-        last = atomicAdd(count, 1u) + 1u == nb;
+    if (tid < N) {
+        double t = 0.0;
+        for (int w = 0; w < nw; ++w) t += sh[tid][w];
+        part[(size_t)N * (blockIdx.y * gridDim.x + blockIdx.x) + tid] = t;
+    }
+}
+
+// Totals of nb partial N-tuples in a fixed order (thread t sums blocks t,
+// t + 256, ..., then the wave tree, then the waves in order); valid in
+// thread 0.
+template <int N>
+__device__ void sum_partials(const double* __restrict__ part, uint32_t nb, double (&t)[N]) {
+    __shared__ double sh[N][4];
+    const int tid = threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double a[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) a[j] = 0.0;
+    for (uint32_t k = (uint32_t)tid; k < nb; k += 256u)
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[j] += part[(size_t)N * k + j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_down(a[j], o, 64);
+        if (lane == 0) sh[j][wid] = a[j];
     }
     __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double a0 = 0.0, a1 = 0.0;
-    for (uint32_t k = (uint32_t)tid; k < nb; k += (uint32_t)nthreads) {
-        a0 += __hip_atomic_load(part + 2 * k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a1 += __hip_atomic_load(part + 2 * k + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        a0 += __shfl_down(a0, o, 64);
-        a1 += __shfl_down(a1, o, 64);
-    }
-    __syncthreads();
-    if (lane == 0) { sh[0][wid] = a0; sh[1][wid] = a1; }
-    __syncthreads();
-    if (tid == 0) {
-        double t0 = 0.0, t1 = 0.0;
-        for (int w = 0; w < nw; ++w) { t0 += sh[0][w]; t1 += sh[1][w]; }
-        tot[0] = t0;
-        tot[1] = t1;
-        *count = 0u;
-    }
-    return tid == 0;
+#pragma unroll
+    for (int j = 0; j < N; ++j) t[j] = ((sh[j][0] + sh[j][1]) + sh[j][2]) + sh[j][3];
 }
 
 #define PCG_XY                                                                         \
@@ -1020,109 +1044,245 @@ __device__ __forceinline__ bool block_reduce2(double v0, double v1, double* part
     const int i = in ? y * n + x : 0;                                                  \
     const bool f = in && V.fl[i];
 
-// r = b - A p, z = 0 (and the preconditioner's copy of r); ||b||^2, ||r||^2.
+// r = b - A x (r[0]); r[1], s, p and w start at 0; ||b||^2 and ||r||^2.
 template <class S>
-__global__ __launch_bounds__(256) void k_pcg_init(PcgVec<S> V, PcgState* st, float tol, uint32_t maxit) {
+__global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
     PCG_XY
     double bb = 0.0, rr = 0.0;
     if (in) {
         float r = 0.0f;
         if (f) {
-            r = V.b[i] - apply_A(V.fl, V.p, n, x, y, V.eps);
+            r = V.b[i] - apply_A(V.fl, V.x, n, x, y, V.eps);
             bb = (double)V.b[i] * (double)V.b[i];
             rr = (double)r * (double)r;
         }
-        V.r[i] = r;
-        if (V.rs) stv(V.rs, i, r);
-        stv(V.z, i, 0.0f);
+        V.r[0][i] = r;
+        V.r[1][i] = 0.0f;
+        V.s[0][i] = 0.0f;
+        V.s[1][i] = 0.0f;
+        V.pd[i] = 0.0f;
+        V.w[i] = 0.0f;
     }
-    double tot[2];
-    if (block_reduce2(bb, rr, st->part, &st->count, tot)) {
-        st->bb = tot[0];
-        const double rel = tot[0] > 0.0 ? sqrt(tot[1] / tot[0]) : 0.0;
+    block_partial<2>({bb, rr}, st->part);
+}
+
+// Level 0 of the V-cycle, downward, with the iteration's vector updates:
+// every tile cell (halo included) forms s' = w + beta s and r' = r - alpha s'
+// from the old r and s, so the tile's right-hand side is the updated
+// residual; core cells also store r', s' and the updated p and x.  Then the
+// reference's 2 x (red, black) pre-smoothing from u = 0 and the restriction
+// (half the sum over fluid children), as k_mg_down.
+template <class S>
+__global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int first) {
+    if (st->done) return;
+    constexpr int H = 5, T = 32;
+    constexpr int R = T + 2 * H;
+    constexpr int PER = (R * R + 255) / 256;
+    __shared__ MgTile<H, T> t;
+    const float alpha = (float)st->alpha, beta = (float)st->beta;
+    const float* r_in = V.r[par];
+    const float* s_in = V.s[par];
+    float* r_out = V.r[1 - par];
+    float* s_out = V.s[1 - par];
+    const int n = (int)V.n;
+    const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
+    float rv[PER], sv[PER], wv[PER], uv[PER], pv[PER], xv[PER];
+    uint32_t fv[PER];
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = (int)threadIdx.x + 256 * j;
+        const int ly = k / R, lx = k % R;
+        const int gx = x0 + lx, gy = y0 + ly;
+        rv[j] = sv[j] = wv[j] = uv[j] = pv[j] = xv[j] = 0.0f;
+        fv[j] = 0u;
+        if (k < R * R && gx >= 0 && gy >= 0 && gx < n && gy < n) {
+            const int i = gy * n + gx;
+            fv[j] = V.fl[i];
+            rv[j] = r_in[i];
+            if (!first) {
+                sv[j] = s_in[i];
+                wv[j] = V.w[i];
+                if (lx >= H && lx < H + T && ly >= H && ly < H + T) {
+                    uv[j] = ldv(V.u, i);
+                    pv[j] = V.pd[i];
+                    xv[j] = V.x[i];
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < PER; ++j) {
+        const int k = (int)threadIdx.x + 256 * j;
+        if (k >= R * R) continue;
+        const int ly = k / R, lx = k % R;
+        const bool f = fv[j] != 0u;
+        float s_new = 0.0f, r_new = rv[j];
+        if (!first) {
+            s_new = wv[j] + beta * sv[j];
+            r_new = rv[j] - alpha * s_new;
+        }
+        t.p[ly][lx] = 0.0f;
+        t.rhs[ly][lx] = f ? round_s(r_new, V.u) : 0.0f;
+        t.fl[ly][lx] = (uint8_t)fv[j];
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (f && lx >= H && lx < H + T && ly >= H && ly < H + T && gx < n && gy < n) {
+            const int i = gy * n + gx;
+            r_out[i] = r_new;
+            s_out[i] = s_new;
+            if (V.rs) stv(V.rs, i, r_new);
+            if (!first) {
+                const float p_new = uv[j] + beta * pv[j];
+                V.pd[i] = p_new;
+                V.x[i] = xv[j] + alpha * p_new;
+            }
+        }
+    }
+    __syncthreads();
+    for (int s = 0; s < 4; ++s) {
+        tile_half_sweep<H, T>(t, s & 1, V.eps, 1 + s, R - 1 - s);
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+        const int gx = x0 + H + k % T, gy = y0 + H + k / T;
+        if (gx < n && gy < n && t.fl[H + k / T][H + k % T]) stv(V.u, gy * n + gx, t.p[H + k / T][H + k % T]);
+    }
+    const int nc = (int)C.n;
+    const int ct = threadIdx.x;  // one thread per coarse cell of the tile
+    if (ct < (T / 2) * (T / 2)) {
+        const int cx = (int)blockIdx.x * (T / 2) + ct % (T / 2), cy = (int)blockIdx.y * (T / 2) + ct / (T / 2);
+        if (cx < nc && cy < nc) {
+            float sum = 0.0f;
+            uint32_t nf = 0;
+            for (int c = 0; c < 4; ++c) {
+                const int lx = 2 * (ct % (T / 2)) + (c & 1) + H, ly = 2 * (ct / (T / 2)) + (c >> 1) + H;
+                if (!t.fl[ly][lx]) continue;
+                float s = 0.0f, kk = 0.0f;
+                if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
+                if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
+                if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
+                if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
+                sum += t.rhs[ly][lx] - (s - (kk + V.eps) * t.p[ly][lx]);
+                nf += 1u;
+            }
+            const int ci = cy * nc + cx;
+            stv(C.rhs, ci, nf > 0u ? 0.5f * sum : 0.0f);
+            stv(C.p, ci, 0.0f);
+        }
+    }
+}
+
+// Level 0 of the V-cycle, upward: prolongation on load and the symmetric
+// post-smoothing (black, red) x 2, as k_mg_up but with a 5-cell halo, so the
+// smoothed u is exact one ring beyond the core; then w = A u on the core
+// (from u as stored in S) and the block's partial sums of (r, u), (w, u),
+// (u, s), (p, w) and (r, r).
+template <class S>
+__global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par) {
+    if (st->done) return;
+    constexpr int H = 5, T = 32;
+    constexpr int R = T + 2 * H;
+    __shared__ MgTile<H, T> t;
+    const float* r_cur = V.r[1 - par];
+    MgLevelT<S> L;
+    L.n = V.n;
+    L.eps = V.eps;
+    L.p = V.u;
+    L.rhs = V.rs ? V.rs : reinterpret_cast<S*>(const_cast<float*>(r_cur));
+    L.fluid = V.fl;
+    const int n = (int)V.n;
+    const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
+    tile_load<H, T, S>(t, L, x0, y0, &C);
+    __syncthreads();
+    for (int s = 0; s < 4; ++s) {
+        tile_half_sweep<H, T>(t, 1 - (s & 1), V.eps, 1 + s, R - 1 - s);
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < (T + 2) * (T + 2); k += blockDim.x) {
+        const int ly = H - 1 + k / (T + 2), lx = H - 1 + k % (T + 2);
+        t.p[ly][lx] = round_s(t.p[ly][lx], V.u);
+    }
+    __syncthreads();
+    const float* s_cur = V.s[1 - par];
+    double gam = 0.0, del = 0.0, e = 0.0, f = 0.0, rr = 0.0;
+    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+        const int ly = H + k / T, lx = H + k % T;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx >= n || gy >= n || !t.fl[ly][lx]) continue;
+        const int i = gy * n + gx;
+        const float u = t.p[ly][lx];
+        float s = 0.0f, kk = 0.0f;
+        if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
+        if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
+        if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
+        if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
+        const float wv = s - (kk + V.eps) * u;
+        stv(V.u, i, u);
+        V.w[i] = wv;
+        const float r = V.rs ? r_cur[i] : t.rhs[ly][lx];
+        gam += (double)r * (double)u;
+        del += (double)wv * (double)u;
+        e += (double)u * (double)s_cur[i];
+        f += (double)V.pd[i] * (double)wv;
+        rr += (double)r * (double)r;
+    }
+    block_partial<5>({gam, del, e, f, rr}, st->part);
+}
+
+// ||b - A x||^2 over fluid cells, for the reported residual.
+template <class S>
+__global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st) {
+    PCG_XY
+    double rr = 0.0;
+    if (f) {
+        const float r = V.b[i] - apply_A(V.fl, V.x, n, x, y, V.eps);
+        rr = (double)r * (double)r;
+    }
+    block_partial<2>({rr, 0.0}, st->part);
+}
+
+// The scalar recurrences of the solve, from the partials of the kernel
+// before: one block, launched after each reduction kernel.
+enum PcgFin { FIN_INIT, FIN_CG0, FIN_CG, FIN_FINAL };
+template <int W>
+__global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit) {
+    if ((W == FIN_CG0 || W == FIN_CG) && st->done) return;
+    constexpr int N = (W == FIN_CG0 || W == FIN_CG) ? 5 : 2;
+    double t[N];
+    sum_partials<N>(st->part, nb, t);
+    if (threadIdx.x != 0) return;
+    if (W == FIN_INIT) {
+        st->bb = t[0];
+        const double rel = t[0] > 0.0 ? sqrt(t[1] / t[0]) : 0.0;
         st->rel = rel;
         st->it = 0u;
         st->done = (rel <= (double)tol || maxit == 0u) ? 1u : 0u;
-        st->rho = 0.0;
+        st->gamma = 0.0;
+        st->pi = 0.0;
+        st->alpha = 0.0;
+        st->beta = 0.0;
         st->tol_seen = tol;
-    }
-}
-
-// rho_new = z . r; beta = rho_new / rho (0 on the first iteration).
-template <class S>
-__global__ __launch_bounds__(256) void k_pcg_rho(PcgVec<S> V, PcgState* st) {
-    if (st->done) return;
-    PCG_XY
-    double zr = 0.0;
-    if (f) zr = (double)ldv(V.z, i) * (double)V.r[i];
-    double tot[2];
-    if (block_reduce2(zr, 0.0, st->part, &st->count, tot)) {
-        st->beta = st->it == 0u ? 0.0 : (st->rho != 0.0 ? tot[0] / st->rho : 0.0);
-        st->rho = tot[0];
-    }
-}
-
-// d = z + beta d, q = A z + beta q; d . q.
-template <class S>
-__global__ __launch_bounds__(256) void k_pcg_dir(PcgVec<S> V, PcgState* st) {
-    if (st->done) return;
-    PCG_XY
-    const float beta = (float)st->beta;
-    const bool first = st->it == 0u;
-    double dq = 0.0;
-    if (in) {
-        float d = 0.0f, q = 0.0f;
-        if (f) {
-            const float az = apply_A_s(V.fl, V.z, n, x, y, V.eps);
-            const float z = ldv(V.z, i);
-            d = first ? z : z + beta * V.d[i];
-            q = first ? az : az + beta * V.q[i];
-            dq = (double)d * (double)q;
-        }
-        V.d[i] = d;
-        V.q[i] = q;
-    }
-    double tot[2];
-    if (block_reduce2(dq, 0.0, st->part, &st->count, tot)) st->alpha = tot[0] != 0.0 ? st->rho / tot[0] : 0.0;
-}
-
-// p += alpha d, r -= alpha q, z = 0; ||r||^2 and the convergence test.
-template <class S>
-__global__ __launch_bounds__(256) void k_pcg_update(PcgVec<S> V, PcgState* st, float tol, uint32_t maxit) {
-    if (st->done) return;
-    PCG_XY
-    const float alpha = (float)st->alpha;
-    double rr = 0.0;
-    if (f) {
-        V.p[i] += alpha * V.d[i];
-        const float r = V.r[i] - alpha * V.q[i];
-        V.r[i] = r;
-        if (V.rs) stv(V.rs, i, r);
-        stv(V.z, i, 0.0f);
-        rr = (double)r * (double)r;
-    }
-    double tot[2];
-    if (block_reduce2(rr, 0.0, st->part, &st->count, tot)) {
+    } else if (W == FIN_CG0) {  // first application: p' = u, s' = w, pi = delta
+        st->gamma = t[0];
+        st->pi = t[1];
+        st->alpha = t[1] != 0.0 ? t[0] / t[1] : 0.0;
+        st->beta = 0.0;
+    } else if (W == FIN_CG) {
         const uint32_t it = st->it + 1u;
         st->it = it;
-        const double rel = st->bb > 0.0 ? sqrt(tot[0] / st->bb) : 0.0;
+        const double rel = st->bb > 0.0 ? sqrt(t[4] / st->bb) : 0.0;
         st->rel = rel;
-        if (rel <= (double)tol || it >= maxit) st->done = 1u;
-    }
-}
-
-// Final true residual ||b - A p|| / ||b|| of the solve, for the statistics.
-template <class S>
-__global__ __launch_bounds__(256) void k_pcg_final(PcgVec<S> V, PcgState* st) {
-    PCG_XY
-    double rr = 0.0;
-    if (f) {
-        const float r = V.b[i] - apply_A(V.fl, V.p, n, x, y, V.eps);
-        rr = (double)r * (double)r;
-    }
-    double tot[2];
-    if (block_reduce2(rr, 0.0, st->part, &st->count, tot)) {
-        st->true_rel = st->bb > 0.0 ? sqrt(tot[0] / st->bb) : 0.0;
+        if (rel <= (double)tol || it >= maxit) {
+            st->done = 1u;
+        } else {
+            const double g = t[0], beta = st->gamma != 0.0 ? g / st->gamma : 0.0;
+            const double pi = t[1] + beta * (t[2] + t[3]) + beta * beta * st->pi;
+            st->beta = beta;
+            st->pi = pi;
+            st->alpha = pi != 0.0 ? g / pi : 0.0;
+            st->gamma = g;
+        }
+    } else {
+        st->true_rel = st->bb > 0.0 ? sqrt(t[0] / st->bb) : 0.0;
         st->steps += 1u;
         st->iters += st->it;
         // a solve that stopped on its iteration slots before converging
@@ -1576,15 +1736,12 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
 }
 
 // ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state
-// (wrapping, as the reference's atomics).  Wave reductions, one partial per
-// block in HBM, and the last block to finish sums the partials; it
-// publishes the totals, optionally checks the per-channel act + spent
-// totals against the armed values, and resets the block counter.
-__global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led,
-                                                uint32_t epoch, const uint32_t* ep, int check) {
-    if (ep) epoch += *ep;
+// (wrapping, as the reference's atomics).  k_ledger leaves one partial per
+// block; k_ledger_fin (one block, the next launch) sums them in a fixed
+// order, publishes the totals and, when asked, checks the per-channel
+// act + spent totals against the armed values.
+__global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led) {
     __shared__ uint32_t sh[4][8];
-    __shared__ bool last;
     uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
         const uint4 v = *reinterpret_cast<const uint4*>(chem + 4u * i);
@@ -1600,43 +1757,35 @@ __global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ che
         for (int k = 0; k < 8; ++k) sh[wid][k] = a[k];
     __syncthreads();
     if (threadIdx.x < 8) led->part[blockIdx.x * 8 + threadIdx.x] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        last = atomicAdd(&led->blocks_done, 1u) + 1u == gridDim.x;
-    }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    // every thread sums a strided subset of the partials, then a fixed tree
+}
+
+__global__ __launch_bounds__(256) void k_ledger_fin(LedgerDev* led, uint32_t nb, uint32_t epoch, const uint32_t* ep, int check) {
+    if (ep) epoch += *ep;
+    __shared__ uint32_t sh[4][8];
     uint32_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 256u)
-        for (int k = 0; k < 8; ++k) t8[k] += __hip_atomic_load(led->part + b * 8 + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (uint32_t b = threadIdx.x; b < nb; b += 256u)
+        for (int k = 0; k < 8; ++k) t8[k] += led->part[b * 8 + k];
     for (int k = 0; k < 8; ++k)
         for (int o = 32; o > 0; o >>= 1) t8[k] += __shfl_down(t8[k], o, 64);
-    __syncthreads();
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
     if (lane == 0)
         for (int k = 0; k < 8; ++k) sh[wid][k] = t8[k];
     __syncthreads();
-    uint32_t tot = 0u;
-    if (threadIdx.x < 8) tot = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
-    if (threadIdx.x < 8) {
-        led->last[threadIdx.x] = tot;
-        sh[0][threadIdx.x] = tot;
+    if (threadIdx.x != 0) return;
+    uint32_t tot[8];
+    for (int k = 0; k < 8; ++k) {
+        tot[k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
+        led->last[k] = tot[k];
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        if (check) {
-            if (!led->armed) {
-                for (int c = 0; c < 4; ++c) led->expected[c] = sh[0][c] + sh[0][4 + c];
-                led->armed = 1u;
-            } else {
-                bool ok = true;
-                for (int c = 0; c < 4; ++c) ok = ok && (led->expected[c] == sh[0][c] + sh[0][4 + c]);
-                if (!ok) atomicMin(&led->first_bad, epoch);
-            }
+    if (check) {
+        if (!led->armed) {
+            for (int c = 0; c < 4; ++c) led->expected[c] = tot[c] + tot[4 + c];
+            led->armed = 1u;
+        } else {
+            bool ok = true;
+            for (int c = 0; c < 4; ++c) ok = ok && (led->expected[c] == tot[c] + tot[4 + c]);
+            if (!ok && epoch < led->first_bad) led->first_bad = epoch;
         }
-        led->blocks_done = 0u;
     }
 }
 
@@ -1702,15 +1851,27 @@ void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, 
 }
 template <class S>
 void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done) {
-    const uint32_t nt = (L.n + MG_T - 1) / MG_T;
-    if (sym) k_mg_down<S, true><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
-    else k_mg_down<S, false><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    if (L.n <= (uint32_t)MG_SMALL) {
+        const uint32_t nt = (L.n + 15) / 16;
+        if (sym) k_mg_down<S, true, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+        else k_mg_down<S, false, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    } else {
+        const uint32_t nt = (L.n + 31) / 32;
+        if (sym) k_mg_down<S, true, 32><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+        else k_mg_down<S, false, 32><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    }
 }
 template <class S>
 void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done) {
-    const uint32_t nt = (L.n + MG_T - 1) / MG_T;
-    if (sym) k_mg_up<S, true><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
-    else k_mg_up<S, false><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    if (L.n <= (uint32_t)MG_SMALL) {
+        const uint32_t nt = (L.n + 15) / 16;
+        if (sym) k_mg_up<S, true, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+        else k_mg_up<S, false, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    } else {
+        const uint32_t nt = (L.n + 31) / 32;
+        if (sym) k_mg_up<S, true, 32><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+        else k_mg_up<S, false, 32><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
+    }
 }
 size_t mg_block_lds_bytes(const uint32_t* n, int count) {
     size_t cells = 0;
@@ -1740,31 +1901,39 @@ void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, 
     if (sym) k_mg_block<S, true><<<1, 1024, lds, s>>>(a, done);
     else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }
-template <class S> void launch_pcg_init(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
-    k_pcg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st, tol, maxit);
+inline uint32_t pcg_blocks(uint32_t n) { return ((n + 15) / 16) * ((n + 15) / 16); }
+inline uint32_t cg_tiles(uint32_t n) { return (n + 31) / 32; }
+template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
+    k_cg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+    k_pcg_fin<FIN_INIT><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), tol, maxit);
 }
-template <class S> void launch_pcg_rho(hipStream_t s, const PcgVec<S>& V, PcgState* st) {
-    k_pcg_rho<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+template <class S>
+void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first) {
+    const uint32_t nt = cg_tiles(V.n);
+    k_cg_down0<S><<<dim3(nt, nt), 256, 0, s>>>(V, C, st, parity & 1, first);
 }
-template <class S> void launch_pcg_dir(hipStream_t s, const PcgVec<S>& V, PcgState* st) {
-    k_pcg_dir<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+template <class S>
+void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first,
+                   float tol, uint32_t maxit) {
+    const uint32_t nt = cg_tiles(V.n);
+    k_cg_up0<S><<<dim3(nt, nt), 256, 0, s>>>(V, C, st, parity & 1);
+    if (first) k_pcg_fin<FIN_CG0><<<1, 256, 0, s>>>(st, nt * nt, tol, maxit);
+    else k_pcg_fin<FIN_CG><<<1, 256, 0, s>>>(st, nt * nt, tol, maxit);
 }
-template <class S> void launch_pcg_update(hipStream_t s, const PcgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
-    k_pcg_update<S><<<grid2(V.n), B16, 0, s>>>(V, st, tol, maxit);
-}
-template <class S> void launch_pcg_final(hipStream_t s, const PcgVec<S>& V, PcgState* st) {
-    k_pcg_final<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st) {
+    k_cg_final<S><<<grid2(V.n), B16, 0, s>>>(V, st);
+    k_pcg_fin<FIN_FINAL><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), 0.0f, 0u);
 }
 #define RIBO_INSTANTIATE(S)                                                                                         \
     template void launch_mg_down<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);   \
     template void launch_mg_up<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);     \
     template void launch_mg_block<S>(hipStream_t, const MgLevelT<S>*, int, bool, const double*, const int32_t*, int, \
                                      const uint32_t*);                                                              \
-    template void launch_pcg_init<S>(hipStream_t, const PcgVec<S>&, PcgState*, float, uint32_t);                   \
-    template void launch_pcg_rho<S>(hipStream_t, const PcgVec<S>&, PcgState*);                                     \
-    template void launch_pcg_dir<S>(hipStream_t, const PcgVec<S>&, PcgState*);                                     \
-    template void launch_pcg_update<S>(hipStream_t, const PcgVec<S>&, PcgState*, float, uint32_t);                 \
-    template void launch_pcg_final<S>(hipStream_t, const PcgVec<S>&, PcgState*);
+    template void launch_cg_init<S>(hipStream_t, const CgVec<S>&, PcgState*, float, uint32_t);                     \
+    template void launch_cg_down0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int, int);        \
+    template void launch_cg_up0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int, int, float,    \
+                                   uint32_t);                                                                      \
+    template void launch_cg_final<S>(hipStream_t, const CgVec<S>&, PcgState*);
 RIBO_INSTANTIATE(float)
 RIBO_INSTANTIATE(bf16)
 #undef RIBO_INSTANTIATE
@@ -1796,7 +1965,8 @@ void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev*
                    int check) {
     uint32_t cells = g.G * g.G;
     uint32_t nb = rb_minu(blocks1(cells, 256), LEDGER_MAX_BLOCKS);
-    k_ledger<<<nb, 256, 0, s>>>(chem, cells, led, epoch, ep, check);
+    k_ledger<<<nb, 256, 0, s>>>(chem, cells, led);
+    k_ledger_fin<<<1, 256, 0, s>>>(led, nb, epoch, ep, check);
 }
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n) {
     k_epoch_add<<<1, 1, 0, s>>>(ep, n);

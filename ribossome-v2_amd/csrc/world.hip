@@ -133,10 +133,13 @@ struct RiboWorld {
     // tolerance mode (preconditioned conjugate gradients)
     PcgState* pcg = nullptr;
     double* pcg_part = nullptr;
-    float* pcg_r = nullptr;
+    float* pcg_r = nullptr;   // residual, two buffers
+    float* pcg_r1 = nullptr;
+    float* pcg_s0 = nullptr;  // A p, two buffers
+    float* pcg_s1 = nullptr;
+    float* pcg_pd = nullptr;  // search direction
+    float* pcg_w = nullptr;   // A u
     float* pcg_z = nullptr;
-    float* pcg_d = nullptr;
-    float* pcg_q = nullptr;
     bf16* pcg_r16 = nullptr;
     bf16* pcg_z16 = nullptr;
     bf16* mg_p16 = nullptr;     // coarse levels of the bfloat16 preconditioner
@@ -413,52 +416,60 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
 
 // The symmetric V-cycle of the tolerance mode on the preconditioner levels
 // (storage S), with the exact coarsest solve when its inverse exists.
-template <class S> void vcycle_sym(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done) {
+// Levels 1 .. L-1 of the symmetric V-cycle (level 0 is fused into the
+// conjugate-gradient kernels).
+template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done) {
     const size_t L = lv.size();
     hipStream_t s = w->stream;
-    const size_t b = w->mg_block_start;
+    const size_t b = std::max<size_t>(w->mg_block_start, 1);
     if (b < L) {
-        for (size_t l = 0; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+        for (size_t l = 1; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
         run(w, "mg_block", [&] { launch_mg_block<S>(s, lv.data() + b, (int)(L - b), true, w->cinv, w->ccell, w->cm, done); });
-        for (size_t l = b; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
+        for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
         return;
     }
-    // coarsest level too large for one workgroup: the tolerance mode needs
-    // the block kernel for its symmetric coarse solve
-    for (size_t l = 0; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
-    for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
+    // coarsest level too large for one workgroup: sweeps only
+    for (size_t l = 1; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+    for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
 }
 
-// Tolerance mode: preconditioned conjugate gradients on A p = div over the
-// fluid cells of level 0, warm-started from the pressure, preconditioned by
-// the symmetric V-cycle stored in S (binary32, or bfloat16 for the mixed
-// precision form); stops on ||div - A p|| / ||div|| <= mg_tol or after
-// mg_max_cycles iterations.  Every iteration's kernels return at once after
+// Tolerance mode: preconditioned conjugate gradients (Chronopoulos-Gear
+// recurrences) on A p = div over the fluid cells of level 0, warm-started
+// from the pressure, preconditioned by the symmetric V-cycle stored in S
+// (binary32, or bfloat16 for the mixed-precision form); stops on
+// ||div - A p|| / ||div|| <= mg_tol or after mg_max_cycles iterations.  The
+// first V-cycle forms u = M r for the initial residual; each of the `slots`
+// iterations is one more V-cycle whose first and last kernels carry the
+// vector updates and inner products.  Every kernel returns at once after
 // convergence, so the recorded work does not depend on host round trips.
 template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, S* rs, S* z, uint32_t slots) {
     hipStream_t s = w->stream;
-    PcgVec<S> V;
+    CgVec<S> V;
     V.n = w->F;
     V.eps = w->levels[0].eps;
     V.fl = w->mg_fluid;
-    V.p = w->p_a;
+    V.x = w->p_a;
     V.b = w->div;
-    V.r = w->pcg_r;
+    V.r[0] = w->pcg_r;
+    V.r[1] = w->pcg_r1;
+    V.s[0] = w->pcg_s0;
+    V.s[1] = w->pcg_s1;
+    V.pd = w->pcg_pd;
+    V.w = w->pcg_w;
+    V.u = z;
     V.rs = rs;
-    V.z = z;
-    V.d = w->pcg_d;
-    V.q = w->pcg_q;
     const float tol = w->st.mg_tol;
     const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
     const uint32_t* done = &w->pcg->done;
-    run(w, "pcg_init", [&] { launch_pcg_init<S>(s, V, w->pcg, tol, maxit); });
-    for (uint32_t it = 0; it < std::min(slots, maxit); ++it) {
-        vcycle_sym<S>(w, lv, done);
-        run(w, "pcg_rho", [&] { launch_pcg_rho<S>(s, V, w->pcg); });
-        run(w, "pcg_dir", [&] { launch_pcg_dir<S>(s, V, w->pcg); });
-        run(w, "pcg_update", [&] { launch_pcg_update<S>(s, V, w->pcg, tol, maxit); });
+    run(w, "pcg_init", [&] { launch_cg_init<S>(s, V, w->pcg, tol, maxit); });
+    const uint32_t n_it = std::min(slots, maxit);
+    for (uint32_t j = 0; j <= n_it; ++j) {
+        const int first = j == 0 ? 1 : 0;
+        run(w, "cg_down0", [&] { launch_cg_down0<S>(s, V, lv[1], w->pcg, (int)(j & 1u), first); });
+        vcycle_sym_inner<S>(w, lv, done);
+        run(w, "cg_up0", [&] { launch_cg_up0<S>(s, V, lv[1], w->pcg, (int)(j & 1u), first, tol, maxit); });
     }
-    run(w, "pcg_final", [&] { launch_pcg_final<S>(s, V, w->pcg); });
+    run(w, "pcg_final", [&] { launch_cg_final<S>(s, V, w->pcg); });
 }
 
 void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588
@@ -655,8 +666,9 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(&w->forces, F)) || (rc = dalloc(&w->heat, F)) || (rc = dalloc(&w->net_flux, 2 * (size_t)w->F)) ||
         (rc = dalloc(&w->cflags, F)) || (rc = dalloc(&w->normal, F)) || (rc = dalloc(&w->fill, F)) ||
         (rc = dalloc(&w->dragg, F)) || (rc = dalloc(&w->slope_f, F)) || (rc = dalloc(&w->pcg, 1)) ||
-        (rc = dalloc(&w->pcg_part, 2 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(&w->pcg_r, F)) ||
-        (rc = dalloc(&w->pcg_z, F)) || (rc = dalloc(&w->pcg_d, F)) || (rc = dalloc(&w->pcg_q, F)) ||
+        (rc = dalloc(&w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(&w->pcg_r, F)) ||
+        (rc = dalloc(&w->pcg_r1, F)) || (rc = dalloc(&w->pcg_s0, F)) || (rc = dalloc(&w->pcg_s1, F)) ||
+        (rc = dalloc(&w->pcg_pd, F)) || (rc = dalloc(&w->pcg_w, F)) || (rc = dalloc(&w->pcg_z, F)) ||
         (rc = dalloc(&w->pcg_r16, F)) || (rc = dalloc(&w->pcg_z16, F)) ||
         (rc = dalloc(&w->ledger, 1)) || (rc = dalloc(&w->scratch_u32, F)) || (rc = dalloc(&w->epoch_dev, 1)) ||
         (rc = build_mg(w))) {
@@ -691,7 +703,8 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
-    dfree(w->pcg_z); dfree(w->pcg_d); dfree(w->pcg_q); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
+    dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w);
+    dfree(w->pcg_z); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
     dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
     clear_graphs(w);
     if (w->pcg_host) (void)hipHostFree(w->pcg_host);
@@ -714,6 +727,8 @@ int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s) {
     if (!w || !s) return fail(RIBO_ERR_ARG, "null argument");
     int rc = validate_settings(s);
     if (rc) return rc;
+    if (s->multigrid && s->mg_tol > 0.0f && make_levels(w->F, std::max(s->mg_min_size, 1u)).size() < 2)
+        return fail(RIBO_ERR_UNSUPPORTED, "the tolerance mode needs at least two multigrid levels (an even fluid size above mg_min_size)");
     const bool relevel = std::max(s->mg_min_size, 1u) != std::max(w->st.mg_min_size, 1u);
     const bool arm = s->ledger_check && !w->st.ledger_check;
     w->st = *s;

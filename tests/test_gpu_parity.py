@@ -188,7 +188,10 @@ def test_tolerance_mode_matches_oracle(precision):
         assert rel <= 3e-5 and cyc < 120
         assert abs(rel - orel) <= 0.5 * orel
         assert abs(cyc - ocyc) <= max(3, ocyc // 5)
-    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-4), (ra.TEMP, 1e-5)):
+    # Pressure is fixed only up to the residual: two solves that each stop
+    # near relative residual 1e-5 differ by up to cond(A) times that in p,
+    # so it gets a looser bound than the projected velocity.
+    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-3), (ra.TEMP, 1e-5)):
         e = rel_l2(gw.download(which), ow.download(which))
         print(f"{ra.BUFFER_NAMES[which]} rel L2 vs oracle {e:.2e}", flush=True)
         assert e < tol

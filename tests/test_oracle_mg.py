@@ -162,14 +162,26 @@ def test_tolerance_mode_reaches_the_residual_by_an_independent_check():
 
 
 def test_binary64_oracle_bounds_the_binary32_fluid_error():
-    """The same tolerance-mode world run with binary32 and binary64 fields:
-    the relative L2 differences of velocity and pressure are small and are
-    what the precision form reports against."""
+    """The same tolerance-mode world run with binary32 and binary64 fields at
+    a tolerance binary32 cannot reach.  The binary64 solve reaches it by the
+    numpy residual; the binary32 one stops on its recursive residual with a
+    true residual at its floor (a few 1e-6 here: the rounding of a pressure
+    of magnitude ~100).  Velocity -- the field the world uses -- differs by
+    under 1e-4 relative L2; pressure, fixed only up to cond(A) times the
+    residual, by a few 1e-4."""
     import ribossome_amd as ra
 
-    w32, _ = _world_after_solve(0, 1e-7)
+    w32, cfg = _world_after_solve(0, 1e-7)
     w64, _ = _world_after_solve(0, 1e-7, fp64=True)
-    for which in (ra.VELOCITY, ra.PRESSURE):
+    n = cfg.fluid_size
+    fluid = (w64.download(ra.SOLID_MASK) == 0).astype(np.uint32)
+    A, f = operator(n, fluid, 1e-4)
+    for w, bound in ((w64, 1.5e-7), (w32, 1e-5)):
+        p = w.download_f64(ra.PRESSURE)
+        b = w.download_f64(ra.DIVERGENCE)
+        rel = np.linalg.norm(b[f] - A @ p[f]) / np.linalg.norm(b[f])
+        assert rel < bound, (rel, bound)
+    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 5e-4)):
         a, b = w32.download_f64(which), w64.download_f64(which)
         e = np.linalg.norm(a - b) / np.linalg.norm(b)
-        assert e < 1e-4, (which, e)
+        assert e < tol, (which, e)

@@ -44,10 +44,16 @@ __host__ __device__ inline uint16_t bf16_bits(float v) {
 }
 
 // One multigrid level resident in HBM; S is the storage type of its arrays.
+// One multigrid level.  p holds the level's unknowns at the start of a
+// V-cycle (0 below level 0) and its result at the end; pp holds them after
+// the downward pre-smoothing, for the upward pass.  The two arrays keep a
+// tile kernel that reads a halo from other tiles from reading values those
+// tiles have already rewritten.
 template <class S> struct MgLevelT {
     uint32_t n;
     float eps;
     S* p;
+    S* pp;
     S* rhs;
     const uint8_t* fluid;
 };
@@ -69,6 +75,7 @@ template <class S> struct CgVec {
     float* pd;       // search direction
     float* w;        // A u
     S* u;
+    S* zpre;         // u after the downward pre-smoothing of level 0
     S* rs;
 };
 

@@ -666,8 +666,36 @@ struct MgTile {
     uint8_t fl[R][R + 1];
 };
 
+// Sum of the fluid 4-neighbours (left, right, down, up: the reference's
+// order) and their count.  All flags and values are loaded before any is
+// tested, and a missing neighbour is skipped by a select (not by adding 0,
+// which would turn a -0 sum into +0), so the result equals the branchy
+// form bit for bit while the loads go out together instead of one dependent
+// LDS round trip per neighbour.
+struct NbSum { float s, k; };
+__device__ __forceinline__ NbSum nb_sum(bool fl, bool fr, bool fd, bool fu, float vl, float vr, float vd, float vu) {
+    float s = 0.0f, k = 0.0f;
+    s = fl ? s + vl : s;
+    k = fl ? k + 1.0f : k;
+    s = fr ? s + vr : s;
+    k = fr ? k + 1.0f : k;
+    s = fd ? s + vd : s;
+    k = fd ? k + 1.0f : k;
+    s = fu ? s + vu : s;
+    k = fu ? k + 1.0f : k;
+    return {s, k};
+}
+
+// Neighbour sum of a tile cell (the tile's halo covers every neighbour read).
+template <int H, int T>
+__device__ __forceinline__ NbSum tile_nb(const MgTile<H, T>& t, int ly, int lx) {
+    return nb_sum(t.fl[ly][lx - 1] != 0, t.fl[ly][lx + 1] != 0, t.fl[ly - 1][lx] != 0, t.fl[ly + 1][lx] != 0,
+                  t.p[ly][lx - 1], t.p[ly][lx + 1], t.p[ly - 1][lx], t.p[ly + 1][lx]);
+}
+
 // p_c <- (sum of fluid neighbours - rhs) / (k + eps), on the cells of one
-// colour inside [lo, hi) of the tile (multigrid.wgsl:35-45,60-67).
+// colour inside [lo, hi) of the tile (multigrid.wgsl:35-45,60-67); solid
+// cells keep their value.
 template <int H, int T>
 __device__ __forceinline__ void tile_half_sweep(MgTile<H, T>& t, int color, float eps, int lo, int hi) {
     const int w = hi - lo;
@@ -677,13 +705,10 @@ __device__ __forceinline__ void tile_half_sweep(MgTile<H, T>& t, int color, floa
         int lx = lo + 2 * (k % half);
         if (((lx + ly) & 1) != color) ++lx;
         if (lx >= hi) continue;
-        if (!t.fl[ly][lx]) continue;
-        float s = 0.0f, kk = 0.0f;
-        if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
-        if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
-        if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
-        if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
-        t.p[ly][lx] = (s - t.rhs[ly][lx]) / (kk + eps);
+        const bool f = t.fl[ly][lx] != 0;
+        const float pc = t.p[ly][lx], rh = t.rhs[ly][lx];
+        const NbSum nb = tile_nb(t, ly, lx);
+        t.p[ly][lx] = f ? (nb.s - rh) / (nb.k + eps) : pc;
     }
 }
 
@@ -692,7 +717,8 @@ __device__ __forceinline__ void tile_half_sweep(MgTile<H, T>& t, int color, floa
 // loop trip); values of solid or outside cells are replaced by 0 after the
 // load, so the fluid mask is not on the load's critical path.
 template <int H, int T, class S>
-__device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L, int x0, int y0, const MgLevelT<S>* C) {
+__device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L, const S* psrc, int x0, int y0,
+                                          const MgLevelT<S>* C) {
     constexpr int R = MgTile<H, T>::R;
     constexpr int PER = (R * R + 255) / 256;
     const int n = (int)L.n;
@@ -710,7 +736,7 @@ __device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L,
         if (k < R * R && gx >= 0 && gy >= 0 && gx < n && gy < n) {
             const int i = gy * n + gx;
             fv[j] = L.fluid[i];
-            pv[j] = ldv(L.p, i);
+            pv[j] = ldv(psrc, i);
             rv[j] = ldv(L.rhs, i);
             if (C) cv[j] = ldv(C->p, (gy / 2) * (int)C->n + gx / 2);
         }
@@ -737,9 +763,10 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
     constexpr int H = 5;
     __shared__ MgTile<H, MG_T> t;
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H, MG_T, S>(t, L, x0, y0, nullptr);
+    tile_load<H, MG_T, S>(t, L, L.p, x0, y0, nullptr);
     __syncthreads();
     constexpr int R = MgTile<H, MG_T>::R;
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
         tile_half_sweep<H, MG_T>(t, s & 1, L.eps, 1 + s, R - 1 - s);
         __syncthreads();
@@ -747,7 +774,7 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
     const int n = (int)L.n;
     for (int k = threadIdx.x; k < MG_T * MG_T; k += blockDim.x) {
         const int gx = x0 + H + k % MG_T, gy = y0 + H + k / MG_T;
-        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) stv(L.p, gy * n + gx, t.p[H + k / MG_T][H + k % MG_T]);
+        if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) stv(L.pp, gy * n + gx, t.p[H + k / MG_T][H + k % MG_T]);
     }
     const int nc = (int)C.n;
     const int ct = threadIdx.x;  // one thread per coarse cell of the tile
@@ -758,14 +785,11 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
             uint32_t nf = 0;
             for (int c = 0; c < 4; ++c) {
                 const int lx = 2 * (ct % (MG_T / 2)) + (c & 1) + H, ly = 2 * (ct / (MG_T / 2)) + (c >> 1) + H;
-                if (!t.fl[ly][lx]) continue;
-                float s = 0.0f, kk = 0.0f;
-                if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
-                if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
-                if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
-                if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
-                sum += t.rhs[ly][lx] - (s - (kk + L.eps) * t.p[ly][lx]);
-                nf += 1u;
+                const bool f = t.fl[ly][lx] != 0;
+                const float pc = t.p[ly][lx], rh = t.rhs[ly][lx];
+                const NbSum nb = tile_nb(t, ly, lx);
+                sum = f ? sum + (rh - (nb.s - (nb.k + L.eps) * pc)) : sum;
+                nf += f ? 1u : 0u;
             }
             const int ci = cy * nc + cx;
             const float rc = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
@@ -783,9 +807,10 @@ __global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, con
     constexpr int H = 4;
     __shared__ MgTile<H, MG_T> t;
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H, MG_T, S>(t, L, x0, y0, &C);
+    tile_load<H, MG_T, S>(t, L, L.pp, x0, y0, &C);
     __syncthreads();
     constexpr int R = MgTile<H, MG_T>::R;
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
         tile_half_sweep<H, MG_T>(t, SYM ? 1 - (s & 1) : (s & 1), L.eps, 1 + s, R - 1 - s);
         __syncthreads();
@@ -818,21 +843,31 @@ struct MgBlockArgs {
     int cm;                        // unknowns of the coarsest level (0: use sweeps)
 };
 
+// The workgroup is 32 x 32 threads; level loops stride rows by 32 and, in a
+// half-sweep, cells of one colour by 64 along the row, so no index needs a
+// division and a 64 x 64 level is 2 cells per thread per half-sweep.
+// Neighbour sum of cell (x, y) of an n x n level held whole in LDS; indices
+// of neighbours outside the level are clamped to the cell (their flag is
+// false, so the value is not used).
+__device__ __forceinline__ NbSum blk_nb(const float* p, const uint8_t* fl, int n, int x, int y) {
+    const int i = y * n + x;
+    const int il = x > 0 ? i - 1 : i, ir = x + 1 < n ? i + 1 : i;
+    const int id = y > 0 ? i - n : i, iu = y + 1 < n ? i + n : i;
+    const bool gl = fl[il] != 0, gr = fl[ir] != 0, gd = fl[id] != 0, gu = fl[iu] != 0;
+    const float vl = p[il], vr = p[ir], vd = p[id], vu = p[iu];
+    return nb_sum((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, vl, vr, vd, vu);
+}
+
 __device__ __forceinline__ void blk_sweep(float* p, const float* rhs, const uint8_t* fl, int n, float eps, int color) {
-    const int half = (n + 1) / 2;
-    for (int k = threadIdx.x; k < n * half; k += blockDim.x) {
-        const int y = k / half;
-        int x = 2 * (k % half);
-        if (((x + y) & 1) != color) ++x;
-        if (x >= n) continue;
-        const int i = y * n + x;
-        if (!fl[i]) continue;
-        float s = 0.0f, kk = 0.0f;
-        if (x > 0 && fl[i - 1]) { s += p[i - 1]; kk += 1.0f; }
-        if (x + 1 < n && fl[i + 1]) { s += p[i + 1]; kk += 1.0f; }
-        if (y > 0 && fl[i - n]) { s += p[i - n]; kk += 1.0f; }
-        if (y + 1 < n && fl[i + n]) { s += p[i + n]; kk += 1.0f; }
-        p[i] = (s - rhs[i]) / (kk + eps);
+    const int tx = (int)(threadIdx.x & 31u), ty = (int)(threadIdx.x >> 5);
+    for (int y = ty; y < n; y += 32) {
+        for (int x = ((color ^ y) & 1) + 2 * tx; x < n; x += 64) {
+            const int i = y * n + x;
+            const bool f = fl[i] != 0;
+            const float pc = p[i], rh = rhs[i];
+            const NbSum nb = blk_nb(p, fl, n, x, y);
+            p[i] = f ? (nb.s - rh) / (nb.k + eps) : pc;
+        }
     }
 }
 
@@ -840,19 +875,26 @@ template <class S, bool SYM>
 __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint32_t* done) {
     if (done && *done) return;
     extern __shared__ float sm[];
+    const int tx = (int)(threadIdx.x & 31u), ty = (int)(threadIdx.x >> 5);
     const int total = a.off[a.levels];
     float* P = sm;
     float* RHS = sm + total;
     uint8_t* FL = reinterpret_cast<uint8_t*>(sm + 2 * total);
     const int n0 = a.n[0];
-    for (int i = threadIdx.x; i < n0 * n0; i += blockDim.x) {
-        const uint8_t f = a.fluid[0][i];
-        FL[i] = f;
-        P[i] = f ? ldv(a.p0, i) : 0.0f;
-        RHS[i] = f ? ldv(a.rhs0, i) : 0.0f;
+    for (int y = ty; y < n0; y += 32)
+        for (int x = tx; x < n0; x += 32) {
+            const int i = y * n0 + x;
+            const uint8_t f = a.fluid[0][i];
+            const float pv = ldv(a.p0, i), rv = ldv(a.rhs0, i);
+            FL[i] = f;
+            P[i] = f ? pv : 0.0f;
+            RHS[i] = f ? rv : 0.0f;
+        }
+    for (int l = 1; l < a.levels; ++l) {
+        const int n = a.n[l];
+        for (int y = ty; y < n; y += 32)
+            for (int x = tx; x < n; x += 32) FL[a.off[l] + y * n + x] = a.fluid[l][y * n + x];
     }
-    for (int l = 1; l < a.levels; ++l)
-        for (int i = threadIdx.x; i < a.n[l] * a.n[l]; i += blockDim.x) FL[a.off[l] + i] = a.fluid[l][i];
     __syncthreads();
     const int L = a.levels;
     for (int l = 0; l + 1 < L; ++l) {
@@ -867,25 +909,23 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
         const int nc = a.n[l + 1];
         float* pc = P + a.off[l + 1];
         float* rc = RHS + a.off[l + 1];
-        for (int ci = threadIdx.x; ci < nc * nc; ci += blockDim.x) {
-            const int gx = ci % nc, gy = ci / nc;
-            float sum = 0.0f;
-            uint32_t nf = 0;
-            for (int c = 0; c < 4; ++c) {
-                const int fx = gx * 2 + (c & 1), fy = gy * 2 + (c >> 1);
-                const int i = fy * n + fx;
-                if (!fl[i]) continue;
-                float s2 = 0.0f, kk = 0.0f;
-                if (fx > 0 && fl[i - 1]) { s2 += p[i - 1]; kk += 1.0f; }
-                if (fx + 1 < n && fl[i + 1]) { s2 += p[i + 1]; kk += 1.0f; }
-                if (fy > 0 && fl[i - n]) { s2 += p[i - n]; kk += 1.0f; }
-                if (fy + 1 < n && fl[i + n]) { s2 += p[i + n]; kk += 1.0f; }
-                sum += rhs[i] - (s2 - (kk + a.eps[l]) * p[i]);
-                nf += 1u;
+        for (int gy = ty; gy < nc; gy += 32)
+            for (int gx = tx; gx < nc; gx += 32) {
+                float sum = 0.0f;
+                uint32_t nf = 0;
+                for (int c = 0; c < 4; ++c) {
+                    const int fx = gx * 2 + (c & 1), fy = gy * 2 + (c >> 1);
+                    const int i = fy * n + fx;
+                    const bool f = fl[i] != 0;
+                    const float pc = p[i], rh = rhs[i];
+                    const NbSum nb = blk_nb(p, fl, n, fx, fy);
+                    sum = f ? sum + (rh - (nb.s - (nb.k + a.eps[l]) * pc)) : sum;
+                    nf += f ? 1u : 0u;
+                }
+                const int ci = gy * nc + gx;
+                rc[ci] = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
+                pc[ci] = 0.0f;
             }
-            rc[ci] = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
-            pc[ci] = 0.0f;
-        }
         __syncthreads();
     }
     {
@@ -947,16 +987,22 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
         const uint8_t* fl = FL + a.off[l];
         const int n = a.n[l], nc = a.n[l + 1];
         const float* pc = P + a.off[l + 1];
-        for (int i = threadIdx.x; i < n * n; i += blockDim.x)
-            if (fl[i]) p[i] += pc[((i / n) / 2) * nc + (i % n) / 2];
+        for (int y = ty; y < n; y += 32)
+            for (int x = tx; x < n; x += 32) {
+                const int i = y * n + x;
+                if (fl[i]) p[i] += pc[(y / 2) * nc + x / 2];
+            }
         __syncthreads();
         for (int s = 0; s < 4; ++s) {
             blk_sweep(p, rhs, fl, n, a.eps[l], SYM ? 1 - (s & 1) : (s & 1));
             __syncthreads();
         }
     }
-    for (int i = threadIdx.x; i < n0 * n0; i += blockDim.x)
-        if (FL[i]) stv(a.p0, i, P[i]);
+    for (int y = ty; y < n0; y += 32)
+        for (int x = tx; x < n0; x += 32) {
+            const int i = y * n0 + x;
+            if (FL[i]) stv(a.p0, i, P[i]);
+        }
 }
 
 // ---- tolerance mode: preconditioned conjugate gradients ------------------------
@@ -976,12 +1022,12 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
 // neighbours - (k + eps) v.
 __device__ __forceinline__ float apply_A(const uint8_t* fl, const float* v, int n, int x, int y, float eps) {
     const int i = y * n + x;
-    float s = 0.0f, k = 0.0f;
-    if (x > 0 && fl[i - 1]) { s += v[i - 1]; k += 1.0f; }
-    if (x + 1 < n && fl[i + 1]) { s += v[i + 1]; k += 1.0f; }
-    if (y > 0 && fl[i - n]) { s += v[i - n]; k += 1.0f; }
-    if (y + 1 < n && fl[i + n]) { s += v[i + n]; k += 1.0f; }
-    return s - (k + eps) * v[i];
+    const int il = x > 0 ? i - 1 : i, ir = x + 1 < n ? i + 1 : i;
+    const int id = y > 0 ? i - n : i, iu = y + 1 < n ? i + n : i;
+    const bool gl = fl[il] != 0, gr = fl[ir] != 0, gd = fl[id] != 0, gu = fl[iu] != 0;
+    const float vl = v[il], vr = v[ir], vd = v[id], vu = v[iu], vc = v[i];
+    const NbSum nb = nb_sum((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, vl, vr, vd, vu);
+    return nb.s - (nb.k + eps) * vc;
 }
 
 // Value after a store to S and a load back.
@@ -1138,13 +1184,14 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
         }
     }
     __syncthreads();
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
         tile_half_sweep<H, T>(t, s & 1, V.eps, 1 + s, R - 1 - s);
         __syncthreads();
     }
     for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
         const int gx = x0 + H + k % T, gy = y0 + H + k / T;
-        if (gx < n && gy < n && t.fl[H + k / T][H + k % T]) stv(V.u, gy * n + gx, t.p[H + k / T][H + k % T]);
+        if (gx < n && gy < n && t.fl[H + k / T][H + k % T]) stv(V.zpre, gy * n + gx, t.p[H + k / T][H + k % T]);
     }
     const int nc = (int)C.n;
     const int ct = threadIdx.x;  // one thread per coarse cell of the tile
@@ -1155,14 +1202,11 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
             uint32_t nf = 0;
             for (int c = 0; c < 4; ++c) {
                 const int lx = 2 * (ct % (T / 2)) + (c & 1) + H, ly = 2 * (ct / (T / 2)) + (c >> 1) + H;
-                if (!t.fl[ly][lx]) continue;
-                float s = 0.0f, kk = 0.0f;
-                if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
-                if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
-                if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
-                if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
-                sum += t.rhs[ly][lx] - (s - (kk + V.eps) * t.p[ly][lx]);
-                nf += 1u;
+                const bool f = t.fl[ly][lx] != 0;
+                const float pc = t.p[ly][lx], rh = t.rhs[ly][lx];
+                const NbSum nb = tile_nb(t, ly, lx);
+                sum = f ? sum + (rh - (nb.s - (nb.k + V.eps) * pc)) : sum;
+                nf += f ? 1u : 0u;
             }
             const int ci = cy * nc + cx;
             stv(C.rhs, ci, nf > 0u ? 0.5f * sum : 0.0f);
@@ -1171,7 +1215,8 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
     }
 }
 
-// Level 0 of the V-cycle, upward: prolongation on load and the symmetric
+// Level 0 of the V-cycle, upward: prolongation on load (of the pre-smoothed
+// u written by k_cg_down0) and the symmetric
 // post-smoothing (black, red) x 2, as k_mg_up but with a 5-cell halo, so the
 // smoothed u is exact one ring beyond the core; then w = A u on the core
 // (from u as stored in S) and the block's partial sums of (r, u), (w, u),
@@ -1187,12 +1232,14 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
     L.n = V.n;
     L.eps = V.eps;
     L.p = V.u;
+    L.pp = V.zpre;
     L.rhs = V.rs ? V.rs : reinterpret_cast<S*>(const_cast<float*>(r_cur));
     L.fluid = V.fl;
     const int n = (int)V.n;
     const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
-    tile_load<H, T, S>(t, L, x0, y0, &C);
+    tile_load<H, T, S>(t, L, V.zpre, x0, y0, &C);
     __syncthreads();
+#pragma unroll
     for (int s = 0; s < 4; ++s) {
         tile_half_sweep<H, T>(t, 1 - (s & 1), V.eps, 1 + s, R - 1 - s);
         __syncthreads();
@@ -1210,12 +1257,8 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
         if (gx >= n || gy >= n || !t.fl[ly][lx]) continue;
         const int i = gy * n + gx;
         const float u = t.p[ly][lx];
-        float s = 0.0f, kk = 0.0f;
-        if (t.fl[ly][lx - 1]) { s += t.p[ly][lx - 1]; kk += 1.0f; }
-        if (t.fl[ly][lx + 1]) { s += t.p[ly][lx + 1]; kk += 1.0f; }
-        if (t.fl[ly - 1][lx]) { s += t.p[ly - 1][lx]; kk += 1.0f; }
-        if (t.fl[ly + 1][lx]) { s += t.p[ly + 1][lx]; kk += 1.0f; }
-        const float wv = s - (kk + V.eps) * u;
+        const NbSum nb = tile_nb(t, ly, lx);
+        const float wv = nb.s - (nb.k + V.eps) * u;
         stv(V.u, i, u);
         V.w[i] = wv;
         const float r = V.rs ? r_cur[i] : t.rhs[ly][lx];

@@ -126,6 +126,7 @@ struct RiboWorld {
     std::vector<RiboMgLevel> levels;
     std::vector<MgLevelDev> mg;
     float* mg_p = nullptr;
+    float* mg_pp = nullptr;     // pre-smoothed unknowns of every level
     float* mg_rhs = nullptr;
     uint8_t* mg_fluid = nullptr;
     size_t mg_block_start = 0;  // first level solved inside one workgroup
@@ -143,6 +144,7 @@ struct RiboWorld {
     bf16* pcg_r16 = nullptr;
     bf16* pcg_z16 = nullptr;
     bf16* mg_p16 = nullptr;     // coarse levels of the bfloat16 preconditioner
+    bf16* mg_pp16 = nullptr;    // pre-smoothed unknowns, every level
     bf16* mg_rhs16 = nullptr;
     std::vector<MgLevelT<float>> mgz;   // preconditioner levels, binary32 (level 0 = z, r)
     std::vector<MgLevelT<bf16>> mgh;    // preconditioner levels, bfloat16 (level 0 = z16, r16)
@@ -267,6 +269,7 @@ void clear_graphs(RiboWorld* w) {
 int build_mg(RiboWorld* w) {
     clear_graphs(w);
     dfree(w->mg_p);
+    dfree(w->mg_pp);
     dfree(w->mg_rhs);
     dfree(w->mg_fluid);
     w->levels = make_levels(w->F, std::max(w->st.mg_min_size, 1u));
@@ -274,10 +277,13 @@ int build_mg(RiboWorld* w) {
     size_t fine = (size_t)w->F * w->F;
     size_t coarse = total - fine;
     int rc;
-    if ((rc = dalloc(&w->mg_p, coarse)) || (rc = dalloc(&w->mg_rhs, coarse)) || (rc = dalloc(&w->mg_fluid, total))) return rc;
+    if ((rc = dalloc(&w->mg_p, coarse)) || (rc = dalloc(&w->mg_pp, total)) || (rc = dalloc(&w->mg_rhs, coarse)) ||
+        (rc = dalloc(&w->mg_fluid, total)))
+        return rc;
     dfree(w->mg_p16);
+    dfree(w->mg_pp16);
     dfree(w->mg_rhs16);
-    if ((rc = dalloc(&w->mg_p16, coarse)) || (rc = dalloc(&w->mg_rhs16, coarse))) return rc;
+    if ((rc = dalloc(&w->mg_p16, coarse)) || (rc = dalloc(&w->mg_pp16, total)) || (rc = dalloc(&w->mg_rhs16, coarse))) return rc;
     w->mg.clear();
     w->mgz.clear();
     w->mgh.clear();
@@ -299,6 +305,8 @@ int build_mg(RiboWorld* w) {
             h.p = w->mg_p16 + (w->levels[l].off - fine);
             h.rhs = w->mg_rhs16 + (w->levels[l].off - fine);
         }
+        d.pp = w->mg_pp + w->levels[l].off;
+        h.pp = w->mg_pp16 + w->levels[l].off;
         d.fluid = w->mg_fluid + w->levels[l].off;
         h.fluid = d.fluid;
         w->mg.push_back(d);
@@ -414,8 +422,6 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
     for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done); });
 }
 
-// The symmetric V-cycle of the tolerance mode on the preconditioner levels
-// (storage S), with the exact coarsest solve when its inverse exists.
 // Levels 1 .. L-1 of the symmetric V-cycle (level 0 is fused into the
 // conjugate-gradient kernels).
 template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done) {
@@ -457,6 +463,7 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
     V.pd = w->pcg_pd;
     V.w = w->pcg_w;
     V.u = z;
+    V.zpre = lv[0].pp;
     V.rs = rs;
     const float tol = w->st.mg_tol;
     const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
@@ -702,7 +709,7 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
-    dfree(w->mg_p); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
+    dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
     dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w);
     dfree(w->pcg_z); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
     dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);

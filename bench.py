@@ -35,10 +35,9 @@ GRID, FLUID = 2048, 1024
 SEED = 2
 METRIC = "world cell-steps/s at 2048^2 (mixed vs FP32, HBM %)"
 
-# Algorithmic bytes per unit of work of each kernel (reads + writes of the
-# arrays it must touch once; stencil halos and gathers counted once), per
-# fluid cell (F) or grid cell (G).  These are the per-kernel terms of the
-# byte model of the world step.
+# Algorithmic bytes of each kernel: every array it must touch read or written
+# once (stencil halos and bilinear gathers are cache traffic; an atomic add
+# counts as a read and a write).  Units: per fluid cell (F), per grid cell (G).
 BYTES_PER_UNIT = {
     "update_temperature": (56, "F"),
     "buoyancy_gather": (12, "F"),
@@ -53,9 +52,51 @@ BYTES_PER_UNIT = {
     "thermal_activation": (17, "G"),
     "agg": (20, "G"),
     "clear_next": (16, "G"),
-    "transport_scatter": (64, "G"),
+    # chem 16 + gamma 4 + aggregation 4 + light 1 + velocity 2 + temperature 1
+    # read, chem_next 16 read + 16 written by the atomics
+    "transport_scatter": (60, "G"),
     "ledger": (16, "G"),
+    # tolerance-mode solver, level 0 (S = bytes of the preconditioner's storage)
+    "pcg_init": (33, "F"),
+    "pcg_final": (9, "F"),
 }
+MG_LDS_LIMIT = 65536
+MG_BLOCK_LEVELS = 8
+
+
+def block_start(sizes):
+    """First multigrid level solved inside one workgroup (the library's rule)."""
+    b = len(sizes)
+    while b > 0 and len(sizes) - (b - 1) <= MG_BLOCK_LEVELS and 9 * sum(n * n for n in sizes[b - 1:]) <= MG_LDS_LIMIT:
+        b -= 1
+    return b
+
+
+def kernel_bytes(name, cfg, sizes, mode):
+    """Algorithmic bytes of one launch (the mean over one V-cycle's launches
+    for the per-level multigrid kernels)."""
+    S = 2 if MODES[mode]["precision"] == 1 else 4
+    tol = MODES[mode]["mg_tol"] > 0
+    b = max(block_start(sizes), 1) if tol else block_start(sizes)
+    lo = 1 if tol else 0
+    F, G = cfg.fluid_cells(), cfg.cells()
+    if name in BYTES_PER_UNIT:
+        per, unit = BYTES_PER_UNIT[name]
+        return per * (G if unit == "G" else F)
+    if name == "mg_down":  # fluid 1, p and rhs read, pre-smoothed p written; coarse rhs, p
+        levels = sizes[lo:b]
+        return sum((1 + 3 * S + S / 2) * n * n for n in levels) / max(len(levels), 1)
+    if name == "mg_up":  # fluid 1, pre-smoothed p and rhs read, coarse p, p written
+        levels = sizes[lo:b]
+        return sum((1 + 3 * S + S / 4) * n * n for n in levels) / max(len(levels), 1)
+    if name == "mg_block":  # top level's p, rhs, fluid in, p out; coarser masks
+        n = sizes[b]
+        return (1 + 3 * S) * n * n + sum(m * m for m in sizes[b + 1:])
+    if name == "cg_down0":  # fluid, r, s, w (halo), u, p, x; r, s, p, x, pre-smoothed u, bf16 r; coarse rhs, p
+        return (1 + 12 + S + 8 + 16 + S + (S if S == 2 else 0) + S / 2) * F
+    if name == "cg_up0":  # fluid, pre-smoothed u, rhs, coarse u, s, p (and r when bf16); u, w
+        return (1 + S + (S if S == 2 else 4) + S / 4 + 8 + (4 if S == 2 else 0) + S + 4) * F
+    return 0.0
 
 
 def world_inputs(cfg):
@@ -158,6 +199,65 @@ def cpu_baseline(cfg, inputs, args):
     }
 
 
+# HIP kernel names of the timing names, for the committed rocprofv3 summaries
+HIP_NAMES = {
+    "transport_scatter": "k_scatter", "mg_down": "k_mg_down", "mg_up": "k_mg_up", "mg_block": "k_mg_block",
+    "cg_down0": "k_cg_down0", "cg_up0": "k_cg_up0", "net_flux": "k_net_flux", "vorticity": "k_vorticity",
+    "agg": "k_agg", "ledger": "k_ledger", "clear_next": "k_fill_u32", "add_forces": "k_add_forces",
+    "diffuse_velocity": "k_diffuse", "advect_velocity": "k_advect", "update_temperature": "k_temperature",
+    "divergence": "k_divergence", "subtract_gradient": "k_subgrad", "thermal_activation": "k_thermal",
+    "enforce_boundaries": "k_boundaries", "buoyancy_gather": "k_forces",
+}
+PMC_SUMMARY = os.path.join(ROOT, "profiles", "world_step_2048_hbm_counters.md")
+
+
+def pmc_traffic(kernel, mode):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary of this workload (KiB per dispatch, averaged over the
+    dispatches; both counters read 1.0x the bytes of a fill and of a cold
+    read of known size on MI355X), or None when it has no row for it."""
+    name = HIP_NAMES.get(kernel)
+    if not name or not os.path.exists(PMC_SUMMARY):
+        return None
+    want = "true" if MODES[mode]["mg_tol"] > 0 else "false"
+    kib, calls = 0.0, 0
+    for line in open(PMC_SUMMARY):
+        cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cols) < 5 or not (cols[0] == name or cols[0].startswith(name + "<")):
+            continue
+        if "<" in cols[0] and name.startswith("k_mg_") and want not in cols[0]:
+            continue
+        try:
+            k, n = float(cols[2]) + float(cols[3]), int(cols[4])
+        except ValueError:
+            continue
+        kib += k * n
+        calls += n
+    return kib * 1024.0 / calls if calls else None
+
+
+def step_model_bytes(cfg, sizes, mode, res):
+    """Algorithmic bytes of one world step: the per-step kernels once, the
+    solver's kernels once per V-cycle it ran."""
+    F, G = cfg.fluid_cells(), cfg.cells()
+    once = ["update_temperature", "buoyancy_gather", "add_forces", "diffuse_velocity", "advect_velocity",
+            "vorticity", "divergence", "subtract_gradient", "net_flux", "enforce_boundaries", "thermal_activation",
+            "agg", "clear_next", "transport_scatter", "ledger"]
+    total = sum(kernel_bytes(k, cfg, sizes, mode) for k in once)
+    tol = MODES[mode]["mg_tol"] > 0
+    b = max(block_start(sizes), 1) if tol else block_start(sizes)
+    nl = b - (1 if tol else 0)
+    vcycle = nl * (kernel_bytes("mg_down", cfg, sizes, mode) + kernel_bytes("mg_up", cfg, sizes, mode))
+    vcycle += kernel_bytes("mg_block", cfg, sizes, mode) if b < len(sizes) else 0.0
+    if tol:
+        vcycle += kernel_bytes("cg_down0", cfg, sizes, mode) + kernel_bytes("cg_up0", cfg, sizes, mode)
+        v = (res.get("mean_iterations") or 0.0) + 1.0
+        total += kernel_bytes("pcg_init", cfg, sizes, mode) + kernel_bytes("pcg_final", cfg, sizes, mode)
+    else:
+        v = 1.0
+    return total + v * vcycle
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,12 +304,13 @@ def main():
     w.run(args.profile_steps)
     times = w.kernel_times()
     w.set_timing(False)
+    sizes = [lv[0] for lv in w.mg_levels()]
     dom = max(times, key=lambda k: times[k][0])
     tot_ms, launches = times[dom]
     avg_s = tot_ms / launches / 1000.0
-    per, unit = BYTES_PER_UNIT.get(dom, (0, "G"))
-    units = cfg.cells() if unit == "G" else cfg.fluid_cells()
-    achieved = per * units / avg_s / 1e9 if per else None
+    per_launch = kernel_bytes(dom, cfg, sizes, head)
+    achieved = per_launch / avg_s / 1e9 if per_launch else None
+    traffic = pmc_traffic(dom, head)
     roofline = {
         "kernel": dom,
         "bound": "hbm",
@@ -217,12 +318,16 @@ def main():
         "peak": 8000.0,
         "unit": "GB/s",
         "frac": (achieved / 8000.0) if achieved else None,
-        "traffic": None,
+        "traffic": traffic,
         "avg_launch_us": avg_s * 1e6,
-        "bytes_per_unit": per,
-        "per": unit,
+        "launches": launches,
+        "algorithmic_bytes_per_launch": per_launch,
     }
     kernels = {k: [round(v[0] / max(v[1], 1) * 1000.0, 2), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
+    step_bytes = {m: step_model_bytes(cfg, sizes, m, results[m]) for m in results}
+    hbm = {m: {"algorithmic_bytes_per_step": step_bytes[m],
+               "GB_per_s": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 1e9,
+               "frac_of_8TBps": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 8e12} for m in results}
 
     cpu = None
     if not args.no_cpu:
@@ -254,6 +359,7 @@ def main():
         "modes": results,
         "mixed_vs_fp32": (results["mixed"]["value"] / results["fp32"]["value"]) if "mixed" in results and "fp32" in results else None,
         "roofline": roofline,
+        "hbm_step_model": hbm,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,
     }

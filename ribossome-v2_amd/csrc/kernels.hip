@@ -1646,8 +1646,10 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         }
         const float dlen = len2(disp);
         const bool jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
+        // smooth_start is a function of the cell alone; every bound mover of
+        // the cell starts there (transport.wgsl:95-117,290-296)
         float2 bound_start = f2(0.0f, 0.0f);
-        bool have_bound_start = false;
+        if (p_bound > 0.0f) bound_start = smooth_start(P, x, y);
         uint32_t* bins = s_bins + tid;
         const int lself = ((int)y - wy0) * SC_W + ((int)x - wx0);
 
@@ -1700,10 +1702,6 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
                 const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
                 const F4 rf = u4_to_f4(mv.at_current());
                 const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
-                if (bound && !have_bound_start) {
-                    bound_start = smooth_start(P, x, y);
-                    have_bound_start = true;
-                }
                 float px = bound ? bound_start.x : rf.x;
                 float py = bound ? bound_start.y : rf.y;
                 px += disp.x;

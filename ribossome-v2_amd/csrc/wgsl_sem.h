@@ -54,19 +54,18 @@ RB_FN float rb_smoothstep01(float x) {
 }
 
 /* WGSL u32(f32): truncation toward zero, saturated to [0, 4294967040]
- * (the largest binary32 value below 2^32); NaN gives 0. */
+ * (the largest binary32 value below 2^32); NaN gives 0.  Written as a clamp
+ * then a select so that GPU code has no branches (the clamp maps NaN to a
+ * bound and the select replaces it). */
 RB_FN uint32_t rb_f2u(float x) {
-    if (!(x > 0.0f)) return 0u;
-    if (x >= 4294967040.0f) return 4294967040u;
-    return (uint32_t)x;
+    const float c = fminf(fmaxf(x, 0.0f), 4294967040.0f);
+    return x > 0.0f ? (uint32_t)c : 0u;
 }
 /* WGSL i32(f32): truncation toward zero, saturated to
  * [-2147483648, 2147483520]; NaN gives 0. */
 RB_FN int32_t rb_f2i(float x) {
-    if (x != x) return 0;
-    if (x <= -2147483648.0f) return (int32_t)0x80000000u;
-    if (x >= 2147483520.0f) return 2147483520;
-    return (int32_t)x;
+    const float c = fminf(fmaxf(x, -2147483648.0f), 2147483520.0f);
+    return x == x ? (int32_t)c : 0;
 }
 /* Wrapping i32 add (WGSL integer arithmetic wraps). */
 RB_FN int32_t rb_addi(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }

@@ -847,7 +847,8 @@ template <class R> struct MG {
     R* rhs;
     uint32_t* fluid;
     bool bf16 = false;              // round stores of the preconditioner to bfloat16
-    const CoarseInv* cinv = nullptr;  // exact coarsest-level solve (tolerance mode)
+    const CoarseInv* cinv = nullptr;  // exact solve of the last level of vcycle_sym (tolerance mode)
+    size_t sym_levels = 0;            // levels vcycle_sym descends through (0: all)
     bool is_fluid(const Level& L, int32_t x, int32_t y) const {  // :29-32
         if (x < 0 || y < 0 || x >= (int32_t)L.n || y >= (int32_t)L.n) return false;
         return fluid[L.off + (uint32_t)y * L.n + (uint32_t)x] != 0u;
@@ -928,7 +929,7 @@ template <class R> struct MG {
     // preconditioner, as conjugate gradients requires.
     void vcycle_sym() {
         const std::vector<Level>& L = *lv;
-        const size_t levels = L.size();
+        const size_t levels = sym_levels ? std::min(sym_levels, L.size()) : L.size();
         for (size_t l = 0; l + 1 < levels; ++l) {
             for (uint32_t s2 = 0; s2 < MG_PRE; ++s2) { smooth(L[l], 0); smooth(L[l], 1); }
             round_level(L[l], p);
@@ -1369,6 +1370,7 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
                     nf += w.mg_fluid[lv[l].off + (gy * 2u + (c >> 1)) * lv[l].n + gx * 2u + (c & 1u)] ? 1u : 0u;
                 w.mg_fluid[lv[l].off_c + gy * lv[l].n_c + gx] = nf > 0 ? 1u : 0u;
             }
+    mg.sym_levels = lv.size();
     CoarseInv ci;
     const Level& C = lv.back();
     mg.cinv = ci.build(C.n, w.mg_fluid.data() + C.off, C.eps) ? &ci : nullptr;

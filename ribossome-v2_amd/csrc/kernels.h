@@ -79,12 +79,14 @@ template <class S> struct CgVec {
     S* rs;
 };
 
+constexpr uint32_t PCG_HOST_RING = 4, PCG_HOST_RING_LEN = 64;
 struct PcgState {
     double bb, gamma, pi, beta, alpha, rel, true_rel;
     uint32_t it, done, steps, iters, missed;
     float tol_seen;
     double* part;       // per-block partial sums (up to 5 per block of 16 x 16 cells)
-    uint32_t* host_out; // host-mapped words: last iterations, misses, solves
+    uint32_t* host_out; // host-mapped words: last iterations, misses, solves, then the
+                        // iterations of the last PCG_HOST_RING_LEN solves (a ring)
 };
 
 constexpr uint32_t LEDGER_MAX_BLOCKS = 1024;
@@ -128,6 +130,10 @@ template <class S> void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, cons
                                        const uint32_t* done);
 template <class S> void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym,
                                      const uint32_t* done);
+// The tolerance mode solves its coarsest level exactly (dense inverse) when
+// it has at most MG_SYM_COARSE_MAX fluid cells; the workgroup kernel stages
+// that right-hand side in LDS.
+constexpr uint32_t MG_SYM_COARSE_MAX = 1024;
 size_t mg_block_lds_bytes(const uint32_t* n, int count);
 template <class S> void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv,
                                         const int32_t* ccell, int cm, const uint32_t* done);

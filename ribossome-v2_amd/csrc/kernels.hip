@@ -858,10 +858,23 @@ __device__ __forceinline__ NbSum blk_nb(const float* p, const uint8_t* fl, int n
     return nb_sum((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, vl, vr, vd, vu);
 }
 
-__device__ __forceinline__ void blk_sweep(float* p, const float* rhs, const uint8_t* fl, int n, float eps, int color) {
-    const int tx = (int)(threadIdx.x & 31u), ty = (int)(threadIdx.x >> 5);
-    for (int y = ty; y < n; y += 32) {
-        for (int x = ((color ^ y) & 1) + 2 * tx; x < n; x += 64) {
+// The threads of k_mg_block as a 32 x 32 team: level loops stride rows by
+// 32 and, in a half-sweep, cells of one colour by 64 along the row, so no
+// index needs a division.
+template <int TW, int TH>
+struct Team {
+    int tx, ty;
+    __device__ explicit Team(int t) : tx(t % TW), ty(t / TW) {}
+    __device__ void sync() const { __syncthreads(); }
+};
+
+// One half-sweep of an n x n level held whole in LDS: rows stride by TH,
+// cells of the colour by 2 TW along the row.
+template <int TW, int TH>
+__device__ __forceinline__ void blk_sweep(const Team<TW, TH>& tm, float* p, const float* rhs, const uint8_t* fl, int n,
+                                          float eps, int color) {
+    for (int y = tm.ty; y < n; y += TH) {
+        for (int x = ((color ^ y) & 1) + 2 * tm.tx; x < n; x += 2 * TW) {
             const int i = y * n + x;
             const bool f = fl[i] != 0;
             const float pc = p[i], rh = rhs[i];
@@ -871,18 +884,141 @@ __device__ __forceinline__ void blk_sweep(float* p, const float* rhs, const uint
     }
 }
 
+// 2 x (red, black) and the restriction of level l into level l + 1.
+template <bool SYM, int TW, int TH, class S>
+__device__ void blk_down(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, float* RHS, const uint8_t* FL, int l) {
+    float* p = P + a.off[l];
+    const float* rhs = RHS + a.off[l];
+    const uint8_t* fl = FL + a.off[l];
+    const int n = a.n[l];
+    for (int s = 0; s < 4; ++s) {
+        blk_sweep(tm, p, rhs, fl, n, a.eps[l], s & 1);
+        tm.sync();
+    }
+    const int nc = a.n[l + 1];
+    float* pc = P + a.off[l + 1];
+    float* rc = RHS + a.off[l + 1];
+    for (int gy = tm.ty; gy < nc; gy += TH)
+        for (int gx = tm.tx; gx < nc; gx += TW) {
+            float sum = 0.0f;
+            uint32_t nf = 0;
+            for (int c = 0; c < 4; ++c) {
+                const int fx = gx * 2 + (c & 1), fy = gy * 2 + (c >> 1);
+                const int i = fy * n + fx;
+                const bool f = fl[i] != 0;
+                const float pcv = p[i], rh = rhs[i];
+                const NbSum nb = blk_nb(p, fl, n, fx, fy);
+                sum = f ? sum + (rh - (nb.s - (nb.k + a.eps[l]) * pcv)) : sum;
+                nf += f ? 1u : 0u;
+            }
+            const int ci = gy * nc + gx;
+            rc[ci] = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
+            pc[ci] = 0.0f;
+        }
+    tm.sync();
+}
+
+// Prolongation from level l + 1 and 2 x (red, black) -- (black, red) when
+// SYM -- on level l.
+template <bool SYM, int TW, int TH, class S>
+__device__ void blk_up(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, const float* RHS, const uint8_t* FL, int l) {
+    float* p = P + a.off[l];
+    const float* rhs = RHS + a.off[l];
+    const uint8_t* fl = FL + a.off[l];
+    const int n = a.n[l], nc = a.n[l + 1];
+    const float* pc = P + a.off[l + 1];
+    for (int y = tm.ty; y < n; y += TH)
+        for (int x = tm.tx; x < n; x += TW) {
+            const int i = y * n + x;
+            if (fl[i]) p[i] += pc[(y / 2) * nc + x / 2];
+        }
+    tm.sync();
+    for (int s = 0; s < 4; ++s) {
+        blk_sweep(tm, p, rhs, fl, n, a.eps[l], SYM ? 1 - (s & 1) : (s & 1));
+        tm.sync();
+    }
+}
+
+// The coarsest level: 30 x (red, black) (then (black, red) when SYM), or in
+// the tolerance mode the exact solve with the binary64 inverse.  Lanes of
+// the team share the rows of the inverse.
+template <bool SYM, int TW, int TH, class S>
+__device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, const float* RHS, const uint8_t* FL,
+                           float* bv) {
+    const int L = a.levels;
+    float* p = P + a.off[L - 1];
+    const float* rhs = RHS + a.off[L - 1];
+    const uint8_t* fl = FL + a.off[L - 1];
+    const int t = tm.ty * TW + tm.tx;
+    if (SYM && a.cm > 0) {
+        // exact solve: p = A^-1 rhs over the fluid cells, binary64.  The
+        // right-hand side is staged compactly; four lanes share a row (the
+        // inverse is stored by columns, so a row's quarter reads words that
+        // neighbouring rows read too) and add their partial sums in a fixed
+        // order.
+        const int m = a.cm;
+        for (int j = t; j < m; j += TW * TH) bv[j] = rhs[a.ccell[j]];
+        tm.sync();
+        const int q = t & 3;
+        for (int r0 = t >> 2; r0 < ((m + 15) & ~15); r0 += (TW * TH) / 4) {
+            double acc = 0.0;
+            if (r0 < m)
+                for (int j = q; j < m; j += 4) acc += a.cinv[(size_t)j * m + r0] * (double)bv[j];
+            acc += __shfl_down(acc, 2, 4);
+            acc += __shfl_down(acc, 1, 4);
+            if (q == 0 && r0 < m) p[a.ccell[r0]] = (float)acc;
+        }
+        tm.sync();
+    } else if (a.n[L - 1] * a.n[L - 1] <= 64) {
+        // at most 64 cells: one wave, one cell per lane, neighbours exchanged
+        // through cross-lane permutes
+        if (t < 64) {
+            const int nn = a.n[L - 1], lane = t;
+            const bool live = lane < nn * nn;
+            const int cx = live ? lane % nn : 0, cy = live ? lane / nn : 0;
+            const bool me = live && fl[lane];
+            const bool fl_l = live && cx > 0 && fl[lane - 1];
+            const bool fl_r = live && cx + 1 < nn && fl[lane + 1];
+            const bool fl_b = live && cy > 0 && fl[lane - nn];
+            const bool fl_t = live && cy + 1 < nn && fl[lane + nn];
+            const float rv = live ? rhs[lane] : 0.0f;
+            const float eps = a.eps[L - 1];
+            const int color = (cx + cy) & 1;
+            float pv = live ? p[lane] : 0.0f;
+            for (int s = 0; s < 2 * 30; ++s) {
+                const int want = (SYM && s >= 30) ? 1 - (s & 1) : (s & 1);
+                const float pl = __shfl(pv, lane - 1, 64), pr = __shfl(pv, lane + 1, 64);
+                const float pb = __shfl(pv, lane - nn, 64), pt = __shfl(pv, lane + nn, 64);
+                if (me && color == want) {
+                    const NbSum nb = nb_sum(fl_l, fl_r, fl_b, fl_t, pl, pr, pb, pt);
+                    pv = (nb.s - rv) / (nb.k + eps);
+                }
+            }
+            if (live) p[lane] = pv;
+        }
+        tm.sync();
+    } else {
+        for (int s = 0; s < 2 * 30; ++s) {
+            const int color = (SYM && s >= 30) ? 1 - (s & 1) : (s & 1);
+            blk_sweep(tm, p, rhs, fl, a.n[L - 1], a.eps[L - 1], color);
+            tm.sync();
+        }
+    }
+}
+
 template <class S, bool SYM>
 __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint32_t* done) {
     if (done && *done) return;
     extern __shared__ float sm[];
-    const int tx = (int)(threadIdx.x & 31u), ty = (int)(threadIdx.x >> 5);
+    const Team<32, 32> blk((int)threadIdx.x);
     const int total = a.off[a.levels];
     float* P = sm;
     float* RHS = sm + total;
     uint8_t* FL = reinterpret_cast<uint8_t*>(sm + 2 * total);
+    float* BV = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sm) + ((9 * total + 3) & ~3));
     const int n0 = a.n[0];
-    for (int y = ty; y < n0; y += 32)
-        for (int x = tx; x < n0; x += 32) {
+    for (int y = blk.ty; y < n0; y += 32)
+        for (int x = blk.tx; x < n0; x += 32) {
             const int i = y * n0 + x;
             const uint8_t f = a.fluid[0][i];
             const float pv = ldv(a.p0, i), rv = ldv(a.rhs0, i);
@@ -892,114 +1028,16 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
         }
     for (int l = 1; l < a.levels; ++l) {
         const int n = a.n[l];
-        for (int y = ty; y < n; y += 32)
-            for (int x = tx; x < n; x += 32) FL[a.off[l] + y * n + x] = a.fluid[l][y * n + x];
+        for (int y = blk.ty; y < n; y += 32)
+            for (int x = blk.tx; x < n; x += 32) FL[a.off[l] + y * n + x] = a.fluid[l][y * n + x];
     }
     __syncthreads();
     const int L = a.levels;
-    for (int l = 0; l + 1 < L; ++l) {
-        float* p = P + a.off[l];
-        const float* rhs = RHS + a.off[l];
-        const uint8_t* fl = FL + a.off[l];
-        const int n = a.n[l];
-        for (int s = 0; s < 4; ++s) {
-            blk_sweep(p, rhs, fl, n, a.eps[l], s & 1);
-            __syncthreads();
-        }
-        const int nc = a.n[l + 1];
-        float* pc = P + a.off[l + 1];
-        float* rc = RHS + a.off[l + 1];
-        for (int gy = ty; gy < nc; gy += 32)
-            for (int gx = tx; gx < nc; gx += 32) {
-                float sum = 0.0f;
-                uint32_t nf = 0;
-                for (int c = 0; c < 4; ++c) {
-                    const int fx = gx * 2 + (c & 1), fy = gy * 2 + (c >> 1);
-                    const int i = fy * n + fx;
-                    const bool f = fl[i] != 0;
-                    const float pc = p[i], rh = rhs[i];
-                    const NbSum nb = blk_nb(p, fl, n, fx, fy);
-                    sum = f ? sum + (rh - (nb.s - (nb.k + a.eps[l]) * pc)) : sum;
-                    nf += f ? 1u : 0u;
-                }
-                const int ci = gy * nc + gx;
-                rc[ci] = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
-                pc[ci] = 0.0f;
-            }
-        __syncthreads();
-    }
-    {
-        float* p = P + a.off[L - 1];
-        const float* rhs = RHS + a.off[L - 1];
-        const uint8_t* fl = FL + a.off[L - 1];
-        if (SYM && a.cm > 0) {
-            // exact coarsest solve: p = A^-1 rhs over fluid cells, binary64
-            for (int r = threadIdx.x; r < a.cm; r += blockDim.x) {
-                double acc = 0.0;
-                const double* row = a.cinv + (size_t)r * a.cm;
-                for (int j = 0; j < a.cm; ++j) acc += row[j] * (double)rhs[a.ccell[j]];
-                p[a.ccell[r]] = (float)acc;
-            }
-            __syncthreads();
-        } else if (a.n[L - 1] * a.n[L - 1] <= 64) {
-            // coarsest level of at most 64 cells: one wave, one cell per lane,
-            // neighbours exchanged through cross-lane permutes
-            if (threadIdx.x < 64) {
-                const int nn = a.n[L - 1], lane = threadIdx.x;
-                const bool live = lane < nn * nn;
-                const int cx = live ? lane % nn : 0, cy = live ? lane / nn : 0;
-                const bool me = live && fl[lane];
-                const bool fl_l = live && cx > 0 && fl[lane - 1];
-                const bool fl_r = live && cx + 1 < nn && fl[lane + 1];
-                const bool fl_b = live && cy > 0 && fl[lane - nn];
-                const bool fl_t = live && cy + 1 < nn && fl[lane + nn];
-                const float rv = live ? rhs[lane] : 0.0f;
-                const float eps = a.eps[L - 1];
-                const int color = (cx + cy) & 1;
-                float pv = live ? p[lane] : 0.0f;
-                for (int s = 0; s < 2 * 30; ++s) {
-                    const int want = (SYM && s >= 30) ? 1 - (s & 1) : (s & 1);
-                    const float pl = __shfl(pv, lane - 1, 64), pr = __shfl(pv, lane + 1, 64);
-                    const float pb = __shfl(pv, lane - nn, 64), pt = __shfl(pv, lane + nn, 64);
-                    if (me && color == want) {
-                        float sum = 0.0f, k = 0.0f;
-                        if (fl_l) { sum += pl; k += 1.0f; }
-                        if (fl_r) { sum += pr; k += 1.0f; }
-                        if (fl_b) { sum += pb; k += 1.0f; }
-                        if (fl_t) { sum += pt; k += 1.0f; }
-                        pv = (sum - rv) / (k + eps);
-                    }
-                }
-                if (live) p[lane] = pv;
-            }
-            __syncthreads();
-        } else {
-            for (int s = 0; s < 2 * 30; ++s) {
-                const int color = (SYM && s >= 30) ? 1 - (s & 1) : (s & 1);
-                blk_sweep(p, rhs, fl, a.n[L - 1], a.eps[L - 1], color);
-                __syncthreads();
-            }
-        }
-    }
-    for (int l = L - 2; l >= 0; --l) {
-        float* p = P + a.off[l];
-        const float* rhs = RHS + a.off[l];
-        const uint8_t* fl = FL + a.off[l];
-        const int n = a.n[l], nc = a.n[l + 1];
-        const float* pc = P + a.off[l + 1];
-        for (int y = ty; y < n; y += 32)
-            for (int x = tx; x < n; x += 32) {
-                const int i = y * n + x;
-                if (fl[i]) p[i] += pc[(y / 2) * nc + x / 2];
-            }
-        __syncthreads();
-        for (int s = 0; s < 4; ++s) {
-            blk_sweep(p, rhs, fl, n, a.eps[l], SYM ? 1 - (s & 1) : (s & 1));
-            __syncthreads();
-        }
-    }
-    for (int y = ty; y < n0; y += 32)
-        for (int x = tx; x < n0; x += 32) {
+    for (int l = 0; l + 1 < L; ++l) blk_down<SYM>(blk, a, P, RHS, FL, l);
+    blk_coarse<SYM>(blk, a, P, RHS, FL, BV);
+    for (int l = L - 2; l >= 0; --l) blk_up<SYM>(blk, a, P, RHS, FL, l);
+    for (int y = blk.ty; y < n0; y += 32)
+        for (int x = blk.tx; x < n0; x += 32) {
             const int i = y * n0 + x;
             if (FL[i]) stv(a.p0, i, P[i]);
         }
@@ -1238,6 +1276,23 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
     const int n = (int)V.n;
     const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
     tile_load<H, T, S>(t, L, V.zpre, x0, y0, &C);
+    // the inner products' operands at this thread's core cells, loaded now so
+    // that they arrive during the sweeps
+    const float* s_cur = V.s[1 - par];
+    constexpr int PC = (T * T) / 256;
+    float sc[PC], pc[PC], rc[PC];
+#pragma unroll
+    for (int j = 0; j < PC; ++j) {
+        const int k = (int)threadIdx.x + 256 * j;
+        const int gx = x0 + H + k % T, gy = y0 + H + k / T;
+        sc[j] = pc[j] = rc[j] = 0.0f;
+        if (gx < n && gy < n) {
+            const int i = gy * n + gx;
+            sc[j] = s_cur[i];
+            pc[j] = V.pd[i];
+            if (V.rs) rc[j] = r_cur[i];
+        }
+    }
     __syncthreads();
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -1249,9 +1304,10 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
         t.p[ly][lx] = round_s(t.p[ly][lx], V.u);
     }
     __syncthreads();
-    const float* s_cur = V.s[1 - par];
     double gam = 0.0, del = 0.0, e = 0.0, f = 0.0, rr = 0.0;
-    for (int k = threadIdx.x; k < T * T; k += blockDim.x) {
+#pragma unroll
+    for (int j = 0; j < PC; ++j) {
+        const int k = (int)threadIdx.x + 256 * j;
         const int ly = H + k / T, lx = H + k % T;
         const int gx = x0 + lx, gy = y0 + ly;
         if (gx >= n || gy >= n || !t.fl[ly][lx]) continue;
@@ -1261,11 +1317,11 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
         const float wv = nb.s - (nb.k + V.eps) * u;
         stv(V.u, i, u);
         V.w[i] = wv;
-        const float r = V.rs ? r_cur[i] : t.rhs[ly][lx];
+        const float r = V.rs ? rc[j] : t.rhs[ly][lx];
         gam += (double)r * (double)u;
         del += (double)wv * (double)u;
-        e += (double)u * (double)s_cur[i];
-        f += (double)V.pd[i] * (double)wv;
+        e += (double)u * (double)sc[j];
+        f += (double)pc[j] * (double)wv;
         rr += (double)r * (double)r;
     }
     block_partial<5>({gam, del, e, f, rr}, st->part);
@@ -1334,6 +1390,7 @@ __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, floa
             st->host_out[0] = st->it;
             st->host_out[1] = st->missed;
             st->host_out[2] = st->steps;
+            st->host_out[PCG_HOST_RING + (st->steps - 1u) % PCG_HOST_RING_LEN] = st->it;
         }
     }
 }
@@ -1914,10 +1971,11 @@ void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, boo
         else k_mg_up<S, false, 32><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
     }
 }
+size_t mg_block_lds_bytes_total(size_t cells) { return ((cells * 9 + 3) & ~(size_t)3) + 4 * MG_SYM_COARSE_MAX; }
 size_t mg_block_lds_bytes(const uint32_t* n, int count) {
     size_t cells = 0;
     for (int i = 0; i < count; ++i) cells += (size_t)n[i] * n[i];
-    return cells * 9;
+    return ((cells * 9 + 3) & ~(size_t)3) + 4 * MG_SYM_COARSE_MAX;
 }
 template <class S>
 void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
@@ -1938,7 +1996,7 @@ void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, 
     a.cinv = cinv;
     a.ccell = ccell;
     a.cm = cm;
-    const size_t lds = (size_t)off * 9;
+    const size_t lds = mg_block_lds_bytes_total((size_t)off);
     if (sym) k_mg_block<S, true><<<1, 1024, lds, s>>>(a, done);
     else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }

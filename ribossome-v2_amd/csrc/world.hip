@@ -130,6 +130,7 @@ struct RiboWorld {
     float* mg_rhs = nullptr;
     uint8_t* mg_fluid = nullptr;
     size_t mg_block_start = 0;  // first level solved inside one workgroup
+    size_t sym_coarse = 0;      // level the tolerance mode solves exactly (the coarsest)
     size_t lds_limit = 65536;
     // tolerance mode (preconditioned conjugate gradients)
     PcgState* pcg = nullptr;
@@ -148,13 +149,14 @@ struct RiboWorld {
     bf16* mg_rhs16 = nullptr;
     std::vector<MgLevelT<float>> mgz;   // preconditioner levels, binary32 (level 0 = z, r)
     std::vector<MgLevelT<bf16>> mgh;    // preconditioner levels, bfloat16 (level 0 = z16, r16)
-    double* cinv = nullptr;             // inverse of the coarsest operator (binary64)
+    double* cinv = nullptr;             // inverse of the exactly solved level's operator (binary64, by columns)
     int32_t* ccell = nullptr;
     int cm = 0;
     bool cinv_valid = false;
     uint32_t* pcg_host = nullptr;   // host-mapped: last iterations, misses, solves
     uint32_t pcg_slots = 0;         // iteration slots recorded per solve (0: not set)
     uint32_t missed_seen = 0;
+    uint32_t solves_seen = 0;
     LedgerDev* ledger = nullptr;
     uint32_t* scratch_u32 = nullptr;  // F words for downloads of derived masks
     // graph replay: the epoch base lives on the device; one executable graph
@@ -328,6 +330,7 @@ int build_mg(RiboWorld* w) {
            mg_block_lds_bytes(sizes.data() + (b - 1), (int)(w->mg.size() - (b - 1))) <= w->lds_limit)
         --b;
     w->mg_block_start = b;
+    w->sym_coarse = w->levels.size() - 1;
     return RIBO_OK;
 }
 
@@ -349,7 +352,7 @@ int rebuild_terrain(RiboWorld* w) {
 // cells, in binary64 (Gauss-Jordan with partial pivoting on the host), for
 // the exact coarse solve of the tolerance mode.  Skipped above 1024 unknowns.
 int build_coarse_inverse(RiboWorld* w) {
-    const RiboMgLevel& C = w->levels.back();
+    const RiboMgLevel& C = w->levels[w->sym_coarse];
     const uint32_t n = C.n;
     std::vector<uint8_t> fl((size_t)n * n);
     HIPCHK(hipMemcpyAsync(fl.data(), w->mg_fluid + C.off, fl.size(), hipMemcpyDeviceToHost, w->stream));
@@ -393,7 +396,11 @@ int build_coarse_inverse(RiboWorld* w) {
     }
     int rc;
     if ((rc = dalloc(&w->cinv, m * m)) || (rc = dalloc(&w->ccell, m))) return rc;
-    HIPCHK(hipMemcpy(w->cinv, inv.data(), m * m * sizeof(double), hipMemcpyHostToDevice));
+    // by columns, so that the rows a workgroup computes read consecutive words
+    std::vector<double> inv_t(m * m);
+    for (size_t r = 0; r < m; ++r)
+        for (size_t j = 0; j < m; ++j) inv_t[j * m + r] = inv[r * m + j];
+    HIPCHK(hipMemcpy(w->cinv, inv_t.data(), m * m * sizeof(double), hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(w->ccell, cell.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
     w->cm = (int)m;
     w->cinv_valid = true;
@@ -422,19 +429,22 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
     for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done); });
 }
 
-// Levels 1 .. L-1 of the symmetric V-cycle (level 0 is fused into the
-// conjugate-gradient kernels).
+// Levels 1 .. sym_coarse (the coarsest, solved exactly with its dense
+// inverse) of the symmetric V-cycle; level 0 is fused into the
+// conjugate-gradient kernels.  (Stopping at 16 x 16 with a 256-unknown
+// inverse was measured: the iterations stayed the same at 1024^2 and the
+// one-workgroup inverse product was slower than the levels it replaced.)
 template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done) {
-    const size_t L = lv.size();
+    const size_t L = lv.size(), lc = w->sym_coarse;
     hipStream_t s = w->stream;
-    const size_t b = std::max<size_t>(w->mg_block_start, 1);
-    if (b < L) {
+    if (w->mg_block_start < L) {
+        const size_t b = std::min(std::max<size_t>(w->mg_block_start, 1), lc);
         for (size_t l = 1; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
-        run(w, "mg_block", [&] { launch_mg_block<S>(s, lv.data() + b, (int)(L - b), true, w->cinv, w->ccell, w->cm, done); });
+        run(w, "mg_block", [&] { launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done); });
         for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
         return;
     }
-    // coarsest level too large for one workgroup: sweeps only
+    // levels too large for one workgroup: sweeps only
     for (size_t l = 1; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
     for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
 }
@@ -687,8 +697,9 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     he = hipMemcpy(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice);
     PcgState pst{};
     pst.part = w->pcg_part;
-    if (he == hipSuccess && hipHostMalloc((void**)&w->pcg_host, 4 * sizeof(uint32_t), hipHostMallocMapped) == hipSuccess) {
-        memset(w->pcg_host, 0, 4 * sizeof(uint32_t));
+    const size_t host_words = PCG_HOST_RING + PCG_HOST_RING_LEN;
+    if (he == hipSuccess && hipHostMalloc((void**)&w->pcg_host, host_words * sizeof(uint32_t), hipHostMallocMapped) == hipSuccess) {
+        memset(w->pcg_host, 0, host_words * sizeof(uint32_t));
         if (hipHostGetDevicePointer((void**)&pst.host_out, w->pcg_host, 0) != hipSuccess) pst.host_out = nullptr;
     }
     if (he == hipSuccess) he = hipMemcpy(w->pcg, &pst, sizeof(pst), hipMemcpyHostToDevice);
@@ -832,17 +843,27 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             rc = build_coarse_inverse(w);
             if (rc) return rc;
         }
-        // Iteration slots per solve for this chunk: the previous solve's
-        // count plus 30 % and 4, in multiples of 4 (after draining the
-        // stream, so the choice depends only on the sequence of calls); a
-        // solve that ran out of slots (counted on the device) restores the
-        // full cap.
+        // Iteration slots per solve for this chunk: the most iterations any
+        // solve of the previous chunk took, plus 3, rounded up to even (read
+        // after draining the stream, so the choice depends only on the
+        // sequence of calls); a solve that ran out of slots (counted on the
+        // device) restores the full cap.
         const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
         HIPCHK(hipStreamSynchronize(w->stream));
-        const uint32_t last = w->pcg_host ? __atomic_load_n(&w->pcg_host[0], __ATOMIC_RELAXED) : maxit;
-        const uint32_t missed = w->pcg_host ? __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED) : 0u;
-        uint32_t slots = std::min(maxit, ((last + last * 3 / 10 + 4) + 3) / 4 * 4);
-        if (w->pcg_slots == 0 || missed != w->missed_seen) slots = maxit;
+        uint32_t slots = w->pcg_slots;
+        uint32_t missed = 0u;
+        if (w->pcg_host) {
+            missed = __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED);
+            const uint32_t solves = __atomic_load_n(&w->pcg_host[2], __ATOMIC_RELAXED);
+            const uint32_t fresh = std::min(solves - w->solves_seen, PCG_HOST_RING_LEN);
+            uint32_t most = 0u;
+            for (uint32_t k = 0; k < fresh; ++k)
+                most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
+                                                      __ATOMIC_RELAXED));
+            if (fresh > 0u) slots = std::min(maxit, (most + 3u + 1u) & ~1u);
+            w->solves_seen = solves;
+        }
+        if (w->pcg_slots == 0 || !w->pcg_host || missed != w->missed_seen) slots = maxit;
         w->missed_seen = missed;
         w->pcg_slots = std::max(slots, 1u);
     }

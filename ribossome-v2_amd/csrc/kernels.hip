@@ -170,9 +170,12 @@ __device__ __forceinline__ float2 reflect(const Geo& g, uint32_t i, float2 v) {
     }
     return v;
 }
+// (the flag and the velocity are both loaded and the result selected, so a
+// stencil's loads go out together instead of flag-then-velocity per cell)
 __device__ __forceinline__ float2 raw_vel(const Geo& g, const float2* vin, uint32_t i) {  // fluid.wgsl:191-194
-    if (solid(g, i)) return f2(0.0f, 0.0f);
-    return sanitize(vin[i]);
+    const bool wall = solid(g, i);
+    const float2 v = sanitize(vin[i]);
+    return wall ? f2(0.0f, 0.0f) : v;
 }
 __device__ __forceinline__ uint32_t env_cell_for_fluid(const Geo& g, uint32_t x, uint32_t y) {  // fluid.wgsl:87-92
     float s = (float)g.G / (float)g.F;
@@ -522,17 +525,42 @@ __device__ __forceinline__ float curl_at(const Geo& g, const float2* vin, uint32
     return 0.5f * (v_r.y - v_l.y) - 0.5f * (v_t.x - v_b.x);
 }
 
-// vorticity_confinement (fluid.wgsl:491-518)
+// vorticity_confinement (fluid.wgsl:491-518).  Each cell needs the curl at
+// itself and its 4 neighbours, and each curl the reflected raw velocity of 4
+// cells, so the 16 x 16 tile stages those velocities with a 2-cell halo
+// (coordinates clamped into the grid, as curl_at clamps them), then the
+// curls with a 1-cell halo, in LDS; the arithmetic is curl_at's.
 __global__ __launch_bounds__(256) void k_vorticity(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
-    FLUID_XY
+    __shared__ float2 rv[20][21];
+    __shared__ float cl[18][19];
+    const uint32_t F = g.F;
+    const int32_t Fm = (int32_t)F - 1;
+    const int32_t bx = (int32_t)(blockIdx.x * 16), by = (int32_t)(blockIdx.y * 16);
+    const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
+    for (uint32_t k = tid; k < 20u * 20u; k += 256u) {
+        const int32_t lx = (int32_t)(k % 20u), ly = (int32_t)(k / 20u);
+        const uint32_t gx = (uint32_t)rb_clampi(bx + lx - 2, 0, Fm), gy = (uint32_t)rb_clampi(by + ly - 2, 0, Fm);
+        const uint32_t q = gy * F + gx;
+        rv[ly][lx] = reflect(g, q, raw_vel(g, vin, q));
+    }
+    __syncthreads();
+    for (uint32_t k = tid; k < 18u * 18u; k += 256u) {
+        const uint32_t lx = k % 18u, ly = k / 18u;  // curl at tile cell (lx - 1, ly - 1)
+        const float2 v_l = rv[ly + 1][lx], v_r = rv[ly + 1][lx + 2], v_b = rv[ly][lx + 1], v_t = rv[ly + 2][lx + 1];
+        cl[ly][lx] = 0.5f * (v_r.y - v_l.y) - 0.5f * (v_t.x - v_b.x);
+    }
+    __syncthreads();
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= F || y >= F) return;
+    const uint32_t idx = y * F + x;
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
     const uint32_t border = 2u;
     const float eps = rb_clamp(P.fluid_vorticity, 0.0f, 10.0f);
     if (x < border || x >= F - border || y < border || y >= F - border || eps < 1e-6f) { vout[idx] = vin[idx]; return; }
-    const float w = curl_at(g, vin, x, y);
+    const uint32_t cx = threadIdx.x + 1u, cy = threadIdx.y + 1u;
+    const float w = cl[cy][cx];
     if (rb_abs(w) < 5e-4f) { vout[idx] = vin[idx]; return; }
-    const float2 grad = mul2(f2(rb_abs(curl_at(g, vin, x + 1, y)) - rb_abs(curl_at(g, vin, x - 1, y)),
-                                rb_abs(curl_at(g, vin, x, y + 1)) - rb_abs(curl_at(g, vin, x, y - 1))),
+    const float2 grad = mul2(f2(rb_abs(cl[cy][cx + 1]) - rb_abs(cl[cy][cx - 1]), rb_abs(cl[cy + 1][cx]) - rb_abs(cl[cy - 1][cx])),
                              0.5f);
     const float gl = rb_max(len2(grad), 1e-5f);
     const float2 n = f2(grad.x / gl, grad.y / gl);
@@ -556,8 +584,10 @@ __global__ __launch_bounds__(256) void k_divergence(Geo g, const float2* __restr
     const float2 v_c = raw_vel(g, vin, idx);
     auto nbv = [&](int32_t nx, int32_t ny, bool axis_x) -> float2 {
         const bool outside = nx < 0 || ny < 0 || nx >= (int32_t)F || ny >= (int32_t)F;
-        if (outside || solid(g, (uint32_t)ny * F + (uint32_t)nx)) return axis_x ? f2(-v_c.x, v_c.y) : f2(v_c.x, -v_c.y);
-        return raw_vel(g, vin, (uint32_t)ny * F + (uint32_t)nx);
+        const uint32_t q = outside ? idx : (uint32_t)ny * F + (uint32_t)nx;
+        const bool wall = outside || solid(g, q);
+        const float2 v = raw_vel(g, vin, q);
+        return wall ? (axis_x ? f2(-v_c.x, v_c.y) : f2(v_c.x, -v_c.y)) : v;
     };
     const int32_t xi = (int32_t)x, yi = (int32_t)y;
     const float2 v_l = nbv(xi - 1, yi, true), v_r = nbv(xi + 1, yi, true);
@@ -1400,69 +1430,75 @@ __global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict
                                                  const float* __restrict__ P) {
     FLUID_XY
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
+    // neighbours outside the box or in walls use p_c; every load is issued
+    // (indices clamped to the cell) and the value selected
+    const uint32_t il = x > 0u ? idx - 1u : idx, ir = x + 1u < F ? idx + 1u : idx;
+    const uint32_t ib = y > 0u ? idx - F : idx, it = y + 1u < F ? idx + F : idx;
     const float p_c = P[idx];
-    float p_l = p_c, p_r = p_c, p_b = p_c, p_t = p_c;
-    if (x > 0u && !solid(g, idx - 1u)) p_l = P[idx - 1u];
-    if (x + 1u < F && !solid(g, idx + 1u)) p_r = P[idx + 1u];
-    if (y > 0u && !solid(g, idx - F)) p_b = P[idx - F];
-    if (y + 1u < F && !solid(g, idx + F)) p_t = P[idx + F];
+    const float ql = P[il], qr = P[ir], qb = P[ib], qt = P[it];
+    const bool wl = solid(g, il), wr = solid(g, ir), wb = solid(g, ib), wt = solid(g, it);
+    const float p_l = (x > 0u && !wl) ? ql : p_c, p_r = (x + 1u < F && !wr) ? qr : p_c;
+    const float p_b = (y > 0u && !wb) ? qb : p_c, p_t = (y + 1u < F && !wt) ? qt : p_c;
     const float2 grad = mul2(f2(p_r - p_l, p_t - p_b), 0.5f);
     float2 v = sub2(sanitize(vin[idx]), grad);
     v = reflect(g, idx, v);
     vout[idx] = clamp_len(sanitize(v), MAX_VEL);
 }
 
-// net_flux (fluid.wgsl:640-657), in the reference's sequential order per row
-// and per column so the means are bit-identical to it.  Each block owns
-// NF_C chains (rows or columns): all threads stage NF_K-long stretches of
-// the chains into LDS with coalesced loads, double-buffered, while NF_C
-// threads walk the previous stretch in order.  Walls are marked NaN (a
-// sanitised velocity is never NaN) and skipped.
-constexpr uint32_t NF_C = 16, NF_K = 64;
-__global__ __launch_bounds__(256) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
-    __shared__ float buf[2][NF_C][NF_K + 1];
+// net_flux (fluid.wgsl:640-657): the mean v_y of each row and the mean v_x
+// of each column over fluid cells, summed in the reference's order (k = 0,
+// 1, ...) so the fluid stays bit-exact.  A workgroup owns NF_C rows or
+// columns; its threads stage up to NF_K values of each chain in LDS in one
+// pass, k-major so that the walkers read consecutive words, with walls
+// stored as -0 (x + -0 == x for every x, so adding it is the reference's
+// skip) and count the fluid cells of their stretch; then NF_C lanes walk
+// their chains with one dependent add per cell.
+constexpr uint32_t NF_C = 8, NF_K = 1024, NF_T = 1024, NF_PER = NF_C * NF_K / NF_T;
+__global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
+    __shared__ float buf[NF_K][NF_C];
+    __shared__ uint32_t cnt[NF_T / NF_C][NF_C];
     const uint32_t F = g.F;
     const uint32_t nb = (F + NF_C - 1) / NF_C;
     const bool cols = blockIdx.x < nb;
     const uint32_t c0 = (cols ? blockIdx.x : blockIdx.x - nb) * NF_C;
     const uint32_t t = threadIdx.x;
-    const uint32_t ntile = (F + NF_K - 1) / NF_K;
-    auto stage = [&](uint32_t tile, int bsel) {
-        const uint32_t k0 = tile * NF_K;
-        for (uint32_t e = t; e < NF_C * NF_K; e += 256u) {
-            uint32_t c, k;
-            if (cols) { c = e % NF_C; k = e / NF_C; }   // 16 consecutive columns of one row
-            else { k = e % NF_K; c = e / NF_K; }        // 64 consecutive columns of one row
-            const uint32_t chain = c0 + c, pos = k0 + k;
-            float val = __builtin_nanf("");
-            if (chain < F && pos < F) {
+    // thread t stages NF_PER consecutive cells of chain c
+    const uint32_t c = cols ? t % NF_C : t / (NF_T / NF_C);
+    const uint32_t h = cols ? t / NF_C : t % (NF_T / NF_C);
+    const uint32_t chain = c0 + c;
+    float sum = 0.0f;
+    uint32_t n = 0u;
+    for (uint32_t k0 = 0; k0 < F; k0 += NF_K) {
+        const uint32_t len = rb_minu(NF_K, F - k0);
+        if (k0 > 0) __syncthreads();
+        uint32_t fl = 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < NF_PER; ++j) {
+            const uint32_t k = h * NF_PER + j, pos = k0 + k;
+            float val = -0.0f;
+            if (chain < F && k < len) {
                 const uint32_t q = cols ? pos * F + chain : chain * F + pos;
-                if (!solid(g, q)) {
-                    const float2 v = sanitize(vin[q]);
-                    val = cols ? v.x : v.y;
-                }
+                const float2 v = sanitize(vin[q]);
+                const bool wall = solid(g, q);
+                val = wall ? -0.0f : (cols ? v.x : v.y);
+                fl += wall ? 0u : 1u;
             }
-            buf[bsel][c][k] = val;
+            buf[k][c] = val;
         }
-    };
-    float sum = 0.0f, n = 0.0f;
-    stage(0, 0);
-    __syncthreads();
-    for (uint32_t tile = 0; tile < ntile; ++tile) {
-        if (tile + 1 < ntile) stage(tile + 1, (tile + 1) & 1);
-        if (t < NF_C) {
-            const int bsel = tile & 1;
-            const uint32_t lim = rb_minu(NF_K, F - tile * NF_K);
-            for (uint32_t k = 0; k < lim; ++k) {
-                const float val = buf[bsel][t][k];
-                if (val != val) continue;
-                sum += val;
-                n += 1.0f;
-            }
-        }
+        cnt[h][c] = fl;
         __syncthreads();
+        if (t < NF_C) {
+            uint32_t k = 0;
+            for (; k + 16 <= len; k += 16) {
+#pragma unroll
+                for (uint32_t u = 0; u < 16; ++u) sum += buf[k + u][t];
+            }
+            for (; k < len; ++k) sum += buf[k][t];
+            for (uint32_t i = 0; i < NF_T / NF_C; ++i) n += cnt[i][t];
+        }
     }
-    if (t < NF_C && c0 + t < F) nf[(cols ? F : 0u) + c0 + t] = n > 0.0f ? sum / n : 0.0f;
+    // the reference counts with n += 1.0 in binary32: exact below 2^24
+    if (t < NF_C && c0 + t < F) nf[(cols ? F : 0u) + c0 + t] = n > 0u ? sum / (float)n : 0.0f;
 }
 
 // enforce_boundaries (fluid.wgsl:663-677)
@@ -2041,7 +2077,7 @@ void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout
 }
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf) {
     uint32_t nb = (g.F + NF_C - 1) / NF_C;
-    k_net_flux<<<2 * nb, 256, 0, s>>>(g, vin, nf);
+    k_net_flux<<<2 * nb, NF_T, 0, s>>>(g, vin, nf);
 }
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf) {
     k_boundaries<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, nf);

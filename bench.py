@@ -168,6 +168,46 @@ def timed_run(cfg, inputs, args, mode):
     return dt, w
 
 
+def fp64_error(cfg, inputs, args):
+    """Error of the binary32 and mixed-precision solves against the binary64
+    oracle on a bounded sample: one step from the same inputs, the binary64
+    solve run to relative residual 1e-10 (the exact answer for this
+    purpose), the GPU runs to 1e-5.  Relative L2 of velocity and pressure."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_ctypes as oc
+
+    gamma, vel, chem, heat = inputs
+    ow = oc.OracleWorld(cfg, SEED, fp64=True)
+    p, s = configure(ow.params, ra.WorldSettings.default(), args, "fp32")
+    s.mg_tol = 1e-10
+    s.mg_max_cycles = 400
+    ow.params = p
+    ow.set_settings(s)
+    for which, a in ((ra.GAMMA, gamma), (ra.VELOCITY, vel), (ra.CHEM, chem), (ra.HEAT_SRC, heat)):
+        ow.upload(which, a)
+    t0 = time.perf_counter()
+    ow.encode_steps(1)
+    dt = time.perf_counter() - t0
+    ref = {which: ow.download_f64(which) for which in (ra.VELOCITY, ra.PRESSURE)}
+    it64, rel64 = ow.mg_stats()
+    out = {"sample": f"1 step from the inputs; binary64 oracle to relative residual {rel64:.1e} in {it64} iterations "
+                     f"({dt:.1f} s on the host)"}
+    for mode in ("fp32", "mixed"):
+        w = build_world(cfg, inputs, args, mode)
+        w.encode_steps(1)
+        w.sync()
+        it, rel = w.mg_stats()
+        e = {}
+        for which, name in ((ra.VELOCITY, "velocity"), (ra.PRESSURE, "pressure")):
+            a, b = w.download(which).astype(np.float64).reshape(-1), ref[which].reshape(-1)
+            e[name] = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        e["iterations"] = it
+        e["true_rel_residual"] = rel
+        out[mode] = e
+        w.close()
+    return out
+
+
 def cpu_baseline(cfg, inputs, args):
     """The CPU oracle (C++ restatement of the reference WGSL, OpenMP over all
     threads it is given) on a bounded sample of the same workload: the same
@@ -258,6 +298,114 @@ def step_model_bytes(cfg, sizes, mode, res):
     return total + v * vcycle
 
 
+def batched(args):
+    """Independent worlds stepped together, one stream each (the batched
+    configuration): sides from {128, ..., 1024} grid cells, rock U(5, 20) %,
+    monomers Poisson(U(2, 8)), the reference's one V-cycle per step, the
+    ledger checked every step.  Returns the JSON record."""
+    sides = [128, 192, 256, 384, 512, 768, 1024]
+    worlds = []
+    for i in range(args.worlds):
+        r = np.random.default_rng(1000 + i)
+        side = int(r.choice(sides))
+        rock, lam = float(r.uniform(0.05, 0.20)), float(r.uniform(2.0, 8.0))
+        cfg = ra.WorldConfig(side, side // 2)
+        gamma = ra.gen_gamma(cfg, 1000 + i, rock, True)
+        vel = ra.gen_velocity(cfg, 1000 + i, 0.5 / 0.017, gamma)
+        chem = ra.gen_chem(cfg, 1000 + i, lam, gamma)
+        w = ra.World(cfg, 1000 + i)
+        p, s = configure(ra.SimParams.default(1000 + i), ra.WorldSettings.default(), args, "reference")
+        w.params = p
+        w.upload(ra.GAMMA, gamma)
+        w.upload(ra.VELOCITY, vel)
+        w.upload(ra.CHEM, chem)
+        w.settings = s
+        worlds.append((w, cfg, ra.Ledger.from_cells(chem)))
+    for w, _, _ in worlds:  # warm-up: every graph the timed loop replays
+        w.encode_steps(64)
+        w.encode_steps(args.batch_steps % 64 or 64)
+    for w, _, _ in worlds:
+        w.sync()
+    left = [args.batch_steps] * len(worlds)
+    t0 = time.perf_counter()
+    while any(left):
+        for i, (w, _, _) in enumerate(worlds):
+            if left[i]:
+                left[i] -= w.encode_steps(min(64, left[i]))
+    for w, _, _ in worlds:
+        w.sync()
+    dt = time.perf_counter() - t0
+    for w, _, led in worlds:
+        if w.ledger_status() is not None or w.ledger_blocking().channels() != led.channels():
+            raise SystemExit("batched: a world's per-channel monomer totals changed")
+    cells = sum(c.cells() for _, c, _ in worlds)
+    for w, _, _ in worlds:
+        w.close()
+    return {"metric": "batched world cell-steps/s", "value": cells * args.batch_steps / dt, "unit": "cell-steps/s",
+            "n_gpus": 1, "steps": args.batch_steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic", "ms_per_step": 1000.0 * dt / args.batch_steps,
+            "config": {"workload": f"{len(worlds)} independent worlds, grid sides from {sides}, reference solver "
+                                   "schedule (one V-cycle per step), ledger checked every step",
+                       "worlds": len(worlds), "grid_cells": cells}}
+
+
+def serving(args):
+    """Scenes arriving as a Poisson process at --qps, each a world of a
+    log-uniform side in [128, 1024] grid cells stepped U(50, 2000) steps
+    with the mixed-precision tolerance-mode solver; a host loop admits
+    arrivals, keeps every active world's stream fed with chunks of up to 64
+    steps and retires finished scenes.  Latency = completion - arrival."""
+    rng = np.random.default_rng(5000)
+    n = args.scenes
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
+    sides = [int(round(np.exp(rng.uniform(np.log(128), np.log(1024))) / 16.0)) * 16 for _ in range(n)]
+    steps = [int(rng.integers(50, 2001)) for _ in range(n)]
+    inputs = []
+    for i in range(n):
+        cfg = ra.WorldConfig(sides[i], sides[i] // 2)
+        gamma = ra.gen_gamma(cfg, 7000 + i, 0.15, True)
+        inputs.append((cfg, gamma, ra.gen_velocity(cfg, 7000 + i, 0.5 / 0.017, gamma), ra.gen_chem(cfg, 7000 + i, 4.0, gamma),
+                       ra.gen_vent_heat(cfg, max(1, sides[i] // 128), 5000.0 * 0.3)))
+    active, done_lat, work = {}, [], 0
+    nxt = 0
+    t0 = time.perf_counter()
+    while len(done_lat) < n:
+        now = time.perf_counter() - t0
+        while nxt < n and arrivals[nxt] <= now:
+            cfg, gamma, vel, chem, heat = inputs[nxt]
+            w = build_world(cfg, (gamma, vel, chem, heat), args, "mixed")
+            active[nxt] = [w, steps[nxt]]
+            nxt += 1
+        busy = False
+        for i in list(active):
+            w, left = active[i]
+            if not w.idle():
+                busy = True
+                continue
+            if left == 0:
+                done_lat.append(time.perf_counter() - t0 - arrivals[i])
+                work += inputs[i][0].cells() * steps[i]
+                w.close()
+                del active[i]
+                continue
+            k = w.encode_steps(min(64, left))
+            active[i][1] = left - k
+            busy = True
+        if not busy and nxt < n:
+            time.sleep(max(0.0, min(0.002, arrivals[nxt] - (time.perf_counter() - t0))))
+        elif not busy:
+            time.sleep(0.0002)
+    span = time.perf_counter() - t0
+    lat = np.sort(np.array(done_lat))
+    p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    return {"metric": "scene latency p50 (s)", "value": p50, "unit": "s", "p99": p99, "n_gpus": 1,
+            "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32 fluid, bf16 preconditioner",
+            "data": "synthetic", "throughput_cell_steps_per_s": work / span,
+            "config": {"workload": f"{n} scenes, Poisson arrivals at {args.qps} per s, sides log-uniform in [128, 1024], "
+                                   "U(50, 2000) steps each, mixed-precision tolerance-mode solver to 1e-5",
+                       "scenes": n, "qps": args.qps, "span_s": span}}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -271,6 +419,13 @@ def main():
     ap.add_argument("--profile-steps", type=int, default=32)
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the binary64-oracle error sample")
+    ap.add_argument("--config", default="world", choices=["world", "batched", "serving"],
+                    help="world: the 2048^2 headline; batched: many independent worlds; serving: Poisson scene arrivals")
+    ap.add_argument("--worlds", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--scenes", type=int, default=24)
+    ap.add_argument("--qps", type=float, default=1.0)
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel eagerly instead of replaying captured HIP graphs (for tracing tools)")
     args = ap.parse_args()
@@ -279,6 +434,12 @@ def main():
               "is not supported (independent worlds would shard one per GPU).", file=sys.stderr)
         sys.exit(2)
 
+    if args.config == "batched":
+        print(json.dumps(batched(args)))
+        return
+    if args.config == "serving":
+        print(json.dumps(serving(args)))
+        return
     cfg = ra.WorldConfig(GRID, FLUID)
     inputs = world_inputs(cfg)
     results = {}
@@ -323,6 +484,15 @@ def main():
         "launches": launches,
         "algorithmic_bytes_per_launch": per_launch,
     }
+    # the transport scatter, the largest single kernel of every step, beside it
+    sc_ms, sc_n = times.get("transport_scatter", (0.0, 0))
+    sc_s = sc_ms / max(sc_n, 1) / 1000.0
+    sc_bytes = kernel_bytes("transport_scatter", cfg, sizes, head)
+    roofline_scatter = {"kernel": "transport_scatter", "bound": "hbm", "achieved": sc_bytes / sc_s / 1e9 if sc_s else None,
+                        "peak": 8000.0, "unit": "GB/s", "frac": sc_bytes / sc_s / 8e12 if sc_s else None,
+                        "traffic": pmc_traffic("transport_scatter", head), "avg_launch_us": sc_s * 1e6,
+                        "launches": sc_n, "algorithmic_bytes_per_launch": sc_bytes,
+                        "note": "VALU-issue bound (PCG4D per monomer); HBM is not its limit"}
     kernels = {k: [round(v[0] / max(v[1], 1) * 1000.0, 2), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
     step_bytes = {m: step_model_bytes(cfg, sizes, m, results[m]) for m in results}
     hbm = {m: {"algorithmic_bytes_per_step": step_bytes[m],
@@ -332,6 +502,9 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, inputs, args)
+    f64 = None
+    if not args.no_cpu and not args.no_fp64 and "fp32" in results and "mixed" in results:
+        f64 = fp64_error(cfg, inputs, args)
 
     out = {
         "metric": METRIC,
@@ -358,7 +531,9 @@ def main():
         },
         "modes": results,
         "mixed_vs_fp32": (results["mixed"]["value"] / results["fp32"]["value"]) if "mixed" in results and "fp32" in results else None,
+        "error_vs_fp64": f64,
         "roofline": roofline,
+        "roofline_transport_scatter": roofline_scatter,
         "hbm_step_model": hbm,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,

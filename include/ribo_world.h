@@ -318,6 +318,10 @@ RIBO_API int ribo_world_encode_steps(RiboWorld* w, uint32_t steps);
 RIBO_API int ribo_world_run(RiboWorld* w, uint32_t steps);
 /* Waits for the world's stream (gpu.wait_idle, src/gpu/mod.rs:50-52). */
 RIBO_API int ribo_world_sync(RiboWorld* w);
+/* Non-blocking form of ribo_world_sync, the wgpu device.poll without wait
+ * (src/gpu/mod.rs:51 waits; a serving loop polls): 1 when every recorded
+ * step has finished, 0 while work is still running, negative on error. */
+RIBO_API int ribo_world_idle(RiboWorld* w);
 
 /* ---- ledger (Ledger, src/world/mod.rs:39-82; ledger.wgsl:5-27) ---------- */
 

@@ -844,7 +844,10 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             if (rc) return rc;
         }
         // Iteration slots per solve for this chunk: the most iterations any
-        // solve of the previous chunk took, plus 3, rounded up to even (read
+        // solve of the previous chunk took, plus 30 % and 4, rounded up to
+        // even (slots after convergence cost ~nothing: their kernels return
+        // at once; a tighter margin missed the tolerance once in 1000 steps)
+        // (read
         // after draining the stream, so the choice depends only on the
         // sequence of calls); a solve that ran out of slots (counted on the
         // device) restores the full cap.
@@ -860,7 +863,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             for (uint32_t k = 0; k < fresh; ++k)
                 most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
                                                       __ATOMIC_RELAXED));
-            if (fresh > 0u) slots = std::min(maxit, (most + 3u + 1u) & ~1u);
+            if (fresh > 0u) slots = std::min(maxit, (most + most * 3u / 10u + 4u + 1u) & ~1u);
             w->solves_seen = solves;
         }
         if (w->pcg_slots == 0 || !w->pcg_host || missed != w->missed_seen) slots = maxit;
@@ -951,6 +954,13 @@ int ribo_world_sync(RiboWorld* w) {
     if (!w) return fail(RIBO_ERR_ARG, "null world");
     HIPCHK(hipStreamSynchronize(w->stream));
     return RIBO_OK;
+}
+int ribo_world_idle(RiboWorld* w) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    const hipError_t e = hipStreamQuery(w->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return fail(RIBO_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
 }
 
 int ribo_world_ledger(RiboWorld* w, uint32_t out[12]) {  // encode_ledger + ledger_blocking (src/world/mod.rs:1661-1674,1834-1839)

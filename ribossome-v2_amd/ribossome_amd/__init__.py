@@ -277,6 +277,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_encode_steps": (I, [P, U32]),
         "ribo_world_run": (I, [P, U32]),
         "ribo_world_sync": (I, [P]),
+        "ribo_world_idle": (I, [P]),
         "ribo_world_ledger": (I, [P, P]),
         "ribo_world_ledger_status": (I, [P, ctypes.POINTER(U32)]),
         "ribo_world_set_timing": (I, [P, I]),
@@ -434,6 +435,10 @@ class World:
 
     def sync(self):
         _check(lib().ribo_world_sync(self._h), "sync")
+
+    def idle(self) -> bool:
+        """True when every recorded step has finished (does not wait)."""
+        return _check(lib().ribo_world_idle(self._h), "idle") == 1
 
     # -- readback helpers named like the reference's test API --
     def read_cells_blocking(self) -> np.ndarray:

@@ -1,0 +1,89 @@
+"""GPU parity at the headline size (2048^2 grid, 1024^2 fluid): the same
+seeded inputs as the benchmark, run on the HIP world and on the CPU oracle.
+
+At this size the oracle still finishes a step in a few seconds, so the
+reference schedule is compared bit for bit after two steps; the tolerance
+mode (whose inner products are summed in another order) is compared by
+iteration counts, residuals and the relative L2 of the fields; the monomer
+ledger is checked on the device every step for a longer run.
+"""
+import numpy as np
+import pytest
+
+import ribossome_amd as ra
+import oracle_ctypes as oc
+from worlds import bit_diff, load, make_inputs, parity_params, parity_settings
+
+pytestmark = pytest.mark.gpu
+
+CFG = ra.WorldConfig(2048, 1024)
+
+
+def headline_inputs():
+    # the benchmark's world: 15 % rock with a rubble shell, Poisson(4)
+    # monomers, a 16-row vent band, 0.5 fluid cells per step
+    return make_inputs(CFG, 2, rock=0.15, rubble=True, lam=4.0, vent_rows=16, vent_h=5000.0 * 0.3)
+
+
+def test_headline_world_reference_schedule_bit_exact():
+    print("case fullsize reference", flush=True)
+    inp = headline_inputs()
+    s = parity_settings(ledger_check=1)
+    gw = ra.World(CFG, 2)
+    load(gw, inp, parity_params(2), s)
+    ow = oc.OracleWorld(CFG, 2)
+    load(ow, inp, parity_params(2), s)
+    gw.encode_steps(2)
+    ow.encode_steps(2)
+    a, b = gw.read_cells_blocking(), ow.download(ra.CHEM)
+    assert np.array_equal(a, b), f"chem differs in {int(np.count_nonzero(a != b))} words"
+    for which in (ra.VELOCITY, ra.PRESSURE, ra.TEMP, ra.DIVERGENCE):
+        d = bit_diff(gw.download(which), ow.download(which))
+        assert d == 0, f"{ra.BUFFER_NAMES[which]}: {d} elements differ"
+    assert gw.ledger_status() is None
+    gw.close()
+    ow.close()
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_headline_world_tolerance_mode_matches_oracle(precision):
+    print(f"case fullsize tolerance precision={precision}", flush=True)
+    inp = headline_inputs()
+    s = parity_settings(mg_tol=1e-5, mg_max_cycles=120, precision=precision)
+    gw = ra.World(CFG, 2)
+    load(gw, inp, parity_params(2), s)
+    ow = oc.OracleWorld(CFG, 2)
+    load(ow, inp, parity_params(2), s)
+    for k in range(2):
+        gw.encode_steps(1)
+        ow.encode_steps(1)
+        cyc, rel = gw.mg_stats()
+        ocyc, orel = ow.mg_stats()
+        print(f"step {k}: GPU {cyc} iterations, true rel {rel:.3e}; oracle {ocyc}, {orel:.3e}", flush=True)
+        assert cyc < 120 and abs(cyc - ocyc) <= max(3, ocyc // 5)
+        assert abs(rel - orel) <= 0.5 * orel
+    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-3), (ra.TEMP, 1e-5)):
+        a, b = gw.download(which).astype(np.float64), ow.download(which).astype(np.float64)
+        e = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+        print(f"{ra.BUFFER_NAMES[which]} rel L2 vs oracle {e:.2e}", flush=True)
+        assert e < tol
+    gw.close()
+    ow.close()
+
+
+def test_headline_world_conserves_monomers_over_a_long_run():
+    """Size-independent property at full size: 256 steps of the tolerance
+    mode with the device ledger checked every step; per-channel activated +
+    spent totals never change, and a host recount agrees."""
+    print("case fullsize conservation", flush=True)
+    inp = headline_inputs()
+    gw = ra.World(CFG, 2)
+    load(gw, inp, parity_params(2), parity_settings(mg_tol=1e-5, mg_max_cycles=80, ledger_check=1))
+    seeded = ra.Ledger.from_cells(inp[ra.CHEM])
+    gw.run(256)
+    gw.sync()
+    assert gw.ledger_status() is None
+    cells = gw.read_cells_blocking()
+    assert ra.Ledger.from_cells(cells).channels() == seeded.channels()
+    assert gw.ledger_blocking().channels() == seeded.channels()
+    gw.close()

@@ -223,3 +223,24 @@ def test_graph_replays_match_eager_recording(tol):
             assert bit_diff(worlds[0].download(which), worlds[1].download(which)) == 0, ra.BUFFER_NAMES[which]
     assert worlds[0].params.epoch == worlds[1].params.epoch == 158
     assert worlds[0].ledger_status() is None and worlds[1].ledger_status() is None
+
+
+def test_idle_reports_finished_work():
+    """ribo_world_idle: false while recorded steps run, true once they are
+    done (bounded polling; sync() must agree)."""
+    import time
+
+    print("case idle", flush=True)
+    cfg = ra.WorldConfig(256, 128)
+    inp = make_inputs(cfg, 21, rock=0.1)
+    w = ra.World(cfg, 21)
+    load(w, inp, parity_params(21), parity_settings())
+    w.sync()
+    assert w.idle()
+    w.encode_steps(64)
+    t0 = time.time()
+    while not w.idle():
+        assert time.time() - t0 < 60.0
+        time.sleep(0.001)
+    w.sync()
+    assert w.idle() and w.params.epoch == 64

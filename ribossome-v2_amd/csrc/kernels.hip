@@ -1611,24 +1611,37 @@ __device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0)
     return f2(1.0f - rb_abs(2.0f * vx - 1.0f), 1.0f - rb_abs(2.0f * vy - 1.0f));
 }
 
-// transport_scatter (transport.wgsl:119-406).  One thread per grid cell of
-// a 16 x 16 tile.  Destinations within SC_R cells of the tile (all of them
-// unless the flow moves more than ~2 cells per step) accumulate in LDS and
-// reach HBM as one coalesced atomic add per non-empty (cell, channel) at the
-// end; farther destinations add to HBM directly.  Integer sums do not depend
-// on order, so the result is the reference's exactly.  The blocked test of
-// each window cell (rock, or more monomers than its capacity) is staged
-// once per block.  Each thread's 16-bin destination histogram lives in LDS.
+// transport_scatter (transport.wgsl:119-406).  One thread per grid cell of a
+// 16 x 16 tile.  The tile's cells are dealt to threads in order of their
+// mover count (a 64-bucket counting sort in LDS), so the 64 threads of a
+// wave carry similar loads, and each thread runs every mover of its cell,
+// channel after channel, in one loop: a wave costs about its largest cell
+// rather than the sum over channels of each channel's largest cell.  Each
+// mover adds its unit to its destination at once: within SC_R cells of the
+// tile (all destinations unless the flow moves more than ~2 cells per step)
+// into an LDS window that reaches HBM as one coalesced atomic add per
+// non-empty (cell, channel) at the end; farther destinations add to HBM
+// directly.  The blocked test (rock, or more monomers than capacity) of the
+// 16 destinations a cell's movers can reach is evaluated once per cell.
+// Integer sums do not depend on order, so the result is the reference's
+// exactly.
 constexpr int SC_T = 16, SC_R = 3, SC_W = SC_T + 2 * SC_R;
+constexpr uint32_t SC_BUCKETS = 64;  // sort key: movers / 2, capped; one bucket per lane of wave 0
+constexpr uint32_t LCG_A = 1664525u;  // the PCG4D input LCG multiplier
 __device__ __forceinline__ float sel4(int d, float a0, float a1, float a2, float a3) {
     return d == 0 ? a0 : (d == 1 ? a1 : (d == 2 ? a2 : a3));
+}
+__device__ __forceinline__ uint32_t movers_of(uint32_t w) {
+    return rb_minu((w & CHEM_STATE_MASK) + (w >> 16), MAX_MOVERS_PER_CH);
 }
 __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
-    __shared__ uint32_t s_bins[16 * 256];
+    __shared__ float4 s_w[4 * 256];  // hop weights of active movers, [channel][thread]
     __shared__ uint32_t s_acc[SC_W * SC_W * 4];
+    __shared__ uint32_t s_hist[SC_BUCKETS];
     __shared__ uint8_t s_tf[SC_W * SC_W];  // bit 0: rock, bit 1: over capacity
+    __shared__ uint8_t s_cell[256];        // tile cell of each thread
     if (ep) P.epoch += *ep;
     const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
     const uint32_t G = g.G, F = g.F;
@@ -1645,9 +1658,34 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         }
         s_tf[k] = f;
     }
+    if (tid < SC_BUCKETS) s_hist[tid] = 0u;
+    uint32_t key = 0u;
+    {
+        const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+        if (x < G && y < G) {
+            const uint4 w = *reinterpret_cast<const uint4*>(chem + 4u * (y * G + x));
+            key = movers_of(w.x) + movers_of(w.y) + movers_of(w.z) + movers_of(w.w);
+        }
+    }
+    const uint32_t bucket = rb_minu(key >> 1, SC_BUCKETS - 1u);
     __syncthreads();
+    const uint32_t rank = atomicAdd(&s_hist[bucket], 1u);
+    __syncthreads();
+    if (tid < SC_BUCKETS) {  // exclusive scan of the bucket counts across wave 0
+        const uint32_t h = s_hist[tid];
+        uint32_t v = h;
+        for (uint32_t off = 1; off < SC_BUCKETS; off <<= 1) {
+            const uint32_t u = __shfl_up(v, off, 64);
+            if (tid >= off) v += u;
+        }
+        s_hist[tid] = v - h;
+    }
+    __syncthreads();
+    s_cell[s_hist[bucket] + rank] = (uint8_t)tid;
+    __syncthreads();
+    const uint32_t cell = s_cell[tid];
 
-    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const uint32_t x = blockIdx.x * 16 + (cell & 15u), y = blockIdx.y * 16 + (cell >> 4);
     const uint32_t idx = y * G + x;
     uint32_t src_total = 0u;
     uint4 cw = make_uint4(0u, 0u, 0u, 0u);
@@ -1743,121 +1781,151 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         // the cell starts there (transport.wgsl:95-117,290-296)
         float2 bound_start = f2(0.0f, 0.0f);
         if (p_bound > 0.0f) bound_start = smooth_start(P, x, y);
-        uint32_t* bins = s_bins + tid;
         const int lself = ((int)y - wy0) * SC_W + ((int)x - wx0);
 
+        // The 4 x 4 destinations of the bins (transport.wgsl:330-380):
+        // a bin in `direct` adds to the window at lbase + oy * SC_W + ox, or
+        // to the cell itself when it is in `blocked`; the other bins (clamped
+        // at the grid edge, or outside the window) take the general path.
+        const int32_t lbase = rb_addi(rb_addi(lself, (int32_t)((uint32_t)base_y * (uint32_t)SC_W)), base_x);
+        uint32_t direct = 0u, blocked = 0u;
+        for (uint32_t b = 0; b < 16; ++b) {
+            const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
+            const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
+            const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
+            const int lx = tx - wx0, ly = ty - wy0;
+            const bool in_win = lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W;
+            bool blk = false;
+            if (t_idx != idx) {
+                uint32_t tf;
+                if (in_win) {
+                    tf = s_tf[ly * SC_W + lx];
+                } else {
+                    const uint32_t g_tgt = g.gamma[t_idx];
+                    tf = g_tgt >= GAMMA_SOLID ? 1u : (cell_total(chem, t_idx) > capacity(g_tgt) ? 2u : 0u);
+                }
+                blk = (tf == 1u) ? (g_src < GAMMA_SOLID) : (tf == 2u);
+            }
+            const bool fast = in_win && ly * SC_W + lx == rb_addi(lbase, (int32_t)((b >> 2) * SC_W + (b & 3u)));
+            blocked |= (uint32_t)blk << b;
+            direct |= (uint32_t)(blk || fast) << b;
+        }
+
+        // Per channel: photoactivation and decay, then the movers' range in
+        // the cell's loop.  Mover k of channel c draws PCG4D at key
+        // idx * 4 + c and stream S_MOVE + k (S_DISPERSE + k for the jitter):
+        // the LCG images of key and stream are affine in c and k.
+        const PcgRun base(idx * 4u, P.epoch, S_MOVE, P.seed);
+        uint32_t mend[4], aend[4], zb[4];
+        uint32_t t0 = 0u;
         for (uint32_t ch = 0; ch < 4; ++ch) {
             const uint32_t slot = idx * 4u + ch;
             const uint32_t v_ch = ch == 0 ? cw.x : (ch == 1 ? cw.y : (ch == 2 ? cw.z : cw.w));
             uint32_t act_n = v_ch & CHEM_STATE_MASK;
             uint32_t spent_n = v_ch >> 16;
-            if (act_n + spent_n == 0u) continue;
-            if (spent_n > 0u) {
-                const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
-                const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
-                                      (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
-                                       rb_max(P.reactivation_rate, 0.0f));
-                uint32_t na = rb_f2u(rb_floor(exp_act));
-                const float frac = exp_act - (float)na;
-                if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
-                na = rb_minu(na, rb_minu(spent_n, 8u));
-                act_n += na;
-                spent_n -= na;
-            }
-            if (act_n > 0u) {
-                const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
-                const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
-                if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
-                    act_n -= 1u;
-                    spent_n += 1u;
+            uint32_t movers = 0u, act_mv = 0u;
+            if (act_n + spent_n != 0u) {
+                if (spent_n > 0u) {
+                    const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
+                    const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
+                                          (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
+                                           rb_max(P.reactivation_rate, 0.0f));
+                    uint32_t na = rb_f2u(rb_floor(exp_act));
+                    const float frac = exp_act - (float)na;
+                    if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
+                    na = rb_minu(na, rb_minu(spent_n, 8u));
+                    act_n += na;
+                    spent_n -= na;
                 }
-            }
-            float ch0 = 1.0f, ch1 = 1.0f, ch2 = 1.0f, ch3 = 1.0f;
-            if (act_n > 0u && P.cohesion > 0.0f) {
-                auto ac = [&](uint32_t cell) { return (float)rb_minu(chem[cell * 4u + ch] & CHEM_STATE_MASK, 8u); };
-                if (x + 1u < G) ch0 = 1.0f + P.cohesion * ac(idx + 1u);
-                if (x > 0u) ch1 = 1.0f + P.cohesion * ac(idx - 1u);
-                if (y + 1u < G) ch2 = 1.0f + P.cohesion * ac(idx + G);
-                if (y > 0u) ch3 = 1.0f + P.cohesion * ac(idx - G);
-            }
-            for (uint32_t b = 0; b < 16; ++b) bins[b * 256u] = 0u;
-            uint32_t used = 0u;
-            uint32_t stay = 0u;
-            const uint32_t n = act_n + spent_n;
-            const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
-            const uint32_t act_stay = act_n - rb_minu(act_n, movers);
-            stay += act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
-            const float wa0 = pr0 * ch0, wa1 = pr1 * ch1, wa2 = pr2 * ch2, wa3 = pr3 * ch3;
-            PcgRun mv(slot, P.epoch, S_MOVE, P.seed);
-            PcgRun dsp(slot, P.epoch, S_DISPERSE, P.seed);
-            for (uint32_t k = 0; k < movers; ++k, mv.advance(), dsp.advance()) {
-                const bool is_act = k < act_n;
-                const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
-                const F4 rf = u4_to_f4(mv.at_current());
-                const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
-                float px = bound ? bound_start.x : rf.x;
-                float py = bound ? bound_start.y : rf.y;
-                px += disp.x;
-                py += disp.y;
-                if (jitter_on) {
-                    const F4 rj = u4_to_f4(dsp.at_current());
-                    const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
-                    const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
-                    px += rb_clamp(jx, -0.9f, 0.9f);
-                    py += rb_clamp(jy, -0.9f, 0.9f);
-                }
-                const float r = rf.z;
-                if (r < p_diff) {
-                    const float w0 = is_act ? wa0 : pr0, w1 = is_act ? wa1 : pr1;
-                    const float w2 = is_act ? wa2 : pr2, w3 = is_act ? wa3 : pr3;
-                    float u = rf.w * (w0 + w1 + w2 + w3);
-                    int d;
-                    if (u < w0) d = 0;
-                    else if ((u -= w0) < w1) d = 1;
-                    else if ((u -= w1) < w2) d = 2;
-                    else d = 3;
-                    const float acc_d = is_act ? sel4(d, ac0, ac1, ac2, ac3) : sel4(d, as0, as1, as2, as3);
-                    const bool stay_put = rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d;
-                    if (!stay_put) {
-                        if (d == 0) px += 1.0f;
-                        else if (d == 1) px -= 1.0f;
-                        else if (d == 2) py += 1.0f;
-                        else py -= 1.0f;
+                if (act_n > 0u) {
+                    const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
+                    const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
+                    if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
+                        act_n -= 1u;
+                        spent_n += 1u;
                     }
-                } else if (r < p_diff + p_settle) {
-                    py -= 1.0f;
                 }
-                const int32_t ox = rb_addi(rb_f2i(rb_floor(px)), -base_x);
-                const int32_t oy = rb_addi(rb_f2i(rb_floor(py)), -base_y);
-                const uint32_t bi = (uint32_t)rb_clampi(oy, 0, 3) * 4u + (uint32_t)rb_clampi(ox, 0, 3);
-                atomicAdd(bins + bi * 256u, unit);
-                used |= 1u << bi;
+                float ch0 = 1.0f, ch1 = 1.0f, ch2 = 1.0f, ch3 = 1.0f;
+                if (act_n > 0u && P.cohesion > 0.0f) {
+                    auto ac = [&](uint32_t c) { return (float)rb_minu(chem[c * 4u + ch] & CHEM_STATE_MASK, 8u); };
+                    if (x + 1u < G) ch0 = 1.0f + P.cohesion * ac(idx + 1u);
+                    if (x > 0u) ch1 = 1.0f + P.cohesion * ac(idx - 1u);
+                    if (y + 1u < G) ch2 = 1.0f + P.cohesion * ac(idx + G);
+                    if (y > 0u) ch3 = 1.0f + P.cohesion * ac(idx - G);
+                }
+                s_w[ch * 256u + tid] = make_float4(pr0 * ch0, pr1 * ch1, pr2 * ch2, pr3 * ch3);
+                const uint32_t n = act_n + spent_n;
+                movers = rb_minu(n, MAX_MOVERS_PER_CH);
+                act_mv = rb_minu(act_n, movers);
+                const uint32_t act_stay = act_n - act_mv;
+                const uint32_t stay = act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
+                if (stay > 0u) atomicAdd(s_acc + lself * 4 + ch, stay);
             }
-            while (used) {
-                const uint32_t b = (uint32_t)__builtin_ctz(used);
-                used &= used - 1u;
-                const uint32_t cnt = bins[b * 256u];
-                const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
-                const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
-                const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
+            aend[ch] = t0 + act_mv;
+            zb[ch] = base.z0 - t0 * LCG_A;
+            t0 += movers;
+            mend[ch] = t0;
+        }
+
+        for (uint32_t t = 0; t < t0; ++t) {
+            const bool c1 = t >= mend[0], c2 = t >= mend[1], c3 = t >= mend[2];
+            const uint32_t ch = c3 ? 3u : (c2 ? 2u : (c1 ? 1u : 0u));
+            const bool is_act = t < (c3 ? aend[3] : (c2 ? aend[2] : (c1 ? aend[1] : aend[0])));
+            PcgRun mv = base;
+            mv.x1 = base.x1 + ch * LCG_A;
+            mv.x1s = mv.x1 ^ (mv.x1 >> 16);
+            mv.z0 = (c3 ? zb[3] : (c2 ? zb[2] : (c1 ? zb[1] : zb[0]))) + __umul24(t, LCG_A);
+            const F4 rf = u4_to_f4(mv.at_current());
+            const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
+            float px = bound ? bound_start.x : rf.x;
+            float py = bound ? bound_start.y : rf.y;
+            px += disp.x;
+            py += disp.y;
+            if (jitter_on) {
+                PcgRun dsp = mv;
+                dsp.z0 = mv.z0 + (S_DISPERSE - S_MOVE) * LCG_A;
+                const F4 rj = u4_to_f4(dsp.at_current());
+                const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                px += rb_clamp(jx, -0.9f, 0.9f);
+                py += rb_clamp(jy, -0.9f, 0.9f);
+            }
+            const float r = rf.z;
+            if (r < p_diff) {
+                const float4 wa = s_w[ch * 256u + tid];
+                const float w0 = is_act ? wa.x : pr0, w1 = is_act ? wa.y : pr1;
+                const float w2 = is_act ? wa.z : pr2, w3 = is_act ? wa.w : pr3;
+                float u = rf.w * (w0 + w1 + w2 + w3);
+                int d;
+                if (u < w0) d = 0;
+                else if ((u -= w0) < w1) d = 1;
+                else if ((u -= w1) < w2) d = 2;
+                else d = 3;
+                const float acc_d = is_act ? sel4(d, ac0, ac1, ac2, ac3) : sel4(d, as0, as1, as2, as3);
+                const bool stay_put = rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d;
+                if (!stay_put) {
+                    if (d == 0) px += 1.0f;
+                    else if (d == 1) px -= 1.0f;
+                    else if (d == 2) py += 1.0f;
+                    else py -= 1.0f;
+                }
+            } else if (r < p_diff + p_settle) {
+                py -= 1.0f;
+            }
+            const int32_t ox = rb_clampi(rb_addi(rb_f2i(rb_floor(px)), -base_x), 0, 3);
+            const int32_t oy = rb_clampi(rb_addi(rb_f2i(rb_floor(py)), -base_y), 0, 3);
+            const uint32_t bi = (uint32_t)oy * 4u + (uint32_t)ox;
+            const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
+            if ((direct >> bi) & 1u) {
+                const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, oy * SC_W + ox);
+                atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
+            } else {
+                const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), ox), 0, (int32_t)G - 1);
+                const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), oy), 0, (int32_t)G - 1);
                 const int lx = tx - wx0, ly = ty - wy0;
-                const bool in_win = lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W;
-                bool blocked = false;
-                if (t_idx != idx) {
-                    uint32_t tf;
-                    if (in_win) {
-                        tf = s_tf[ly * SC_W + lx];
-                    } else {
-                        const uint32_t g_tgt = g.gamma[t_idx];
-                        tf = g_tgt >= GAMMA_SOLID ? 1u : (cell_total(chem, t_idx) > capacity(g_tgt) ? 2u : 0u);
-                    }
-                    blocked = (tf == 1u) ? (g_src < GAMMA_SOLID) : (tf == 2u);
-                }
-                if (blocked) stay += cnt;
-                else if (in_win) atomicAdd(s_acc + (ly * SC_W + lx) * 4 + ch, cnt);
-                else atomicAdd(next + t_idx * 4u + ch, cnt);
+                if (lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W) atomicAdd(s_acc + (ly * SC_W + lx) * 4 + (int)ch, unit);
+                else atomicAdd(next + ((uint32_t)ty * G + (uint32_t)tx) * 4u + ch, unit);
             }
-            if (stay > 0u) atomicAdd(s_acc + lself * 4 + ch, stay);
         }
     }
     __syncthreads();

@@ -1597,15 +1597,24 @@ __device__ __forceinline__ float2 lattice_rand(const RiboSimParams& P, uint32_t 
     F4 r = rng_f4((ix * 73856093u) ^ (iy * 19349663u), P.epoch, S_MOVE + MAX_MOVERS_PER_CH + 1u, P.seed);
     return f2(r.x, r.y);
 }
-__device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0) {
+// The lattice points of smooth_start a 16 x 16 tile at (bx, by) * 16 reads:
+// x + jitter lies in [16 bx, 16 bx + 30], so gx and gx + 1 in bx .. bx + 2.
+// lat holds lattice_rand(bx + i % 3, by + i / 3) at i.
+__device__ __forceinline__ void smooth_jitter(const RiboSimParams& P, uint32_t& jx, uint32_t& jy) {
     const F4 jit = rng_f4(0x85EBCA6Bu, P.epoch, S_MOVE + MAX_MOVERS_PER_CH + 2u, P.seed);
-    const uint32_t x = x0 + rb_f2u(jit.x * (float)SMOOTH_CELL);
-    const uint32_t y = y0 + rb_f2u(jit.y * (float)SMOOTH_CELL);
-    const uint32_t gx = x / SMOOTH_CELL, gy = y / SMOOTH_CELL;
+    jx = rb_f2u(jit.x * (float)SMOOTH_CELL);
+    jy = rb_f2u(jit.y * (float)SMOOTH_CELL);
+}
+__device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0, const float2* lat, uint32_t bx, uint32_t by) {
+    uint32_t jx, jy;
+    smooth_jitter(P, jx, jy);
+    const uint32_t x = x0 + jx;
+    const uint32_t y = y0 + jy;
+    const uint32_t ix = x / SMOOTH_CELL - bx, iy = y / SMOOTH_CELL - by;
     const float tx = rb_smoothstep01(((float)(x % SMOOTH_CELL) + 0.5f) / (float)SMOOTH_CELL);
     const float ty = rb_smoothstep01(((float)(y % SMOOTH_CELL) + 0.5f) / (float)SMOOTH_CELL);
-    const float2 n = mix2(mix2(lattice_rand(P, gx, gy), lattice_rand(P, gx + 1u, gy), tx),
-                          mix2(lattice_rand(P, gx, gy + 1u), lattice_rand(P, gx + 1u, gy + 1u), tx), ty);
+    const float2* l0 = lat + iy * 3u + ix;
+    const float2 n = mix2(mix2(l0[0], l0[1], tx), mix2(l0[3], l0[4], tx), ty);
     const F4 gg = rng_f4(0x9E3779B9u, P.epoch, S_MOVE + MAX_MOVERS_PER_CH, P.seed);
     const float vx = rb_fract(n.x + gg.x), vy = rb_fract(n.y + gg.y);
     return f2(1.0f - rb_abs(2.0f * vx - 1.0f), 1.0f - rb_abs(2.0f * vy - 1.0f));
@@ -1626,29 +1635,32 @@ __device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0)
 // Integer sums do not depend on order, so the result is the reference's
 // exactly.
 constexpr int SC_T = 16, SC_R = 3, SC_W = SC_T + 2 * SC_R;
+constexpr int SC_TS = 24;            // LDS row stride of the window flags (bytes, a multiple of 4)
 constexpr uint32_t SC_BUCKETS = 64;  // sort key: movers / 2, capped; one bucket per lane of wave 0
+static_assert(SMOOTH_CELL == (uint32_t)SC_T, "the smooth_start lattice is staged per tile");
 constexpr uint32_t LCG_A = 1664525u;  // the PCG4D input LCG multiplier
-__device__ __forceinline__ float sel4(int d, float a0, float a1, float a2, float a3) {
-    return d == 0 ? a0 : (d == 1 ? a1 : (d == 2 ? a2 : a3));
-}
 __device__ __forceinline__ uint32_t movers_of(uint32_t w) {
     return rb_minu((w & CHEM_STATE_MASK) + (w >> 16), MAX_MOVERS_PER_CH);
 }
 __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
-    __shared__ float4 s_w[4 * 256];  // hop weights of active movers, [channel][thread]
-    __shared__ uint32_t s_acc[SC_W * SC_W * 4];
+    __shared__ float4 s_w[4 * 256];        // hop weights of active movers, [channel][thread]
+    __shared__ uint4 s_acc4[SC_W * SC_W];  // [window cell][channel]
     __shared__ uint32_t s_hist[SC_BUCKETS];
-    __shared__ uint8_t s_tf[SC_W * SC_W];  // bit 0: rock, bit 1: over capacity
-    __shared__ uint8_t s_cell[256];        // tile cell of each thread
+    __shared__ uint32_t s_tf4[SC_W * SC_TS / 4];  // window flags, rows of SC_TS bytes: 1 rock, 2 over capacity
+    __shared__ float2 s_lat[9];
+    __shared__ uint8_t s_cell[256];  // tile cell of each thread
+    uint32_t* const s_acc = reinterpret_cast<uint32_t*>(s_acc4);
+    uint8_t* const s_tf = reinterpret_cast<uint8_t*>(s_tf4);
     if (ep) P.epoch += *ep;
     const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
     const uint32_t G = g.G, F = g.F;
     const int wx0 = (int)(blockIdx.x * SC_T) - SC_R, wy0 = (int)(blockIdx.y * SC_T) - SC_R;
-    for (uint32_t k = tid; k < SC_W * SC_W * 4; k += 256) s_acc[k] = 0u;
+    for (uint32_t k = tid; k < SC_W * SC_W; k += 256) s_acc4[k] = make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t k = tid; k < SC_W * SC_W; k += 256) {
-        const int gx = wx0 + (int)(k % SC_W), gy = wy0 + (int)(k / SC_W);
+        const uint32_t kx = k % SC_W, ky = k / SC_W;
+        const int gx = wx0 + (int)kx, gy = wy0 + (int)ky;
         uint8_t f = 0;
         if (gx >= 0 && gy >= 0 && gx < (int)G && gy < (int)G) {
             const uint32_t c = (uint32_t)gy * G + (uint32_t)gx;
@@ -1656,9 +1668,10 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
             if (gm >= GAMMA_SOLID) f = 1;
             else if (cell_total(chem, c) > capacity(gm)) f = 2;
         }
-        s_tf[k] = f;
+        s_tf[ky * SC_TS + kx] = f;
     }
     if (tid < SC_BUCKETS) s_hist[tid] = 0u;
+    if (tid < 9u) s_lat[tid] = lattice_rand(P, blockIdx.x + tid % 3u, blockIdx.y + tid / 3u);
     uint32_t key = 0u;
     {
         const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
@@ -1763,11 +1776,13 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
             const float t_rel = 1.0f + rb_max(temp[fy * F + fx], 0.0f) / TEMP_ACT_THRESHOLD;
             const float room = rb_clamp(5.0f * (1.0f - (float)src_total / (float)rb_maxu(cap_self, 1u)), 0.0f, 1.0f);
             const float k_e = P.aggregation * room / t_rel;
+            // en - e0 is exactly -(e0 - en), so one exp of -k_e |e0 - en|
+            // serves the side that is positive
             auto acc_of = [&](float en, float& acc, float& accs) {
                 const float drop = e0 - en;
-                if (drop > 0.0f) acc = rb_expf(-k_e * drop);
-                const float rise = en - e0;
-                if (rise > 0.0f) accs = rb_expf(-k_e * rise);
+                const float v = rb_expf(-k_e * rb_abs(drop));
+                acc = drop > 0.0f ? v : acc;
+                accs = drop < 0.0f ? v : accs;
             };
             acc_of(e1, ac0, as0);
             acc_of(e2, ac1, as1);
@@ -1780,7 +1795,7 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         // smooth_start is a function of the cell alone; every bound mover of
         // the cell starts there (transport.wgsl:95-117,290-296)
         float2 bound_start = f2(0.0f, 0.0f);
-        if (p_bound > 0.0f) bound_start = smooth_start(P, x, y);
+        if (p_bound > 0.0f) bound_start = smooth_start(P, x, y, s_lat, blockIdx.x, blockIdx.y);
         const int lself = ((int)y - wy0) * SC_W + ((int)x - wx0);
 
         // The 4 x 4 destinations of the bins (transport.wgsl:330-380):
@@ -1788,27 +1803,49 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         // to the cell itself when it is in `blocked`; the other bins (clamped
         // at the grid edge, or outside the window) take the general path.
         const int32_t lbase = rb_addi(rb_addi(lself, (int32_t)((uint32_t)base_y * (uint32_t)SC_W)), base_x);
-        uint32_t direct = 0u, blocked = 0u;
-        for (uint32_t b = 0; b < 16; ++b) {
-            const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
-            const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
-            const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
-            const int lx = tx - wx0, ly = ty - wy0;
-            const bool in_win = lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W;
-            bool blk = false;
-            if (t_idx != idx) {
-                uint32_t tf;
-                if (in_win) {
-                    tf = s_tf[ly * SC_W + lx];
-                } else {
-                    const uint32_t g_tgt = g.gamma[t_idx];
-                    tf = g_tgt >= GAMMA_SOLID ? 1u : (cell_total(chem, t_idx) > capacity(g_tgt) ? 2u : 0u);
-                }
-                blk = (tf == 1u) ? (g_src < GAMMA_SOLID) : (tf == 2u);
+        uint32_t direct, blocked = 0u;
+        const int32_t lx0 = rb_addi((int)x - wx0, base_x), ly0 = rb_addi((int)y - wy0, base_y);
+        const bool block4 = base_x >= -SC_W && base_x <= SC_W && base_y >= -SC_W && base_y <= SC_W &&
+                            lx0 >= 0 && lx0 + 3 < SC_W && ly0 >= 0 && ly0 + 3 < SC_W &&
+                            (int)x + base_x >= 0 && (int)x + base_x + 3 < (int)G &&
+                            (int)y + base_y >= 0 && (int)y + base_y + 3 < (int)G;
+        if (block4) {
+            // all 16 destinations inside the grid and the window: one
+            // 4-byte row of flags per bin row; a byte is blocked when it
+            // is 2, or 1 while this cell is not rock
+            const uint32_t rockm = g_src < GAMMA_SOLID ? 0x01010101u : 0u;
+            for (int i = 0; i < 4; ++i) {
+                const uint32_t off = (uint32_t)((ly0 + i) * SC_TS + lx0);
+                const uint32_t v = __builtin_amdgcn_alignbyte(s_tf4[(off >> 2) + 1u], s_tf4[off >> 2], off & 3u);
+                const uint32_t b = ((v >> 1) & 0x01010101u) | (v & rockm);
+                blocked |= ((b * 0x01020408u) >> 24) << (4 * i);  // byte k's low bit to bit k
             }
-            const bool fast = in_win && ly * SC_W + lx == rb_addi(lbase, (int32_t)((b >> 2) * SC_W + (b & 3u)));
-            blocked |= (uint32_t)blk << b;
-            direct |= (uint32_t)(blk || fast) << b;
+            if (base_x <= 0 && base_x >= -3 && base_y <= 0 && base_y >= -3)  // the cell itself
+                blocked &= ~(1u << (uint32_t)(-base_y * 4 - base_x));
+            direct = 0xFFFFu;
+        } else {
+            direct = 0u;
+            for (uint32_t b = 0; b < 16; ++b) {
+                const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), (int32_t)(b & 3u)), 0, (int32_t)G - 1);
+                const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), (int32_t)(b >> 2)), 0, (int32_t)G - 1);
+                const uint32_t t_idx = (uint32_t)ty * G + (uint32_t)tx;
+                const int lx = tx - wx0, ly = ty - wy0;
+                const bool in_win = lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W;
+                bool blk = false;
+                if (t_idx != idx) {
+                    uint32_t tf;
+                    if (in_win) {
+                        tf = s_tf[ly * SC_TS + lx];
+                    } else {
+                        const uint32_t g_tgt = g.gamma[t_idx];
+                        tf = g_tgt >= GAMMA_SOLID ? 1u : (cell_total(chem, t_idx) > capacity(g_tgt) ? 2u : 0u);
+                    }
+                    blk = (tf == 1u) ? (g_src < GAMMA_SOLID) : (tf == 2u);
+                }
+                const bool fast = in_win && ly * SC_W + lx == rb_addi(lbase, (int32_t)((b >> 2) * SC_W + (b & 3u)));
+                blocked |= (uint32_t)blk << b;
+                direct |= (uint32_t)(blk || fast) << b;
+            }
         }
 
         // Per channel: photoactivation and decay, then the movers' range in
@@ -1890,28 +1927,25 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
                 px += rb_clamp(jx, -0.9f, 0.9f);
                 py += rb_clamp(jy, -0.9f, 0.9f);
             }
+            // the hop (transport.wgsl:300-330) as selects: direction d is
+            // the first of u < w0, u - w0 < w1, u - w0 - w1 < w2 to hold
+            // (else 3); a position steps by +-1 or by 0, and x + 0 differs
+            // from x only for x = -0, which floors to the same bin
             const float r = rf.z;
-            if (r < p_diff) {
-                const float4 wa = s_w[ch * 256u + tid];
-                const float w0 = is_act ? wa.x : pr0, w1 = is_act ? wa.y : pr1;
-                const float w2 = is_act ? wa.z : pr2, w3 = is_act ? wa.w : pr3;
-                float u = rf.w * (w0 + w1 + w2 + w3);
-                int d;
-                if (u < w0) d = 0;
-                else if ((u -= w0) < w1) d = 1;
-                else if ((u -= w1) < w2) d = 2;
-                else d = 3;
-                const float acc_d = is_act ? sel4(d, ac0, ac1, ac2, ac3) : sel4(d, as0, as1, as2, as3);
-                const bool stay_put = rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d;
-                if (!stay_put) {
-                    if (d == 0) px += 1.0f;
-                    else if (d == 1) px -= 1.0f;
-                    else if (d == 2) py += 1.0f;
-                    else py -= 1.0f;
-                }
-            } else if (r < p_diff + p_settle) {
-                py -= 1.0f;
-            }
+            const bool hop = r < p_diff;
+            const float4 wa = s_w[ch * 256u + tid];
+            const float w0 = is_act ? wa.x : pr0, w1 = is_act ? wa.y : pr1;
+            const float w2 = is_act ? wa.z : pr2, w3 = is_act ? wa.w : pr3;
+            const float u0 = rf.w * (w0 + w1 + w2 + w3);
+            const float u1 = u0 - w0, u2 = u1 - w1;
+            const bool d0 = u0 < w0, d1 = u1 < w1, d2 = u2 < w2;
+            const float a0 = is_act ? ac0 : as0, a1 = is_act ? ac1 : as1;
+            const float a2 = is_act ? ac2 : as2, a3 = is_act ? ac3 : as3;
+            const float acc_d = d0 ? a0 : (d1 ? a1 : (d2 ? a2 : a3));
+            const bool moves = hop && !(rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d);
+            const bool vert = !d0 && !d1;
+            px += (moves && !vert) ? (d0 ? 1.0f : -1.0f) : 0.0f;
+            py += (moves && vert) ? (d2 ? 1.0f : -1.0f) : ((!hop && r < p_diff + p_settle) ? -1.0f : 0.0f);
             const int32_t ox = rb_clampi(rb_addi(rb_f2i(rb_floor(px)), -base_x), 0, 3);
             const int32_t oy = rb_clampi(rb_addi(rb_f2i(rb_floor(py)), -base_y), 0, 3);
             const uint32_t bi = (uint32_t)oy * 4u + (uint32_t)ox;
@@ -1929,11 +1963,15 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         }
     }
     __syncthreads();
-    for (uint32_t k = tid; k < SC_W * SC_W * 4; k += 256) {
-        const uint32_t v = s_acc[k];
-        if (v == 0u) continue;
-        const int gx = wx0 + (int)((k / 4) % SC_W), gy = wy0 + (int)((k / 4) / SC_W);
-        atomicAdd(next + ((uint32_t)gy * G + (uint32_t)gx) * 4u + (k & 3u), v);
+    for (uint32_t k = tid; k < SC_W * SC_W; k += 256) {
+        const uint4 v = s_acc4[k];
+        if ((v.x | v.y | v.z | v.w) == 0u) continue;  // (only cells inside the grid receive adds)
+        const int gx = wx0 + (int)(k % SC_W), gy = wy0 + (int)(k / SC_W);
+        uint32_t* const dst = next + ((uint32_t)gy * G + (uint32_t)gx) * 4u;
+        if (v.x != 0u) atomicAdd(dst, v.x);
+        if (v.y != 0u) atomicAdd(dst + 1, v.y);
+        if (v.z != 0u) atomicAdd(dst + 2, v.z);
+        if (v.w != 0u) atomicAdd(dst + 3, v.w);
     }
 }
 

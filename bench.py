@@ -62,12 +62,21 @@ BYTES_PER_UNIT = {
 }
 MG_LDS_LIMIT = 65536
 MG_BLOCK_LEVELS = 8
+MG_SYM_COARSE_MAX = 1024
+
+
+def mg_block_lds(sizes):
+    """LDS bytes of the one-workgroup multigrid kernel over these levels:
+    unknowns, right-hand sides and masks, the staged coarsest right-hand side
+    and res / W of the top level."""
+    cells = sum(n * n for n in sizes)
+    return ((9 * cells + 3) & ~3) + 4 * MG_SYM_COARSE_MAX + 4 * sizes[0] * sizes[0]
 
 
 def block_start(sizes):
     """First multigrid level solved inside one workgroup (the library's rule)."""
     b = len(sizes)
-    while b > 0 and len(sizes) - (b - 1) <= MG_BLOCK_LEVELS and 9 * sum(n * n for n in sizes[b - 1:]) <= MG_LDS_LIMIT:
+    while b > 0 and len(sizes) - (b - 1) <= MG_BLOCK_LEVELS and mg_block_lds(sizes[b - 1:]) <= MG_LDS_LIMIT:
         b -= 1
     return b
 

@@ -849,6 +849,7 @@ template <class R> struct MG {
     bool bf16 = false;              // round stores of the preconditioner to bfloat16
     const CoarseInv* cinv = nullptr;  // exact solve of the last level of vcycle_sym (tolerance mode)
     size_t sym_levels = 0;            // levels vcycle_sym descends through (0: all)
+    std::vector<R> rn_scratch;        // res / W of a fine level during restriction
     bool is_fluid(const Level& L, int32_t x, int32_t y) const {  // :29-32
         if (x < 0 || y < 0 || x >= (int32_t)L.n || y >= (int32_t)L.n) return false;
         return fluid[L.off + (uint32_t)y * L.n + (uint32_t)x] != 0u;
@@ -921,37 +922,119 @@ template <class R> struct MG {
         if (!bf16) return;
         for (uint32_t i = 0; i < L.n * L.n; ++i) a[L.off + i] = R(round_bf16((float)a[L.off + i]));
     }
+    // Bilinear (cell-centred) transfer of the tolerance mode between a level
+    // of even side n and its coarse level (n / 2).  In each direction fine
+    // index f takes its parent a0 = f / 2 with weight 3/4 and the coarse
+    // cell on its side, a1 (clamped at the edge, where it repeats a0), with
+    // weight 1/4.  The four products, in the order (a0y, a0x), (a0y, a1x),
+    // (a1y, a0x), (a1y, a1x), count only fluid coarse cells and are divided
+    // by the sum W of their weights (W >= 9/16: the parent of a fluid cell
+    // is fluid).  Restriction is the transpose: a fluid coarse cell gathers
+    // wt(f, c) res_f / W_f over its 4 x 4 fine cells 2c - 1 .. 2c + 2, rows
+    // then columns, where wt(f, c) = 3/4 [a0(f) = c] + 1/4 [a1(f) = c].
+    static void bl_nf(uint32_t f, uint32_t nc, uint32_t* a0, uint32_t* a1) {
+        *a0 = f >> 1;
+        *a1 = (f & 1u) ? std::min(*a0 + 1u, nc - 1u) : (*a0 > 0u ? *a0 - 1u : 0u);
+    }
+    static R bl_wt(uint32_t f, uint32_t c, uint32_t nc) {
+        uint32_t a0, a1;
+        bl_nf(f, nc, &a0, &a1);
+        return (a0 == c ? R(0.75) : R(0)) + (a1 == c ? R(0.25) : R(0));
+    }
+    // W_f and the interpolated coarse value at fine cell (fx, fy) of L.
+    void bl_interp(const Level& L, uint32_t fx, uint32_t fy, R* wsum, R* val) const {
+        const uint32_t nc = L.n_c;
+        uint32_t x0, x1, y0, y1;
+        bl_nf(fx, nc, &x0, &x1);
+        bl_nf(fy, nc, &y0, &y1);
+        const uint32_t cx[4] = {x0, x1, x0, x1}, cy[4] = {y0, y0, y1, y1};
+        const R wk[4] = {R(0.5625), R(0.1875), R(0.1875), R(0.0625)};
+        R acc = 0, ws = 0;
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t ci = L.off_c + cy[k] * nc + cx[k];
+            const bool f = fluid[ci] != 0u;
+            acc = acc + (f ? wk[k] * p[ci] : R(0));
+            ws = ws + (f ? wk[k] : R(0));
+        }
+        *wsum = ws;
+        *val = acc;
+    }
+    void restrict_bl(const Level& Lf, std::vector<R>& rn) {
+        const uint32_t n = Lf.n, nc = Lf.n_c;
+        rn.assign((size_t)n * n, R(0));
+        for (uint32_t fy = 0; fy < n; ++fy)
+            for (uint32_t fx = 0; fx < n; ++fx) {
+                const uint32_t i = Lf.off + fy * n + fx;
+                if (fluid[i] == 0u) continue;
+                R s, k, ws, v;
+                neighbors(Lf, fx, fy, &s, &k);
+                bl_interp(Lf, fx, fy, &ws, &v);
+                rn[(size_t)fy * n + fx] = (rhs[i] - (s - (k + R(Lf.eps)) * p[i])) / ws;
+            }
+        for (uint32_t gy = 0; gy < nc; ++gy)
+            for (uint32_t gx = 0; gx < nc; ++gx) {
+                R sum = 0;
+                for (int dy = -1; dy <= 2; ++dy) {
+                    const int64_t fy = 2 * (int64_t)gy + dy;
+                    for (int dx = -1; dx <= 2; ++dx) {
+                        const int64_t fx = 2 * (int64_t)gx + dx;
+                        if (fy < 0 || fx < 0 || fy >= n || fx >= n) continue;
+                        const R w = bl_wt((uint32_t)fy, gy, nc) * bl_wt((uint32_t)fx, gx, nc);
+                        sum = sum + w * rn[(size_t)fy * n + (size_t)fx];
+                    }
+                }
+                const uint32_t ci = Lf.off_c + gy * nc + gx;
+                rhs[ci] = fluid[ci] ? sum : R(0);
+                p[ci] = 0;
+            }
+    }
+    void prolong_bl(const Level& L) {
+        for (uint32_t fy = 0; fy < L.n; ++fy)
+            for (uint32_t fx = 0; fx < L.n; ++fx) {
+                const uint32_t i = L.off + fy * L.n + fx;
+                if (fluid[i] == 0u) continue;
+                R ws, v;
+                bl_interp(L, fx, fy, &ws, &v);
+                p[i] += v / ws;
+            }
+    }
+    // Restriction proportional to the transpose of the injection: half the
+    // sum over fluid children (the reference's twice their mean, on
+    // all-fluid cells).
+    void restrict_half_sum(const Level& Lf) {
+        for (uint32_t gy = 0; gy < Lf.n_c; ++gy)
+            for (uint32_t gx = 0; gx < Lf.n_c; ++gx) {
+                R sum = 0;
+                uint32_t nf = 0;
+                for (uint32_t c = 0; c < 4; ++c) {
+                    const uint32_t fx = gx * 2u + (c & 1u), fy = gy * 2u + (c >> 1);
+                    const uint32_t i = Lf.off + fy * Lf.n + fx;
+                    if (fluid[i] == 0u) continue;
+                    R s, k;
+                    neighbors(Lf, fx, fy, &s, &k);
+                    sum += rhs[i] - (s - (k + R(Lf.eps)) * p[i]);
+                    nf += 1;
+                }
+                const uint32_t ci = Lf.off_c + gy * Lf.n_c + gx;
+                rhs[ci] = nf > 0 ? R(0.5) * sum : R(0);
+                p[ci] = 0;
+            }
+    }
     // The symmetric V-cycle of the tolerance mode: the reference V-cycle with
-    // the post-smoother in reverse colour order (black, red), restriction
-    // proportional to the transpose of the injection (half the sum over fluid
-    // children instead of twice their mean; equal on all-fluid cells) and an
-    // exact (or colour-symmetric) coarsest solve.  It is a symmetric
-    // preconditioner, as conjugate gradients requires.
+    // the post-smoother in reverse colour order (black, red), a symmetric
+    // transfer -- the bilinear one above in binary32 and binary64; with
+    // bfloat16 storage, injection and half the sum over fluid children
+    // (rounding the bilinear cycle's output to bfloat16 stalls conjugate
+    // gradients) -- and an exact (or colour-symmetric) coarsest solve.  It
+    // is a symmetric preconditioner, as conjugate gradients requires.
     void vcycle_sym() {
         const std::vector<Level>& L = *lv;
         const size_t levels = sym_levels ? std::min(sym_levels, L.size()) : L.size();
         for (size_t l = 0; l + 1 < levels; ++l) {
             for (uint32_t s2 = 0; s2 < MG_PRE; ++s2) { smooth(L[l], 0); smooth(L[l], 1); }
             round_level(L[l], p);
-            const Level& Lf = L[l];
-            for (uint32_t gy = 0; gy < Lf.n_c; ++gy)
-                for (uint32_t gx = 0; gx < Lf.n_c; ++gx) {
-                    R sum = 0;
-                    uint32_t nf = 0;
-                    for (uint32_t c = 0; c < 4; ++c) {
-                        const uint32_t fx = gx * 2u + (c & 1u), fy = gy * 2u + (c >> 1);
-                        const uint32_t i = Lf.off + fy * Lf.n + fx;
-                        if (fluid[i] == 0u) continue;
-                        R s, k;
-                        neighbors(Lf, fx, fy, &s, &k);
-                        sum += rhs[i] - (s - (k + R(Lf.eps)) * p[i]);
-                        nf += 1;
-                    }
-                    const uint32_t ci = Lf.off_c + gy * Lf.n_c + gx;
-                    fluid[ci] = nf > 0 ? 1u : 0u;
-                    rhs[ci] = nf > 0 ? R(0.5) * sum : R(0);
-                    p[ci] = 0;
-                }
+            if (bf16) restrict_half_sum(L[l]);
+            else restrict_bl(L[l], rn_scratch);
             round_level(L[l + 1], rhs);
         }
         const Level& C = L[levels - 1];
@@ -969,7 +1052,8 @@ template <class R> struct MG {
             for (uint32_t s2 = 0; s2 < MG_COARSE / 2; ++s2) { smooth(C, 1); smooth(C, 0); }
         }
         for (size_t l = levels - 1; l-- > 0;) {
-            prolong(L[l]);
+            if (bf16) prolong(L[l]);
+            else prolong_bl(L[l]);
             for (uint32_t s2 = 0; s2 < MG_POST; ++s2) { smooth(L[l], 1); smooth(L[l], 0); }
             round_level(L[l], p);
         }

@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "wgsl_sem.h"
 
@@ -785,14 +787,158 @@ __device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L,
     }
 }
 
+// Bilinear (cell-centred) transfer of the tolerance mode between a level
+// of even side n and its coarse level (n / 2).  In each direction fine
+// index f takes its parent a0 = f / 2 with weight 3/4 and the coarse cell
+// on its side, a1 (clamped at the edge, where it repeats a0), with weight
+// 1/4.  Prolongation adds the four products (a0y a0x, a0y a1x, a1y a0x,
+// a1y a1x), over fluid coarse cells, divided by the sum W of their weights
+// (W >= 9/16: the parent of a fluid cell is fluid).  Restriction is its
+// transpose: a fluid coarse cell gathers wt(f, c) res_f / W_f over its 4 x 4
+// fine cells 2c - 1 .. 2c + 2, rows then columns, with
+// wt(f, c) = 3/4 [a0(f) = c] + 1/4 [a1(f) = c].
+// The binary32 preconditioner uses it; the bfloat16 one keeps the
+// piecewise-constant transfer (prolongation by injection, restriction half
+// the sum over fluid children): rounding the bilinear V-cycle's output to
+// bfloat16 stalls conjugate gradients.
+template <class S> constexpr bool BL_TRANSFER = std::is_same<S, float>::value;
+__device__ __forceinline__ void bl_nf(int f, int nc, int& a0, int& a1) {
+    a0 = f >> 1;
+    a1 = (f & 1) ? (a0 + 1 < nc ? a0 + 1 : a0) : (a0 > 0 ? a0 - 1 : 0);
+}
+__device__ __forceinline__ float bl_wt(int f, int c, int nc) {
+    int a0, a1;
+    bl_nf(f, nc, a0, a1);
+    return (a0 == c ? 0.75f : 0.0f) + (a1 == c ? 0.25f : 0.0f);
+}
+// W and the weighted sum of the coarse values at fine cell (fx, fy):
+// cf(cx, cy) and cv(cx, cy) read a coarse flag and value.
+template <class CF, class CV>
+__device__ __forceinline__ void bl_interp(int fx, int fy, int nc, CF cf, CV cv, float& ws, float& acc) {
+    int x0, x1, y0, y1;
+    bl_nf(fx, nc, x0, x1);
+    bl_nf(fy, nc, y0, y1);
+    const bool f0 = cf(x0, y0), f1 = cf(x1, y0), f2 = cf(x0, y1), f3 = cf(x1, y1);
+    const float v0 = cv(x0, y0), v1 = cv(x1, y0), v2 = cv(x0, y1), v3 = cv(x1, y1);
+    acc = 0.0f;
+    ws = 0.0f;
+    acc = acc + (f0 ? 0.5625f * v0 : 0.0f);
+    ws = ws + (f0 ? 0.5625f : 0.0f);
+    acc = acc + (f1 ? 0.1875f * v1 : 0.0f);
+    ws = ws + (f1 ? 0.1875f : 0.0f);
+    acc = acc + (f2 ? 0.1875f * v2 : 0.0f);
+    ws = ws + (f2 ? 0.1875f : 0.0f);
+    acc = acc + (f3 ? 0.0625f * v3 : 0.0f);
+    ws = ws + (f3 ? 0.0625f : 0.0f);
+}
+// The restriction half of a tile kernel: rn (TT + 2)^2 holds res / W of
+// the fine cells one ring around the T x T core at fine origin fx0 - 1 (0
+// outside the grid or off the fluid); cfl the coarse flags (T/2 + 2)^2 at
+// coarse origin cx0 - 1.  rn is filled from the tile (whose values must be
+// exact two cells beyond the core), then each coarse cell of the core
+// gathers its 16 fine cells.
+template <int H, int T, class S>
+__device__ __forceinline__ void bl_restrict_tile(const MgTile<H, T>& t, float* rn, const uint8_t* cfl, int n, float eps,
+                                                 const MgLevelT<S>& C, int x0, int y0) {
+    constexpr int RN = T + 2, CF = T / 2 + 2;
+    const int nc = (int)C.n;
+    const int fx0 = x0 + H - 1, fy0 = y0 + H - 1;
+    const int cx0 = (x0 + H) / 2 - 1, cy0 = (y0 + H) / 2 - 1;
+    for (int k = threadIdx.x; k < RN * RN; k += blockDim.x) {
+        const int ly = H - 1 + k / RN, lx = H - 1 + k % RN;
+        const int gx = fx0 + k % RN, gy = fy0 + k / RN;
+        float v = 0.0f;
+        if (gx >= 0 && gy >= 0 && gx < n && gy < n && t.fl[ly][lx]) {
+            const NbSum nb = tile_nb(t, ly, lx);
+            const float res = t.rhs[ly][lx] - (nb.s - (nb.k + eps) * t.p[ly][lx]);
+            float ws, acc;
+            bl_interp(
+                gx, gy, nc, [&](int cx, int cy) { return cfl[(cy - cy0) * CF + (cx - cx0)] != 0; },
+                [](int, int) { return 0.0f; }, ws, acc);
+            v = res / ws;
+        }
+        rn[k] = v;
+    }
+    __syncthreads();
+    const int ct = threadIdx.x;  // one thread per coarse cell of the tile
+    if (ct < (T / 2) * (T / 2)) {
+        const int cx = (x0 + H) / 2 + ct % (T / 2), cy = (y0 + H) / 2 + ct / (T / 2);
+        if (cx < nc && cy < nc) {
+            float sum = 0.0f;
+            for (int dy = -1; dy <= 2; ++dy) {
+                const int fy = 2 * cy + dy;
+                const float wy = (fy >= 0 && fy < n) ? bl_wt(fy, cy, nc) : 0.0f;
+                for (int dx = -1; dx <= 2; ++dx) {
+                    const int fx = 2 * cx + dx;
+                    const float wx = (fx >= 0 && fx < n) ? bl_wt(fx, cx, nc) : 0.0f;
+                    sum = sum + (wy * wx) * rn[(fy - fy0) * RN + (fx - fx0)];
+                }
+            }
+            const int ci = cy * nc + cx;
+            stv(C.rhs, ci, cfl[(cy - cy0) * CF + (cx - cx0)] ? sum : 0.0f);
+            stv(C.p, ci, 0.0f);
+        }
+    }
+}
+// Coarse flags of the restriction: (T/2 + 2)^2 from coarse origin
+// (x0 + H) / 2 - 1.
+template <int H, int T, class S>
+__device__ __forceinline__ void bl_load_cfl(uint8_t* cfl, const MgLevelT<S>& C, int x0, int y0) {
+    constexpr int CF = T / 2 + 2;
+    const int nc = (int)C.n;
+    const int cx0 = (x0 + H) / 2 - 1, cy0 = (y0 + H) / 2 - 1;
+    for (int k = threadIdx.x; k < CF * CF; k += blockDim.x) {
+        const int cx = cx0 + k % CF, cy = cy0 + k / CF;
+        cfl[k] = (cx >= 0 && cy >= 0 && cx < nc && cy < nc) ? C.fluid[cy * nc + cx] : (uint8_t)0;
+    }
+}
+// Coarse values and flags of the prolongation onto a tile with halo H <= 5:
+// (T/2 + 6)^2 from coarse origin (x0 + H) / 2 - 3.
+template <int H, int T, class S>
+__device__ __forceinline__ void bl_load_coarse(float* cpv, uint8_t* cfl, const MgLevelT<S>& C, int x0, int y0) {
+    static_assert(H <= 5, "the coarse tile covers a fine halo of at most 5");
+    constexpr int CU = T / 2 + 6;
+    const int nc = (int)C.n;
+    const int cx0 = (x0 + H) / 2 - 3, cy0 = (y0 + H) / 2 - 3;
+    for (int k = threadIdx.x; k < CU * CU; k += blockDim.x) {
+        const int cx = cx0 + k % CU, cy = cy0 + k / CU;
+        const bool in = cx >= 0 && cy >= 0 && cx < nc && cy < nc;
+        cfl[k] = in ? C.fluid[cy * nc + cx] : (uint8_t)0;
+        cpv[k] = in ? ldv(C.p, cy * nc + cx) : 0.0f;
+    }
+}
+// Adds the bilinear prolongation to every fluid cell of the tile.
+template <int H, int T>
+__device__ __forceinline__ void bl_prolong_tile(MgTile<H, T>& t, const float* cpv, const uint8_t* cfl, int n, int nc, int x0,
+                                                int y0) {
+    constexpr int R = MgTile<H, T>::R, CU = T / 2 + 6;
+    const int cx0 = (x0 + H) / 2 - 3, cy0 = (y0 + H) / 2 - 3;
+    for (int k = threadIdx.x; k < R * R; k += blockDim.x) {
+        const int ly = k / R, lx = k % R;
+        const int gx = x0 + lx, gy = y0 + ly;
+        if (gx < 0 || gy < 0 || gx >= n || gy >= n || !t.fl[ly][lx]) continue;
+        float ws, acc;
+        bl_interp(
+            gx, gy, nc, [&](int cx, int cy) { return cfl[(cy - cy0) * CU + (cx - cx0)] != 0; },
+            [&](int cx, int cy) { return cpv[(cy - cy0) * CU + (cx - cx0)]; }, ws, acc);
+        t.p[ly][lx] = t.p[ly][lx] + acc / ws;
+    }
+}
+
 // Pre-smoothing (2 x red, black) and restriction of one level
 // (src/world/mod.rs:1556-1563; multigrid.wgsl:60-105).
 template <class S, bool SYM, int MG_T>
 __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
-    constexpr int H = 5;
+    // the bilinear restriction reads residuals one ring beyond the core,
+    // so its tiles keep exact values two cells out
+    constexpr bool BL = SYM && BL_TRANSFER<S>;
+    constexpr int H = BL ? 6 : 5;
     __shared__ MgTile<H, MG_T> t;
+    __shared__ float rn[BL ? (MG_T + 2) * (MG_T + 2) : 1];
+    __shared__ uint8_t cfl[BL ? (MG_T / 2 + 2) * (MG_T / 2 + 2) : 1];
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
+    if (BL) bl_load_cfl<H, MG_T, S>(cfl, C, x0, y0);
     tile_load<H, MG_T, S>(t, L, L.p, x0, y0, nullptr);
     __syncthreads();
     constexpr int R = MgTile<H, MG_T>::R;
@@ -805,6 +951,10 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
     for (int k = threadIdx.x; k < MG_T * MG_T; k += blockDim.x) {
         const int gx = x0 + H + k % MG_T, gy = y0 + H + k / MG_T;
         if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) stv(L.pp, gy * n + gx, t.p[H + k / MG_T][H + k % MG_T]);
+    }
+    if (BL) {
+        bl_restrict_tile<H, MG_T, S>(t, rn, cfl, n, L.eps, C, x0, y0);
+        return;
     }
     const int nc = (int)C.n;
     const int ct = threadIdx.x;  // one thread per coarse cell of the tile
@@ -836,9 +986,17 @@ __global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, con
     if (done && *done) return;
     constexpr int H = 4;
     __shared__ MgTile<H, MG_T> t;
+    constexpr bool BL = SYM && BL_TRANSFER<S>;
+    __shared__ float cpv[BL ? (MG_T / 2 + 6) * (MG_T / 2 + 6) : 1];
+    __shared__ uint8_t cfl[BL ? (MG_T / 2 + 6) * (MG_T / 2 + 6) : 1];
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
-    tile_load<H, MG_T, S>(t, L, L.pp, x0, y0, &C);
+    if (BL) bl_load_coarse<H, MG_T, S>(cpv, cfl, C, x0, y0);
+    tile_load<H, MG_T, S>(t, L, L.pp, x0, y0, BL ? nullptr : &C);
     __syncthreads();
+    if (BL) {
+        bl_prolong_tile<H, MG_T>(t, cpv, cfl, (int)L.n, (int)C.n, x0, y0);
+        __syncthreads();
+    }
     constexpr int R = MgTile<H, MG_T>::R;
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
@@ -914,9 +1072,11 @@ __device__ __forceinline__ void blk_sweep(const Team<TW, TH>& tm, float* p, cons
     }
 }
 
-// 2 x (red, black) and the restriction of level l into level l + 1.
+// 2 x (red, black) and the restriction of level l into level l + 1
+// (bilinear when SYM, through res / W of every fine cell in RN).
 template <bool SYM, int TW, int TH, class S>
-__device__ void blk_down(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, float* RHS, const uint8_t* FL, int l) {
+__device__ void blk_down(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, float* RHS, const uint8_t* FL, float* RN,
+                         int l) {
     float* p = P + a.off[l];
     const float* rhs = RHS + a.off[l];
     const uint8_t* fl = FL + a.off[l];
@@ -928,6 +1088,45 @@ __device__ void blk_down(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float*
     const int nc = a.n[l + 1];
     float* pc = P + a.off[l + 1];
     float* rc = RHS + a.off[l + 1];
+    if (SYM && BL_TRANSFER<S>) {
+        const uint8_t* flc = FL + a.off[l + 1];
+        for (int y = tm.ty; y < n; y += TH)
+            for (int x = tm.tx; x < n; x += TW) {
+                const int i = y * n + x;
+                float v = 0.0f;
+                if (fl[i]) {
+                    const NbSum nb = blk_nb(p, fl, n, x, y);
+                    const float res = rhs[i] - (nb.s - (nb.k + a.eps[l]) * p[i]);
+                    float ws, acc;
+                    bl_interp(
+                        x, y, nc, [&](int cx, int cy) { return flc[cy * nc + cx] != 0; }, [](int, int) { return 0.0f; }, ws,
+                        acc);
+                    v = res / ws;
+                }
+                RN[i] = v;
+            }
+        tm.sync();
+        for (int gy = tm.ty; gy < nc; gy += TH)
+            for (int gx = tm.tx; gx < nc; gx += TW) {
+                float sum = 0.0f;
+                for (int dy = -1; dy <= 2; ++dy) {
+                    const int fy = 2 * gy + dy;
+                    const bool iy = fy >= 0 && fy < n;
+                    const float wy = iy ? bl_wt(fy, gy, nc) : 0.0f;
+                    for (int dx = -1; dx <= 2; ++dx) {
+                        const int fx = 2 * gx + dx;
+                        const bool ix = fx >= 0 && fx < n;
+                        const float wx = ix ? bl_wt(fx, gx, nc) : 0.0f;
+                        sum = sum + (wy * wx) * ((iy && ix) ? RN[fy * n + fx] : 0.0f);
+                    }
+                }
+                const int ci = gy * nc + gx;
+                rc[ci] = flc[ci] ? sum : 0.0f;
+                pc[ci] = 0.0f;
+            }
+        tm.sync();
+        return;
+    }
     for (int gy = tm.ty; gy < nc; gy += TH)
         for (int gx = tm.tx; gx < nc; gx += TW) {
             float sum = 0.0f;
@@ -957,10 +1156,20 @@ __device__ void blk_up(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P
     const uint8_t* fl = FL + a.off[l];
     const int n = a.n[l], nc = a.n[l + 1];
     const float* pc = P + a.off[l + 1];
+    const uint8_t* flc = FL + a.off[l + 1];
     for (int y = tm.ty; y < n; y += TH)
         for (int x = tm.tx; x < n; x += TW) {
             const int i = y * n + x;
-            if (fl[i]) p[i] += pc[(y / 2) * nc + x / 2];
+            if (!fl[i]) continue;
+            if (SYM && BL_TRANSFER<S>) {
+                float ws, acc;
+                bl_interp(
+                    x, y, nc, [&](int cx, int cy) { return flc[cy * nc + cx] != 0; },
+                    [&](int cx, int cy) { return pc[cy * nc + cx]; }, ws, acc);
+                p[i] = p[i] + acc / ws;
+            } else {
+                p[i] += pc[(y / 2) * nc + x / 2];
+            }
         }
     tm.sync();
     for (int s = 0; s < 4; ++s) {
@@ -1046,6 +1255,7 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
     float* RHS = sm + total;
     uint8_t* FL = reinterpret_cast<uint8_t*>(sm + 2 * total);
     float* BV = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sm) + ((9 * total + 3) & ~3));
+    float* RN = BV + MG_SYM_COARSE_MAX;  // res / W of a fine level (SYM), n0^2 floats
     const int n0 = a.n[0];
     for (int y = blk.ty; y < n0; y += 32)
         for (int x = blk.tx; x < n0; x += 32) {
@@ -1063,7 +1273,7 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
     }
     __syncthreads();
     const int L = a.levels;
-    for (int l = 0; l + 1 < L; ++l) blk_down<SYM>(blk, a, P, RHS, FL, l);
+    for (int l = 0; l + 1 < L; ++l) blk_down<SYM>(blk, a, P, RHS, FL, RN, l);
     blk_coarse<SYM>(blk, a, P, RHS, FL, BV);
     for (int l = L - 2; l >= 0; --l) blk_up<SYM>(blk, a, P, RHS, FL, l);
     for (int y = blk.ty; y < n0; y += 32)
@@ -1184,15 +1394,19 @@ __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
 // every tile cell (halo included) forms s' = w + beta s and r' = r - alpha s'
 // from the old r and s, so the tile's right-hand side is the updated
 // residual; core cells also store r', s' and the updated p and x.  Then the
-// reference's 2 x (red, black) pre-smoothing from u = 0 and the restriction
-// (half the sum over fluid children), as k_mg_down.
+// reference's 2 x (red, black) pre-smoothing from u = 0 and the bilinear
+// restriction (or, for bfloat16 storage, half the sum over fluid children),
+// as k_mg_down.
 template <class S>
 __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int first) {
     if (st->done) return;
-    constexpr int H = 5, T = 32;
+    constexpr bool BL = BL_TRANSFER<S>;
+    constexpr int H = BL ? 6 : 5, T = 32;
     constexpr int R = T + 2 * H;
     constexpr int PER = (R * R + 255) / 256;
     __shared__ MgTile<H, T> t;
+    __shared__ float rn[BL ? (T + 2) * (T + 2) : 1];
+    __shared__ uint8_t cfl[BL ? (T / 2 + 2) * (T / 2 + 2) : 1];
     const float alpha = (float)st->alpha, beta = (float)st->beta;
     const float* r_in = V.r[par];
     const float* s_in = V.s[par];
@@ -1200,6 +1414,7 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
     float* s_out = V.s[1 - par];
     const int n = (int)V.n;
     const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
+    if (BL) bl_load_cfl<H, T, S>(cfl, C, x0, y0);
     float rv[PER], sv[PER], wv[PER], uv[PER], pv[PER], xv[PER];
     uint32_t fv[PER];
 #pragma unroll
@@ -1261,6 +1476,10 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
         const int gx = x0 + H + k % T, gy = y0 + H + k / T;
         if (gx < n && gy < n && t.fl[H + k / T][H + k % T]) stv(V.zpre, gy * n + gx, t.p[H + k / T][H + k % T]);
     }
+    if (BL) {
+        bl_restrict_tile<H, T, S>(t, rn, cfl, n, V.eps, C, x0, y0);
+        return;
+    }
     const int nc = (int)C.n;
     const int ct = threadIdx.x;  // one thread per coarse cell of the tile
     if (ct < (T / 2) * (T / 2)) {
@@ -1283,8 +1502,8 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
     }
 }
 
-// Level 0 of the V-cycle, upward: prolongation on load (of the pre-smoothed
-// u written by k_cg_down0) and the symmetric
+// Level 0 of the V-cycle, upward: the bilinear prolongation added to the
+// pre-smoothed u written by k_cg_down0, and the symmetric
 // post-smoothing (black, red) x 2, as k_mg_up but with a 5-cell halo, so the
 // smoothed u is exact one ring beyond the core; then w = A u on the core
 // (from u as stored in S) and the block's partial sums of (r, u), (w, u),
@@ -1295,6 +1514,9 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
     constexpr int H = 5, T = 32;
     constexpr int R = T + 2 * H;
     __shared__ MgTile<H, T> t;
+    constexpr bool BL = BL_TRANSFER<S>;
+    __shared__ float cpv[BL ? (T / 2 + 6) * (T / 2 + 6) : 1];
+    __shared__ uint8_t cfl[BL ? (T / 2 + 6) * (T / 2 + 6) : 1];
     const float* r_cur = V.r[1 - par];
     MgLevelT<S> L;
     L.n = V.n;
@@ -1305,7 +1527,8 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
     L.fluid = V.fl;
     const int n = (int)V.n;
     const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
-    tile_load<H, T, S>(t, L, V.zpre, x0, y0, &C);
+    if (BL) bl_load_coarse<H, T, S>(cpv, cfl, C, x0, y0);
+    tile_load<H, T, S>(t, L, V.zpre, x0, y0, BL ? nullptr : &C);
     // the inner products' operands at this thread's core cells, loaded now so
     // that they arrive during the sweeps
     const float* s_cur = V.s[1 - par];
@@ -1324,6 +1547,10 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
         }
     }
     __syncthreads();
+    if (BL) {
+        bl_prolong_tile<H, T>(t, cpv, cfl, n, (int)C.n, x0, y0);
+        __syncthreads();
+    }
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
         tile_half_sweep<H, T>(t, 1 - (s & 1), V.eps, 1 + s, R - 1 - s);
@@ -2113,11 +2340,16 @@ void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, boo
         else k_mg_up<S, false, 32><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
     }
 }
-size_t mg_block_lds_bytes_total(size_t cells) { return ((cells * 9 + 3) & ~(size_t)3) + 4 * MG_SYM_COARSE_MAX; }
+// LDS of k_mg_block: unknowns and right-hand sides (binary32) and masks of
+// every level, the staged coarsest right-hand side, and res / W of the top
+// level for the bilinear restriction.
+size_t mg_block_lds_bytes_total(size_t cells, size_t top_cells) {
+    return ((cells * 9 + 3) & ~(size_t)3) + 4 * MG_SYM_COARSE_MAX + 4 * top_cells;
+}
 size_t mg_block_lds_bytes(const uint32_t* n, int count) {
     size_t cells = 0;
     for (int i = 0; i < count; ++i) cells += (size_t)n[i] * n[i];
-    return ((cells * 9 + 3) & ~(size_t)3) + 4 * MG_SYM_COARSE_MAX;
+    return mg_block_lds_bytes_total(cells, count > 0 ? (size_t)n[0] * n[0] : 0);
 }
 template <class S>
 void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
@@ -2138,7 +2370,7 @@ void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, 
     a.cinv = cinv;
     a.ccell = ccell;
     a.cm = cm;
-    const size_t lds = mg_block_lds_bytes_total((size_t)off);
+    const size_t lds = mg_block_lds_bytes_total((size_t)off, (size_t)lv[0].n * lv[0].n);
     if (sym) k_mg_block<S, true><<<1, 1024, lds, s>>>(a, done);
     else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }

@@ -168,7 +168,7 @@ def test_binary64_oracle_bounds_the_binary32_fluid_error():
     true residual at its floor (a few 1e-6 here: the rounding of a pressure
     of magnitude ~100).  Velocity -- the field the world uses -- differs by
     under 1e-4 relative L2; pressure, fixed only up to cond(A) times the
-    residual, by a few 1e-4."""
+    residual, by a few 1e-4 (bound 1e-3)."""
     import ribossome_amd as ra
 
     w32, cfg = _world_after_solve(0, 1e-7)
@@ -181,7 +181,7 @@ def test_binary64_oracle_bounds_the_binary32_fluid_error():
         b = w.download_f64(ra.DIVERGENCE)
         rel = np.linalg.norm(b[f] - A @ p[f]) / np.linalg.norm(b[f])
         assert rel < bound, (rel, bound)
-    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 5e-4)):
+    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-3)):
         a, b = w32.download_f64(which), w64.download_f64(which)
         e = np.linalg.norm(a - b) / np.linalg.norm(b)
         assert e < tol, (which, e)

@@ -60,7 +60,7 @@ BYTES_PER_UNIT = {
     "pcg_init": (33, "F"),
     "pcg_final": (9, "F"),
 }
-MG_LDS_LIMIT = 65536
+MG_LDS_LIMIT = 163840  # LDS per workgroup on gfx950
 MG_BLOCK_LEVELS = 8
 MG_SYM_COARSE_MAX = 1024
 

@@ -664,7 +664,9 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     w->Lsz = cfg->grid_size / LIGHT_DIV;
     w->sim_size = (float)(cfg->grid_size * cfg->world_units_per_cell);
     w->device = dev;
-    w->lds_limit = std::min<size_t>(65536, (size_t)prop.sharedMemPerBlock);
+    // k_mg_block takes its LDS dynamically, up to the workgroup maximum
+    // (160 KB on gfx950)
+    w->lds_limit = (size_t)prop.sharedMemPerBlock;
     ribo_sim_params_fill_default(&w->params);
     w->params.seed = seed;
     ribo_world_settings_default(&w->st);
@@ -843,30 +845,32 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             rc = build_coarse_inverse(w);
             if (rc) return rc;
         }
-        // Iteration slots per solve for this chunk: the most iterations any
-        // solve of the previous chunk took, plus 30 % and 4, rounded up to
-        // even (slots after convergence cost ~nothing: their kernels return
-        // at once; a tighter margin missed the tolerance once in 1000 steps)
-        // (read
-        // after draining the stream, so the choice depends only on the
-        // sequence of calls); a solve that ran out of slots (counted on the
-        // device) restores the full cap.
+        // Iteration slots per solve for this chunk (read after draining the
+        // stream, so the choice depends only on the sequence of calls): the
+        // most iterations any of the last 64 solves took, plus max(3, 1/8 of
+        // it), rounded up to even.  An unused slot is ten kernels that return
+        // at once (~3 us each); on the 2048^2 benchmark world no solve took
+        // more than 1 iteration over that maximum in 400 steps.  A solve that
+        // ran out of slots (counted on the device) doubles the next chunk's
+        // slots; with no history the chunk gets the full cap.
         const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
         HIPCHK(hipStreamSynchronize(w->stream));
-        uint32_t slots = w->pcg_slots;
+        uint32_t slots = maxit;
         uint32_t missed = 0u;
         if (w->pcg_host) {
             missed = __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED);
             const uint32_t solves = __atomic_load_n(&w->pcg_host[2], __ATOMIC_RELAXED);
-            const uint32_t fresh = std::min(solves - w->solves_seen, PCG_HOST_RING_LEN);
+            const uint32_t known = std::min(solves, PCG_HOST_RING_LEN);
             uint32_t most = 0u;
-            for (uint32_t k = 0; k < fresh; ++k)
+            for (uint32_t k = 0; k < known; ++k)
                 most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
                                                       __ATOMIC_RELAXED));
-            if (fresh > 0u) slots = std::min(maxit, (most + most * 3u / 10u + 4u + 1u) & ~1u);
+            if (known > 0u) {
+                const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + std::max(3u, most / 8u);
+                slots = std::min(maxit, (want + 1u) & ~1u);
+            }
             w->solves_seen = solves;
         }
-        if (w->pcg_slots == 0 || !w->pcg_host || missed != w->missed_seen) slots = maxit;
         w->missed_seen = missed;
         w->pcg_slots = std::max(slots, 1u);
     }

@@ -198,6 +198,32 @@ def test_tolerance_mode_matches_oracle(precision):
     assert ra.Ledger.from_cells(gw.read_cells_blocking()).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
 
 
+@pytest.mark.parametrize("iters", [1, 2])
+def test_tolerance_mode_preconditioner_matches_oracle(iters):
+    """One or two conjugate-gradient iterations of the binary32 tolerance
+    mode: the pressure is then x0 + alpha M r0 (+ a second direction), so it
+    follows the preconditioner M -- the bilinear V-cycle over tile, block and
+    level-0 kernels -- cell by cell.  GPU and oracle differ only in the
+    summation order of the dot products behind alpha and beta (a few ulp of
+    each scalar); a wrong transfer weight or halo shows up as an error many
+    orders of magnitude larger."""
+    print(f"case preconditioner iters={iters}", flush=True)
+    cfg = ra.WorldConfig(256, 128)
+    inp = make_inputs(cfg, 5, rock=0.15, rubble=True, vent_rows=4)
+    s = parity_settings(mg_tol=1e-9, mg_max_cycles=iters, precision=0)
+    gw = ra.World(cfg, 5)
+    load(gw, inp, parity_params(5), s)
+    ow = oc.OracleWorld(cfg, 5)
+    load(ow, inp, parity_params(5), s)
+    gw.encode_steps(1)
+    ow.encode_steps(1)
+    assert gw.mg_stats()[0] == iters and ow.mg_stats()[0] == iters
+    for which in (ra.PRESSURE, ra.VELOCITY):
+        e = rel_l2(gw.download(which), ow.download(which))
+        print(f"{ra.BUFFER_NAMES[which]} rel L2 vs oracle after {iters} iterations: {e:.2e}", flush=True)
+        assert e < 1e-5
+
+
 @pytest.mark.parametrize("tol", [0.0, 1e-4])
 def test_graph_replays_match_eager_recording(tol):
     """Captured HIP graphs, replayed over several chunks with host work and

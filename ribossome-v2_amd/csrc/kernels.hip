@@ -806,10 +806,15 @@ __device__ __forceinline__ void bl_nf(int f, int nc, int& a0, int& a1) {
     a0 = f >> 1;
     a1 = (f & 1) ? (a0 + 1 < nc ? a0 + 1 : a0) : (a0 > 0 ? a0 - 1 : 0);
 }
-__device__ __forceinline__ float bl_wt(int f, int c, int nc) {
-    int a0, a1;
-    bl_nf(f, nc, a0, a1);
-    return (a0 == c ? 0.75f : 0.0f) + (a1 == c ? 0.25f : 0.0f);
+// wt(2c - 1 + d, c) for d = 0..3 (wt(f, c) = 3/4 [a0(f) = c] +
+// 1/4 [a1(f) = c]), 0 outside the level: 1/4, 3/4 (1 at the
+// low edge, where the clamped far neighbour is c too), 3/4 (1 at the high
+// edge), 1/4.
+__device__ __forceinline__ void bl_w4(int c, int nc, float (&w)[4]) {
+    w[0] = c > 0 ? 0.25f : 0.0f;
+    w[1] = c == 0 ? 1.0f : 0.75f;
+    w[2] = c == nc - 1 ? 1.0f : 0.75f;
+    w[3] = c < nc - 1 ? 0.25f : 0.0f;
 }
 // W and the weighted sum of the coarse values at fine cell (fx, fy):
 // cf(cx, cy) and cv(cx, cy) read a coarse flag and value.
@@ -864,16 +869,15 @@ __device__ __forceinline__ void bl_restrict_tile(const MgTile<H, T>& t, float* r
     if (ct < (T / 2) * (T / 2)) {
         const int cx = (x0 + H) / 2 + ct % (T / 2), cy = (y0 + H) / 2 + ct / (T / 2);
         if (cx < nc && cy < nc) {
+            float wy[4], wx[4];
+            bl_w4(cy, nc, wy);
+            bl_w4(cx, nc, wx);
+            const float* r0 = rn + (2 * cy - 1 - fy0) * RN + (2 * cx - 1 - fx0);
             float sum = 0.0f;
-            for (int dy = -1; dy <= 2; ++dy) {
-                const int fy = 2 * cy + dy;
-                const float wy = (fy >= 0 && fy < n) ? bl_wt(fy, cy, nc) : 0.0f;
-                for (int dx = -1; dx <= 2; ++dx) {
-                    const int fx = 2 * cx + dx;
-                    const float wx = (fx >= 0 && fx < n) ? bl_wt(fx, cx, nc) : 0.0f;
-                    sum = sum + (wy * wx) * rn[(fy - fy0) * RN + (fx - fx0)];
-                }
-            }
+#pragma unroll
+            for (int dy = 0; dy < 4; ++dy)
+#pragma unroll
+                for (int dx = 0; dx < 4; ++dx) sum = sum + (wy[dy] * wx[dx]) * r0[dy * RN + dx];
             const int ci = cy * nc + cx;
             stv(C.rhs, ci, cfl[(cy - cy0) * CF + (cx - cx0)] ? sum : 0.0f);
             stv(C.p, ci, 0.0f);
@@ -1108,16 +1112,19 @@ __device__ void blk_down(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float*
         tm.sync();
         for (int gy = tm.ty; gy < nc; gy += TH)
             for (int gx = tm.tx; gx < nc; gx += TW) {
+                float wy[4], wx[4];
+                bl_w4(gy, nc, wy);
+                bl_w4(gx, nc, wx);
                 float sum = 0.0f;
-                for (int dy = -1; dy <= 2; ++dy) {
-                    const int fy = 2 * gy + dy;
-                    const bool iy = fy >= 0 && fy < n;
-                    const float wy = iy ? bl_wt(fy, gy, nc) : 0.0f;
-                    for (int dx = -1; dx <= 2; ++dx) {
-                        const int fx = 2 * gx + dx;
-                        const bool ix = fx >= 0 && fx < n;
-                        const float wx = ix ? bl_wt(fx, gx, nc) : 0.0f;
-                        sum = sum + (wy * wx) * ((iy && ix) ? RN[fy * n + fx] : 0.0f);
+#pragma unroll
+                for (int dy = 0; dy < 4; ++dy) {
+                    const int fy = 2 * gy - 1 + dy;
+                    const int ry = fy < 0 ? 0 : (fy >= n ? n - 1 : fy);  // clamped rows carry weight 0
+#pragma unroll
+                    for (int dx = 0; dx < 4; ++dx) {
+                        const int fx = 2 * gx - 1 + dx;
+                        const int rx = fx < 0 ? 0 : (fx >= n ? n - 1 : fx);
+                        sum = sum + (wy[dy] * wx[dx]) * RN[ry * n + rx];
                     }
                 }
                 const int ci = gy * nc + gx;
@@ -1180,10 +1187,13 @@ __device__ void blk_up(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P
 
 // The coarsest level: 30 x (red, black) (then (black, red) when SYM), or in
 // the tolerance mode the exact solve with the binary64 inverse.  Lanes of
-// the team share the rows of the inverse.
+// the team share the rows of the inverse.  With at most MG_CINV_REG
+// unknowns, each of the first 4 MG_CINV_REG lanes holds its quarter row in
+// registers, loaded when the kernel starts (ci).
+constexpr int MG_CINV_REG = 64;
 template <bool SYM, int TW, int TH, class S>
 __device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, const float* RHS, const uint8_t* FL,
-                           float* bv) {
+                           float* bv, const double (&ci)[MG_CINV_REG / 4]) {
     const int L = a.levels;
     float* p = P + a.off[L - 1];
     const float* rhs = RHS + a.off[L - 1];
@@ -1201,8 +1211,13 @@ __device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, floa
         const int q = t & 3;
         for (int r0 = t >> 2; r0 < ((m + 15) & ~15); r0 += (TW * TH) / 4) {
             double acc = 0.0;
-            if (r0 < m)
+            if (m <= MG_CINV_REG) {
+#pragma unroll
+                for (int k = 0; k < MG_CINV_REG / 4; ++k)
+                    if (r0 < m && q + 4 * k < m) acc += ci[k] * (double)bv[q + 4 * k];
+            } else if (r0 < m) {
                 for (int j = q; j < m; j += 4) acc += a.cinv[(size_t)j * m + r0] * (double)bv[j];
+            }
             acc += __shfl_down(acc, 2, 4);
             acc += __shfl_down(acc, 1, 4);
             if (q == 0 && r0 < m) p[a.ccell[r0]] = (float)acc;
@@ -1271,10 +1286,18 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
         for (int y = blk.ty; y < n; y += 32)
             for (int x = blk.tx; x < n; x += 32) FL[a.off[l] + y * n + x] = a.fluid[l][y * n + x];
     }
+    // this lane's quarter row of the coarsest inverse (see blk_coarse)
+    double ci[MG_CINV_REG / 4];
+    {
+        const int m = a.cm, t = (int)threadIdx.x, q = t & 3, r0 = t >> 2;
+#pragma unroll
+        for (int k = 0; k < MG_CINV_REG / 4; ++k)
+            ci[k] = (SYM && m > 0 && m <= MG_CINV_REG && r0 < m && q + 4 * k < m) ? a.cinv[(size_t)(q + 4 * k) * m + r0] : 0.0;
+    }
     __syncthreads();
     const int L = a.levels;
     for (int l = 0; l + 1 < L; ++l) blk_down<SYM>(blk, a, P, RHS, FL, RN, l);
-    blk_coarse<SYM>(blk, a, P, RHS, FL, BV);
+    blk_coarse<SYM>(blk, a, P, RHS, FL, BV, ci);
     for (int l = L - 2; l >= 0; --l) blk_up<SYM>(blk, a, P, RHS, FL, l);
     for (int y = blk.ty; y < n0; y += 32)
         for (int x = blk.tx; x < n0; x += 32) {

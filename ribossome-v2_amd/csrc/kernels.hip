@@ -2213,15 +2213,13 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         }
     }
     __syncthreads();
-    for (uint32_t k = tid; k < SC_W * SC_W; k += 256) {
-        const uint4 v = s_acc4[k];
-        if ((v.x | v.y | v.z | v.w) == 0u) continue;  // (only cells inside the grid receive adds)
-        const int gx = wx0 + (int)(k % SC_W), gy = wy0 + (int)(k / SC_W);
-        uint32_t* const dst = next + ((uint32_t)gy * G + (uint32_t)gx) * 4u;
-        if (v.x != 0u) atomicAdd(dst, v.x);
-        if (v.y != 0u) atomicAdd(dst + 1, v.y);
-        if (v.z != 0u) atomicAdd(dst + 2, v.z);
-        if (v.w != 0u) atomicAdd(dst + 3, v.w);
+    // consecutive threads take consecutive words, so a wave's atomics cover
+    // whole cache lines of chem_next (only cells inside the grid receive adds)
+    for (uint32_t k = tid; k < SC_W * SC_W * 4; k += 256) {
+        const uint32_t v = s_acc[k];
+        if (v == 0u) continue;
+        const int gx = wx0 + (int)((k / 4) % SC_W), gy = wy0 + (int)((k / 4) / SC_W);
+        atomicAdd(next + ((uint32_t)gy * G + (uint32_t)gx) * 4u + (k & 3u), v);
     }
 }
 

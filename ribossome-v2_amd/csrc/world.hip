@@ -847,7 +847,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         }
         // Iteration slots per solve for this chunk (read after draining the
         // stream, so the choice depends only on the sequence of calls): the
-        // most iterations any of the last 64 solves took, plus max(3, 1/8 of
+        // most iterations any of the last 64 solves took, plus max(3, 1/5 of
         // it), rounded up to even.  An unused slot is ten kernels that return
         // at once (~3 us each); on the 2048^2 benchmark world no solve took
         // more than 1 iteration over that maximum in 400 steps.  A solve that
@@ -866,7 +866,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
                 most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
                                                       __ATOMIC_RELAXED));
             if (known > 0u) {
-                const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + std::max(3u, most / 8u);
+                const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + std::max(3u, most / 5u);
                 slots = std::min(maxit, (want + 1u) & ~1u);
             }
             w->solves_seen = solves;

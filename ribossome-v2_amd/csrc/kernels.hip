@@ -748,17 +748,17 @@ __device__ __forceinline__ void tile_half_sweep(MgTile<H, T>& t, int color, floa
 // before the first LDS store (one memory round trip per tile, not one per
 // loop trip); values of solid or outside cells are replaced by 0 after the
 // load, so the fluid mask is not on the load's critical path.
-template <int H, int T, class S>
+template <int H, int T, class S, int NT = 256>
 __device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L, const S* psrc, int x0, int y0,
                                           const MgLevelT<S>* C) {
     constexpr int R = MgTile<H, T>::R;
-    constexpr int PER = (R * R + 255) / 256;
+    constexpr int PER = (R * R + NT - 1) / NT;
     const int n = (int)L.n;
     float pv[PER], rv[PER], cv[PER];
     uint32_t fv[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = (int)threadIdx.x + NT * j;
         const int ly = k / R, lx = k % R;
         const int gx = x0 + lx, gy = y0 + ly;
         pv[j] = 0.0f;
@@ -775,7 +775,7 @@ __device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L,
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = (int)threadIdx.x + NT * j;
         if (k < R * R) {
             const int ly = k / R, lx = k % R;
             const bool f = fv[j] != 0u;
@@ -1413,6 +1413,12 @@ __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
     block_partial<2>({bb, rr}, st->part);
 }
 
+// Level-0 tiles of the tolerance mode: 64 x 64 cells per 1024-thread
+// workgroup (one workgroup per CU at 1024^2), so that the 5- and 6-cell
+// halos add 20-40 % to the core's work instead of 50-90 % at 32 x 32.
+constexpr int CG_T = 64, CG_NT = 1024;
+static_assert((CG_T / 2) * (CG_T / 2) <= CG_NT, "one thread per coarse cell of a tile");
+
 // Level 0 of the V-cycle, downward, with the iteration's vector updates:
 // every tile cell (halo included) forms s' = w + beta s and r' = r - alpha s'
 // from the old r and s, so the tile's right-hand side is the updated
@@ -1421,12 +1427,12 @@ __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
 // restriction (or, for bfloat16 storage, half the sum over fluid children),
 // as k_mg_down.
 template <class S>
-__global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int first) {
+__global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int first) {
     if (st->done) return;
     constexpr bool BL = BL_TRANSFER<S>;
-    constexpr int H = BL ? 6 : 5, T = 32;
+    constexpr int H = BL ? 6 : 5, T = CG_T, NT = CG_NT;
     constexpr int R = T + 2 * H;
-    constexpr int PER = (R * R + 255) / 256;
+    constexpr int PER = (R * R + NT - 1) / NT;
     __shared__ MgTile<H, T> t;
     __shared__ float rn[BL ? (T + 2) * (T + 2) : 1];
     __shared__ uint8_t cfl[BL ? (T / 2 + 2) * (T / 2 + 2) : 1];
@@ -1442,7 +1448,7 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
     uint32_t fv[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = (int)threadIdx.x + NT * j;
         const int ly = k / R, lx = k % R;
         const int gx = x0 + lx, gy = y0 + ly;
         rv[j] = sv[j] = wv[j] = uv[j] = pv[j] = xv[j] = 0.0f;
@@ -1464,7 +1470,7 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
     }
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = (int)threadIdx.x + NT * j;
         if (k >= R * R) continue;
         const int ly = k / R, lx = k % R;
         const bool f = fv[j] != 0u;
@@ -1532,9 +1538,9 @@ __global__ __launch_bounds__(256) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, Pcg
 // (from u as stored in S) and the block's partial sums of (r, u), (w, u),
 // (u, s), (p, w) and (r, r).
 template <class S>
-__global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par) {
+__global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par) {
     if (st->done) return;
-    constexpr int H = 5, T = 32;
+    constexpr int H = 5, T = CG_T, NT = CG_NT;
     constexpr int R = T + 2 * H;
     __shared__ MgTile<H, T> t;
     constexpr bool BL = BL_TRANSFER<S>;
@@ -1551,15 +1557,15 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
     const int n = (int)V.n;
     const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
     if (BL) bl_load_coarse<H, T, S>(cpv, cfl, C, x0, y0);
-    tile_load<H, T, S>(t, L, V.zpre, x0, y0, BL ? nullptr : &C);
+    tile_load<H, T, S, NT>(t, L, V.zpre, x0, y0, BL ? nullptr : &C);
     // the inner products' operands at this thread's core cells, loaded now so
     // that they arrive during the sweeps
     const float* s_cur = V.s[1 - par];
-    constexpr int PC = (T * T) / 256;
+    constexpr int PC = (T * T) / NT;
     float sc[PC], pc[PC], rc[PC];
 #pragma unroll
     for (int j = 0; j < PC; ++j) {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = (int)threadIdx.x + NT * j;
         const int gx = x0 + H + k % T, gy = y0 + H + k / T;
         sc[j] = pc[j] = rc[j] = 0.0f;
         if (gx < n && gy < n) {
@@ -1587,7 +1593,7 @@ __global__ __launch_bounds__(256) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgSt
     double gam = 0.0, del = 0.0, e = 0.0, f = 0.0, rr = 0.0;
 #pragma unroll
     for (int j = 0; j < PC; ++j) {
-        const int k = (int)threadIdx.x + 256 * j;
+        const int k = (int)threadIdx.x + NT * j;
         const int ly = H + k / T, lx = H + k % T;
         const int gx = x0 + lx, gy = y0 + ly;
         if (gx >= n || gy >= n || !t.fl[ly][lx]) continue;
@@ -2396,7 +2402,7 @@ void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, 
     else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }
 inline uint32_t pcg_blocks(uint32_t n) { return ((n + 15) / 16) * ((n + 15) / 16); }
-inline uint32_t cg_tiles(uint32_t n) { return (n + 31) / 32; }
+inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
     k_cg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st);
     k_pcg_fin<FIN_INIT><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), tol, maxit);
@@ -2404,13 +2410,13 @@ template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgStat
 template <class S>
 void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first) {
     const uint32_t nt = cg_tiles(V.n);
-    k_cg_down0<S><<<dim3(nt, nt), 256, 0, s>>>(V, C, st, parity & 1, first);
+    k_cg_down0<S><<<dim3(nt, nt), CG_NT, 0, s>>>(V, C, st, parity & 1, first);
 }
 template <class S>
 void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first,
                    float tol, uint32_t maxit) {
     const uint32_t nt = cg_tiles(V.n);
-    k_cg_up0<S><<<dim3(nt, nt), 256, 0, s>>>(V, C, st, parity & 1);
+    k_cg_up0<S><<<dim3(nt, nt), CG_NT, 0, s>>>(V, C, st, parity & 1);
     if (first) k_pcg_fin<FIN_CG0><<<1, 256, 0, s>>>(st, nt * nt, tol, maxit);
     else k_pcg_fin<FIN_CG><<<1, 256, 0, s>>>(st, nt * nt, tol, maxit);
 }

@@ -130,6 +130,9 @@ template <class S> void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, cons
                                        const uint32_t* done);
 template <class S> void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym,
                                      const uint32_t* done);
+// Zeroes a level's unknowns (the symmetric cycle's coarsest level when no
+// suffix of levels fits one workgroup).
+template <class S> void launch_zero_level(hipStream_t s, const MgLevelT<S>& L);
 // The tolerance mode solves its coarsest level exactly (dense inverse) when
 // it has at most MG_SYM_COARSE_MAX fluid cells; the workgroup kernel stages
 // that right-hand side in LDS.

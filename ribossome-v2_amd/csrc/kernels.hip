@@ -744,7 +744,8 @@ __device__ __forceinline__ void tile_half_sweep(MgTile<H, T>& t, int color, floa
     }
 }
 
-// Loads the tile with its halo.  Every thread issues all of its loads
+// Loads the tile with its halo (unknowns from psrc, or 0 when psrc is
+// null).  Every thread issues all of its loads
 // before the first LDS store (one memory round trip per tile, not one per
 // loop trip); values of solid or outside cells are replaced by 0 after the
 // load, so the fluid mask is not on the load's critical path.
@@ -768,7 +769,7 @@ __device__ __forceinline__ void tile_load(MgTile<H, T>& t, const MgLevelT<S>& L,
         if (k < R * R && gx >= 0 && gy >= 0 && gx < n && gy < n) {
             const int i = gy * n + gx;
             fv[j] = L.fluid[i];
-            pv[j] = ldv(psrc, i);
+            pv[j] = psrc ? ldv(psrc, i) : 0.0f;
             rv[j] = ldv(L.rhs, i);
             if (C) cv[j] = ldv(C->p, (gy / 2) * (int)C->n + gx / 2);
         }
@@ -880,7 +881,6 @@ __device__ __forceinline__ void bl_restrict_tile(const MgTile<H, T>& t, float* r
                 for (int dx = 0; dx < 4; ++dx) sum = sum + (wy[dy] * wx[dx]) * r0[dy * RN + dx];
             const int ci = cy * nc + cx;
             stv(C.rhs, ci, cfl[(cy - cy0) * CF + (cx - cx0)] ? sum : 0.0f);
-            stv(C.p, ci, 0.0f);
         }
     }
 }
@@ -943,7 +943,9 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
     __shared__ uint8_t cfl[BL ? (MG_T / 2 + 2) * (MG_T / 2 + 2) : 1];
     const int x0 = (int)blockIdx.x * MG_T - H, y0 = (int)blockIdx.y * MG_T - H;
     if (BL) bl_load_cfl<H, MG_T, S>(cfl, C, x0, y0);
-    tile_load<H, MG_T, S>(t, L, L.p, x0, y0, nullptr);
+    // the symmetric cycle's coarse levels always start from 0 (nothing of
+    // the previous cycle survives), so their unknowns are not read
+    tile_load<H, MG_T, S>(t, L, SYM ? nullptr : L.p, x0, y0, nullptr);
     __syncthreads();
     constexpr int R = MgTile<H, MG_T>::R;
 #pragma unroll
@@ -978,7 +980,7 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
             const int ci = cy * nc + cx;
             const float rc = nf > 0u ? (SYM ? 0.5f * sum : 2.0f * sum / (float)rb_maxu(nf, 1u)) : 0.0f;
             stv(C.rhs, ci, rc);
-            stv(C.p, ci, 0.0f);
+            if (!SYM) stv(C.p, ci, 0.0f);
         }
     }
 }
@@ -1019,7 +1021,8 @@ __global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, con
 // goes through pre-smoothing and restriction, the coarsest gets 30 x (red,
 // black) -- or, in the tolerance mode, an exact solve with the binary64
 // inverse of its operator -- then prolongation and post-smoothing back up;
-// the top level's unknowns are read from and written back to HBM.
+// the top level's unknowns are read from (the symmetric cycle starts them
+// from 0) and written back to HBM.
 constexpr int MG_BLOCK_LEVELS = 8;
 template <class S>
 struct MgBlockArgs {
@@ -1276,7 +1279,7 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
         for (int x = blk.tx; x < n0; x += 32) {
             const int i = y * n0 + x;
             const uint8_t f = a.fluid[0][i];
-            const float pv = ldv(a.p0, i), rv = ldv(a.rhs0, i);
+            const float pv = SYM ? 0.0f : ldv(a.p0, i), rv = ldv(a.rhs0, i);
             FL[i] = f;
             P[i] = f ? pv : 0.0f;
             RHS[i] = f ? rv : 0.0f;
@@ -1526,7 +1529,6 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
             }
             const int ci = cy * nc + cx;
             stv(C.rhs, ci, nf > 0u ? 0.5f * sum : 0.0f);
-            stv(C.p, ci, 0.0f);
         }
     }
 }
@@ -1825,6 +1827,11 @@ __global__ __launch_bounds__(256) void k_agg(Geo g, const uint32_t* __restrict__
 }
 
 __global__ void k_epoch_add(uint32_t* ep, uint32_t n) { *ep += n; }
+
+template <class S> __global__ __launch_bounds__(256) void k_zero(S* p, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) stv(p, (int)i, 0.0f);
+}
 
 __global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x * 4u;
@@ -2378,6 +2385,10 @@ size_t mg_block_lds_bytes(const uint32_t* n, int count) {
     for (int i = 0; i < count; ++i) cells += (size_t)n[i] * n[i];
     return mg_block_lds_bytes_total(cells, count > 0 ? (size_t)n[0] * n[0] : 0);
 }
+template <class S> void launch_zero_level(hipStream_t s, const MgLevelT<S>& L) {
+    const uint32_t n = L.n * L.n;
+    k_zero<S><<<(n + 255) / 256, 256, 0, s>>>(L.p, n);
+}
 template <class S>
 void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
                      int cm, const uint32_t* done) {
@@ -2427,6 +2438,7 @@ template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgSta
 #define RIBO_INSTANTIATE(S)                                                                                         \
     template void launch_mg_down<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);   \
     template void launch_mg_up<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);     \
+    template void launch_zero_level<S>(hipStream_t, const MgLevelT<S>&);                                            \
     template void launch_mg_block<S>(hipStream_t, const MgLevelT<S>*, int, bool, const double*, const int32_t*, int, \
                                      const uint32_t*);                                                              \
     template void launch_cg_init<S>(hipStream_t, const CgVec<S>&, PcgState*, float, uint32_t);                     \

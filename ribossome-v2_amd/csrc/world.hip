@@ -444,8 +444,10 @@ template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevel
         for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
         return;
     }
-    // levels too large for one workgroup: sweeps only
+    // levels too large for one workgroup: sweeps only (the coarsest level
+    // keeps the zero correction its restriction implies)
     for (size_t l = 1; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+    run(w, "mg_zero", [&] { launch_zero_level<S>(s, lv[L - 1]); });
     for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
 }
 

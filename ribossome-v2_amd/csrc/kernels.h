@@ -80,8 +80,19 @@ template <class S> struct CgVec {
 };
 
 constexpr uint32_t PCG_HOST_RING = 4, PCG_HOST_RING_LEN = 64;
+// The scalars of the conjugate-gradient recurrences after an iteration.
+struct PcgScal {
+    double gamma, pi, beta, alpha, rel;
+    uint32_t it, done;
+};
+// Solver state.  The scalars are double-buffered: the kernel that applies an
+// iteration's inner products reads one slot (every workgroup forms the same
+// update) and its workgroup 0 writes the other.  `done` is the published
+// flag the V-cycle's kernels test; it, rel and true_rel describe the last
+// finished solve.
 struct PcgState {
-    double bb, gamma, pi, beta, alpha, rel, true_rel;
+    double bb, rel, true_rel;
+    PcgScal sc[2];
     uint32_t it, done, steps, iters, missed;
     float tol_seen;
     double* part;       // per-block partial sums (up to 5 per block of 16 x 16 cells)
@@ -145,16 +156,19 @@ template <class S> void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, in
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
 // Pre-smoothing and restriction of level 0, fused with the vector updates of
 // the iteration (p, s, x, r; none on the first application, first = 1);
-// parity selects which r and s buffers are read.
+// parity selects which r and s buffers are read.  Iteration j >= 1 first
+// applies the inner products of iteration j - 1 to the scalar state
+// (reading slot rd, writing 1 - rd; iteration 0 reads slot 1).
 template <class S>
-void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first);
+void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first,
+                     int rd, float tol, uint32_t maxit);
 // Prolongation and post-smoothing of level 0 fused with w = A u and the
-// inner products; then the scalar recurrences (first: the initial ones).
+// inner products.
 template <class S>
-void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first,
-                   float tol, uint32_t maxit);
-// True residual ||b - A x|| / ||b|| of the solve, and the statistics.
-template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st);
+void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity);
+// The last iteration's scalar update (slot rd to 1 - rd), the true residual
+// ||b - A x|| / ||b|| of the solve, and the statistics.
+template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st, int rd, float tol, uint32_t maxit);
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);

@@ -1319,11 +1319,12 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
 // alpha = gamma / pi, and
 //   p' = u + beta p,  s' = w + beta s,  x += alpha p',  r -= alpha s'.
 // The vector updates ride on the V-cycle's first kernel, w = A u and the
-// inner products on its last: an iteration is the V-cycle's launches plus
-// one one-block kernel for the scalars.  Level-0 vectors cover fluid cells;
-// inner products are binary64, summed per block and then over the blocks in
-// a fixed order, so every run gives the same bits.  A v = sum of fluid
-// neighbours - (k + eps) v.
+// inner products on its last, and the scalar step on the next V-cycle's
+// first kernel (every workgroup sums the partials in the same order and
+// forms the same scalars): an iteration is exactly the V-cycle's launches.
+// Level-0 vectors cover fluid cells; inner products are binary64, summed
+// per block and then over the blocks in a fixed order, so every run gives
+// the same bits.  A v = sum of fluid neighbours - (k + eps) v.
 __device__ __forceinline__ float apply_A(const uint8_t* fl, const float* v, int n, int x, int y, float eps) {
     const int i = y * n + x;
     const int il = x > 0 ? i - 1 : i, ir = x + 1 < n ? i + 1 : i;
@@ -1387,6 +1388,86 @@ __device__ void sum_partials(const double* __restrict__ part, uint32_t nb, doubl
     for (int j = 0; j < N; ++j) t[j] = ((sh[j][0] + sh[j][1]) + sh[j][2]) + sh[j][3];
 }
 
+// Totals of nb partial N-tuples by a whole workgroup of NT threads, in a
+// fixed order (thread t sums blocks t, t + NT, ..., then the wave tree, then
+// the waves in order); every workgroup that calls it gets the same bits.
+// Valid in thread 0.
+template <int N, int NT>
+__device__ void sum_partials_wg(const double* __restrict__ part, uint32_t nb, double (&t)[N]) {
+    __shared__ double sh[N][NT / 64];
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x, lane = tid & 63, wid = tid >> 6;
+    double a[N];
+#pragma unroll
+    for (int j = 0; j < N; ++j) a[j] = 0.0;
+    for (uint32_t k = (uint32_t)tid; k < nb; k += NT)
+#pragma unroll
+        for (int j = 0; j < N; ++j) a[j] += part[(size_t)N * k + j];
+#pragma unroll
+    for (int j = 0; j < N; ++j) {
+        for (int o = 32; o > 0; o >>= 1) a[j] += __shfl_down(a[j], o, 64);
+        if (lane == 0) sh[j][wid] = a[j];
+    }
+    __syncthreads();
+    if (tid == 0)
+#pragma unroll
+        for (int j = 0; j < N; ++j) {
+            double v = 0.0;
+            for (int w = 0; w < NT / 64; ++w) v += sh[j][w];
+            t[j] = v;
+        }
+}
+
+// The scalar recurrences after an iteration's inner products t = (gamma,
+// delta, e, f, rr) (see above): state a to b.  The first application only
+// sets gamma, pi = delta and alpha; later ones count the iteration, test the
+// residual and form beta, pi and alpha.  A finished state passes unchanged.
+__device__ inline PcgScal pcg_advance(const PcgState* st, const PcgScal& a, const double (&t)[5], bool first_apply,
+                                      float tol, uint32_t maxit) {
+    PcgScal b = a;
+    if (a.done) return b;
+    if (first_apply) {
+        b.gamma = t[0];
+        b.pi = t[1];
+        b.alpha = t[1] != 0.0 ? t[0] / t[1] : 0.0;
+        b.beta = 0.0;
+        return b;
+    }
+    b.it = a.it + 1u;
+    b.rel = st->bb > 0.0 ? sqrt(t[4] / st->bb) : 0.0;
+    if (b.rel <= (double)tol || b.it >= maxit) {
+        b.done = 1u;
+    } else {
+        const double g = t[0], beta = a.gamma != 0.0 ? g / a.gamma : 0.0;
+        const double pi = t[1] + beta * (t[2] + t[3]) + beta * beta * a.pi;
+        b.beta = beta;
+        b.pi = pi;
+        b.alpha = pi != 0.0 ? g / pi : 0.0;
+        b.gamma = g;
+    }
+    return b;
+}
+
+// A workgroup of NT threads applies the partials of the last iteration to
+// slot rd (first_apply for the first V-cycle's products), shares the result
+// through LDS and (workgroup 0) writes it to slot 1 - rd and publishes done.
+template <int NT>
+__device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, bool first_apply, float tol, uint32_t maxit) {
+    __shared__ PcgScal sh_sc;
+    double t[5];
+    sum_partials_wg<5, NT>(st->part, nb, t);
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    if (tid == 0) {
+        const PcgScal b = pcg_advance(st, st->sc[rd], t, first_apply, tol, maxit);
+        sh_sc = b;
+        if (blockIdx.x == 0 && blockIdx.y == 0) {
+            st->sc[1 - rd] = b;
+            st->done = b.done;
+        }
+    }
+    __syncthreads();
+    return sh_sc;
+}
+
 #define PCG_XY                                                                         \
     const int x = (int)(blockIdx.x * 16 + threadIdx.x), y = (int)(blockIdx.y * 16 + threadIdx.y); \
     const int n = (int)V.n;                                                            \
@@ -1430,8 +1511,18 @@ static_assert((CG_T / 2) * (CG_T / 2) <= CG_NT, "one thread per coarse cell of a
 // restriction (or, for bfloat16 storage, half the sum over fluid children),
 // as k_mg_down.
 template <class S>
-__global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int first) {
-    if (st->done) return;
+__global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int stage, int rd,
+                                                   uint32_t nb, float tol, uint32_t maxit) {
+    // stage 0: the first V-cycle (the initial state in slot rd); stage 1 and
+    // 2: first apply the previous V-cycle's inner products to the scalars
+    // (stage 1: their first application)
+    const int first = stage == 0 ? 1 : 0;
+    if (st->sc[rd].done) {
+        if (!first && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) st->sc[1 - rd] = st->sc[rd];
+        return;
+    }
+    const PcgScal sc = first ? st->sc[rd] : pcg_step_wg<CG_NT>(st, rd, nb, stage == 1, tol, maxit);
+    if (sc.done) return;
     constexpr bool BL = BL_TRANSFER<S>;
     constexpr int H = BL ? 6 : 5, T = CG_T, NT = CG_NT;
     constexpr int R = T + 2 * H;
@@ -1439,7 +1530,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
     __shared__ MgTile<H, T> t;
     __shared__ float rn[BL ? (T + 2) * (T + 2) : 1];
     __shared__ uint8_t cfl[BL ? (T / 2 + 2) * (T / 2 + 2) : 1];
-    const float alpha = (float)st->alpha, beta = (float)st->beta;
+    const float alpha = (float)sc.alpha, beta = (float)sc.beta;
     const float* r_in = V.r[par];
     const float* s_in = V.s[par];
     float* r_out = V.r[1 - par];
@@ -1617,68 +1708,54 @@ __global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, Pcg
 
 // ||b - A x||^2 over fluid cells, for the reported residual.
 template <class S>
-__global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st) {
+__global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit,
+                                                  uint32_t off) {
+    // workgroup 0 applies the last V-cycle's inner products (slot rd to 1 - rd)
+    if (blockIdx.x == 0 && blockIdx.y == 0) (void)pcg_step_wg<256>(st, rd, nb, false, tol, maxit);
     PCG_XY
     double rr = 0.0;
     if (f) {
         const float r = V.b[i] - apply_A(V.fl, V.x, n, x, y, V.eps);
         rr = (double)r * (double)r;
     }
-    block_partial<2>({rr, 0.0}, st->part);
+    block_partial<2>({rr, 0.0}, st->part + off);  // after the partials read above
 }
 
-// The scalar recurrences of the solve, from the partials of the kernel
-// before: one block, launched after each reduction kernel.
-enum PcgFin { FIN_INIT, FIN_CG0, FIN_CG, FIN_FINAL };
+// The solve's scalar state from the partials of the kernel before: one
+// block, after k_cg_init (the initial state, into slot 1) and after
+// k_cg_final (the statistics of the solve whose last state is in slot rd).
+enum PcgFin { FIN_INIT, FIN_FINAL };
 template <int W>
-__global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit) {
-    if ((W == FIN_CG0 || W == FIN_CG) && st->done) return;
-    constexpr int N = (W == FIN_CG0 || W == FIN_CG) ? 5 : 2;
-    double t[N];
-    sum_partials<N>(st->part, nb, t);
+__global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit, int rd, uint32_t off) {
+    double t[2];
+    sum_partials<2>(st->part + off, nb, t);
     if (threadIdx.x != 0) return;
     if (W == FIN_INIT) {
         st->bb = t[0];
         const double rel = t[0] > 0.0 ? sqrt(t[1] / t[0]) : 0.0;
+        PcgScal a{};
+        a.rel = rel;
+        a.done = (rel <= (double)tol || maxit == 0u) ? 1u : 0u;
+        st->sc[1] = a;
         st->rel = rel;
         st->it = 0u;
-        st->done = (rel <= (double)tol || maxit == 0u) ? 1u : 0u;
-        st->gamma = 0.0;
-        st->pi = 0.0;
-        st->alpha = 0.0;
-        st->beta = 0.0;
+        st->done = a.done;
         st->tol_seen = tol;
-    } else if (W == FIN_CG0) {  // first application: p' = u, s' = w, pi = delta
-        st->gamma = t[0];
-        st->pi = t[1];
-        st->alpha = t[1] != 0.0 ? t[0] / t[1] : 0.0;
-        st->beta = 0.0;
-    } else if (W == FIN_CG) {
-        const uint32_t it = st->it + 1u;
-        st->it = it;
-        const double rel = st->bb > 0.0 ? sqrt(t[4] / st->bb) : 0.0;
-        st->rel = rel;
-        if (rel <= (double)tol || it >= maxit) {
-            st->done = 1u;
-        } else {
-            const double g = t[0], beta = st->gamma != 0.0 ? g / st->gamma : 0.0;
-            const double pi = t[1] + beta * (t[2] + t[3]) + beta * beta * st->pi;
-            st->beta = beta;
-            st->pi = pi;
-            st->alpha = pi != 0.0 ? g / pi : 0.0;
-            st->gamma = g;
-        }
     } else {
+        const PcgScal a = st->sc[rd];
+        st->it = a.it;
+        st->rel = a.rel;
+        st->done = a.done;
         st->true_rel = st->bb > 0.0 ? sqrt(t[0] / st->bb) : 0.0;
         st->steps += 1u;
-        st->iters += st->it;
+        st->iters += a.it;
         // a solve that stopped on its iteration slots before converging
-        if (!st->done) st->missed += 1u;
+        if (!a.done) st->missed += 1u;
         if (st->host_out) {
-            st->host_out[0] = st->it;
+            st->host_out[0] = a.it;
             st->host_out[1] = st->missed;
             st->host_out[2] = st->steps;
-            st->host_out[PCG_HOST_RING + (st->steps - 1u) % PCG_HOST_RING_LEN] = st->it;
+            st->host_out[PCG_HOST_RING + (st->steps - 1u) % PCG_HOST_RING_LEN] = a.it;
         }
     }
 }
@@ -2416,24 +2493,22 @@ inline uint32_t pcg_blocks(uint32_t n) { return ((n + 15) / 16) * ((n + 15) / 16
 inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
     k_cg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st);
-    k_pcg_fin<FIN_INIT><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), tol, maxit);
+    k_pcg_fin<FIN_INIT><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), tol, maxit, 0, 0u);
 }
 template <class S>
-void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first) {
+void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int stage, int rd,
+                     float tol, uint32_t maxit) {
     const uint32_t nt = cg_tiles(V.n);
-    k_cg_down0<S><<<dim3(nt, nt), CG_NT, 0, s>>>(V, C, st, parity & 1, first);
+    k_cg_down0<S><<<dim3(nt, nt), CG_NT, 0, s>>>(V, C, st, parity & 1, stage, rd & 1, nt * nt, tol, maxit);
 }
-template <class S>
-void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int first,
-                   float tol, uint32_t maxit) {
+template <class S> void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity) {
     const uint32_t nt = cg_tiles(V.n);
     k_cg_up0<S><<<dim3(nt, nt), CG_NT, 0, s>>>(V, C, st, parity & 1);
-    if (first) k_pcg_fin<FIN_CG0><<<1, 256, 0, s>>>(st, nt * nt, tol, maxit);
-    else k_pcg_fin<FIN_CG><<<1, 256, 0, s>>>(st, nt * nt, tol, maxit);
 }
-template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st) {
-    k_cg_final<S><<<grid2(V.n), B16, 0, s>>>(V, st);
-    k_pcg_fin<FIN_FINAL><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), 0.0f, 0u);
+template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st, int rd, float tol, uint32_t maxit) {
+    const uint32_t nt = cg_tiles(V.n), off = 5u * nt * nt;
+    k_cg_final<S><<<grid2(V.n), B16, 0, s>>>(V, st, rd & 1, nt * nt, tol, maxit, off);
+    k_pcg_fin<FIN_FINAL><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), 0.0f, 0u, 1 - (rd & 1), off);
 }
 #define RIBO_INSTANTIATE(S)                                                                                         \
     template void launch_mg_down<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);   \
@@ -2442,10 +2517,10 @@ template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgSta
     template void launch_mg_block<S>(hipStream_t, const MgLevelT<S>*, int, bool, const double*, const int32_t*, int, \
                                      const uint32_t*);                                                              \
     template void launch_cg_init<S>(hipStream_t, const CgVec<S>&, PcgState*, float, uint32_t);                     \
-    template void launch_cg_down0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int, int);        \
-    template void launch_cg_up0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int, int, float,    \
-                                   uint32_t);                                                                      \
-    template void launch_cg_final<S>(hipStream_t, const CgVec<S>&, PcgState*);
+    template void launch_cg_down0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int, int, int,    \
+                                     float, uint32_t);                                                             \
+    template void launch_cg_up0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int);             \
+    template void launch_cg_final<S>(hipStream_t, const CgVec<S>&, PcgState*, int, float, uint32_t);
 RIBO_INSTANTIATE(float)
 RIBO_INSTANTIATE(bf16)
 #undef RIBO_INSTANTIATE

@@ -481,14 +481,17 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
     const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
     const uint32_t* done = &w->pcg->done;
     run(w, "pcg_init", [&] { launch_cg_init<S>(s, V, w->pcg, tol, maxit); });
-    const uint32_t n_it = std::min(slots, maxit);
+    const uint32_t n_it = std::max(std::min(slots, maxit), 1u);
+    // V-cycle j applies the inner products of V-cycle j - 1 to the scalars
+    // kept in slot rd (the initial state is in slot 1) and writes slot 1 - rd
     for (uint32_t j = 0; j <= n_it; ++j) {
-        const int first = j == 0 ? 1 : 0;
-        run(w, "cg_down0", [&] { launch_cg_down0<S>(s, V, lv[1], w->pcg, (int)(j & 1u), first); });
+        const int stage = j == 0 ? 0 : (j == 1 ? 1 : 2);
+        const int rd = j == 0 ? 1 : (int)(j & 1u);
+        run(w, "cg_down0", [&] { launch_cg_down0<S>(s, V, lv[1], w->pcg, (int)(j & 1u), stage, rd, tol, maxit); });
         vcycle_sym_inner<S>(w, lv, done);
-        run(w, "cg_up0", [&] { launch_cg_up0<S>(s, V, lv[1], w->pcg, (int)(j & 1u), first, tol, maxit); });
+        run(w, "cg_up0", [&] { launch_cg_up0<S>(s, V, lv[1], w->pcg, (int)(j & 1u)); });
     }
-    run(w, "pcg_final", [&] { launch_cg_final<S>(s, V, w->pcg); });
+    run(w, "pcg_final", [&] { launch_cg_final<S>(s, V, w->pcg, (int)((n_it - 1u) & 1u), tol, maxit); });
 }
 
 void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define RIBO_ABI_VERSION 1u
+#define RIBO_ABI_VERSION 2u
 
 /* Opaque world handle. */
 typedef struct RiboWorld RiboWorld;
@@ -194,16 +194,20 @@ typedef struct RiboMgLevel {
 /* WorldSettings (src/world/mod.rs:91-134) plus the solver extensions.
  * terrain_enabled must be 0: grain dynamics are not part of the world-step
  * path, the terrain (gamma) is a frozen input and its slope is derived from
- * it on upload.  Light is a frozen input (RIBO_BUF_LIGHT); light_* settings
- * and contact_enabled are accepted and ignored. */
+ * it on upload.  Light follows light_mode (an extension): 0 keeps the light
+ * field a frozen input (RIBO_BUF_LIGHT), 1 runs the reference's light
+ * schedule with light_rows_per_step and light_interval.  contact_enabled is
+ * accepted and ignored (agents are not part of the path). */
 typedef struct RiboWorldSettings {
     uint32_t fluid_enabled;       /* reference default 1 */
     uint32_t fluid_substep;       /* reference default 2, clamped to 1..4 */
     uint32_t multigrid;           /* 1 = multigrid, 0 = Jacobi ping-pong */
     uint32_t mg_cycles;           /* V-cycles per fluid step (default 1) */
     uint32_t jacobi_iters;        /* rounded up to even, clamped 2..128 */
-    uint32_t light_rows_per_step; /* ignored */
-    uint32_t light_interval;      /* ignored */
+    uint32_t light_rows_per_step; /* light_mode 1: propagation passes per step
+                                     (at most 16; reference default 1); 0 = the
+                                     full sweep every light_interval steps */
+    uint32_t light_interval;      /* reference default 10 */
     uint32_t terrain_enabled;     /* must be 0 */
     uint32_t contact_enabled;     /* ignored */
     /* --- extensions (not in the reference) --- */
@@ -215,17 +219,27 @@ typedef struct RiboWorldSettings {
                                      fluid cells of level 0 */
     uint32_t mg_max_cycles;       /* iteration cap of the tolerance mode.  Each
                                      ribo_world_encode_steps call records, per
-                                     solve, 1.3 x the previous solve's iterations
-                                     + 4 (multiple of 4, at most the cap); a solve
+                                     solve, the most iterations any of the last
+                                     64 solves took plus max(3, 1/5 of it),
+                                     rounded up to even, at most the cap; a solve
                                      that runs out is counted (see
                                      ribo_world_solver_stats) and the next call
-                                     records the full cap again. */
+                                     records twice the maximum + 4. */
     uint32_t precision;           /* RIBO_PRECISION_FP32 (drop-in) or _MIXED */
     uint32_t ledger_check;        /* 1: exact per-channel totals checked on the
                                      device after every transport pass */
     uint32_t no_graphs;           /* 1: record every step eagerly instead of
                                      replaying captured HIP graphs */
-    uint32_t _pad1;
+    uint32_t light_mode;          /* 0: the light field is a frozen input (the
+                                     default); 1: the reference's light step
+                                     before the fluid of every step
+                                     (src/world/mod.rs:1506-1525): the full sweep
+                                     when the light is stale (after an upload of
+                                     GAMMA or CHEM, or ribo_world_invalidate_light)
+                                     or, with light_rows_per_step = 0, every
+                                     light_interval steps; else
+                                     light_rows_per_step propagation passes.  An
+                                     upload of LIGHT makes it current. */
 } RiboWorldSettings;
 
 enum { RIBO_PRECISION_FP32 = 0, RIBO_PRECISION_MIXED = 1 };
@@ -322,6 +336,10 @@ RIBO_API int ribo_world_sync(RiboWorld* w);
  * (src/gpu/mod.rs:51 waits; a serving loop polls): 1 when every recorded
  * step has finished, 0 while work is still running, negative on error. */
 RIBO_API int ribo_world_idle(RiboWorld* w);
+
+/* World::invalidate_light (src/world/mod.rs:1400-1402): the next step runs
+ * the full light sweep (light_mode 1). */
+RIBO_API int ribo_world_invalidate_light(RiboWorld* w);
 
 /* ---- ledger (Ledger, src/world/mod.rs:39-82; ledger.wgsl:5-27) ---------- */
 

@@ -27,6 +27,7 @@ def lib() -> ctypes.CDLL:
         sig = {
             "oracle_create": (P, [P, U32, I]),
             "oracle_destroy": (None, [P]),
+            "oracle_invalidate_light": (None, [P]),
             "oracle_set_params": (None, [P, P]),
             "oracle_get_params": (None, [P, P]),
             "oracle_set_settings": (I, [P, P]),
@@ -133,6 +134,9 @@ class OracleWorld:
 
     def encode_steps(self, n: int) -> int:
         return lib().oracle_encode_steps(self._h, n)
+
+    def invalidate_light(self):
+        lib().oracle_invalidate_light(self._h)
 
     def run(self, n: int):
         done = 0

@@ -182,7 +182,8 @@ struct OracleWorld {
     bool fluid_was_on = true;  // src/world/mod.rs:929
     std::vector<uint32_t> chem, chem_next, gamma, agg_act, agg_nb, solid_mask, redox_eaten, mg_fluid;
     std::vector<uint32_t> force_vectors;
-    std::vector<float> light, slope, heat_src;
+    std::vector<float> light, light_next, slope, heat_src;
+    bool light_dirty = true;  // src/world/mod.rs:209
     std::vector<uint32_t> mg_n, mg_off;
     uint32_t mg_total = 0;
     Fluid<float> f32;
@@ -442,6 +443,61 @@ float light_absorbed_at(const OracleWorld& w, const RiboSimParams& p, uint32_t e
 inline float uv_light_at_cell(const OracleWorld& w, uint32_t x, uint32_t y) {  // light.wgsl:46-50
     uint32_t lx = std::min(x, w.G - 1u) / LIGHT_DIV, ly = std::min(y, w.G - 1u) / LIGHT_DIV;
     return w.light[ly * w.Lsz + lx];
+}
+
+// ---- light propagation (light.wgsl:44-46,104-176) ------------------------------
+float row_water_of(const OracleWorld& w, const RiboSimParams& p, uint32_t band) {  // light.wgsl:44-46
+    return rb_expf(-rb_max(p.uv_depth, 0.0f) * (band == 1u ? 0.03f : 1.0f) * (float)LIGHT_DIV / (float)w.G);
+}
+// light_transmit_pass (light.wgsl:121-127): every light cell's transmission,
+// both bands, written into the light field.
+void light_transmit_pass(OracleWorld& w, const RiboSimParams& p) {
+    const uint32_t LS = w.Lsz, LC = LS * LS;
+    for (uint32_t i = 0; i < LC; ++i) {
+        w.light[i] = light_transmit(w, p, i % LS, i / LS, 0u);
+        w.light[LC + i] = light_transmit(w, p, i % LS, i / LS, 1u);
+    }
+}
+// compute_uv_light (light.wgsl:129-160): top row times the sun and the water,
+// then row by row downwards, each cell times the light above and the water.
+void compute_uv_light(OracleWorld& w, const RiboSimParams& p) {
+    const uint32_t LS = w.Lsz, LC = LS * LS, top = LS - 1u;
+    for (uint32_t x = 0; x < LS; ++x)
+        for (uint32_t band = 0; band < 2u; ++band) w.light[band * LC + top * LS + x] *= row_water_of(w, p, band) * p.sun_now;
+    for (uint32_t t = 1; t < LS; ++t) {
+        const uint32_t y = LS - 1u - t;
+        for (uint32_t x = 0; x < LS; ++x)
+            for (uint32_t band = 0; band < 2u; ++band) {
+                float above = light_above(w, p, x, y + 1u, band);
+                w.light[band * LC + y * LS + x] *= above * row_water_of(w, p, band);
+            }
+    }
+}
+// light_propagate + light_commit (light.wgsl:162-183): every cell takes the
+// row above from the previous field.
+void light_propagate(OracleWorld& w, const RiboSimParams& p) {
+    const uint32_t LS = w.Lsz, LC = LS * LS;
+    for (uint32_t i = 0; i < LC; ++i) {
+        const uint32_t x = i % LS, y = i / LS;
+        for (uint32_t band = 0; band < 2u; ++band) {
+            float above = p.sun_now;
+            if (y + 1u < LS) above = light_above(w, p, x, y + 1u, band);
+            w.light_next[band * LC + i] = above * row_water_of(w, p, band) * light_transmit(w, p, x, y, band);
+        }
+    }
+    w.light = w.light_next;
+}
+// The light step of encode_steps (src/world/mod.rs:1506-1525).
+void light_step(OracleWorld& w, const RiboSimParams& p) {
+    if (!w.st.light_mode) return;
+    const bool sweep = w.light_dirty || (w.st.light_rows_per_step == 0 && p.epoch % std::max(w.st.light_interval, 1u) == 0);
+    if (sweep) {
+        w.light_dirty = false;
+        light_transmit_pass(w, p);
+        compute_uv_light(w, p);
+    } else {
+        for (uint32_t k = 0; k < std::min(w.st.light_rows_per_step, 16u); ++k) light_propagate(w, p);
+    }
 }
 
 template <class R> R vent_cycle(const Ctx& c, uint32_t x) {  // fluid.wgsl:46-54
@@ -1569,6 +1625,7 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
 }
 
 template <class R> void world_step(OracleWorld& w, Fluid<R>& fl, const RiboSimParams& P, uint32_t substep) {
+    light_step(w, P);
     if (w.st.fluid_enabled && P.epoch % substep == 0) fluid_step<R>(w, fl, P);
     if (P.epoch % std::max(P.transport_every, 1u) == 0) transport<R>(w, fl, P);
 }
@@ -1646,7 +1703,7 @@ OracleWorld* oracle_create(const RiboWorldConfig* cfg, uint32_t seed, int fp64) 
     w->chem.assign(4 * G, 0); w->chem_next.assign(4 * G, 0); w->gamma.assign(G, 0);
     w->agg_act.assign(G, 0); w->agg_nb.assign(G, 0);
     w->solid_mask.assign(F, 0); w->redox_eaten.assign(F, 0); w->force_vectors.assign(2 * F, 0);
-    w->light.assign(2 * (size_t)w->Lsz * w->Lsz, 0.0f); w->slope.assign(2 * G, 0.0f); w->heat_src.assign(2 * F, 0.0f);
+    w->light.assign(2 * (size_t)w->Lsz * w->Lsz, 0.0f); w->light_next = w->light; w->slope.assign(2 * G, 0.0f); w->heat_src.assign(2 * F, 0.0f);
     make_levels(w->F, 8, &w->mg_total);
     w->mg_fluid.assign(w->mg_total, 0);
     if (fp64) w->f64.alloc(F, w->F, w->mg_total);
@@ -1655,12 +1712,13 @@ OracleWorld* oracle_create(const RiboWorldConfig* cfg, uint32_t seed, int fp64) 
 }
 
 void oracle_destroy(OracleWorld* w) { delete w; }
+void oracle_invalidate_light(OracleWorld* w) { w->light_dirty = true; }  // src/world/mod.rs:1400-1402
 void oracle_set_params(OracleWorld* w, const RiboSimParams* p) { w->params = *p; }
 void oracle_get_params(const OracleWorld* w, RiboSimParams* p) { *p = w->params; }
 
 int oracle_set_settings(OracleWorld* w, const RiboWorldSettings* s) {
     if (s->terrain_enabled) return RIBO_ERR_UNSUPPORTED;
-    if (s->mg_min_size < 1) return RIBO_ERR_ARG;
+    if (s->mg_min_size < 1 || s->light_mode > 1u) return RIBO_ERR_ARG;
     w->st = *s;
     uint32_t total;
     make_levels(w->F, s->mg_min_size, &total);
@@ -1695,14 +1753,14 @@ static void compute_slope(OracleWorld* w) {  // terrain.wgsl:104-133
 int oracle_upload(OracleWorld* w, int which, const void* src, size_t bytes) {
     if (bytes != buf_bytes(*w, which)) return RIBO_ERR_ARG;
     switch (which) {
-        case RIBO_BUF_CHEM: memcpy(w->chem.data(), src, bytes); return 0;
-        case RIBO_BUF_GAMMA: memcpy(w->gamma.data(), src, bytes); compute_slope(w);
+        case RIBO_BUF_CHEM: memcpy(w->chem.data(), src, bytes); w->light_dirty = true; return 0;
+        case RIBO_BUF_GAMMA: memcpy(w->gamma.data(), src, bytes); compute_slope(w); w->light_dirty = true;
             {
                 Ctx c{w, w->G, w->F, &w->params};
                 build_solid_mask<float>(*w, c);
             }
             return 0;
-        case RIBO_BUF_LIGHT: memcpy(w->light.data(), src, bytes); return 0;
+        case RIBO_BUF_LIGHT: memcpy(w->light.data(), src, bytes); w->light_dirty = false; return 0;
         case RIBO_BUF_HEAT_SRC: memcpy(w->heat_src.data(), src, bytes); return 0;
         case RIBO_BUF_REDOX_EATEN: memcpy(w->redox_eaten.data(), src, bytes); return 0;
         default: break;

@@ -9,7 +9,8 @@
  * It restates, kernel by kernel and in the reference's dispatch order, the
  * WGSL in shaders/world/{fluid,multigrid,transport,ledger}.wgsl and
  * shaders/common/{rng,chem,terrain}.wgsl as scheduled by
- * World::encode_steps (src/world/mod.rs:1406-1632).  Every function in
+ * World::encode_steps (src/world/mod.rs:1406-1632), and the light step of
+ * shaders/world/light.wgsl (light_mode 1).  Every function in
  * ribo_oracle.cpp cites the reference lines it follows.
  *
  * The fluid and multigrid are templated on the field type: the float
@@ -22,7 +23,10 @@
  * 5-point operator and against manufactured solutions; transport against
  * exact invariants (per-channel conservation, v = 0 identity, integer
  * shifts, rock blocking, edge clamping, the 256-mover cap) and hop
- * statistics; the ledger against a plain numpy count.
+ * statistics; the ledger against a plain numpy count; the light against
+ * closed forms (water attenuation per row, transmission of known grain and
+ * monomer loads, the sun's sideways shift) and the propagation's fixed
+ * point, which is the sweep.
  */
 #ifndef RIBO_ORACLE_H
 #define RIBO_ORACLE_H
@@ -41,6 +45,8 @@ typedef struct OracleWorld OracleWorld;
 /* fp64 = 0: binary32 fields (parity oracle); 1: binary64 fluid fields. */
 OracleWorld* oracle_create(const RiboWorldConfig* cfg, uint32_t seed, int fp64);
 void oracle_destroy(OracleWorld* w);
+/* World::invalidate_light: the next step runs the full light sweep. */
+void oracle_invalidate_light(OracleWorld* w);
 void oracle_set_params(OracleWorld* w, const RiboSimParams* p);
 void oracle_get_params(const OracleWorld* w, RiboSimParams* p);
 int oracle_set_settings(OracleWorld* w, const RiboWorldSettings* s);

@@ -175,6 +175,15 @@ void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* v
 void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
                     uint32_t* chem);
 
+// light (two bands of Lsz x Lsz cells, band after band)
+// Transmission of every light cell into `light` (the sweep's pre-pass).
+void launch_light_transmit(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, float* light);
+// The full top-to-bottom sweep, in place over the transmissions.
+void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, float* light);
+// One propagation pass: lout from the rows above in lin (distinct buffers).
+void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
+                            float* lout);
+
 // transport
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e);
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);

@@ -335,6 +335,129 @@ __device__ float light_absorbed_at(const Geo& g, const RiboSimParams& P, const u
     incoming = rb_max(incoming, 0.0f);
     return incoming * ((1.0f - light_transmit0(g, P, chem, lx, ly)) + SUN_WATER_ABSORB);
 }
+// ---- light propagation (light.wgsl:44-46,56-82,84-102,104-176) ----------------------
+// The light field is two bands of LS x LS cells (LS = grid / LIGHT_DIV) stored
+// band after band; row LS - 1 is the top, where the sun enters.
+constexpr float DEEP_LIGHT_DEPTH = 0.35f;
+constexpr float DEEP_WATER_DEPTH = 0.03f;
+
+// row_water_of (light.wgsl:44-46): water attenuation across one light row.
+__device__ __forceinline__ float row_water(const RiboSimParams& P, uint32_t G, uint32_t band) {
+    return rb_expf(-rb_max(P.uv_depth, 0.0f) * (band == 1u ? DEEP_WATER_DEPTH : 1.0f) * (float)LIGHT_DIV / (float)G);
+}
+// The sun's sideways shift of light_above (light.wgsl:91-93): light at (x, y)
+// comes from (x - sun_slope, y + 1), integer part `base`, fraction f.
+struct SunShift { float sh, f; int32_t base; };
+__device__ __forceinline__ SunShift sun_shift(const RiboSimParams& P) {
+    SunShift s;
+    s.sh = -P.sun_slope;
+    s.base = rb_f2i(rb_floor(s.sh));
+    s.f = s.sh - (float)s.base;
+    return s;
+}
+// light_above (light.wgsl:90-102) over one row of one band, from HBM or LDS.
+template <class Row> __device__ __forceinline__ float light_above_row(Row row, uint32_t LS, uint32_t x, const SunShift& s) {
+    float acc = 0.0f;
+#pragma unroll
+    for (int d = 0; d < LIGHT_TAPS; ++d) {
+        const int32_t x0 = rb_clampi((int32_t)x + d - LIGHT_TAPS / 2 + s.base, 0, (int32_t)LS - 1);
+        const int32_t x1 = rb_clampi(x0 + 1, 0, (int32_t)LS - 1);
+        acc += rb_mix(row[x0], row[x1], s.f);
+    }
+    const float cpos = (float)x + s.sh;
+    const float inside = rb_clamp(cpos + 1.0f, 0.0f, 1.0f) * rb_clamp((float)LS - cpos, 0.0f, 1.0f);
+    return acc / (float)LIGHT_TAPS * inside;
+}
+// light_transmit (light.wgsl:56-82) of light cell (lx, ly) for both bands:
+// the grain and monomer sums over its LIGHT_DIV^2 grid cells are formed once
+// (the bands differ only in the monomer depth factor).  Agent shade is zero.
+__device__ __forceinline__ void light_transmit2(const Geo& g, const RiboSimParams& P, const uint32_t* chem, uint32_t lx,
+                                                uint32_t ly, float* t0, float* t1) {
+    const uint4* c4 = reinterpret_cast<const uint4*>(chem);
+    float gg = 0.0f;
+    uint32_t m = 0;
+#pragma unroll
+    for (uint32_t dy = 0; dy < LIGHT_DIV; ++dy)
+#pragma unroll
+        for (uint32_t dx = 0; dx < LIGHT_DIV; ++dx) {
+            const uint32_t c = (ly * LIGHT_DIV + dy) * g.G + lx * LIGHT_DIV + dx;
+            gg += (float)g.gamma[c];
+            const uint4 v = c4[c];
+            m += (v.x & CHEM_STATE_MASK) + (v.x >> 16) + (v.y & CHEM_STATE_MASK) + (v.y >> 16) +
+                 (v.z & CHEM_STATE_MASK) + (v.z >> 16) + (v.w & CHEM_STATE_MASK) + (v.w >> 16);
+        }
+    gg /= (float)LIGHT_DIV;
+    const float dens = (float)m / (float)(LIGHT_DIV * LIGHT_DIV);
+    const float tau = rb_max(P.monomer_uv_absorb, 0.0f) * dens * (float)LIGHT_DIV / (float)g.G;
+    const float shade = 0.0f;
+    *t0 = rb_expf(-UV_SHADOW_ABSORB * gg - 0.15f * shade - tau * 1.0f);
+    *t1 = rb_expf(-UV_SHADOW_ABSORB * gg - 0.15f * shade - tau * DEEP_LIGHT_DEPTH);
+}
+// light_transmit_pass (light.wgsl:121-127): the sweep's pre-pass writes each
+// light cell's transmission into the light field.
+__global__ __launch_bounds__(256) void k_light_transmit(Geo g, RiboSimParams P, const uint32_t* chem, float* light) {
+    const uint32_t LS = g.Lsz, LC = LS * LS;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= LC) return;
+    float t0, t1;
+    light_transmit2(g, P, chem, i % LS, i / LS, &t0, &t1);
+    light[i] = t0;
+    light[LC + i] = t1;
+}
+// compute_uv_light (light.wgsl:129-160): the sequential top-to-bottom sweep.
+// The bands never mix, so each band is one workgroup; the finished row above
+// lives in LDS (two row buffers, one barrier per row: a row's readers finish
+// before the barrier that precedes its buffer's reuse two rows later).
+__global__ __launch_bounds__(1024) void k_light_sweep(Geo g, RiboSimParams P, float* light) {
+    extern __shared__ float rows[];
+    const uint32_t LS = g.Lsz, LC = LS * LS, band = blockIdx.x, tid = threadIdx.x;
+    float* L = light + band * LC;
+    const float rw = row_water(P, g.G, band);
+    const SunShift s = sun_shift(P);
+    float* prev = rows;
+    float* cur = rows + LS;
+    const uint32_t top = LS - 1u;
+    for (uint32_t x = tid; x < LS; x += 1024u) {
+        const float v = L[top * LS + x] * (rw * P.sun_now);
+        L[top * LS + x] = v;
+        prev[x] = v;
+    }
+    __syncthreads();
+    for (uint32_t t = 1; t < LS; ++t) {
+        const uint32_t y = LS - 1u - t;
+        for (uint32_t x = tid; x < LS; x += 1024u) {
+            const float above = light_above_row(prev, LS, x, s);
+            const float v = L[y * LS + x] * (above * rw);
+            L[y * LS + x] = v;
+            cur[x] = v;
+        }
+        __syncthreads();
+        float* tmp = prev;
+        prev = cur;
+        cur = tmp;
+    }
+}
+// light_propagate (light.wgsl:162-176): every light cell at once takes the
+// row above from the previous field; light_commit's copy is a ping-pong of
+// the two fields.
+__global__ __launch_bounds__(256) void k_light_propagate(Geo g, RiboSimParams P, const uint32_t* chem, const float* lin,
+                                                         float* lout) {
+    const uint32_t LS = g.Lsz, LC = LS * LS;
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= LC) return;
+    const uint32_t x = i % LS, y = i / LS;
+    float t0, t1;
+    light_transmit2(g, P, chem, x, y, &t0, &t1);
+    float a0 = P.sun_now, a1 = P.sun_now;
+    if (y + 1u < LS) {
+        const SunShift s = sun_shift(P);
+        a0 = light_above_row(lin + (y + 1u) * LS, LS, x, s);
+        a1 = light_above_row(lin + LC + (y + 1u) * LS, LS, x, s);
+    }
+    lout[i] = a0 * row_water(P, g.G, 0u) * t0;
+    lout[LC + i] = a1 * row_water(P, g.G, 1u) * t1;
+}
+
 __device__ __forceinline__ float vent_cycle(const RiboSimParams& P, uint32_t x, uint32_t F) {  // fluid.wgsl:46-54
     float m = 1.0f;
     if (P.vent_cycle_amp > 0.0f && P.vent_cycle_period >= 1.0f) {
@@ -2537,6 +2660,16 @@ void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* v
 void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
                     uint32_t* chem) {
     k_thermal<<<grid2(g.G), B16, 0, s>>>(g, P, ep, temp, chem);
+}
+void launch_light_transmit(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, float* light) {
+    k_light_transmit<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, chem, light);
+}
+void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, float* light) {
+    k_light_sweep<<<2, 1024, 2 * (size_t)g.Lsz * sizeof(float), s>>>(g, P, light);
+}
+void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
+                            float* lout) {
+    k_light_propagate<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, chem, lin, lout);
 }
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e) {
     k_agg<<<grid2(g.G), B16, 0, s>>>(g, chem, agg_e);

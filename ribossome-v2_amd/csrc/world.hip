@@ -99,7 +99,9 @@ struct RiboWorld {
     uint32_t* chem[2] = {nullptr, nullptr};  // chem_grid / chem_next, ping-pong
     int cur = 0;
     uint32_t* gamma = nullptr;
-    float* light = nullptr;
+    float* light[2] = {nullptr, nullptr};  // light_grid / light_next, ping-pong
+    int lcur = 0;
+    bool light_dirty = true;  // src/world/mod.rs:209
     float2* slope = nullptr;
     uint32_t* agg_e = nullptr;
     // fluid-resolution buffers (group 2)
@@ -164,7 +166,7 @@ struct RiboWorld {
     uint32_t* epoch_dev = nullptr;
     uint32_t epoch_dev_value = 0;
     bool epoch_dev_known = false;
-    struct GraphEntry { uint32_t phase, n, slots; int cur, tcur, cur_end, tcur_end; hipGraphExec_t exec; };
+    struct GraphEntry { uint32_t phase, n, slots; int cur, tcur, lcur, cur_end, tcur_end, lcur_end; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
     RiboSimParams graph_params{};
     RiboWorldSettings graph_settings{};
@@ -179,7 +181,7 @@ struct RiboWorld {
     Geo geo() const {
         Geo g;
         g.G = G; g.F = F; g.Lsz = Lsz; g.wupc = cfg.world_units_per_cell; g.sim_size = sim_size;
-        g.gamma = gamma; g.light = light; g.slope = slope; g.slope_f = slope_f; g.cflags = cflags;
+        g.gamma = gamma; g.light = light[lcur]; g.slope = slope; g.slope_f = slope_f; g.cflags = cflags;
         g.normal = normal; g.fill = fill; g.dragg = dragg; g.heat = heat;
         return g;
     }
@@ -545,6 +547,31 @@ void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) { 
     if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, w->chem[w->cur], w->ledger, P.epoch, ep, 1); });
 }
 
+// Light (src/world/mod.rs:1506-1525) with light_mode 1, before the fluid:
+// the full sweep (light_transmit_pass, then compute_uv_light) when the light
+// is stale or, with light_rows_per_step = 0, every light_interval steps;
+// otherwise light_rows_per_step (at most 16) propagation passes, each a
+// ping-pong of the two fields instead of light_commit's copy.  Agent shade
+// (clear_shade, agents_shade) is zero: there are no agents.
+void light_step(RiboWorld* w, const RiboSimParams& P, uint32_t epoch) {
+    if (!w->st.light_mode) return;
+    hipStream_t s = w->stream;
+    const bool sweep = w->light_dirty || (w->st.light_rows_per_step == 0 && epoch % std::max(w->st.light_interval, 1u) == 0);
+    if (sweep) {
+        w->light_dirty = false;
+        Geo g = w->geo();
+        run(w, "light_transmit", [&] { launch_light_transmit(s, g, P, w->chem[w->cur], w->light[w->lcur]); });
+        run(w, "light_sweep", [&] { launch_light_sweep(s, g, P, w->light[w->lcur]); });
+        return;
+    }
+    for (uint32_t k = 0; k < std::min(w->st.light_rows_per_step, 16u); ++k) {
+        Geo g = w->geo();
+        run(w, "light_propagate",
+            [&] { launch_light_propagate(s, g, P, w->chem[w->cur], w->light[w->lcur], w->light[1 - w->lcur]); });
+        w->lcur = 1 - w->lcur;
+    }
+}
+
 // Steps [first, first + count) of one encode_steps call: epochs
 // params.epoch + first + i, passed as is (eager) or as i on top of the
 // device epoch base `ep` (graph capture).
@@ -555,6 +582,7 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
         P.epoch = ep ? i : epoch;
         P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, epoch) : 0.0f;
         P.sun_slope = sun_slope_at(w->params, epoch);
+        light_step(w, P, epoch);
         if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
         if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep);
     }
@@ -565,7 +593,10 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
 // uses the host-computed sun of each step, and a pending redox_eaten upload
 // is consumed once.
 bool graph_eligible(const RiboWorld* w) {
-    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !(w->params.sun_heat > 0.0f);
+    // the light step reads the sun of each step (constant without a day
+    // cycle), and a stale light's sweep is a one-off
+    const bool light_varies = w->st.light_mode && (w->light_dirty || w->params.day_period >= 1.0f);
+    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !(w->params.sun_heat > 0.0f) && !light_varies;
 }
 
 // Arms the per-step ledger check with the totals of the current grid (when
@@ -583,6 +614,7 @@ int arm_ledger(RiboWorld* w) {
 
 int validate_settings(const RiboWorldSettings* s) {
     if (s->terrain_enabled) return fail(RIBO_ERR_UNSUPPORTED, "terrain physics is not part of the world-step path (terrain_enabled must be 0)");
+    if (s->light_mode > 1u) return fail(RIBO_ERR_ARG, "light_mode must be 0 (frozen) or 1 (the reference's light schedule)");
     if (s->precision != RIBO_PRECISION_FP32 && s->precision != RIBO_PRECISION_MIXED)
         return fail(RIBO_ERR_ARG, "unknown precision");
     if (s->precision == RIBO_PRECISION_MIXED && !(s->mg_tol > 0.0f))
@@ -683,7 +715,8 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         return fail(RIBO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     if ((rc = dalloc(&w->chem[0], 4 * G)) || (rc = dalloc(&w->chem[1], 4 * G)) || (rc = dalloc(&w->gamma, G)) ||
-        (rc = dalloc(&w->light, 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(&w->slope, G)) || (rc = dalloc(&w->agg_e, G)) ||
+        (rc = dalloc(&w->light[0], 2 * (size_t)w->Lsz * w->Lsz)) ||
+        (rc = dalloc(&w->light[1], 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(&w->slope, G)) || (rc = dalloc(&w->agg_e, G)) ||
         (rc = dalloc(&w->vel_a, F)) || (rc = dalloc(&w->vel_b, F)) || (rc = dalloc(&w->p_a, F)) || (rc = dalloc(&w->p_b, F)) ||
         (rc = dalloc(&w->div, F)) || (rc = dalloc(&w->temp[0], F)) || (rc = dalloc(&w->temp[1], F)) ||
         (rc = dalloc(&w->redox[0], F)) || (rc = dalloc(&w->redox[1], F)) || (rc = dalloc(&w->eaten, F)) ||
@@ -723,7 +756,7 @@ int ribo_world_destroy(RiboWorld* w) {
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     resolve_timers(w);
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
-    dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light); dfree(w->slope); dfree(w->agg_e);
+    dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light[0]); dfree(w->light[1]); dfree(w->slope); dfree(w->agg_e);
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
@@ -786,7 +819,7 @@ int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t by
     switch (which) {
         case RIBO_BUF_CHEM: dst = w->chem[w->cur]; break;
         case RIBO_BUF_GAMMA: dst = w->gamma; break;
-        case RIBO_BUF_LIGHT: dst = w->light; break;
+        case RIBO_BUF_LIGHT: dst = w->light[w->lcur]; w->light_dirty = false; break;
         case RIBO_BUF_VELOCITY: dst = w->vel_a; break;
         case RIBO_BUF_PRESSURE: dst = w->p_a; break;
         case RIBO_BUF_TEMP: dst = w->temp[w->tcur]; break;
@@ -796,6 +829,8 @@ int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t by
         default: return fail(RIBO_ERR_ARG, "buffer is derived and cannot be uploaded");
     }
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w->stream));
+    // new terrain or matter leaves the light stale (src/world/mod.rs:986,1026)
+    if (which == RIBO_BUF_GAMMA || which == RIBO_BUF_CHEM) w->light_dirty = true;
     if (which == RIBO_BUF_GAMMA) {
         int rc = rebuild_terrain(w);
         if (rc) return rc;
@@ -815,7 +850,7 @@ int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes)
     switch (which) {
         case RIBO_BUF_CHEM: src = w->chem[w->cur]; break;
         case RIBO_BUF_GAMMA: src = w->gamma; break;
-        case RIBO_BUF_LIGHT: src = w->light; break;
+        case RIBO_BUF_LIGHT: src = w->light[w->lcur]; break;
         case RIBO_BUF_SLOPE: src = w->slope; break;
         case RIBO_BUF_VELOCITY: src = w->vel_a; break;
         case RIBO_BUF_PRESSURE: src = w->p_a; break;
@@ -907,6 +942,11 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         const uint32_t te = std::max(w->params.transport_every, 1u);
         uint32_t period = substep;
         while (period % te != 0) period += substep;
+        if (w->st.light_mode && w->st.light_rows_per_step == 0) {  // periodic sweeps
+            const uint32_t li = std::max(w->st.light_interval, 1u);
+            const uint32_t base = period;
+            while (period % li != 0) period += base;
+        }
         // tolerance-mode steps carry their solver iterations: shorter graphs
         const uint32_t max_graph = (w->st.multigrid && w->st.mg_tol > 0.0f) ? 8u : RIBO_MAX_STEPS_PER_CALL;
         for (uint32_t first = 0; first < steps;) {
@@ -914,10 +954,12 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             const uint32_t phase = (w->params.epoch + first) % period;
             RiboWorld::GraphEntry* ge = nullptr;
             for (auto& e : w->graphs)
-                if (e.phase == phase && e.n == n && e.slots == w->pcg_slots && e.cur == w->cur && e.tcur == w->tcur) ge = &e;
+                if (e.phase == phase && e.n == n && e.slots == w->pcg_slots && e.cur == w->cur && e.tcur == w->tcur &&
+                    e.lcur == w->lcur)
+                    ge = &e;
             if (!ge) {
                 if (w->graphs.size() >= 32) clear_graphs(w);
-                RiboWorld::GraphEntry e{phase, n, w->pcg_slots, w->cur, w->tcur, 0, 0, nullptr};
+                RiboWorld::GraphEntry e{phase, n, w->pcg_slots, w->cur, w->tcur, w->lcur, 0, 0, 0, nullptr};
                 hipGraph_t graph = nullptr;
                 HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
                 record_steps(w, first, n, substep, w->epoch_dev);
@@ -925,8 +967,10 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
                 hipError_t ce = hipStreamEndCapture(w->stream, &graph);
                 e.cur_end = w->cur;
                 e.tcur_end = w->tcur;
+                e.lcur_end = w->lcur;
                 w->cur = e.cur;
                 w->tcur = e.tcur;
+                w->lcur = e.lcur;
                 if (ce != hipSuccess) return fail(RIBO_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(ce));
                 hipError_t ie = hipGraphInstantiate(&e.exec, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
@@ -937,6 +981,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             HIPCHK(hipGraphLaunch(ge->exec, w->stream));
             w->cur = ge->cur_end;
             w->tcur = ge->tcur_end;
+            w->lcur = ge->lcur_end;
             w->epoch_dev_value += n;
             first += n;
         }
@@ -956,6 +1001,12 @@ int ribo_world_run(RiboWorld* w, uint32_t steps) {
         if (n < 0) return n;
         done += (uint32_t)n;
     }
+    return RIBO_OK;
+}
+
+int ribo_world_invalidate_light(RiboWorld* w) {  // src/world/mod.rs:1400-1402
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    w->light_dirty = true;
     return RIBO_OK;
 }
 

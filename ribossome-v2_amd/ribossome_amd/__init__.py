@@ -178,7 +178,7 @@ class WorldSettings(ctypes.Structure):
         ("precision", ctypes.c_uint32),
         ("ledger_check", ctypes.c_uint32),
         ("no_graphs", ctypes.c_uint32),
-        ("_pad1", ctypes.c_uint32),
+        ("light_mode", ctypes.c_uint32),
     ]
 
     @classmethod
@@ -278,6 +278,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_run": (I, [P, U32]),
         "ribo_world_sync": (I, [P]),
         "ribo_world_idle": (I, [P]),
+        "ribo_world_invalidate_light": (I, [P]),
         "ribo_world_ledger": (I, [P, P]),
         "ribo_world_ledger_status": (I, [P, ctypes.POINTER(U32)]),
         "ribo_world_set_timing": (I, [P, I]),
@@ -439,6 +440,10 @@ class World:
     def idle(self) -> bool:
         """True when every recorded step has finished (does not wait)."""
         return _check(lib().ribo_world_idle(self._h), "idle") == 1
+
+    def invalidate_light(self):
+        """World::invalidate_light: the next step runs the full light sweep."""
+        _check(lib().ribo_world_invalidate_light(self._h), "invalidate_light")
 
     # -- readback helpers named like the reference's test API --
     def read_cells_blocking(self) -> np.ndarray:

@@ -37,6 +37,21 @@ CASES = {
     # UV light (frozen field) and solar heating on
     "light": dict(cfg=ra.WorldConfig(64, 32), seed=13, inputs=dict(rock=0.1, vent_rows=2),
                   params=dict(uv_strength=3.0, sun_heat=1.0, direct_photoactivation=0.5), settings={}, light=True),
+    # the reference's light step: a sweep on the first step, then one
+    # propagation pass per step under the day cycle's slanted sun (eager
+    # recording: the sun changes every step), water attenuation, solar heating
+    "light_propagate": dict(cfg=ra.WorldConfig(128, 64), seed=15, inputs=dict(rock=0.12, rubble=True, vent_rows=3),
+                            params=dict(uv_strength=3.0, uv_depth=6.0, sun_heat=1.0, direct_photoactivation=0.5,
+                                        epoch=2000),
+                            settings=dict(light_mode=1)),
+    # constant sun (no day cycle): graph replay with three ping-pong passes per
+    # step, and periodic full sweeps every third step
+    "light_graph_rows": dict(cfg=ra.WorldConfig(128, 64), seed=17, inputs=dict(rock=0.1, vent_rows=2),
+                             params=dict(uv_strength=3.0, uv_depth=4.0, day_period=0.0, direct_photoactivation=0.5),
+                             settings=dict(light_mode=1, light_rows_per_step=3)),
+    "light_graph_sweeps": dict(cfg=ra.WorldConfig(64, 32), seed=19, inputs=dict(rock=0.15, rubble=True),
+                               params=dict(uv_strength=3.0, uv_depth=4.0, day_period=0.0, direct_photoactivation=0.5),
+                               settings=dict(light_mode=1, light_rows_per_step=0, light_interval=3)),
 }
 
 
@@ -63,7 +78,7 @@ def compare(gw, ow, tag):
     nchem = int(np.count_nonzero(a != b))
     report.append(f"chem words differing: {nchem}")
     ok &= nchem == 0
-    for which in FIELDS + (ra.SOLID_MASK, ra.SLOPE):
+    for which in FIELDS + (ra.SOLID_MASK, ra.SLOPE, ra.LIGHT):
         a, b = gw.download(which), ow.download(which)
         nd = bit_diff(a, b)
         if nd:

@@ -56,6 +56,12 @@ BYTES_PER_UNIT = {
     # read, chem_next 16 read + 16 written by the atomics
     "transport_scatter": (60, "G"),
     "ledger": (16, "G"),
+    # light (per grid cell; a light cell covers 4): gamma 16 + chem 64 read, both
+    # bands written (+ the rows above, 8, for a propagation pass) per light cell
+    "light_transmit": (22, "G"),
+    "light_propagate": (24, "G"),
+    # the sequential sweep reads and writes both bands once
+    "light_sweep": (4, "G"),
     # tolerance-mode solver, level 0 (S = bytes of the preconditioner's storage)
     "pcg_init": (33, "F"),
     "pcg_final": (9, "F"),
@@ -214,6 +220,43 @@ def fp64_error(cfg, inputs, args):
         e["true_rel_residual"] = rel
         out[mode] = e
         w.close()
+    return out
+
+
+def light_measure(cfg, inputs, args):
+    """The light step on the same 2048^2 world (light_mode 1: the reference's
+    light schedule, one propagation pass per step, constant sun): HIP-event
+    times of the propagation pass and of the full sweep with its pre-pass,
+    and the propagation pass against the HBM roofline."""
+    w = build_world(cfg, inputs, args, "reference")
+    p = w.params
+    p.uv_strength = 3.0
+    p.uv_depth = 4.0
+    p.day_period = 0.0
+    w.params = p
+    s = w.settings
+    s.light_mode = 1
+    s.light_rows_per_step = 1
+    w.settings = s
+    w.run(4)
+    w.set_timing(True)
+    w.reset_kernel_times()
+    w.run(args.profile_steps)
+    for _ in range(4):
+        w.invalidate_light()
+        w.encode_steps(1)
+    times = w.kernel_times()
+    w.set_timing(False)
+    w.close()
+    out = {}
+    for k in ("light_propagate", "light_transmit", "light_sweep"):
+        tot, n = times.get(k, (0.0, 0))
+        if not n:
+            continue
+        us = tot / n * 1000.0
+        b = kernel_bytes(k, cfg, [], "reference")
+        out[k] = {"avg_launch_us": us, "launches": n, "algorithmic_bytes_per_launch": b,
+                  "achieved_GB_per_s": b / (us * 1e-6) / 1e9, "frac_of_8TBps": b / (us * 1e-6) / 8e12}
     return out
 
 
@@ -429,6 +472,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64-oracle error sample")
+    ap.add_argument("--no-light", action="store_true", help="skip the light-step kernel timing")
     ap.add_argument("--config", default="world", choices=["world", "batched", "serving"],
                     help="world: the 2048^2 headline; batched: many independent worlds; serving: Poisson scene arrivals")
     ap.add_argument("--worlds", type=int, default=64)
@@ -508,6 +552,7 @@ def main():
                "GB_per_s": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 1e9,
                "frac_of_8TBps": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 8e12} for m in results}
 
+    light = None if args.no_light else light_measure(cfg, inputs, args)
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, inputs, args)
@@ -544,6 +589,7 @@ def main():
         "roofline": roofline,
         "roofline_transport_scatter": roofline_scatter,
         "hbm_step_model": hbm,
+        "light": light,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,
     }

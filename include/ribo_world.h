@@ -300,6 +300,13 @@ RIBO_API size_t ribo_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which);
  * multigrid level table and the stream.  All state starts at zero; params =
  * SimParams::default() with the given seed; settings = defaults above. */
 RIBO_API int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out);
+/* Device bytes a world of this configuration holds with these settings
+ * (NULL: the defaults); the exact coarse solve of the tolerance mode is
+ * counted at its largest.  The reference sizes its buffers in World::new
+ * (src/world/mod.rs:416-674); the serving form admits scenes by this figure. */
+RIBO_API size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings* settings);
+/* Device bytes this world holds now. */
+RIBO_API int ribo_world_device_bytes(const RiboWorld* w, size_t* out);
 RIBO_API int ribo_world_destroy(RiboWorld* w);
 
 /* World.params / World.settings (pub fields, src/world/mod.rs:205-207). */

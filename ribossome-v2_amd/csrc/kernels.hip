@@ -407,8 +407,12 @@ __global__ __launch_bounds__(256) void k_light_transmit(Geo g, RiboSimParams P, 
 // compute_uv_light (light.wgsl:129-160): the sequential top-to-bottom sweep.
 // The bands never mix, so each band is one workgroup; the finished row above
 // lives in LDS (two row buffers, one barrier per row: a row's readers finish
-// before the barrier that precedes its buffer's reuse two rows later).
-__global__ __launch_bounds__(1024) void k_light_sweep(Geo g, RiboSimParams P, float* light) {
+// before the barrier that precedes its buffer's reuse two rows later).  Each
+// thread owns columns tid + 1024 c (c < CPT) and loads the next row's
+// transmissions while it works on the current row, so a row costs LDS reads
+// and a barrier rather than an HBM round trip.  Rows below the current one
+// still hold the pre-pass values when they are prefetched.
+template <int CPT> __global__ __launch_bounds__(1024) void k_light_sweep(Geo g, RiboSimParams P, float* light) {
     extern __shared__ float rows[];
     const uint32_t LS = g.Lsz, LC = LS * LS, band = blockIdx.x, tid = threadIdx.x;
     float* L = light + band * LC;
@@ -417,24 +421,42 @@ __global__ __launch_bounds__(1024) void k_light_sweep(Geo g, RiboSimParams P, fl
     float* prev = rows;
     float* cur = rows + LS;
     const uint32_t top = LS - 1u;
-    for (uint32_t x = tid; x < LS; x += 1024u) {
-        const float v = L[top * LS + x] * (rw * P.sun_now);
-        L[top * LS + x] = v;
-        prev[x] = v;
+    float tr[CPT];
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+        const uint32_t x = tid + 1024u * (uint32_t)c;
+        if (x < LS) {
+            const float v = L[top * LS + x] * (rw * P.sun_now);
+            L[top * LS + x] = v;
+            prev[x] = v;
+        }
+        tr[c] = (x < LS && top >= 1u) ? L[(top - 1u) * LS + x] : 0.0f;
     }
     __syncthreads();
     for (uint32_t t = 1; t < LS; ++t) {
         const uint32_t y = LS - 1u - t;
-        for (uint32_t x = tid; x < LS; x += 1024u) {
-            const float above = light_above_row(prev, LS, x, s);
-            const float v = L[y * LS + x] * (above * rw);
-            L[y * LS + x] = v;
-            cur[x] = v;
+        float tn[CPT];
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const uint32_t x = tid + 1024u * (uint32_t)c;
+            tn[c] = (x < LS && y >= 1u) ? L[(y - 1u) * LS + x] : 0.0f;
+        }
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) {
+            const uint32_t x = tid + 1024u * (uint32_t)c;
+            if (x < LS) {
+                const float above = light_above_row(prev, LS, x, s);
+                const float v = tr[c] * (above * rw);
+                L[y * LS + x] = v;
+                cur[x] = v;
+            }
         }
         __syncthreads();
         float* tmp = prev;
         prev = cur;
         cur = tmp;
+#pragma unroll
+        for (int c = 0; c < CPT; ++c) tr[c] = tn[c];
     }
 }
 // light_propagate (light.wgsl:162-176): every light cell at once takes the
@@ -2665,7 +2687,13 @@ void launch_light_transmit(hipStream_t s, const Geo& g, const RiboSimParams& P, 
     k_light_transmit<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, chem, light);
 }
 void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, float* light) {
-    k_light_sweep<<<2, 1024, 2 * (size_t)g.Lsz * sizeof(float), s>>>(g, P, light);
+    const size_t lds = 2 * (size_t)g.Lsz * sizeof(float);  // grid <= 32768: Lsz <= 16384, 128 KB
+    const uint32_t cpt = (g.Lsz + 1023u) / 1024u;
+    if (cpt <= 1) k_light_sweep<1><<<2, 1024, lds, s>>>(g, P, light);
+    else if (cpt <= 2) k_light_sweep<2><<<2, 1024, lds, s>>>(g, P, light);
+    else if (cpt <= 4) k_light_sweep<4><<<2, 1024, lds, s>>>(g, P, light);
+    else if (cpt <= 8) k_light_sweep<8><<<2, 1024, lds, s>>>(g, P, light);
+    else k_light_sweep<16><<<2, 1024, lds, s>>>(g, P, light);
 }
 void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
                             float* lout) {

@@ -1,0 +1,293 @@
+// server.cpp - the serving form: scenes admitted by a device-memory budget,
+// each world resident on its own HIP stream, one scheduler thread feeding
+// every admitted world with chunks of up to 64 steps as its stream drains.
+//
+// Host code only: it drives worlds through the C ABI of world.hip, so a
+// scene's work is exactly what ribo_world_encode_steps records for it alone.
+
+#include <stdint.h>
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ribo_server.h"
+
+namespace {
+
+struct Scene {
+    uint64_t ticket = 0;
+    RiboScene req{};
+    std::vector<std::vector<uint8_t>> inputs;  // owned copies of the request's buffers
+    uint64_t footprint = 0;
+    RiboWorld* world = nullptr;
+    bool charged = false;  // footprint counted in RiboServer::used
+    uint32_t left = 0;
+    RiboSceneStatus st{};
+};
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace
+
+struct RiboServer {
+    uint64_t budget = 0;
+    uint32_t max_running = 0;
+    uint64_t used = 0;
+    uint32_t admitted = 0;  // scenes charged against the budget and not yet finished
+    std::mutex mu;
+    std::condition_variable cv;
+    std::deque<Scene*> queue;
+    std::unordered_map<uint64_t, Scene*> scenes;
+    uint64_t next_ticket = 1;
+    bool stop = false;
+    std::thread thread;
+    std::chrono::steady_clock::time_point t0;
+};
+
+namespace {
+
+// Creates and loads an admitted scene's world (scheduler thread, no lock).
+int start_scene(Scene* sc) {
+    RiboWorld* w = nullptr;
+    int rc = ribo_world_create(&sc->req.cfg, sc->req.seed, &w);
+    if (rc) return rc;
+    if ((rc = ribo_world_set_params(w, &sc->req.params)) || (rc = ribo_world_set_settings(w, &sc->req.settings))) {
+        ribo_world_destroy(w);
+        return rc;
+    }
+    // gamma first: the derived terrain (wall mask, slope, multigrid masks)
+    // follows it
+    std::vector<int> order{RIBO_BUF_GAMMA};
+    for (int b = 0; b < RIBO_BUF_COUNT; ++b)
+        if (b != RIBO_BUF_GAMMA) order.push_back(b);
+    for (int b : order) {
+        if (sc->inputs[b].empty()) continue;
+        if ((rc = ribo_world_upload(w, (RiboBuffer)b, sc->inputs[b].data(), sc->inputs[b].size()))) {
+            ribo_world_destroy(w);
+            return rc;
+        }
+    }
+    sc->inputs.clear();
+    sc->inputs.shrink_to_fit();
+    sc->world = w;
+    return RIBO_OK;
+}
+
+void scheduler(RiboServer* s) {
+    std::vector<Scene*> running, starting;
+    for (;;) {
+        {
+            std::unique_lock<std::mutex> lk(s->mu);
+            if (s->stop) break;
+            // first in, first admitted, while the budget and the slot cap allow
+            while (!s->queue.empty()) {
+                Scene* sc = s->queue.front();
+                if (s->used + sc->footprint > s->budget) break;
+                if (s->max_running && s->admitted >= s->max_running) break;
+                s->queue.pop_front();
+                s->used += sc->footprint;
+                sc->charged = true;
+                ++s->admitted;
+                sc->st.state = RIBO_SCENE_RUNNING;  // from here on only this thread touches its world
+                starting.push_back(sc);
+            }
+            if (starting.empty() && running.empty()) {
+                s->cv.wait_for(lk, std::chrono::milliseconds(2));
+                continue;
+            }
+        }
+        for (Scene* sc : starting) {
+            const int rc = start_scene(sc);
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (rc) {
+                sc->st.state = RIBO_SCENE_FAILED;
+                sc->st.error = rc;
+                s->used -= sc->footprint;
+                sc->charged = false;
+                --s->admitted;
+            } else {
+                sc->st.t_admit = seconds_since(s->t0);
+                running.push_back(sc);
+            }
+        }
+        starting.clear();
+        bool fed = false;
+        for (size_t i = 0; i < running.size();) {
+            Scene* sc = running[i];
+            const int idle = ribo_world_idle(sc->world);
+            int rc = idle < 0 ? idle : RIBO_OK;
+            if (idle == 1 && sc->left > 0) {
+                const int n = ribo_world_encode_steps(sc->world, std::min(sc->left, RIBO_MAX_STEPS_PER_CALL));
+                if (n > 0) {
+                    sc->left -= (uint32_t)n;
+                    std::lock_guard<std::mutex> lk(s->mu);
+                    sc->st.steps_done += (uint32_t)n;
+                    fed = true;
+                    ++i;
+                    continue;
+                }
+                rc = n < 0 ? n : RIBO_ERR_ARG;
+            }
+            if (idle == 0 && rc == RIBO_OK) {
+                ++i;
+                continue;
+            }
+            // finished (every step done on the device) or failed
+            std::lock_guard<std::mutex> lk(s->mu);
+            if (rc) {
+                sc->st.state = RIBO_SCENE_FAILED;
+                sc->st.error = rc;
+            } else {
+                sc->st.state = RIBO_SCENE_DONE;
+                sc->st.t_done = seconds_since(s->t0);
+            }
+            --s->admitted;  // its memory stays charged until release
+            running.erase(running.begin() + (ptrdiff_t)i);
+            fed = true;
+        }
+        if (!fed) std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, RiboServer** out) {
+    if (!out || hbm_budget == 0) return RIBO_ERR_ARG;
+    RiboServer* s = new RiboServer();
+    s->budget = hbm_budget;
+    s->max_running = max_running;
+    s->t0 = std::chrono::steady_clock::now();
+    s->thread = std::thread(scheduler, s);
+    *out = s;
+    return RIBO_OK;
+}
+
+int ribo_server_destroy(RiboServer* s) {
+    if (!s) return RIBO_ERR_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv.notify_all();
+    s->thread.join();
+    for (auto& kv : s->scenes) {
+        if (kv.second->world) ribo_world_destroy(kv.second->world);
+        delete kv.second;
+    }
+    delete s;
+    return RIBO_OK;
+}
+
+int ribo_server_submit(RiboServer* s, const RiboScene* scene, uint64_t* ticket) {
+    if (!s || !scene || !ticket) return RIBO_ERR_ARG;
+    Scene* sc = new Scene();
+    sc->req = *scene;
+    sc->inputs.resize(RIBO_BUF_COUNT);
+    for (int b = 0; b < RIBO_BUF_COUNT; ++b) {
+        sc->req.input[b] = nullptr;
+        if (!scene->input[b]) continue;
+        if (scene->input_bytes[b] != ribo_buffer_bytes(&scene->cfg, (RiboBuffer)b)) {
+            delete sc;
+            return RIBO_ERR_ARG;
+        }
+        const uint8_t* p = (const uint8_t*)scene->input[b];
+        sc->inputs[b].assign(p, p + scene->input_bytes[b]);
+    }
+    sc->footprint = ribo_world_footprint(&scene->cfg, &scene->settings);
+    sc->left = scene->steps;
+    if (sc->footprint == 0) {
+        delete sc;
+        return RIBO_ERR_ARG;
+    }
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (sc->footprint > s->budget) {
+            delete sc;
+            return RIBO_ERR_OOM;
+        }
+        sc->ticket = s->next_ticket++;
+        sc->st.state = RIBO_SCENE_QUEUED;
+        sc->st.t_submit = seconds_since(s->t0);
+        sc->st.footprint = sc->footprint;
+        s->scenes[sc->ticket] = sc;
+        s->queue.push_back(sc);
+        *ticket = sc->ticket;
+    }
+    s->cv.notify_all();
+    return RIBO_OK;
+}
+
+int ribo_server_status(RiboServer* s, uint64_t ticket, RiboSceneStatus* out) {
+    if (!s || !out) return RIBO_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->scenes.find(ticket);
+    if (it == s->scenes.end()) return RIBO_ERR_ARG;
+    *out = it->second->st;
+    return RIBO_OK;
+}
+
+int ribo_server_download(RiboServer* s, uint64_t ticket, RiboBuffer which, void* dst, size_t bytes) {
+    if (!s || !dst) return RIBO_ERR_ARG;
+    RiboWorld* w = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->scenes.find(ticket);
+        if (it == s->scenes.end() || it->second->st.state != RIBO_SCENE_DONE) return RIBO_ERR_ARG;
+        w = it->second->world;
+    }
+    // a finished world is no longer touched by the scheduler
+    return ribo_world_download(w, which, dst, bytes);
+}
+
+int ribo_server_release(RiboServer* s, uint64_t ticket) {
+    if (!s) return RIBO_ERR_ARG;
+    Scene* sc = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        auto it = s->scenes.find(ticket);
+        if (it == s->scenes.end()) return RIBO_ERR_ARG;
+        sc = it->second;
+        if (sc->st.state == RIBO_SCENE_RUNNING) return RIBO_ERR_ARG;
+        if (sc->st.state == RIBO_SCENE_QUEUED) {
+            auto q = std::find(s->queue.begin(), s->queue.end(), sc);
+            if (q != s->queue.end()) s->queue.erase(q);
+        }
+        if (sc->charged) s->used -= sc->footprint;
+        s->scenes.erase(it);
+    }
+    if (sc->world) ribo_world_destroy(sc->world);
+    delete sc;
+    s->cv.notify_all();
+    return RIBO_OK;
+}
+
+int ribo_server_stats(RiboServer* s, uint32_t* queued, uint32_t* running, uint32_t* finished, uint64_t* used) {
+    if (!s) return RIBO_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint32_t q = 0, r = 0, f = 0;
+    for (auto& kv : s->scenes) {
+        const int st = kv.second->st.state;
+        if (st == RIBO_SCENE_QUEUED) ++q;
+        else if (st == RIBO_SCENE_RUNNING) ++r;
+        else ++f;
+    }
+    if (queued) *queued = q;
+    if (running) *running = r;
+    if (finished) *finished = f;
+    if (used) *used = s->used;
+    return RIBO_OK;
+}
+
+}  // extern "C"

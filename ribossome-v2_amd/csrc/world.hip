@@ -160,7 +160,10 @@ struct RiboWorld {
     uint32_t missed_seen = 0;
     uint32_t solves_seen = 0;
     LedgerDev* ledger = nullptr;
-    uint32_t* scratch_u32 = nullptr;  // F words for downloads of derived masks
+    uint32_t* scratch_u32 = nullptr;
+    // device bytes: allocated at creation (multigrid arrays aside), by the
+    // level table, and by the exact coarse solve's inverse
+    size_t base_bytes = 0, mg_bytes = 0, cinv_bytes = 0;  // F words for downloads of derived masks
     // graph replay: the epoch base lives on the device; one executable graph
     // per (schedule phase, step count, ping-pong parities)
     uint32_t* epoch_dev = nullptr;
@@ -189,8 +192,18 @@ struct RiboWorld {
 
 namespace {
 
+// Bytes of device memory allocated while an AllocCount is alive on this
+// thread (the world's footprint, by part).
+thread_local size_t* g_alloc_acc = nullptr;
+struct AllocCount {
+    size_t* saved;
+    explicit AllocCount(size_t* acc) : saved(g_alloc_acc) { *acc = 0; g_alloc_acc = acc; }
+    ~AllocCount() { g_alloc_acc = saved; }
+};
+
 template <class T> int dalloc(T** p, size_t count) {
     if (count == 0) count = 1;
+    if (g_alloc_acc) *g_alloc_acc += count * sizeof(T);
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) return fail(RIBO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
     e = hipMemset(*p, 0, count * sizeof(T));
@@ -271,6 +284,7 @@ void clear_graphs(RiboWorld* w) {
 }
 
 int build_mg(RiboWorld* w) {
+    AllocCount count(&w->mg_bytes);
     clear_graphs(w);
     dfree(w->mg_p);
     dfree(w->mg_pp);
@@ -363,6 +377,7 @@ int build_coarse_inverse(RiboWorld* w) {
     for (uint32_t i = 0; i < n * n; ++i)
         if (fl[i]) { idx[i] = (int32_t)cell.size(); cell.push_back((int32_t)i); }
     const size_t m = cell.size();
+    AllocCount count(&w->cinv_bytes);
     dfree(w->cinv);
     dfree(w->ccell);
     w->cm = 0;
@@ -679,6 +694,33 @@ size_t ribo_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which) {
     }
 }
 
+size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings* st) {
+    if (!cfg || cfg->grid_size == 0 || cfg->fluid_size == 0) return 0;
+    auto a = [](size_t count, size_t elem) { return std::max<size_t>(count, 1) * elem; };  // dalloc's sizes
+    const size_t G = (size_t)cfg->grid_size * cfg->grid_size, F = (size_t)cfg->fluid_size * cfg->fluid_size;
+    const size_t Ls = cfg->grid_size / LIGHT_DIV, nb = (cfg->fluid_size + 15) / 16;
+    size_t b = 2 * a(4 * G, 4) + a(G, 4) + 2 * a(2 * Ls * Ls, 4) + a(G, 8) + a(G, 4);  // chem x2, gamma, light x2, slope, agg
+    b += 2 * a(F, 8) + 3 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x2, div, temp/redox x2, eaten, forces, heat
+    b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
+    b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 7 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
+    b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
+    const uint32_t min_size = st ? std::max(st->mg_min_size, 1u) : 8u;
+    const std::vector<RiboMgLevel> lv = make_levels(cfg->fluid_size, min_size);
+    const size_t total = lv.back().off_c, coarse = total - F;
+    b += 2 * a(coarse, 4) + a(total, 4) + a(total, 1) + 2 * a(coarse, 2) + a(total, 2);
+    if (st && st->multigrid && st->mg_tol > 0.0f) {  // the exact coarse solve: at most every cell of the coarsest level
+        const size_t m = (size_t)lv.back().n * lv.back().n;
+        if (m <= 1024) b += a(m * m, 8) + a(m, 4);
+    }
+    return b;
+}
+
+int ribo_world_device_bytes(const RiboWorld* w, size_t* out) {
+    if (!w || !out) return fail(RIBO_ERR_ARG, "null argument");
+    *out = w->base_bytes + w->mg_bytes + w->cinv_bytes;
+    return RIBO_OK;
+}
+
 int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out) {
     if (!cfg || !out) return fail(RIBO_ERR_ARG, "null argument");
     *out = nullptr;
@@ -714,6 +756,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         delete w;
         return fail(RIBO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
+    AllocCount count(&w->base_bytes);
     if ((rc = dalloc(&w->chem[0], 4 * G)) || (rc = dalloc(&w->chem[1], 4 * G)) || (rc = dalloc(&w->gamma, G)) ||
         (rc = dalloc(&w->light[0], 2 * (size_t)w->Lsz * w->Lsz)) ||
         (rc = dalloc(&w->light[1], 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(&w->slope, G)) || (rc = dalloc(&w->agg_e, G)) ||

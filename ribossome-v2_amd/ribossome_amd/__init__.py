@@ -293,6 +293,16 @@ def lib() -> ctypes.CDLL:
         "ribo_gen_velocity": (I, [P, ctypes.c_uint64, U32, ctypes.c_float, P, P]),
         "ribo_gen_chem": (I, [P, ctypes.c_uint64, ctypes.c_float, P, P]),
         "ribo_gen_vent_heat": (I, [P, U32, ctypes.c_float, P]),
+        "ribo_world_footprint": (SZ, [P, P]),
+        "ribo_world_device_bytes": (I, [P, ctypes.POINTER(SZ)]),
+        "ribo_server_create": (I, [ctypes.c_uint64, U32, ctypes.POINTER(P)]),
+        "ribo_server_destroy": (I, [P]),
+        "ribo_server_submit": (I, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
+        "ribo_server_status": (I, [P, ctypes.c_uint64, P]),
+        "ribo_server_download": (I, [P, ctypes.c_uint64, I, P, SZ]),
+        "ribo_server_release": (I, [P, ctypes.c_uint64]),
+        "ribo_server_stats": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
+                                  ctypes.POINTER(ctypes.c_uint64)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -445,6 +455,12 @@ class World:
         """World::invalidate_light: the next step runs the full light sweep."""
         _check(lib().ribo_world_invalidate_light(self._h), "invalidate_light")
 
+    def device_bytes(self) -> int:
+        """Device bytes this world holds now."""
+        n = ctypes.c_size_t()
+        _check(lib().ribo_world_device_bytes(self._h, ctypes.byref(n)), "device_bytes")
+        return int(n.value)
+
     # -- readback helpers named like the reference's test API --
     def read_cells_blocking(self) -> np.ndarray:
         return self.download(CHEM)
@@ -535,3 +551,83 @@ def device_available() -> bool:
         return False
     except Exception:
         return False
+
+
+def world_footprint(cfg: WorldConfig, settings: "WorldSettings | None" = None) -> int:
+    """Device bytes a world of this configuration holds (ribo_world_footprint)."""
+    c = cfg.c()
+    return int(lib().ribo_world_footprint(ctypes.byref(c), None if settings is None else ctypes.byref(settings)))
+
+
+class _CScene(ctypes.Structure):
+    _fields_ = [("cfg", _CWorldConfig), ("seed", ctypes.c_uint32), ("steps", ctypes.c_uint32),
+                ("params", SimParams), ("settings", WorldSettings),
+                ("input", ctypes.c_void_p * 14), ("input_bytes", ctypes.c_size_t * 14)]
+
+
+class SceneStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("error", ctypes.c_int32), ("steps_done", ctypes.c_uint32),
+                ("_pad", ctypes.c_uint32), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
+                ("t_done", ctypes.c_double), ("footprint", ctypes.c_uint64)]
+
+
+SCENE_QUEUED, SCENE_RUNNING, SCENE_DONE, SCENE_FAILED = range(4)
+
+
+class Server:
+    """The serving form (ribo_server_*): scenes admitted by an HBM budget,
+    resident on their own streams, stepped by the library's scheduler
+    thread."""
+
+    def __init__(self, hbm_budget: int, max_running: int = 0):
+        self._h = ctypes.c_void_p()
+        _check(lib().ribo_server_create(int(hbm_budget), int(max_running), ctypes.byref(self._h)), "server_create")
+
+    def close(self):
+        if self._h:
+            lib().ribo_server_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def submit(self, cfg: WorldConfig, seed: int, steps: int, params: SimParams, settings: WorldSettings,
+               inputs: dict) -> int:
+        sc = _CScene()
+        sc.cfg = cfg.c()
+        sc.seed = seed
+        sc.steps = steps
+        sc.params = params
+        sc.settings = settings
+        keep = []
+        for which, data in inputs.items():
+            a = np.ascontiguousarray(data, dtype=_BUF_DTYPE[which])
+            keep.append(a)
+            sc.input[which] = a.ctypes.data
+            sc.input_bytes[which] = a.nbytes
+        t = ctypes.c_uint64()
+        _check(lib().ribo_server_submit(self._h, ctypes.byref(sc), ctypes.byref(t)), "server_submit")
+        return int(t.value)
+
+    def status(self, ticket: int) -> SceneStatus:
+        st = SceneStatus()
+        _check(lib().ribo_server_status(self._h, ticket, ctypes.byref(st)), "server_status")
+        return st
+
+    def download(self, ticket: int, cfg: WorldConfig, which: int) -> np.ndarray:
+        n = buffer_bytes(cfg, which)
+        a = np.empty(n // 4, dtype=_BUF_DTYPE[which])
+        _check(lib().ribo_server_download(self._h, ticket, which, _ptr(a), n), "server_download")
+        return a
+
+    def release(self, ticket: int):
+        _check(lib().ribo_server_release(self._h, ticket), "server_release")
+
+    def stats(self) -> tuple:
+        q, r, f, u = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()
+        _check(lib().ribo_server_stats(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(f), ctypes.byref(u)),
+               "server_stats")
+        return int(q.value), int(r.value), int(f.value), int(u.value)

@@ -88,3 +88,60 @@ def test_generators_deterministic_and_shaped():
     assert abs(counts.mean() - 4.0) < 0.05 and abs(counts.var() - 4.0) < 0.15
     v = ra.gen_velocity(cfg, 3, 10.0, g1).reshape(-1, 2)
     assert abs(np.sqrt((v.astype(np.float64) ** 2).sum(1)).max() - 10.0) < 1e-4
+
+
+def test_scene_struct_layout_matches_header(tmp_path):
+    """RiboScene / RiboSceneStatus as the C compiler lays them out."""
+    import subprocess
+
+    src = tmp_path / "sz.c"
+    src.write_text('#include <stdio.h>\n#include <stddef.h>\n#include "ribo_server.h"\n'
+                   'int main(void){printf("%zu %zu %zu %zu %zu %zu\\n", sizeof(RiboScene), offsetof(RiboScene, params),'
+                   ' offsetof(RiboScene, settings), offsetof(RiboScene, input), offsetof(RiboScene, input_bytes),'
+                   ' sizeof(RiboSceneStatus)); return 0;}\n')
+    exe = tmp_path / "sz"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), "-o", str(exe), str(src)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    S = ra._CScene
+    assert got == [ctypes.sizeof(S), S.params.offset, S.settings.offset, S.input.offset, S.input_bytes.offset,
+                   ctypes.sizeof(ra.SceneStatus)]
+
+
+def test_footprint_model():
+    """The admission figure grows with the world and counts the exact coarse
+    solve only in the tolerance mode."""
+    s = ra.WorldSettings.default()
+    small, big = ra.world_footprint(ra.WorldConfig(256, 128), s), ra.world_footprint(ra.WorldConfig(2048, 1024), s)
+    assert 60 * 256 * 256 < small < 120 * 256 * 256
+    assert 60 * 2048 * 2048 < big < 120 * 2048 * 2048
+    t = ra.WorldSettings.default()
+    t.mg_tol = 1e-5
+    assert ra.world_footprint(ra.WorldConfig(256, 128), t) == small + 64 * 64 * 8 + 64 * 4
+
+
+def test_server_host_logic_without_device():
+    """Submission copies and checks the inputs, refuses a scene larger than
+    the budget, and a scene whose world cannot be created ends FAILED with
+    the library's error; release frees it."""
+    if ra.device_available():
+        return
+    cfg = ra.WorldConfig(64, 32)
+    fp = ra.world_footprint(cfg, ra.WorldSettings.default())
+    srv = ra.Server(hbm_budget=3 * fp)
+    p, s = ra.SimParams.default(1), ra.WorldSettings.default()
+    with np.testing.assert_raises(ra.RiboError):
+        srv.submit(ra.WorldConfig(2048, 1024), 1, 10, p, s, {})
+    gamma = np.zeros(cfg.cells(), np.uint32)
+    t = srv.submit(cfg, 1, 10, p, s, {ra.GAMMA: gamma})
+    import time
+
+    t0 = time.time()
+    while srv.status(t).state in (ra.SCENE_QUEUED, ra.SCENE_RUNNING):
+        assert time.time() - t0 < 10
+        time.sleep(0.01)
+    st = srv.status(t)
+    assert st.state == ra.SCENE_FAILED and st.error == -3
+    assert srv.stats() == (0, 0, 1, 0)
+    srv.release(t)
+    assert srv.stats() == (0, 0, 0, 0)
+    srv.close()

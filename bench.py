@@ -401,61 +401,81 @@ def batched(args):
                        "worlds": len(worlds), "grid_cells": cells}}
 
 
-def serving(args):
-    """Scenes arriving as a Poisson process at --qps, each a world of a
-    log-uniform side in [128, 1024] grid cells stepped U(50, 2000) steps
-    with the mixed-precision tolerance-mode solver; a host loop admits
-    arrivals, keeps every active world's stream fed with chunks of up to 64
-    steps and retires finished scenes.  Latency = completion - arrival."""
+def serving_scenes(args):
+    """Scene requests of the serving configuration: sides log-uniform in
+    [128, 1024] grid cells (multiples of 16), U(50, steps_max) steps, the
+    benchmark generators, the mixed-precision tolerance-mode solver."""
     rng = np.random.default_rng(5000)
-    n = args.scenes
-    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, n))
-    sides = [int(round(np.exp(rng.uniform(np.log(128), np.log(1024))) / 16.0)) * 16 for _ in range(n)]
-    steps = [int(rng.integers(50, 2001)) for _ in range(n)]
-    inputs = []
-    for i in range(n):
-        cfg = ra.WorldConfig(sides[i], sides[i] // 2)
+    scenes = []
+    for i in range(args.scenes):
+        side = int(round(np.exp(rng.uniform(np.log(128), np.log(1024))) / 16.0)) * 16
+        steps = int(rng.integers(50, args.serve_steps_max + 1))
+        cfg = ra.WorldConfig(side, side // 2)
         gamma = ra.gen_gamma(cfg, 7000 + i, 0.15, True)
-        inputs.append((cfg, gamma, ra.gen_velocity(cfg, 7000 + i, 0.5 / 0.017, gamma), ra.gen_chem(cfg, 7000 + i, 4.0, gamma),
-                       ra.gen_vent_heat(cfg, max(1, sides[i] // 128), 5000.0 * 0.3)))
-    active, done_lat, work = {}, [], 0
-    nxt = 0
+        inp = {ra.GAMMA: gamma, ra.VELOCITY: ra.gen_velocity(cfg, 7000 + i, 0.5 / 0.017, gamma),
+               ra.CHEM: ra.gen_chem(cfg, 7000 + i, 4.0, gamma),
+               ra.HEAT_SRC: ra.gen_vent_heat(cfg, max(1, side // 128), 5000.0 * 0.3)}
+        p, s = configure(ra.SimParams.default(7000 + i), ra.WorldSettings.default(), args, "mixed")
+        scenes.append((cfg, 7000 + i, steps, p, s, inp))
+    return scenes
+
+
+def serve(scenes, budget, max_running=0, arrivals=None, threads=0):
+    """Runs the scenes through the library's server; arrivals (seconds from
+    the start) or all at once.  Returns (span, latencies, cell-steps)."""
+    srv = ra.Server(budget, max_running, threads)
+    n = len(scenes)
+    at = arrivals if arrivals is not None else np.zeros(n)
+    tickets, nxt = [], 0
     t0 = time.perf_counter()
-    while len(done_lat) < n:
+    while nxt < n or any(srv.status(t).state in (ra.SCENE_QUEUED, ra.SCENE_RUNNING) for t in tickets):
         now = time.perf_counter() - t0
-        while nxt < n and arrivals[nxt] <= now:
-            cfg, gamma, vel, chem, heat = inputs[nxt]
-            w = build_world(cfg, (gamma, vel, chem, heat), args, "mixed")
-            active[nxt] = [w, steps[nxt]]
+        while nxt < n and at[nxt] <= now:
+            cfg, seed, steps, p, s, inp = scenes[nxt]
+            tickets.append(srv.submit(cfg, seed, steps, p, s, inp))
             nxt += 1
-        busy = False
-        for i in list(active):
-            w, left = active[i]
-            if not w.idle():
-                busy = True
-                continue
-            if left == 0:
-                done_lat.append(time.perf_counter() - t0 - arrivals[i])
-                work += inputs[i][0].cells() * steps[i]
-                w.close()
-                del active[i]
-                continue
-            k = w.encode_steps(min(64, left))
-            active[i][1] = left - k
-            busy = True
-        if not busy and nxt < n:
-            time.sleep(max(0.0, min(0.002, arrivals[nxt] - (time.perf_counter() - t0))))
-        elif not busy:
-            time.sleep(0.0002)
+        time.sleep(0.0005)
     span = time.perf_counter() - t0
-    lat = np.sort(np.array(done_lat))
+    lat, work = [], 0
+    for t, (cfg, _, steps, _, _, _) in zip(tickets, scenes):
+        st = srv.status(t)
+        if st.state != ra.SCENE_DONE:
+            raise SystemExit(f"serving: scene {t} ended in state {st.state} (error {st.error})")
+        lat.append(st.t_done - st.t_submit)
+        work += cfg.cells() * steps
+        srv.release(t)
+    srv.close()
+    return span, np.array(lat), work
+
+
+def serving(args):
+    """The serving configuration: the same scene set (1) one scene at a time
+    (the plain per-scene path), (2) all submitted at once to the server
+    (continuous batching: every scene resident and stepped on its own
+    stream as soon as the budget admits it), (3) Poisson arrivals at --qps.
+    Latency = finish - submission, from the server's clock."""
+    scenes = serving_scenes(args)
+    budget = int(args.hbm_budget_gb * (1 << 30))
+    plain_span, _, work = serve(scenes, budget, max_running=1, threads=1)
+    batch_span, _, _ = serve(scenes, budget, threads=args.serve_threads)
+    rng = np.random.default_rng(5001)
+    arrivals = np.cumsum(rng.exponential(1.0 / args.qps, len(scenes)))
+    span, lat, _ = serve(scenes, budget, arrivals=arrivals, threads=args.serve_threads)
     p50, p99 = float(np.percentile(lat, 50)), float(np.percentile(lat, 99))
+    plain, batch = work / plain_span, work / batch_span
     return {"metric": "scene latency p50 (s)", "value": p50, "unit": "s", "p99": p99, "n_gpus": 1,
-            "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f32 fluid, bf16 preconditioner",
-            "data": "synthetic", "throughput_cell_steps_per_s": work / span,
-            "config": {"workload": f"{n} scenes, Poisson arrivals at {args.qps} per s, sides log-uniform in [128, 1024], "
-                                   "U(50, 2000) steps each, mixed-precision tolerance-mode solver to 1e-5",
-                       "scenes": n, "qps": args.qps, "span_s": span}}
+            "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+            "dtype": "f32 fluid, bf16 preconditioner", "data": "synthetic",
+            "throughput_cell_steps_per_s": work / span,
+            "saturated_cell_steps_per_s": batch, "plain_per_scene_cell_steps_per_s": plain,
+            "saturated_vs_plain": batch / plain,
+            "config": {"workload": f"{len(scenes)} scenes, sides log-uniform in [128, 1024], U(50, "
+                                   f"{args.serve_steps_max}) steps each, mixed-precision tolerance-mode solver to 1e-5; "
+                                   f"Poisson arrivals at {args.qps} per s; HBM budget {args.hbm_budget_gb} GiB",
+                       "scenes": len(scenes), "qps": args.qps, "span_s": span, "plain_span_s": plain_span,
+                       "saturated_span_s": batch_span,
+                       "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
+                       "scheduler_threads": args.serve_threads or "library default"}}
 
 
 def main():
@@ -479,9 +499,17 @@ def main():
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--scenes", type=int, default=24)
     ap.add_argument("--qps", type=float, default=1.0)
+    ap.add_argument("--serve-steps-max", type=int, default=2000)
+    ap.add_argument("--hbm-budget-gb", type=float, default=256.0, help="device memory the serving form may admit")
+    ap.add_argument("--serve-threads", type=int, default=0, help="scheduler threads of the server (0: its default)")
+    ap.add_argument("--hw-queues", type=int, default=0,
+                    help="GPU_MAX_HW_QUEUES for this process (0: the runtime's default; at most 32)")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel eagerly instead of replaying captured HIP graphs (for tracing tools)")
     args = ap.parse_args()
+    if args.hw_queues:
+        # read by the HIP runtime when it initialises, at the first world
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(args.hw_queues, 1), 32))
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the world step is a single-device path here; --gpus {args.gpus} "
               "is not supported (independent worlds would shard one per GPU).", file=sys.stderr)

@@ -1,7 +1,7 @@
 /*
  * ribo_server.h - the serving form of the world step: scene requests are
  * admitted by a device-memory budget, their worlds stay resident in HBM,
- * each on its own HIP stream, and one scheduler thread keeps every admitted
+ * each on its own HIP stream, and scheduler threads keep every admitted
  * world fed with chunks of up to 64 steps (ribo_world_encode_steps) as its
  * stream drains, so the work of scenes that arrive while others run
  * overlaps on the device.
@@ -12,8 +12,9 @@
  * a loop; each scene's results are bit-identical to running its world alone
  * with the same chunking (min(64, steps left) per call).
  *
- * Threading: one caller thread at a time per server; the scheduler thread is
- * internal.  A scene's buffers may be read once its state is DONE.
+ * Threading: one caller thread at a time per server; the scheduler threads
+ * are internal (each world is stepped by one of them only).  A scene's
+ * buffers may be read once its state is DONE.
  */
 #ifndef RIBO_SERVER_H
 #define RIBO_SERVER_H
@@ -55,14 +56,19 @@ typedef struct RiboSceneStatus {
 
 /* hbm_budget: device bytes the admitted scenes may hold together (each
  * scene is charged ribo_world_footprint); max_running: at most this many
- * scenes admitted at once (0 = no limit). */
-RIBO_API int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, RiboServer** out);
+ * scenes admitted at once (0 = no limit); threads: scheduler threads, each
+ * stepping the scenes admitted to it (0 = half the host's hardware
+ * threads, at most 8). */
+RIBO_API int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, uint32_t threads, RiboServer** out);
 /* Stops the scheduler and frees every scene. */
 RIBO_API int ribo_server_destroy(RiboServer* s);
 /* Queues a scene (first in, first admitted).  A scene larger than the whole
  * budget is refused with RIBO_ERR_OOM. */
 RIBO_API int ribo_server_submit(RiboServer* s, const RiboScene* scene, uint64_t* ticket);
 RIBO_API int ribo_server_status(RiboServer* s, uint64_t ticket, RiboSceneStatus* out);
+/* The library's error message of a failed scene ("" otherwise), at most
+ * cap - 1 bytes and a terminating zero. */
+RIBO_API int ribo_server_error(RiboServer* s, uint64_t ticket, char* buf, size_t cap);
 /* A finished scene's buffer (same ids and sizes as ribo_world_download). */
 RIBO_API int ribo_server_download(RiboServer* s, uint64_t ticket, RiboBuffer which, void* dst, size_t bytes);
 /* Frees a finished, failed or still queued scene and its share of the

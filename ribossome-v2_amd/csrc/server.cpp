@@ -1,6 +1,6 @@
 // server.cpp - the serving form: scenes admitted by a device-memory budget,
-// each world resident on its own HIP stream, one scheduler thread feeding
-// every admitted world with chunks of up to 64 steps as its stream drains.
+// each world resident on its own HIP stream, scheduler threads feeding every
+// admitted world with chunks of up to 64 steps as its stream drains.
 //
 // Host code only: it drives worlds through the C ABI of world.hip, so a
 // scene's work is exactly what ribo_world_encode_steps records for it alone.
@@ -31,6 +31,7 @@ struct Scene {
     bool charged = false;  // footprint counted in RiboServer::used
     uint32_t left = 0;
     RiboSceneStatus st{};
+    std::string error;  // the library's message when the scene failed
 };
 
 double seconds_since(std::chrono::steady_clock::time_point t0) {
@@ -50,7 +51,12 @@ struct RiboServer {
     std::unordered_map<uint64_t, Scene*> scenes;
     uint64_t next_ticket = 1;
     bool stop = false;
-    std::thread thread;
+    // scheduler threads: each steps the worlds admitted to it, so the host
+    // work of recording and launching steps (graph launches cost host time
+    // per node) proceeds for several worlds at once
+    std::vector<std::thread> threads;
+    std::vector<std::vector<Scene*>> inbox;  // admitted scenes not yet picked up, per thread
+    std::vector<uint32_t> load;              // scenes each thread is running or about to start
     std::chrono::steady_clock::time_point t0;
 };
 
@@ -83,24 +89,32 @@ int start_scene(Scene* sc) {
     return RIBO_OK;
 }
 
-void scheduler(RiboServer* s) {
+// First in, first admitted, while the budget and the running cap allow; an
+// admitted scene goes to the least loaded thread.  Called with the lock held.
+void admit(RiboServer* s) {
+    while (!s->queue.empty()) {
+        Scene* sc = s->queue.front();
+        if (s->used + sc->footprint > s->budget) break;
+        if (s->max_running && s->admitted >= s->max_running) break;
+        s->queue.pop_front();
+        s->used += sc->footprint;
+        sc->charged = true;
+        ++s->admitted;
+        sc->st.state = RIBO_SCENE_RUNNING;  // from here on only its thread touches its world
+        const size_t k = (size_t)(std::min_element(s->load.begin(), s->load.end()) - s->load.begin());
+        ++s->load[k];
+        s->inbox[k].push_back(sc);
+    }
+}
+
+void scheduler(RiboServer* s, size_t me) {
     std::vector<Scene*> running, starting;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             if (s->stop) break;
-            // first in, first admitted, while the budget and the slot cap allow
-            while (!s->queue.empty()) {
-                Scene* sc = s->queue.front();
-                if (s->used + sc->footprint > s->budget) break;
-                if (s->max_running && s->admitted >= s->max_running) break;
-                s->queue.pop_front();
-                s->used += sc->footprint;
-                sc->charged = true;
-                ++s->admitted;
-                sc->st.state = RIBO_SCENE_RUNNING;  // from here on only this thread touches its world
-                starting.push_back(sc);
-            }
+            admit(s);
+            starting.swap(s->inbox[me]);
             if (starting.empty() && running.empty()) {
                 s->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
@@ -112,9 +126,11 @@ void scheduler(RiboServer* s) {
             if (rc) {
                 sc->st.state = RIBO_SCENE_FAILED;
                 sc->st.error = rc;
+                sc->error = ribo_last_error();
                 s->used -= sc->footprint;
                 sc->charged = false;
                 --s->admitted;
+                --s->load[me];
             } else {
                 sc->st.t_admit = seconds_since(s->t0);
                 running.push_back(sc);
@@ -143,15 +159,20 @@ void scheduler(RiboServer* s) {
                 continue;
             }
             // finished (every step done on the device) or failed
-            std::lock_guard<std::mutex> lk(s->mu);
-            if (rc) {
-                sc->st.state = RIBO_SCENE_FAILED;
-                sc->st.error = rc;
-            } else {
-                sc->st.state = RIBO_SCENE_DONE;
-                sc->st.t_done = seconds_since(s->t0);
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (rc) {
+                    sc->st.state = RIBO_SCENE_FAILED;
+                    sc->st.error = rc;
+                    sc->error = ribo_last_error();
+                } else {
+                    sc->st.state = RIBO_SCENE_DONE;
+                    sc->st.t_done = seconds_since(s->t0);
+                }
+                --s->admitted;  // its memory stays charged until release
+                --s->load[me];
             }
-            --s->admitted;  // its memory stays charged until release
+            s->cv.notify_all();  // a running slot is free
             running.erase(running.begin() + (ptrdiff_t)i);
             fed = true;
         }
@@ -163,13 +184,17 @@ void scheduler(RiboServer* s) {
 
 extern "C" {
 
-int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, RiboServer** out) {
+int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, uint32_t threads, RiboServer** out) {
     if (!out || hbm_budget == 0) return RIBO_ERR_ARG;
+    if (threads == 0) threads = std::max(1u, std::min(8u, std::thread::hardware_concurrency() / 2));
+    threads = std::min(threads, 64u);
     RiboServer* s = new RiboServer();
     s->budget = hbm_budget;
     s->max_running = max_running;
     s->t0 = std::chrono::steady_clock::now();
-    s->thread = std::thread(scheduler, s);
+    s->inbox.resize(threads);
+    s->load.assign(threads, 0u);
+    for (uint32_t k = 0; k < threads; ++k) s->threads.emplace_back(scheduler, s, (size_t)k);
     *out = s;
     return RIBO_OK;
 }
@@ -181,7 +206,7 @@ int ribo_server_destroy(RiboServer* s) {
         s->stop = true;
     }
     s->cv.notify_all();
-    s->thread.join();
+    for (auto& t : s->threads) t.join();
     for (auto& kv : s->scenes) {
         if (kv.second->world) ribo_world_destroy(kv.second->world);
         delete kv.second;
@@ -235,6 +260,18 @@ int ribo_server_status(RiboServer* s, uint64_t ticket, RiboSceneStatus* out) {
     auto it = s->scenes.find(ticket);
     if (it == s->scenes.end()) return RIBO_ERR_ARG;
     *out = it->second->st;
+    return RIBO_OK;
+}
+
+int ribo_server_error(RiboServer* s, uint64_t ticket, char* buf, size_t cap) {
+    if (!s || !buf || cap == 0) return RIBO_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->scenes.find(ticket);
+    if (it == s->scenes.end()) return RIBO_ERR_ARG;
+    const std::string& e = it->second->error;
+    const size_t n = std::min(cap - 1, e.size());
+    memcpy(buf, e.data(), n);
+    buf[n] = 0;
     return RIBO_OK;
 }
 

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -201,16 +202,29 @@ struct AllocCount {
     ~AllocCount() { g_alloc_acc = saved; }
 };
 
-template <class T> int dalloc(T** p, size_t count) {
+// Allocation and release of device memory, and graph capture, are serialised
+// across the threads of the process: a world created or destroyed on one
+// thread while another thread captures a graph would otherwise touch the
+// capturing stream through the runtime's implicit synchronisation.
+std::recursive_mutex& device_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
+// Zeroed device allocation; the clear is ordered on the world's stream (no
+// legacy-stream operation).
+template <class T> int dalloc(hipStream_t s, T** p, size_t count) {
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     if (count == 0) count = 1;
     if (g_alloc_acc) *g_alloc_acc += count * sizeof(T);
     hipError_t e = hipMalloc((void**)p, count * sizeof(T));
     if (e != hipSuccess) return fail(RIBO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
-    e = hipMemset(*p, 0, count * sizeof(T));
+    e = hipMemsetAsync(*p, 0, count * sizeof(T), s);
     if (e != hipSuccess) return fail(RIBO_ERR_HIP, std::string("hipMemset: ") + hipGetErrorString(e));
     return RIBO_OK;
 }
 template <class T> void dfree(T*& p) {
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     if (p) (void)hipFree((void*)p);
     p = nullptr;
 }
@@ -295,13 +309,13 @@ int build_mg(RiboWorld* w) {
     size_t fine = (size_t)w->F * w->F;
     size_t coarse = total - fine;
     int rc;
-    if ((rc = dalloc(&w->mg_p, coarse)) || (rc = dalloc(&w->mg_pp, total)) || (rc = dalloc(&w->mg_rhs, coarse)) ||
-        (rc = dalloc(&w->mg_fluid, total)))
+    if ((rc = dalloc(w->stream, &w->mg_p, coarse)) || (rc = dalloc(w->stream, &w->mg_pp, total)) || (rc = dalloc(w->stream, &w->mg_rhs, coarse)) ||
+        (rc = dalloc(w->stream, &w->mg_fluid, total)))
         return rc;
     dfree(w->mg_p16);
     dfree(w->mg_pp16);
     dfree(w->mg_rhs16);
-    if ((rc = dalloc(&w->mg_p16, coarse)) || (rc = dalloc(&w->mg_pp16, total)) || (rc = dalloc(&w->mg_rhs16, coarse))) return rc;
+    if ((rc = dalloc(w->stream, &w->mg_p16, coarse)) || (rc = dalloc(w->stream, &w->mg_pp16, total)) || (rc = dalloc(w->stream, &w->mg_rhs16, coarse))) return rc;
     w->mg.clear();
     w->mgz.clear();
     w->mgh.clear();
@@ -412,13 +426,14 @@ int build_coarse_inverse(RiboWorld* w) {
         }
     }
     int rc;
-    if ((rc = dalloc(&w->cinv, m * m)) || (rc = dalloc(&w->ccell, m))) return rc;
+    if ((rc = dalloc(w->stream, &w->cinv, m * m)) || (rc = dalloc(w->stream, &w->ccell, m))) return rc;
     // by columns, so that the rows a workgroup computes read consecutive words
     std::vector<double> inv_t(m * m);
     for (size_t r = 0; r < m; ++r)
         for (size_t j = 0; j < m; ++j) inv_t[j * m + r] = inv[r * m + j];
-    HIPCHK(hipMemcpy(w->cinv, inv_t.data(), m * m * sizeof(double), hipMemcpyHostToDevice));
-    HIPCHK(hipMemcpy(w->ccell, cell.data(), m * sizeof(int32_t), hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpyAsync(w->cinv, inv_t.data(), m * m * sizeof(double), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipMemcpyAsync(w->ccell, cell.data(), m * sizeof(int32_t), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
     w->cm = (int)m;
     w->cinv_valid = true;
     return RIBO_OK;
@@ -723,6 +738,7 @@ int ribo_world_device_bytes(const RiboWorld* w, size_t* out) {
 
 int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out) {
     if (!cfg || !out) return fail(RIBO_ERR_ARG, "null argument");
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     *out = nullptr;
     if (cfg->grid_size == 0 || cfg->fluid_size == 0 || cfg->grid_size % cfg->fluid_size != 0 || cfg->grid_size < 2 ||
         cfg->world_units_per_cell == 0 || cfg->grid_size > 32768)
@@ -757,27 +773,27 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         return fail(RIBO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     AllocCount count(&w->base_bytes);
-    if ((rc = dalloc(&w->chem[0], 4 * G)) || (rc = dalloc(&w->chem[1], 4 * G)) || (rc = dalloc(&w->gamma, G)) ||
-        (rc = dalloc(&w->light[0], 2 * (size_t)w->Lsz * w->Lsz)) ||
-        (rc = dalloc(&w->light[1], 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(&w->slope, G)) || (rc = dalloc(&w->agg_e, G)) ||
-        (rc = dalloc(&w->vel_a, F)) || (rc = dalloc(&w->vel_b, F)) || (rc = dalloc(&w->p_a, F)) || (rc = dalloc(&w->p_b, F)) ||
-        (rc = dalloc(&w->div, F)) || (rc = dalloc(&w->temp[0], F)) || (rc = dalloc(&w->temp[1], F)) ||
-        (rc = dalloc(&w->redox[0], F)) || (rc = dalloc(&w->redox[1], F)) || (rc = dalloc(&w->eaten, F)) ||
-        (rc = dalloc(&w->forces, F)) || (rc = dalloc(&w->heat, F)) || (rc = dalloc(&w->net_flux, 2 * (size_t)w->F)) ||
-        (rc = dalloc(&w->cflags, F)) || (rc = dalloc(&w->normal, F)) || (rc = dalloc(&w->fill, F)) ||
-        (rc = dalloc(&w->dragg, F)) || (rc = dalloc(&w->slope_f, F)) || (rc = dalloc(&w->pcg, 1)) ||
-        (rc = dalloc(&w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(&w->pcg_r, F)) ||
-        (rc = dalloc(&w->pcg_r1, F)) || (rc = dalloc(&w->pcg_s0, F)) || (rc = dalloc(&w->pcg_s1, F)) ||
-        (rc = dalloc(&w->pcg_pd, F)) || (rc = dalloc(&w->pcg_w, F)) || (rc = dalloc(&w->pcg_z, F)) ||
-        (rc = dalloc(&w->pcg_r16, F)) || (rc = dalloc(&w->pcg_z16, F)) ||
-        (rc = dalloc(&w->ledger, 1)) || (rc = dalloc(&w->scratch_u32, F)) || (rc = dalloc(&w->epoch_dev, 1)) ||
+    if ((rc = dalloc(w->stream, &w->chem[0], 4 * G)) || (rc = dalloc(w->stream, &w->chem[1], 4 * G)) || (rc = dalloc(w->stream, &w->gamma, G)) ||
+        (rc = dalloc(w->stream, &w->light[0], 2 * (size_t)w->Lsz * w->Lsz)) ||
+        (rc = dalloc(w->stream, &w->light[1], 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(w->stream, &w->slope, G)) || (rc = dalloc(w->stream, &w->agg_e, G)) ||
+        (rc = dalloc(w->stream, &w->vel_a, F)) || (rc = dalloc(w->stream, &w->vel_b, F)) || (rc = dalloc(w->stream, &w->p_a, F)) || (rc = dalloc(w->stream, &w->p_b, F)) ||
+        (rc = dalloc(w->stream, &w->div, F)) || (rc = dalloc(w->stream, &w->temp[0], F)) || (rc = dalloc(w->stream, &w->temp[1], F)) ||
+        (rc = dalloc(w->stream, &w->redox[0], F)) || (rc = dalloc(w->stream, &w->redox[1], F)) || (rc = dalloc(w->stream, &w->eaten, F)) ||
+        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
+        (rc = dalloc(w->stream, &w->cflags, F)) || (rc = dalloc(w->stream, &w->normal, F)) || (rc = dalloc(w->stream, &w->fill, F)) ||
+        (rc = dalloc(w->stream, &w->dragg, F)) || (rc = dalloc(w->stream, &w->slope_f, F)) || (rc = dalloc(w->stream, &w->pcg, 1)) ||
+        (rc = dalloc(w->stream, &w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(w->stream, &w->pcg_r, F)) ||
+        (rc = dalloc(w->stream, &w->pcg_r1, F)) || (rc = dalloc(w->stream, &w->pcg_s0, F)) || (rc = dalloc(w->stream, &w->pcg_s1, F)) ||
+        (rc = dalloc(w->stream, &w->pcg_pd, F)) || (rc = dalloc(w->stream, &w->pcg_w, F)) || (rc = dalloc(w->stream, &w->pcg_z, F)) ||
+        (rc = dalloc(w->stream, &w->pcg_r16, F)) || (rc = dalloc(w->stream, &w->pcg_z16, F)) ||
+        (rc = dalloc(w->stream, &w->ledger, 1)) || (rc = dalloc(w->stream, &w->scratch_u32, F)) || (rc = dalloc(w->stream, &w->epoch_dev, 1)) ||
         (rc = build_mg(w))) {
         ribo_world_destroy(w);
         return rc;
     }
     LedgerDev init{};
     init.first_bad = 0xFFFFFFFFu;
-    he = hipMemcpy(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice);
+    he = hipMemcpyAsync(w->ledger, &init, sizeof(init), hipMemcpyHostToDevice, w->stream);
     PcgState pst{};
     pst.part = w->pcg_part;
     const size_t host_words = PCG_HOST_RING + PCG_HOST_RING_LEN;
@@ -785,7 +801,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         memset(w->pcg_host, 0, host_words * sizeof(uint32_t));
         if (hipHostGetDevicePointer((void**)&pst.host_out, w->pcg_host, 0) != hipSuccess) pst.host_out = nullptr;
     }
-    if (he == hipSuccess) he = hipMemcpy(w->pcg, &pst, sizeof(pst), hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpyAsync(w->pcg, &pst, sizeof(pst), hipMemcpyHostToDevice, w->stream);
     if (he != hipSuccess || (rc = rebuild_terrain(w)) != RIBO_OK || hipStreamSynchronize(w->stream) != hipSuccess) {
         ribo_world_destroy(w);
         return fail(RIBO_ERR_HIP, "world initialisation failed");
@@ -796,6 +812,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
 
 int ribo_world_destroy(RiboWorld* w) {
     if (!w) return fail(RIBO_ERR_ARG, "null world");
+    std::lock_guard<std::recursive_mutex> lk(device_mutex());
     if (w->stream) (void)hipStreamSynchronize(w->stream);
     resolve_timers(w);
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
@@ -1004,6 +1021,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
                 if (w->graphs.size() >= 32) clear_graphs(w);
                 RiboWorld::GraphEntry e{phase, n, w->pcg_slots, w->cur, w->tcur, w->lcur, 0, 0, 0, nullptr};
                 hipGraph_t graph = nullptr;
+                std::lock_guard<std::recursive_mutex> lk(device_mutex());
                 HIPCHK(hipStreamBeginCapture(w->stream, hipStreamCaptureModeThreadLocal));
                 record_steps(w, first, n, substep, w->epoch_dev);
                 launch_epoch_add(w->stream, w->epoch_dev, n);

@@ -295,12 +295,13 @@ def lib() -> ctypes.CDLL:
         "ribo_gen_vent_heat": (I, [P, U32, ctypes.c_float, P]),
         "ribo_world_footprint": (SZ, [P, P]),
         "ribo_world_device_bytes": (I, [P, ctypes.POINTER(SZ)]),
-        "ribo_server_create": (I, [ctypes.c_uint64, U32, ctypes.POINTER(P)]),
+        "ribo_server_create": (I, [ctypes.c_uint64, U32, U32, ctypes.POINTER(P)]),
         "ribo_server_destroy": (I, [P]),
         "ribo_server_submit": (I, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
         "ribo_server_status": (I, [P, ctypes.c_uint64, P]),
         "ribo_server_download": (I, [P, ctypes.c_uint64, I, P, SZ]),
         "ribo_server_release": (I, [P, ctypes.c_uint64]),
+        "ribo_server_error": (I, [P, ctypes.c_uint64, ctypes.c_char_p, SZ]),
         "ribo_server_stats": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(U32), ctypes.POINTER(U32),
                                   ctypes.POINTER(ctypes.c_uint64)]),
     }
@@ -577,11 +578,12 @@ SCENE_QUEUED, SCENE_RUNNING, SCENE_DONE, SCENE_FAILED = range(4)
 class Server:
     """The serving form (ribo_server_*): scenes admitted by an HBM budget,
     resident on their own streams, stepped by the library's scheduler
-    thread."""
+    threads."""
 
-    def __init__(self, hbm_budget: int, max_running: int = 0):
+    def __init__(self, hbm_budget: int, max_running: int = 0, threads: int = 0):
         self._h = ctypes.c_void_p()
-        _check(lib().ribo_server_create(int(hbm_budget), int(max_running), ctypes.byref(self._h)), "server_create")
+        _check(lib().ribo_server_create(int(hbm_budget), int(max_running), int(threads), ctypes.byref(self._h)),
+               "server_create")
 
     def close(self):
         if self._h:
@@ -625,6 +627,12 @@ class Server:
 
     def release(self, ticket: int):
         _check(lib().ribo_server_release(self._h, ticket), "server_release")
+
+    def error(self, ticket: int) -> str:
+        """The library's message for a failed scene."""
+        buf = ctypes.create_string_buffer(512)
+        _check(lib().ribo_server_error(self._h, ticket, buf, 512), "server_error")
+        return buf.value.decode()
 
     def stats(self) -> tuple:
         q, r, f, u = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint64()

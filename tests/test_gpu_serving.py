@@ -60,7 +60,7 @@ def run_server(srv, reqs):
     out = []
     for t, (cfg, _, _, _) in zip(tickets, reqs):
         st = srv.status(t)
-        assert st.state == ra.SCENE_DONE, (t, st.state, st.error)
+        assert st.state == ra.SCENE_DONE, (t, st.state, st.error, srv.error(t))
         out.append({k: srv.download(t, cfg, k) for k in (ra.CHEM, ra.VELOCITY, ra.PRESSURE, ra.TEMP)})
         srv.release(t)
     return out, peak
@@ -74,8 +74,8 @@ def test_scenes_identical_alone_and_in_the_mix():
     print("case serving alone vs mix", flush=True)
     reqs = [scene_request(*sc) for sc in SCENES]
     budget = 64 << 30
-    alone, peak1 = run_server(ra.Server(budget, max_running=1), reqs)
-    mixed, peakn = run_server(ra.Server(budget), reqs)
+    alone, peak1 = run_server(ra.Server(budget, max_running=1, threads=1), reqs)
+    mixed, peakn = run_server(ra.Server(budget, threads=4), reqs)
     print(f"peak running: alone {peak1}, mix {peakn}", flush=True)
     assert peak1 == 1 and peakn >= 3
     for k, (a, b) in enumerate(zip(alone, mixed)):

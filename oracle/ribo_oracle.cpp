@@ -156,9 +156,11 @@ template <class R> inline R vdot(V2<R> a, V2<R> b) { return a.x * b.x + a.y * b.
 template <class R> struct Fluid {
     std::vector<R> vel_a, vel_b, p_a, p_b, div, temp_in, temp_out, redox_in, redox_out, forces, net_flux;
     std::vector<R> mg_p, mg_rhs;
+    std::vector<R> p_old;  // tolerance mode: the previous solve's pressure
+    int hist = 0;          // solves since the history restarted (at most 2): at 2, p_a and p_old are two solves' results
     void alloc(size_t F, size_t N, size_t mg_total) {
         vel_a.assign(2 * F, 0); vel_b.assign(2 * F, 0);
-        p_a.assign(F, 0); p_b.assign(F, 0); div.assign(F, 0);
+        p_a.assign(F, 0); p_b.assign(F, 0); div.assign(F, 0); p_old.assign(F, 0);
         temp_in.assign(F, 0); temp_out.assign(F, 0); redox_in.assign(F, 0); redox_out.assign(F, 0);
         forces.assign(2 * F, 0); net_flux.assign(2 * N, 0);
         mg_p.assign(mg_total, 0); mg_rhs.assign(mg_total, 0);
@@ -1474,8 +1476,11 @@ template <class R> void transport(OracleWorld& w, const Fluid<R>& fl, const Ribo
 
 // Tolerance mode (an extension; the reference runs a fixed number of
 // V-cycles): preconditioned conjugate gradients on the eps-anchored operator
-// A p = div over fluid cells of level 0, warm-started from the previous
-// pressure, preconditioned by the symmetric V-cycle (MG::vcycle_sym), until
+// A p = div over fluid cells of level 0, started from the previous pressure
+// moved along the change between the last two solves to the point of least
+// residual (once two solves are done; the history restarts on new settings
+// and on uploads of pressure, velocity or terrain), preconditioned by the
+// symmetric V-cycle (MG::vcycle_sym), until
 // ||div - A p||2 / ||div||2 <= mg_tol or mg_max_cycles iterations.
 //
 // The recurrences put all inner products of an iteration at one point,
@@ -1522,8 +1527,33 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
     for (size_t i = 0; i < n2; ++i) {
         r[i] = w.mg_fluid[i] ? b[i] - wv[i] : R(0);
         bb += (double)b[i] * (double)b[i];
-        rr += (double)r[i] * (double)r[i];
     }
+    // The start: once the pressure and p_old are the last two solves'
+    // results, move along their change d = p - p_old to the point of least
+    // residual, x + theta d with theta = (r, A d) / (A d, A d) (theta = 1 is
+    // the linear extrapolation in time); p_old takes the old pressure.
+    if (fl.hist >= 2) {
+        std::vector<R> d(n2), q(n2);
+        for (size_t i = 0; i < n2; ++i) d[i] = x[i] - fl.p_old[i];
+        mg.apply(d.data(), q.data());
+        double rq = 0.0, qq = 0.0;
+        for (size_t i = 0; i < n2; ++i) {
+            if (!w.mg_fluid[i]) continue;
+            rq += (double)r[i] * (double)q[i];
+            qq += (double)q[i] * (double)q[i];
+        }
+        const R th = R(qq > 0.0 ? rq / qq : 0.0);
+        for (size_t i = 0; i < n2; ++i) {
+            fl.p_old[i] = x[i];
+            if (th != R(0) && w.mg_fluid[i]) {
+                x[i] = x[i] + th * d[i];
+                r[i] = r[i] - th * q[i];
+            }
+        }
+    } else {
+        for (size_t i = 0; i < n2; ++i) fl.p_old[i] = x[i];
+    }
+    for (size_t i = 0; i < n2; ++i) rr += (double)r[i] * (double)r[i];
     // u = M r, w = A u and the iteration's inner products
     double gamma = 0.0, delta = 0.0, e = 0.0, f = 0.0;
     auto precond = [&]() {
@@ -1572,6 +1602,7 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
     }
     w.last_cycles = it;
     w.last_rel_res = mg.rel_residual_of(x, b);
+    fl.hist = std::min(fl.hist + 1, 2);
 }
 
 // ---- one fluid step (src/world/mod.rs:1527-1588) ---------------------------------
@@ -1720,6 +1751,7 @@ int oracle_set_settings(OracleWorld* w, const RiboWorldSettings* s) {
     if (s->terrain_enabled) return RIBO_ERR_UNSUPPORTED;
     if (s->mg_min_size < 1 || s->light_mode > 1u) return RIBO_ERR_ARG;
     w->st = *s;
+    w->f32.hist = w->f64.hist = 0;  // new settings start the pressure history again
     uint32_t total;
     make_levels(w->F, s->mg_min_size, &total);
     if (total != w->mg_total) {
@@ -1752,6 +1784,8 @@ static void compute_slope(OracleWorld* w) {  // terrain.wgsl:104-133
 
 int oracle_upload(OracleWorld* w, int which, const void* src, size_t bytes) {
     if (bytes != buf_bytes(*w, which)) return RIBO_ERR_ARG;
+    if (which == RIBO_BUF_GAMMA || which == RIBO_BUF_VELOCITY || which == RIBO_BUF_PRESSURE)
+        w->f32.hist = w->f64.hist = 0;  // the pressure history starts again
     switch (which) {
         case RIBO_BUF_CHEM: memcpy(w->chem.data(), src, bytes); w->light_dirty = true; return 0;
         case RIBO_BUF_GAMMA: memcpy(w->gamma.data(), src, bytes); compute_slope(w); w->light_dirty = true;

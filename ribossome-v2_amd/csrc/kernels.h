@@ -69,6 +69,7 @@ template <class S> struct CgVec {
     float eps;
     const uint8_t* fl;
     float* x;        // the pressure
+    float* xold;     // the previous solve's pressure (null: no extrapolated start)
     const float* b;  // the divergence
     float* r[2];
     float* s[2];
@@ -92,8 +93,11 @@ struct PcgScal {
 // finished solve.
 struct PcgState {
     double bb, rel, true_rel;
+    double theta;       // the start's step along the last two solves' change
     PcgScal sc[2];
     uint32_t it, done, steps, iters, missed;
+    uint32_t hist;      // solves since the pressure history restarted, at most 2: at 2 the
+                        // pressure and the previous one are two solves' results
     float tol_seen;
     double* part;       // per-block partial sums (up to 5 per block of 16 x 16 cells)
     uint32_t* host_out; // host-mapped words: last iterations, misses, solves, then the
@@ -152,7 +156,9 @@ size_t mg_block_lds_bytes(const uint32_t* n, int count);
 template <class S> void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv,
                                         const int32_t* ccell, int cm, const uint32_t* done);
 // preconditioned conjugate gradients
-// r = b - A x and the solve's initial state (zeroes the other vectors).
+// r = b - A x and the solve's initial state (zeroes the other vectors); when
+// V.xold is set and the last two solves' pressures are on hand, the start
+// moves along their change to the point of least residual.
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit);
 // Pre-smoothing and restriction of level 0, fused with the vector updates of
 // the iteration (p, s, x, r; none on the first application, first = 1);

@@ -1620,26 +1620,63 @@ __device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, bool first_app
     const int i = in ? y * n + x : 0;                                                  \
     const bool f = in && V.fl[i];
 
-// r = b - A x (r[0]); r[1], s, p and w start at 0; ||b||^2 and ||r||^2.
+// A (x - xold) at fluid cell (x, y): the operator on the change between the
+// last two solves' pressures, each difference formed in binary32.
+__device__ __forceinline__ float apply_A_diff(const uint8_t* fl, const float* v, const float* o, int n, int x, int y,
+                                              float eps) {
+    const int i = y * n + x;
+    const int il = x > 0 ? i - 1 : i, ir = x + 1 < n ? i + 1 : i;
+    const int id = y > 0 ? i - n : i, iu = y + 1 < n ? i + n : i;
+    const bool gl = fl[il] != 0, gr = fl[ir] != 0, gd = fl[id] != 0, gu = fl[iu] != 0;
+    const float dl = v[il] - o[il], dr = v[ir] - o[ir], dd = v[id] - o[id], du = v[iu] - o[iu], dc = v[i] - o[i];
+    const NbSum nb = nb_sum((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, dl, dr, dd, du);
+    return nb.s - (nb.k + eps) * dc;
+}
+
+// r = b - A x (r[0]); r[1], s, w start at 0; ||b||^2 and ||r||^2.  Once the
+// pressure and xold are the last two solves' results, also q = A (x - xold)
+// (into the search direction p, which k_pcg_shift clears) and (r, q), (q, q):
+// the start x + theta (x - xold) with theta = (r, q) / (q, q) has the least
+// residual on that line (theta = 1 is the linear extrapolation in time).
 template <class S>
 __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
     PCG_XY
-    double bb = 0.0, rr = 0.0;
+    const bool extrap = V.xold != nullptr && st->hist >= 2u;
+    double bb = 0.0, rr = 0.0, rq = 0.0, qq = 0.0;
     if (in) {
-        float r = 0.0f;
+        float r = 0.0f, q = 0.0f;
         if (f) {
             r = V.b[i] - apply_A(V.fl, V.x, n, x, y, V.eps);
+            if (extrap) q = apply_A_diff(V.fl, V.x, V.xold, n, x, y, V.eps);
             bb = (double)V.b[i] * (double)V.b[i];
             rr = (double)r * (double)r;
+            rq = (double)r * (double)q;
+            qq = (double)q * (double)q;
         }
         V.r[0][i] = r;
         V.r[1][i] = 0.0f;
         V.s[0][i] = 0.0f;
         V.s[1][i] = 0.0f;
-        V.pd[i] = 0.0f;
+        V.pd[i] = q;
         V.w[i] = 0.0f;
     }
-    block_partial<2>({bb, rr}, st->part);
+    block_partial<4>({bb, rr, rq, qq}, st->part);
+}
+
+// The start on the line: x += theta (x - xold), r -= theta q in fluid cells;
+// xold takes the old x; the search direction (q until now) starts at 0.
+__global__ __launch_bounds__(256) void k_pcg_shift(uint32_t cells, const uint8_t* __restrict__ fl, float* x, float* xold,
+                                                   float* r, float* pd, const PcgState* st) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= cells) return;
+    const float th = (float)st->theta;
+    const float a = x[i];
+    if (th != 0.0f && fl[i]) {
+        x[i] = a + th * (a - xold[i]);
+        r[i] = r[i] - th * pd[i];
+    }
+    xold[i] = a;
+    pd[i] = 0.0f;
 }
 
 // Level-0 tiles of the tolerance mode: 64 x 64 cells per 1024-thread
@@ -1872,12 +1909,19 @@ __global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int 
 enum PcgFin { FIN_INIT, FIN_FINAL };
 template <int W>
 __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit, int rd, uint32_t off) {
-    double t[2];
-    sum_partials<2>(st->part + off, nb, t);
+    constexpr int N = W == FIN_INIT ? 4 : 2;
+    double t[N];
+    sum_partials<N>(st->part + off, nb, t);
     if (threadIdx.x != 0) return;
     if (W == FIN_INIT) {
         st->bb = t[0];
-        const double rel = t[0] > 0.0 ? sqrt(t[1] / t[0]) : 0.0;
+        // the start of least residual on the line through the last two
+        // pressures (theta = 0 without that history); its residual norm
+        // follows from the sums: |r - theta q|^2 = |r|^2 - (r, q)^2 / (q, q)
+        const double theta = t[3] > 0.0 ? t[2] / t[3] : 0.0;
+        st->theta = theta;
+        const double rr0 = fmax(t[1] - theta * t[2], 0.0);
+        const double rel = t[0] > 0.0 ? sqrt(rr0 / t[0]) : 0.0;
         PcgScal a{};
         a.rel = rel;
         a.done = (rel <= (double)tol || maxit == 0u) ? 1u : 0u;
@@ -1892,6 +1936,7 @@ __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, floa
         st->rel = a.rel;
         st->done = a.done;
         st->true_rel = st->bb > 0.0 ? sqrt(t[0] / st->bb) : 0.0;
+        st->hist = st->hist < 2u ? st->hist + 1u : 2u;  // solves done since the history restarted (at most 2)
         st->steps += 1u;
         st->iters += a.it;
         // a solve that stopped on its iteration slots before converging
@@ -2639,6 +2684,7 @@ inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
     k_cg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st);
     k_pcg_fin<FIN_INIT><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), tol, maxit, 0, 0u);
+    if (V.xold) k_pcg_shift<<<blocks1((size_t)V.n * V.n, 256), 256, 0, s>>>(V.n * V.n, V.fl, V.x, V.xold, V.r[0], V.pd, st);
 }
 template <class S>
 void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int stage, int rd,

@@ -10,6 +10,7 @@
 
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <stddef.h>
 #include <stdio.h>
 #include <string.h>
 
@@ -110,6 +111,7 @@ struct RiboWorld {
     float2* vel_b = nullptr;
     float* p_a = nullptr;
     float* p_b = nullptr;
+    float* p_old = nullptr;  // tolerance mode: the previous solve's pressure
     float* div = nullptr;
     float* temp[2] = {nullptr, nullptr};
     int tcur = 0;
@@ -499,6 +501,7 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
     V.eps = w->levels[0].eps;
     V.fl = w->mg_fluid;
     V.x = w->p_a;
+    V.xold = w->p_old;
     V.b = w->div;
     V.r[0] = w->pcg_r;
     V.r[1] = w->pcg_r1;
@@ -642,6 +645,14 @@ int arm_ledger(RiboWorld* w) {
     return RIBO_OK;
 }
 
+// The tolerance mode starts each solve from the extrapolation of the two
+// previous solves' pressures; a new pressure, terrain, velocity or settings
+// starts the history again (the next solve starts from the pressure itself).
+int reset_pressure_history(RiboWorld* w) {
+    HIPCHK(hipMemsetAsync((char*)w->pcg + offsetof(PcgState, hist), 0, sizeof(uint32_t), w->stream));
+    return RIBO_OK;
+}
+
 int validate_settings(const RiboWorldSettings* s) {
     if (s->terrain_enabled) return fail(RIBO_ERR_UNSUPPORTED, "terrain physics is not part of the world-step path (terrain_enabled must be 0)");
     if (s->light_mode > 1u) return fail(RIBO_ERR_ARG, "light_mode must be 0 (frozen) or 1 (the reference's light schedule)");
@@ -715,7 +726,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     const size_t G = (size_t)cfg->grid_size * cfg->grid_size, F = (size_t)cfg->fluid_size * cfg->fluid_size;
     const size_t Ls = cfg->grid_size / LIGHT_DIV, nb = (cfg->fluid_size + 15) / 16;
     size_t b = 2 * a(4 * G, 4) + a(G, 4) + 2 * a(2 * Ls * Ls, 4) + a(G, 8) + a(G, 4);  // chem x2, gamma, light x2, slope, agg
-    b += 2 * a(F, 8) + 3 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x2, div, temp/redox x2, eaten, forces, heat
+    b += 2 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x3, div, temp/redox x2, eaten, forces, heat
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
     b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 7 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
     b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
@@ -777,6 +788,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(w->stream, &w->light[0], 2 * (size_t)w->Lsz * w->Lsz)) ||
         (rc = dalloc(w->stream, &w->light[1], 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(w->stream, &w->slope, G)) || (rc = dalloc(w->stream, &w->agg_e, G)) ||
         (rc = dalloc(w->stream, &w->vel_a, F)) || (rc = dalloc(w->stream, &w->vel_b, F)) || (rc = dalloc(w->stream, &w->p_a, F)) || (rc = dalloc(w->stream, &w->p_b, F)) ||
+        (rc = dalloc(w->stream, &w->p_old, F)) ||
         (rc = dalloc(w->stream, &w->div, F)) || (rc = dalloc(w->stream, &w->temp[0], F)) || (rc = dalloc(w->stream, &w->temp[1], F)) ||
         (rc = dalloc(w->stream, &w->redox[0], F)) || (rc = dalloc(w->stream, &w->redox[1], F)) || (rc = dalloc(w->stream, &w->eaten, F)) ||
         (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
@@ -817,7 +829,7 @@ int ribo_world_destroy(RiboWorld* w) {
     resolve_timers(w);
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
     dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light[0]); dfree(w->light[1]); dfree(w->slope); dfree(w->agg_e);
-    dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
+    dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->p_old); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
@@ -850,6 +862,7 @@ int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s) {
     const bool relevel = std::max(s->mg_min_size, 1u) != std::max(w->st.mg_min_size, 1u);
     const bool arm = s->ledger_check && !w->st.ledger_check;
     w->st = *s;
+    if ((rc = reset_pressure_history(w))) return rc;
     if (relevel) {
         HIPCHK(hipStreamSynchronize(w->stream));
         if ((rc = build_mg(w)) || (rc = rebuild_terrain(w))) return rc;
@@ -891,6 +904,10 @@ int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t by
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w->stream));
     // new terrain or matter leaves the light stale (src/world/mod.rs:986,1026)
     if (which == RIBO_BUF_GAMMA || which == RIBO_BUF_CHEM) w->light_dirty = true;
+    if (which == RIBO_BUF_GAMMA || which == RIBO_BUF_VELOCITY || which == RIBO_BUF_PRESSURE) {
+        int rc = reset_pressure_history(w);
+        if (rc) return rc;
+    }
     if (which == RIBO_BUF_GAMMA) {
         int rc = rebuild_terrain(w);
         if (rc) return rc;

@@ -129,6 +129,13 @@ MODES = {
     "mixed": dict(mg_tol=1e-5, precision=1),
     # the reference's own schedule: one V-cycle per step, no residual target
     "reference": dict(mg_tol=0.0, precision=0),
+    # the plain FP32 path as the benchmark configuration states it: the
+    # reference V-cycle repeated until the relative residual reaches 1e-5
+    "vcycle": dict(mg_tol=1e-5, precision=0, stationary=1),
+    # the same, stopped at the true residual the conjugate-gradient modes end
+    # at (they stop on the recursive residual; in binary32 the recomputed one
+    # ends at 3-4e-5 on this world)
+    "vcycle_matched": dict(mg_tol=4e-5, precision=0, stationary=1),
 }
 
 
@@ -141,6 +148,9 @@ def configure(p, s, args, mode="fp32"):
     s.mg_max_cycles = args.mg_max_cycles
     s.mg_tol = MODES[mode]["mg_tol"]
     s.precision = MODES[mode]["precision"]
+    s.mg_stationary = MODES[mode].get("stationary", 0)
+    if s.mg_stationary:
+        s.mg_max_cycles = args.vcycle_max_cycles
     s.ledger_check = 1
     s.no_graphs = 1 if getattr(args, "no_graphs", False) else 0
     return p, s
@@ -171,10 +181,13 @@ def timed_run(cfg, inputs, args, mode):
         w.encode_steps(n)
         w.encode_steps(n)
     w.sync()
+    before = w.solver_stats()
     t0 = time.perf_counter()
     w.run(args.steps)
     w.sync()
     dt = time.perf_counter() - t0
+    after = w.solver_stats()
+    w.timed_solver_stats = tuple(a - b for a, b in zip(after, before))
     bad = w.ledger_status()
     if bad is not None:
         raise SystemExit(f"{mode}: per-channel monomer totals changed at epoch {bad}")
@@ -493,6 +506,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64-oracle error sample")
     ap.add_argument("--no-light", action="store_true", help="skip the light-step kernel timing")
+    ap.add_argument("--vcycle-steps", type=int, default=100,
+                    help="steps of the stationary V-cycle mode (the plain FP32 path to 1e-5); 0 skips it")
+    ap.add_argument("--vcycle-max-cycles", type=int, default=400, help="V-cycle cap per solve of the stationary mode")
     ap.add_argument("--config", default="world", choices=["world", "batched", "serving"],
                     help="world: the 2048^2 headline; batched: many independent worlds; serving: Poisson scene arrivals")
     ap.add_argument("--worlds", type=int, default=64)
@@ -527,7 +543,7 @@ def main():
     for mode in args.modes.split(","):
         dt, w = timed_run(cfg, inputs, args, mode)
         it, rel = w.mg_stats()
-        solves, iters, missed = w.solver_stats()
+        solves, iters, missed = w.timed_solver_stats
         results[mode] = {"value": cfg.cells() * args.steps / dt, "ms_per_step": 1000.0 * dt / args.steps,
                          "last_solve_iterations": it, "last_true_rel_residual": rel,
                          "mean_iterations": (iters / solves) if solves else None, "solves_missing_tol": missed}
@@ -581,6 +597,21 @@ def main():
                "frac_of_8TBps": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 8e12} for m in results}
 
     light = None if args.no_light else light_measure(cfg, inputs, args)
+    plain = {}
+    if args.vcycle_steps > 0:
+        # the stationary V-cycle, on a bounded number of steps (each step runs
+        # up to --vcycle-max-cycles V-cycles)
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup = args.vcycle_steps, 16
+        for mode in ("vcycle", "vcycle_matched"):
+            dt_v, wv = timed_run(cfg, inputs, sub, mode)
+            sv, iv, mv = wv.timed_solver_stats
+            it_v, rel_v = wv.mg_stats()
+            plain[mode] = {"value": cfg.cells() * sub.steps / dt_v, "ms_per_step": 1000.0 * dt_v / sub.steps,
+                           "steps": sub.steps, "mg_tol": MODES[mode]["mg_tol"],
+                           "mean_vcycles": (iv / sv) if sv else None, "solves_missing_tol": mv,
+                           "cap": args.vcycle_max_cycles, "last_rel_residual": rel_v}
+            wv.close()
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, inputs, args)
@@ -613,6 +644,13 @@ def main():
         },
         "modes": results,
         "mixed_vs_fp32": (results["mixed"]["value"] / results["fp32"]["value"]) if "mixed" in results and "fp32" in results else None,
+        "plain_vcycle": plain or None,
+        # ratios against the stationary V-cycle at the configuration's 1e-5 and
+        # at the true residual the conjugate-gradient modes end at
+        "mixed_vs_plain_vcycle": {m: results["mixed"]["value"] / v["value"] for m, v in plain.items()}
+        if plain and "mixed" in results else None,
+        "fp32_vs_plain_vcycle": {m: results["fp32"]["value"] / v["value"] for m, v in plain.items()}
+        if plain and "fp32" in results else None,
         "error_vs_fp64": f64,
         "roofline": roofline,
         "roofline_transport_scatter": roofline_scatter,

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define RIBO_ABI_VERSION 2u
+#define RIBO_ABI_VERSION 3u
 
 /* Opaque world handle. */
 typedef struct RiboWorld RiboWorld;
@@ -240,6 +240,11 @@ typedef struct RiboWorldSettings {
                                      light_interval steps; else
                                      light_rows_per_step propagation passes.  An
                                      upload of LIGHT makes it current. */
+    uint32_t mg_stationary;       /* with mg_tol > 0: 0 = conjugate gradients
+                                     preconditioned by the symmetric V-cycle (the
+                                     default); 1 = the reference V-cycle repeated
+                                     until the relative residual reaches mg_tol
+                                     (binary32 only; at most mg_max_cycles) */
 } RiboWorldSettings;
 
 enum { RIBO_PRECISION_FP32 = 0, RIBO_PRECISION_MIXED = 1 };

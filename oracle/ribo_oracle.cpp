@@ -1635,7 +1635,34 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
             fl.mg_p[i] = fluid ? fl.p_a[i] : R(0);
         }
         uint32_t cycles = std::max(st.mg_cycles, 1u);
-        if (st.mg_tol > 0.0f) {
+        if (st.mg_tol > 0.0f && st.mg_stationary) {
+            // the reference V-cycle repeated until ||rhs - A p|| / ||rhs||
+            // over level-0 fluid cells reaches mg_tol (A p and the residual
+            // in the field type, the norms in binary64)
+            const uint32_t maxit = std::max(st.mg_max_cycles, 1u);
+            const size_t n0 = (size_t)L0.n * L0.n;
+            std::vector<R> ap(n0);
+            auto rel_of = [&]() {
+                mg.apply(fl.mg_p.data(), ap.data());
+                double bb = 0.0, rr = 0.0;
+                for (size_t i = 0; i < n0; ++i) {
+                    if (!w.mg_fluid[i]) continue;
+                    const R r = fl.mg_rhs[i] - ap[i];
+                    bb += (double)fl.mg_rhs[i] * (double)fl.mg_rhs[i];
+                    rr += (double)r * (double)r;
+                }
+                return bb > 0.0 ? sqrt(rr / bb) : 0.0;
+            };
+            uint32_t it = 0;
+            double rel = rel_of();
+            while (rel > (double)st.mg_tol && it < maxit) {
+                mg.vcycle();
+                ++it;
+                rel = rel_of();
+            }
+            w.last_cycles = it;
+            w.last_rel_res = rel;
+        } else if (st.mg_tol > 0.0f) {
             pcg_solve<R>(w, fl, mg, lv);
             for (uint32_t i = 0; i < L0.n * L0.n; ++i) fl.mg_p[i] = fl.p_a[i];
         } else {

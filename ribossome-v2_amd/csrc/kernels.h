@@ -175,6 +175,11 @@ void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgSt
 // The last iteration's scalar update (slot rd to 1 - rd), the true residual
 // ||b - A x|| / ||b|| of the solve, and the statistics.
 template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st, int rd, float tol, uint32_t maxit);
+// Stationary tolerance mode around the reference V-cycle: phase 0 starts the
+// solve (residual of the warm start), 1 checks the residual after a V-cycle
+// and publishes `done`, 2 records the solve's statistics.
+void launch_vc_step(hipStream_t s, int phase, uint32_t n, float eps, const uint8_t* fl, const float* x, const float* b,
+                    PcgState* st, float tol, uint32_t maxit);
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
 void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);

@@ -1950,6 +1950,63 @@ __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, floa
     }
 }
 
+// Stationary tolerance mode: the reference V-cycle repeated until the
+// relative residual reaches the tolerance.  k_vc_resid leaves per-block
+// partials of ||b||^2 and ||b - A x||^2 over level-0 fluid cells (at the
+// start of a solve always, after a V-cycle only while the solve is not
+// done); k_vc_fin (one block) sums them in a fixed order:
+// VC_BEGIN starts the solve, VC_CHECK counts a V-cycle and publishes `done`
+// (residual reached or iteration cap), VC_END records the solve's statistics
+// as the conjugate-gradient mode does.
+__global__ __launch_bounds__(256) void k_vc_resid(uint32_t n, float eps, const uint8_t* __restrict__ fl,
+                                                  const float* __restrict__ xp, const float* __restrict__ b,
+                                                  PcgState* st, int begin) {
+    if (!begin && st->done) return;
+    const int x = (int)(blockIdx.x * 16 + threadIdx.x), y = (int)(blockIdx.y * 16 + threadIdx.y);
+    const int nn = (int)n;
+    const bool in = x < nn && y < nn;
+    const int i = in ? y * nn + x : 0;
+    double bb = 0.0, rr = 0.0;
+    if (in && fl[i]) {
+        const float r = b[i] - apply_A(fl, xp, nn, x, y, eps);
+        bb = (double)b[i] * (double)b[i];
+        rr = (double)r * (double)r;
+    }
+    block_partial<2>({bb, rr}, st->part);
+}
+enum VcFin { VC_BEGIN, VC_CHECK, VC_END };
+template <int W>
+__global__ __launch_bounds__(256) void k_vc_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit) {
+    if (W == VC_CHECK && st->done) return;
+    double t[2] = {0.0, 0.0};
+    if (W != VC_END) sum_partials<2>(st->part, nb, t);
+    if (threadIdx.x != 0) return;
+    if (W == VC_BEGIN) {
+        st->bb = t[0];
+        st->rel = t[0] > 0.0 ? sqrt(t[1] / t[0]) : 0.0;
+        st->it = 0u;
+        st->done = (st->rel <= (double)tol || maxit == 0u) ? 1u : 0u;
+        st->tol_seen = tol;
+    } else if (W == VC_CHECK) {
+        st->rel = st->bb > 0.0 ? sqrt(t[1] / st->bb) : 0.0;
+        st->it += 1u;
+        st->done = (st->rel <= (double)tol || st->it >= maxit) ? 1u : 0u;
+    } else {
+        const bool reached = st->rel <= (double)tol;
+        st->true_rel = st->rel;
+        st->steps += 1u;
+        st->iters += st->it;
+        if (!reached) st->missed += 1u;
+        st->done = 1u;
+        if (st->host_out) {
+            st->host_out[0] = st->it;
+            st->host_out[1] = st->missed;
+            st->host_out[2] = st->steps;
+            st->host_out[PCG_HOST_RING + (st->steps - 1u) % PCG_HOST_RING_LEN] = st->it;
+        }
+    }
+}
+
 // subtract_gradient (fluid.wgsl:610-630)
 __global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
                                                  const float* __restrict__ P) {
@@ -2695,6 +2752,13 @@ void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, Pcg
 template <class S> void launch_cg_up0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity) {
     const uint32_t nt = cg_tiles(V.n);
     k_cg_up0<S><<<dim3(nt, nt), CG_NT, 0, s>>>(V, C, st, parity & 1);
+}
+void launch_vc_step(hipStream_t s, int phase, uint32_t n, float eps, const uint8_t* fl, const float* x, const float* b,
+                    PcgState* st, float tol, uint32_t maxit) {
+    if (phase != 2) k_vc_resid<<<grid2(n), B16, 0, s>>>(n, eps, fl, x, b, st, phase == 0 ? 1 : 0);
+    if (phase == 0) k_vc_fin<VC_BEGIN><<<1, 256, 0, s>>>(st, pcg_blocks(n), tol, maxit);
+    else if (phase == 1) k_vc_fin<VC_CHECK><<<1, 256, 0, s>>>(st, pcg_blocks(n), tol, maxit);
+    else k_vc_fin<VC_END><<<1, 256, 0, s>>>(st, 0u, tol, maxit);
 }
 template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st, int rd, float tol, uint32_t maxit) {
     const uint32_t nt = cg_tiles(V.n), off = 5u * nt * nt;

@@ -162,6 +162,7 @@ struct RiboWorld {
     uint32_t pcg_slots = 0;         // iteration slots recorded per solve (0: not set)
     uint32_t missed_seen = 0;
     uint32_t solves_seen = 0;
+    uint32_t solves_at_settings = 0;  // solves recorded before the last settings change
     LedgerDev* ledger = nullptr;
     uint32_t* scratch_u32 = nullptr;
     // device bytes: allocated at creation (multigrid arrays aside), by the
@@ -529,6 +530,28 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
     run(w, "pcg_final", [&] { launch_cg_final<S>(s, V, w->pcg, (int)((n_it - 1u) & 1u), tol, maxit); });
 }
 
+// Stationary tolerance mode: the reference V-cycle (as vcycle() records it)
+// repeated until ||div - A p|| / ||div|| over level-0 fluid cells reaches
+// mg_tol or mg_max_cycles V-cycles ran; the residual is checked after every
+// V-cycle and the kernels of the remaining slots return at once.
+void vcycle_solve(RiboWorld* w, uint32_t slots) {
+    hipStream_t s = w->stream;
+    const float tol = w->st.mg_tol;
+    const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
+    const uint32_t* done = &w->pcg->done;
+    auto check = [&](int phase) {
+        run(w, "vc_check",
+            [&] { launch_vc_step(s, phase, w->F, w->levels[0].eps, w->mg_fluid, w->p_a, w->div, w->pcg, tol, maxit); });
+    };
+    check(0);
+    const uint32_t n_it = std::max(std::min(slots, maxit), 1u);
+    for (uint32_t j = 0; j < n_it; ++j) {
+        vcycle(w, done);
+        check(1);
+    }
+    check(2);
+}
+
 void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588
     hipStream_t s = w->stream;
     Geo g = w->geo();
@@ -548,7 +571,9 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     run(w, "vorticity", [&] { launch_vorticity(s, g, P, w->vel_b, w->vel_a); });
     run(w, "divergence", [&] { launch_divergence(s, g, w->vel_a, w->div, w->p_a); });
     if (w->st.multigrid) {
-        if (w->st.mg_tol > 0.0f) {
+        if (w->st.mg_tol > 0.0f && w->st.mg_stationary) {
+            vcycle_solve(w, w->pcg_slots);
+        } else if (w->st.mg_tol > 0.0f) {
             if (w->st.precision == RIBO_PRECISION_MIXED) pcg_solve<bf16>(w, w->mgh, w->pcg_r16, w->pcg_z16, w->pcg_slots);
             else pcg_solve<float>(w, w->mgz, nullptr, w->pcg_z, w->pcg_slots);
         } else {
@@ -661,6 +686,9 @@ int validate_settings(const RiboWorldSettings* s) {
     if (s->precision == RIBO_PRECISION_MIXED && !(s->mg_tol > 0.0f))
         return fail(RIBO_ERR_ARG, "the mixed-precision form is a tolerance-mode solver: set mg_tol > 0");
     if (!(s->mg_tol >= 0.0f)) return fail(RIBO_ERR_ARG, "mg_tol must be >= 0");
+    if (s->mg_stationary > 1u) return fail(RIBO_ERR_ARG, "mg_stationary must be 0 or 1");
+    if (s->mg_stationary && s->precision != RIBO_PRECISION_FP32)
+        return fail(RIBO_ERR_UNSUPPORTED, "the stationary tolerance mode is binary32 (precision 0)");
     return RIBO_OK;
 }
 
@@ -863,6 +891,9 @@ int ribo_world_set_settings(RiboWorld* w, const RiboWorldSettings* s) {
     const bool arm = s->ledger_check && !w->st.ledger_check;
     w->st = *s;
     if ((rc = reset_pressure_history(w))) return rc;
+    // the slot history starts again with the new settings (read after draining)
+    HIPCHK(hipStreamSynchronize(w->stream));
+    if (w->pcg_host) w->solves_at_settings = __atomic_load_n(&w->pcg_host[2], __ATOMIC_RELAXED);
     if (relevel) {
         HIPCHK(hipStreamSynchronize(w->stream));
         if ((rc = build_mg(w)) || (rc = rebuild_terrain(w))) return rc;
@@ -965,11 +996,13 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         // Iteration slots per solve for this chunk (read after draining the
         // stream, so the choice depends only on the sequence of calls): the
         // most iterations any of the last 64 solves took, plus max(3, 1/5 of
-        // it), rounded up to even.  An unused slot is ten kernels that return
-        // at once (~3 us each); on the 2048^2 benchmark world no solve took
-        // more than 1 iteration over that maximum in 400 steps.  A solve that
-        // ran out of slots (counted on the device) doubles the next chunk's
-        // slots; with no history the chunk gets the full cap.
+        // it) (max(8, 1/2 of it) for the stationary V-cycle, whose counts move
+        // more), rounded up to even.  An unused slot is ten kernels that
+        // return at once (~3 us each); on the 2048^2 benchmark world no solve
+        // took more than 1 iteration over that maximum in 400 steps.  A solve
+        // that ran out of slots (counted on the device) doubles the next
+        // chunk's slots; until 8 solves have run with the current settings
+        // every solve gets the full cap.
         const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
         HIPCHK(hipStreamSynchronize(w->stream));
         uint32_t slots = maxit;
@@ -977,13 +1010,14 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         if (w->pcg_host) {
             missed = __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED);
             const uint32_t solves = __atomic_load_n(&w->pcg_host[2], __ATOMIC_RELAXED);
-            const uint32_t known = std::min(solves, PCG_HOST_RING_LEN);
+            const uint32_t known = std::min(solves - w->solves_at_settings, PCG_HOST_RING_LEN);
             uint32_t most = 0u;
             for (uint32_t k = 0; k < known; ++k)
                 most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
                                                       __ATOMIC_RELAXED));
-            if (known > 0u) {
-                const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + std::max(3u, most / 5u);
+            if (known >= 8u) {
+                const uint32_t margin = w->st.mg_stationary ? std::max(8u, most / 2u) : std::max(3u, most / 5u);
+                const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + margin;
                 slots = std::min(maxit, (want + 1u) & ~1u);
             }
             w->solves_seen = solves;
@@ -1025,7 +1059,10 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             while (period % li != 0) period += base;
         }
         // tolerance-mode steps carry their solver iterations: shorter graphs
-        const uint32_t max_graph = (w->st.multigrid && w->st.mg_tol > 0.0f) ? 8u : RIBO_MAX_STEPS_PER_CALL;
+        // (the stationary mode records up to mg_max_cycles V-cycles per step:
+        // one step per graph)
+        const uint32_t max_graph = !(w->st.multigrid && w->st.mg_tol > 0.0f) ? RIBO_MAX_STEPS_PER_CALL
+                                   : (w->st.mg_stationary ? 1u : 8u);
         for (uint32_t first = 0; first < steps;) {
             const uint32_t n = std::min(max_graph, steps - first);
             const uint32_t phase = (w->params.epoch + first) % period;

@@ -179,6 +179,7 @@ class WorldSettings(ctypes.Structure):
         ("ledger_check", ctypes.c_uint32),
         ("no_graphs", ctypes.c_uint32),
         ("light_mode", ctypes.c_uint32),
+        ("mg_stationary", ctypes.c_uint32),
     ]
 
     @classmethod

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 25
     for s in sorted(syms):
         assert hasattr(L, s), s
-    assert L.ribo_abi_version() == 2
+    assert L.ribo_abi_version() == 3
 
 
 def test_struct_layouts_match_reference():

@@ -213,6 +213,29 @@ def test_tolerance_mode_matches_oracle(precision):
     assert ra.Ledger.from_cells(gw.read_cells_blocking()).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
 
 
+def test_stationary_vcycle_mode_bit_exact():
+    """The stationary tolerance mode repeats the reference V-cycle until the
+    relative residual reaches the tolerance.  Its V-cycles are the reference
+    mode's kernels, so GPU and oracle run the same number of cycles per solve
+    and give bit-identical fields (the stopping residual is summed in another
+    order, which moves it by ~1e-16 relative)."""
+    print("case stationary", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 23, rock=0.12, rubble=True, vent_rows=3)
+    s = parity_settings(mg_tol=1e-4, mg_max_cycles=300, mg_stationary=1)
+    gw, ow = ra.World(cfg, 23), oc.OracleWorld(cfg, 23)
+    load(gw, inp, parity_params(23), s)
+    load(ow, inp, parity_params(23), s)
+    for k in range(4):
+        gw.encode_steps(1)
+        ow.encode_steps(1)
+        (cg, rg), (co, ro) = gw.mg_stats(), ow.mg_stats()
+        print(f"step {k}: GPU {cg} V-cycles, rel {rg:.6e}; oracle {co} V-cycles, rel {ro:.6e}", flush=True)
+        assert cg == co and 1 < cg < 300 and rg <= 1e-4
+        assert abs(rg - ro) <= 1e-6 * ro
+    assert compare(gw, ow, "stationary"), "stationary mode: GPU differs from the oracle"
+
+
 @pytest.mark.parametrize("iters", [1, 2])
 def test_tolerance_mode_preconditioner_matches_oracle(iters):
     """One or two conjugate-gradient iterations of the binary32 tolerance

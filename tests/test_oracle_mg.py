@@ -186,3 +186,27 @@ def test_binary64_oracle_bounds_the_binary32_fluid_error():
         a, b = w32.download_f64(which), w64.download_f64(which)
         e = np.linalg.norm(a - b) / np.linalg.norm(b)
         assert e < tol, (which, e)
+
+
+def test_stationary_vcycle_mode_reaches_the_residual():
+    """The stationary tolerance mode (the reference V-cycle repeated until the
+    relative residual reaches the tolerance): the final pressure's residual,
+    recomputed with the numpy operator above, is within the tolerance, and the
+    reported residual is that one."""
+    import ribossome_amd as ra
+    from worlds import load, make_inputs, parity_params, parity_settings
+
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 2, rock=0.15, rubble=True, vent_rows=2)
+    w = oc.OracleWorld(cfg, 2)
+    load(w, inp, parity_params(2), parity_settings(mg_tol=1e-4, mg_max_cycles=400, mg_stationary=1))
+    w.encode_steps(2)
+    n = cfg.fluid_size
+    fluid = (w.download(ra.SOLID_MASK) == 0).astype(np.uint32)
+    A, f = operator(n, fluid, 1e-4)
+    p = w.download_f64(ra.PRESSURE)
+    b = w.download_f64(ra.DIVERGENCE)
+    rel = np.linalg.norm(b[f] - A @ p[f]) / np.linalg.norm(b[f])
+    it, rel_reported = w.mg_stats()
+    assert 1 < it < 400
+    assert rel <= 1.0001e-4 and abs(rel - rel_reported) < 1e-3 * rel_reported

@@ -313,6 +313,8 @@ RIBO_API size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorld
 /* Device bytes this world holds now. */
 RIBO_API int ribo_world_device_bytes(const RiboWorld* w, size_t* out);
 RIBO_API int ribo_world_destroy(RiboWorld* w);
+/* The world's configuration (World::cfg, src/world/mod.rs:205). */
+RIBO_API int ribo_world_get_config(const RiboWorld* w, RiboWorldConfig* out);
 
 /* World.params / World.settings (pub fields, src/world/mod.rs:205-207). */
 RIBO_API int ribo_world_set_params(RiboWorld* w, const RiboSimParams* p);

@@ -871,6 +871,12 @@ int ribo_world_destroy(RiboWorld* w) {
     return RIBO_OK;
 }
 
+int ribo_world_get_config(const RiboWorld* w, RiboWorldConfig* out) {
+    if (!w || !out) return fail(RIBO_ERR_ARG, "null argument");
+    *out = w->cfg;
+    return RIBO_OK;
+}
+
 int ribo_world_set_params(RiboWorld* w, const RiboSimParams* p) {
     if (!w || !p) return fail(RIBO_ERR_ARG, "null argument");
     w->params = *p;

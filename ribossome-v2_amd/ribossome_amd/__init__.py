@@ -295,6 +295,14 @@ def lib() -> ctypes.CDLL:
         "ribo_gen_chem": (I, [P, ctypes.c_uint64, ctypes.c_float, P, P]),
         "ribo_gen_vent_heat": (I, [P, U32, ctypes.c_float, P]),
         "ribo_world_footprint": (SZ, [P, P]),
+        "ribo_world_get_config": (I, [P, P]),
+        "ribo_scene_write": (I, [ctypes.c_char_p, P, P, P, P]),
+        "ribo_scene_read": (I, [ctypes.c_char_p, P, P, P, P]),
+        "ribo_world_save_scene": (I, [P, ctypes.c_char_p]),
+        "ribo_world_load_scene": (I, [P, ctypes.c_char_p]),
+        "ribo_lz4_bound": (SZ, [SZ]),
+        "ribo_lz4_compress": (ctypes.c_int64, [P, SZ, P, SZ]),
+        "ribo_lz4_decompress": (ctypes.c_int64, [P, SZ, P, SZ]),
         "ribo_world_device_bytes": (I, [P, ctypes.POINTER(SZ)]),
         "ribo_server_create": (I, [ctypes.c_uint64, U32, U32, ctypes.POINTER(P)]),
         "ribo_server_destroy": (I, [P]),
@@ -456,6 +464,14 @@ class World:
     def invalidate_light(self):
         """World::invalidate_light: the next step runs the full light sweep."""
         _check(lib().ribo_world_invalidate_light(self._h), "invalidate_light")
+
+    def save_scene(self, path: str):
+        """World::save_scene: the grids, parameters and settings as a .ribo file."""
+        _check(lib().ribo_world_save_scene(self._h, os.fsencode(path)), "save_scene")
+
+    def load_scene(self, path: str):
+        """Scene::read + World::load_scene into this world (same configuration)."""
+        _check(lib().ribo_world_load_scene(self._h, os.fsencode(path)), "load_scene")
 
     def device_bytes(self) -> int:
         """Device bytes this world holds now."""
@@ -640,3 +656,62 @@ class Server:
         _check(lib().ribo_server_stats(self._h, ctypes.byref(q), ctypes.byref(r), ctypes.byref(f), ctypes.byref(u)),
                "server_stats")
         return int(q.value), int(r.value), int(f.value), int(u.value)
+
+
+class _CSceneGrids(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in ("chem", "gamma", "velocity", "pressure", "temperature", "redox",
+                                                "heat_src")]
+
+
+SCENE_GRIDS = (("chem", CHEM), ("gamma", GAMMA), ("velocity", VELOCITY), ("pressure", PRESSURE),
+               ("temperature", TEMP), ("redox", REDOX), ("heat_src", HEAT_SRC))
+
+
+def scene_write(path: str, cfg: WorldConfig, params: SimParams, settings: WorldSettings, grids: dict):
+    """ribo_scene_write: grids maps CHEM, GAMMA, VELOCITY, PRESSURE, TEMP, REDOX to host arrays."""
+    g = _CSceneGrids()
+    keep = []
+    for name, which in SCENE_GRIDS:
+        if which == HEAT_SRC and which not in grids:
+            continue  # optional block
+        a = np.ascontiguousarray(grids[which], dtype=_BUF_DTYPE[which])
+        if a.nbytes != buffer_bytes(cfg, which):
+            raise RiboError(f"scene_write: {name} has {a.nbytes} bytes")
+        keep.append(a)
+        setattr(g, name, a.ctypes.data)
+    c = cfg.c()
+    _check(lib().ribo_scene_write(os.fsencode(path), ctypes.byref(c), ctypes.byref(params), ctypes.byref(settings),
+                                  ctypes.byref(g)), "scene_write")
+
+
+def scene_read(path: str, cfg: WorldConfig, seed: int = 1, settings: "WorldSettings | None" = None):
+    """ribo_scene_read: (params, settings, grids) of a .ribo file for a world of configuration cfg."""
+    p = SimParams.default(seed)
+    s = settings if settings is not None else WorldSettings.default()
+    g = _CSceneGrids()
+    out = {}
+    for name, which in SCENE_GRIDS:
+        a = np.zeros(buffer_bytes(cfg, which) // 4, dtype=_BUF_DTYPE[which])
+        out[which] = a
+        setattr(g, name, a.ctypes.data)
+    c = cfg.c()
+    _check(lib().ribo_scene_read(os.fsencode(path), ctypes.byref(c), ctypes.byref(p), ctypes.byref(s), ctypes.byref(g)),
+           "scene_read")
+    return p, s, out
+
+
+def lz4_compress(data: bytes) -> bytes:
+    src = np.frombuffer(bytes(data), dtype=np.uint8)
+    cap = int(lib().ribo_lz4_bound(len(src)))
+    dst = np.empty(max(cap, 1), dtype=np.uint8)
+    n = _check(int(lib().ribo_lz4_compress(_ptr(src) if len(src) else None, len(src), _ptr(dst), cap)), "lz4_compress")
+    return dst[:n].tobytes()
+
+
+def lz4_decompress(block: bytes, size: int) -> bytes:
+    src = np.frombuffer(bytes(block), dtype=np.uint8)
+    dst = np.empty(max(size, 1), dtype=np.uint8)
+    n = int(lib().ribo_lz4_decompress(_ptr(src) if len(src) else None, len(src), _ptr(dst), size))
+    if n < 0:
+        raise RiboError("lz4_decompress: malformed block")
+    return dst[:n].tobytes()

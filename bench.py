@@ -314,6 +314,24 @@ HIP_NAMES = {
     "enforce_boundaries": "k_boundaries", "buoyancy_gather": "k_forces",
 }
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "world_step_2048_hbm_counters.md")
+SQ_SUMMARY = os.path.join(ROOT, "profiles", "transport_scatter_2048_sq_counters.md")
+# one wave-level VALU instruction per CU per cycle: 256 CUs at 2.4 GHz
+VALU_PEAK_PER_S = 256 * 2.4e9
+
+
+def sq_valu_per_launch(kernel="k_scatter"):
+    """VALU instructions (wave level) per launch of `kernel` from the
+    committed SQ counter summary of this workload, or None."""
+    if not os.path.exists(SQ_SUMMARY):
+        return None
+    for line in open(SQ_SUMMARY):
+        cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if len(cols) > 3 and cols[0] == kernel:
+            try:
+                return float(cols[3])
+            except ValueError:
+                return None
+    return None
 
 
 def pmc_traffic(kernel, mode):
@@ -590,6 +608,15 @@ def main():
                         "traffic": pmc_traffic("transport_scatter", head), "avg_launch_us": sc_s * 1e6,
                         "launches": sc_n, "algorithmic_bytes_per_launch": sc_bytes,
                         "note": "VALU-issue bound (PCG4D per monomer); HBM is not its limit"}
+    # the scatter against its real bound: wave-level VALU instructions per
+    # launch (committed SQ_INSTS_VALU) over the live launch time
+    valu = sq_valu_per_launch()
+    roofline_valu = None
+    if valu and sc_s:
+        roofline_valu = {"kernel": "transport_scatter", "bound": "valu", "achieved": valu / sc_s / 1e9,
+                         "peak": VALU_PEAK_PER_S / 1e9, "unit": "G wave-instructions/s",
+                         "frac": valu / sc_s / VALU_PEAK_PER_S, "instructions_per_launch": valu,
+                         "avg_launch_us": sc_s * 1e6}
     kernels = {k: [round(v[0] / max(v[1], 1) * 1000.0, 2), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
     step_bytes = {m: step_model_bytes(cfg, sizes, m, results[m]) for m in results}
     hbm = {m: {"algorithmic_bytes_per_step": step_bytes[m],
@@ -654,6 +681,7 @@ def main():
         "error_vs_fp64": f64,
         "roofline": roofline,
         "roofline_transport_scatter": roofline_scatter,
+        "roofline_transport_scatter_valu": roofline_valu,
         "hbm_step_model": hbm,
         "light": light,
         "cpu_baseline": cpu,

@@ -42,7 +42,9 @@ typedef struct RiboSceneGrids {
 } RiboSceneGrids;
 
 /* Writes a scene (World::save_scene without agents; the file appears
- * whole: it is written to "<path>.tmp" and renamed). */
+ * whole: it is written to "<path>.tmp" and renamed).  A block holds at most
+ * 4 GiB (the format's u32 size): worlds above 16384^2 grid cells cannot be
+ * saved (RIBO_ERR_ARG). */
 RIBO_API int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
                               const RiboWorldSettings* settings, const RiboSceneGrids* grids);
 /* Reads a scene for a world of configuration cfg (Scene::read +

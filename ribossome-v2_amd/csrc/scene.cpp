@@ -406,7 +406,10 @@ const char* const AGENT_BLOCKS[] = {"agents", "genomes", "bodies", "organs", "rn
 constexpr uint32_t SCENE_VERSION = 2;  // src/world/snapshot.rs:21
 const char SCENE_MAGIC[8] = {'R', 'I', 'B', 'O', 'S', 'C', 'N', '1'};
 
-void put_block(std::vector<uint8_t>& out, const std::string& name, const void* data, size_t bytes) {
+// One block; false when it is too large for the u32 size in front (the
+// format's limit: 4 GiB per block).
+bool put_block(std::vector<uint8_t>& out, const std::string& name, const void* data, size_t bytes) {
+    if (bytes > 0xFFFFFFFFull) return false;
     const std::vector<uint8_t> c = lz4_compress((const uint8_t*)data, bytes);
     const uint16_t nl = (uint16_t)name.size();
     const uint64_t cl = 4 + c.size();
@@ -416,6 +419,7 @@ void put_block(std::vector<uint8_t>& out, const std::string& name, const void* d
     out.insert(out.end(), (const uint8_t*)&cl, (const uint8_t*)&cl + 8);
     out.insert(out.end(), (const uint8_t*)&raw, (const uint8_t*)&raw + 4);
     out.insert(out.end(), c.begin(), c.end());
+    return true;
 }
 
 struct GridSpec { const char* block; RiboBuffer buf; };
@@ -488,8 +492,7 @@ int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSim
     for (int k = 0; k < NGRID; ++k) {
         const void* data = *grid_ptr(&g, k);
         if (!data && k == NGRID - 1) continue;  // no heat source block
-        if (!data) return RIBO_ERR_ARG;
-        put_block(out, GRIDS[k].block, data, ribo_buffer_bytes(cfg, GRIDS[k].buf));
+        if (!data || !put_block(out, GRIDS[k].block, data, ribo_buffer_bytes(cfg, GRIDS[k].buf))) return RIBO_ERR_ARG;
     }
     const std::string tmp = std::string(path) + ".tmp";
     FILE* f = fopen(tmp.c_str(), "wb");

@@ -43,7 +43,7 @@ typedef struct RiboSceneGrids {
 
 /* Writes a scene (World::save_scene without agents; the file appears
  * whole: it is written to "<path>.tmp" and renamed).  A block holds at most
- * 4 GiB (the format's u32 size): worlds above 16384^2 grid cells cannot be
+ * 4 GiB - 1 (the format's u32 size): worlds of 16384^2 grid cells or more cannot be
  * saved (RIBO_ERR_ARG). */
 RIBO_API int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
                               const RiboWorldSettings* settings, const RiboSceneGrids* grids);

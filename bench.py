@@ -166,6 +166,8 @@ def build_world(cfg, inputs, args, mode="fp32"):
     w.upload(ra.CHEM, chem)
     w.upload(ra.HEAT_SRC, heat)
     w.settings = s
+    if not getattr(args, "no_autotune", False):
+        w.autotune()
     return w
 
 
@@ -403,6 +405,7 @@ def batched(args):
         w.upload(ra.VELOCITY, vel)
         w.upload(ra.CHEM, chem)
         w.settings = s
+        w.autotune()
         worlds.append((w, cfg, ra.Ledger.from_cells(chem)))
     for w, _, _ in worlds:  # warm-up: every graph the timed loop replays
         w.encode_steps(64)
@@ -422,14 +425,16 @@ def batched(args):
         if w.ledger_status() is not None or w.ledger_blocking().channels() != led.channels():
             raise SystemExit("batched: a world's per-channel monomer totals changed")
     cells = sum(c.cells() for _, c, _ in worlds)
+    tiles = {c.grid_size: w.mg_tiles() for w, c, _ in worlds}
     for w, _, _ in worlds:
         w.close()
     return {"metric": "batched world cell-steps/s", "value": cells * args.batch_steps / dt, "unit": "cell-steps/s",
             "n_gpus": 1, "steps": args.batch_steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
             "dtype": "f32", "data": "synthetic", "ms_per_step": 1000.0 * dt / args.batch_steps,
             "config": {"workload": f"{len(worlds)} independent worlds, grid sides from {sides}, reference solver "
-                                   "schedule (one V-cycle per step), ledger checked every step",
-                       "worlds": len(worlds), "grid_cells": cells}}
+                                   "schedule (one V-cycle per step), ledger checked every step, multigrid tiles "
+                                   "from the autotuner",
+                       "worlds": len(worlds), "grid_cells": cells, "tiles_by_side": tiles}}
 
 
 def serving_scenes(args):
@@ -538,6 +543,7 @@ def main():
     ap.add_argument("--serve-threads", type=int, default=0, help="scheduler threads of the server (0: its default)")
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (0: the runtime's default; at most 32)")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the multigrid tile size rule")
     ap.add_argument("--no-graphs", action="store_true",
                     help="launch every kernel eagerly instead of replaying captured HIP graphs (for tracing tools)")
     args = ap.parse_args()
@@ -666,6 +672,7 @@ def main():
             "grid": GRID,
             "fluid": FLUID,
             "mg_levels": len(w.mg_levels()),
+            "mg_tiles": w.mg_tiles(),
             "mode": head,
             "mg_tol": MODES[head]["mg_tol"],
         },

@@ -351,6 +351,17 @@ RIBO_API int ribo_world_sync(RiboWorld* w);
  * step has finished, 0 while work is still running, negative on error. */
 RIBO_API int ribo_world_idle(RiboWorld* w);
 
+/* The tile side (16 or 32 cells) of each multigrid level's fused down / up
+ * kernels, measured on this world's levels for its current solver (no
+ * counterpart in the reference, which dispatches fixed 16 x 16 workgroups):
+ * force 0 times both sides per level and keeps the faster (the choice is
+ * remembered per level side for every world of the process); force 16 or 32
+ * sets that side on every level.  Results do not depend on the tiles.
+ * ribo_world_mg_tiles reads the sides back (0: the size rule, 32 above 256
+ * cells a side, else 16). */
+RIBO_API int ribo_world_autotune(RiboWorld* w, int force);
+RIBO_API int ribo_world_mg_tiles(const RiboWorld* w, uint8_t* out, uint32_t cap, uint32_t* count);
+
 /* World::invalidate_light (src/world/mod.rs:1400-1402): the next step runs
  * the full light sweep (light_mode 1). */
 RIBO_API int ribo_world_invalidate_light(RiboWorld* w);

@@ -141,10 +141,12 @@ void launch_mg_smooth(hipStream_t s, const MgLevelDev& L, uint32_t color, const 
 // fused multigrid: pre-smooth + restrict, prolong + post-smooth per level,
 // and the coarse end of the cycle (at most 8 levels) in one workgroup.
 // sym = the symmetric V-cycle of the tolerance mode.
+// tile: cells a side per workgroup, 16 or 32 (0: 32 above MG_SMALL cells a
+// side, else 16); the results do not depend on it.
 template <class S> void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym,
-                                       const uint32_t* done);
+                                       const uint32_t* done, int tile = 0);
 template <class S> void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym,
-                                     const uint32_t* done);
+                                     const uint32_t* done, int tile = 0);
 // Zeroes a level's unknowns (the symmetric cycle's coarsest level when no
 // suffix of levels fits one workgroup).
 template <class S> void launch_zero_level(hipStream_t s, const MgLevelT<S>& L);

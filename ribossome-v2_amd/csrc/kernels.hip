@@ -2675,8 +2675,8 @@ void launch_mg_prolong(hipStream_t s, const MgLevelDev& L, const MgLevelDev& C, 
     k_mg_prolong<<<grid2(L.n), B16, 0, s>>>(L, C, done);
 }
 template <class S>
-void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done) {
-    if (L.n <= (uint32_t)MG_SMALL) {
+void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done, int tile) {
+    if (tile ? tile == 16 : L.n <= (uint32_t)MG_SMALL) {
         const uint32_t nt = (L.n + 15) / 16;
         if (sym) k_mg_down<S, true, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
         else k_mg_down<S, false, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
@@ -2687,8 +2687,8 @@ void launch_mg_down(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, b
     }
 }
 template <class S>
-void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done) {
-    if (L.n <= (uint32_t)MG_SMALL) {
+void launch_mg_up(hipStream_t s, const MgLevelT<S>& L, const MgLevelT<S>& C, bool sym, const uint32_t* done, int tile) {
+    if (tile ? tile == 16 : L.n <= (uint32_t)MG_SMALL) {
         const uint32_t nt = (L.n + 15) / 16;
         if (sym) k_mg_up<S, true, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
         else k_mg_up<S, false, 16><<<dim3(nt, nt), 256, 0, s>>>(L, C, done);
@@ -2766,8 +2766,8 @@ template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgSta
     k_pcg_fin<FIN_FINAL><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), 0.0f, 0u, 1 - (rd & 1), off);
 }
 #define RIBO_INSTANTIATE(S)                                                                                         \
-    template void launch_mg_down<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);   \
-    template void launch_mg_up<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*);     \
+    template void launch_mg_down<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*, int); \
+    template void launch_mg_up<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*, int); \
     template void launch_zero_level<S>(hipStream_t, const MgLevelT<S>&);                                            \
     template void launch_mg_block<S>(hipStream_t, const MgLevelT<S>*, int, bool, const double*, const int32_t*, int, \
                                      const uint32_t*);                                                              \

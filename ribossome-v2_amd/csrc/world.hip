@@ -15,7 +15,9 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <vector>
 
@@ -135,6 +137,7 @@ struct RiboWorld {
     float* mg_rhs = nullptr;
     uint8_t* mg_fluid = nullptr;
     size_t mg_block_start = 0;  // first level solved inside one workgroup
+    std::vector<uint8_t> mg_tile;  // tile side of each level's down / up kernels (0: the size rule)
     size_t sym_coarse = 0;      // level the tolerance mode solves exactly (the coarsest)
     size_t lds_limit = 65536;
     // tolerance mode (preconditioned conjugate gradients)
@@ -308,6 +311,7 @@ int build_mg(RiboWorld* w) {
     dfree(w->mg_rhs);
     dfree(w->mg_fluid);
     w->levels = make_levels(w->F, std::max(w->st.mg_min_size, 1u));
+    w->mg_tile.assign(w->levels.size(), 0);
     size_t total = w->levels.back().off_c;
     size_t fine = (size_t)w->F * w->F;
     size_t coarse = total - fine;
@@ -450,18 +454,22 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
     hipStream_t s = w->stream;
     const size_t b = w->mg_block_start;
     if (b < L) {
-        for (size_t l = 0; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<float>(s, w->mg[l], w->mg[l + 1], false, done); });
+        for (size_t l = 0; l < b; ++l)
+            run(w, "mg_down", [&] { launch_mg_down<float>(s, w->mg[l], w->mg[l + 1], false, done, w->mg_tile[l]); });
         run(w, "mg_block", [&] { launch_mg_block<float>(s, w->mg.data() + b, (int)(L - b), false, nullptr, nullptr, 0, done); });
-        for (size_t l = b; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done); });
+        for (size_t l = b; l-- > 0;)
+            run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done, w->mg_tile[l]); });
         return;
     }
     // coarsest level too large for one workgroup: per-operation dispatches
-    for (size_t l = 0; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<float>(s, w->mg[l], w->mg[l + 1], false, done); });
+    for (size_t l = 0; l + 1 < L; ++l)
+        run(w, "mg_down", [&] { launch_mg_down<float>(s, w->mg[l], w->mg[l + 1], false, done, w->mg_tile[l]); });
     for (uint32_t k = 0; k < MG_COARSE; ++k) {
         run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 0, done); });
         run(w, "mg_smooth", [&] { launch_mg_smooth(s, w->mg[L - 1], 1, done); });
     }
-    for (size_t l = L - 1; l-- > 0;) run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done); });
+    for (size_t l = L - 1; l-- > 0;)
+        run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done, w->mg_tile[l]); });
 }
 
 // Levels 1 .. sym_coarse (the coarsest, solved exactly with its dense
@@ -474,16 +482,18 @@ template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevel
     hipStream_t s = w->stream;
     if (w->mg_block_start < L) {
         const size_t b = std::min(std::max<size_t>(w->mg_block_start, 1), lc);
-        for (size_t l = 1; l < b; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+        for (size_t l = 1; l < b; ++l)
+            run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
         run(w, "mg_block", [&] { launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done); });
-        for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
+        for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
         return;
     }
     // levels too large for one workgroup: sweeps only (the coarsest level
     // keeps the zero correction its restriction implies)
-    for (size_t l = 1; l + 1 < L; ++l) run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done); });
+    for (size_t l = 1; l + 1 < L; ++l)
+        run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
     run(w, "mg_zero", [&] { launch_zero_level<S>(s, lv[L - 1]); });
-    for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done); });
+    for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
 }
 
 // Tolerance mode: preconditioned conjugate gradients (Chronopoulos-Gear
@@ -1122,6 +1132,95 @@ int ribo_world_run(RiboWorld* w, uint32_t steps) {
         if (n < 0) return n;
         done += (uint32_t)n;
     }
+    return RIBO_OK;
+}
+
+// Tile sides measured per (level side, V-cycle kind, storage) by any world
+// of the process, so that many worlds of one size measure once.
+std::mutex g_tune_mu;
+std::map<std::tuple<uint32_t, int, int>, uint8_t> g_tune_cache;
+
+int ribo_world_autotune(RiboWorld* w, int force) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    if (force != 0 && force != 16 && force != 32) return fail(RIBO_ERR_ARG, "force must be 0, 16 or 32");
+    const size_t L = w->levels.size();
+    clear_graphs(w);
+    const bool sym = w->st.multigrid && w->st.mg_tol > 0.0f && !w->st.mg_stationary;
+    const int bf = sym && w->st.precision == RIBO_PRECISION_MIXED ? 1 : 0;
+    // the levels that run as tile kernels: below the one-workgroup block
+    // (level 0 of the symmetric cycle belongs to the conjugate-gradient
+    // kernels), all but the coarsest when no suffix fits one workgroup
+    const size_t lo = sym ? 1 : 0;
+    const size_t hi = w->mg_block_start < L ? w->mg_block_start : L - 1;
+    for (size_t l = 0; l < L; ++l) w->mg_tile[l] = force ? (uint8_t)force : 0;
+    if (force || hi <= lo) return RIBO_OK;
+    HIPCHK(hipStreamSynchronize(w->stream));
+    // scratch levels (the world's masks, zeroed values): timing only, the
+    // world's unknowns are not touched
+    const size_t n0 = w->levels[lo].n;
+    float* buf = nullptr;
+    int rc = dalloc(w->stream, &buf, 6 * n0 * n0);
+    if (rc) return rc;
+    hipEvent_t e0 = get_event(w), e1 = get_event(w);
+    auto measure = [&]() -> int {
+        for (size_t l = lo; l < hi; ++l) {
+            const uint32_t n = w->levels[l].n, nc = w->levels[l + 1].n;
+            const auto key = std::make_tuple(n, sym ? 1 : 0, bf);
+            {
+                std::lock_guard<std::mutex> lk(g_tune_mu);
+                auto it = g_tune_cache.find(key);
+                if (it != g_tune_cache.end()) {
+                    w->mg_tile[l] = it->second;
+                    continue;
+                }
+            }
+            MgLevelT<float> A{n, w->levels[l].eps, buf, buf + n0 * n0, buf + 2 * n0 * n0, w->mg[l].fluid};
+            MgLevelT<float> C{nc, w->levels[l + 1].eps, buf + 3 * n0 * n0, buf + 4 * n0 * n0, buf + 5 * n0 * n0,
+                              w->mg[l + 1].fluid};
+            MgLevelT<bf16> Ah{n, A.eps, (bf16*)A.p, (bf16*)A.pp, (bf16*)A.rhs, A.fluid};
+            MgLevelT<bf16> Ch{nc, C.eps, (bf16*)C.p, (bf16*)C.pp, (bf16*)C.rhs, C.fluid};
+            float best = 0.0f;
+            uint8_t pick = 0;
+            for (int tile : {16, 32}) {
+                auto cycle = [&]() {
+                    if (bf) {
+                        launch_mg_down<bf16>(w->stream, Ah, Ch, sym, nullptr, tile);
+                        launch_mg_up<bf16>(w->stream, Ah, Ch, sym, nullptr, tile);
+                    } else {
+                        launch_mg_down<float>(w->stream, A, C, sym, nullptr, tile);
+                        launch_mg_up<float>(w->stream, A, C, sym, nullptr, tile);
+                    }
+                };
+                for (int k = 0; k < 3; ++k) cycle();
+                HIPCHK(hipEventRecord(e0, w->stream));
+                for (int k = 0; k < 20; ++k) cycle();
+                HIPCHK(hipEventRecord(e1, w->stream));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                if (pick == 0 || ms < best) {
+                    best = ms;
+                    pick = (uint8_t)tile;
+                }
+            }
+            w->mg_tile[l] = pick;
+            std::lock_guard<std::mutex> lk(g_tune_mu);
+            g_tune_cache[key] = pick;
+        }
+        return RIBO_OK;
+    };
+    rc = measure();
+    (void)hipStreamSynchronize(w->stream);
+    w->event_pool.push_back(e0);
+    w->event_pool.push_back(e1);
+    dfree(buf);
+    return rc;
+}
+
+int ribo_world_mg_tiles(const RiboWorld* w, uint8_t* out, uint32_t cap, uint32_t* count) {
+    if (!w || !count) return fail(RIBO_ERR_ARG, "null argument");
+    *count = (uint32_t)w->mg_tile.size();
+    for (uint32_t i = 0; i < cap && i < *count; ++i) out[i] = w->mg_tile[i];
     return RIBO_OK;
 }
 

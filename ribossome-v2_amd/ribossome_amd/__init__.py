@@ -296,6 +296,8 @@ def lib() -> ctypes.CDLL:
         "ribo_gen_vent_heat": (I, [P, U32, ctypes.c_float, P]),
         "ribo_world_footprint": (SZ, [P, P]),
         "ribo_world_get_config": (I, [P, P]),
+        "ribo_world_autotune": (I, [P, I]),
+        "ribo_world_mg_tiles": (I, [P, P, U32, ctypes.POINTER(U32)]),
         "ribo_scene_write": (I, [ctypes.c_char_p, P, P, P, P]),
         "ribo_scene_read": (I, [ctypes.c_char_p, P, P, P, P]),
         "ribo_world_save_scene": (I, [P, ctypes.c_char_p]),
@@ -464,6 +466,17 @@ class World:
     def invalidate_light(self):
         """World::invalidate_light: the next step runs the full light sweep."""
         _check(lib().ribo_world_invalidate_light(self._h), "invalidate_light")
+
+    def autotune(self, force: int = 0) -> list:
+        """Measures (force 0) or sets (16, 32) the multigrid tile sides; returns them per level."""
+        _check(lib().ribo_world_autotune(self._h, int(force)), "autotune")
+        return self.mg_tiles()
+
+    def mg_tiles(self) -> list:
+        out = (ctypes.c_uint8 * 32)()
+        n = ctypes.c_uint32()
+        _check(lib().ribo_world_mg_tiles(self._h, out, 32, ctypes.byref(n)), "mg_tiles")
+        return list(out[:n.value])
 
     def save_scene(self, path: str):
         """World::save_scene: the grids, parameters and settings as a .ribo file."""

@@ -236,6 +236,29 @@ def test_stationary_vcycle_mode_bit_exact():
     assert compare(gw, ow, "stationary"), "stationary mode: GPU differs from the oracle"
 
 
+@pytest.mark.parametrize("tol", [0.0, 1e-5])
+def test_multigrid_tiles_do_not_change_results(tol):
+    """The fused multigrid kernels give the same bits with 16- and 32-cell
+    tiles on every level and with the autotuner's choice (a 512^2 world has
+    two tile levels in the reference cycle, one in the symmetric one)."""
+    print(f"case tiles tol={tol}", flush=True)
+    cfg = ra.WorldConfig(512, 256)
+    inp = make_inputs(cfg, 29, rock=0.12, rubble=True, vent_rows=4)
+    extra = dict(mg_tol=tol, mg_max_cycles=40) if tol else {}
+    out = []
+    for force in (16, 32, 0):
+        w = ra.World(cfg, 29)
+        load(w, inp, parity_params(29), parity_settings(**extra))
+        tiles = w.autotune(force)
+        print(f"force {force}: tiles {tiles}", flush=True)
+        assert all(t in (0, 16, 32) for t in tiles)
+        w.encode_steps(3)
+        out.append((w.read_cells_blocking(), w.download(ra.VELOCITY), w.download(ra.PRESSURE)))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0])
+        assert bit_diff(o[1], out[0][1]) == 0 and bit_diff(o[2], out[0][2]) == 0
+
+
 @pytest.mark.parametrize("iters", [1, 2])
 def test_tolerance_mode_preconditioner_matches_oracle(iters):
     """One or two conjugate-gradient iterations of the binary32 tolerance

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define RIBO_ABI_VERSION 3u
+#define RIBO_ABI_VERSION 4u
 
 /* Opaque world handle. */
 typedef struct RiboWorld RiboWorld;
@@ -265,6 +265,12 @@ enum { RIBO_PRECISION_FP32 = 0, RIBO_PRECISION_MIXED = 1 };
  *   SOLID_MASK  u32[F]   fluid wall mask, 1 = solid (download only)
  *   NET_FLUX    f32[2*fluid_size] row means of v_y then column means of v_x
  *   REDOX_EATEN u32[F]   fixed-point reductant eaten since the last fluid step
+ *   FORCE_VECTORS i32[2F] agent forces, fixed point x 256 per fluid cell (x, y):
+ *                        force_vectors (fluid.wgsl:280-288) as agents leave it
+ *                        (fold.wgsl:495-509); added to buoyancy by the next
+ *                        fluid step, then cleared (upload only)
+ *   VELOCITY_SMOOTH f32[2F] smooth_velocity (fluid.wgsl:680-693) of the
+ *                        current velocity, computed at download (download only)
  *   AGG         u32[G]   activated count in each 3x3 block (agg_nb + agg_act;
  *                        download only, valid after a transport pass) */
 typedef enum RiboBuffer {
@@ -282,7 +288,9 @@ typedef enum RiboBuffer {
     RIBO_BUF_NET_FLUX = 11,
     RIBO_BUF_REDOX_EATEN = 12,
     RIBO_BUF_AGG = 13,
-    RIBO_BUF_COUNT = 14
+    RIBO_BUF_FORCE_VECTORS = 14,
+    RIBO_BUF_VELOCITY_SMOOTH = 15,
+    RIBO_BUF_COUNT = 16
 } RiboBuffer;
 
 /* Maximum steps one ribo_world_encode_steps call records
@@ -365,6 +373,46 @@ RIBO_API int ribo_world_mg_tiles(const RiboWorld* w, uint8_t* out, uint32_t cap,
 /* World::invalidate_light (src/world/mod.rs:1400-1402): the next step runs
  * the full light sweep (light_mode 1). */
 RIBO_API int ribo_world_invalidate_light(RiboWorld* w);
+
+/* ---- agent coupling: monomer exchanges (chem.wgsl:40-98) ----------------- */
+
+/* One exchange between the monomer grid and the caller (the life system's
+ * agents): `count` single-monomer atomic steps on channel `channel` of grid
+ * cell `cell`, each of kind:
+ *   RIBO_CHEM_SPEND       activated -> spent in place (chem_spend_one)
+ *   RIBO_CHEM_ACTIVATE    spent -> activated in place (chem_activate_one)
+ *   RIBO_CHEM_TAKE_ACT    remove an activated monomer (chem_take_state_one)
+ *   RIBO_CHEM_TAKE_SPENT  remove a spent monomer
+ *   RIBO_CHEM_ADD_ACT     deposit activated monomers (chem_add_state)
+ *   RIBO_CHEM_ADD_SPENT   deposit spent monomers
+ *   RIBO_CHEM_DEPOSIT_ACT, RIBO_CHEM_DEPOSIT_SPENT  deposit in
+ *                         chem_open_cell(cell): rock sends the deposit to the
+ *                         nearest water or rubble cell (chem.wgsl:76-98)
+ * A step that finds nothing to take stops that exchange. */
+typedef struct RiboChemOp {
+    uint32_t cell;
+    uint32_t channel;
+    uint32_t kind;
+    uint32_t count;
+} RiboChemOp;
+enum {
+    RIBO_CHEM_SPEND = 0,
+    RIBO_CHEM_ACTIVATE = 1,
+    RIBO_CHEM_TAKE_ACT = 2,
+    RIBO_CHEM_TAKE_SPENT = 3,
+    RIBO_CHEM_ADD_ACT = 4,
+    RIBO_CHEM_ADD_SPENT = 5,
+    RIBO_CHEM_DEPOSIT_ACT = 6,
+    RIBO_CHEM_DEPOSIT_SPENT = 7
+};
+/* Applies n exchanges on the device, all at once, between steps (the agents
+ * of the reference act between world kernels in the same frame); done[i]
+ * (may be NULL) receives the monomers exchange i moved.  Exchanges on
+ * different channel words are independent; exchanges that take from the same
+ * word compete as the reference's atomics do.  The per-channel totals the
+ * ledger check expects follow what was taken and deposited (the reference's
+ * Ledger counts it as held by agents). */
+RIBO_API int ribo_world_chem_ops(RiboWorld* w, const RiboChemOp* ops, uint32_t n, uint32_t* done);
 
 /* ---- ledger (Ledger, src/world/mod.rs:39-82; ledger.wgsl:5-27) ---------- */
 

@@ -36,6 +36,7 @@ def lib() -> ctypes.CDLL:
             "oracle_download_f64": (I, [P, I, P, SZ]),
             "oracle_encode_steps": (I, [P, U32]),
             "oracle_ledger": (None, [P, P]),
+            "oracle_chem_ops": (I, [P, P, U32, P]),
             "oracle_run_kernel": (I, [P, ctypes.c_char_p]),
             "oracle_mg_residual": (ctypes.c_double, [P]),
             "oracle_mg_stats": (None, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
@@ -60,7 +61,12 @@ def _ptr(a: np.ndarray):
     return ctypes.c_void_p(a.ctypes.data)
 
 
-_F32 = {2, 3, 4, 5, 6, 7, 8, 9, 11}
+_F32 = {2, 3, 4, 5, 6, 7, 8, 9, 11, 15}
+_I32 = {14}
+
+
+def _dtype(which: int):
+    return np.float32 if which in _F32 else np.int32 if which in _I32 else np.uint32
 
 
 class OracleWorld:
@@ -109,16 +115,17 @@ class OracleWorld:
             raise RuntimeError(f"oracle_set_settings: {rc}")
 
     def upload(self, which: int, data: np.ndarray):
-        dt = np.float32 if which in _F32 else np.uint32
-        a = np.ascontiguousarray(data, dtype=dt)
+        a, dt = np.asarray(data), _dtype(which)
+        if a.dtype != dt and a.dtype.kind in "iu" and np.dtype(dt).kind in "iu" and a.dtype.itemsize == 4:
+            a = a.view(dt)  # 32-bit words: signed and unsigned are the same bits
+        a = np.ascontiguousarray(a, dtype=dt)
         rc = lib().oracle_upload(self._h, which, _ptr(a), a.nbytes)
         if rc != 0:
             raise RuntimeError(f"oracle_upload {which}: {rc}")
 
     def download(self, which: int) -> np.ndarray:
-        dt = np.float32 if which in _F32 else np.uint32
         n = self.ra.buffer_bytes(self.cfg, which) // 4
-        a = np.empty(n, dtype=dt)
+        a = np.empty(n, dtype=_dtype(which))
         rc = lib().oracle_download(self._h, which, _ptr(a), a.nbytes)
         if rc != 0:
             raise RuntimeError(f"oracle_download {which}: {rc}")
@@ -142,6 +149,16 @@ class OracleWorld:
         done = 0
         while done < n:
             done += self.encode_steps(min(64, n - done))
+
+    def chem_ops(self, ops) -> np.ndarray:
+        """Exchanges (cell, channel, kind, count) rows, applied in order;
+        returns the monomers each moved."""
+        a = np.ascontiguousarray(ops, dtype=np.uint32).reshape(-1, 4)
+        done = np.zeros(len(a), dtype=np.uint32)
+        rc = lib().oracle_chem_ops(self._h, _ptr(a), len(a), _ptr(done))
+        if rc != 0:
+            raise RuntimeError(f"oracle_chem_ops: {rc}")
+        return done
 
     def ledger(self):
         out = (ctypes.c_uint32 * 12)()

@@ -1711,11 +1711,32 @@ static size_t oracle_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which) 
         case RIBO_BUF_NET_FLUX: return 8 * (size_t)cfg->fluid_size;
         case RIBO_BUF_REDOX_EATEN: return 4 * F;
         case RIBO_BUF_AGG: return 4 * G;
+        case RIBO_BUF_FORCE_VECTORS: return 8 * F;
+        case RIBO_BUF_VELOCITY_SMOOTH: return 8 * F;
         default: return 0;
     }
 }
 
 namespace {
+
+// smooth_velocity (fluid.wgsl:680-693): the sanitized velocity of the cell
+// and its four neighbours (clamped at the edges), summed in that order, x 0.2.
+template <class R> void smooth_velocity(uint32_t F, const R* vin, R* vout) {
+    auto at = [&](uint32_t x, uint32_t y) { return sanitize(V2<R>{vin[2 * (y * F + x)], vin[2 * (y * F + x) + 1]}); };
+    for (uint32_t y = 0; y < F; ++y)
+        for (uint32_t x = 0; x < F; ++x) {
+            const uint32_t xm = x > 0 ? x - 1 : x, xp = std::min(x + 1, F - 1);
+            const uint32_t ym = y > 0 ? y - 1 : y, yp = std::min(y + 1, F - 1);
+            V2<R> a = at(x, y), b = at(xm, y), c = at(xp, y), d = at(x, ym), e = at(x, yp);
+            R vx = a.x + b.x, vy = a.y + b.y;
+            vx = vx + c.x; vy = vy + c.y;
+            vx = vx + d.x; vy = vy + d.y;
+            vx = vx + e.x; vy = vy + e.y;
+            const R fifth = sizeof(R) == 4 ? R(0.2f) : R(0.2);
+            vout[2 * (y * F + x)] = vx * fifth;
+            vout[2 * (y * F + x) + 1] = vy * fifth;
+        }
+}
 
 size_t buf_bytes(const OracleWorld& w, int which) {
     RiboWorldConfig cfg{w.G, w.F, w.wupc, 0};
@@ -1824,6 +1845,8 @@ int oracle_upload(OracleWorld* w, int which, const void* src, size_t bytes) {
         case RIBO_BUF_LIGHT: memcpy(w->light.data(), src, bytes); w->light_dirty = false; return 0;
         case RIBO_BUF_HEAT_SRC: memcpy(w->heat_src.data(), src, bytes); return 0;
         case RIBO_BUF_REDOX_EATEN: memcpy(w->redox_eaten.data(), src, bytes); return 0;
+        case RIBO_BUF_FORCE_VECTORS: memcpy(w->force_vectors.data(), src, bytes); return 0;  // fold.wgsl:495-509
+        case RIBO_BUF_VELOCITY_SMOOTH: return RIBO_ERR_ARG;
         default: break;
     }
     size_t count;
@@ -1853,6 +1876,16 @@ int oracle_download(const OracleWorld* cw, int which, void* dst, size_t bytes) {
         case RIBO_BUF_REDOX_EATEN: memcpy(dst, w->redox_eaten.data(), bytes); return 0;
         case RIBO_BUF_AGG:
             for (size_t i = 0; i < w->agg_act.size(); ++i) ((uint32_t*)dst)[i] = w->agg_act[i] + w->agg_nb[i];
+            return 0;
+        case RIBO_BUF_FORCE_VECTORS: memcpy(dst, w->force_vectors.data(), bytes); return 0;
+        case RIBO_BUF_VELOCITY_SMOOTH:
+            if (w->fp64) {
+                std::vector<double> v(2 * (size_t)w->F * w->F);
+                smooth_velocity<double>(w->F, w->f64.vel_a.data(), v.data());
+                for (size_t i = 0; i < v.size(); ++i) ((float*)dst)[i] = (float)v[i];
+            } else {
+                smooth_velocity<float>(w->F, w->f32.vel_a.data(), (float*)dst);
+            }
             return 0;
         default: break;
     }
@@ -1947,6 +1980,48 @@ int oracle_run_kernel(OracleWorld* w, const char* name) {
     else if (n == "transport") transport<float>(*w, fl, P);
     else if (n == "thermal_activation") thermal_activation<float>(*w, fl, P);
     else return -1;
+    return 0;
+}
+
+// The exchanges of chem.wgsl:40-98, one after another in the order given
+// (chem_cas, chem_add_state, chem_open_cell).
+int oracle_chem_ops(OracleWorld* w, const RiboChemOp* ops, uint32_t n, uint32_t* done) {
+    const uint64_t cells = (uint64_t)w->G * w->G;
+    for (uint32_t i = 0; i < n; ++i)
+        if (ops[i].cell >= cells || ops[i].channel >= 4u || ops[i].kind > RIBO_CHEM_DEPOSIT_SPENT) return RIBO_ERR_ARG;
+    for (uint32_t i = 0; i < n; ++i) {
+        const RiboChemOp& op = ops[i];
+        uint32_t moved = 0;
+        if (op.kind <= RIBO_CHEM_TAKE_SPENT) {
+            static const uint32_t delta[4] = {CHEM_SPENT_ONE - 1u, 0xFFFF0001u, 0xFFFFFFFFu, 0xFFFF0000u};
+            const bool need_act = op.kind == RIBO_CHEM_SPEND || op.kind == RIBO_CHEM_TAKE_ACT;
+            uint32_t& word = w->chem[(size_t)op.cell * 4 + op.channel];
+            while (moved < op.count) {
+                if (need_act ? (word & CHEM_STATE_MASK) == 0u : (word >> 16) == 0u) break;
+                word += delta[op.kind];
+                ++moved;
+            }
+        } else {
+            uint32_t cell = op.cell;
+            if (op.kind >= RIBO_CHEM_DEPOSIT_ACT && w->gamma[cell] >= GAMMA_SOLID_THRESHOLD) {  // chem_open_cell
+                const int gx = (int)(cell % w->G), gy = (int)(cell / w->G);
+                bool found = false;
+                for (int r = 1; r <= 12 && !found; ++r)
+                    for (int dy = -r; dy <= r && !found; ++dy)
+                        for (int dx = -r; dx <= r && !found; ++dx) {
+                            if (abs(dx) != r && abs(dy) != r) continue;
+                            const int qx = gx + dx, qy = gy + dy;
+                            if (qx < 0 || qy < 0 || qx >= (int)w->G || qy >= (int)w->G) continue;
+                            const uint32_t q = (uint32_t)qy * w->G + (uint32_t)qx;
+                            if (w->gamma[q] < GAMMA_SOLID_THRESHOLD) { cell = q; found = true; }
+                        }
+            }
+            const bool spent = op.kind == RIBO_CHEM_ADD_SPENT || op.kind == RIBO_CHEM_DEPOSIT_SPENT;
+            w->chem[(size_t)cell * 4 + op.channel] += spent ? op.count * CHEM_SPENT_ONE : op.count;
+            moved = op.count;
+        }
+        if (done) done[i] = moved;
+    }
     return 0;
 }
 

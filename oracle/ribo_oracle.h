@@ -59,6 +59,10 @@ int oracle_download(const OracleWorld* w, int which, void* dst, size_t bytes);
 int oracle_download_f64(const OracleWorld* w, int which, double* dst, size_t count);
 /* encode_steps: min(steps, 64) steps; returns the count. */
 int oracle_encode_steps(OracleWorld* w, uint32_t steps);
+// The agents' exchanges with the grid, applied one after another in order
+// (ribo_world_chem_ops applies them at once; the results agree whenever no
+// two exchanges take from the same word beyond what it holds).
+int oracle_chem_ops(OracleWorld* w, const RiboChemOp* ops, uint32_t n, uint32_t* done);
 void oracle_ledger(const OracleWorld* w, uint32_t out[12]);
 /* One reference kernel on the current state (closed-form tests): names
  * build_solid_mask, temperature, forces, diffuse, advect, vorticity,

@@ -128,8 +128,12 @@ void launch_mask_u32(hipStream_t s, const Geo& g, uint32_t* out);
 void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float2* vel,
                         const float* t_in, const float* r_in, float* t_out, float* r_out,
                         uint32_t* eaten, int eaten_active, const uint32_t* chem);
+// agent: the agents' fixed-point force words (i32[2F]), added and cleared,
+// or null.
 void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
-                   float2* forces);
+                   float2* forces, uint32_t* agent);
+// smooth_velocity of vin into vout.
+void launch_smooth_velocity(hipStream_t s, const Geo& g, const float2* vin, float2* vout);
 void launch_add_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* forces,
                        const float2* vin, float2* vout);
 void launch_diffuse(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* vin, float2* vout);
@@ -196,6 +200,11 @@ void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, flo
 // One propagation pass: lout from the rows above in lin (distinct buffers).
 void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
                             float* lout);
+
+// exchanges between the grid and the agents: n RiboChemOp records in device
+// memory (validated), done[n] (may be null), the ledger's expected totals.
+void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t* ops, uint32_t n, uint32_t* done,
+                     uint32_t* expected);
 
 // transport
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e);

@@ -547,9 +547,12 @@ __global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, con
 }
 
 // buoyancy (fluid.wgsl:376-389) into the fixed-point force words, then
-// gather_forces (:392-398).  Without agents each word only ever receives its
-// own cell's buoyancy, so quantise -> dequantise -> clamp happens in place.
-__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T, float2* forces) {
+// gather_forces (:392-398) and clear_force_vectors (:290-296).  A word holds
+// the agents' fixed-point forces (agent, when non-null) plus the cell's own
+// buoyancy; the sum is an integer sum modulo 2^32, as the reference's atomic
+// adds, so the order of the adds does not matter.
+__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T,
+                                                float2* forces, uint32_t* agent) {
     FLUID_XY
     if (ep) P.epoch += *ep;
     int32_t qx = 0, qy = 0;
@@ -565,8 +568,27 @@ __global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const ui
             qy = rb_f2i(rb_rint(rb_clamp(fy, -1.0e5f, 1.0e5f) * FORCE_FP));
         }
     }
+    if (agent) {
+        const uint2 a = reinterpret_cast<uint2*>(agent)[idx];
+        qx = (int32_t)((uint32_t)qx + a.x);
+        qy = (int32_t)((uint32_t)qy + a.y);
+        reinterpret_cast<uint2*>(agent)[idx] = make_uint2(0u, 0u);
+    }
     const float2 f = f2((float)qx / FORCE_FP, (float)qy / FORCE_FP);
     forces[idx] = clamp_len(sanitize(f), MAX_FORCE);
+}
+
+// smooth_velocity (fluid.wgsl:680-693): the mean of the cell and its four
+// neighbours (clamped at the edges), summed in the reference's order.
+__global__ __launch_bounds__(256) void k_smooth_velocity(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout) {
+    FLUID_XY
+    const uint32_t xm = x > 0u ? x - 1u : x, xp = x + 1u < F ? x + 1u : F - 1u;
+    const uint32_t ym = y > 0u ? y - 1u : y, yp = y + 1u < F ? y + 1u : F - 1u;
+    float2 v = add2(sanitize(vin[idx]), sanitize(vin[y * F + xm]));
+    v = add2(v, sanitize(vin[y * F + xp]));
+    v = add2(v, sanitize(vin[ym * F + x]));
+    v = add2(v, sanitize(vin[yp * F + x]));
+    vout[idx] = mul2(v, 0.2f);
 }
 
 // slope_steer_velocity (fluid.wgsl:165-178)
@@ -2122,6 +2144,68 @@ __global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const u
     if (changed) *reinterpret_cast<uint4*>(chem + 4u * idx) = make_uint4(w[0], w[1], w[2], w[3]);
 }
 
+// ---- exchanges with the agents (chem.wgsl:40-98) ---------------------------------------
+// chem_cas: replaces the word by old + delta (modulo 2^32) when it holds an
+// activated (need_act) or spent (need_spent) monomer.
+__device__ __forceinline__ bool chem_cas(uint32_t* slot, uint32_t delta, bool need_act, bool need_spent) {
+    uint32_t old = __hip_atomic_load(slot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    for (;;) {
+        if (need_act && (old & 0xFFFFu) == 0u) return false;
+        if (need_spent && (old >> 16) == 0u) return false;
+        const uint32_t prev = atomicCAS(slot, old, old + delta);
+        if (prev == old) return true;
+        old = prev;
+    }
+}
+
+// chem_open_cell (chem.wgsl:76-98): the cell itself unless it is rock, else
+// the first water or rubble cell on the square rings of radius 1..12 (rows
+// from the top of each ring, cells from the left), else the cell itself.
+__device__ uint32_t chem_open_cell(const uint32_t* __restrict__ gamma, uint32_t G, uint32_t idx) {
+    if (gamma[idx] < GAMMA_SOLID) return idx;
+    const int gx = (int)(idx % G), gy = (int)(idx / G);
+    for (int r = 1; r <= 12; ++r)
+        for (int dy = -r; dy <= r; ++dy)
+            for (int dx = -r; dx <= r; ++dx) {
+                if (abs(dx) != r && abs(dy) != r) continue;
+                const int qx = gx + dx, qy = gy + dy;
+                if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
+                const uint32_t q = (uint32_t)qy * G + (uint32_t)qx;
+                if (gamma[q] < GAMMA_SOLID) return q;
+            }
+    return idx;
+}
+
+// One exchange per thread: ops holds (cell, channel, kind, count) words,
+// validated on the host.  done[i] receives the monomers moved; the ledger's
+// expected channel totals follow what left or entered the grid.
+__global__ __launch_bounds__(256) void k_chem_ops(uint32_t* chem, const uint32_t* __restrict__ gamma, uint32_t G,
+                                                  const uint4* __restrict__ ops, uint32_t n, uint32_t* __restrict__ done,
+                                                  uint32_t* expected) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint4 op = ops[i];
+    const uint32_t cell = op.x, ch = op.y, kind = op.z, count = op.w;
+    uint32_t moved = 0, net = 0;
+    if (kind <= RIBO_CHEM_TAKE_SPENT) {
+        // spend: -1 activated +1 spent; activate: the reverse; take: -1 in that state
+        const uint32_t delta = kind == RIBO_CHEM_SPEND ? CHEM_SPENT_ONE - 1u
+                               : kind == RIBO_CHEM_ACTIVATE ? 0xFFFF0001u
+                               : kind == RIBO_CHEM_TAKE_ACT ? 0xFFFFFFFFu : 0xFFFF0000u;
+        const bool need_act = kind == RIBO_CHEM_SPEND || kind == RIBO_CHEM_TAKE_ACT;
+        uint32_t* slot = chem + (size_t)cell * 4u + ch;
+        while (moved < count && chem_cas(slot, delta, need_act, !need_act)) ++moved;
+        if (kind >= RIBO_CHEM_TAKE_ACT) net = 0u - moved;
+    } else {
+        const bool spent = kind == RIBO_CHEM_ADD_SPENT || kind == RIBO_CHEM_DEPOSIT_SPENT;
+        const uint32_t c = kind >= RIBO_CHEM_DEPOSIT_ACT ? chem_open_cell(gamma, G, cell) : cell;
+        if (count > 0u) atomicAdd(chem + (size_t)c * 4u + ch, spent ? count * CHEM_SPENT_ONE : count);  // chem_add_state
+        moved = net = count;
+    }
+    if (done) done[i] = moved;
+    if (net != 0u) atomicAdd(expected + ch, net);
+}
+
 // ---- transport ------------------------------------------------------------------------
 // agg_count + agg_neighbours (transport.wgsl:61-87) in one pass: activated
 // counts of the tile and its 1-cell ring go to LDS, then each cell sums its
@@ -2643,8 +2727,11 @@ void launch_temperature(hipStream_t s, const Geo& g, const RiboSimParams& P, con
     k_temperature<<<grid2(g.F), B16, 0, s>>>(g, P, ep, vel, t_in, r_in, t_out, r_out, eaten, eaten_active, chem);
 }
 void launch_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
-                   float2* forces) {
-    k_forces<<<grid2(g.F), B16, 0, s>>>(g, P, ep, temp, forces);
+                   float2* forces, uint32_t* agent) {
+    k_forces<<<grid2(g.F), B16, 0, s>>>(g, P, ep, temp, forces, agent);
+}
+void launch_smooth_velocity(hipStream_t s, const Geo& g, const float2* vin, float2* vout) {
+    k_smooth_velocity<<<grid2(g.F), B16, 0, s>>>(g, vin, vout);
 }
 void launch_add_forces(hipStream_t s, const Geo& g, const RiboSimParams& P, const float2* forces, const float2* vin,
                        float2* vout) {
@@ -2808,6 +2895,11 @@ void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, flo
 void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
                             float* lout) {
     k_light_propagate<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, chem, lin, lout);
+}
+void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t* ops, uint32_t n, uint32_t* done,
+                     uint32_t* expected) {
+    if (n == 0) return;
+    k_chem_ops<<<(n + 255u) / 256u, 256, 0, s>>>(chem, g.gamma, g.G, (const uint4*)ops, n, done, expected);
 }
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e) {
     k_agg<<<grid2(g.G), B16, 0, s>>>(g, chem, agg_e);

@@ -121,6 +121,8 @@ struct RiboWorld {
     uint32_t* eaten = nullptr;
     bool eaten_active = false;
     float2* forces = nullptr;
+    uint32_t* agent_force = nullptr;  // force_vectors from the agents, i32[2F] fixed point
+    bool agent_force_active = false;  // consumed (added, then cleared) by the next fluid step
     float2* heat = nullptr;
     float* net_flux = nullptr;
     // terrain-derived
@@ -573,7 +575,8 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     w->eaten_active = false;
     w->tcur = 1 - tc;  // copy_temperature
     const float* T = w->temp[w->tcur];
-    run(w, "buoyancy_gather", [&] { launch_forces(s, g, P, ep, T, w->forces); });
+    run(w, "buoyancy_gather", [&] { launch_forces(s, g, P, ep, T, w->forces, w->agent_force_active ? w->agent_force : nullptr); });
+    w->agent_force_active = false;
     run(w, "add_forces", [&] { launch_add_forces(s, g, P, w->forces, w->vel_a, w->vel_b); });
     run(w, "diffuse_velocity", [&] { launch_diffuse(s, g, P, w->vel_b, w->vel_a); });
     const float decay = decay_factor(P.fluid_decay, P.fluid_dt);
@@ -658,13 +661,13 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
 
 // Graph replay is used when nothing in the recorded work depends on host
 // values that change from step to step: the light-dependent solar heating
-// uses the host-computed sun of each step, and a pending redox_eaten upload
-// is consumed once.
+// uses the host-computed sun of each step, and a pending redox_eaten or
+// force_vectors upload is consumed once.
 bool graph_eligible(const RiboWorld* w) {
     // the light step reads the sun of each step (constant without a day
     // cycle), and a stale light's sweep is a one-off
     const bool light_varies = w->st.light_mode && (w->light_dirty || w->params.day_period >= 1.0f);
-    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !(w->params.sun_heat > 0.0f) && !light_varies;
+    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !w->agent_force_active && !(w->params.sun_heat > 0.0f) && !light_varies;
 }
 
 // Arms the per-step ledger check with the totals of the current grid (when
@@ -754,6 +757,8 @@ size_t ribo_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which) {
         case RIBO_BUF_NET_FLUX: return 8 * (size_t)cfg->fluid_size;
         case RIBO_BUF_REDOX_EATEN: return 4 * F;
         case RIBO_BUF_AGG: return 4 * G;
+        case RIBO_BUF_FORCE_VECTORS: return 8 * F;
+        case RIBO_BUF_VELOCITY_SMOOTH: return 8 * F;
         default: return 0;
     }
 }
@@ -765,6 +770,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     const size_t Ls = cfg->grid_size / LIGHT_DIV, nb = (cfg->fluid_size + 15) / 16;
     size_t b = 2 * a(4 * G, 4) + a(G, 4) + 2 * a(2 * Ls * Ls, 4) + a(G, 8) + a(G, 4);  // chem x2, gamma, light x2, slope, agg
     b += 2 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x3, div, temp/redox x2, eaten, forces, heat
+    b += a(2 * F, 4);  // the agents' force words
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
     b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 7 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
     b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
@@ -829,7 +835,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(w->stream, &w->p_old, F)) ||
         (rc = dalloc(w->stream, &w->div, F)) || (rc = dalloc(w->stream, &w->temp[0], F)) || (rc = dalloc(w->stream, &w->temp[1], F)) ||
         (rc = dalloc(w->stream, &w->redox[0], F)) || (rc = dalloc(w->stream, &w->redox[1], F)) || (rc = dalloc(w->stream, &w->eaten, F)) ||
-        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
+        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->agent_force, 2 * F)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
         (rc = dalloc(w->stream, &w->cflags, F)) || (rc = dalloc(w->stream, &w->normal, F)) || (rc = dalloc(w->stream, &w->fill, F)) ||
         (rc = dalloc(w->stream, &w->dragg, F)) || (rc = dalloc(w->stream, &w->slope_f, F)) || (rc = dalloc(w->stream, &w->pcg, 1)) ||
         (rc = dalloc(w->stream, &w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(w->stream, &w->pcg_r, F)) ||
@@ -868,7 +874,7 @@ int ribo_world_destroy(RiboWorld* w) {
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
     dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light[0]); dfree(w->light[1]); dfree(w->slope); dfree(w->agg_e);
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->p_old); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
-    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->heat); dfree(w->net_flux);
+    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
     dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w);
@@ -946,6 +952,7 @@ int ribo_world_upload(RiboWorld* w, RiboBuffer which, const void* src, size_t by
         case RIBO_BUF_REDOX: dst = w->redox[w->tcur]; break;
         case RIBO_BUF_HEAT_SRC: dst = w->heat; break;
         case RIBO_BUF_REDOX_EATEN: dst = w->eaten; w->eaten_active = true; break;
+        case RIBO_BUF_FORCE_VECTORS: dst = w->agent_force; w->agent_force_active = true; break;
         default: return fail(RIBO_ERR_ARG, "buffer is derived and cannot be uploaded");
     }
     HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, w->stream));
@@ -985,10 +992,16 @@ int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes)
         case RIBO_BUF_NET_FLUX: src = w->net_flux; break;
         case RIBO_BUF_REDOX_EATEN: src = w->eaten; break;
         case RIBO_BUF_AGG: src = w->agg_e; break;
+        case RIBO_BUF_FORCE_VECTORS: src = w->agent_force; break;
         case RIBO_BUF_SOLID_MASK:
             launch_mask_u32(w->stream, w->geo(), w->scratch_u32);
             HIPCHK(hipGetLastError());
             src = w->scratch_u32;
+            break;
+        case RIBO_BUF_VELOCITY_SMOOTH:  // the force buffer is dead between steps
+            launch_smooth_velocity(w->stream, w->geo(), w->vel_a, w->forces);
+            HIPCHK(hipGetLastError());
+            src = w->forces;
             break;
         default: return fail(RIBO_ERR_ARG, "unknown buffer");
     }
@@ -1241,6 +1254,28 @@ int ribo_world_idle(RiboWorld* w) {
     if (e == hipSuccess) return 1;
     if (e == hipErrorNotReady) return 0;
     return fail(RIBO_ERR_HIP, std::string("hipStreamQuery: ") + hipGetErrorString(e));
+}
+
+int ribo_world_chem_ops(RiboWorld* w, const RiboChemOp* ops, uint32_t n, uint32_t* done) {  // chem.wgsl:40-98
+    if (!w || (n > 0 && !ops)) return fail(RIBO_ERR_ARG, "null argument");
+    const uint64_t cells = (uint64_t)w->cfg.grid_size * w->cfg.grid_size;
+    for (uint32_t i = 0; i < n; ++i)
+        if (ops[i].cell >= cells || ops[i].channel >= 4u || ops[i].kind > RIBO_CHEM_DEPOSIT_SPENT)
+            return fail(RIBO_ERR_ARG, "exchange " + std::to_string(i) + ": cell, channel or kind out of range");
+    static_assert(sizeof(RiboChemOp) == 16, "an exchange is four words");
+    // Between steps the force buffer (8 bytes a fluid cell) and the scratch
+    // words are free: the exchanges go through them in batches of F / 2.
+    const uint32_t batch = std::max<uint32_t>((uint32_t)std::min<size_t>((size_t)w->F * w->F / 2, 0xFFFFFFFFu), 1u);
+    const Geo g = w->geo();
+    for (uint32_t b = 0; b < n; b += batch) {
+        const uint32_t m = std::min(batch, n - b);
+        HIPCHK(hipMemcpyAsync(w->forces, ops + b, (size_t)m * sizeof(RiboChemOp), hipMemcpyHostToDevice, w->stream));
+        launch_chem_ops(w->stream, g, w->chem[w->cur], (const uint32_t*)w->forces, m, w->scratch_u32, w->ledger->expected);
+        HIPCHK(hipGetLastError());
+        if (done) HIPCHK(hipMemcpyAsync(done + b, w->scratch_u32, (size_t)m * 4, hipMemcpyDeviceToHost, w->stream));
+    }
+    HIPCHK(hipStreamSynchronize(w->stream));
+    return RIBO_OK;
 }
 
 int ribo_world_ledger(RiboWorld* w, uint32_t out[12]) {  // encode_ledger + ledger_blocking (src/world/mod.rs:1661-1674,1834-1839)

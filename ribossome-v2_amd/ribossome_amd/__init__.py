@@ -234,15 +234,28 @@ WorldConfig.TEST = WorldConfig(256, 128, 30, 4096)
 
 # Buffer ids (RiboBuffer)
 CHEM, GAMMA, LIGHT, SLOPE, VELOCITY, PRESSURE, TEMP, REDOX, HEAT_SRC, DIVERGENCE, SOLID_MASK, NET_FLUX, \
-    REDOX_EATEN, AGG = range(14)
+    REDOX_EATEN, AGG, FORCE_VECTORS, VELOCITY_SMOOTH = range(16)
+BUF_COUNT = 16
 _BUF_DTYPE = {CHEM: np.uint32, GAMMA: np.uint32, LIGHT: np.float32, SLOPE: np.float32, VELOCITY: np.float32,
               PRESSURE: np.float32, TEMP: np.float32, REDOX: np.float32, HEAT_SRC: np.float32,
               DIVERGENCE: np.float32, SOLID_MASK: np.uint32, NET_FLUX: np.float32, REDOX_EATEN: np.uint32,
-              AGG: np.uint32}
+              AGG: np.uint32, FORCE_VECTORS: np.int32, VELOCITY_SMOOTH: np.float32}
 BUFFER_NAMES = {CHEM: "chem", GAMMA: "gamma", LIGHT: "light", SLOPE: "slope", VELOCITY: "velocity",
                 PRESSURE: "pressure", TEMP: "temp", REDOX: "redox", HEAT_SRC: "heat_src",
                 DIVERGENCE: "divergence", SOLID_MASK: "solid_mask", NET_FLUX: "net_flux",
-                REDOX_EATEN: "redox_eaten", AGG: "agg"}
+                REDOX_EATEN: "redox_eaten", AGG: "agg", FORCE_VECTORS: "force_vectors",
+                VELOCITY_SMOOTH: "velocity_smooth"}
+# Exchanges between the grid and the agents (RiboChemOp.kind, chem.wgsl:40-98)
+CHEM_SPEND, CHEM_ACTIVATE, CHEM_TAKE_ACT, CHEM_TAKE_SPENT, CHEM_ADD_ACT, CHEM_ADD_SPENT, CHEM_DEPOSIT_ACT, \
+    CHEM_DEPOSIT_SPENT = range(8)
+FORCE_FP = 256.0  # fixed-point scale of the force words (fluid.wgsl:280)
+
+
+def force_words(f: np.ndarray) -> np.ndarray:
+    """atomic_add_force's quantisation (fluid.wgsl:281-284) of forces f32[2F]
+    into the words of FORCE_VECTORS (one agent's push per cell component)."""
+    f32 = np.asarray(f, dtype=np.float32)
+    return np.rint(np.clip(f32, np.float32(-1.0e5), np.float32(1.0e5)) * np.float32(FORCE_FP)).astype(np.int32)
 MAX_STEPS_PER_FRAME = 64
 
 
@@ -281,6 +294,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_idle": (I, [P]),
         "ribo_world_invalidate_light": (I, [P]),
         "ribo_world_ledger": (I, [P, P]),
+        "ribo_world_chem_ops": (I, [P, P, U32, P]),
         "ribo_world_ledger_status": (I, [P, ctypes.POINTER(U32)]),
         "ribo_world_set_timing": (I, [P, I]),
         "ribo_world_kernel_times": (I, [P, P, P, P, U32, ctypes.POINTER(U32)]),
@@ -499,6 +513,14 @@ class World:
     def read_f32_blocking(self, which: int) -> np.ndarray:
         return self.download(which)
 
+    def chem_ops(self, ops) -> np.ndarray:
+        """The agents' exchanges with the grid: rows (cell, channel, kind,
+        count), applied at once; returns the monomers each moved."""
+        a = np.ascontiguousarray(ops, dtype=np.uint32).reshape(-1, 4)
+        done = np.zeros(len(a), dtype=np.uint32)
+        _check(lib().ribo_world_chem_ops(self._h, _ptr(a), len(a), _ptr(done)), "chem_ops")
+        return done
+
     def ledger_blocking(self) -> Ledger:
         out = (ctypes.c_uint32 * 12)()
         _check(lib().ribo_world_ledger(self._h, out), "ledger")
@@ -593,7 +615,7 @@ def world_footprint(cfg: WorldConfig, settings: "WorldSettings | None" = None) -
 class _CScene(ctypes.Structure):
     _fields_ = [("cfg", _CWorldConfig), ("seed", ctypes.c_uint32), ("steps", ctypes.c_uint32),
                 ("params", SimParams), ("settings", WorldSettings),
-                ("input", ctypes.c_void_p * 14), ("input_bytes", ctypes.c_size_t * 14)]
+                ("input", ctypes.c_void_p * BUF_COUNT), ("input_bytes", ctypes.c_size_t * BUF_COUNT)]
 
 
 class SceneStatus(ctypes.Structure):

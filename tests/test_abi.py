@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 25
     for s in sorted(syms):
         assert hasattr(L, s), s
-    assert L.ribo_abi_version() == 3
+    assert L.ribo_abi_version() == 4
 
 
 def test_struct_layouts_match_reference():
@@ -55,6 +55,8 @@ def test_buffer_sizes():
     assert ra.buffer_bytes(cfg, ra.VELOCITY) == 8 * F
     assert ra.buffer_bytes(cfg, ra.LIGHT) == 8 * 1024 * 1024
     assert ra.buffer_bytes(cfg, ra.NET_FLUX) == 8 * 1024
+    assert ra.buffer_bytes(cfg, ra.FORCE_VECTORS) == 8 * F and ra.buffer_bytes(cfg, ra.VELOCITY_SMOOTH) == 8 * F
+    assert ra.BUF_COUNT == len(ra.BUFFER_NAMES) == 16
     assert ra.buffer_bytes(cfg, 99) == 0
 
 

@@ -37,7 +37,7 @@
 extern "C" {
 #endif
 
-#define RIBO_ABI_VERSION 4u
+#define RIBO_ABI_VERSION 5u
 
 /* Opaque world handle. */
 typedef struct RiboWorld RiboWorld;
@@ -192,9 +192,14 @@ typedef struct RiboMgLevel {
 } RiboMgLevel;
 
 /* WorldSettings (src/world/mod.rs:91-134) plus the solver extensions.
- * terrain_enabled must be 0: grain dynamics are not part of the world-step
- * path, the terrain (gamma) is a frozen input and its slope is derived from
- * it on upload.  Light follows light_mode (an extension): 0 keeps the light
+ * terrain_enabled = 1 runs the grain dynamics each step (relax_gamma_a,
+ * relax_gamma_b, grain_fall when sediment_settle > 0, then the slope;
+ * src/world/mod.rs:1495-1503), in one fixed admissible order of the
+ * reference's concurrent moves (each pass decides on the counts at its
+ * start, then moves the grains, then evicts the monomers that no longer fit,
+ * cell colour (x mod 3, y mod 3) after colour); with 0 the terrain is a
+ * frozen input.  Moving terrain is refused with the tolerance mode (its
+ * exact coarse solve is built per terrain).  Light follows light_mode (an extension): 0 keeps the light
  * field a frozen input (RIBO_BUF_LIGHT), 1 runs the reference's light
  * schedule with light_rows_per_step and light_interval.  contact_enabled is
  * accepted and ignored (agents are not part of the path). */
@@ -208,7 +213,7 @@ typedef struct RiboWorldSettings {
                                      (at most 16; reference default 1); 0 = the
                                      full sweep every light_interval steps */
     uint32_t light_interval;      /* reference default 10 */
-    uint32_t terrain_enabled;     /* must be 0 */
+    uint32_t terrain_enabled;     /* grain dynamics; reference default 1, here 0 */
     uint32_t contact_enabled;     /* ignored */
     /* --- extensions (not in the reference) --- */
     uint32_t mg_min_size;         /* coarsest level bound, reference MG_MIN_SIZE = 8 */
@@ -302,7 +307,7 @@ typedef enum RiboBuffer {
 /* SimParams::default() (src/params.rs:553-690), with seed = 1. */
 RIBO_API void ribo_sim_params_default(RiboSimParams* out);
 /* WorldSettings::default() (src/world/mod.rs:121-134) except terrain_enabled
- * = 0, plus mg_min_size 8, mg_tol 0, mg_max_cycles 1, precision FP32. */
+ * = 0 (the frozen terrain of the benchmark's profile), plus mg_min_size 8, mg_tol 0, mg_max_cycles 1, precision FP32. */
 RIBO_API void ribo_world_settings_default(RiboWorldSettings* out);
 /* Byte size of a buffer for a config (0 for an unknown buffer). */
 RIBO_API size_t ribo_buffer_bytes(const RiboWorldConfig* cfg, RiboBuffer which);

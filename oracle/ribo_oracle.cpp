@@ -1682,7 +1682,302 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
     thermal_activation<R>(w, fl, P);
 }
 
+}  // namespace
+extern "C" {
+static void compute_slope(OracleWorld* w);
+}
+namespace {
+
+// ---- terrain grain dynamics (terrain.wgsl:38-102,159-430) -------------------
+// The reference runs every cell's decision at once, each reading its
+// neighbours' grain counts while other cells move grains; no order of those
+// reads is fixed.  This restatement fixes one admissible order, the same as
+// the HIP kernels': every decision of a pass reads the counts at the start
+// of the pass; then every grain moves (a cell only ever gives its own grain
+// away, so each take succeeds); then the monomers that no longer fit leave
+// the cells that received grains, cell colour (x mod 3, y mod 3) after cell
+// colour, and within a cell the grains from its eight neighbours in row
+// order.  Cells of one colour are 3 apart, so their evictions touch
+// disjoint cells and their order does not matter.
+constexpr uint32_t SANDPILE_DIFF = 3u, GAMMA_ROCK_FULL = 6u;
+constexpr uint32_t S_RELAX = 16u, S_SAND = 32u;
+constexpr float GAMMA_RELAX_P = 0.12f, GAMMA_STRAY_WALK_P = 0.05f, GAMMA_SEDIMENT_FACTOR = 0.5f;
+constexpr float GAMMA_SETTLE_SPEED = 0.5f, VENT_BLOW = 8.0f, GAMMA_SHED_P = 0.03f, GRAIN_ADV_CAP_T = 0.9f;
+
+float hop_p(const OracleWorld& w, const RiboSimParams& P, float vx, float vy) {  // parcel_hop_p, transport.wgsl:54-58
+    const float env_per_fluid = (float)w.G / (float)w.F;
+    const float l1 = fabsf(vx) + fabsf(vy);
+    return rb_clamp(l1 * env_per_fluid * rb_max(P.dt, 1e-3f), 0.0f, GRAIN_ADV_CAP_T);
+}
+
+// evict_target (terrain.wgsl:49-66)
+uint32_t evict_target(const OracleWorld& w, uint32_t src, uint32_t dst) {
+    const uint32_t* ch = w.chem.data();
+    const uint32_t* g = w.gamma.data();
+    if (cell_total(ch, src) < capacity(g, src)) return src;
+    const int x = (int)(dst % w.G), y = (int)(dst / w.G);
+    const int nb[4][2] = {{x + 1, y}, {x - 1, y}, {x, y + 1}, {x, y - 1}};
+    uint32_t not_rock = g[src] < GAMMA_SOLID_THRESHOLD ? src : 0xFFFFFFFFu;
+    for (int i = 0; i < 4; ++i) {
+        if (nb[i][0] < 0 || nb[i][1] < 0 || nb[i][0] >= (int)w.G || nb[i][1] >= (int)w.G) continue;
+        const uint32_t n = (uint32_t)nb[i][1] * w.G + (uint32_t)nb[i][0];
+        if (cell_total(ch, n) < capacity(g, n)) return n;
+        if (not_rock == 0xFFFFFFFFu && g[n] < GAMMA_SOLID_THRESHOLD) not_rock = n;
+    }
+    return not_rock == 0xFFFFFFFFu ? src : not_rock;
+}
+
+// The eviction loop of gamma_move_one (terrain.wgsl:76-96), after the grain
+// has arrived in dst.
+void evict(OracleWorld& w, uint32_t src, uint32_t dst) {
+    const uint32_t cap = capacity(w.gamma.data(), dst);
+    for (uint32_t guard = 0; guard < 64u; ++guard) {
+        if (cell_total(w.chem.data(), dst) <= cap) break;
+        const uint32_t to = evict_target(w, src, dst);
+        bool moved = false;
+        for (uint32_t c = 0; c < 4u && !moved; ++c) {
+            uint32_t& slot = w.chem[(size_t)dst * 4 + c];
+            if ((slot >> 16) != 0u) {  // chem_take_state_one(slot, spent)
+                slot -= CHEM_SPENT_ONE;
+                w.chem[(size_t)to * 4 + c] += CHEM_SPENT_ONE;
+                moved = true;
+            } else if ((slot & CHEM_STATE_MASK) != 0u) {
+                slot -= 1u;
+                w.chem[(size_t)to * 4 + c] += 1u;
+                moved = true;
+            }
+        }
+        if (!moved) break;
+    }
+}
+
+// compaction_stable (terrain.wgsl:139-157) on the pass's counts g.
+bool compaction_stable(const OracleWorld& w, const uint32_t* g, uint32_t src, uint32_t dst, uint32_t n, uint32_t c) {
+    const int d4[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+    bool ok = true;
+    const int G = (int)w.G;
+    for (int k = 0; k < 4; ++k) {
+        const int qdx = (int)(dst % w.G) + d4[k][0], qdy = (int)(dst / w.G) + d4[k][1];
+        if (qdx >= 0 && qdy >= 0 && qdx < G && qdy < G) {
+            const uint32_t qi = (uint32_t)qdy * w.G + (uint32_t)qdx;
+            const uint32_t low = qi == src ? n - 1u : g[qi];
+            if (c + 1u >= low + SANDPILE_DIFF) ok = false;
+        }
+        const int qsx = (int)(src % w.G) + d4[k][0], qsy = (int)(src / w.G) + d4[k][1];
+        if (qsx >= 0 && qsy >= 0 && qsx < G && qsy < G) {
+            const uint32_t qi = (uint32_t)qsy * w.G + (uint32_t)qsx;
+            const uint32_t high = qi == dst ? c + 1u : g[qi];
+            if (high >= n - 1u + SANDPILE_DIFF) ok = false;
+        }
+    }
+    return ok;
+}
+
+// The decision of relax_gamma_pass (terrain.wgsl:159-406) for one cell:
+// where its grain goes (idx itself for none).
+uint32_t relax_decide(const OracleWorld& w, const RiboSimParams& P, const uint32_t* g, const float* vel, uint32_t idx,
+                      uint32_t phase) {
+    const uint32_t G = w.G, F = w.F, x = idx % G, y = idx / G;
+    const uint32_t n = g[idx];
+    if (n >= 1u && n < SANDPILE_DIFF) {
+        uint32_t bonds = 0, best_idx = idx, best_c = 0xFFFFu, em_idx = idx;
+        int best_score = -1, em_score = -1;
+        bool em_found = false;
+        uint32_t rr[4] = {idx, P.epoch, S_RELAX + phase, P.seed};
+        pcg4d(rr);
+        uint32_t tie_h = rr[0];
+        uint32_t comp_idx = idx, comp_c = 0;
+        const float comp_roll = (float)(hash_u32(rr[0] ^ 0x5bd1e995u) >> 8) * (1.0f / 16777216.0f);
+        const bool try_comp = comp_roll < P.sediment_compaction;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int nx = (int)x + dx, ny = (int)y + dy;
+                if (nx < 0 || ny < 0 || nx >= (int)G || ny >= (int)G) continue;
+                const uint32_t ni = (uint32_t)ny * G + (uint32_t)nx;
+                const uint32_t c = g[ni];
+                if (c >= 1u) bonds += 1u;
+                if (try_comp && c >= 1u && c < GAMMA_ROCK_FULL &&
+                    (c > comp_c || (c == comp_c && (hash_u32(tie_h ^ (ni * 31u)) & 1u) == 1u)) &&
+                    compaction_stable(w, g, idx, ni, n, c)) {
+                    comp_c = c;
+                    comp_idx = ni;
+                }
+                if (c <= n) {
+                    int score = 0;
+                    for (int sy = -1; sy <= 1; ++sy)
+                        for (int sx = -1; sx <= 1; ++sx) {
+                            if (sx == 0 && sy == 0) continue;
+                            const int qx = nx + sx, qy = ny + sy;
+                            if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
+                            const uint32_t qi = (uint32_t)qy * G + (uint32_t)qx;
+                            if (qi == idx) continue;
+                            if (g[qi] >= 1u) score += 1;
+                        }
+                    tie_h = hash_u32(tie_h ^ ni);
+                    if (score > best_score || (score == best_score && c < best_c) ||
+                        (score == best_score && c == best_c && (tie_h & 1u) == 1u)) {
+                        best_score = score;
+                        best_c = c;
+                        best_idx = ni;
+                    }
+                    if (c == 0u && (score > em_score || (score == em_score && (tie_h & 2u) == 2u))) {
+                        em_score = score;
+                        em_idx = ni;
+                        em_found = true;
+                    }
+                }
+            }
+        const float mob = ldexpf(1.0f, -2 * (int)bonds);  // pow(0.25, bonds), exact
+        if (try_comp && bonds >= 3u && comp_idx != idx &&
+            comp_roll < P.sediment_compaction * (float)(bonds - 2u) / 6.0f)
+            return comp_idx;
+        float vsx = 0.0f, vsy = 0.0f;
+        if (P.fluid_enabled != 0u) {
+            const int nb4[4][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}};
+            for (int k = 0; k < 4; ++k) {
+                const int qx = (int)x + nb4[k][0], qy = (int)y + nb4[k][1];
+                if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
+                if (g[(uint32_t)qy * G + (uint32_t)qx] != 0u) continue;
+                const uint32_t fxi = std::min(((uint32_t)qx * F) / G, F - 1u), fyi = std::min(((uint32_t)qy * F) / G, F - 1u);
+                float vx = vel[2 * (fyi * F + fxi)], vy = vel[2 * (fyi * F + fxi) + 1];
+                if (is_bad(vx) || is_bad(vy)) vx = vy = 0.0f;  // sanitize_vec2
+                if (vx * vx + vy * vy > vsx * vsx + vsy * vsy) { vsx = vx; vsy = vy; }
+            }
+        }
+        const bool unsupported = y > 0u && g[idx - G] == 0u;
+        float mvx = 0.0f, mvy = 0.0f, p_sed = 0.0f;
+        if (unsupported && bonds == 0u) {
+            mvx = vsx; mvy = vsy;
+            p_sed = hop_p(w, P, mvx, mvy);
+        } else {
+            const float speed = rb_len2(vsx, vsy);
+            const float crit = rb_max(P.sediment_threshold, 0.0f) * (1.0f + rb_max(P.sediment_cohesion, 0.0f) * (float)bonds);
+            const float excess = rb_max(speed - crit, 0.0f);
+            if (speed > 1e-6f) {
+                const float k = excess / rb_max(speed, 1e-6f);
+                mvx = vsx * k; mvy = vsy * k;
+            }
+            p_sed = hop_p(w, P, mvx, mvy) * GAMMA_SEDIMENT_FACTOR * rb_max(P.sediment_transport, 0.0f);
+        }
+        const uint32_t hf = std::min((y * F) / G, F - 1u) * F + std::min((x * F) / G, F - 1u);
+        const float blow = VENT_BLOW * rb_max(w.heat_src[2 * hf], w.heat_src[2 * hf + 1]);
+        if (blow > 0.0f) {
+            mvx = ((rr[1] & 1u) == 1u ? 0.6f : -0.6f) * blow;
+            mvy = 1.0f * blow;
+            p_sed = rb_max(p_sed, hop_p(w, P, mvx, mvy));
+        }
+        if (bonds >= 4u && p_sed <= 0.0f) return idx;
+        const uint32_t hw = rr[1];
+        const float rw = (float)(rr[2] >> 8) * (1.0f / 16777216.0f);
+        const float rw2 = (float)(rr[3] >> 8) * (1.0f / 16777216.0f);
+        uint32_t dest = idx;
+        if (n >= 2u && em_found && rw2 < GAMMA_SHED_P * rb_clamp(P.sediment_settle, 0.0f, 1.0f)) {
+            dest = em_idx;
+        } else if (rw < p_sed) {
+            int ddx = 0, ddy = 0;
+            const float ax = fabsf(mvx), ay = fabsf(mvy);
+            if ((float)(hw >> 8) * (1.0f / 16777216.0f) < ax / rb_max(ax + ay, 1e-6f)) ddx = mvx > 0.0f ? 1 : -1;
+            else ddy = mvy > 0.0f ? 1 : -1;
+            const int wx = (int)x + ddx, wy = (int)y + ddy;
+            if (wx >= 0 && wy >= 0 && wx < (int)G && wy < (int)G) {
+                const uint32_t wi = (uint32_t)wy * G + (uint32_t)wx;
+                if (g[wi] <= n) dest = wi;
+            }
+        } else if (rw < p_sed + GAMMA_STRAY_WALK_P * mob) {
+            if (best_score > (int)bonds) {
+                dest = best_idx;
+            } else if (bonds == 0u) {
+                static const int dir[8][2] = {{1, 0}, {-1, 0}, {0, 1}, {0, -1}, {1, 1}, {-1, 1}, {1, -1}, {-1, -1}};
+                const uint32_t k = (hw >> 8) % 8u;
+                const int wx = (int)x + dir[k][0], wy = (int)y + dir[k][1];
+                if (wx >= 0 && wy >= 0 && wx < (int)G && wy < (int)G) {
+                    const uint32_t wi = (uint32_t)wy * G + (uint32_t)wx;
+                    if (g[wi] <= n) dest = wi;
+                }
+            }
+        }
+        return dest;
+    }
+    if (n < SANDPILE_DIFF) return idx;
+    uint32_t best_idx = idx, best_count = n;
+    if (x > 0u && g[idx - 1u] < best_count) { best_count = g[idx - 1u]; best_idx = idx - 1u; }
+    if (x + 1u < G && g[idx + 1u] < best_count) { best_count = g[idx + 1u]; best_idx = idx + 1u; }
+    if (y > 0u && g[idx - G] < best_count) { best_count = g[idx - G]; best_idx = idx - G; }
+    if (y + 1u < G && g[idx + G] < best_count) { best_count = g[idx + G]; best_idx = idx + G; }
+    if (best_idx == idx || n - best_count < SANDPILE_DIFF) return idx;
+    const float p = GAMMA_RELAX_P * rb_max(rb_clamp(P.sediment_settle, 0.0f, 1.0f), rb_clamp(P.rock_crumble, 0.0f, 1.0f));
+    if (rng_f4(idx, P.epoch, S_SAND + phase, P.seed).x < p) return best_idx;
+    return idx;
+}
+
+// One relax pass (relax_gamma_a / relax_gamma_b) in the order described above.
+void relax_pass(OracleWorld& w, const RiboSimParams& P, const float* vel, uint32_t phase) {
+    const uint32_t G = w.G;
+    const std::vector<uint32_t> g0 = w.gamma;
+    std::vector<uint32_t> dest((size_t)G * G);
+    for (uint32_t i = 0; i < G * G; ++i) dest[i] = relax_decide(w, P, g0.data(), vel, i, phase);
+    for (uint32_t i = 0; i < G * G; ++i)
+        if (dest[i] != i) { w.gamma[i] -= 1u; w.gamma[dest[i]] += 1u; }
+    for (uint32_t color = 0; color < 9u; ++color)
+        for (uint32_t y = color / 3u; y < G; y += 3u)
+            for (uint32_t x = color % 3u; x < G; x += 3u) {
+                const uint32_t d = y * G + x;
+                for (int dy = -1; dy <= 1; ++dy)
+                    for (int dx = -1; dx <= 1; ++dx) {
+                        if (dx == 0 && dy == 0) continue;
+                        const int sx = (int)x + dx, sy = (int)y + dy;
+                        if (sx < 0 || sy < 0 || sx >= (int)G || sy >= (int)G) continue;
+                        const uint32_t src = (uint32_t)sy * G + (uint32_t)sx;
+                        if (dest[src] == d) evict(w, src, d);
+                    }
+            }
+}
+
+// grain_fall (terrain.wgsl:405-437): one column at a time, columns of
+// colour x mod 3 = 0, 1, 2 in turn (their evictions touch columns x - 1 .. x + 1).
+void grain_fall(OracleWorld& w, const RiboSimParams& P) {
+    const uint32_t G = w.G;
+    const float w_s = GAMMA_SETTLE_SPEED * rb_max(P.sediment_settle, 0.0f);
+    const float rate = hop_p(w, P, 0.0f, w_s);
+    if (rate <= 0.0f) return;
+    const uint32_t rf = rb_f2u(rate * 65536.0f);
+    if (((P.epoch * rf) >> 16) == (((P.epoch + 1u) * rf) >> 16)) return;
+    for (uint32_t color = 0; color < 3u; ++color)
+        for (uint32_t x = color; x < G; x += 3u) {
+            bool supported = true;
+            for (uint32_t y = 0; y < G; ++y) {
+                const uint32_t idx = y * G + x;
+                const uint32_t n = w.gamma[idx];
+                if (n == 0u) { supported = false; continue; }
+                if (n >= GAMMA_SOLID_THRESHOLD) { supported = true; continue; }
+                if (supported) continue;
+                for (uint32_t k = 0; k < n; ++k) {
+                    w.gamma[idx] -= 1u;
+                    w.gamma[idx - G] += 1u;
+                    evict(w, idx, idx - G);
+                }
+            }
+        }
+}
+
+// The terrain stage of a step (src/world/mod.rs:1495-1503).
+void terrain_step(OracleWorld& w, const float* vel, const RiboSimParams& P) {
+    relax_pass(w, P, vel, 0u);
+    relax_pass(w, P, vel, 1u);
+    if (P.sediment_settle > 0.0f) grain_fall(w, P);
+    compute_slope(&w);
+}
+
 template <class R> void world_step(OracleWorld& w, Fluid<R>& fl, const RiboSimParams& P, uint32_t substep) {
+    if (w.st.terrain_enabled) {
+        if (sizeof(R) == 4) {
+            terrain_step(w, (const float*)fl.vel_a.data(), P);
+        } else {
+            std::vector<float> v(fl.vel_a.begin(), fl.vel_a.end());
+            terrain_step(w, v.data(), P);
+        }
+    }
     light_step(w, P);
     if (w.st.fluid_enabled && P.epoch % substep == 0) fluid_step<R>(w, fl, P);
     if (P.epoch % std::max(P.transport_every, 1u) == 0) transport<R>(w, fl, P);
@@ -1796,7 +2091,7 @@ void oracle_set_params(OracleWorld* w, const RiboSimParams* p) { w->params = *p;
 void oracle_get_params(const OracleWorld* w, RiboSimParams* p) { *p = w->params; }
 
 int oracle_set_settings(OracleWorld* w, const RiboWorldSettings* s) {
-    if (s->terrain_enabled) return RIBO_ERR_UNSUPPORTED;
+    if (s->terrain_enabled && s->multigrid && s->mg_tol > 0.0f) return RIBO_ERR_UNSUPPORTED;
     if (s->mg_min_size < 1 || s->light_mode > 1u) return RIBO_ERR_ARG;
     w->st = *s;
     w->f32.hist = w->f64.hist = 0;  // new settings start the pressure history again
@@ -1958,7 +2253,9 @@ int oracle_run_kernel(OracleWorld* w, const char* name) {
     const RiboSimParams& P = w->params;
     Ctx c{w, w->G, w->F, &P};
     const std::string n(name);
-    if (n == "build_solid_mask") build_solid_mask<float>(*w, c);
+    if (n == "relax_a" || n == "relax_b") relax_pass(*w, P, fl.vel_a.data(), n == "relax_b" ? 1u : 0u);
+    else if (n == "grain_fall") grain_fall(*w, P);
+    else if (n == "build_solid_mask") build_solid_mask<float>(*w, c);
     else if (n == "temperature") { update_temperature<float>(*w, fl, c); copy_temperature<float>(*w, fl); }
     else if (n == "forces") {
         buoyancy<float>(*w, fl, c);

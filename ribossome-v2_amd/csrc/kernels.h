@@ -206,6 +206,14 @@ void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P,
 void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t* ops, uint32_t n, uint32_t* done,
                      uint32_t* expected);
 
+// terrain grain dynamics: one relax pass (phase 0 or 1: decisions on the
+// pass's counts into code[G], the grain moves, the colour-ordered
+// evictions) and grain_fall (three column colours; a no-op off the clock)
+void launch_relax_pass(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t phase,
+                       const float2* vel, uint32_t* gamma, uint32_t* chem, uint8_t* code);
+void launch_grain_fall(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t* gamma,
+                       uint32_t* chem);
+
 // transport
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e);
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);

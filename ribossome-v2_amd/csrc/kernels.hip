@@ -2206,6 +2206,315 @@ __global__ __launch_bounds__(256) void k_chem_ops(uint32_t* chem, const uint32_t
     if (net != 0u) atomicAdd(expected + ch, net);
 }
 
+// ---- terrain grain dynamics (terrain.wgsl:38-102,159-437) -------------------------------
+// The reference decides every cell at once, each reading its neighbours'
+// grain counts while other cells move grains; the order of those reads is
+// not fixed.  These kernels fix one admissible order: every decision of a
+// pass reads the counts at the start of the pass (k_relax_decide writes a
+// move code per cell), every grain then moves (k_relax_apply: a cell only
+// gives its own grain away, so every take succeeds), and the monomers that
+// no longer fit a cell that received grains leave it in k_relax_evict, one
+// launch per cell colour (x mod 3, y mod 3): cells of one colour are 3
+// apart, so their evictions touch disjoint cells and need no atomics.
+constexpr uint32_t SANDPILE_DIFF = 3u, GAMMA_ROCK_FULL = 6u, S_RELAX = 16u, S_SAND = 32u;
+constexpr float GAMMA_RELAX_P = 0.12f, GAMMA_STRAY_WALK_P = 0.05f, GAMMA_SEDIMENT_FACTOR = 0.5f;
+constexpr float GAMMA_SETTLE_SPEED = 0.5f, VENT_BLOW = 8.0f, GAMMA_SHED_P = 0.03f, GRAIN_ADV_CAP = 0.9f;
+
+// The eight neighbours in row order; move code k + 1 means neighbour k.
+__constant__ int8_t NB8_DX[8] = {-1, 0, 1, -1, 1, -1, 0, 1};
+__constant__ int8_t NB8_DY[8] = {-1, -1, -1, 0, 0, 1, 1, 1};
+__device__ __forceinline__ uint32_t nb8_code(int dx, int dy) {  // dx, dy in -1..1, not both 0
+    const int k = (dy + 1) * 3 + (dx + 1);
+    return (uint32_t)(k < 4 ? k + 1 : k);
+}
+
+__device__ __forceinline__ uint32_t mix_hash(uint32_t v) {  // hash (rng.wgsl:46-50)
+    uint32_t x = v;
+    x = x ^ (x >> 16); x = x * 0x7feb352du; x = x ^ (x >> 15); x = x * 0x846ca68bu; x = x ^ (x >> 16);
+    return x;
+}
+// parcel_hop_p (transport.wgsl:54-58)
+__device__ __forceinline__ float hop_p(const Geo& g, const RiboSimParams& P, float vx, float vy) {
+    const float env_per_fluid = (float)g.G / (float)g.F;
+    const float l1 = rb_abs(vx) + rb_abs(vy);
+    return rb_clamp(l1 * env_per_fluid * rb_max(P.dt, 1e-3f), 0.0f, GRAIN_ADV_CAP);
+}
+
+// compaction_stable (terrain.wgsl:139-157) on the pass's counts.
+__device__ bool compaction_stable(const uint32_t* __restrict__ gm, uint32_t G, uint32_t src, uint32_t dst, uint32_t n,
+                                  uint32_t c) {
+    bool ok = true;
+    const int sx = (int)(src % G), sy = (int)(src / G), tx = (int)(dst % G), ty = (int)(dst / G);
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int ddx = k == 0 ? 1 : k == 1 ? -1 : 0, ddy = k == 2 ? 1 : k == 3 ? -1 : 0;
+        const int qdx = tx + ddx, qdy = ty + ddy;
+        if (qdx >= 0 && qdy >= 0 && qdx < (int)G && qdy < (int)G) {
+            const uint32_t qi = (uint32_t)qdy * G + (uint32_t)qdx;
+            const uint32_t low = qi == src ? n - 1u : gm[qi];
+            if (c + 1u >= low + SANDPILE_DIFF) ok = false;
+        }
+        const int qsx = sx + ddx, qsy = sy + ddy;
+        if (qsx >= 0 && qsy >= 0 && qsx < (int)G && qsy < (int)G) {
+            const uint32_t qi = (uint32_t)qsy * G + (uint32_t)qsx;
+            const uint32_t high = qi == dst ? c + 1u : gm[qi];
+            if (high >= n - 1u + SANDPILE_DIFF) ok = false;
+        }
+    }
+    return ok;
+}
+
+// relax_gamma_pass's decision (terrain.wgsl:159-406): the cell's move code.
+__global__ __launch_bounds__(256) void k_relax_decide(Geo g, RiboSimParams P, const uint32_t* ep, uint32_t phase,
+                                                      const float2* __restrict__ vel, uint8_t* __restrict__ code) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const uint32_t G = g.G, F = g.F;
+    if (x >= G || y >= G) return;
+    if (ep) P.epoch += *ep;
+    const uint32_t* __restrict__ gm = g.gamma;
+    const uint32_t idx = y * G + x;
+    const uint32_t n = gm[idx];
+    uint32_t dest = idx;
+    if (n >= 1u && n < SANDPILE_DIFF) {
+        uint32_t bonds = 0, best_idx = idx, best_c = 0xFFFFu, em_idx = idx;
+        int best_score = -1, em_score = -1;
+        bool em_found = false;
+        const U4 rr = pcg4d({idx, P.epoch, S_RELAX + phase, P.seed});
+        uint32_t tie_h = rr.x;
+        uint32_t comp_idx = idx, comp_c = 0;
+        const float comp_roll = (float)(mix_hash(rr.x ^ 0x5bd1e995u) >> 8) * (1.0f / 16777216.0f);
+        const bool try_comp = comp_roll < P.sediment_compaction;
+        for (int dy = -1; dy <= 1; ++dy)
+            for (int dx = -1; dx <= 1; ++dx) {
+                if (dx == 0 && dy == 0) continue;
+                const int nx = (int)x + dx, ny = (int)y + dy;
+                if (nx < 0 || ny < 0 || nx >= (int)G || ny >= (int)G) continue;
+                const uint32_t ni = (uint32_t)ny * G + (uint32_t)nx;
+                const uint32_t c = gm[ni];
+                if (c >= 1u) bonds += 1u;
+                if (try_comp && c >= 1u && c < GAMMA_ROCK_FULL &&
+                    (c > comp_c || (c == comp_c && (mix_hash(tie_h ^ (ni * 31u)) & 1u) == 1u)) &&
+                    compaction_stable(gm, G, idx, ni, n, c)) {
+                    comp_c = c;
+                    comp_idx = ni;
+                }
+                if (c <= n) {
+                    int score = 0;
+                    for (int sy = -1; sy <= 1; ++sy)
+                        for (int sx = -1; sx <= 1; ++sx) {
+                            if (sx == 0 && sy == 0) continue;
+                            const int qx = nx + sx, qy = ny + sy;
+                            if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
+                            const uint32_t qi = (uint32_t)qy * G + (uint32_t)qx;
+                            if (qi == idx) continue;
+                            if (gm[qi] >= 1u) score += 1;
+                        }
+                    tie_h = mix_hash(tie_h ^ ni);
+                    if (score > best_score || (score == best_score && c < best_c) ||
+                        (score == best_score && c == best_c && (tie_h & 1u) == 1u)) {
+                        best_score = score;
+                        best_c = c;
+                        best_idx = ni;
+                    }
+                    if (c == 0u && (score > em_score || (score == em_score && (tie_h & 2u) == 2u))) {
+                        em_score = score;
+                        em_idx = ni;
+                        em_found = true;
+                    }
+                }
+            }
+        const float mob = __builtin_ldexpf(1.0f, -2 * (int)bonds);  // pow(0.25, bonds), exact
+        if (try_comp && bonds >= 3u && comp_idx != idx &&
+            comp_roll < P.sediment_compaction * (float)(bonds - 2u) / 6.0f) {
+            dest = comp_idx;
+        } else {
+            float vsx = 0.0f, vsy = 0.0f;
+            if (P.fluid_enabled != 0u) {
+#pragma unroll
+                for (int k = 0; k < 4; ++k) {
+                    const int qx = (int)x + (k == 0 ? 1 : k == 1 ? -1 : 0), qy = (int)y + (k == 2 ? 1 : k == 3 ? -1 : 0);
+                    if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
+                    if (gm[(uint32_t)qy * G + (uint32_t)qx] != 0u) continue;
+                    const uint32_t fxi = rb_minu(((uint32_t)qx * F) / G, F - 1u), fyi = rb_minu(((uint32_t)qy * F) / G, F - 1u);
+                    const float2 v = sanitize(vel[fyi * F + fxi]);
+                    if (v.x * v.x + v.y * v.y > vsx * vsx + vsy * vsy) { vsx = v.x; vsy = v.y; }
+                }
+            }
+            const bool unsupported = y > 0u && gm[idx - G] == 0u;
+            float mvx = 0.0f, mvy = 0.0f, p_sed;
+            if (unsupported && bonds == 0u) {
+                mvx = vsx; mvy = vsy;
+                p_sed = hop_p(g, P, mvx, mvy);
+            } else {
+                const float speed = rb_len2(vsx, vsy);
+                const float crit = rb_max(P.sediment_threshold, 0.0f) * (1.0f + rb_max(P.sediment_cohesion, 0.0f) * (float)bonds);
+                const float excess = rb_max(speed - crit, 0.0f);
+                if (speed > 1e-6f) {
+                    const float k = excess / rb_max(speed, 1e-6f);
+                    mvx = vsx * k; mvy = vsy * k;
+                }
+                p_sed = hop_p(g, P, mvx, mvy) * GAMMA_SEDIMENT_FACTOR * rb_max(P.sediment_transport, 0.0f);
+            }
+            const float2 vent = g.heat[rb_minu((y * F) / G, F - 1u) * F + rb_minu((x * F) / G, F - 1u)];
+            const float blow = VENT_BLOW * rb_max(vent.x, vent.y);
+            if (blow > 0.0f) {
+                mvx = ((rr.y & 1u) == 1u ? 0.6f : -0.6f) * blow;
+                mvy = 1.0f * blow;
+                p_sed = rb_max(p_sed, hop_p(g, P, mvx, mvy));
+            }
+            if (!(bonds >= 4u && p_sed <= 0.0f)) {
+                const uint32_t hw = rr.y;
+                const float rw = (float)(rr.z >> 8) * (1.0f / 16777216.0f);
+                const float rw2 = (float)(rr.w >> 8) * (1.0f / 16777216.0f);
+                if (n >= 2u && em_found && rw2 < GAMMA_SHED_P * rb_clamp(P.sediment_settle, 0.0f, 1.0f)) {
+                    dest = em_idx;
+                } else if (rw < p_sed) {
+                    int ddx = 0, ddy = 0;
+                    const float ax = rb_abs(mvx), ay = rb_abs(mvy);
+                    if ((float)(hw >> 8) * (1.0f / 16777216.0f) < ax / rb_max(ax + ay, 1e-6f)) ddx = mvx > 0.0f ? 1 : -1;
+                    else ddy = mvy > 0.0f ? 1 : -1;
+                    const int wx = (int)x + ddx, wy = (int)y + ddy;
+                    if (wx >= 0 && wy >= 0 && wx < (int)G && wy < (int)G) {
+                        const uint32_t wi = (uint32_t)wy * G + (uint32_t)wx;
+                        if (gm[wi] <= n) dest = wi;
+                    }
+                } else if (rw < p_sed + GAMMA_STRAY_WALK_P * mob) {
+                    if (best_score > (int)bonds) {
+                        dest = best_idx;
+                    } else if (bonds == 0u) {
+                        const uint32_t k = (hw >> 8) % 8u;  // +x, -x, +y, -y, then the diagonals
+                        const int ddx = k == 0 || k == 4 || k == 6 ? 1 : k == 2 || k == 3 ? 0 : -1;
+                        const int ddy = k == 2 || k == 4 || k == 5 ? 1 : k <= 1 ? 0 : -1;
+                        const int wx = (int)x + ddx, wy = (int)y + ddy;
+                        if (wx >= 0 && wy >= 0 && wx < (int)G && wy < (int)G) {
+                            const uint32_t wi = (uint32_t)wy * G + (uint32_t)wx;
+                            if (gm[wi] <= n) dest = wi;
+                        }
+                    }
+                }
+            }
+        }
+    } else if (n >= SANDPILE_DIFF) {
+        uint32_t best_idx = idx, best_count = n;
+        if (x > 0u && gm[idx - 1u] < best_count) { best_count = gm[idx - 1u]; best_idx = idx - 1u; }
+        if (x + 1u < G && gm[idx + 1u] < best_count) { best_count = gm[idx + 1u]; best_idx = idx + 1u; }
+        if (y > 0u && gm[idx - G] < best_count) { best_count = gm[idx - G]; best_idx = idx - G; }
+        if (y + 1u < G && gm[idx + G] < best_count) { best_count = gm[idx + G]; best_idx = idx + G; }
+        if (best_idx != idx && n - best_count >= SANDPILE_DIFF) {
+            const float p = GAMMA_RELAX_P * rb_max(rb_clamp(P.sediment_settle, 0.0f, 1.0f), rb_clamp(P.rock_crumble, 0.0f, 1.0f));
+            if (rng_f4(idx, P.epoch, S_SAND + phase, P.seed).x < p) dest = best_idx;
+        }
+    }
+    uint32_t cd = 0;
+    if (dest != idx) cd = nb8_code((int)(dest % G) - (int)x, (int)(dest / G) - (int)y);
+    code[idx] = (uint8_t)cd;
+}
+
+// The grain moves of a pass: each cell loses its own grain (if it moves one)
+// and gains one from every neighbour whose code points at it.
+__global__ __launch_bounds__(256) void k_relax_apply(uint32_t G, uint32_t* __restrict__ gamma, const uint8_t* __restrict__ code) {
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    if (x >= G || y >= G) return;
+    const uint32_t idx = y * G + x;
+    uint32_t gn = gamma[idx] - (code[idx] != 0u ? 1u : 0u);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const int sx = (int)x - NB8_DX[k], sy = (int)y - NB8_DY[k];  // the neighbour whose move k lands here
+        if (sx < 0 || sy < 0 || sx >= (int)G || sy >= (int)G) continue;
+        if (code[(uint32_t)sy * G + (uint32_t)sx] == (uint8_t)(k + 1)) gn += 1u;
+    }
+    gamma[idx] = gn;
+}
+
+// evict_target (terrain.wgsl:49-66)
+__device__ uint32_t evict_target(const uint32_t* chem, const uint32_t* gm, uint32_t G, uint32_t src, uint32_t dst) {
+    if (cell_total(chem, src) < capacity(gm[src])) return src;
+    const int x = (int)(dst % G), y = (int)(dst / G);
+    uint32_t not_rock = gm[src] < GAMMA_SOLID ? src : 0xFFFFFFFFu;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int cx = x + (i == 0 ? 1 : i == 1 ? -1 : 0), cy = y + (i == 2 ? 1 : i == 3 ? -1 : 0);
+        if (cx < 0 || cy < 0 || cx >= (int)G || cy >= (int)G) continue;
+        const uint32_t n = (uint32_t)cy * G + (uint32_t)cx;
+        if (cell_total(chem, n) < capacity(gm[n])) return n;
+        if (not_rock == 0xFFFFFFFFu && gm[n] < GAMMA_SOLID) not_rock = n;
+    }
+    return not_rock == 0xFFFFFFFFu ? src : not_rock;
+}
+
+// The eviction loop of gamma_move_one (terrain.wgsl:76-96) once the grain
+// from src sits in dst; the caller owns every cell it touches.
+__device__ void evict_one_move(uint32_t* chem, const uint32_t* gm, uint32_t G, uint32_t src, uint32_t dst) {
+    const uint32_t cap = capacity(gm[dst]);
+    for (uint32_t guard = 0; guard < 64u; ++guard) {
+        if (cell_total(chem, dst) <= cap) break;
+        const uint32_t to = evict_target(chem, gm, G, src, dst);
+        bool moved = false;
+        for (uint32_t c = 0; c < 4u && !moved; ++c) {
+            const uint32_t v = chem[dst * 4u + c];
+            if ((v >> 16) != 0u) {  // chem_take_state_one(slot, spent), then chem_add_state
+                chem[dst * 4u + c] = v - CHEM_SPENT_ONE;
+                chem[to * 4u + c] += CHEM_SPENT_ONE;
+                moved = true;
+            } else if ((v & CHEM_STATE_MASK) != 0u) {
+                chem[dst * 4u + c] = v - 1u;
+                chem[to * 4u + c] += 1u;
+                moved = true;
+            }
+        }
+        if (!moved) break;
+    }
+}
+
+// Evictions of the cells of colour (cx, cy) that received grains, their
+// incoming grains in row order of the source.
+__global__ __launch_bounds__(256) void k_relax_evict(uint32_t G, uint32_t cx, uint32_t cy, uint32_t* chem,
+                                                     const uint32_t* __restrict__ gm, const uint8_t* __restrict__ code) {
+    const uint32_t nx = (G - cx + 2u) / 3u, ny = (G - cy + 2u) / 3u;
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= nx * ny) return;
+    const uint32_t x = cx + 3u * (t % nx), y = cy + 3u * (t / nx);
+    const uint32_t d = y * G + x;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (dx == 0 && dy == 0) continue;
+            const int sx = (int)x + dx, sy = (int)y + dy;
+            if (sx < 0 || sy < 0 || sx >= (int)G || sy >= (int)G) continue;
+            const uint32_t src = (uint32_t)sy * G + (uint32_t)sx;
+            if (code[src] == (uint8_t)nb8_code(-dx, -dy)) evict_one_move(chem, gm, G, src, d);
+        }
+}
+
+// grain_fall (terrain.wgsl:405-437) for the columns x = colour + 3k, one
+// thread a column from the bottom up (evictions touch columns x - 1 .. x + 1).
+__global__ __launch_bounds__(64) void k_grain_fall(uint32_t G, uint32_t F, RiboSimParams P, const uint32_t* ep,
+                                                   uint32_t colour, uint32_t* gamma, uint32_t* chem) {
+    const uint32_t x = colour + 3u * (blockIdx.x * 64u + threadIdx.x);
+    if (x >= G) return;
+    if (ep) P.epoch += *ep;
+    Geo gg;
+    gg.G = G;
+    gg.F = F;
+    const float w_s = GAMMA_SETTLE_SPEED * rb_max(P.sediment_settle, 0.0f);
+    const float rate = hop_p(gg, P, 0.0f, w_s);
+    if (rate <= 0.0f) return;
+    const uint32_t rf = rb_f2u(rate * 65536.0f);
+    if (((P.epoch * rf) >> 16) == (((P.epoch + 1u) * rf) >> 16)) return;
+    bool supported = true;
+    for (uint32_t y = 0; y < G; ++y) {
+        const uint32_t idx = y * G + x;
+        const uint32_t n = gamma[idx];
+        if (n == 0u) { supported = false; continue; }
+        if (n >= GAMMA_SOLID) { supported = true; continue; }
+        if (supported) continue;
+        for (uint32_t k = 0; k < n; ++k) {
+            gamma[idx] -= 1u;
+            gamma[idx - G] += 1u;
+            evict_one_move(chem, gamma, G, idx, idx - G);
+        }
+    }
+}
+
 // ---- transport ------------------------------------------------------------------------
 // agg_count + agg_neighbours (transport.wgsl:61-87) in one pass: activated
 // counts of the tile and its 1-cell ring go to LDS, then each cell sums its
@@ -2900,6 +3209,24 @@ void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t
                      uint32_t* expected) {
     if (n == 0) return;
     k_chem_ops<<<(n + 255u) / 256u, 256, 0, s>>>(chem, g.gamma, g.G, (const uint4*)ops, n, done, expected);
+}
+void launch_relax_pass(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t phase,
+                       const float2* vel, uint32_t* gamma, uint32_t* chem, uint8_t* code) {
+    k_relax_decide<<<grid2(g.G), B16, 0, s>>>(g, P, ep, phase, vel, code);
+    k_relax_apply<<<grid2(g.G), B16, 0, s>>>(g.G, gamma, code);
+    for (uint32_t c = 0; c < 9u; ++c) {
+        const uint32_t cx = c % 3u, cy = c / 3u;
+        if (cx >= g.G || cy >= g.G) continue;
+        const uint32_t n = ((g.G - cx + 2u) / 3u) * ((g.G - cy + 2u) / 3u);
+        k_relax_evict<<<(n + 255u) / 256u, 256, 0, s>>>(g.G, cx, cy, chem, gamma, code);
+    }
+}
+void launch_grain_fall(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t* gamma,
+                       uint32_t* chem) {
+    for (uint32_t c = 0; c < 3u && c < g.G; ++c) {
+        const uint32_t cols = (g.G - c + 2u) / 3u;
+        k_grain_fall<<<(cols + 63u) / 64u, 64, 0, s>>>(g.G, g.F, P, ep, c, gamma, chem);
+    }
 }
 void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg_e) {
     k_agg<<<grid2(g.G), B16, 0, s>>>(g, chem, agg_e);

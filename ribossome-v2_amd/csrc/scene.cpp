@@ -631,7 +631,6 @@ int ribo_world_load_scene(RiboWorld* w, const char* path) {
     for (int k = NREQUIRED; k < NGRID; ++k)
         if ((rc = ribo_world_download(w, GRIDS[k].buf, host[k].data(), host[k].size()))) return rc;
     if ((rc = ribo_scene_read(path, &cfg, &p, &s, &g))) return rc;
-    s.terrain_enabled = 0;  // the terrain is a frozen input of the world step
     if ((rc = ribo_world_set_params(w, &p)) || (rc = ribo_world_set_settings(w, &s))) return rc;
     // gamma first: the derived terrain follows it
     const int order[NGRID] = {1, 0, 2, 3, 4, 5, 6};

@@ -122,6 +122,7 @@ struct RiboWorld {
     bool eaten_active = false;
     float2* forces = nullptr;
     uint32_t* agent_force = nullptr;  // force_vectors from the agents, i32[2F] fixed point
+    uint8_t* terrain_code = nullptr;  // per grid cell: the move a relax pass decided (0: none)
     bool agent_force_active = false;  // consumed (added, then cleared) by the next fluid step
     float2* heat = nullptr;
     float* net_flux = nullptr;
@@ -564,8 +565,36 @@ void vcycle_solve(RiboWorld* w, uint32_t slots) {
     check(2);
 }
 
+// The terrain stage of a step (src/world/mod.rs:1495-1503): two relax
+// passes, grain_fall when grains settle, then the slope.
+void terrain_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {
+    hipStream_t s = w->stream;
+    const Geo g = w->geo();
+    for (uint32_t phase = 0; phase < 2u; ++phase)
+        run(w, phase ? "relax_gamma_b" : "relax_gamma_a",
+            [&] { launch_relax_pass(s, g, P, ep, phase, w->vel_a, w->gamma, w->chem[w->cur], w->terrain_code); });
+    if (P.sediment_settle > 0.0f) run(w, "grain_fall", [&] { launch_grain_fall(s, g, P, ep, w->gamma, w->chem[w->cur]); });
+    run(w, "compute_gamma_slope", [&] { launch_slope(s, g, w->slope); });
+}
+
+// The fluid's terrain-derived data (wall flags, fill, drag, slope at the
+// fluid cells, wall normals, multigrid masks), rebuilt at each fluid step
+// while the terrain moves, as build_solid_mask is (fluid.wgsl:185-189).
+void fluid_terrain(RiboWorld* w) {
+    hipStream_t s = w->stream;
+    const Geo g = w->geo();
+    run(w, "build_solid_mask", [&] {
+        launch_fluid_static(s, g, w->cflags, w->fill, w->dragg, w->slope_f);
+        launch_fluid_normals(s, g, w->cflags, w->normal);
+        launch_mg_mask0(s, g, w->mg_fluid);
+        for (size_t l = 1; l < w->levels.size(); ++l)
+            launch_mg_mask_coarsen(s, w->levels[l].n, w->mg_fluid + w->levels[l - 1].off, w->mg_fluid + w->levels[l].off);
+    });
+}
+
 void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588
     hipStream_t s = w->stream;
+    if (w->st.terrain_enabled) fluid_terrain(w);
     Geo g = w->geo();
     const int tc = w->tcur;
     run(w, "update_temperature", [&] {
@@ -653,6 +682,7 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
         P.epoch = ep ? i : epoch;
         P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, epoch) : 0.0f;
         P.sun_slope = sun_slope_at(w->params, epoch);
+        if (w->st.terrain_enabled) terrain_step(w, P, ep);
         light_step(w, P, epoch);
         if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
         if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep);
@@ -692,7 +722,9 @@ int reset_pressure_history(RiboWorld* w) {
 }
 
 int validate_settings(const RiboWorldSettings* s) {
-    if (s->terrain_enabled) return fail(RIBO_ERR_UNSUPPORTED, "terrain physics is not part of the world-step path (terrain_enabled must be 0)");
+    if (s->terrain_enabled > 1u) return fail(RIBO_ERR_ARG, "terrain_enabled must be 0 or 1");
+    if (s->terrain_enabled && s->multigrid && s->mg_tol > 0.0f)
+        return fail(RIBO_ERR_UNSUPPORTED, "moving terrain (terrain_enabled) is not supported with the tolerance mode (mg_tol > 0)");
     if (s->light_mode > 1u) return fail(RIBO_ERR_ARG, "light_mode must be 0 (frozen) or 1 (the reference's light schedule)");
     if (s->precision != RIBO_PRECISION_FP32 && s->precision != RIBO_PRECISION_MIXED)
         return fail(RIBO_ERR_ARG, "unknown precision");
@@ -770,7 +802,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     const size_t Ls = cfg->grid_size / LIGHT_DIV, nb = (cfg->fluid_size + 15) / 16;
     size_t b = 2 * a(4 * G, 4) + a(G, 4) + 2 * a(2 * Ls * Ls, 4) + a(G, 8) + a(G, 4);  // chem x2, gamma, light x2, slope, agg
     b += 2 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x3, div, temp/redox x2, eaten, forces, heat
-    b += a(2 * F, 4);  // the agents' force words
+    b += a(2 * F, 4) + a(G, 1);  // the agents' force words, the grain move codes
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
     b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 7 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
     b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
@@ -835,7 +867,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(w->stream, &w->p_old, F)) ||
         (rc = dalloc(w->stream, &w->div, F)) || (rc = dalloc(w->stream, &w->temp[0], F)) || (rc = dalloc(w->stream, &w->temp[1], F)) ||
         (rc = dalloc(w->stream, &w->redox[0], F)) || (rc = dalloc(w->stream, &w->redox[1], F)) || (rc = dalloc(w->stream, &w->eaten, F)) ||
-        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->agent_force, 2 * F)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
+        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->agent_force, 2 * F)) || (rc = dalloc(w->stream, &w->terrain_code, G)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
         (rc = dalloc(w->stream, &w->cflags, F)) || (rc = dalloc(w->stream, &w->normal, F)) || (rc = dalloc(w->stream, &w->fill, F)) ||
         (rc = dalloc(w->stream, &w->dragg, F)) || (rc = dalloc(w->stream, &w->slope_f, F)) || (rc = dalloc(w->stream, &w->pcg, 1)) ||
         (rc = dalloc(w->stream, &w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(w->stream, &w->pcg_r, F)) ||
@@ -874,7 +906,7 @@ int ribo_world_destroy(RiboWorld* w) {
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
     dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light[0]); dfree(w->light[1]); dfree(w->slope); dfree(w->agg_e);
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->p_old); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
-    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->heat); dfree(w->net_flux);
+    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->terrain_code); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
     dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w);

@@ -26,7 +26,7 @@ def test_library_exports_every_declared_symbol():
     assert len(syms) >= 25
     for s in sorted(syms):
         assert hasattr(L, s), s
-    assert L.ribo_abi_version() == 4
+    assert L.ribo_abi_version() == 5
 
 
 def test_struct_layouts_match_reference():

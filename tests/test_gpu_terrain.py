@@ -1,0 +1,73 @@
+"""Terrain grain dynamics on the HIP world against the oracle: both follow
+the same admissible order of the reference's concurrent grain moves, so
+grains, monomers, the slope and every fluid field are bit-identical, with
+graph replay and with eager recording; grains and monomers are conserved
+at a larger size under the device's per-step ledger check."""
+import numpy as np
+import pytest
+
+import ribossome_amd as ra
+import oracle_ctypes as oc
+from test_gpu_parity import compare
+from worlds import bit_diff, load, make_inputs, parity_params, parity_settings
+
+pytestmark = pytest.mark.gpu
+
+ACTIVE = dict(sediment_settle=1.0, sediment_compaction=0.3, rock_crumble=1.0, sediment_transport=3.0,
+              sediment_threshold=0.05, diffusion=5.0)
+
+CASES = {
+    "parity_profile": dict(cfg=ra.WorldConfig(128, 64), seed=41, params={}, settings={}),
+    "ref_schedule": dict(cfg=ra.WorldConfig(128, 64), seed=43, params=dict(transport_every=3),
+                         settings=dict(fluid_substep=2)),
+    "jacobi_eager": dict(cfg=ra.WorldConfig(96, 48), seed=45, params={}, settings=dict(multigrid=0, jacobi_iters=16,
+                                                                                       no_graphs=1)),
+}
+
+
+def build(case):
+    c = CASES[case]
+    cfg = c["cfg"]
+    inp = make_inputs(cfg, c["seed"], rock=0.12, rubble=True, vent_rows=4, vcells_per_step=2.0)
+    p = parity_params(c["seed"], **{**ACTIVE, **c["params"]})
+    s = parity_settings(terrain_enabled=1, **c["settings"])
+    gw, ow = ra.World(cfg, c["seed"]), oc.OracleWorld(cfg, c["seed"])
+    load(gw, inp, p, s)
+    load(ow, inp, p, s)
+    return inp, gw, ow
+
+
+@pytest.mark.parametrize("case", list(CASES))
+def test_terrain_dynamics_bit_exact(case):
+    print(f"case terrain {case}", flush=True)
+    inp, gw, ow = build(case)
+    for k, n in enumerate((1, 1, 3, 7, 20)):
+        assert gw.encode_steps(n) == n and ow.encode_steps(n) == n
+        nd = bit_diff(gw.download(ra.GAMMA), ow.download(ra.GAMMA))
+        print(f"[{case} chunk {k}] gamma words differing: {nd}", flush=True)
+        assert nd == 0 and compare(gw, ow, f"{case} chunk {k}"), f"{case}: GPU differs from the oracle"
+    g = gw.download(ra.GAMMA)
+    assert int(np.count_nonzero(g != inp[ra.GAMMA])) > 20
+    assert int(g.sum()) == int(inp[ra.GAMMA].sum())
+
+
+def test_terrain_conserves_at_size():
+    print("case terrain conservation 512", flush=True)
+    cfg = ra.WorldConfig(512, 256)
+    inp = make_inputs(cfg, 47, rock=0.12, rubble=True, vent_rows=8, vcells_per_step=2.0)
+    p = parity_params(47, **ACTIVE)
+    s = parity_settings(terrain_enabled=1, ledger_check=1, fluid_substep=2)
+    gw = ra.World(cfg, 47)
+    load(gw, inp, p, s)
+    gw.run(300)
+    g = gw.download(ra.GAMMA)
+    assert int(g.sum()) == int(inp[ra.GAMMA].sum())
+    assert int(np.count_nonzero(g != inp[ra.GAMMA])) > 1000
+    assert gw.ledger_status() is None
+    assert ra.Ledger.from_cells(gw.download(ra.CHEM)).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
+
+
+def test_moving_terrain_refused_with_tolerance_mode():
+    w = ra.World(ra.WorldConfig(64, 32), 1)
+    with pytest.raises(ra.RiboError):
+        w.settings = parity_settings(terrain_enabled=1, mg_tol=1e-5)

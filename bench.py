@@ -62,6 +62,13 @@ BYTES_PER_UNIT = {
     "light_propagate": (24, "G"),
     # the sequential sweep reads and writes both bands once
     "light_sweep": (4, "G"),
+    # terrain grain dynamics, per pass: the decision reads gamma and writes a
+    # move byte, the move reads gamma and the codes and writes gamma, the
+    # evictions read the codes (the monomers of the few cells that received a
+    # grain are not counted); the slope reads gamma and writes 8 B
+    "relax_gamma_a": (15, "G"),
+    "relax_gamma_b": (15, "G"),
+    "compute_gamma_slope": (12, "G"),
     # tolerance-mode solver, level 0 (S = bytes of the preconditioner's storage)
     "pcg_init": (33, "F"),
     "pcg_final": (9, "F"),
@@ -272,6 +279,49 @@ def light_measure(cfg, inputs, args):
         b = kernel_bytes(k, cfg, [], "reference")
         out[k] = {"avg_launch_us": us, "launches": n, "algorithmic_bytes_per_launch": b,
                   "achieved_GB_per_s": b / (us * 1e-6) / 1e9, "frac_of_8TBps": b / (us * 1e-6) / 8e12}
+    return out
+
+
+def terrain_measure(cfg, inputs, args):
+    """Terrain grain dynamics on the same 2048^2 world (terrain_enabled 1,
+    reference V-cycle, grains settling so grain_fall runs on its clock):
+    HIP-event times of each terrain stage against the HBM roofline, and the
+    whole step's rate with the terrain moving."""
+    w = build_world(cfg, inputs, args, "reference")
+    p = w.params
+    p.sediment_settle = 1.0
+    w.params = p
+    s = w.settings
+    s.terrain_enabled = 1
+    w.settings = s
+    w.run(8)
+    w.set_timing(True)
+    w.reset_kernel_times()
+    w.run(args.profile_steps)
+    times = w.kernel_times()
+    w.set_timing(False)
+    w.run(8)
+    w.sync()
+    n = max(args.terrain_steps, 1)
+    t0 = time.perf_counter()
+    w.run(n)
+    w.sync()
+    dt = time.perf_counter() - t0
+    moved = int(np.count_nonzero(w.download(ra.GAMMA) != inputs[0]))
+    w.close()
+    out = {"steps": n, "ms_per_step": dt / n * 1000.0, "cell_steps_per_s": cfg.cells() * n / dt,
+           "grid_cells_changed": moved}
+    for k in ("relax_gamma_a", "relax_gamma_b", "grain_fall", "compute_gamma_slope", "build_solid_mask"):
+        tot, cnt = times.get(k, (0.0, 0))
+        if not cnt:
+            continue
+        us = tot / cnt * 1000.0
+        b = kernel_bytes(k, cfg, [], "reference")
+        e = {"avg_launch_us": us, "launches": cnt}
+        if b:
+            e.update({"algorithmic_bytes_per_launch": b, "achieved_GB_per_s": b / (us * 1e-6) / 1e9,
+                      "frac_of_8TBps": b / (us * 1e-6) / 8e12})
+        out[k] = e
     return out
 
 
@@ -529,6 +579,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64-oracle error sample")
     ap.add_argument("--no-light", action="store_true", help="skip the light-step kernel timing")
+    ap.add_argument("--terrain-steps", type=int, default=64,
+                    help="steps of the moving-terrain measurement (0 skips it)")
     ap.add_argument("--vcycle-steps", type=int, default=100,
                     help="steps of the stationary V-cycle mode (the plain FP32 path to 1e-5); 0 skips it")
     ap.add_argument("--vcycle-max-cycles", type=int, default=400, help="V-cycle cap per solve of the stationary mode")
@@ -630,6 +682,7 @@ def main():
                "frac_of_8TBps": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 8e12} for m in results}
 
     light = None if args.no_light else light_measure(cfg, inputs, args)
+    terrain = terrain_measure(cfg, inputs, args) if args.terrain_steps > 0 else None
     plain = {}
     if args.vcycle_steps > 0:
         # the stationary V-cycle, on a bounded number of steps (each step runs
@@ -691,6 +744,7 @@ def main():
         "roofline_transport_scatter_valu": roofline_valu,
         "hbm_step_model": hbm,
         "light": light,
+        "terrain": terrain,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,
     }

@@ -207,10 +207,11 @@ void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t
                      uint32_t* expected);
 
 // terrain grain dynamics: one relax pass (phase 0 or 1: decisions on the
-// pass's counts into code[G], the grain moves, the colour-ordered
-// evictions) and grain_fall (three column colours; a no-op off the clock)
+// pass's counts into code[G], the grain moves with the incoming-grain masks
+// in inc[G], the colour-ordered evictions) and grain_fall (three column
+// colours; a no-op off the clock)
 void launch_relax_pass(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t phase,
-                       const float2* vel, uint32_t* gamma, uint32_t* chem, uint8_t* code);
+                       const float2* vel, uint32_t* gamma, uint32_t* chem, uint8_t* code, uint8_t* inc);
 void launch_grain_fall(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t* gamma,
                        uint32_t* chem);
 

@@ -2240,24 +2240,31 @@ __device__ __forceinline__ float hop_p(const Geo& g, const RiboSimParams& P, flo
     return rb_clamp(l1 * env_per_fluid * rb_max(P.dt, 1e-3f), 0.0f, GRAIN_ADV_CAP);
 }
 
-// compaction_stable (terrain.wgsl:139-157) on the pass's counts.
-__device__ bool compaction_stable(const uint32_t* __restrict__ gm, uint32_t G, uint32_t src, uint32_t dst, uint32_t n,
+// The pass's grain counts of a 16 x 16 tile and a 2-cell halo (every cell a
+// decision reads lies within 2 of the deciding cell), and the occupied-cell
+// count of the 8-neighbourhood of the tile and a 1-cell halo; cells outside
+// the world read 0 and are never counted.
+constexpr int RT = 16, RH = 2, RW = RT + 2 * RH, OW = RT + 2;
+struct RelaxTile {
+    uint32_t g[RW * RW];
+    uint8_t occ[OW * OW];
+};
+
+// compaction_stable (terrain.wgsl:139-157): src = (sx, sy), dst = (tx, ty).
+__device__ bool compaction_stable(const RelaxTile& T, int x0, int y0, int G, int sx, int sy, int tx, int ty, uint32_t n,
                                   uint32_t c) {
     bool ok = true;
-    const int sx = (int)(src % G), sy = (int)(src / G), tx = (int)(dst % G), ty = (int)(dst / G);
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const int ddx = k == 0 ? 1 : k == 1 ? -1 : 0, ddy = k == 2 ? 1 : k == 3 ? -1 : 0;
         const int qdx = tx + ddx, qdy = ty + ddy;
-        if (qdx >= 0 && qdy >= 0 && qdx < (int)G && qdy < (int)G) {
-            const uint32_t qi = (uint32_t)qdy * G + (uint32_t)qdx;
-            const uint32_t low = qi == src ? n - 1u : gm[qi];
+        if (qdx >= 0 && qdy >= 0 && qdx < G && qdy < G) {
+            const uint32_t low = (qdx == sx && qdy == sy) ? n - 1u : T.g[(qdy - y0 + RH) * RW + (qdx - x0 + RH)];
             if (c + 1u >= low + SANDPILE_DIFF) ok = false;
         }
         const int qsx = sx + ddx, qsy = sy + ddy;
-        if (qsx >= 0 && qsy >= 0 && qsx < (int)G && qsy < (int)G) {
-            const uint32_t qi = (uint32_t)qsy * G + (uint32_t)qsx;
-            const uint32_t high = qi == dst ? c + 1u : gm[qi];
+        if (qsx >= 0 && qsy >= 0 && qsx < G && qsy < G) {
+            const uint32_t high = (qsx == tx && qsy == ty) ? c + 1u : T.g[(qsy - y0 + RH) * RW + (qsx - x0 + RH)];
             if (high >= n - 1u + SANDPILE_DIFF) ok = false;
         }
     }
@@ -2267,80 +2274,94 @@ __device__ bool compaction_stable(const uint32_t* __restrict__ gm, uint32_t G, u
 // relax_gamma_pass's decision (terrain.wgsl:159-406): the cell's move code.
 __global__ __launch_bounds__(256) void k_relax_decide(Geo g, RiboSimParams P, const uint32_t* ep, uint32_t phase,
                                                       const float2* __restrict__ vel, uint8_t* __restrict__ code) {
-    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
-    const uint32_t G = g.G, F = g.F;
+    __shared__ RelaxTile T;
+    const int G = (int)g.G;
+    const uint32_t F = g.F;
+    const int x0 = (int)blockIdx.x * RT, y0 = (int)blockIdx.y * RT;
+    const int tid = (int)(threadIdx.y * 16 + threadIdx.x);
+    for (int i = tid; i < RW * RW; i += 256) {
+        const int gx = x0 - RH + i % RW, gy = y0 - RH + i / RW;
+        T.g[i] = (gx >= 0 && gy >= 0 && gx < G && gy < G) ? g.gamma[(uint32_t)gy * (uint32_t)G + (uint32_t)gx] : 0u;
+    }
+    __syncthreads();
+    for (int i = tid; i < OW * OW; i += 256) {
+        const int lx = i % OW + 1, ly = i / OW + 1;  // in the staged tile
+        uint32_t c = 0;
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c += T.g[(ly + NB8_DY[k]) * RW + lx + NB8_DX[k]] >= 1u ? 1u : 0u;
+        T.occ[i] = (uint8_t)c;
+    }
+    __syncthreads();
+    const int x = x0 + (int)threadIdx.x, y = y0 + (int)threadIdx.y;
     if (x >= G || y >= G) return;
     if (ep) P.epoch += *ep;
-    const uint32_t* __restrict__ gm = g.gamma;
-    const uint32_t idx = y * G + x;
-    const uint32_t n = gm[idx];
-    uint32_t dest = idx;
+    auto cnt = [&](int gx, int gy) { return T.g[(gy - y0 + RH) * RW + (gx - x0 + RH)]; };
+    const uint32_t idx = (uint32_t)y * (uint32_t)G + (uint32_t)x;
+    const uint32_t n = cnt(x, y);
+    int dxm = 0, dym = 0;  // the move
     if (n >= 1u && n < SANDPILE_DIFF) {
-        uint32_t bonds = 0, best_idx = idx, best_c = 0xFFFFu, em_idx = idx;
-        int best_score = -1, em_score = -1;
+        uint32_t bonds = 0, best_c = 0xFFFFu;
+        int best_score = -1, em_score = -1, bdx = 0, bdy = 0, edx = 0, edy = 0, cdx = 0, cdy = 0;
         bool em_found = false;
         const U4 rr = pcg4d({idx, P.epoch, S_RELAX + phase, P.seed});
         uint32_t tie_h = rr.x;
-        uint32_t comp_idx = idx, comp_c = 0;
+        uint32_t comp_c = 0;
         const float comp_roll = (float)(mix_hash(rr.x ^ 0x5bd1e995u) >> 8) * (1.0f / 16777216.0f);
         const bool try_comp = comp_roll < P.sediment_compaction;
         for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx) {
                 if (dx == 0 && dy == 0) continue;
-                const int nx = (int)x + dx, ny = (int)y + dy;
-                if (nx < 0 || ny < 0 || nx >= (int)G || ny >= (int)G) continue;
-                const uint32_t ni = (uint32_t)ny * G + (uint32_t)nx;
-                const uint32_t c = gm[ni];
+                const int nx = x + dx, ny = y + dy;
+                if (nx < 0 || ny < 0 || nx >= G || ny >= G) continue;
+                const uint32_t ni = (uint32_t)ny * (uint32_t)G + (uint32_t)nx;
+                const uint32_t c = cnt(nx, ny);
                 if (c >= 1u) bonds += 1u;
                 if (try_comp && c >= 1u && c < GAMMA_ROCK_FULL &&
                     (c > comp_c || (c == comp_c && (mix_hash(tie_h ^ (ni * 31u)) & 1u) == 1u)) &&
-                    compaction_stable(gm, G, idx, ni, n, c)) {
+                    compaction_stable(T, x0, y0, G, x, y, nx, ny, n, c)) {
                     comp_c = c;
-                    comp_idx = ni;
+                    cdx = dx;
+                    cdy = dy;
                 }
                 if (c <= n) {
-                    int score = 0;
-                    for (int sy = -1; sy <= 1; ++sy)
-                        for (int sx = -1; sx <= 1; ++sx) {
-                            if (sx == 0 && sy == 0) continue;
-                            const int qx = nx + sx, qy = ny + sy;
-                            if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
-                            const uint32_t qi = (uint32_t)qy * G + (uint32_t)qx;
-                            if (qi == idx) continue;
-                            if (gm[qi] >= 1u) score += 1;
-                        }
+                    // occupied cells around the neighbour, less this (occupied) cell
+                    const int score = (int)T.occ[(ny - y0 + 1) * OW + (nx - x0 + 1)] - 1;
                     tie_h = mix_hash(tie_h ^ ni);
                     if (score > best_score || (score == best_score && c < best_c) ||
                         (score == best_score && c == best_c && (tie_h & 1u) == 1u)) {
                         best_score = score;
                         best_c = c;
-                        best_idx = ni;
+                        bdx = dx;
+                        bdy = dy;
                     }
                     if (c == 0u && (score > em_score || (score == em_score && (tie_h & 2u) == 2u))) {
                         em_score = score;
-                        em_idx = ni;
+                        edx = dx;
+                        edy = dy;
                         em_found = true;
                     }
                 }
             }
         const float mob = __builtin_ldexpf(1.0f, -2 * (int)bonds);  // pow(0.25, bonds), exact
-        if (try_comp && bonds >= 3u && comp_idx != idx &&
+        if (try_comp && bonds >= 3u && (cdx != 0 || cdy != 0) &&
             comp_roll < P.sediment_compaction * (float)(bonds - 2u) / 6.0f) {
-            dest = comp_idx;
+            dxm = cdx;
+            dym = cdy;
         } else {
             float vsx = 0.0f, vsy = 0.0f;
             if (P.fluid_enabled != 0u) {
 #pragma unroll
                 for (int k = 0; k < 4; ++k) {
-                    const int qx = (int)x + (k == 0 ? 1 : k == 1 ? -1 : 0), qy = (int)y + (k == 2 ? 1 : k == 3 ? -1 : 0);
-                    if (qx < 0 || qy < 0 || qx >= (int)G || qy >= (int)G) continue;
-                    if (gm[(uint32_t)qy * G + (uint32_t)qx] != 0u) continue;
-                    const uint32_t fxi = rb_minu(((uint32_t)qx * F) / G, F - 1u), fyi = rb_minu(((uint32_t)qy * F) / G, F - 1u);
+                    const int qx = x + (k == 0 ? 1 : k == 1 ? -1 : 0), qy = y + (k == 2 ? 1 : k == 3 ? -1 : 0);
+                    if (qx < 0 || qy < 0 || qx >= G || qy >= G) continue;
+                    if (cnt(qx, qy) != 0u) continue;
+                    const uint32_t fxi = rb_minu(((uint32_t)qx * F) / (uint32_t)G, F - 1u);
+                    const uint32_t fyi = rb_minu(((uint32_t)qy * F) / (uint32_t)G, F - 1u);
                     const float2 v = sanitize(vel[fyi * F + fxi]);
                     if (v.x * v.x + v.y * v.y > vsx * vsx + vsy * vsy) { vsx = v.x; vsy = v.y; }
                 }
             }
-            const bool unsupported = y > 0u && gm[idx - G] == 0u;
+            const bool unsupported = y > 0 && cnt(x, y - 1) == 0u;
             float mvx = 0.0f, mvy = 0.0f, p_sed;
             if (unsupported && bonds == 0u) {
                 mvx = vsx; mvy = vsy;
@@ -2355,7 +2376,8 @@ __global__ __launch_bounds__(256) void k_relax_decide(Geo g, RiboSimParams P, co
                 }
                 p_sed = hop_p(g, P, mvx, mvy) * GAMMA_SEDIMENT_FACTOR * rb_max(P.sediment_transport, 0.0f);
             }
-            const float2 vent = g.heat[rb_minu((y * F) / G, F - 1u) * F + rb_minu((x * F) / G, F - 1u)];
+            const float2 vent = g.heat[rb_minu(((uint32_t)y * F) / (uint32_t)G, F - 1u) * F +
+                                       rb_minu(((uint32_t)x * F) / (uint32_t)G, F - 1u)];
             const float blow = VENT_BLOW * rb_max(vent.x, vent.y);
             if (blow > 0.0f) {
                 mvx = ((rr.y & 1u) == 1u ? 0.6f : -0.6f) * blow;
@@ -2366,123 +2388,146 @@ __global__ __launch_bounds__(256) void k_relax_decide(Geo g, RiboSimParams P, co
                 const uint32_t hw = rr.y;
                 const float rw = (float)(rr.z >> 8) * (1.0f / 16777216.0f);
                 const float rw2 = (float)(rr.w >> 8) * (1.0f / 16777216.0f);
+                int wdx = 0, wdy = 0;
+                bool walk = false;
                 if (n >= 2u && em_found && rw2 < GAMMA_SHED_P * rb_clamp(P.sediment_settle, 0.0f, 1.0f)) {
-                    dest = em_idx;
+                    dxm = edx;
+                    dym = edy;
                 } else if (rw < p_sed) {
-                    int ddx = 0, ddy = 0;
                     const float ax = rb_abs(mvx), ay = rb_abs(mvy);
-                    if ((float)(hw >> 8) * (1.0f / 16777216.0f) < ax / rb_max(ax + ay, 1e-6f)) ddx = mvx > 0.0f ? 1 : -1;
-                    else ddy = mvy > 0.0f ? 1 : -1;
-                    const int wx = (int)x + ddx, wy = (int)y + ddy;
-                    if (wx >= 0 && wy >= 0 && wx < (int)G && wy < (int)G) {
-                        const uint32_t wi = (uint32_t)wy * G + (uint32_t)wx;
-                        if (gm[wi] <= n) dest = wi;
-                    }
+                    if ((float)(hw >> 8) * (1.0f / 16777216.0f) < ax / rb_max(ax + ay, 1e-6f)) wdx = mvx > 0.0f ? 1 : -1;
+                    else wdy = mvy > 0.0f ? 1 : -1;
+                    walk = true;
                 } else if (rw < p_sed + GAMMA_STRAY_WALK_P * mob) {
                     if (best_score > (int)bonds) {
-                        dest = best_idx;
+                        dxm = bdx;
+                        dym = bdy;
                     } else if (bonds == 0u) {
                         const uint32_t k = (hw >> 8) % 8u;  // +x, -x, +y, -y, then the diagonals
-                        const int ddx = k == 0 || k == 4 || k == 6 ? 1 : k == 2 || k == 3 ? 0 : -1;
-                        const int ddy = k == 2 || k == 4 || k == 5 ? 1 : k <= 1 ? 0 : -1;
-                        const int wx = (int)x + ddx, wy = (int)y + ddy;
-                        if (wx >= 0 && wy >= 0 && wx < (int)G && wy < (int)G) {
-                            const uint32_t wi = (uint32_t)wy * G + (uint32_t)wx;
-                            if (gm[wi] <= n) dest = wi;
-                        }
+                        wdx = k == 0 || k == 4 || k == 6 ? 1 : k == 2 || k == 3 ? 0 : -1;
+                        wdy = k == 2 || k == 4 || k == 5 ? 1 : k <= 1 ? 0 : -1;
+                        walk = true;
+                    }
+                }
+                if (walk) {  // a step onto a cell with no more grains than this one
+                    const int wx = x + wdx, wy = y + wdy;
+                    if (wx >= 0 && wy >= 0 && wx < G && wy < G && cnt(wx, wy) <= n) {
+                        dxm = wdx;
+                        dym = wdy;
                     }
                 }
             }
         }
     } else if (n >= SANDPILE_DIFF) {
-        uint32_t best_idx = idx, best_count = n;
-        if (x > 0u && gm[idx - 1u] < best_count) { best_count = gm[idx - 1u]; best_idx = idx - 1u; }
-        if (x + 1u < G && gm[idx + 1u] < best_count) { best_count = gm[idx + 1u]; best_idx = idx + 1u; }
-        if (y > 0u && gm[idx - G] < best_count) { best_count = gm[idx - G]; best_idx = idx - G; }
-        if (y + 1u < G && gm[idx + G] < best_count) { best_count = gm[idx + G]; best_idx = idx + G; }
-        if (best_idx != idx && n - best_count >= SANDPILE_DIFF) {
+        uint32_t best_count = n;
+        int sdx = 0, sdy = 0;
+        if (x > 0 && cnt(x - 1, y) < best_count) { best_count = cnt(x - 1, y); sdx = -1; sdy = 0; }
+        if (x + 1 < G && cnt(x + 1, y) < best_count) { best_count = cnt(x + 1, y); sdx = 1; sdy = 0; }
+        if (y > 0 && cnt(x, y - 1) < best_count) { best_count = cnt(x, y - 1); sdx = 0; sdy = -1; }
+        if (y + 1 < G && cnt(x, y + 1) < best_count) { best_count = cnt(x, y + 1); sdx = 0; sdy = 1; }
+        if ((sdx != 0 || sdy != 0) && n - best_count >= SANDPILE_DIFF) {
             const float p = GAMMA_RELAX_P * rb_max(rb_clamp(P.sediment_settle, 0.0f, 1.0f), rb_clamp(P.rock_crumble, 0.0f, 1.0f));
-            if (rng_f4(idx, P.epoch, S_SAND + phase, P.seed).x < p) dest = best_idx;
+            if (rng_f4(idx, P.epoch, S_SAND + phase, P.seed).x < p) { dxm = sdx; dym = sdy; }
         }
     }
-    uint32_t cd = 0;
-    if (dest != idx) cd = nb8_code((int)(dest % G) - (int)x, (int)(dest / G) - (int)y);
-    code[idx] = (uint8_t)cd;
+    code[idx] = (uint8_t)((dxm != 0 || dym != 0) ? nb8_code(dxm, dym) : 0u);
 }
 
 // The grain moves of a pass: each cell loses its own grain (if it moves one)
-// and gains one from every neighbour whose code points at it.
-__global__ __launch_bounds__(256) void k_relax_apply(uint32_t G, uint32_t* __restrict__ gamma, const uint8_t* __restrict__ code) {
+// and gains one from every neighbour whose code points at it; inc[idx] gets
+// bit j for a grain from neighbour j (row order), for the evictions.
+__global__ __launch_bounds__(256) void k_relax_apply(uint32_t G, uint32_t* __restrict__ gamma, const uint8_t* __restrict__ code,
+                                                     uint8_t* __restrict__ inc) {
     const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     if (x >= G || y >= G) return;
     const uint32_t idx = y * G + x;
-    uint32_t gn = gamma[idx] - (code[idx] != 0u ? 1u : 0u);
+    const uint32_t own = code[idx] != 0u ? 1u : 0u;
+    uint32_t in = 0, bits = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const int sx = (int)x - NB8_DX[k], sy = (int)y - NB8_DY[k];  // the neighbour whose move k lands here
         if (sx < 0 || sy < 0 || sx >= (int)G || sy >= (int)G) continue;
-        if (code[(uint32_t)sy * G + (uint32_t)sx] == (uint8_t)(k + 1)) gn += 1u;
+        if (code[(uint32_t)sy * G + (uint32_t)sx] == (uint8_t)(k + 1)) {
+            in += 1u;
+            bits |= 1u << (7 - k);  // that neighbour sits at offset -(move k), row-order index 7 - k
+        }
     }
-    gamma[idx] = gn;
+    if (own | in) gamma[idx] = gamma[idx] - own + in;
+    inc[idx] = (uint8_t)bits;
 }
 
-// evict_target (terrain.wgsl:49-66)
-__device__ uint32_t evict_target(const uint32_t* chem, const uint32_t* gm, uint32_t G, uint32_t src, uint32_t dst) {
-    if (cell_total(chem, src) < capacity(gm[src])) return src;
+// The eviction loop of gamma_move_one (terrain.wgsl:76-96) once the grain
+// from src sits in dst; the caller owns every cell it touches.  The loop
+// moves one monomer an iteration while dst is over capacity (at most 64),
+// taking them in the order spent then activated of channel 0, 1, 2, 3, each
+// to evict_target: src while it has room, then dst's neighbours in order
+// while they have room, then the fallback.  Since only this loop changes
+// those cells, each target takes a run of consecutive monomers until its
+// room is used up, and the runs are moved here in a few steps.
+__device__ void evict_one_move(uint32_t* chem, const uint32_t* gm, uint32_t G, uint32_t src, uint32_t dst) {
+    const uint32_t cap = capacity(gm[dst]);
+    const uint32_t tot = cell_total(chem, dst);
+    if (tot <= cap) return;
+    uint32_t left = rb_minu(tot - cap, 64u);
+    uint4 dv = *reinterpret_cast<const uint4*>(chem + 4u * dst);
+    uint32_t seg[8] = {dv.x >> 16, dv.x & CHEM_STATE_MASK, dv.y >> 16, dv.y & CHEM_STATE_MASK,
+                       dv.z >> 16, dv.z & CHEM_STATE_MASK, dv.w >> 16, dv.w & CHEM_STATE_MASK};
+    uint32_t si = 0;
+    // moves `take` monomers of the removal order into cell `to`
+    auto give = [&](uint32_t to, uint32_t take) {
+        uint4 tv = *reinterpret_cast<const uint4*>(chem + 4u * to);
+        while (take > 0u) {
+            while (seg[si] == 0u) ++si;
+            const uint32_t m = rb_minu(seg[si], take);
+            seg[si] -= m;
+            take -= m;
+            const uint32_t add = (si & 1u) ? m : m * CHEM_SPENT_ONE;
+            const uint32_t c = si >> 1;
+            if (c == 0) tv.x += add; else if (c == 1) tv.y += add; else if (c == 2) tv.z += add; else tv.w += add;
+        }
+        *reinterpret_cast<uint4*>(chem + 4u * to) = tv;
+    };
+    auto room = [&](uint32_t cell) {
+        const uint32_t t = cell_total(chem, cell), k = capacity(gm[cell]);
+        return t < k ? k - t : 0u;
+    };
+    {
+        const uint32_t r = rb_minu(room(src), left);
+        if (r) { give(src, r); left -= r; }
+    }
     const int x = (int)(dst % G), y = (int)(dst / G);
     uint32_t not_rock = gm[src] < GAMMA_SOLID ? src : 0xFFFFFFFFu;
-#pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int cx = x + (i == 0 ? 1 : i == 1 ? -1 : 0), cy = y + (i == 2 ? 1 : i == 3 ? -1 : 0);
         if (cx < 0 || cy < 0 || cx >= (int)G || cy >= (int)G) continue;
         const uint32_t n = (uint32_t)cy * G + (uint32_t)cx;
-        if (cell_total(chem, n) < capacity(gm[n])) return n;
+        if (left) {
+            const uint32_t r = rb_minu(room(n), left);
+            if (r) { give(n, r); left -= r; }
+        }
         if (not_rock == 0xFFFFFFFFu && gm[n] < GAMMA_SOLID) not_rock = n;
     }
-    return not_rock == 0xFFFFFFFFu ? src : not_rock;
-}
-
-// The eviction loop of gamma_move_one (terrain.wgsl:76-96) once the grain
-// from src sits in dst; the caller owns every cell it touches.
-__device__ void evict_one_move(uint32_t* chem, const uint32_t* gm, uint32_t G, uint32_t src, uint32_t dst) {
-    const uint32_t cap = capacity(gm[dst]);
-    for (uint32_t guard = 0; guard < 64u; ++guard) {
-        if (cell_total(chem, dst) <= cap) break;
-        const uint32_t to = evict_target(chem, gm, G, src, dst);
-        bool moved = false;
-        for (uint32_t c = 0; c < 4u && !moved; ++c) {
-            const uint32_t v = chem[dst * 4u + c];
-            if ((v >> 16) != 0u) {  // chem_take_state_one(slot, spent), then chem_add_state
-                chem[dst * 4u + c] = v - CHEM_SPENT_ONE;
-                chem[to * 4u + c] += CHEM_SPENT_ONE;
-                moved = true;
-            } else if ((v & CHEM_STATE_MASK) != 0u) {
-                chem[dst * 4u + c] = v - 1u;
-                chem[to * 4u + c] += 1u;
-                moved = true;
-            }
-        }
-        if (!moved) break;
-    }
+    if (left) give(not_rock == 0xFFFFFFFFu ? src : not_rock, left);
+    *reinterpret_cast<uint4*>(chem + 4u * dst) =
+        make_uint4((seg[0] << 16) | seg[1], (seg[2] << 16) | seg[3], (seg[4] << 16) | seg[5], (seg[6] << 16) | seg[7]);
 }
 
 // Evictions of the cells of colour (cx, cy) that received grains, their
 // incoming grains in row order of the source.
 __global__ __launch_bounds__(256) void k_relax_evict(uint32_t G, uint32_t cx, uint32_t cy, uint32_t* chem,
-                                                     const uint32_t* __restrict__ gm, const uint8_t* __restrict__ code) {
+                                                     const uint32_t* __restrict__ gm, const uint8_t* __restrict__ inc) {
     const uint32_t nx = (G - cx + 2u) / 3u, ny = (G - cy + 2u) / 3u;
     const uint32_t t = blockIdx.x * 256u + threadIdx.x;
     if (t >= nx * ny) return;
     const uint32_t x = cx + 3u * (t % nx), y = cy + 3u * (t / nx);
     const uint32_t d = y * G + x;
-    for (int dy = -1; dy <= 1; ++dy)
-        for (int dx = -1; dx <= 1; ++dx) {
-            if (dx == 0 && dy == 0) continue;
-            const int sx = (int)x + dx, sy = (int)y + dy;
-            if (sx < 0 || sy < 0 || sx >= (int)G || sy >= (int)G) continue;
-            const uint32_t src = (uint32_t)sy * G + (uint32_t)sx;
-            if (code[src] == (uint8_t)nb8_code(-dx, -dy)) evict_one_move(chem, gm, G, src, d);
-        }
+    uint32_t bits = inc[d];
+    while (bits) {
+        const int j = __builtin_ctz(bits);
+        bits &= bits - 1u;
+        const uint32_t src = (uint32_t)((int)y + NB8_DY[j]) * G + (uint32_t)((int)x + NB8_DX[j]);
+        evict_one_move(chem, gm, G, src, d);
+    }
 }
 
 // grain_fall (terrain.wgsl:405-437) for the columns x = colour + 3k, one
@@ -2500,17 +2545,28 @@ __global__ __launch_bounds__(64) void k_grain_fall(uint32_t G, uint32_t F, RiboS
     if (rate <= 0.0f) return;
     const uint32_t rf = rb_f2u(rate * 65536.0f);
     if (((P.epoch * rf) >> 16) == (((P.epoch + 1u) * rf) >> 16)) return;
+    // Row y's count changes only when rows y and y + 1 are walked, so the
+    // counts are loaded ahead in batches and the walk waits on none of them.
+    constexpr uint32_t B = 32;
     bool supported = true;
-    for (uint32_t y = 0; y < G; ++y) {
-        const uint32_t idx = y * G + x;
-        const uint32_t n = gamma[idx];
-        if (n == 0u) { supported = false; continue; }
-        if (n >= GAMMA_SOLID) { supported = true; continue; }
-        if (supported) continue;
-        for (uint32_t k = 0; k < n; ++k) {
-            gamma[idx] -= 1u;
-            gamma[idx - G] += 1u;
-            evict_one_move(chem, gamma, G, idx, idx - G);
+    for (uint32_t y0 = 0; y0 < G; y0 += B) {
+        uint32_t cnt[B];
+#pragma unroll
+        for (uint32_t j = 0; j < B; ++j) cnt[j] = y0 + j < G ? gamma[(y0 + j) * G + x] : 0u;
+#pragma unroll
+        for (uint32_t j = 0; j < B; ++j) {
+            const uint32_t y = y0 + j;
+            if (y >= G) break;
+            const uint32_t n = cnt[j];
+            if (n == 0u) { supported = false; continue; }
+            if (n >= GAMMA_SOLID) { supported = true; continue; }
+            if (supported) continue;
+            const uint32_t idx = y * G + x;
+            for (uint32_t k = 0; k < n; ++k) {  // gamma_move_one(idx, idx - G), grain by grain
+                gamma[idx] -= 1u;
+                gamma[idx - G] += 1u;
+                evict_one_move(chem, gamma, G, idx, idx - G);
+            }
         }
     }
 }
@@ -3211,14 +3267,14 @@ void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t
     k_chem_ops<<<(n + 255u) / 256u, 256, 0, s>>>(chem, g.gamma, g.G, (const uint4*)ops, n, done, expected);
 }
 void launch_relax_pass(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t phase,
-                       const float2* vel, uint32_t* gamma, uint32_t* chem, uint8_t* code) {
+                       const float2* vel, uint32_t* gamma, uint32_t* chem, uint8_t* code, uint8_t* inc) {
     k_relax_decide<<<grid2(g.G), B16, 0, s>>>(g, P, ep, phase, vel, code);
-    k_relax_apply<<<grid2(g.G), B16, 0, s>>>(g.G, gamma, code);
+    k_relax_apply<<<grid2(g.G), B16, 0, s>>>(g.G, gamma, code, inc);
     for (uint32_t c = 0; c < 9u; ++c) {
         const uint32_t cx = c % 3u, cy = c / 3u;
         if (cx >= g.G || cy >= g.G) continue;
         const uint32_t n = ((g.G - cx + 2u) / 3u) * ((g.G - cy + 2u) / 3u);
-        k_relax_evict<<<(n + 255u) / 256u, 256, 0, s>>>(g.G, cx, cy, chem, gamma, code);
+        k_relax_evict<<<(n + 255u) / 256u, 256, 0, s>>>(g.G, cx, cy, chem, gamma, inc);
     }
 }
 void launch_grain_fall(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, uint32_t* gamma,

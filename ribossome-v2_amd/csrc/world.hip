@@ -123,6 +123,7 @@ struct RiboWorld {
     float2* forces = nullptr;
     uint32_t* agent_force = nullptr;  // force_vectors from the agents, i32[2F] fixed point
     uint8_t* terrain_code = nullptr;  // per grid cell: the move a relax pass decided (0: none)
+    uint8_t* terrain_inc = nullptr;   // per grid cell: the neighbours whose grain arrived (bit mask)
     bool agent_force_active = false;  // consumed (added, then cleared) by the next fluid step
     float2* heat = nullptr;
     float* net_flux = nullptr;
@@ -572,7 +573,7 @@ void terrain_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {
     const Geo g = w->geo();
     for (uint32_t phase = 0; phase < 2u; ++phase)
         run(w, phase ? "relax_gamma_b" : "relax_gamma_a",
-            [&] { launch_relax_pass(s, g, P, ep, phase, w->vel_a, w->gamma, w->chem[w->cur], w->terrain_code); });
+            [&] { launch_relax_pass(s, g, P, ep, phase, w->vel_a, w->gamma, w->chem[w->cur], w->terrain_code, w->terrain_inc); });
     if (P.sediment_settle > 0.0f) run(w, "grain_fall", [&] { launch_grain_fall(s, g, P, ep, w->gamma, w->chem[w->cur]); });
     run(w, "compute_gamma_slope", [&] { launch_slope(s, g, w->slope); });
 }
@@ -802,7 +803,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     const size_t Ls = cfg->grid_size / LIGHT_DIV, nb = (cfg->fluid_size + 15) / 16;
     size_t b = 2 * a(4 * G, 4) + a(G, 4) + 2 * a(2 * Ls * Ls, 4) + a(G, 8) + a(G, 4);  // chem x2, gamma, light x2, slope, agg
     b += 2 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x3, div, temp/redox x2, eaten, forces, heat
-    b += a(2 * F, 4) + a(G, 1);  // the agents' force words, the grain move codes
+    b += a(2 * F, 4) + 2 * a(G, 1);  // the agents' force words, the grain move codes and masks
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
     b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 7 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
     b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
@@ -867,7 +868,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(w->stream, &w->p_old, F)) ||
         (rc = dalloc(w->stream, &w->div, F)) || (rc = dalloc(w->stream, &w->temp[0], F)) || (rc = dalloc(w->stream, &w->temp[1], F)) ||
         (rc = dalloc(w->stream, &w->redox[0], F)) || (rc = dalloc(w->stream, &w->redox[1], F)) || (rc = dalloc(w->stream, &w->eaten, F)) ||
-        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->agent_force, 2 * F)) || (rc = dalloc(w->stream, &w->terrain_code, G)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
+        (rc = dalloc(w->stream, &w->forces, F)) || (rc = dalloc(w->stream, &w->agent_force, 2 * F)) || (rc = dalloc(w->stream, &w->terrain_code, G)) || (rc = dalloc(w->stream, &w->terrain_inc, G)) || (rc = dalloc(w->stream, &w->heat, F)) || (rc = dalloc(w->stream, &w->net_flux, 2 * (size_t)w->F)) ||
         (rc = dalloc(w->stream, &w->cflags, F)) || (rc = dalloc(w->stream, &w->normal, F)) || (rc = dalloc(w->stream, &w->fill, F)) ||
         (rc = dalloc(w->stream, &w->dragg, F)) || (rc = dalloc(w->stream, &w->slope_f, F)) || (rc = dalloc(w->stream, &w->pcg, 1)) ||
         (rc = dalloc(w->stream, &w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(w->stream, &w->pcg_r, F)) ||
@@ -906,7 +907,7 @@ int ribo_world_destroy(RiboWorld* w) {
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
     dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light[0]); dfree(w->light[1]); dfree(w->slope); dfree(w->agg_e);
     dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->p_old); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
-    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->terrain_code); dfree(w->heat); dfree(w->net_flux);
+    dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->terrain_code); dfree(w->terrain_inc); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
     dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w);

@@ -514,8 +514,15 @@ def serve(scenes, budget, max_running=0, arrivals=None, threads=0):
     at = arrivals if arrivals is not None else np.zeros(n)
     tickets, nxt = [], 0
     t0 = time.perf_counter()
-    while nxt < n or any(srv.status(t).state in (ra.SCENE_QUEUED, ra.SCENE_RUNNING) for t in tickets):
+    last_note = 0.0
+    while True:
+        pending = sum(srv.status(t).state in (ra.SCENE_QUEUED, ra.SCENE_RUNNING) for t in tickets)
+        if nxt >= n and pending == 0:
+            break
         now = time.perf_counter() - t0
+        if now - last_note >= 20.0:  # a progress line on stderr while a long serving run goes on
+            print(f"serving: {now:.0f} s, {nxt} of {n} submitted, {nxt - pending} done", file=sys.stderr, flush=True)
+            last_note = now
         while nxt < n and at[nxt] <= now:
             cfg, seed, steps, p, s, inp = scenes[nxt]
             tickets.append(srv.submit(cfg, seed, steps, p, s, inp))

@@ -55,6 +55,33 @@ RIBO_API int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, cons
 RIBO_API int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams* params,
                              RiboWorldSettings* settings, RiboSceneGrids* grids);
 
+/* A block a caller adds to a scene: one of the reference's blocks besides
+ * the world grids ("slots", "life_counters" or an agent block: "agents",
+ * "genomes", "bodies", "organs", "rna_tails4", "bitten", "body_pos",
+ * "signals4", "joint_angle", "joint_base", "joint_state", "joint_active",
+ * "sensor_mem", "bonds2", "pair_mask"), in the reference's own layout;
+ * the library passes the bytes through. */
+typedef struct RiboSceneBlock {
+    const char* name;
+    const void* data;
+    uint64_t bytes;
+} RiboSceneBlock;
+
+/* ribo_scene_write with a caller's agents (World::save_scene,
+ * src/world/snapshot.rs:274-363): the header's "agentes" count (at most
+ * cfg->max_agents) and the blocks given; the blocks not given are written
+ * empty (life_counters as eight zero words).  RIBO_ERR_ARG for an unknown or
+ * repeated name. */
+RIBO_API int ribo_scene_write_agents(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+                                     const RiboWorldSettings* settings, const RiboSceneGrids* grids, uint32_t agents,
+                                     const RiboSceneBlock* blocks, uint32_t n_blocks);
+/* One block of a scene, decompressed (any name, the grids included):
+ * *bytes receives its size; dst may be NULL with cap 0 to ask the size.
+ * RIBO_ERR_ARG for a malformed file, a missing block or cap < *bytes. */
+RIBO_API int ribo_scene_read_block(const char* path, const char* name, void* dst, uint64_t cap, uint64_t* bytes);
+/* The header's agent count ("agentes"). */
+RIBO_API int ribo_scene_agents(const char* path, uint32_t* agents);
+
 /* World::save_scene / Scene::read + World::load_scene on a world: its grids,
  * parameters and settings. */
 RIBO_API int ribo_world_save_scene(RiboWorld* w, const char* path);

@@ -461,9 +461,28 @@ int64_t ribo_lz4_decompress(const void* src, size_t n, void* dst, size_t dst_cap
 
 int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
                      const RiboWorldSettings* settings, const RiboSceneGrids* grids) {
-    if (!path || !cfg || !params || !settings || !grids) return RIBO_ERR_ARG;
-    // the header (src/world/snapshot.rs:325-343), without agents
-    std::string h = "{\n  \"versao\": " + std::to_string(SCENE_VERSION) + ",\n  \"gravado\": 0,\n  \"agentes\": 0,\n";
+    return ribo_scene_write_agents(path, cfg, params, settings, grids, 0, nullptr, 0);
+}
+
+int ribo_scene_write_agents(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+                            const RiboWorldSettings* settings, const RiboSceneGrids* grids, uint32_t agents,
+                            const RiboSceneBlock* blocks, uint32_t n_blocks) {
+    if (!path || !cfg || !params || !settings || !grids || (n_blocks && !blocks)) return RIBO_ERR_ARG;
+    if (agents > cfg->max_agents) return RIBO_ERR_ARG;
+    // the caller's blocks: only the names the reference's scene carries
+    // besides the world grids (slots, life_counters, the agent blocks), each once
+    std::map<std::string, const RiboSceneBlock*> given;
+    for (uint32_t i = 0; i < n_blocks; ++i) {
+        if (!blocks[i].name || (blocks[i].bytes && !blocks[i].data)) return RIBO_ERR_ARG;
+        const std::string nm(blocks[i].name);
+        bool known = nm == "slots" || nm == "life_counters";
+        for (const char* a : AGENT_BLOCKS) known = known || nm == a;
+        if (!known || given.count(nm)) return RIBO_ERR_ARG;
+        given[nm] = &blocks[i];
+    }
+    // the header (src/world/snapshot.rs:325-343)
+    std::string h = "{\n  \"versao\": " + std::to_string(SCENE_VERSION) + ",\n  \"gravado\": 0,\n  \"agentes\": " +
+                    std::to_string(agents) + ",\n";
     h += "  \"cfg\": " + cfg_json(*cfg) + ",\n  \"params\": {";
     bool first = true;
     for (const ParamField& f : PARAM_FIELDS) {
@@ -485,9 +504,14 @@ int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSim
     out.insert(out.end(), (const uint8_t*)&hl, (const uint8_t*)&hl + 4);
     out.insert(out.end(), h.begin(), h.end());
     uint32_t counters[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // life_counters of a world without agents
-    put_block(out, "slots", nullptr, 0);
-    put_block(out, "life_counters", counters, sizeof counters);
-    for (const char* name : AGENT_BLOCKS) put_block(out, name, nullptr, 0);
+    auto put_given = [&](const char* name, const void* data, size_t bytes) {
+        auto it = given.find(name);
+        if (it != given.end()) return put_block(out, name, it->second->data, it->second->bytes);
+        return put_block(out, name, data, bytes);
+    };
+    if (!put_given("slots", nullptr, 0) || !put_given("life_counters", counters, sizeof counters)) return RIBO_ERR_ARG;
+    for (const char* name : AGENT_BLOCKS)
+        if (!put_given(name, nullptr, 0)) return RIBO_ERR_ARG;
     RiboSceneGrids g = *grids;
     for (int k = 0; k < NGRID; ++k) {
         const void* data = *grid_ptr(&g, k);
@@ -505,41 +529,47 @@ int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSim
     return RIBO_OK;
 }
 
-int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams* params, RiboWorldSettings* settings,
-                    RiboSceneGrids* grids) {
-    if (!path || !cfg || !params || !settings || !grids) return RIBO_ERR_ARG;
-    FILE* f = fopen(path, "rb");
-    if (!f) return RIBO_ERR_ARG;
+namespace {
+
+// A scene file in memory: Scene::read (src/world/snapshot.rs:145-167).
+struct SceneFile {
     std::vector<uint8_t> bytes;
-    uint8_t buf[1 << 16];
-    size_t got;
-    while ((got = fread(buf, 1, sizeof buf, f)) > 0) bytes.insert(bytes.end(), buf, buf + got);
-    fclose(f);
-    // Scene::read (src/world/snapshot.rs:145-167)
-    if (bytes.size() < 12 || memcmp(bytes.data(), SCENE_MAGIC, 8) != 0) return RIBO_ERR_ARG;
-    uint32_t hl;
-    memcpy(&hl, bytes.data() + 8, 4);
-    if ((size_t)hl > bytes.size() - 12) return RIBO_ERR_ARG;
-    JsonParser jp{(const char*)bytes.data() + 12, (const char*)bytes.data() + 12 + hl};
-    const Json header = jp.value();
-    if (!jp.ok || header.kind != Json::OBJ) return RIBO_ERR_ARG;
+    Json header;
     std::map<std::string, std::pair<size_t, size_t>> blocks;  // name -> (offset, length) of the compressed block
-    size_t p = 12 + (size_t)hl;
-    while (p < bytes.size()) {
-        if (bytes.size() - p < 2) return RIBO_ERR_ARG;
-        uint16_t nl;
-        memcpy(&nl, bytes.data() + p, 2);
-        if (bytes.size() - p - 2 < (size_t)nl + 8) return RIBO_ERR_ARG;
-        const std::string name((const char*)bytes.data() + p + 2, nl);
-        p += 2 + nl;
-        uint64_t cl;
-        memcpy(&cl, bytes.data() + p, 8);
-        p += 8;
-        if (cl > bytes.size() - p || cl < 4) return RIBO_ERR_ARG;
-        blocks[name] = {p, (size_t)cl};
-        p += (size_t)cl;
+
+    bool open(const char* path) {
+        FILE* f = fopen(path, "rb");
+        if (!f) return false;
+        uint8_t buf[1 << 16];
+        size_t got;
+        while ((got = fread(buf, 1, sizeof buf, f)) > 0) bytes.insert(bytes.end(), buf, buf + got);
+        fclose(f);
+        if (bytes.size() < 12 || memcmp(bytes.data(), SCENE_MAGIC, 8) != 0) return false;
+        uint32_t hl;
+        memcpy(&hl, bytes.data() + 8, 4);
+        if ((size_t)hl > bytes.size() - 12) return false;
+        JsonParser jp{(const char*)bytes.data() + 12, (const char*)bytes.data() + 12 + hl};
+        header = jp.value();
+        if (!jp.ok || header.kind != Json::OBJ) return false;
+        size_t p = 12 + (size_t)hl;
+        while (p < bytes.size()) {
+            if (bytes.size() - p < 2) return false;
+            uint16_t nl;
+            memcpy(&nl, bytes.data() + p, 2);
+            if (bytes.size() - p - 2 < (size_t)nl + 8) return false;
+            const std::string name((const char*)bytes.data() + p + 2, nl);
+            p += 2 + nl;
+            uint64_t cl;
+            memcpy(&cl, bytes.data() + p, 8);
+            p += 8;
+            if (cl > bytes.size() - p || cl < 4) return false;
+            blocks[name] = {p, (size_t)cl};
+            p += (size_t)cl;
+        }
+        return true;
     }
-    auto block = [&](const std::string& name, std::vector<uint8_t>* out) -> int {  // decompress_size_prepended
+    // decompress_size_prepended of one block: 0, 1 when absent, -1 when malformed
+    int block(const std::string& name, std::vector<uint8_t>* out) const {
         auto it = blocks.find(name);
         if (it == blocks.end()) return 1;
         uint32_t raw;
@@ -547,7 +577,40 @@ int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams*
         out->assign(raw, 0);
         const int64_t r = lz4_decompress(bytes.data() + it->second.first + 4, it->second.second - 4, out->data(), raw);
         return r == (int64_t)raw ? 0 : -1;
-    };
+    }
+};
+
+}  // namespace
+
+int ribo_scene_read_block(const char* path, const char* name, void* dst, uint64_t cap, uint64_t* bytes) {
+    if (!path || !name || !bytes || (cap && !dst)) return RIBO_ERR_ARG;
+    SceneFile sf;
+    if (!sf.open(path)) return RIBO_ERR_ARG;
+    std::vector<uint8_t> data;
+    const int rc = sf.block(name, &data);
+    if (rc != 0) return RIBO_ERR_ARG;
+    *bytes = data.size();
+    if (data.size() > cap) return RIBO_ERR_ARG;
+    if (!data.empty()) memcpy(dst, data.data(), data.size());
+    return RIBO_OK;
+}
+
+int ribo_scene_agents(const char* path, uint32_t* agents) {
+    if (!path || !agents) return RIBO_ERR_ARG;
+    SceneFile sf;
+    if (!sf.open(path)) return RIBO_ERR_ARG;
+    const Json* a = sf.header.get("agentes");
+    *agents = a && a->kind == Json::NUM && a->num > 0.0 ? (uint32_t)a->num : 0u;
+    return RIBO_OK;
+}
+
+int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams* params, RiboWorldSettings* settings,
+                    RiboSceneGrids* grids) {
+    if (!path || !cfg || !params || !settings || !grids) return RIBO_ERR_ARG;
+    SceneFile sf;
+    if (!sf.open(path)) return RIBO_ERR_ARG;
+    const Json& header = sf.header;
+    auto block = [&](const std::string& name, std::vector<uint8_t>* out) { return sf.block(name, out); };
     // World::load_scene's checks (src/world/snapshot.rs:370-400): the same
     // world size (a smaller max_agents in the file is accepted)
     const Json* c = header.get("cfg");

@@ -314,6 +314,9 @@ def lib() -> ctypes.CDLL:
         "ribo_world_mg_tiles": (I, [P, P, U32, ctypes.POINTER(U32)]),
         "ribo_scene_write": (I, [ctypes.c_char_p, P, P, P, P]),
         "ribo_scene_read": (I, [ctypes.c_char_p, P, P, P, P]),
+        "ribo_scene_write_agents": (I, [ctypes.c_char_p, P, P, P, P, U32, P, U32]),
+        "ribo_scene_read_block": (I, [ctypes.c_char_p, ctypes.c_char_p, P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)]),
+        "ribo_scene_agents": (I, [ctypes.c_char_p, ctypes.POINTER(U32)]),
         "ribo_world_save_scene": (I, [P, ctypes.c_char_p]),
         "ribo_world_load_scene": (I, [P, ctypes.c_char_p]),
         "ribo_lz4_bound": (SZ, [SZ]),
@@ -702,8 +705,15 @@ SCENE_GRIDS = (("chem", CHEM), ("gamma", GAMMA), ("velocity", VELOCITY), ("press
                ("temperature", TEMP), ("redox", REDOX), ("heat_src", HEAT_SRC))
 
 
-def scene_write(path: str, cfg: WorldConfig, params: SimParams, settings: WorldSettings, grids: dict):
-    """ribo_scene_write: grids maps CHEM, GAMMA, VELOCITY, PRESSURE, TEMP, REDOX to host arrays."""
+class _CSceneBlock(ctypes.Structure):
+    _fields_ = [("name", ctypes.c_char_p), ("data", ctypes.c_void_p), ("bytes", ctypes.c_uint64)]
+
+
+def scene_write(path: str, cfg: WorldConfig, params: SimParams, settings: WorldSettings, grids: dict,
+                agents: int = 0, blocks: "dict | None" = None):
+    """ribo_scene_write (ribo_scene_write_agents with agents or blocks): grids
+    maps CHEM, GAMMA, VELOCITY, PRESSURE, TEMP, REDOX (and HEAT_SRC) to host
+    arrays; blocks maps the reference's agent block names to their bytes."""
     g = _CSceneGrids()
     keep = []
     for name, which in SCENE_GRIDS:
@@ -715,8 +725,40 @@ def scene_write(path: str, cfg: WorldConfig, params: SimParams, settings: WorldS
         keep.append(a)
         setattr(g, name, a.ctypes.data)
     c = cfg.c()
-    _check(lib().ribo_scene_write(os.fsencode(path), ctypes.byref(c), ctypes.byref(params), ctypes.byref(settings),
-                                  ctypes.byref(g)), "scene_write")
+    if not agents and not blocks:
+        _check(lib().ribo_scene_write(os.fsencode(path), ctypes.byref(c), ctypes.byref(params), ctypes.byref(settings),
+                                      ctypes.byref(g)), "scene_write")
+        return
+    items = list((blocks or {}).items())
+    arr = (_CSceneBlock * max(len(items), 1))()
+    for i, (name, data) in enumerate(items):
+        b = np.frombuffer(bytes(data), dtype=np.uint8).copy()
+        keep.append(b)
+        arr[i].name = name.encode()
+        arr[i].data = b.ctypes.data if b.size else None
+        arr[i].bytes = b.size
+    _check(lib().ribo_scene_write_agents(os.fsencode(path), ctypes.byref(c), ctypes.byref(params),
+                                         ctypes.byref(settings), ctypes.byref(g), int(agents), arr, len(items)),
+           "scene_write_agents")
+
+
+def scene_read_block(path: str, name: str) -> bytes:
+    """ribo_scene_read_block: one block of a scene file, decompressed."""
+    n = ctypes.c_uint64()
+    rc = lib().ribo_scene_read_block(os.fsencode(path), name.encode(), None, 0, ctypes.byref(n))
+    if rc != 0 and n.value == 0:
+        _check(rc, f"scene_read_block {name}")
+    buf = np.empty(max(int(n.value), 1), dtype=np.uint8)
+    _check(lib().ribo_scene_read_block(os.fsencode(path), name.encode(), buf.ctypes.data, int(n.value), ctypes.byref(n)),
+           f"scene_read_block {name}")
+    return buf[:n.value].tobytes()
+
+
+def scene_agents(path: str) -> int:
+    """ribo_scene_agents: the header's agent count."""
+    n = ctypes.c_uint32()
+    _check(lib().ribo_scene_agents(os.fsencode(path), ctypes.byref(n)), "scene_agents")
+    return int(n.value)
 
 
 def scene_read(path: str, cfg: WorldConfig, seed: int = 1, settings: "WorldSettings | None" = None):

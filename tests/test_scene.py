@@ -219,3 +219,37 @@ def test_scene_read_rules(tmp_path):
     write_raw(bad, header, short)
     with pytest.raises(ra.RiboError):
         ra.scene_read(bad, cfg)
+
+
+def test_agent_blocks_pass_through(tmp_path):
+    """A caller with agents writes the reference's agent blocks and count into
+    the scene and reads them back; blocks not given are written empty; only
+    the reference's block names are accepted."""
+    cfg = ra.WorldConfig(64, 32, 30, 100)
+    g = grids(cfg, 8)
+    p, s = ra.SimParams.default(3), ra.WorldSettings.default()
+    rng = np.random.default_rng(8)
+    agents = rng.integers(0, 1 << 32, 40 * 17, dtype=np.uint64).astype(np.uint32).tobytes()
+    genomes = bytes(rng.integers(0, 256, 5000, dtype=np.uint8))
+    counters = np.arange(8, dtype=np.uint32).tobytes()
+    path = str(tmp_path / "vida.ribo")
+    ra.scene_write(path, cfg, p, s, g, agents=40,
+                   blocks={"agents": agents, "genomes": genomes, "life_counters": counters, "pair_mask": b""})
+    header, blocks = parse_file(path)
+    assert header["agentes"] == 40 and ra.scene_agents(path) == 40
+    assert blocks["agents"] == agents and blocks["genomes"] == genomes and blocks["life_counters"] == counters
+    for name in set(AGENT_BLOCKS) - {"agents", "genomes"}:
+        assert blocks[name] == b"", name
+    assert ra.scene_read_block(path, "genomes") == genomes
+    assert ra.scene_read_block(path, "agents") == agents
+    assert ra.scene_read_block(path, "bodies") == b""
+    assert ra.scene_read_block(path, "chem") == g[ra.CHEM].tobytes()
+    with pytest.raises(ra.RiboError):
+        ra.scene_read_block(path, "no_such_block")
+    q, t, h = ra.scene_read(path, cfg, seed=3)  # the world part reads as before
+    assert np.array_equal(h[ra.GAMMA], g[ra.GAMMA])
+    for bad in ({"chem": b"x"}, {"no_such": b"x"}):
+        with pytest.raises(ra.RiboError):
+            ra.scene_write(str(tmp_path / "bad.ribo"), cfg, p, s, g, agents=1, blocks=bad)
+    with pytest.raises(ra.RiboError):
+        ra.scene_write(str(tmp_path / "bad.ribo"), cfg, p, s, g, agents=101)  # more than max_agents

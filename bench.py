@@ -69,6 +69,11 @@ BYTES_PER_UNIT = {
     "relax_gamma_a": (15, "G"),
     "relax_gamma_b": (15, "G"),
     "compute_gamma_slope": (12, "G"),
+    # agents' coupling points: the smoothed velocity reads the velocity once
+    # (the neighbours from cache) and writes 8 B; the force gather with agent
+    # words reads and clears 8 B more than buoyancy_gather
+    "smooth_velocity": (16, "F"),
+    "buoyancy_gather_agents": (28, "F"),
     # tolerance-mode solver, level 0 (S = bytes of the preconditioner's storage)
     "pcg_init": (33, "F"),
     "pcg_final": (9, "F"),
@@ -322,6 +327,62 @@ def terrain_measure(cfg, inputs, args):
             e.update({"algorithmic_bytes_per_launch": b, "achieved_GB_per_s": b / (us * 1e-6) / 1e9,
                       "frac_of_8TBps": b / (us * 1e-6) / 8e12})
         out[k] = e
+    return out
+
+
+def agents_measure(cfg, inputs, args):
+    """The agents' coupling points on the same 2048^2 world: the force
+    gather of a fluid step that consumes uploaded force words (5 % of the
+    fluid cells pushed), the smoothed-velocity download's kernel, and one
+    million monomer exchanges on distinct words (spend, activate, take, add),
+    with HIP-event times; the exchanges' bytes are the record read, the
+    count written and the 4-byte word read and written (in 64-byte sectors,
+    one per exchange)."""
+    w = build_world(cfg, inputs, args, "reference")
+    rng = np.random.default_rng(77)
+    F = cfg.fluid_cells()
+    words = np.zeros(2 * F, np.int32)
+    cells = rng.choice(F, F // 20, replace=False)
+    words[2 * cells] = rng.integers(-4000, 4000, cells.size)
+    words[2 * cells + 1] = rng.integers(-4000, 4000, cells.size)
+    n_ops = 1 << 20
+    slots = rng.choice(4 * cfg.cells(), n_ops, replace=False)
+    ops = np.stack([slots // 4, slots % 4, rng.integers(0, 6, n_ops), rng.integers(1, 4, n_ops)], 1).astype(np.uint32)
+    w.run(4)
+    w.set_timing(True)
+    w.reset_kernel_times()
+    sub = max(1, min(4, w.settings.fluid_substep))
+    for _ in range(4):
+        w.upload(ra.FORCE_VECTORS, words)
+        w.run(sub)  # holds exactly one fluid step, which consumes the words
+    agent_gather = w.kernel_times().get("buoyancy_gather", (0.0, 0))
+    w.reset_kernel_times()
+    w.run(8)
+    plain_gather = w.kernel_times().get("buoyancy_gather", (0.0, 0))
+    w.reset_kernel_times()
+    for _ in range(4):
+        w.download(ra.VELOCITY_SMOOTH)
+        w.chem_ops(ops)
+    times = w.kernel_times()
+    w.set_timing(False)
+    w.close()
+    out = {}
+
+    def entry(us, b):
+        return {"avg_launch_us": us, "algorithmic_bytes_per_launch": b, "achieved_GB_per_s": b / (us * 1e-6) / 1e9,
+                "frac_of_8TBps": b / (us * 1e-6) / 8e12}
+
+    for key, (tot, cnt), name in (("buoyancy_gather_with_agent_words", agent_gather, "buoyancy_gather_agents"),
+                                  ("buoyancy_gather_plain", plain_gather, "buoyancy_gather")):
+        if cnt:
+            out[key] = entry(tot / cnt * 1000.0, kernel_bytes(name, cfg, [], "reference"))
+    tot, cnt = times.get("smooth_velocity", (0.0, 0))
+    if cnt:
+        out["smooth_velocity"] = entry(tot / cnt * 1000.0, kernel_bytes("smooth_velocity", cfg, [], "reference"))
+    tot, cnt = times.get("chem_ops", (0.0, 0))
+    if cnt:
+        us = tot / cnt * 1000.0
+        out["chem_ops"] = dict(entry(us, n_ops * (16 + 4 + 64)), exchanges=n_ops, exchanges_per_s=n_ops / (us * 1e-6))
     return out
 
 
@@ -586,6 +647,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64-oracle error sample")
     ap.add_argument("--no-light", action="store_true", help="skip the light-step kernel timing")
+    ap.add_argument("--no-agents", action="store_true", help="skip the agents' coupling-point timing")
     ap.add_argument("--terrain-steps", type=int, default=64,
                     help="steps of the moving-terrain measurement (0 skips it)")
     ap.add_argument("--vcycle-steps", type=int, default=100,
@@ -690,6 +752,7 @@ def main():
 
     light = None if args.no_light else light_measure(cfg, inputs, args)
     terrain = terrain_measure(cfg, inputs, args) if args.terrain_steps > 0 else None
+    agents = None if args.no_agents else agents_measure(cfg, inputs, args)
     plain = {}
     if args.vcycle_steps > 0:
         # the stationary V-cycle, on a bounded number of steps (each step runs
@@ -752,6 +815,7 @@ def main():
         "hbm_step_model": hbm,
         "light": light,
         "terrain": terrain,
+        "agents": agents,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,
     }

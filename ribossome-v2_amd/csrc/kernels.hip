@@ -2182,28 +2182,36 @@ __device__ uint32_t chem_open_cell(const uint32_t* __restrict__ gamma, uint32_t 
 __global__ __launch_bounds__(256) void k_chem_ops(uint32_t* chem, const uint32_t* __restrict__ gamma, uint32_t G,
                                                   const uint4* __restrict__ ops, uint32_t n, uint32_t* __restrict__ done,
                                                   uint32_t* expected) {
+    // the net change per channel is summed in the workgroup first: one
+    // device-wide atomic per channel and workgroup instead of one per exchange
+    __shared__ uint32_t acc[4];
+    if (threadIdx.x < 4) acc[threadIdx.x] = 0u;
+    __syncthreads();
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= n) return;
-    const uint4 op = ops[i];
-    const uint32_t cell = op.x, ch = op.y, kind = op.z, count = op.w;
-    uint32_t moved = 0, net = 0;
-    if (kind <= RIBO_CHEM_TAKE_SPENT) {
-        // spend: -1 activated +1 spent; activate: the reverse; take: -1 in that state
-        const uint32_t delta = kind == RIBO_CHEM_SPEND ? CHEM_SPENT_ONE - 1u
-                               : kind == RIBO_CHEM_ACTIVATE ? 0xFFFF0001u
-                               : kind == RIBO_CHEM_TAKE_ACT ? 0xFFFFFFFFu : 0xFFFF0000u;
-        const bool need_act = kind == RIBO_CHEM_SPEND || kind == RIBO_CHEM_TAKE_ACT;
-        uint32_t* slot = chem + (size_t)cell * 4u + ch;
-        while (moved < count && chem_cas(slot, delta, need_act, !need_act)) ++moved;
-        if (kind >= RIBO_CHEM_TAKE_ACT) net = 0u - moved;
-    } else {
-        const bool spent = kind == RIBO_CHEM_ADD_SPENT || kind == RIBO_CHEM_DEPOSIT_SPENT;
-        const uint32_t c = kind >= RIBO_CHEM_DEPOSIT_ACT ? chem_open_cell(gamma, G, cell) : cell;
-        if (count > 0u) atomicAdd(chem + (size_t)c * 4u + ch, spent ? count * CHEM_SPENT_ONE : count);  // chem_add_state
-        moved = net = count;
+    if (i < n) {
+        const uint4 op = ops[i];
+        const uint32_t cell = op.x, ch = op.y, kind = op.z, count = op.w;
+        uint32_t moved = 0, net = 0;
+        if (kind <= RIBO_CHEM_TAKE_SPENT) {
+            // spend: -1 activated +1 spent; activate: the reverse; take: -1 in that state
+            const uint32_t delta = kind == RIBO_CHEM_SPEND ? CHEM_SPENT_ONE - 1u
+                                   : kind == RIBO_CHEM_ACTIVATE ? 0xFFFF0001u
+                                   : kind == RIBO_CHEM_TAKE_ACT ? 0xFFFFFFFFu : 0xFFFF0000u;
+            const bool need_act = kind == RIBO_CHEM_SPEND || kind == RIBO_CHEM_TAKE_ACT;
+            uint32_t* slot = chem + (size_t)cell * 4u + ch;
+            while (moved < count && chem_cas(slot, delta, need_act, !need_act)) ++moved;
+            if (kind >= RIBO_CHEM_TAKE_ACT) net = 0u - moved;
+        } else {
+            const bool spent = kind == RIBO_CHEM_ADD_SPENT || kind == RIBO_CHEM_DEPOSIT_SPENT;
+            const uint32_t c = kind >= RIBO_CHEM_DEPOSIT_ACT ? chem_open_cell(gamma, G, cell) : cell;
+            if (count > 0u) atomicAdd(chem + (size_t)c * 4u + ch, spent ? count * CHEM_SPENT_ONE : count);  // chem_add_state
+            moved = net = count;
+        }
+        if (done) done[i] = moved;
+        if (net != 0u) atomicAdd(&acc[ch], net);
     }
-    if (done) done[i] = moved;
-    if (net != 0u) atomicAdd(expected + ch, net);
+    __syncthreads();
+    if (threadIdx.x < 4 && acc[threadIdx.x] != 0u) atomicAdd(expected + threadIdx.x, acc[threadIdx.x]);
 }
 
 // ---- terrain grain dynamics (terrain.wgsl:38-102,159-437) -------------------------------

@@ -1032,7 +1032,7 @@ int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes)
             src = w->scratch_u32;
             break;
         case RIBO_BUF_VELOCITY_SMOOTH:  // the force buffer is dead between steps
-            launch_smooth_velocity(w->stream, w->geo(), w->vel_a, w->forces);
+            run(w, "smooth_velocity", [&] { launch_smooth_velocity(w->stream, w->geo(), w->vel_a, w->forces); });
             HIPCHK(hipGetLastError());
             src = w->forces;
             break;
@@ -1303,7 +1303,8 @@ int ribo_world_chem_ops(RiboWorld* w, const RiboChemOp* ops, uint32_t n, uint32_
     for (uint32_t b = 0; b < n; b += batch) {
         const uint32_t m = std::min(batch, n - b);
         HIPCHK(hipMemcpyAsync(w->forces, ops + b, (size_t)m * sizeof(RiboChemOp), hipMemcpyHostToDevice, w->stream));
-        launch_chem_ops(w->stream, g, w->chem[w->cur], (const uint32_t*)w->forces, m, w->scratch_u32, w->ledger->expected);
+        run(w, "chem_ops",
+            [&] { launch_chem_ops(w->stream, g, w->chem[w->cur], (const uint32_t*)w->forces, m, w->scratch_u32, w->ledger->expected); });
         HIPCHK(hipGetLastError());
         if (done) HIPCHK(hipMemcpyAsync(done + b, w->scratch_u32, (size_t)m * 4, hipMemcpyDeviceToHost, w->stream));
     }

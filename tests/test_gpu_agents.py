@@ -130,3 +130,20 @@ def test_competing_exchanges_keep_the_totals():
     assert np.array_equal(got, want)
     gw.encode_steps(3)
     assert gw.ledger_status() is None
+
+
+def test_coupling_points_reject_bad_input():
+    """Exchanges out of range are refused before anything changes; the
+    smoothed velocity cannot be uploaded; force words must be 2F words."""
+    print("case agent errors", flush=True)
+    cfg, inp, gw, _ = exchange_world(35, ledger=False)
+    before = gw.download(ra.CHEM)
+    assert gw.chem_ops(np.zeros((0, 4), np.uint32)).size == 0
+    for bad in ([(cfg.cells(), 0, ra.CHEM_ADD_ACT, 1)], [(0, 4, ra.CHEM_ADD_ACT, 1)], [(0, 0, 8, 1)]):
+        with pytest.raises(ra.RiboError):
+            gw.chem_ops([(5, 1, ra.CHEM_ADD_ACT, 3)] + bad)
+    assert np.array_equal(gw.download(ra.CHEM), before)
+    with pytest.raises(ra.RiboError):
+        gw.upload(ra.VELOCITY_SMOOTH, np.zeros(2 * cfg.fluid_cells(), np.float32))
+    with pytest.raises(ra.RiboError):
+        gw.upload(ra.FORCE_VECTORS, np.zeros(cfg.fluid_cells(), np.int32))

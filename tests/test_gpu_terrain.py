@@ -22,6 +22,11 @@ CASES = {
                          settings=dict(fluid_substep=2)),
     "jacobi_eager": dict(cfg=ra.WorldConfig(96, 48), seed=45, params={}, settings=dict(multigrid=0, jacobi_iters=16,
                                                                                        no_graphs=1)),
+    # the light step reads the grains the terrain stage has just moved
+    "light_schedule": dict(cfg=ra.WorldConfig(128, 64), seed=49,
+                           params=dict(uv_strength=3.0, uv_depth=4.0, day_period=0.0, direct_photoactivation=0.5,
+                                       sun_heat=1.0),
+                           settings=dict(light_mode=1, light_rows_per_step=2)),
 }
 
 

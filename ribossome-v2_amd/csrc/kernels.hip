@@ -2446,16 +2446,24 @@ __global__ __launch_bounds__(256) void k_relax_decide(Geo g, RiboSimParams P, co
 // bit j for a grain from neighbour j (row order), for the evictions.
 __global__ __launch_bounds__(256) void k_relax_apply(uint32_t G, uint32_t* __restrict__ gamma, const uint8_t* __restrict__ code,
                                                      uint8_t* __restrict__ inc) {
-    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    // the tile's move codes and a 1-cell halo (0 outside the world)
+    __shared__ uint8_t C[OW * OW];
+    const int x0 = (int)blockIdx.x * RT, y0 = (int)blockIdx.y * RT;
+    for (int i = (int)(threadIdx.y * 16 + threadIdx.x); i < OW * OW; i += 256) {
+        const int gx = x0 - 1 + i % OW, gy = y0 - 1 + i / OW;
+        C[i] = (gx >= 0 && gy >= 0 && gx < (int)G && gy < (int)G) ? code[(uint32_t)gy * G + (uint32_t)gx] : (uint8_t)0;
+    }
+    __syncthreads();
+    const uint32_t x = (uint32_t)x0 + threadIdx.x, y = (uint32_t)y0 + threadIdx.y;
     if (x >= G || y >= G) return;
     const uint32_t idx = y * G + x;
-    const uint32_t own = code[idx] != 0u ? 1u : 0u;
+    const int lx = (int)threadIdx.x + 1, ly = (int)threadIdx.y + 1;
+    const uint32_t own = C[ly * OW + lx] != 0u ? 1u : 0u;
     uint32_t in = 0, bits = 0;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
-        const int sx = (int)x - NB8_DX[k], sy = (int)y - NB8_DY[k];  // the neighbour whose move k lands here
-        if (sx < 0 || sy < 0 || sx >= (int)G || sy >= (int)G) continue;
-        if (code[(uint32_t)sy * G + (uint32_t)sx] == (uint8_t)(k + 1)) {
+        // the neighbour whose move k lands here; outside the world its code reads 0
+        if (C[(ly - NB8_DY[k]) * OW + (lx - NB8_DX[k])] == (uint8_t)(k + 1)) {
             in += 1u;
             bits |= 1u << (7 - k);  // that neighbour sits at offset -(move k), row-order index 7 - k
         }

@@ -196,12 +196,14 @@ def timed_run(cfg, inputs, args, mode):
         w.encode_steps(n)
     w.sync()
     before = w.solver_stats()
+    w.solver_residuals(reset=True)
     t0 = time.perf_counter()
     w.run(args.steps)
     w.sync()
     dt = time.perf_counter() - t0
     after = w.solver_stats()
     w.timed_solver_stats = tuple(a - b for a, b in zip(after, before))
+    w.timed_residuals = w.solver_residuals()
     bad = w.ledger_status()
     if bad is not None:
         raise SystemExit(f"{mode}: per-channel monomer totals changed at epoch {bad}")
@@ -689,8 +691,16 @@ def main():
         dt, w = timed_run(cfg, inputs, args, mode)
         it, rel = w.mg_stats()
         solves, iters, missed = w.timed_solver_stats
+        res = w.timed_residuals
+        tol_mode = MODES[mode]["mg_tol"] > 0
         results[mode] = {"value": cfg.cells() * args.steps / dt, "ms_per_step": 1000.0 * dt / args.steps,
                          "last_solve_iterations": it, "last_true_rel_residual": rel,
+                         # over every timed solve: the refined iterate's true
+                         # residual (binary64), the stored binary32 pressure's,
+                         # and the restarts from the true residual
+                         "max_true_rel_residual": res["max"] if tol_mode else None,
+                         "max_stored_pressure_rel_residual": res["max_rounded"] if tol_mode else None,
+                         "restarts": res["restarts"] if tol_mode else None,
                          "mean_iterations": (iters / solves) if solves else None, "solves_missing_tol": missed}
         if mode == args.modes.split(",")[0]:
             head_world = w

@@ -450,6 +450,24 @@ RIBO_API int ribo_world_mg_stats(RiboWorld* w, uint32_t* cycles, double* rel_res
  * mg_tol. */
 RIBO_API int ribo_world_solver_stats(RiboWorld* w, uint64_t* solves, uint64_t* iterations, uint64_t* missed);
 
+/* Residuals of the tolerance mode (no counterpart in the reference, which
+ * runs a fixed number of V-cycles).  The solve iterates on a double-float
+ * x = hi + lo (hi is the stored binary32 pressure), evaluates w = A u and
+ * every residual in binary64 and stops when the true residual
+ * ||div - A x||2 / ||div||2 of x reaches mg_tol, restarting from that true
+ * residual when the recursive one got there first (iterative refinement
+ * with a binary64 refinement residual).  last / max: that residual of the
+ * last solve and its maximum over the solves since the last reset;
+ * last_rounded / max_rounded: the same for the stored binary32 pressure
+ * (hi alone), whose rounding sets a floor of about 2 ulp(|p|) / rms(div);
+ * restarts: restarts since the last reset.  reset != 0 clears the maxima and
+ * the restart count after reading them. */
+typedef struct RiboSolverResiduals {
+    double last, last_rounded, max, max_rounded;
+    uint64_t restarts;
+} RiboSolverResiduals;
+RIBO_API int ribo_world_solver_residuals(RiboWorld* w, RiboSolverResiduals* out, int reset);
+
 /* ---- errors -------------------------------------------------------------- */
 RIBO_API const char* ribo_last_error(void);
 RIBO_API uint32_t ribo_abi_version(void);

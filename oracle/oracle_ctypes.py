@@ -40,6 +40,8 @@ def lib() -> ctypes.CDLL:
             "oracle_run_kernel": (I, [P, ctypes.c_char_p]),
             "oracle_mg_residual": (ctypes.c_double, [P]),
             "oracle_mg_stats": (None, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
+            "oracle_solver_residuals": (None, [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                            ctypes.POINTER(U32)]),
             "oracle_set_threads": (None, [I]),
             "oracle_get_threads": (I, []),
             "oracle_pcg4d": (None, [P, P, SZ]),
@@ -173,6 +175,11 @@ class OracleWorld:
         c, r = ctypes.c_uint32(), ctypes.c_double()
         lib().oracle_mg_stats(self._h, ctypes.byref(c), ctypes.byref(r))
         return int(c.value), float(r.value)
+
+    def solver_residuals(self) -> dict:
+        a, b, c = ctypes.c_double(), ctypes.c_double(), ctypes.c_uint32()
+        lib().oracle_solver_residuals(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c))
+        return {"last": float(a.value), "last_rounded": float(b.value), "restarts": int(c.value)}
 
     def mg_residual(self) -> float:
         return float(lib().oracle_mg_residual(self._h))

@@ -190,8 +190,10 @@ struct OracleWorld {
     uint32_t mg_total = 0;
     Fluid<float> f32;
     Fluid<double> f64;
-    double last_rel_res = -1.0;
+    double last_rel_res = -1.0;      // tolerance mode: the refined iterate's residual (binary64)
+    double last_rounded_rel = -1.0;  // and that of the pressure as stored (binary32 in this world)
     uint32_t last_cycles = 0;
+    uint32_t last_restarts = 0;
 };
 
 namespace {
@@ -1131,6 +1133,24 @@ template <class R> struct MG {
                 out[i] = s - (k + R(L.eps)) * v[i];
             }
     }
+    // A v in binary64 on fluid cells of level 0, for v = hi + lo formed in
+    // binary64 per cell (lo may be null): the fluid neighbours summed in the
+    // order left, right, down, up, minus (k + eps) v; 0 in walls.
+    void apply64(const R* hi, const R* lo, double* out) const {
+        const Level& L = (*lv)[0];
+        auto val = [&](uint32_t j) { return (double)hi[j] + (lo ? (double)lo[j] : 0.0); };
+        for (uint32_t y = 0; y < L.n; ++y)
+            for (uint32_t x = 0; x < L.n; ++x) {
+                const uint32_t i = y * L.n + x;
+                if (fluid[L.off + i] == 0u) { out[i] = 0.0; continue; }
+                double s = 0.0, k = 0.0;
+                if (x > 0 && fluid[L.off + i - 1]) { s += val(i - 1); k += 1.0; }
+                if (x + 1 < L.n && fluid[L.off + i + 1]) { s += val(i + 1); k += 1.0; }
+                if (y > 0 && fluid[L.off + i - L.n]) { s += val(i - L.n); k += 1.0; }
+                if (y + 1 < L.n && fluid[L.off + i + L.n]) { s += val(i + L.n); k += 1.0; }
+                out[i] = s - (k + (double)L.eps) * val(i);
+            }
+    }
     // ||b - A p||2 / ||b||2 over fluid cells of level 0 for level-0 vectors.
     double rel_residual_of(const R* p0, const R* b) const {
         const Level& L = (*lv)[0];
@@ -1497,12 +1517,38 @@ template <class R> void transport(OracleWorld& w, const Fluid<R>& fl, const Ribo
 // exactly linear preconditioner: with the bfloat16 preconditioner the short
 // form stalls and this one converges like binary32.
 //
-// Scalars and inner products are binary64; vectors binary32 (or binary64 in
-// the FP64 oracle).  The stop uses the recursively updated residual; the
-// reported residual is recomputed from the final pressure (in binary32 it
-// cannot fall much below ~1e-5 on rock-laden worlds: the pressure's own
-// rounding bounds it).  precision = MIXED stores the preconditioner's
-// unknowns and right-hand sides (level 0 included) in bfloat16.
+// Precision.  Scalars and inner products are binary64; vectors binary32
+// (binary64 in the FP64 oracle), with three exceptions that keep the
+// recursively updated residual on the true one:
+//   - the iterate is a double-float pair x = hi + lo (hi is the pressure as
+//     stored; every update adds alpha p' with an exact two-sum and
+//     renormalises, so hi stays the nearest stored value of x);
+//   - w = A u and every residual b - A x are evaluated in binary64 (A x on
+//     hi + lo) and rounded once; in binary32 the stencil's cancellation on
+//     smooth vectors (|A v| ~ lambda |v| with lambda down to ~1e-5) would
+//     otherwise put an error of ~3 eps32 / lambda on w;
+//   - the stop is the true residual: once the recursive residual is within
+//     NEAR x tol, each V-cycle also forms ||b - A x||^2 in binary64, and the
+//     solve ends when that reaches tol.  If the recursive residual reaches
+//     tol while the true one has not, the iteration restarts from the true
+//     residual (p = s = 0, the next products applied as a first step): one
+//     step of iterative refinement, with the refinement residual in binary64.
+// precision = MIXED stores the preconditioner's unknowns and right-hand
+// sides (level 0 included) in bfloat16.
+constexpr double PCG_NEAR = 4.0;
+
+// x = hi + lo += t: the exact two-sum of hi and t, the error added to lo,
+// then renormalised so that |lo| <= ulp(hi) / 2.
+template <class R> inline void df_add(R& hi, R& lo, R t) {
+    const R s = hi + t;
+    const R bv = s - hi;
+    const R e = (hi - (s - bv)) + (t - bv);
+    const R l = lo + e;
+    const R h = s + l;
+    lo = l - (h - s);
+    hi = h;
+}
+
 template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const std::vector<Level>& lv) {
     const RiboWorldSettings& st = w.st;
     const size_t n2 = (size_t)lv[0].n * lv[0].n;
@@ -1519,43 +1565,71 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
     CoarseInv ci;
     const Level& C = lv.back();
     mg.cinv = ci.build(C.n, w.mg_fluid.data() + C.off, C.eps) ? &ci : nullptr;
-    std::vector<R> r(n2), u(n2), wv(n2), pd(n2, R(0)), sd(n2, R(0));
+    std::vector<R> r(n2), u(n2), wv(n2), pd(n2, R(0)), sd(n2, R(0)), xlo(n2, R(0));
+    std::vector<double> a64(n2);
     R* x = fl.p_a.data();
     const R* b = fl.div.data();
-    mg.apply(x, wv.data());
-    double bb = 0.0, rr = 0.0;
-    for (size_t i = 0; i < n2; ++i) {
-        r[i] = w.mg_fluid[i] ? b[i] - wv[i] : R(0);
-        bb += (double)b[i] * (double)b[i];
-    }
+    const double tol = (double)st.mg_tol;
+    auto fluid0 = [&](size_t i) { return w.mg_fluid[i] != 0u; };
+    // r = b - A x in binary64, rounded once
+    auto true_residual = [&]() {
+        mg.apply64(x, xlo.data(), a64.data());
+        for (size_t i = 0; i < n2; ++i) r[i] = fluid0(i) ? R((double)b[i] - a64[i]) : R(0);
+    };
+    auto true_rr = [&]() {
+        mg.apply64(x, xlo.data(), a64.data());
+        double t = 0.0;
+        for (size_t i = 0; i < n2; ++i) {
+            if (!fluid0(i)) continue;
+            const double v = (double)b[i] - a64[i];
+            t += v * v;
+        }
+        return t;
+    };
+    true_residual();
+    double bb = 0.0, rr = 0.0, rq = 0.0, qq = 0.0, theta = 0.0;
     // The start: once the pressure and p_old are the last two solves'
     // results, move along their change d = p - p_old to the point of least
     // residual, x + theta d with theta = (r, A d) / (A d, A d) (theta = 1 is
-    // the linear extrapolation in time); p_old takes the old pressure.
+    // the linear extrapolation in time); p_old takes the old pressure.  The
+    // residual of the start follows from the sums:
+    // |r - theta q|^2 = |r|^2 - (r, q)^2 / (q, q).
+    std::vector<R> d, q;
     if (fl.hist >= 2) {
-        std::vector<R> d(n2), q(n2);
+        d.resize(n2);
+        q.resize(n2);
         for (size_t i = 0; i < n2; ++i) d[i] = x[i] - fl.p_old[i];
-        mg.apply(d.data(), q.data());
-        double rq = 0.0, qq = 0.0;
-        for (size_t i = 0; i < n2; ++i) {
-            if (!w.mg_fluid[i]) continue;
+        mg.apply64(d.data(), nullptr, a64.data());
+        for (size_t i = 0; i < n2; ++i) q[i] = fluid0(i) ? R(a64[i]) : R(0);
+    }
+    for (size_t i = 0; i < n2; ++i) {
+        if (!fluid0(i)) continue;
+        bb += (double)b[i] * (double)b[i];
+        rr += (double)r[i] * (double)r[i];
+        if (!q.empty()) {
             rq += (double)r[i] * (double)q[i];
             qq += (double)q[i] * (double)q[i];
         }
-        const R th = R(qq > 0.0 ? rq / qq : 0.0);
+    }
+    if (!q.empty()) {
+        theta = qq > 0.0 ? rq / qq : 0.0;
+        const R th = R(theta);
         for (size_t i = 0; i < n2; ++i) {
-            fl.p_old[i] = x[i];
-            if (th != R(0) && w.mg_fluid[i]) {
-                x[i] = x[i] + th * d[i];
+            const R a = x[i];
+            if (th != R(0) && fluid0(i)) {
+                df_add(x[i], xlo[i], th * (a - fl.p_old[i]));
                 r[i] = r[i] - th * q[i];
             }
+            fl.p_old[i] = a;
         }
     } else {
         for (size_t i = 0; i < n2; ++i) fl.p_old[i] = x[i];
     }
-    for (size_t i = 0; i < n2; ++i) rr += (double)r[i] * (double)r[i];
-    // u = M r, w = A u and the iteration's inner products
-    double gamma = 0.0, delta = 0.0, e = 0.0, f = 0.0;
+    double rel = bb > 0.0 ? sqrt(std::max(rr - theta * rq, 0.0) / bb) : 0.0;
+    bool near = rel <= PCG_NEAR * tol;
+    // u = M r, w = A u and the V-cycle's inner products (with the true
+    // residual of the current x when near)
+    double gamma = 0.0, delta = 0.0, e = 0.0, f = 0.0, tt = -1.0;
     auto precond = [&]() {
         for (size_t i = 0; i < n2; ++i) {
             fl.mg_rhs[i] = mg.bf16 ? R(round_bf16((float)r[i])) : r[i];
@@ -1563,45 +1637,76 @@ template <class R> void pcg_solve(OracleWorld& w, Fluid<R>& fl, MG<R>& mg, const
         }
         mg.vcycle_sym();
         for (size_t i = 0; i < n2; ++i) u[i] = fl.mg_p[i];
-        mg.apply(u.data(), wv.data());
+        mg.apply64(u.data(), nullptr, a64.data());
+        for (size_t i = 0; i < n2; ++i) wv[i] = fluid0(i) ? R(a64[i]) : R(0);
         gamma = delta = e = f = rr = 0.0;
         for (size_t i = 0; i < n2; ++i) {
-            if (!w.mg_fluid[i]) continue;
+            if (!fluid0(i)) continue;
             gamma += (double)r[i] * (double)u[i];
             delta += (double)wv[i] * (double)u[i];
             e += (double)u[i] * (double)sd[i];
             f += (double)pd[i] * (double)wv[i];
             rr += (double)r[i] * (double)r[i];
         }
+        tt = near ? true_rr() : -1.0;
     };
-    uint32_t it = 0;
+    uint32_t it = 0, restarts = 0;
     const uint32_t maxit = std::max(st.mg_max_cycles, 1u);
-    double rel = bb > 0 ? sqrt(rr / bb) : 0.0;
-    if (rel > (double)st.mg_tol) {
+    if (!(rel <= tol)) {
         precond();
-        double beta = 0.0, pi = delta, g_prev = gamma;
-        double alpha = pi != 0.0 ? gamma / pi : 0.0;
-        while (it < maxit) {
-            const R a = R(alpha), be = R(beta);
-            for (size_t i = 0; i < n2; ++i) {
-                if (!w.mg_fluid[i]) continue;
-                pd[i] = u[i] + be * pd[i];
-                sd[i] = wv[i] + be * sd[i];
-                x[i] += a * pd[i];
-                r[i] -= a * sd[i];
+        bool fresh = true;
+        double beta = 0.0, pi = 0.0, alpha = 0.0, g_prev = 0.0;
+        for (;;) {
+            // the scalars from the last V-cycle's inner products
+            bool restart = false;
+            if (fresh) {
+                g_prev = gamma;
+                pi = delta;
+                alpha = delta != 0.0 ? gamma / delta : 0.0;
+                beta = 0.0;
+                fresh = false;
+            } else {
+                ++it;
+                rel = bb > 0.0 ? sqrt(rr / bb) : 0.0;
+                const bool have_true = tt >= 0.0;
+                const double trel = have_true && bb > 0.0 ? sqrt(tt / bb) : (have_true ? 0.0 : -1.0);
+                if (have_true && trel <= tol) break;
+                if (it >= maxit) break;
+                if (have_true && rel <= tol) {
+                    restart = true;
+                } else {
+                    beta = g_prev != 0.0 ? gamma / g_prev : 0.0;
+                    pi = delta + beta * (e + f) + beta * beta * pi;
+                    alpha = pi != 0.0 ? gamma / pi : 0.0;
+                    g_prev = gamma;
+                }
+                near = rel <= PCG_NEAR * tol;
             }
-            ++it;
+            if (restart) {
+                // one refinement step: the iteration starts again from the
+                // true residual of the current iterate
+                true_residual();
+                std::fill(pd.begin(), pd.end(), R(0));
+                std::fill(sd.begin(), sd.end(), R(0));
+                fresh = true;
+                ++restarts;
+            } else {
+                const R a = R(alpha), be = R(beta);
+                for (size_t i = 0; i < n2; ++i) {
+                    if (!fluid0(i)) continue;
+                    pd[i] = u[i] + be * pd[i];
+                    sd[i] = wv[i] + be * sd[i];
+                    df_add(x[i], xlo[i], a * pd[i]);
+                    r[i] -= a * sd[i];
+                }
+            }
             precond();
-            rel = bb > 0 ? sqrt(rr / bb) : 0.0;
-            if (rel <= (double)st.mg_tol || it >= maxit) break;
-            beta = g_prev != 0.0 ? gamma / g_prev : 0.0;
-            pi = delta + beta * (e + f) + beta * beta * pi;
-            alpha = pi != 0.0 ? gamma / pi : 0.0;
-            g_prev = gamma;
         }
     }
     w.last_cycles = it;
-    w.last_rel_res = mg.rel_residual_of(x, b);
+    w.last_restarts = restarts;
+    w.last_rel_res = bb > 0.0 ? sqrt(true_rr() / bb) : 0.0;
+    w.last_rounded_rel = mg.rel_residual_of(x, b);
     fl.hist = std::min(fl.hist + 1, 2);
 }
 
@@ -1637,19 +1742,19 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
         uint32_t cycles = std::max(st.mg_cycles, 1u);
         if (st.mg_tol > 0.0f && st.mg_stationary) {
             // the reference V-cycle repeated until ||rhs - A p|| / ||rhs||
-            // over level-0 fluid cells reaches mg_tol (A p and the residual
-            // in the field type, the norms in binary64)
+            // over level-0 fluid cells reaches mg_tol (the residual and the
+            // norms in binary64)
             const uint32_t maxit = std::max(st.mg_max_cycles, 1u);
             const size_t n0 = (size_t)L0.n * L0.n;
-            std::vector<R> ap(n0);
+            std::vector<double> ap(n0);
             auto rel_of = [&]() {
-                mg.apply(fl.mg_p.data(), ap.data());
+                mg.apply64(fl.mg_p.data(), nullptr, ap.data());
                 double bb = 0.0, rr = 0.0;
                 for (size_t i = 0; i < n0; ++i) {
                     if (!w.mg_fluid[i]) continue;
-                    const R r = fl.mg_rhs[i] - ap[i];
+                    const double r = (double)fl.mg_rhs[i] - ap[i];
                     bb += (double)fl.mg_rhs[i] * (double)fl.mg_rhs[i];
-                    rr += (double)r * (double)r;
+                    rr += r * r;
                 }
                 return bb > 0.0 ? sqrt(rr / bb) : 0.0;
             };
@@ -1662,6 +1767,7 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
             }
             w.last_cycles = it;
             w.last_rel_res = rel;
+            w.last_rounded_rel = rel;
         } else if (st.mg_tol > 0.0f) {
             pcg_solve<R>(w, fl, mg, lv);
             for (uint32_t i = 0; i < L0.n * L0.n; ++i) fl.mg_p[i] = fl.p_a[i];
@@ -2333,6 +2439,11 @@ void oracle_ledger(const OracleWorld* w, uint32_t out[12]) {  // Ledger::from_ce
 void oracle_mg_stats(const OracleWorld* w, uint32_t* cycles, double* rel) {
     *cycles = w->last_cycles;
     *rel = w->last_rel_res;
+}
+void oracle_solver_residuals(const OracleWorld* w, double* refined, double* rounded, uint32_t* restarts) {
+    *refined = w->last_rel_res;
+    *rounded = w->last_rounded_rel;
+    *restarts = w->last_restarts;
 }
 
 double oracle_mg_residual(const OracleWorld* cw) {

@@ -74,6 +74,10 @@ int oracle_run_kernel(OracleWorld* w, const char* name);
 double oracle_mg_residual(const OracleWorld* w);
 /* Iterations and final relative residual of the last tolerance-mode solve. */
 void oracle_mg_stats(const OracleWorld* w, uint32_t* cycles, double* rel);
+/* Tolerance mode, last solve: the refined (double-float) iterate's relative
+ * residual in binary64, the stored pressure's, and the restarts from the
+ * true residual. */
+void oracle_solver_residuals(const OracleWorld* w, double* refined, double* rounded, uint32_t* restarts);
 /* Threads used by the parallel loops (OpenMP); 0 = library default. */
 void oracle_set_threads(int n);
 int oracle_get_threads(void);

@@ -68,7 +68,8 @@ template <class S> struct CgVec {
     uint32_t n;
     float eps;
     const uint8_t* fl;
-    float* x;        // the pressure
+    float* x;        // the pressure: the high part of the iterate
+    float* xlo;      // the iterate's low part (x + xlo is the double-float iterate)
     float* xold;     // the previous solve's pressure (null: no extrapolated start)
     const float* b;  // the divergence
     float* r[2];
@@ -82,10 +83,15 @@ template <class S> struct CgVec {
 
 constexpr uint32_t PCG_HOST_RING = 4, PCG_HOST_RING_LEN = 64;
 // The scalars of the conjugate-gradient recurrences after an iteration.
+// fresh: the next inner products are the first of a (re)start; near: the
+// V-cycle after this update also forms the true residual's norm.
 struct PcgScal {
     double gamma, pi, beta, alpha, rel;
-    uint32_t it, done;
+    uint32_t it, done, fresh, near, restart, restarts;
 };
+// Once the recursive residual is within PCG_NEAR x tol, every V-cycle also
+// forms the true residual ||b - A x||^2 in binary64 and the solve stops on it.
+constexpr double PCG_NEAR = 4.0;
 // Solver state.  The scalars are double-buffered: the kernel that applies an
 // iteration's inner products reads one slot (every workgroup forms the same
 // update) and its workgroup 0 writes the other.  `done` is the published
@@ -93,6 +99,10 @@ struct PcgScal {
 // finished solve.
 struct PcgState {
     double bb, rel, true_rel;
+    double rounded_rel;             // residual of the stored (binary32) pressure of the last solve
+    double max_true_rel, max_rounded_rel;  // maxima over the solves since the last reset
+    uint32_t restarts;              // restarts from the true residual since the last reset
+    uint32_t near;                  // the current V-cycle forms the true residual (see PcgScal)
     double theta;       // the start's step along the last two solves' change
     PcgScal sc[2];
     uint32_t it, done, steps, iters, missed;

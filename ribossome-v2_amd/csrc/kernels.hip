@@ -1502,6 +1502,54 @@ __device__ __forceinline__ float apply_A(const uint8_t* fl, const float* v, int 
     return nb.s - (nb.k + eps) * vc;
 }
 
+// Binary64 forms of the operator.  Values are widened and summed in the
+// order of nb_sum; v = hi + lo per cell, formed in binary64 (lo may be null:
+// v = hi + 0).  The tolerance mode evaluates w = A u and every residual
+// b - A x this way and rounds once: in binary32 the stencil's cancellation on
+// smooth vectors (|A v| ~ lambda |v|, lambda down to ~1e-5 at 1024^2) leaves
+// an error of ~3 eps32 / lambda on the result, which the recursively updated
+// residual then carries.
+struct NbSum64 { double s, k; };
+__device__ __forceinline__ NbSum64 nb_sum64(bool fl, bool fr, bool fd, bool fu, double vl, double vr, double vd, double vu) {
+    double s = 0.0, k = 0.0;
+    s = fl ? s + vl : s;
+    k = fl ? k + 1.0 : k;
+    s = fr ? s + vr : s;
+    k = fr ? k + 1.0 : k;
+    s = fd ? s + vd : s;
+    k = fd ? k + 1.0 : k;
+    s = fu ? s + vu : s;
+    k = fu ? k + 1.0 : k;
+    return {s, k};
+}
+__device__ __forceinline__ double dval(const float* hi, const float* lo, int j) {
+    return (double)hi[j] + (lo ? (double)lo[j] : 0.0);
+}
+__device__ __forceinline__ double apply_A64(const uint8_t* fl, const float* hi, const float* lo, int n, int x, int y,
+                                            float eps) {
+    const int i = y * n + x;
+    const int il = x > 0 ? i - 1 : i, ir = x + 1 < n ? i + 1 : i;
+    const int id = y > 0 ? i - n : i, iu = y + 1 < n ? i + n : i;
+    const bool gl = fl[il] != 0, gr = fl[ir] != 0, gd = fl[id] != 0, gu = fl[iu] != 0;
+    const double vl = dval(hi, lo, il), vr = dval(hi, lo, ir), vd = dval(hi, lo, id), vu = dval(hi, lo, iu);
+    const double vc = dval(hi, lo, i);
+    const NbSum64 nb = nb_sum64((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, vl, vr, vd, vu);
+    return nb.s - (nb.k + (double)eps) * vc;
+}
+
+// x = hi + lo += t: the exact two-sum of hi and t, its error added to lo,
+// then renormalised (|lo| <= ulp(hi) / 2), so that hi stays the nearest
+// binary32 value of the double-float iterate.
+__device__ __forceinline__ void df_add(float& hi, float& lo, float t) {
+    const float s = hi + t;
+    const float bv = s - hi;
+    const float e = (hi - (s - bv)) + (t - bv);
+    const float l = lo + e;
+    const float h = s + l;
+    lo = l - (h - s);
+    hi = h;
+}
+
 // Value after a store to S and a load back.
 __device__ __forceinline__ float round_s(float v, const float*) { return v; }
 __device__ __forceinline__ float round_s(float v, const bf16*) { return rb_as_f32((uint32_t)bf16_bits(v) << 16); }
@@ -1585,24 +1633,38 @@ __device__ void sum_partials_wg(const double* __restrict__ part, uint32_t nb, do
 }
 
 // The scalar recurrences after an iteration's inner products t = (gamma,
-// delta, e, f, rr) (see above): state a to b.  The first application only
-// sets gamma, pi = delta and alpha; later ones count the iteration, test the
-// residual and form beta, pi and alpha.  A finished state passes unchanged.
-__device__ inline PcgScal pcg_advance(const PcgState* st, const PcgScal& a, const double (&t)[5], bool first_apply,
-                                      float tol, uint32_t maxit) {
+// delta, e, f, rr, tt) (see above; tt = ||b - A x||^2 in binary64, formed
+// when a.near): state a to b.  The first application after a (re)start only
+// sets gamma, pi = delta and alpha.  Later ones count the iteration and
+// stop when the true residual reaches tol (or at maxit); when the recursive
+// residual has reached tol but the true one has not, the next V-cycle
+// restarts from the true residual (one step of iterative refinement).  A
+// finished state passes unchanged.
+__device__ inline PcgScal pcg_advance(const PcgState* st, const PcgScal& a, const double (&t)[6], float tol,
+                                      uint32_t maxit) {
     PcgScal b = a;
+    b.restart = 0u;
     if (a.done) return b;
-    if (first_apply) {
+    if (a.fresh) {
         b.gamma = t[0];
         b.pi = t[1];
         b.alpha = t[1] != 0.0 ? t[0] / t[1] : 0.0;
         b.beta = 0.0;
+        b.fresh = 0u;
         return b;
     }
     b.it = a.it + 1u;
     b.rel = st->bb > 0.0 ? sqrt(t[4] / st->bb) : 0.0;
-    if (b.rel <= (double)tol || b.it >= maxit) {
+    const bool have_true = a.near != 0u;
+    const double trel = st->bb > 0.0 ? sqrt(t[5] / st->bb) : 0.0;
+    if ((have_true && trel <= (double)tol) || b.it >= maxit) {
         b.done = 1u;
+        return b;
+    }
+    if (have_true && b.rel <= (double)tol) {
+        b.restart = 1u;
+        b.fresh = 1u;
+        b.restarts = a.restarts + 1u;
     } else {
         const double g = t[0], beta = a.gamma != 0.0 ? g / a.gamma : 0.0;
         const double pi = t[1] + beta * (t[2] + t[3]) + beta * beta * a.pi;
@@ -1611,24 +1673,26 @@ __device__ inline PcgScal pcg_advance(const PcgState* st, const PcgScal& a, cons
         b.alpha = pi != 0.0 ? g / pi : 0.0;
         b.gamma = g;
     }
+    b.near = b.rel <= PCG_NEAR * (double)tol ? 1u : 0u;
     return b;
 }
 
-// A workgroup of NT threads applies the partials of the last iteration to
-// slot rd (first_apply for the first V-cycle's products), shares the result
-// through LDS and (workgroup 0) writes it to slot 1 - rd and publishes done.
+// A workgroup of NT threads applies the partials of the last V-cycle to
+// slot rd, shares the result through LDS and (workgroup 0) writes it to
+// slot 1 - rd and publishes done and near.
 template <int NT>
-__device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, bool first_apply, float tol, uint32_t maxit) {
+__device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit) {
     __shared__ PcgScal sh_sc;
-    double t[5];
-    sum_partials_wg<5, NT>(st->part, nb, t);
+    double t[6];
+    sum_partials_wg<6, NT>(st->part, nb, t);
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     if (tid == 0) {
-        const PcgScal b = pcg_advance(st, st->sc[rd], t, first_apply, tol, maxit);
+        const PcgScal b = pcg_advance(st, st->sc[rd], t, tol, maxit);
         sh_sc = b;
         if (blockIdx.x == 0 && blockIdx.y == 0) {
             st->sc[1 - rd] = b;
             st->done = b.done;
+            st->near = b.near;
         }
     }
     __syncthreads();
@@ -1642,17 +1706,20 @@ __device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, bool first_app
     const int i = in ? y * n + x : 0;                                                  \
     const bool f = in && V.fl[i];
 
-// A (x - xold) at fluid cell (x, y): the operator on the change between the
-// last two solves' pressures, each difference formed in binary32.
-__device__ __forceinline__ float apply_A_diff(const uint8_t* fl, const float* v, const float* o, int n, int x, int y,
-                                              float eps) {
+// A (x - xold) at fluid cell (x, y) in binary64: the operator on the change
+// between the last two solves' pressures, each difference formed in
+// binary32.
+__device__ __forceinline__ double apply_A64_diff(const uint8_t* fl, const float* v, const float* o, int n, int x, int y,
+                                                 float eps) {
     const int i = y * n + x;
     const int il = x > 0 ? i - 1 : i, ir = x + 1 < n ? i + 1 : i;
     const int id = y > 0 ? i - n : i, iu = y + 1 < n ? i + n : i;
     const bool gl = fl[il] != 0, gr = fl[ir] != 0, gd = fl[id] != 0, gu = fl[iu] != 0;
-    const float dl = v[il] - o[il], dr = v[ir] - o[ir], dd = v[id] - o[id], du = v[iu] - o[iu], dc = v[i] - o[i];
-    const NbSum nb = nb_sum((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, dl, dr, dd, du);
-    return nb.s - (nb.k + eps) * dc;
+    const double dl = (double)(v[il] - o[il]) + 0.0, dr = (double)(v[ir] - o[ir]) + 0.0;
+    const double dd = (double)(v[id] - o[id]) + 0.0, du = (double)(v[iu] - o[iu]) + 0.0;
+    const double dc = (double)(v[i] - o[i]) + 0.0;
+    const NbSum64 nb = nb_sum64((x > 0) & gl, (x + 1 < n) & gr, (y > 0) & gd, (y + 1 < n) & gu, dl, dr, dd, du);
+    return nb.s - (nb.k + (double)eps) * dc;
 }
 
 // r = b - A x (r[0]); r[1], s, w start at 0; ||b||^2 and ||r||^2.  Once the
@@ -1668,8 +1735,8 @@ __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
     if (in) {
         float r = 0.0f, q = 0.0f;
         if (f) {
-            r = V.b[i] - apply_A(V.fl, V.x, n, x, y, V.eps);
-            if (extrap) q = apply_A_diff(V.fl, V.x, V.xold, n, x, y, V.eps);
+            r = (float)((double)V.b[i] - apply_A64(V.fl, V.x, nullptr, n, x, y, V.eps));
+            if (extrap) q = (float)apply_A64_diff(V.fl, V.x, V.xold, n, x, y, V.eps);
             bb = (double)V.b[i] * (double)V.b[i];
             rr = (double)r * (double)r;
             rq = (double)r * (double)q;
@@ -1681,20 +1748,24 @@ __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
         V.s[1][i] = 0.0f;
         V.pd[i] = q;
         V.w[i] = 0.0f;
+        V.xlo[i] = 0.0f;
     }
     block_partial<4>({bb, rr, rq, qq}, st->part);
 }
 
 // The start on the line: x += theta (x - xold), r -= theta q in fluid cells;
 // xold takes the old x; the search direction (q until now) starts at 0.
-__global__ __launch_bounds__(256) void k_pcg_shift(uint32_t cells, const uint8_t* __restrict__ fl, float* x, float* xold,
-                                                   float* r, float* pd, const PcgState* st) {
+__global__ __launch_bounds__(256) void k_pcg_shift(uint32_t cells, const uint8_t* __restrict__ fl, float* x, float* xlo,
+                                                   float* xold, float* r, float* pd, const PcgState* st) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= cells) return;
     const float th = (float)st->theta;
     const float a = x[i];
     if (th != 0.0f && fl[i]) {
-        x[i] = a + th * (a - xold[i]);
+        float h = a, l = 0.0f;
+        df_add(h, l, th * (a - xold[i]));
+        x[i] = h;
+        xlo[i] = l;
         r[i] = r[i] - th * pd[i];
     }
     xold[i] = a;
@@ -1717,16 +1788,20 @@ static_assert((CG_T / 2) * (CG_T / 2) <= CG_NT, "one thread per coarse cell of a
 template <class S>
 __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int stage, int rd,
                                                    uint32_t nb, float tol, uint32_t maxit) {
-    // stage 0: the first V-cycle (the initial state in slot rd); stage 1 and
-    // 2: first apply the previous V-cycle's inner products to the scalars
-    // (stage 1: their first application)
+    // stage 0: the first V-cycle (the initial state in slot rd); later
+    // stages first apply the previous V-cycle's inner products to the
+    // scalars.  A restart (the recursive residual reached tol, the true one
+    // did not) takes the true residual b - A x in binary64 as r, with p and s
+    // at 0 and x unchanged, like the first V-cycle.
     const int first = stage == 0 ? 1 : 0;
     if (st->sc[rd].done) {
         if (!first && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) st->sc[1 - rd] = st->sc[rd];
         return;
     }
-    const PcgScal sc = first ? st->sc[rd] : pcg_step_wg<CG_NT>(st, rd, nb, stage == 1, tol, maxit);
+    const PcgScal sc = first ? st->sc[rd] : pcg_step_wg<CG_NT>(st, rd, nb, tol, maxit);
     if (sc.done) return;
+    const bool rst = !first && sc.restart != 0u;
+    const bool upd = !first && !rst;
     constexpr bool BL = BL_TRANSFER<S>;
     constexpr int H = BL ? 6 : 5, T = CG_T, NT = CG_NT;
     constexpr int R = T + 2 * H;
@@ -1742,26 +1817,28 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
     const int n = (int)V.n;
     const int x0 = (int)blockIdx.x * T - H, y0 = (int)blockIdx.y * T - H;
     if (BL) bl_load_cfl<H, T, S>(cfl, C, x0, y0);
-    float rv[PER], sv[PER], wv[PER], uv[PER], pv[PER], xv[PER];
+    float rv[PER], sv[PER], wv[PER], uv[PER], pv[PER], xv[PER], lv[PER];
     uint32_t fv[PER];
 #pragma unroll
     for (int j = 0; j < PER; ++j) {
         const int k = (int)threadIdx.x + NT * j;
         const int ly = k / R, lx = k % R;
         const int gx = x0 + lx, gy = y0 + ly;
-        rv[j] = sv[j] = wv[j] = uv[j] = pv[j] = xv[j] = 0.0f;
+        rv[j] = sv[j] = wv[j] = uv[j] = pv[j] = xv[j] = lv[j] = 0.0f;
         fv[j] = 0u;
         if (k < R * R && gx >= 0 && gy >= 0 && gx < n && gy < n) {
             const int i = gy * n + gx;
             fv[j] = V.fl[i];
-            rv[j] = r_in[i];
-            if (!first) {
+            if (rst) rv[j] = fv[j] ? (float)((double)V.b[i] - apply_A64(V.fl, V.x, V.xlo, n, gx, gy, V.eps)) : 0.0f;
+            else rv[j] = r_in[i];
+            if (upd) {
                 sv[j] = s_in[i];
                 wv[j] = V.w[i];
                 if (lx >= H && lx < H + T && ly >= H && ly < H + T) {
                     uv[j] = ldv(V.u, i);
                     pv[j] = V.pd[i];
                     xv[j] = V.x[i];
+                    lv[j] = V.xlo[i];
                 }
             }
         }
@@ -1773,7 +1850,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
         const int ly = k / R, lx = k % R;
         const bool f = fv[j] != 0u;
         float s_new = 0.0f, r_new = rv[j];
-        if (!first) {
+        if (upd) {
             s_new = wv[j] + beta * sv[j];
             r_new = rv[j] - alpha * s_new;
         }
@@ -1786,10 +1863,15 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
             r_out[i] = r_new;
             s_out[i] = s_new;
             if (V.rs) stv(V.rs, i, r_new);
-            if (!first) {
+            if (upd) {
                 const float p_new = uv[j] + beta * pv[j];
                 V.pd[i] = p_new;
-                V.x[i] = xv[j] + alpha * p_new;
+                float h = xv[j], l = lv[j];
+                df_add(h, l, alpha * p_new);
+                V.x[i] = h;
+                V.xlo[i] = l;
+            } else if (rst) {
+                V.pd[i] = 0.0f;
             }
         }
     }
@@ -1837,6 +1919,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
 template <class S>
 __global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par) {
     if (st->done) return;
+    const bool tr = st->near != 0u;  // also form ||b - A x||^2 (binary64)
     constexpr int H = 5, T = CG_T, NT = CG_NT;
     constexpr int R = T + 2 * H;
     __shared__ MgTile<H, T> t;
@@ -1887,7 +1970,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, Pcg
         t.p[ly][lx] = round_s(t.p[ly][lx], V.u);
     }
     __syncthreads();
-    double gam = 0.0, del = 0.0, e = 0.0, f = 0.0, rr = 0.0;
+    double gam = 0.0, del = 0.0, e = 0.0, f = 0.0, rr = 0.0, tt = 0.0;
 #pragma unroll
     for (int j = 0; j < PC; ++j) {
         const int k = (int)threadIdx.x + NT * j;
@@ -1896,8 +1979,10 @@ __global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, Pcg
         if (gx >= n || gy >= n || !t.fl[ly][lx]) continue;
         const int i = gy * n + gx;
         const float u = t.p[ly][lx];
-        const NbSum nb = tile_nb(t, ly, lx);
-        const float wv = nb.s - (nb.k + V.eps) * u;
+        const NbSum64 nb = nb_sum64(t.fl[ly][lx - 1] != 0, t.fl[ly][lx + 1] != 0, t.fl[ly - 1][lx] != 0,
+                                    t.fl[ly + 1][lx] != 0, (double)t.p[ly][lx - 1] + 0.0, (double)t.p[ly][lx + 1] + 0.0,
+                                    (double)t.p[ly - 1][lx] + 0.0, (double)t.p[ly + 1][lx] + 0.0);
+        const float wv = (float)(nb.s - (nb.k + (double)V.eps) * ((double)u + 0.0));
         stv(V.u, i, u);
         V.w[i] = wv;
         const float r = V.rs ? rc[j] : t.rhs[ly][lx];
@@ -1906,23 +1991,30 @@ __global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, Pcg
         e += (double)u * (double)sc[j];
         f += (double)pc[j] * (double)wv;
         rr += (double)r * (double)r;
+        if (tr) {
+            const double rt = (double)V.b[i] - apply_A64(V.fl, V.x, V.xlo, n, gx, gy, V.eps);
+            tt += rt * rt;
+        }
     }
-    block_partial<5>({gam, del, e, f, rr}, st->part);
+    block_partial<6>({gam, del, e, f, rr, tt}, st->part);
 }
 
-// ||b - A x||^2 over fluid cells, for the reported residual.
+// ||b - A x||^2 over fluid cells, for the reported residuals.
 template <class S>
 __global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit,
                                                   uint32_t off) {
     // workgroup 0 applies the last V-cycle's inner products (slot rd to 1 - rd)
-    if (blockIdx.x == 0 && blockIdx.y == 0) (void)pcg_step_wg<256>(st, rd, nb, false, tol, maxit);
+    if (blockIdx.x == 0 && blockIdx.y == 0) (void)pcg_step_wg<256>(st, rd, nb, tol, maxit);
     PCG_XY
-    double rr = 0.0;
+    // the refined iterate's residual and the stored pressure's, binary64
+    double rr = 0.0, rh = 0.0;
     if (f) {
-        const float r = V.b[i] - apply_A(V.fl, V.x, n, x, y, V.eps);
-        rr = (double)r * (double)r;
+        const double r = (double)V.b[i] - apply_A64(V.fl, V.x, V.xlo, n, x, y, V.eps);
+        const double q = (double)V.b[i] - apply_A64(V.fl, V.x, nullptr, n, x, y, V.eps);
+        rr = r * r;
+        rh = q * q;
     }
-    block_partial<2>({rr, 0.0}, st->part + off);  // after the partials read above
+    block_partial<2>({rr, rh}, st->part + off);  // after the partials read above
 }
 
 // The solve's scalar state from the partials of the kernel before: one
@@ -1947,10 +2039,13 @@ __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, floa
         PcgScal a{};
         a.rel = rel;
         a.done = (rel <= (double)tol || maxit == 0u) ? 1u : 0u;
+        a.fresh = 1u;
+        a.near = rel <= PCG_NEAR * (double)tol ? 1u : 0u;
         st->sc[1] = a;
         st->rel = rel;
         st->it = 0u;
         st->done = a.done;
+        st->near = a.near;
         st->tol_seen = tol;
     } else {
         const PcgScal a = st->sc[rd];
@@ -1958,6 +2053,10 @@ __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, floa
         st->rel = a.rel;
         st->done = a.done;
         st->true_rel = st->bb > 0.0 ? sqrt(t[0] / st->bb) : 0.0;
+        st->rounded_rel = st->bb > 0.0 ? sqrt(t[1] / st->bb) : 0.0;
+        st->max_true_rel = fmax(st->max_true_rel, st->true_rel);
+        st->max_rounded_rel = fmax(st->max_rounded_rel, st->rounded_rel);
+        st->restarts += a.restarts;
         st->hist = st->hist < 2u ? st->hist + 1u : 2u;  // solves done since the history restarted (at most 2)
         st->steps += 1u;
         st->iters += a.it;
@@ -1990,9 +2089,9 @@ __global__ __launch_bounds__(256) void k_vc_resid(uint32_t n, float eps, const u
     const int i = in ? y * nn + x : 0;
     double bb = 0.0, rr = 0.0;
     if (in && fl[i]) {
-        const float r = b[i] - apply_A(fl, xp, nn, x, y, eps);
+        const double r = (double)b[i] - apply_A64(fl, xp, nullptr, nn, x, y, eps);
         bb = (double)b[i] * (double)b[i];
-        rr = (double)r * (double)r;
+        rr = r * r;
     }
     block_partial<2>({bb, rr}, st->part);
 }
@@ -2016,6 +2115,9 @@ __global__ __launch_bounds__(256) void k_vc_fin(PcgState* st, uint32_t nb, float
     } else {
         const bool reached = st->rel <= (double)tol;
         st->true_rel = st->rel;
+        st->rounded_rel = st->rel;
+        st->max_true_rel = fmax(st->max_true_rel, st->rel);
+        st->max_rounded_rel = fmax(st->max_rounded_rel, st->rel);
         st->steps += 1u;
         st->iters += st->it;
         if (!reached) st->missed += 1u;
@@ -3209,7 +3311,7 @@ inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
     k_cg_init<S><<<grid2(V.n), B16, 0, s>>>(V, st);
     k_pcg_fin<FIN_INIT><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), tol, maxit, 0, 0u);
-    if (V.xold) k_pcg_shift<<<blocks1((size_t)V.n * V.n, 256), 256, 0, s>>>(V.n * V.n, V.fl, V.x, V.xold, V.r[0], V.pd, st);
+    if (V.xold) k_pcg_shift<<<blocks1((size_t)V.n * V.n, 256), 256, 0, s>>>(V.n * V.n, V.fl, V.x, V.xlo, V.xold, V.r[0], V.pd, st);
 }
 template <class S>
 void launch_cg_down0(hipStream_t s, const CgVec<S>& V, const MgLevelT<S>& C, PcgState* st, int parity, int stage, int rd,
@@ -3229,7 +3331,7 @@ void launch_vc_step(hipStream_t s, int phase, uint32_t n, float eps, const uint8
     else k_vc_fin<VC_END><<<1, 256, 0, s>>>(st, 0u, tol, maxit);
 }
 template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgState* st, int rd, float tol, uint32_t maxit) {
-    const uint32_t nt = cg_tiles(V.n), off = 5u * nt * nt;
+    const uint32_t nt = cg_tiles(V.n), off = 6u * nt * nt;
     k_cg_final<S><<<grid2(V.n), B16, 0, s>>>(V, st, rd & 1, nt * nt, tol, maxit, off);
     k_pcg_fin<FIN_FINAL><<<1, 256, 0, s>>>(st, pcg_blocks(V.n), 0.0f, 0u, 1 - (rd & 1), off);
 }

@@ -152,6 +152,7 @@ struct RiboWorld {
     float* pcg_s0 = nullptr;  // A p, two buffers
     float* pcg_s1 = nullptr;
     float* pcg_pd = nullptr;  // search direction
+    float* pcg_xlo = nullptr; // low part of the double-float iterate
     float* pcg_w = nullptr;   // A u
     float* pcg_z = nullptr;
     bf16* pcg_r16 = nullptr;
@@ -516,6 +517,7 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
     V.eps = w->levels[0].eps;
     V.fl = w->mg_fluid;
     V.x = w->p_a;
+    V.xlo = w->pcg_xlo;
     V.xold = w->p_old;
     V.b = w->div;
     V.r[0] = w->pcg_r;
@@ -805,7 +807,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     b += 2 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x3, div, temp/redox x2, eaten, forces, heat
     b += a(2 * F, 4) + 2 * a(G, 1);  // the agents' force words, the grain move codes and masks
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
-    b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 7 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
+    b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 8 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
     b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
     const uint32_t min_size = st ? std::max(st->mg_min_size, 1u) : 8u;
     const std::vector<RiboMgLevel> lv = make_levels(cfg->fluid_size, min_size);
@@ -874,6 +876,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(w->stream, &w->pcg_part, 5 * (size_t)((w->F + 15) / 16) * ((w->F + 15) / 16))) || (rc = dalloc(w->stream, &w->pcg_r, F)) ||
         (rc = dalloc(w->stream, &w->pcg_r1, F)) || (rc = dalloc(w->stream, &w->pcg_s0, F)) || (rc = dalloc(w->stream, &w->pcg_s1, F)) ||
         (rc = dalloc(w->stream, &w->pcg_pd, F)) || (rc = dalloc(w->stream, &w->pcg_w, F)) || (rc = dalloc(w->stream, &w->pcg_z, F)) ||
+        (rc = dalloc(w->stream, &w->pcg_xlo, F)) ||
         (rc = dalloc(w->stream, &w->pcg_r16, F)) || (rc = dalloc(w->stream, &w->pcg_z16, F)) ||
         (rc = dalloc(w->stream, &w->ledger, 1)) || (rc = dalloc(w->stream, &w->scratch_u32, F)) || (rc = dalloc(w->stream, &w->epoch_dev, 1)) ||
         (rc = build_mg(w))) {
@@ -910,7 +913,7 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->terrain_code); dfree(w->terrain_inc); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
-    dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w);
+    dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w); dfree(w->pcg_xlo);
     dfree(w->pcg_z); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
     dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
     clear_graphs(w);
@@ -1043,6 +1046,48 @@ int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes)
     return RIBO_OK;
 }
 
+// Iteration slots per solve for the next graph chunk of the tolerance mode
+// (read after draining the stream, so the choice depends only on the
+// sequence of calls): the most iterations any of the last SLOT_WINDOW
+// solves took, plus max(4, 1/3 of it) (max(8, 1/2 of it) for the stationary
+// V-cycle, whose counts move more), rounded up to a multiple of 4 (few
+// distinct graphs).  The choice is made again before every chunk of at most
+// TOL_CHUNK steps, so it is never older than that.  The window spans two
+// periods of the buoyancy noise (a new bucket every 16 epochs, fluid.wgsl:384,
+// each adding ~8 iterations for a few steps on the 2048^2 benchmark world)
+// and is short enough that the first solves of a cold start (31-53
+// iterations there, 10-20 in steady flow) leave it soon; an unused slot is
+// ten kernels that return at once (~3 us each).  A solve that ran out of
+// slots (counted on the device) gives the next chunk twice the maximum + 4;
+// until 8 solves have run with the current settings every solve gets the
+// full cap.
+constexpr uint32_t SLOT_WINDOW = 32u, TOL_CHUNK = 8u;
+static_assert(SLOT_WINDOW <= PCG_HOST_RING_LEN, "the window lies in the host ring");
+static int choose_slots(RiboWorld* w) {
+    const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
+    HIPCHK(hipStreamSynchronize(w->stream));
+    uint32_t slots = maxit;
+    uint32_t missed = 0u;
+    if (w->pcg_host) {
+        missed = __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED);
+        const uint32_t solves = __atomic_load_n(&w->pcg_host[2], __ATOMIC_RELAXED);
+        const uint32_t known = std::min(solves - w->solves_at_settings, SLOT_WINDOW);
+        uint32_t most = 0u;
+        for (uint32_t k = 0; k < known; ++k)
+            most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
+                                                  __ATOMIC_RELAXED));
+        if (known >= 8u) {
+            const uint32_t margin = w->st.mg_stationary ? std::max(8u, most / 2u) : std::max(4u, most / 3u);
+            const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + margin;
+            slots = std::min(maxit, (want + 3u) & ~3u);
+        }
+        w->solves_seen = solves;
+    }
+    w->missed_seen = missed;
+    w->pcg_slots = std::max(slots, 1u);
+    return RIBO_OK;
+}
+
 int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs:1406-1446,1622,1631
     if (!w) return fail(RIBO_ERR_ARG, "null world");
     int rc = validate_settings(&w->st);
@@ -1055,37 +1100,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             rc = build_coarse_inverse(w);
             if (rc) return rc;
         }
-        // Iteration slots per solve for this chunk (read after draining the
-        // stream, so the choice depends only on the sequence of calls): the
-        // most iterations any of the last 64 solves took, plus max(3, 1/5 of
-        // it) (max(8, 1/2 of it) for the stationary V-cycle, whose counts move
-        // more), rounded up to even.  An unused slot is ten kernels that
-        // return at once (~3 us each); on the 2048^2 benchmark world no solve
-        // took more than 1 iteration over that maximum in 400 steps.  A solve
-        // that ran out of slots (counted on the device) doubles the next
-        // chunk's slots; until 8 solves have run with the current settings
-        // every solve gets the full cap.
-        const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
-        HIPCHK(hipStreamSynchronize(w->stream));
-        uint32_t slots = maxit;
-        uint32_t missed = 0u;
-        if (w->pcg_host) {
-            missed = __atomic_load_n(&w->pcg_host[1], __ATOMIC_RELAXED);
-            const uint32_t solves = __atomic_load_n(&w->pcg_host[2], __ATOMIC_RELAXED);
-            const uint32_t known = std::min(solves - w->solves_at_settings, PCG_HOST_RING_LEN);
-            uint32_t most = 0u;
-            for (uint32_t k = 0; k < known; ++k)
-                most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
-                                                      __ATOMIC_RELAXED));
-            if (known >= 8u) {
-                const uint32_t margin = w->st.mg_stationary ? std::max(8u, most / 2u) : std::max(3u, most / 5u);
-                const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + margin;
-                slots = std::min(maxit, (want + 1u) & ~1u);
-            }
-            w->solves_seen = solves;
-        }
-        w->missed_seen = missed;
-        w->pcg_slots = std::max(slots, 1u);
+        if ((rc = choose_slots(w))) return rc;
     }
     const RiboWorldSettings st = w->st;
     const uint32_t substep = std::min(std::max(st.fluid_substep, 1u), 4u);
@@ -1123,10 +1138,11 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         // tolerance-mode steps carry their solver iterations: shorter graphs
         // (the stationary mode records up to mg_max_cycles V-cycles per step:
         // one step per graph)
-        const uint32_t max_graph = !(w->st.multigrid && w->st.mg_tol > 0.0f) ? RIBO_MAX_STEPS_PER_CALL
-                                   : (w->st.mg_stationary ? 1u : 8u);
+        const bool tol_mode = w->st.multigrid && w->st.mg_tol > 0.0f;
+        const uint32_t max_graph = !tol_mode ? RIBO_MAX_STEPS_PER_CALL : (w->st.mg_stationary ? 1u : TOL_CHUNK);
         for (uint32_t first = 0; first < steps;) {
             const uint32_t n = std::min(max_graph, steps - first);
+            if (tol_mode && first > 0 && (rc = choose_slots(w))) return rc;
             const uint32_t phase = (w->params.epoch + first) % period;
             RiboWorld::GraphEntry* ge = nullptr;
             for (auto& e : w->graphs)
@@ -1161,6 +1177,11 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
             w->lcur = ge->lcur_end;
             w->epoch_dev_value += n;
             first += n;
+        }
+    } else if (w->st.multigrid && w->st.mg_tol > 0.0f) {
+        for (uint32_t first = 0; first < steps; first += TOL_CHUNK) {
+            if (first > 0 && (rc = choose_slots(w))) return rc;
+            record_steps(w, first, std::min(TOL_CHUNK, steps - first), substep, nullptr);
         }
     } else {
         record_steps(w, 0, steps, substep, nullptr);
@@ -1382,6 +1403,27 @@ int ribo_world_solver_stats(RiboWorld* w, uint64_t* solves, uint64_t* iterations
     *solves = c.steps;
     *iterations = c.iters;
     *missed = c.missed;
+    return RIBO_OK;
+}
+
+int ribo_world_solver_residuals(RiboWorld* w, RiboSolverResiduals* out, int reset) {
+    if (!w || !out) return fail(RIBO_ERR_ARG, "null argument");
+    PcgState c;
+    HIPCHK(hipMemcpyAsync(&c, w->pcg, sizeof(c), hipMemcpyDeviceToHost, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    out->last = c.true_rel;
+    out->last_rounded = c.rounded_rel;
+    out->max = c.max_true_rel;
+    out->max_rounded = c.max_rounded_rel;
+    out->restarts = c.restarts;
+    if (reset) {
+        struct { double a, b; uint32_t r; } z{0.0, 0.0, 0u};
+        static_assert(offsetof(PcgState, max_rounded_rel) == offsetof(PcgState, max_true_rel) + 8 &&
+                          offsetof(PcgState, restarts) == offsetof(PcgState, max_true_rel) + 16,
+                      "the reset fields are consecutive");
+        HIPCHK(hipMemcpyAsync((char*)w->pcg + offsetof(PcgState, max_true_rel), &z, 20, hipMemcpyHostToDevice, w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+    }
     return RIBO_OK;
 }
 

@@ -259,6 +259,11 @@ def force_words(f: np.ndarray) -> np.ndarray:
 MAX_STEPS_PER_FRAME = 64
 
 
+class _CSolverResiduals(ctypes.Structure):
+    _fields_ = [("last", ctypes.c_double), ("last_rounded", ctypes.c_double), ("max", ctypes.c_double),
+                ("max_rounded", ctypes.c_double), ("restarts", ctypes.c_uint64)]
+
+
 class RiboError(RuntimeError):
     pass
 
@@ -302,6 +307,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_mg_stats": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
         "ribo_world_solver_stats": (I, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                         ctypes.POINTER(ctypes.c_uint64)]),
+        "ribo_world_solver_residuals": (I, [P, P, I]),
         "ribo_last_error": (ctypes.c_char_p, []),
         "ribo_abi_version": (U32, []),
         "ribo_gen_gamma": (I, [P, ctypes.c_uint64, ctypes.c_float, I, P]),
@@ -554,6 +560,15 @@ class World:
         a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
         _check(lib().ribo_world_solver_stats(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "solver_stats")
         return int(a.value), int(b.value), int(c.value)
+
+    def solver_residuals(self, reset: bool = False) -> dict:
+        """Tolerance-mode residuals: the refined iterate's (binary64) and the
+        stored binary32 pressure's, last solve and maxima since the last
+        reset, and the restarts from the true residual."""
+        r = _CSolverResiduals()
+        _check(lib().ribo_world_solver_residuals(self._h, ctypes.byref(r), 1 if reset else 0), "solver_residuals")
+        return {"last": r.last, "last_rounded": r.last_rounded, "max": r.max, "max_rounded": r.max_rounded,
+                "restarts": int(r.restarts)}
 
     def mg_stats(self) -> tuple:
         c = ctypes.c_uint32()

@@ -147,3 +147,35 @@ def test_coupling_points_reject_bad_input():
         gw.upload(ra.VELOCITY_SMOOTH, np.zeros(2 * cfg.fluid_cells(), np.float32))
     with pytest.raises(ra.RiboError):
         gw.upload(ra.FORCE_VECTORS, np.zeros(cfg.fluid_cells(), np.int32))
+
+
+def test_redox_eaten_words_bit_exact_across_graph_replay():
+    """The oxidant the agents ate (RIBO_BUF_REDOX_EATEN, fluid.wgsl:360,371,
+    fed by lifecycle.wgsl:821): words uploaded between steps, on epochs with
+    and without a fluid step of the reference schedule (fluid every second
+    step), are subtracted by the next fluid step and then cleared; the steps
+    around them replay graphs.  Reductant and every other field stay
+    bit-identical to the oracle's."""
+    print("case redox eaten", flush=True)
+    gw, ow = build("ref_schedule")
+    _, twin = build("ref_schedule")  # the same world with no words eaten
+    cfg = ra.WorldConfig(128, 64)
+    F = cfg.fluid_cells()
+    # reductant to eat: a uniform field of 3.0 plus what the vents add
+    for w in (gw, ow, twin):
+        w.upload(ra.REDOX, np.full(F, 3.0, np.float32))
+        w.encode_steps(4)
+    assert compare(gw, ow, "eaten before")
+    rng = np.random.default_rng(77)
+    for k, n in enumerate((1, 2, 5, 8)):
+        words = np.zeros(F, np.uint32)
+        cells = rng.choice(F, F // 8, replace=False)
+        words[cells] = rng.integers(1, 40000, cells.size)
+        gw.upload(ra.REDOX_EATEN, words)
+        ow.upload(ra.REDOX_EATEN, words)
+        assert np.array_equal(gw.download(ra.REDOX_EATEN), words)
+        assert gw.encode_steps(n) == n and ow.encode_steps(n) == n and twin.encode_steps(n) == n
+        assert compare(gw, ow, f"eaten {k}, after {n} steps"), f"eaten {k}: GPU differs from the oracle"
+        assert not gw.download(ra.REDOX_EATEN).any() and not ow.download(ra.REDOX_EATEN).any()
+    # the words took reductant away: the twin that ate nothing holds more
+    assert ow.download(ra.REDOX).astype(np.float64).sum() < twin.download(ra.REDOX).astype(np.float64).sum()

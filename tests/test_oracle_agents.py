@@ -160,3 +160,37 @@ def test_deposit_ring_order():
     assert c[60, 10, 2] == word(4, 0)   # 20 rows into the band: nothing open within 12 cells
     assert c[30, 30, 3] == word(1, 0)   # a plain add ignores the rock
     assert int(np.count_nonzero(c)) == 4
+
+
+def test_redox_eaten_is_subtracted_before_decay_and_cleared():
+    """redox_eaten (fluid.wgsl:360,371; fed by lifecycle.wgsl:821): on a
+    uniform reductant field with no flow, no vent and no rock, advection and
+    diffusion leave r unchanged, then r' = max(r - e / 10^4, 0) x
+    exp(-redox_decay dt); copy_temperature clears the words (:365-372), so a
+    second fluid step only decays."""
+    dt = np.float32(0.0625)
+    p = parity_params(5, vent_cycle_amp=0.0)
+    p.dt = p.fluid_dt = float(dt)
+    w = oc.OracleWorld(CFG, 5)
+    w.params = p
+    w.set_settings(parity_settings())
+    w.upload(ra.GAMMA, np.zeros(CFG.cells(), np.uint32))
+    w.upload(ra.REDOX, np.full(F * F, 4.0, np.float32))
+    eaten = np.zeros(F * F, np.uint32)
+    eaten[: F * F // 2] = 15000          # 1.5: r = 2.5
+    eaten[F * F // 2:] = 60000           # 6.0 > r: clamps at 0
+    w.upload(ra.REDOX_EATEN, eaten)
+    w.run_kernel("temperature")
+    r = w.download(ra.REDOX)
+    decay = np.float32(oc.expf(float(-np.float32(p.redox_decay) * dt)))
+    # the first row and column see the clamped neighbour of the other half,
+    # which is equal inside each half: compare rows away from the seam
+    assert np.all(r[: F * F // 2 - F] == np.float32(2.5) * decay)
+    assert np.all(r[F * F // 2 + F:] == 0.0)
+    assert not w.download(ra.REDOX_EATEN).any()
+    # a plausible slip fails here: subtracting after the decay would give
+    # 4 decay - 1.5 instead of (4 - 1.5) decay
+    assert np.float32(2.5) * decay != np.float32(4.0) * decay - np.float32(1.5)
+    w.run_kernel("temperature")
+    r2 = w.download(ra.REDOX)
+    assert np.allclose(r2[: F * F // 2 - 2 * F], np.float32(2.5) * decay * decay, rtol=1e-6)

@@ -424,6 +424,17 @@ RIBO_API int ribo_world_chem_ops(RiboWorld* w, const RiboChemOp* ops, uint32_t n
 /* Exact free-matter ledger of the current grid: out[0..3] activated A U G C,
  * out[4..7] spent, out[8..11] held in agents (always 0 here).  u32 wrap. */
 RIBO_API int ribo_world_ledger(RiboWorld* w, uint32_t out[12]);
+/* The same totals as ribo_world_ledger for a grid the caller holds in its
+ * own device memory (`cells` cells of 4 u32 words on the current device),
+ * computed on `stream` (a hipStream_t; NULL = the null stream), which the
+ * call then waits for.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (serving loops, harnesses that forbid
+ * hipMalloc).  It allocates nothing: its partial sums live in a fixed
+ * device scratch of ~33 KB per GPU, held until the process ends and shared
+ * by every call on that GPU, so calls that overlap on one GPU race.  At
+ * most RIBO_LEDGER_NOALLOC_MAX_CELLS cells (RIBO_ERR_ARG beyond). */
+#define RIBO_LEDGER_NOALLOC_MAX_CELLS (1ull << 30)
+RIBO_API int ribo_ledger_noalloc(const uint32_t* chem, uint64_t cells, void* stream, uint32_t out[12]);
 /* With settings.ledger_check = 1: returns RIBO_OK if every checked step kept
  * the per-channel totals seen when checking was armed (when ledger_check was
  * switched on, and again at every upload of RIBO_BUF_CHEM), else

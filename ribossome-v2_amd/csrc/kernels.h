@@ -233,6 +233,13 @@ void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check);
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n);
+// The free-matter totals of `cells` cells of a grid into led->last (no
+// check, no epoch).
+void launch_ledger_cells(hipStream_t s, const uint32_t* chem, uint32_t cells, LedgerDev* led);
+// Device address of this device's ledger scratch (a file-scope device
+// array of kernels.hip, looked up once per device): the only buffer
+// ribo_ledger_noalloc uses beyond its input.
+hipError_t ledger_scratch(LedgerDev** out);
 
 }  // namespace ribo
 

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <type_traits>
 
 #include "kernels.h"
@@ -3419,8 +3420,42 @@ void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev*
     k_ledger<<<nb, 256, 0, s>>>(chem, cells, led);
     k_ledger_fin<<<1, 256, 0, s>>>(led, nb, epoch, ep, check);
 }
+void launch_ledger_cells(hipStream_t s, const uint32_t* chem, uint32_t cells, LedgerDev* led) {
+    const uint32_t nb = rb_minu(blocks1(cells, 256), LEDGER_MAX_BLOCKS);
+    k_ledger<<<nb, 256, 0, s>>>(chem, cells, led);
+    k_ledger_fin<<<1, 256, 0, s>>>(led, nb, 0u, nullptr, 0);
+}
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n) {
     k_epoch_add<<<1, 1, 0, s>>>(ep, n);
+}
+
+
+// ---- scratch of the ledger that allocates nothing (ribo_ledger_noalloc) ----
+// One ledger's per-block partials and totals (sizeof(LedgerDev), ~33 KB),
+// resident on every device the library runs on until the process ends.  In
+// host code `scratch` is only a stand-in for the symbol: its device address
+// comes from hipGetSymbolAddress, once per device, and only that address is
+// passed to kernels or copies.
+static __device__ __align__(256) uint32_t scratch[sizeof(LedgerDev) / sizeof(uint32_t)];
+static_assert(sizeof(LedgerDev) % sizeof(uint32_t) == 0, "the ledger is whole words");
+
+hipError_t ledger_scratch(LedgerDev** out) {
+    constexpr int MAX_DEVICES = 64;
+    static std::mutex mu;
+    static LedgerDev* ptr[MAX_DEVICES] = {};
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(mu);
+    if (!ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        ptr[dev] = static_cast<LedgerDev*>(p);
+    }
+    *out = ptr[dev];
+    return hipSuccess;
 }
 
 }  // namespace ribo

@@ -1345,6 +1345,28 @@ int ribo_world_ledger(RiboWorld* w, uint32_t out[12]) {  // encode_ledger + ledg
     return RIBO_OK;
 }
 
+int ribo_ledger_noalloc(const uint32_t* chem, uint64_t cells, void* stream, uint32_t out[12]) {
+    if (!out || (cells > 0 && !chem)) return fail(RIBO_ERR_ARG, "null argument");
+    if (cells > RIBO_LEDGER_NOALLOC_MAX_CELLS)
+        return fail(RIBO_ERR_ARG, "ribo_ledger_noalloc: " + std::to_string(cells) + " cells exceed the maximum of " +
+                                      std::to_string(RIBO_LEDGER_NOALLOC_MAX_CELLS) + " (a 32768^2 grid)");
+    for (int i = 0; i < 12; ++i) out[i] = 0u;
+    if (cells == 0) return RIBO_OK;
+    LedgerDev* led = nullptr;
+    const hipError_t e = ledger_scratch(&led);
+    if (e != hipSuccess) return fail(RIBO_ERR_HIP, std::string("ribo_ledger_noalloc: scratch address: ") + hipGetErrorString(e));
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    // k_ledger rewrites every partial k_ledger_fin reads, and k_ledger_fin
+    // every total: nothing of the last call's scratch needs clearing
+    launch_ledger_cells(s, chem, (uint32_t)cells, led);
+    HIPCHK(hipGetLastError());
+    uint32_t last[8];
+    HIPCHK(hipMemcpyAsync(last, led->last, sizeof(last), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    for (int i = 0; i < 8; ++i) out[i] = last[i];
+    return RIBO_OK;
+}
+
 int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch) {
     if (!w || !epoch) return fail(RIBO_ERR_ARG, "null argument");
     uint32_t fb = 0;

@@ -308,6 +308,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_solver_stats": (I, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                         ctypes.POINTER(ctypes.c_uint64)]),
         "ribo_world_solver_residuals": (I, [P, P, I]),
+        "ribo_ledger_noalloc": (I, [P, ctypes.c_uint64, P, P]),
         "ribo_last_error": (ctypes.c_char_p, []),
         "ribo_abi_version": (U32, []),
         "ribo_gen_gamma": (I, [P, ctypes.c_uint64, ctypes.c_float, I, P]),
@@ -575,6 +576,15 @@ class World:
         r = ctypes.c_double()
         _check(lib().ribo_world_mg_stats(self._h, ctypes.byref(c), ctypes.byref(r)), "mg_stats")
         return int(c.value), float(r.value)
+
+
+def ledger_noalloc(chem_device_ptr: int, cells: int, stream: int = 0) -> Ledger:
+    """Ledger of a grid in the caller's device memory (`cells` cells of 4
+    u32 words at chem_device_ptr), computed without allocating on `stream`."""
+    out = (ctypes.c_uint32 * 12)()
+    _check(lib().ribo_ledger_noalloc(ctypes.c_void_p(chem_device_ptr), cells, ctypes.c_void_p(stream), out),
+           "ledger_noalloc")
+    return Ledger.from_words(out)
 
 
 # ---- synthetic inputs (host generators in the library) ----

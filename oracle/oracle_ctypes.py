@@ -50,6 +50,10 @@ def lib() -> ctypes.CDLL:
             "oracle_sinf": (ctypes.c_float, [ctypes.c_float]),
             "oracle_mg_solve": (I, [U32, U32, P, P, P, U32, I]),
             "oracle_mg_solve_f64": (I, [U32, U32, P, P, P, U32]),
+            "oracle_mg_restrict": (I, [U32, P, P, P, P, P]),
+            "oracle_mg_prolong": (I, [U32, P, P, P]),
+            "oracle_bl_transfer": (I, [U32, P, P, P, P, P, P]),
+            "oracle_coarse_inverse_error": (ctypes.c_double, [U32, P, ctypes.c_float]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -212,3 +216,43 @@ def mg_solve(n: int, min_size: int, fluid: np.ndarray, rhs: np.ndarray, p0: np.n
     p = np.array(p0, dtype=np.float32, copy=True)
     lib().oracle_mg_solve(n, min_size, _ptr(f), _ptr(r), _ptr(p), cycles, 0)
     return p
+
+
+def mg_restrict(n: int, fluid: np.ndarray, rhs: np.ndarray, p: np.ndarray):
+    """One mg_restrict of an n x n level: (coarse rhs, coarse fluid mask)."""
+    f = np.ascontiguousarray(fluid, dtype=np.uint32)
+    r = np.ascontiguousarray(rhs, dtype=np.float32)
+    q = np.ascontiguousarray(p, dtype=np.float32)
+    rc = np.zeros((n // 2) * (n // 2), np.float32)
+    fc = np.zeros((n // 2) * (n // 2), np.uint32)
+    if lib().oracle_mg_restrict(n, _ptr(f), _ptr(r), _ptr(q), _ptr(rc), _ptr(fc)) != 0:
+        raise ValueError("n must be even")
+    return rc, fc
+
+
+def mg_prolong(n: int, fluid: np.ndarray, p: np.ndarray, p_c: np.ndarray) -> np.ndarray:
+    """One mg_prolong of an n x n level: p + the parent's coarse value on fluid cells."""
+    f = np.ascontiguousarray(fluid, dtype=np.uint32)
+    q = np.array(p, dtype=np.float32, copy=True)
+    c = np.ascontiguousarray(p_c, dtype=np.float32)
+    if lib().oracle_mg_prolong(n, _ptr(f), _ptr(q), _ptr(c)) != 0:
+        raise ValueError("n must be even")
+    return q
+
+
+def bl_transfer(n: int, fluid: np.ndarray, fluid_c: np.ndarray, c: np.ndarray, r: np.ndarray):
+    """The tolerance mode's bilinear transfer in binary64: (P c, R r)."""
+    f = np.ascontiguousarray(fluid, dtype=np.uint32)
+    fc = np.ascontiguousarray(fluid_c, dtype=np.uint32)
+    ci = np.ascontiguousarray(c, dtype=np.float64)
+    ri = np.ascontiguousarray(r, dtype=np.float64)
+    po = np.zeros(n * n)
+    rc = np.zeros((n // 2) * (n // 2))
+    if lib().oracle_bl_transfer(n, _ptr(f), _ptr(fc), _ptr(ci), _ptr(po), _ptr(ri), _ptr(rc)) != 0:
+        raise ValueError("n must be even")
+    return po, rc
+
+
+def coarse_inverse_error(n: int, fluid: np.ndarray, eps: float) -> float:
+    f = np.ascontiguousarray(fluid, dtype=np.uint32)
+    return float(lib().oracle_coarse_inverse_error(n, _ptr(f), eps))

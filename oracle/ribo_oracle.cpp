@@ -2380,6 +2380,7 @@ int oracle_run_kernel(OracleWorld* w, const char* name) {
     else if (n == "subtract_gradient") { subtract_gradient<float>(c, fl.vel_a.data(), fl.vel_b.data(), fl.p_a.data()); fl.vel_a = fl.vel_b; }
     else if (n == "net_flux") net_flux<float>(c, fl.vel_a.data(), fl.net_flux.data());
     else if (n == "boundaries") { enforce_boundaries<float>(c, fl.vel_a.data(), fl.vel_b.data(), fl.net_flux.data()); fl.vel_a = fl.vel_b; }
+    else if (n == "jacobi") { jacobi_pressure<float>(c, fl.p_a.data(), fl.p_b.data(), fl.div.data()); fl.p_a = fl.p_b; }
     else if (n == "transport") transport<float>(*w, fl, P);
     else if (n == "thermal_activation") thermal_activation<float>(*w, fl, P);
     else return -1;
@@ -2521,6 +2522,98 @@ int oracle_mg_solve(uint32_t n, uint32_t min_size, const uint32_t* fluid, const 
 int oracle_mg_solve_f64(uint32_t n, uint32_t min_size, const uint32_t* fluid, const double* rhs, double* p,
                         uint32_t cycles) {
     return mg_solve_t<double>(n, min_size, fluid, rhs, p, cycles);
+}
+
+// One mg_prolong (multigrid.wgsl:108-114): p += coarse p of the parent on
+// fluid cells of an n x n level.
+int oracle_mg_prolong(uint32_t n, const uint32_t* fluid, float* p, const float* p_c) {
+    if (n < 2 || n % 2) return -1;
+    uint32_t total;
+    std::vector<Level> lv = make_levels(n, n / 2, &total);
+    std::vector<float> P(total, 0.0f), RHS(total, 0.0f);
+    std::vector<uint32_t> FL(total, 0);
+    for (uint32_t i = 0; i < n * n; ++i) { FL[i] = fluid[i] ? 1u : 0u; P[i] = p[i]; }
+    for (uint32_t i = 0; i < (n / 2) * (n / 2); ++i) P[lv[0].off_c + i] = p_c[i];
+    MG<float> mg{&lv, P.data(), RHS.data(), FL.data()};
+    mg.prolong(lv[0]);
+    for (uint32_t i = 0; i < n * n; ++i) p[i] = P[i];
+    return 0;
+}
+
+// The bilinear transfer of the tolerance mode between an n x n level and
+// its coarse level, binary64: prolong adds the interpolated coarse values
+// (P c) to p, restrict writes R r (p = 0, so the residual is rhs) into rhs_c.
+// Fluid masks: fine u32[n*n], coarse u32[(n/2)^2].
+int oracle_bl_transfer(uint32_t n, const uint32_t* fluid, const uint32_t* fluid_c, const double* c_in, double* p_out,
+                       const double* r_in, double* rhs_c) {
+    if (n < 2 || n % 2) return -1;
+    uint32_t total;
+    std::vector<Level> lv = make_levels(n, n / 2, &total);
+    const uint32_t m = (n / 2) * (n / 2);
+    std::vector<double> P(total, 0.0), RHS(total, 0.0);
+    std::vector<uint32_t> FL(total, 0);
+    for (uint32_t i = 0; i < n * n; ++i) FL[i] = fluid[i] ? 1u : 0u;
+    for (uint32_t i = 0; i < m; ++i) FL[lv[0].off_c + i] = fluid_c[i] ? 1u : 0u;
+    MG<double> mg{&lv, P.data(), RHS.data(), FL.data()};
+    for (uint32_t i = 0; i < m; ++i) P[lv[0].off_c + i] = c_in[i];
+    mg.prolong_bl(lv[0]);
+    for (uint32_t i = 0; i < n * n; ++i) p_out[i] = P[i];
+    std::fill(P.begin(), P.end(), 0.0);
+    for (uint32_t i = 0; i < n * n; ++i) RHS[i] = r_in[i];
+    std::vector<double> rn;
+    mg.restrict_bl(lv[0], rn);
+    for (uint32_t i = 0; i < m; ++i) rhs_c[i] = RHS[lv[0].off_c + i];
+    return 0;
+}
+
+// The exact coarse solve of the tolerance mode: max |A inv - I| over the
+// fluid unknowns of an n x n mask (eps as given), or -1 when not built
+// (no fluid, or more than 1024 unknowns).
+double oracle_coarse_inverse_error(uint32_t n, const uint32_t* fluid, float eps) {
+    CoarseInv ci;
+    if (!ci.build(n, fluid, eps)) return -1.0;
+    const size_t m = ci.cell.size();
+    double worst = 0.0;
+    for (size_t r = 0; r < m; ++r) {
+        const uint32_t c = ci.cell[r], x = c % n, y = c / n;
+        for (size_t j = 0; j < m; ++j) {
+            // row r of A (fluid neighbours 1, diagonal -(k + eps)) times column j of inv
+            double acc = 0.0, k = 0.0;
+            const int64_t nb[4] = {x > 0 ? (int64_t)c - 1 : -1, x + 1 < n ? (int64_t)c + 1 : -1,
+                                   y > 0 ? (int64_t)c - n : -1, y + 1 < n ? (int64_t)c + n : -1};
+            for (int d = 0; d < 4; ++d)
+                if (nb[d] >= 0 && ci.idx[(size_t)nb[d]] >= 0) {
+                    acc += ci.inv[(size_t)ci.idx[(size_t)nb[d]] * m + j];
+                    k += 1.0;
+                }
+            acc -= (k + (double)eps) * ci.inv[r * m + j];
+            worst = std::max(worst, fabs(acc - (r == j ? 1.0 : 0.0)));
+        }
+    }
+    return worst;
+}
+
+// One mg_restrict (multigrid.wgsl:82-105) of an n x n level (n even).
+int oracle_mg_restrict(uint32_t n, const uint32_t* fluid, const float* rhs, const float* p, float* rhs_c,
+                       uint32_t* fluid_c) {
+    if (n < 2 || n % 2) return -1;
+    uint32_t total;
+    std::vector<Level> lv = make_levels(n, n / 2, &total);
+    std::vector<float> P(total, 0.0f), RHS(total, 0.0f);
+    std::vector<uint32_t> FL(total, 0);
+    for (uint32_t i = 0; i < n * n; ++i) {
+        FL[i] = fluid[i] ? 1u : 0u;
+        RHS[i] = rhs[i];
+        P[i] = p[i];
+    }
+    MG<float> mg{&lv, P.data(), RHS.data(), FL.data()};
+    mg.restrict_(lv[0]);
+    const uint32_t m = (n / 2) * (n / 2);
+    for (uint32_t i = 0; i < m; ++i) {
+        rhs_c[i] = RHS[lv[0].off_c + i];
+        fluid_c[i] = FL[lv[0].off_c + i];
+    }
+    return 0;
 }
 
 }  // extern "C"

@@ -99,6 +99,23 @@ int oracle_mg_solve(uint32_t n, uint32_t min_size, const uint32_t* fluid, const 
 int oracle_mg_solve_f64(uint32_t n, uint32_t min_size, const uint32_t* fluid, const double* rhs, double* p,
                         uint32_t cycles);
 
+/* One restriction (mg_restrict, multigrid.wgsl:82-105) of an n x n level
+ * (n even): fluid u32[n*n], rhs and p f32[n*n] in; the coarse right-hand
+ * side f32[(n/2)^2] and fluid mask u32[(n/2)^2] out. */
+int oracle_mg_restrict(uint32_t n, const uint32_t* fluid, const float* rhs, const float* p, float* rhs_c,
+                       uint32_t* fluid_c);
+
+/* One prolongation (mg_prolong, multigrid.wgsl:108-114): p += the parent's
+ * coarse value on fluid cells of an n x n level. */
+int oracle_mg_prolong(uint32_t n, const uint32_t* fluid, float* p, const float* p_c);
+/* The tolerance mode's bilinear transfer, binary64: p_out = P c_in on the
+ * n x n level, rhs_c = R r_in on its coarse level. */
+int oracle_bl_transfer(uint32_t n, const uint32_t* fluid, const uint32_t* fluid_c, const double* c_in, double* p_out,
+                       const double* r_in, double* rhs_c);
+/* max |A inv - I| of the tolerance mode's exact coarse solve over an n x n
+ * mask (-1 when it is not built). */
+double oracle_coarse_inverse_error(uint32_t n, const uint32_t* fluid, float eps);
+
 #ifdef __cplusplus
 }
 #endif

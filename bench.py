@@ -383,8 +383,12 @@ def agents_measure(cfg, inputs, args):
         out["smooth_velocity"] = entry(tot / cnt * 1000.0, kernel_bytes("smooth_velocity", cfg, [], "reference"))
     tot, cnt = times.get("chem_ops", (0.0, 0))
     if cnt:
+        # the library splits a call into launches of at most F^2 / 2
+        # exchanges (its staging buffer): 4 calls of n_ops make `cnt` launches
         us = tot / cnt * 1000.0
-        out["chem_ops"] = dict(entry(us, n_ops * (16 + 4 + 64)), exchanges=n_ops, exchanges_per_s=n_ops / (us * 1e-6))
+        per_launch = 4 * n_ops / cnt
+        out["chem_ops"] = dict(entry(us, per_launch * (16 + 4 + 64)), exchanges_per_launch=per_launch,
+                               exchanges_per_call=n_ops, exchanges_per_s=4 * n_ops / (tot * 1e-3))
     return out
 
 

@@ -71,13 +71,17 @@ typedef struct RiboSceneBlock {
  * src/world/snapshot.rs:274-363): the header's "agentes" count (at most
  * cfg->max_agents) and the blocks given; the blocks not given are written
  * empty (life_counters as eight zero words).  RIBO_ERR_ARG for an unknown or
- * repeated name. */
+ * repeated name, and for blocks the reference's load_scene would refuse
+ * (snapshot.rs:403-405): "slots" must hold 4 bytes and "agents" 64 bytes
+ * (one Agent) per agent, "life_counters" 32 bytes. */
 RIBO_API int ribo_scene_write_agents(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
                                      const RiboWorldSettings* settings, const RiboSceneGrids* grids, uint32_t agents,
                                      const RiboSceneBlock* blocks, uint32_t n_blocks);
 /* One block of a scene, decompressed (any name, the grids included):
- * *bytes receives its size; dst may be NULL with cap 0 to ask the size.
- * RIBO_ERR_ARG for a malformed file, a missing block or cap < *bytes. */
+ * *bytes receives its size; dst may be NULL with cap 0 to ask the size
+ * (RIBO_OK, nothing decompressed).  RIBO_ERR_ARG for a malformed file (a
+ * block whose stated size exceeds what its compressed bytes can expand to
+ * is refused before any allocation), a missing block or 0 < cap < *bytes. */
 RIBO_API int ribo_scene_read_block(const char* path, const char* name, void* dst, uint64_t cap, uint64_t* bytes);
 /* The header's agent count ("agentes"). */
 RIBO_API int ribo_scene_agents(const char* path, uint32_t* agents);

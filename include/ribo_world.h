@@ -410,6 +410,10 @@ enum {
     RIBO_CHEM_DEPOSIT_ACT = 6,
     RIBO_CHEM_DEPOSIT_SPENT = 7
 };
+/* Deposits (ADD_*, DEPOSIT_*) add their count to the cell's 16-bit field at
+ * once, as chem_add_state does (chem.wgsl:70-74): a count above 0xFFFF is
+ * refused (RIBO_ERR_ARG) before anything changes; like the reference, a
+ * deposit into a field that is nearly full wraps into the next field. */
 /* Applies n exchanges on the device, all at once, between steps (the agents
  * of the reference act between world kernels in the same frame); done[i]
  * (may be NULL) receives the monomers exchange i moved.  Exchanges on
@@ -498,6 +502,60 @@ RIBO_API int ribo_gen_velocity(const RiboWorldConfig* cfg, uint64_t seed, uint32
                       const uint32_t* gamma, float* out);
 RIBO_API int ribo_gen_chem(const RiboWorldConfig* cfg, uint64_t seed, float lambda, const uint32_t* gamma, uint32_t* out);
 RIBO_API int ribo_gen_vent_heat(const RiboWorldConfig* cfg, uint32_t rows, float h, float* out);
+
+/* ---- vents and matter seeding (host only; the reference's CPU code) ----- */
+
+/* Fumarole (src/params.rs:795-810): a point vent, position as a fraction of
+ * the world, radius in world units; 32 bytes as the reference's repr(C). */
+typedef struct RiboFumarole {
+    float x_frac, y_frac;
+    float strength;
+    float spread;
+    uint32_t enabled, _pad0, _pad1, _pad2;
+} RiboFumarole;
+/* Heat of a pure-red image pixel, FUMAROLE_PIXEL_STRENGTH (src/world/terrain.rs:247),
+ * the default vent gain FUMAROLE_GAIN_DEFAULT and the seeding defaults
+ * SEED_DENSITY_DEFAULT / SEED_ACTIVE_DEFAULT (src/world/mod.rs:1980-1984). */
+#define RIBO_FUMAROLE_PIXEL_STRENGTH 5000.0f
+#define RIBO_FUMAROLE_GAIN_DEFAULT 0.3f
+#define RIBO_SEED_DENSITY_DEFAULT 0.83f
+#define RIBO_SEED_ACTIVE_DEFAULT 1.0f
+
+/* rasterize_fumaroles (src/world/terrain.rs:316-336): heat f32[G] of the
+ * enabled fumaroles, strength (1 - d/r)^2 inside the radius, as a fraction
+ * of RIBO_FUMAROLE_PIXEL_STRENGTH. */
+RIBO_API int ribo_rasterize_fumaroles(const RiboWorldConfig* cfg, const RiboFumarole* f, uint32_t count, float* heat);
+/* upload_heat's grid -> fluid step (src/world/mod.rs:1366-1392): out f32[2F]
+ * = (heat, chemistry) box sums x RIBO_FUMAROLE_PIXEL_STRENGTH x max(gain, 0)
+ * / k^2, k = grid / fluid, from the grid-resolution heat and chemistry. */
+RIBO_API int ribo_vent_heat_src(const RiboWorldConfig* cfg, const float* heat_grid, const float* chem_grid, float gain,
+                                float* out);
+/* World::upload_heat with heat_grid / chem_grid (src/world/mod.rs:1343-1392):
+ * the fumaroles' raster plus the optional heat image f32[G] (heat_image,
+ * NULL for none) give the heat; the raster plus chem_image (or, without it,
+ * the heat image) the chemistry; the result becomes RIBO_BUF_HEAT_SRC. */
+RIBO_API int ribo_world_set_vents(RiboWorld* w, const RiboFumarole* f, uint32_t count, float gain,
+                                  const float* heat_image, const float* chem_image);
+/* seed_cells (src/world/mod.rs:1990-2014): the initial matter u32[4G], per
+ * channel an fBm density field of 6 octaves, Poisson-like rounding and the
+ * activated fraction `active` (SplitMix64 from seed, bit-identical to the
+ * reference). */
+RIBO_API int ribo_seed_cells(const RiboWorldConfig* cfg, uint64_t seed, float density, float active, uint32_t* cells);
+/* fit_matter_to_terrain (src/world/mod.rs:1970-1983): rock empty, rubble
+ * full to capacity with half activated on average, n cells. */
+RIBO_API int ribo_fit_matter_to_terrain(uint32_t* cells, const uint32_t* gamma, uint64_t n, uint64_t seed);
+/* refit_matter_to_terrain (src/world/mod.rs:1870-1968): matter over a
+ * cell's capacity leaves it in proportion to its kinds and is spread over
+ * every cell with room in proportion to the room; exact, nothing lost. */
+RIBO_API int ribo_refit_matter_to_terrain(uint32_t* cells, const uint32_t* gamma, uint64_t n);
+/* World::restart_keeping_terrain (src/world/mod.rs:990-1004): new matter
+ * from seed_cells + fit_matter_to_terrain on the world's current terrain,
+ * epoch 0; out[12] the seeded ledger (may be NULL). */
+RIBO_API int ribo_world_restart_matter(RiboWorld* w, uint64_t seed, float density, float active, uint32_t out[12]);
+/* World::apply_terrain_live (src/world/mod.rs:1283-1297) for the terrain:
+ * the live matter is refitted to gamma u32[G] (refit_matter_to_terrain),
+ * then both are uploaded. */
+RIBO_API int ribo_world_apply_terrain_live(RiboWorld* w, const uint32_t* gamma);
 
 #ifdef __cplusplus
 }

@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <map>
+#include <new>
 #include <memory>
 #include <string>
 #include <vector>
@@ -400,6 +401,7 @@ const uint32_t SETTING_DEFAULTS[] = {1u, 2u, 1u, 1u, 128u, 1u, 10u, 1u, 1u};
 
 // The agent blocks of World::slot_bufs (src/world/snapshot.rs:212-236): a
 // scene without agents holds each of them empty.
+constexpr uint32_t AGENT_BYTES = 64;  // size_of::<Agent>() (src/params.rs:695-729)
 const char* const AGENT_BLOCKS[] = {"agents", "genomes", "bodies", "organs", "rna_tails4", "bitten", "body_pos",
                                     "signals4", "joint_angle", "joint_base", "joint_state", "joint_active",
                                     "sensor_mem", "bonds2", "pair_mask"};
@@ -441,11 +443,12 @@ std::string cfg_json(const RiboWorldConfig& c) {  // cfg_json (src/world/snapsho
 
 }  // namespace
 
-extern "C" {
+namespace {
 
-size_t ribo_lz4_bound(size_t n) { return n + n / 255 + 16; }
 
-int64_t ribo_lz4_compress(const void* src, size_t n, void* dst, size_t dst_cap) {
+size_t lz4_bound_impl(size_t n) { return n + n / 255 + 16; }
+
+int64_t lz4_compress_impl(const void* src, size_t n, void* dst, size_t dst_cap) {
     if ((!src && n) || !dst) return RIBO_ERR_ARG;
     const std::vector<uint8_t> c = lz4_compress((const uint8_t*)src, n);
     if (c.size() > dst_cap) return RIBO_ERR_ARG;
@@ -453,18 +456,22 @@ int64_t ribo_lz4_compress(const void* src, size_t n, void* dst, size_t dst_cap) 
     return (int64_t)c.size();
 }
 
-int64_t ribo_lz4_decompress(const void* src, size_t n, void* dst, size_t dst_cap) {
+int64_t lz4_decompress_impl(const void* src, size_t n, void* dst, size_t dst_cap) {
     if ((!src && n) || (!dst && dst_cap)) return RIBO_ERR_ARG;
     const int64_t r = lz4_decompress((const uint8_t*)src, n, (uint8_t*)dst, dst_cap);
     return r < 0 ? RIBO_ERR_ARG : r;
 }
 
-int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+int scene_write_agents_impl(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+                            const RiboWorldSettings* settings, const RiboSceneGrids* grids, uint32_t agents,
+                            const RiboSceneBlock* blocks, uint32_t n_blocks);
+
+int scene_write_impl(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
                      const RiboWorldSettings* settings, const RiboSceneGrids* grids) {
-    return ribo_scene_write_agents(path, cfg, params, settings, grids, 0, nullptr, 0);
+    return scene_write_agents_impl(path, cfg, params, settings, grids, 0, nullptr, 0);
 }
 
-int ribo_scene_write_agents(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+int scene_write_agents_impl(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
                             const RiboWorldSettings* settings, const RiboSceneGrids* grids, uint32_t agents,
                             const RiboSceneBlock* blocks, uint32_t n_blocks) {
     if (!path || !cfg || !params || !settings || !grids || (n_blocks && !blocks)) return RIBO_ERR_ARG;
@@ -480,6 +487,16 @@ int ribo_scene_write_agents(const char* path, const RiboWorldConfig* cfg, const 
         if (!known || given.count(nm)) return RIBO_ERR_ARG;
         given[nm] = &blocks[i];
     }
+    // what World::load_scene checks (src/world/snapshot.rs:403-405): one
+    // slot word and one Agent (16 four-byte fields, src/params.rs:695-729)
+    // per agent, 8 life counters
+    auto given_bytes = [&](const char* nm) -> uint64_t {
+        auto it = given.find(nm);
+        return it == given.end() ? 0u : it->second->bytes;
+    };
+    if (given_bytes("slots") != 4ull * agents || given_bytes("agents") != (uint64_t)AGENT_BYTES * agents ||
+        (given.count("life_counters") && given_bytes("life_counters") != 32u))
+        return RIBO_ERR_ARG;
     // the header (src/world/snapshot.rs:325-343)
     std::string h = "{\n  \"versao\": " + std::to_string(SCENE_VERSION) + ",\n  \"gravado\": 0,\n  \"agentes\": " +
                     std::to_string(agents) + ",\n";
@@ -568,12 +585,24 @@ struct SceneFile {
         }
         return true;
     }
-    // decompress_size_prepended of one block: 0, 1 when absent, -1 when malformed
-    int block(const std::string& name, std::vector<uint8_t>* out) const {
+    // The size an LZ4 block says it holds (the 4 bytes lz4_flex prepends):
+    // 0, 1 when absent, -1 when it exceeds what its compressed length can
+    // expand to (an LZ4 sequence gives at most 255 bytes a byte, plus the
+    // last literals), so a corrupt or crafted size is refused before any
+    // allocation.
+    int block_size(const std::string& name, uint32_t* raw) const {
         auto it = blocks.find(name);
         if (it == blocks.end()) return 1;
-        uint32_t raw;
-        memcpy(&raw, bytes.data() + it->second.first, 4);
+        memcpy(raw, bytes.data() + it->second.first, 4);
+        const uint64_t most = 255ull * (uint64_t)(it->second.second - 4) + 16ull;
+        return (uint64_t)*raw <= most ? 0 : -1;
+    }
+    // decompress_size_prepended of one block: 0, 1 when absent, -1 when malformed
+    int block(const std::string& name, std::vector<uint8_t>* out) const {
+        uint32_t raw = 0;
+        const int sz = block_size(name, &raw);
+        if (sz != 0) return sz;
+        auto it = blocks.find(name);
         out->assign(raw, 0);
         const int64_t r = lz4_decompress(bytes.data() + it->second.first + 4, it->second.second - 4, out->data(), raw);
         return r == (int64_t)raw ? 0 : -1;
@@ -582,20 +611,22 @@ struct SceneFile {
 
 }  // namespace
 
-int ribo_scene_read_block(const char* path, const char* name, void* dst, uint64_t cap, uint64_t* bytes) {
+int scene_read_block_impl(const char* path, const char* name, void* dst, uint64_t cap, uint64_t* bytes) {
     if (!path || !name || !bytes || (cap && !dst)) return RIBO_ERR_ARG;
     SceneFile sf;
     if (!sf.open(path)) return RIBO_ERR_ARG;
+    uint32_t raw = 0;
+    if (sf.block_size(name, &raw) != 0) return RIBO_ERR_ARG;
+    *bytes = raw;
+    if (raw > cap) return cap == 0 ? RIBO_OK : RIBO_ERR_ARG;  // cap 0: the size query, nothing decompressed
     std::vector<uint8_t> data;
     const int rc = sf.block(name, &data);
     if (rc != 0) return RIBO_ERR_ARG;
-    *bytes = data.size();
-    if (data.size() > cap) return RIBO_ERR_ARG;
     if (!data.empty()) memcpy(dst, data.data(), data.size());
     return RIBO_OK;
 }
 
-int ribo_scene_agents(const char* path, uint32_t* agents) {
+int scene_agents_impl(const char* path, uint32_t* agents) {
     if (!path || !agents) return RIBO_ERR_ARG;
     SceneFile sf;
     if (!sf.open(path)) return RIBO_ERR_ARG;
@@ -604,7 +635,7 @@ int ribo_scene_agents(const char* path, uint32_t* agents) {
     return RIBO_OK;
 }
 
-int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams* params, RiboWorldSettings* settings,
+int scene_read_impl(const char* path, const RiboWorldConfig* cfg, RiboSimParams* params, RiboWorldSettings* settings,
                     RiboSceneGrids* grids) {
     if (!path || !cfg || !params || !settings || !grids) return RIBO_ERR_ARG;
     SceneFile sf;
@@ -660,7 +691,7 @@ int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams*
     return RIBO_OK;
 }
 
-int ribo_world_save_scene(RiboWorld* w, const char* path) {
+int world_save_scene_impl(RiboWorld* w, const char* path) {
     if (!w || !path) return RIBO_ERR_ARG;
     RiboWorldConfig cfg;
     RiboSimParams p;
@@ -674,10 +705,10 @@ int ribo_world_save_scene(RiboWorld* w, const char* path) {
         if ((rc = ribo_world_download(w, GRIDS[k].buf, host[k].data(), host[k].size()))) return rc;
         *grid_ptr(&g, k) = host[k].data();
     }
-    return ribo_scene_write(path, &cfg, &p, &s, &g);
+    return scene_write_impl(path, &cfg, &p, &s, &g);
 }
 
-int ribo_world_load_scene(RiboWorld* w, const char* path) {
+int world_load_scene_impl(RiboWorld* w, const char* path) {
     if (!w || !path) return RIBO_ERR_ARG;
     RiboWorldConfig cfg;
     RiboSimParams p;
@@ -693,7 +724,7 @@ int ribo_world_load_scene(RiboWorld* w, const char* path) {
     // keep the world's redox and heat source when the scene has none
     for (int k = NREQUIRED; k < NGRID; ++k)
         if ((rc = ribo_world_download(w, GRIDS[k].buf, host[k].data(), host[k].size()))) return rc;
-    if ((rc = ribo_scene_read(path, &cfg, &p, &s, &g))) return rc;
+    if ((rc = scene_read_impl(path, &cfg, &p, &s, &g))) return rc;
     if ((rc = ribo_world_set_params(w, &p)) || (rc = ribo_world_set_settings(w, &s))) return rc;
     // gamma first: the derived terrain follows it
     const int order[NGRID] = {1, 0, 2, 3, 4, 5, 6};
@@ -702,4 +733,48 @@ int ribo_world_load_scene(RiboWorld* w, const char* path) {
     return RIBO_OK;
 }
 
+}  // namespace
+
+// Every entry point returns a status and never throws: a file whose sizes
+// would need more memory than the host has gives RIBO_ERR_OOM, anything
+// else unexpected RIBO_ERR_ARG.
+#define RIBO_GUARD(expr)                                  \
+    try {                                                 \
+        return (expr);                                    \
+    } catch (const std::bad_alloc&) {                     \
+        return RIBO_ERR_OOM;                              \
+    } catch (...) {                                       \
+        return RIBO_ERR_ARG;                              \
+    }
+
+extern "C" {
+
+size_t ribo_lz4_bound(size_t n) { return lz4_bound_impl(n); }
+int64_t ribo_lz4_compress(const void* src, size_t n, void* dst, size_t dst_cap) {
+    RIBO_GUARD(lz4_compress_impl(src, n, dst, dst_cap))
+}
+int64_t ribo_lz4_decompress(const void* src, size_t n, void* dst, size_t dst_cap) {
+    RIBO_GUARD(lz4_decompress_impl(src, n, dst, dst_cap))
+}
+int ribo_scene_write(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+                     const RiboWorldSettings* settings, const RiboSceneGrids* grids) {
+    RIBO_GUARD(scene_write_impl(path, cfg, params, settings, grids))
+}
+int ribo_scene_write_agents(const char* path, const RiboWorldConfig* cfg, const RiboSimParams* params,
+                            const RiboWorldSettings* settings, const RiboSceneGrids* grids, uint32_t agents,
+                            const RiboSceneBlock* blocks, uint32_t n_blocks) {
+    RIBO_GUARD(scene_write_agents_impl(path, cfg, params, settings, grids, agents, blocks, n_blocks))
+}
+int ribo_scene_read_block(const char* path, const char* name, void* dst, uint64_t cap, uint64_t* bytes) {
+    RIBO_GUARD(scene_read_block_impl(path, name, dst, cap, bytes))
+}
+int ribo_scene_agents(const char* path, uint32_t* agents) { RIBO_GUARD(scene_agents_impl(path, agents)) }
+int ribo_scene_read(const char* path, const RiboWorldConfig* cfg, RiboSimParams* params, RiboWorldSettings* settings,
+                    RiboSceneGrids* grids) {
+    RIBO_GUARD(scene_read_impl(path, cfg, params, settings, grids))
+}
+int ribo_world_save_scene(RiboWorld* w, const char* path) { RIBO_GUARD(world_save_scene_impl(w, path)) }
+int ribo_world_load_scene(RiboWorld* w, const char* path) { RIBO_GUARD(world_load_scene_impl(w, path)) }
+
 }  // extern "C"
+

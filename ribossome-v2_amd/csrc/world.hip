@@ -1313,9 +1313,14 @@ int ribo_world_idle(RiboWorld* w) {
 int ribo_world_chem_ops(RiboWorld* w, const RiboChemOp* ops, uint32_t n, uint32_t* done) {  // chem.wgsl:40-98
     if (!w || (n > 0 && !ops)) return fail(RIBO_ERR_ARG, "null argument");
     const uint64_t cells = (uint64_t)w->cfg.grid_size * w->cfg.grid_size;
-    for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t i = 0; i < n; ++i) {
         if (ops[i].cell >= cells || ops[i].channel >= 4u || ops[i].kind > RIBO_CHEM_DEPOSIT_SPENT)
             return fail(RIBO_ERR_ARG, "exchange " + std::to_string(i) + ": cell, channel or kind out of range");
+        // a deposit adds its count to a 16-bit field at once (chem_add_state,
+        // chem.wgsl:70-74): more than 0xFFFF cannot be held by any cell
+        if (ops[i].kind >= RIBO_CHEM_ADD_ACT && ops[i].count > 0xFFFFu)
+            return fail(RIBO_ERR_ARG, "exchange " + std::to_string(i) + ": a deposit of more than 65535 monomers");
+    }
     static_assert(sizeof(RiboChemOp) == 16, "an exchange is four words");
     // Between steps the force buffer (8 bytes a fluid cell) and the scratch
     // words are free: the exchanges go through them in batches of F / 2.
@@ -1426,6 +1431,57 @@ int ribo_world_solver_stats(RiboWorld* w, uint64_t* solves, uint64_t* iterations
     *iterations = c.iters;
     *missed = c.missed;
     return RIBO_OK;
+}
+
+int ribo_world_set_vents(RiboWorld* w, const RiboFumarole* f, uint32_t count, float gain, const float* heat_image,
+                         const float* chem_image) {  // src/world/mod.rs:1343-1392
+    if (!w || (count > 0 && !f)) return fail(RIBO_ERR_ARG, "null argument");
+    const size_t G = (size_t)w->G * w->G, F = (size_t)w->F * w->F;
+    std::vector<float> heat(G), chem(G), src(2 * F);
+    int rc = ribo_rasterize_fumaroles(&w->cfg, f, count, heat.data());
+    if (rc) return fail(rc, "ribo_world_set_vents: bad fumarole list");
+    chem = heat;
+    const float* cimg = chem_image ? chem_image : heat_image;
+    if (heat_image)
+        for (size_t i = 0; i < G; ++i) heat[i] += heat_image[i];
+    if (cimg)
+        for (size_t i = 0; i < G; ++i) chem[i] += cimg[i];
+    if ((rc = ribo_vent_heat_src(&w->cfg, heat.data(), chem.data(), gain, src.data()))) return fail(rc, "ribo_vent_heat_src");
+    return ribo_world_upload(w, RIBO_BUF_HEAT_SRC, src.data(), src.size() * sizeof(float));
+}
+
+int ribo_world_restart_matter(RiboWorld* w, uint64_t seed, float density, float active, uint32_t out[12]) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    const size_t G = (size_t)w->G * w->G;
+    std::vector<uint32_t> gamma(G), cells(4 * G);
+    int rc;
+    if ((rc = ribo_world_download(w, RIBO_BUF_GAMMA, gamma.data(), 4 * G))) return rc;
+    if ((rc = ribo_seed_cells(&w->cfg, seed, density, active, cells.data()))) return fail(rc, "ribo_seed_cells");
+    if ((rc = ribo_fit_matter_to_terrain(cells.data(), gamma.data(), G, seed))) return fail(rc, "ribo_fit_matter_to_terrain");
+    if ((rc = ribo_world_upload(w, RIBO_BUF_CHEM, cells.data(), 16 * G))) return rc;
+    w->params.epoch = 0;
+    if (out) {
+        uint64_t t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (size_t i = 0; i < G; ++i)
+            for (int c = 0; c < 4; ++c) {
+                t[c] += cells[4 * i + c] & 0xFFFFu;
+                t[4 + c] += cells[4 * i + c] >> 16;
+            }
+        for (int k = 0; k < 8; ++k) out[k] = (uint32_t)t[k];
+        for (int k = 8; k < 12; ++k) out[k] = 0u;
+    }
+    return RIBO_OK;
+}
+
+int ribo_world_apply_terrain_live(RiboWorld* w, const uint32_t* gamma) {  // src/world/mod.rs:1283-1297
+    if (!w || !gamma) return fail(RIBO_ERR_ARG, "null argument");
+    const size_t G = (size_t)w->G * w->G;
+    std::vector<uint32_t> cells(4 * G);
+    int rc;
+    if ((rc = ribo_world_download(w, RIBO_BUF_CHEM, cells.data(), 16 * G))) return rc;
+    if ((rc = ribo_refit_matter_to_terrain(cells.data(), gamma, G))) return fail(rc, "ribo_refit_matter_to_terrain");
+    if ((rc = ribo_world_upload(w, RIBO_BUF_GAMMA, gamma, 4 * G))) return rc;
+    return ribo_world_upload(w, RIBO_BUF_CHEM, cells.data(), 16 * G);
 }
 
 int ribo_world_solver_residuals(RiboWorld* w, RiboSolverResiduals* out, int reset) {

@@ -249,6 +249,12 @@ BUFFER_NAMES = {CHEM: "chem", GAMMA: "gamma", LIGHT: "light", SLOPE: "slope", VE
 CHEM_SPEND, CHEM_ACTIVATE, CHEM_TAKE_ACT, CHEM_TAKE_SPENT, CHEM_ADD_ACT, CHEM_ADD_SPENT, CHEM_DEPOSIT_ACT, \
     CHEM_DEPOSIT_SPENT = range(8)
 FORCE_FP = 256.0  # fixed-point scale of the force words (fluid.wgsl:280)
+# FUMAROLE_PIXEL_STRENGTH (src/world/terrain.rs:247), FUMAROLE_GAIN_DEFAULT,
+# SEED_DENSITY_DEFAULT and SEED_ACTIVE_DEFAULT (src/world/mod.rs:1980-1984)
+FUMAROLE_PIXEL_STRENGTH = 5000.0
+FUMAROLE_GAIN_DEFAULT = 0.3
+SEED_DENSITY_DEFAULT = 0.83
+SEED_ACTIVE_DEFAULT = 1.0
 
 
 def force_words(f: np.ndarray) -> np.ndarray:
@@ -309,6 +315,14 @@ def lib() -> ctypes.CDLL:
                                         ctypes.POINTER(ctypes.c_uint64)]),
         "ribo_world_solver_residuals": (I, [P, P, I]),
         "ribo_ledger_noalloc": (I, [P, ctypes.c_uint64, P, P]),
+        "ribo_rasterize_fumaroles": (I, [P, P, U32, P]),
+        "ribo_vent_heat_src": (I, [P, P, P, ctypes.c_float, P]),
+        "ribo_world_set_vents": (I, [P, P, U32, ctypes.c_float, P, P]),
+        "ribo_seed_cells": (I, [P, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, P]),
+        "ribo_fit_matter_to_terrain": (I, [P, P, ctypes.c_uint64, ctypes.c_uint64]),
+        "ribo_refit_matter_to_terrain": (I, [P, P, ctypes.c_uint64]),
+        "ribo_world_restart_matter": (I, [P, ctypes.c_uint64, ctypes.c_float, ctypes.c_float, P]),
+        "ribo_world_apply_terrain_live": (I, [P, P]),
         "ribo_last_error": (ctypes.c_char_p, []),
         "ribo_abi_version": (U32, []),
         "ribo_gen_gamma": (I, [P, ctypes.c_uint64, ctypes.c_float, I, P]),
@@ -571,6 +585,24 @@ class World:
         return {"last": r.last, "last_rounded": r.last_rounded, "max": r.max, "max_rounded": r.max_rounded,
                 "restarts": int(r.restarts)}
 
+    def set_vents(self, fumaroles, gain: float = FUMAROLE_GAIN_DEFAULT, heat_image=None, chem_image=None):
+        """World::upload_heat: fumaroles and images -> RIBO_BUF_HEAT_SRC."""
+        arr, n = _fumaroles(fumaroles)
+        hi = None if heat_image is None else np.ascontiguousarray(heat_image, dtype=np.float32)
+        ci = None if chem_image is None else np.ascontiguousarray(chem_image, dtype=np.float32)
+        _check(lib().ribo_world_set_vents(self._h, arr, n, gain, None if hi is None else _ptr(hi),
+                                          None if ci is None else _ptr(ci)), "set_vents")
+
+    def restart_matter(self, seed: int, density: float = SEED_DENSITY_DEFAULT,
+                       active: float = SEED_ACTIVE_DEFAULT) -> "Ledger":
+        out = (ctypes.c_uint32 * 12)()
+        _check(lib().ribo_world_restart_matter(self._h, seed, density, active, out), "restart_matter")
+        return Ledger.from_words(out)
+
+    def apply_terrain_live(self, gamma: np.ndarray):
+        g = np.ascontiguousarray(gamma, dtype=np.uint32)
+        _check(lib().ribo_world_apply_terrain_live(self._h, _ptr(g)), "apply_terrain_live")
+
     def mg_stats(self) -> tuple:
         c = ctypes.c_uint32()
         r = ctypes.c_double()
@@ -585,6 +617,65 @@ def ledger_noalloc(chem_device_ptr: int, cells: int, stream: int = 0) -> Ledger:
     _check(lib().ribo_ledger_noalloc(ctypes.c_void_p(chem_device_ptr), cells, ctypes.c_void_p(stream), out),
            "ledger_noalloc")
     return Ledger.from_words(out)
+
+
+# ---- vents and matter seeding (the reference's host code, in the library) ----
+class Fumarole(ctypes.Structure):
+    """Fumarole (src/params.rs:795-810), 32 bytes."""
+
+    _fields_ = [("x_frac", ctypes.c_float), ("y_frac", ctypes.c_float), ("strength", ctypes.c_float),
+                ("spread", ctypes.c_float), ("enabled", ctypes.c_uint32), ("_pad0", ctypes.c_uint32),
+                ("_pad1", ctypes.c_uint32), ("_pad2", ctypes.c_uint32)]
+
+
+
+
+def _fumaroles(fs):
+    arr = (Fumarole * max(len(fs), 1))()
+    for i, f in enumerate(fs):
+        arr[i] = f if isinstance(f, Fumarole) else Fumarole(*f[:4], f[4] if len(f) > 4 else 1, 0, 0, 0)
+    return arr, len(fs)
+
+
+def rasterize_fumaroles(cfg: WorldConfig, fumaroles) -> np.ndarray:
+    """rasterize_fumaroles: heat f32[G]; fumaroles as Fumarole or (x_frac,
+    y_frac, strength, spread[, enabled]) tuples."""
+    arr, n = _fumaroles(fumaroles)
+    out = np.empty(cfg.cells(), np.float32)
+    c = cfg.c()
+    _check(lib().ribo_rasterize_fumaroles(ctypes.byref(c), arr, n, _ptr(out)), "rasterize_fumaroles")
+    return out
+
+
+def vent_heat_src(cfg: WorldConfig, heat: np.ndarray, chem: np.ndarray, gain: float) -> np.ndarray:
+    h = np.ascontiguousarray(heat, dtype=np.float32)
+    ch = np.ascontiguousarray(chem, dtype=np.float32)
+    out = np.empty(2 * cfg.fluid_cells(), np.float32)
+    c = cfg.c()
+    _check(lib().ribo_vent_heat_src(ctypes.byref(c), _ptr(h), _ptr(ch), gain, _ptr(out)), "vent_heat_src")
+    return out
+
+
+def seed_cells(cfg: WorldConfig, seed: int, density: float = SEED_DENSITY_DEFAULT,
+               active: float = SEED_ACTIVE_DEFAULT) -> np.ndarray:
+    out = np.empty(4 * cfg.cells(), np.uint32)
+    c = cfg.c()
+    _check(lib().ribo_seed_cells(ctypes.byref(c), seed, density, active, _ptr(out)), "seed_cells")
+    return out
+
+
+def fit_matter_to_terrain(cells: np.ndarray, gamma: np.ndarray, seed: int) -> np.ndarray:
+    out = np.array(cells, dtype=np.uint32, copy=True)
+    g = np.ascontiguousarray(gamma, dtype=np.uint32)
+    _check(lib().ribo_fit_matter_to_terrain(_ptr(out), _ptr(g), g.size, seed), "fit_matter_to_terrain")
+    return out
+
+
+def refit_matter_to_terrain(cells: np.ndarray, gamma: np.ndarray) -> np.ndarray:
+    out = np.array(cells, dtype=np.uint32, copy=True)
+    g = np.ascontiguousarray(gamma, dtype=np.uint32)
+    _check(lib().ribo_refit_matter_to_terrain(_ptr(out), _ptr(g), g.size), "refit_matter_to_terrain")
+    return out
 
 
 # ---- synthetic inputs (host generators in the library) ----
@@ -770,10 +861,11 @@ def scene_write(path: str, cfg: WorldConfig, params: SimParams, settings: WorldS
 def scene_read_block(path: str, name: str) -> bytes:
     """ribo_scene_read_block: one block of a scene file, decompressed."""
     n = ctypes.c_uint64()
-    rc = lib().ribo_scene_read_block(os.fsencode(path), name.encode(), None, 0, ctypes.byref(n))
-    if rc != 0 and n.value == 0:
-        _check(rc, f"scene_read_block {name}")
-    buf = np.empty(max(int(n.value), 1), dtype=np.uint8)
+    _check(lib().ribo_scene_read_block(os.fsencode(path), name.encode(), None, 0, ctypes.byref(n)),
+           f"scene_read_block {name}")
+    if n.value == 0:
+        return b""
+    buf = np.empty(int(n.value), dtype=np.uint8)
     _check(lib().ribo_scene_read_block(os.fsencode(path), name.encode(), buf.ctypes.data, int(n.value), ctypes.byref(n)),
            f"scene_read_block {name}")
     return buf[:n.value].tobytes()

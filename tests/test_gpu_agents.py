@@ -139,7 +139,8 @@ def test_coupling_points_reject_bad_input():
     cfg, inp, gw, _ = exchange_world(35, ledger=False)
     before = gw.download(ra.CHEM)
     assert gw.chem_ops(np.zeros((0, 4), np.uint32)).size == 0
-    for bad in ([(cfg.cells(), 0, ra.CHEM_ADD_ACT, 1)], [(0, 4, ra.CHEM_ADD_ACT, 1)], [(0, 0, 8, 1)]):
+    for bad in ([(cfg.cells(), 0, ra.CHEM_ADD_ACT, 1)], [(0, 4, ra.CHEM_ADD_ACT, 1)], [(0, 0, 8, 1)],
+                [(0, 0, ra.CHEM_ADD_SPENT, 65536)], [(0, 0, ra.CHEM_DEPOSIT_ACT, 70000)]):
         with pytest.raises(ra.RiboError):
             gw.chem_ops([(5, 1, ra.CHEM_ADD_ACT, 3)] + bad)
     assert np.array_equal(gw.download(ra.CHEM), before)

@@ -229,15 +229,18 @@ def test_agent_blocks_pass_through(tmp_path):
     g = grids(cfg, 8)
     p, s = ra.SimParams.default(3), ra.WorldSettings.default()
     rng = np.random.default_rng(8)
-    agents = rng.integers(0, 1 << 32, 40 * 17, dtype=np.uint64).astype(np.uint32).tobytes()
+    agents = rng.integers(0, 1 << 32, 40 * 16, dtype=np.uint64).astype(np.uint32).tobytes()  # 64-byte Agents
+    slots = np.arange(40, dtype=np.uint32).tobytes()
     genomes = bytes(rng.integers(0, 256, 5000, dtype=np.uint8))
     counters = np.arange(8, dtype=np.uint32).tobytes()
     path = str(tmp_path / "vida.ribo")
     ra.scene_write(path, cfg, p, s, g, agents=40,
-                   blocks={"agents": agents, "genomes": genomes, "life_counters": counters, "pair_mask": b""})
+                   blocks={"agents": agents, "slots": slots, "genomes": genomes, "life_counters": counters,
+                           "pair_mask": b""})
     header, blocks = parse_file(path)
     assert header["agentes"] == 40 and ra.scene_agents(path) == 40
     assert blocks["agents"] == agents and blocks["genomes"] == genomes and blocks["life_counters"] == counters
+    assert blocks["slots"] == slots
     for name in set(AGENT_BLOCKS) - {"agents", "genomes"}:
         assert blocks[name] == b"", name
     assert ra.scene_read_block(path, "genomes") == genomes
@@ -253,3 +256,48 @@ def test_agent_blocks_pass_through(tmp_path):
             ra.scene_write(str(tmp_path / "bad.ribo"), cfg, p, s, g, agents=1, blocks=bad)
     with pytest.raises(ra.RiboError):
         ra.scene_write(str(tmp_path / "bad.ribo"), cfg, p, s, g, agents=101)  # more than max_agents
+
+
+def test_agent_counts_must_match_the_blocks(tmp_path):
+    """The writer refuses what the reference's load_scene refuses
+    (src/world/snapshot.rs:403-405): a count without matching slot and Agent
+    blocks, Agents of the wrong size, or life counters other than 8 words."""
+    cfg = ra.WorldConfig(64, 32, 30, 100)
+    g = grids(cfg, 9)
+    p, s = ra.SimParams.default(3), ra.WorldSettings.default()
+    ag = bytes(64 * 3)
+    sl = bytes(4 * 3)
+    path = str(tmp_path / "x.ribo")
+    ra.scene_write(path, cfg, p, s, g, agents=3, blocks={"agents": ag, "slots": sl})  # consistent
+    for agents, blocks in ((3, {"agents": ag}),                                # no slots
+                           (3, {"slots": sl}),                                 # no Agents
+                           (3, {"agents": bytes(68 * 3), "slots": sl}),        # 68-byte Agents
+                           (2, {"agents": ag, "slots": sl}),                   # count differs
+                           (0, {"life_counters": bytes(28)})):                 # 7 counters
+        with pytest.raises(ra.RiboError):
+            ra.scene_write(str(tmp_path / "bad.ribo"), cfg, p, s, g, agents=agents, blocks=blocks)
+
+
+def test_crafted_block_size_is_refused_without_allocating(tmp_path):
+    """A block whose prepended size claims far more than its compressed bytes
+    can expand to (here 4 GiB - 16 from a few bytes) is a malformed file: an
+    error, not an allocation of that size."""
+    cfg = ra.WorldConfig(64, 32, 30, 0)
+    path = str(tmp_path / "w.ribo")
+    ra.scene_write(path, cfg, ra.SimParams.default(1), ra.WorldSettings.default(), grids(cfg, 10))
+    raw = bytearray(open(path, "rb").read())
+    hl = int.from_bytes(raw[8:12], "little")
+    p = 12 + hl
+    while p < len(raw):  # find the "slots" block (empty: 4 + a few bytes)
+        nl = int.from_bytes(raw[p:p + 2], "little")
+        name = raw[p + 2:p + 2 + nl].decode()
+        cl = int.from_bytes(raw[p + 2 + nl:p + 10 + nl], "little")
+        if name == "slots":
+            raw[p + 10 + nl:p + 14 + nl] = (0xFFFFFFF0).to_bytes(4, "little")
+            break
+        p += 10 + nl + cl
+    bad = str(tmp_path / "bad.ribo")
+    open(bad, "wb").write(bytes(raw))
+    with pytest.raises(ra.RiboError):
+        ra.scene_read_block(bad, "slots")
+    assert ra.scene_read_block(path, "slots") == b""

@@ -392,6 +392,47 @@ def agents_measure(cfg, inputs, args):
     return out
 
 
+def default_profile_measure(cfg, inputs, args):
+    """The reference's default profile on the same 2048^2 world:
+    SimParams::default() and WorldSettings::default() (fluid every second
+    step, transport every third, one V-cycle), with the terrain moving, the
+    light step (one propagation pass per step) under the day cycle's moving
+    sun, solar heating and the vent cycle; the steps replay as HIP graphs,
+    the sun of each epoch read from the device table.  The timed window
+    starts in the day (epoch 2000)."""
+    gamma, vel, chem, heat = inputs
+    w = ra.World(cfg, SEED)
+    p = ra.SimParams.default(SEED)
+    p.epoch = 2000
+    w.params = p
+    s = ra.WorldSettings.default()
+    s.terrain_enabled = 1
+    s.light_mode = 1
+    s.ledger_check = 1
+    w.settings = s
+    for which, a in ((ra.GAMMA, gamma), (ra.VELOCITY, vel), (ra.CHEM, chem), (ra.HEAT_SRC, heat)):
+        w.upload(which, a)
+    seeded = ra.Ledger.from_cells(chem)
+    w.run(args.warmup)
+    for n in sorted({64 if args.default_steps >= 64 else 0, args.default_steps % 64} - {0}):
+        w.encode_steps(n)
+        w.encode_steps(n)
+    w.sync()
+    t0 = time.perf_counter()
+    w.run(args.default_steps)
+    w.sync()
+    dt = time.perf_counter() - t0
+    bad = w.ledger_status()
+    ok = bad is None and w.ledger_blocking().channels() == seeded.channels()
+    w.close()
+    if not ok:
+        raise SystemExit(f"default profile: per-channel monomer totals changed (epoch {bad})")
+    n = args.default_steps
+    return {"steps": n, "ms_per_step": dt / n * 1000.0, "cell_steps_per_s": cfg.cells() * n / dt,
+            "profile": "SimParams::default(), WorldSettings::default() + terrain moving + light step; "
+                       "fluid every 2nd step, transport every 3rd, 1 V-cycle, day cycle from epoch 2000"}
+
+
 def cpu_baseline(cfg, inputs, args):
     """The CPU oracle (C++ restatement of the reference WGSL, OpenMP over all
     threads it is given) on a bounded sample of the same workload: the same
@@ -656,6 +697,8 @@ def main():
     ap.add_argument("--no-agents", action="store_true", help="skip the agents' coupling-point timing")
     ap.add_argument("--terrain-steps", type=int, default=64,
                     help="steps of the moving-terrain measurement (0 skips it)")
+    ap.add_argument("--default-steps", type=int, default=256,
+                    help="steps of the reference's default profile (0 skips it)")
     ap.add_argument("--vcycle-steps", type=int, default=100,
                     help="steps of the stationary V-cycle mode (the plain FP32 path to 1e-5); 0 skips it")
     ap.add_argument("--vcycle-max-cycles", type=int, default=400, help="V-cycle cap per solve of the stationary mode")
@@ -767,6 +810,7 @@ def main():
     light = None if args.no_light else light_measure(cfg, inputs, args)
     terrain = terrain_measure(cfg, inputs, args) if args.terrain_steps > 0 else None
     agents = None if args.no_agents else agents_measure(cfg, inputs, args)
+    default_profile = default_profile_measure(cfg, inputs, args) if args.default_steps > 0 else None
     plain = {}
     if args.vcycle_steps > 0:
         # the stationary V-cycle, on a bounded number of steps (each step runs
@@ -830,6 +874,7 @@ def main():
         "light": light,
         "terrain": terrain,
         "agents": agents,
+        "reference_default_profile": default_profile,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,
     }

@@ -15,6 +15,17 @@ enum : uint8_t {
     CF_NORMAL = 2,  // solid_normal_from_neighbors is non-zero
 };
 
+// The sun of each step over a window of epochs [base, base + SUN_TAB_LEN):
+// (sun_now, sun_slope) as the host computes them in binary64 for every
+// epoch (src/params.rs:505-537, src/world/mod.rs:1439-1441).  Kernels that
+// read the sun take it from here when a day cycle makes it change from step
+// to step, so that captured steps replay with the sun of their epoch.
+constexpr uint32_t SUN_TAB_LEN = 8192;
+struct SunTab {
+    uint32_t base, pad;
+    float2 v[SUN_TAB_LEN];
+};
+
 // Terrain-derived, per-world constant data (rebuilt when gamma changes) and
 // the sizes every kernel needs.  Passed by value.
 struct Geo {
@@ -29,6 +40,7 @@ struct Geo {
     const float* fill;        // gamma_fill_at_fluid_cell per fluid cell
     const float* dragg;       // rubble grain fraction g of rubble_drag per fluid cell
     const float2* heat;       // heat_src, f32x2[F]
+    const SunTab* sun;        // per-epoch sun (null: the sun in the step's parameters)
 };
 
 // bfloat16 storage word (round to nearest even on the high 16 bits of the
@@ -206,9 +218,10 @@ void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
 // Transmission of every light cell into `light` (the sweep's pre-pass).
 void launch_light_transmit(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, float* light);
 // The full top-to-bottom sweep, in place over the transmissions.
-void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, float* light);
+void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, float* light);
 // One propagation pass: lout from the rows above in lin (distinct buffers).
-void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
+void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const uint32_t* chem,
+                            const float* lin,
                             float* lout);
 
 // exchanges between the grid and the agents: n RiboChemOp records in device

@@ -298,6 +298,15 @@ __global__ __launch_bounds__(256) void k_mask_u32(Geo g, uint32_t* out) {
     out[i] = (g.cflags[i] & CF_SOLID) ? 1u : 0u;
 }
 
+// The step's sun from the per-epoch table, when the world keeps one.
+__device__ __forceinline__ void sun_of_step(const Geo& g, RiboSimParams& P, const uint32_t* ep) {
+    if (!g.sun) return;
+    const uint32_t e = P.epoch + (ep ? *ep : 0u);
+    const float2 v = g.sun->v[e - g.sun->base];
+    P.sun_now = v.x;
+    P.sun_slope = v.y;
+}
+
 // ---- light on the frozen light field (light.wgsl:46-102,182-189) -------------------
 __device__ float light_above(const Geo& g, const RiboSimParams& P, uint32_t x, uint32_t row, uint32_t band) {
     const uint32_t LS = g.Lsz, LC = LS * LS;
@@ -413,8 +422,9 @@ __global__ __launch_bounds__(256) void k_light_transmit(Geo g, RiboSimParams P, 
 // transmissions while it works on the current row, so a row costs LDS reads
 // and a barrier rather than an HBM round trip.  Rows below the current one
 // still hold the pre-pass values when they are prefetched.
-template <int CPT> __global__ __launch_bounds__(1024) void k_light_sweep(Geo g, RiboSimParams P, float* light) {
+template <int CPT> __global__ __launch_bounds__(1024) void k_light_sweep(Geo g, RiboSimParams P, const uint32_t* ep, float* light) {
     extern __shared__ float rows[];
+    sun_of_step(g, P, ep);
     const uint32_t LS = g.Lsz, LC = LS * LS, band = blockIdx.x, tid = threadIdx.x;
     float* L = light + band * LC;
     const float rw = row_water(P, g.G, band);
@@ -463,11 +473,12 @@ template <int CPT> __global__ __launch_bounds__(1024) void k_light_sweep(Geo g, 
 // light_propagate (light.wgsl:162-176): every light cell at once takes the
 // row above from the previous field; light_commit's copy is a ping-pong of
 // the two fields.
-__global__ __launch_bounds__(256) void k_light_propagate(Geo g, RiboSimParams P, const uint32_t* chem, const float* lin,
-                                                         float* lout) {
+__global__ __launch_bounds__(256) void k_light_propagate(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* chem,
+                                                         const float* lin, float* lout) {
     const uint32_t LS = g.Lsz, LC = LS * LS;
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= LC) return;
+    sun_of_step(g, P, ep);
     const uint32_t x = i % LS, y = i / LS;
     float t0, t1;
     light_transmit2(g, P, chem, x, y, &t0, &t1);
@@ -505,6 +516,7 @@ __global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, con
                                                      float* __restrict__ t_out, float* __restrict__ r_out,
                                                      uint32_t* eaten, int eaten_active, const uint32_t* chem) {
     FLUID_XY
+    sun_of_step(g, P, ep);
     if (ep) P.epoch += *ep;
     const float dt = fluid_dt(P);
     const float2 v = raw_vel(g, vel, idx);
@@ -3367,18 +3379,18 @@ void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
 void launch_light_transmit(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, float* light) {
     k_light_transmit<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, chem, light);
 }
-void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, float* light) {
+void launch_light_sweep(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, float* light) {
     const size_t lds = 2 * (size_t)g.Lsz * sizeof(float);  // grid <= 32768: Lsz <= 16384, 128 KB
     const uint32_t cpt = (g.Lsz + 1023u) / 1024u;
-    if (cpt <= 1) k_light_sweep<1><<<2, 1024, lds, s>>>(g, P, light);
-    else if (cpt <= 2) k_light_sweep<2><<<2, 1024, lds, s>>>(g, P, light);
-    else if (cpt <= 4) k_light_sweep<4><<<2, 1024, lds, s>>>(g, P, light);
-    else if (cpt <= 8) k_light_sweep<8><<<2, 1024, lds, s>>>(g, P, light);
-    else k_light_sweep<16><<<2, 1024, lds, s>>>(g, P, light);
+    if (cpt <= 1) k_light_sweep<1><<<2, 1024, lds, s>>>(g, P, ep, light);
+    else if (cpt <= 2) k_light_sweep<2><<<2, 1024, lds, s>>>(g, P, ep, light);
+    else if (cpt <= 4) k_light_sweep<4><<<2, 1024, lds, s>>>(g, P, ep, light);
+    else if (cpt <= 8) k_light_sweep<8><<<2, 1024, lds, s>>>(g, P, ep, light);
+    else k_light_sweep<16><<<2, 1024, lds, s>>>(g, P, ep, light);
 }
-void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* chem, const float* lin,
-                            float* lout) {
-    k_light_propagate<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, chem, lin, lout);
+void launch_light_propagate(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const uint32_t* chem,
+                            const float* lin, float* lout) {
+    k_light_propagate<<<blocks1((size_t)g.Lsz * g.Lsz, 256), 256, 0, s>>>(g, P, ep, chem, lin, lout);
 }
 void launch_chem_ops(hipStream_t s, const Geo& g, uint32_t* chem, const uint32_t* ops, uint32_t n, uint32_t* done,
                      uint32_t* expected) {

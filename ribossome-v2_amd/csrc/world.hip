@@ -180,6 +180,11 @@ struct RiboWorld {
     // per (schedule phase, step count, ping-pong parities)
     uint32_t* epoch_dev = nullptr;
     uint32_t epoch_dev_value = 0;
+    // the sun of each epoch over a window, when a day cycle moves it
+    SunTab* sun_tab = nullptr;
+    std::vector<SunTab> sun_host;  // one entry: the last window written
+    bool sun_active = false, sun_valid = false;
+    float sun_key[4] = {0, 0, 0, 0};
     bool epoch_dev_known = false;
     struct GraphEntry { uint32_t phase, n, slots; int cur, tcur, lcur, cur_end, tcur_end, lcur_end; hipGraphExec_t exec; };
     std::vector<GraphEntry> graphs;
@@ -198,6 +203,7 @@ struct RiboWorld {
         g.G = G; g.F = F; g.Lsz = Lsz; g.wupc = cfg.world_units_per_cell; g.sim_size = sim_size;
         g.gamma = gamma; g.light = light[lcur]; g.slope = slope; g.slope_f = slope_f; g.cflags = cflags;
         g.normal = normal; g.fill = fill; g.dragg = dragg; g.heat = heat;
+        g.sun = sun_active ? sun_tab : nullptr;
         return g;
     }
 };
@@ -656,7 +662,7 @@ void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) { 
 // otherwise light_rows_per_step (at most 16) propagation passes, each a
 // ping-pong of the two fields instead of light_commit's copy.  Agent shade
 // (clear_shade, agents_shade) is zero: there are no agents.
-void light_step(RiboWorld* w, const RiboSimParams& P, uint32_t epoch) {
+void light_step(RiboWorld* w, const RiboSimParams& P, uint32_t epoch, const uint32_t* ep) {
     if (!w->st.light_mode) return;
     hipStream_t s = w->stream;
     const bool sweep = w->light_dirty || (w->st.light_rows_per_step == 0 && epoch % std::max(w->st.light_interval, 1u) == 0);
@@ -664,13 +670,13 @@ void light_step(RiboWorld* w, const RiboSimParams& P, uint32_t epoch) {
         w->light_dirty = false;
         Geo g = w->geo();
         run(w, "light_transmit", [&] { launch_light_transmit(s, g, P, w->chem[w->cur], w->light[w->lcur]); });
-        run(w, "light_sweep", [&] { launch_light_sweep(s, g, P, w->light[w->lcur]); });
+        run(w, "light_sweep", [&] { launch_light_sweep(s, g, P, ep, w->light[w->lcur]); });
         return;
     }
     for (uint32_t k = 0; k < std::min(w->st.light_rows_per_step, 16u); ++k) {
         Geo g = w->geo();
         run(w, "light_propagate",
-            [&] { launch_light_propagate(s, g, P, w->chem[w->cur], w->light[w->lcur], w->light[1 - w->lcur]); });
+            [&] { launch_light_propagate(s, g, P, ep, w->chem[w->cur], w->light[w->lcur], w->light[1 - w->lcur]); });
         w->lcur = 1 - w->lcur;
     }
 }
@@ -686,21 +692,47 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
         P.sun_now = w->params.uv_strength > 0.0f ? day_light(w->params, epoch) : 0.0f;
         P.sun_slope = sun_slope_at(w->params, epoch);
         if (w->st.terrain_enabled) terrain_step(w, P, ep);
-        light_step(w, P, epoch);
+        light_step(w, P, epoch, ep);
         if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
         if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep);
     }
 }
 
 // Graph replay is used when nothing in the recorded work depends on host
-// values that change from step to step: the light-dependent solar heating
-// uses the host-computed sun of each step, and a pending redox_eaten or
-// force_vectors upload is consumed once.
+// values that change from step to step: the sun of a day cycle comes from
+// the per-epoch table on the device, but a stale light's sweep is a
+// one-off, and a pending redox_eaten or force_vectors upload is consumed
+// once.
 bool graph_eligible(const RiboWorld* w) {
-    // the light step reads the sun of each step (constant without a day
-    // cycle), and a stale light's sweep is a one-off
-    const bool light_varies = w->st.light_mode && (w->light_dirty || w->params.day_period >= 1.0f);
-    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !w->agent_force_active && !(w->params.sun_heat > 0.0f) && !light_varies;
+    return w->st.no_graphs == 0 && !w->timing && !w->eaten_active && !w->agent_force_active &&
+           !(w->st.light_mode && w->light_dirty);
+}
+
+// The sun table (SunTab) holds record_steps' (sun_now, sun_slope) of every
+// epoch of a window; it is active while a day cycle moves the sun
+// (day_period >= 1), and rewritten (after draining the stream) when the
+// steps about to be recorded leave the window or the sun's parameters change.
+int ensure_sun(RiboWorld* w, uint32_t steps) {
+    const RiboSimParams& p = w->params;
+    w->sun_active = p.day_period >= 1.0f;
+    if (!w->sun_active) return RIBO_OK;
+    const float key[4] = {p.day_period, p.day_fraction, p.sun_angle, p.uv_strength > 0.0f ? 1.0f : 0.0f};
+    const uint32_t base = w->sun_valid ? w->sun_host[0].base : 0u;
+    const bool inside = w->sun_valid && p.epoch >= base && (uint64_t)p.epoch + steps <= (uint64_t)base + SUN_TAB_LEN;
+    if (inside && memcmp(key, w->sun_key, sizeof key) == 0) return RIBO_OK;
+    HIPCHK(hipStreamSynchronize(w->stream));  // no recorded step still reads the old window
+    SunTab& t = w->sun_host[0];
+    t.base = p.epoch;
+    t.pad = 0u;
+    for (uint32_t i = 0; i < SUN_TAB_LEN; ++i) {
+        const uint32_t e = p.epoch + i;
+        t.v[i] = make_float2(p.uv_strength > 0.0f ? day_light(p, e) : 0.0f, sun_slope_at(p, e));
+    }
+    HIPCHK(hipMemcpyAsync(w->sun_tab, &t, sizeof(SunTab), hipMemcpyHostToDevice, w->stream));
+    HIPCHK(hipStreamSynchronize(w->stream));
+    memcpy(w->sun_key, key, sizeof key);
+    w->sun_valid = true;
+    return RIBO_OK;
 }
 
 // Arms the per-step ledger check with the totals of the current grid (when
@@ -808,7 +840,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     b += a(2 * F, 4) + 2 * a(G, 1);  // the agents' force words, the grain move codes and masks
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
     b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 8 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
-    b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4);  // ledger, scratch, device epoch
+    b += a(1, sizeof(LedgerDev)) + a(F, 4) + a(1, 4) + a(1, sizeof(SunTab));  // ledger, scratch, device epoch, sun table
     const uint32_t min_size = st ? std::max(st->mg_min_size, 1u) : 8u;
     const std::vector<RiboMgLevel> lv = make_levels(cfg->fluid_size, min_size);
     const size_t total = lv.back().off_c, coarse = total - F;
@@ -843,6 +875,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         return fail(RIBO_ERR_NO_DEVICE, std::string("this library is built for gfx950; device is ") + prop.gcnArchName);
 
     RiboWorld* w = new RiboWorld();
+    w->sun_host.resize(1);
     w->cfg = *cfg;
     w->G = cfg->grid_size;
     w->F = cfg->fluid_size;
@@ -879,6 +912,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
         (rc = dalloc(w->stream, &w->pcg_xlo, F)) ||
         (rc = dalloc(w->stream, &w->pcg_r16, F)) || (rc = dalloc(w->stream, &w->pcg_z16, F)) ||
         (rc = dalloc(w->stream, &w->ledger, 1)) || (rc = dalloc(w->stream, &w->scratch_u32, F)) || (rc = dalloc(w->stream, &w->epoch_dev, 1)) ||
+        (rc = dalloc(w->stream, &w->sun_tab, 1)) ||
         (rc = build_mg(w))) {
         ribo_world_destroy(w);
         return rc;
@@ -916,6 +950,7 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w); dfree(w->pcg_xlo);
     dfree(w->pcg_z); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
     dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
+    dfree(w->sun_tab);
     clear_graphs(w);
     if (w->pcg_host) (void)hipHostFree(w->pcg_host);
     if (w->stream) (void)hipStreamDestroy(w->stream);
@@ -1102,6 +1137,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
         }
         if ((rc = choose_slots(w))) return rc;
     }
+    if ((rc = ensure_sun(w, steps))) return rc;
     const RiboWorldSettings st = w->st;
     const uint32_t substep = std::min(std::max(st.fluid_substep, 1u), 4u);
     if (!st.fluid_enabled && w->fluid_was_on) {

@@ -52,6 +52,14 @@ CASES = {
     "light_graph_sweeps": dict(cfg=ra.WorldConfig(64, 32), seed=19, inputs=dict(rock=0.15, rubble=True),
                                params=dict(uv_strength=3.0, uv_depth=4.0, day_period=0.0, direct_photoactivation=0.5),
                                settings=dict(light_mode=1, light_rows_per_step=0, light_interval=3)),
+    # the reference's default profile (SimParams::default(), WorldSettings::
+    # default()): fluid every second step, transport every third, the light
+    # step under the day cycle's moving sun with solar heating, the vent
+    # cycle, and the terrain moving -- replayed as graphs with the sun of
+    # each epoch from the device table
+    "reference_default": dict(cfg=ra.WorldConfig(128, 64), seed=21, inputs=dict(rock=0.12, rubble=True, vent_rows=4),
+                              params=dict(transport_every=3, uv_strength=3.0, sun_heat=1.0, epoch=2000),
+                              settings=dict(fluid_substep=2, terrain_enabled=1, light_mode=1)),
 }
 
 
@@ -310,6 +318,30 @@ def test_graph_replays_match_eager_recording(tol):
         for which in (ra.VELOCITY, ra.PRESSURE, ra.TEMP):
             assert bit_diff(worlds[0].download(which), worlds[1].download(which)) == 0, ra.BUFFER_NAMES[which]
     assert worlds[0].params.epoch == worlds[1].params.epoch == 158
+    assert worlds[0].ledger_status() is None and worlds[1].ledger_status() is None
+
+
+def test_reference_default_profile_graphs_match_eager():
+    """The reference's default profile (day cycle, solar heating, light
+    propagation, moving terrain, substep 2, transport every 3) replays as
+    graphs, the moving sun read per epoch from the device table, with the
+    same bits as eager recording over chunks that cross day and night."""
+    print("case reference default graphs", flush=True)
+    c = CASES["reference_default"]
+    cfg = c["cfg"]
+    inp = make_inputs(cfg, c["seed"], **c["inputs"])
+    worlds = []
+    for no_graphs in (1, 0):
+        w = ra.World(cfg, c["seed"])
+        load(w, inp, parity_params(c["seed"], **dict(c["params"], epoch=5900)),
+             parity_settings(no_graphs=no_graphs, ledger_check=1, **c["settings"]))
+        worlds.append(w)
+    for chunk in (5, 17, 64, 64, 64, 3):  # epochs 5900..6117: sunset of the 8000-epoch day at 6000
+        for w in worlds:
+            assert w.encode_steps(chunk) == chunk
+        for which in (ra.CHEM, ra.GAMMA, ra.VELOCITY, ra.TEMP, ra.REDOX, ra.LIGHT):
+            nd = bit_diff(worlds[0].download(which), worlds[1].download(which))
+            assert nd == 0, f"chunk {chunk}: {ra.BUFFER_NAMES[which]} differs in {nd}"
     assert worlds[0].ledger_status() is None and worlds[1].ledger_status() is None
 
 

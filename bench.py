@@ -542,10 +542,12 @@ def step_model_bytes(cfg, sizes, mode, res):
 
 
 def batched(args):
-    """Independent worlds stepped together, one stream each (the batched
-    configuration): sides from {128, ..., 1024} grid cells, rock U(5, 20) %,
-    monomers Poisson(U(2, 8)), the reference's one V-cycle per step, the
-    ledger checked every step.  Returns the JSON record."""
+    """Independent worlds stepped together (the batched configuration):
+    sides from {128, ..., 1024} grid cells, rock U(5, 20) %, monomers
+    Poisson(U(2, 8)), the reference's one V-cycle per step, the ledger
+    checked every step.  Timed twice over the same worlds: one launch per
+    kernel for the whole batch (ribo_batch_*, the value), and one stream per
+    world replaying its own graphs (per_stream).  Returns the JSON record."""
     sides = [128, 192, 256, 384, 512, 768, 1024]
     worlds = []
     for i in range(args.worlds):
@@ -565,32 +567,59 @@ def batched(args):
         w.settings = s
         w.autotune()
         worlds.append((w, cfg, ra.Ledger.from_cells(chem)))
-    for w, _, _ in worlds:  # warm-up: every graph the timed loop replays
-        w.encode_steps(64)
-        w.encode_steps(args.batch_steps % 64 or 64)
-    for w, _, _ in worlds:
-        w.sync()
-    left = [args.batch_steps] * len(worlds)
-    t0 = time.perf_counter()
-    while any(left):
-        for i, (w, _, _) in enumerate(worlds):
-            if left[i]:
-                left[i] -= w.encode_steps(min(64, left[i]))
-    for w, _, _ in worlds:
-        w.sync()
-    dt = time.perf_counter() - t0
+    steps = args.batch_steps
+    cells = sum(c.cells() for _, c, _ in worlds)
+
+    def per_stream():
+        for w, _, _ in worlds:  # warm-up: every graph the timed loop replays
+            w.encode_steps(64)
+            w.encode_steps(steps % 64 or 64)
+        for w, _, _ in worlds:
+            w.sync()
+        left = [steps] * len(worlds)
+        t0 = time.perf_counter()
+        while any(left):
+            for i, (w, _, _) in enumerate(worlds):
+                if left[i]:
+                    left[i] -= w.encode_steps(min(64, left[i]))
+        for w, _, _ in worlds:
+            w.sync()
+        return time.perf_counter() - t0
+
+    def one_launch():
+        b = ra.Batch([w for w, _, _ in worlds])
+        b.encode_steps(8)  # warm-up
+        b.sync()
+        n0 = b.launches()
+        t0 = time.perf_counter()
+        b.run(steps)
+        b.sync()
+        dt = time.perf_counter() - t0
+        launches = b.launches() - n0
+        b.close()
+        return dt, launches
+
+    dt_streams = per_stream()
+    print(f"batched: per-stream {cells * steps / dt_streams / 1e9:.3f} G cell-steps/s", file=sys.stderr, flush=True)
+    dt_batch, launches = one_launch()
+    print(f"batched: one launch per kernel {cells * steps / dt_batch / 1e9:.3f} G cell-steps/s", file=sys.stderr,
+          flush=True)
     for w, _, led in worlds:
         if w.ledger_status() is not None or w.ledger_blocking().channels() != led.channels():
             raise SystemExit("batched: a world's per-channel monomer totals changed")
-    cells = sum(c.cells() for _, c, _ in worlds)
     tiles = {c.grid_size: w.mg_tiles() for w, c, _ in worlds}
     for w, _, _ in worlds:
         w.close()
-    return {"metric": "batched world cell-steps/s", "value": cells * args.batch_steps / dt, "unit": "cell-steps/s",
-            "n_gpus": 1, "steps": args.batch_steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-            "dtype": "f32", "data": "synthetic", "ms_per_step": 1000.0 * dt / args.batch_steps,
+    return {"metric": "batched world cell-steps/s", "value": cells * steps / dt_batch, "unit": "cell-steps/s",
+            "n_gpus": 1, "steps": steps, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "f32", "data": "synthetic", "ms_per_step": 1000.0 * dt_batch / steps,
+            "per_stream": {"value": cells * steps / dt_streams, "ms_per_step": 1000.0 * dt_streams / steps,
+                           "streams": len(worlds)},
+            "one_launch_vs_per_stream": dt_streams / dt_batch,
+            "launches_per_step": launches / steps,
             "config": {"workload": f"{len(worlds)} independent worlds, grid sides from {sides}, reference solver "
-                                   "schedule (one V-cycle per step), ledger checked every step, multigrid tiles "
+                                   "schedule (one V-cycle per step), ledger checked every step; value: one launch "
+                                   "per kernel over every world; per_stream: one stream per world, multigrid tiles "
                                    "from the autotuner",
                        "worlds": len(worlds), "grid_cells": cells, "tiles_by_side": tiles}}
 

@@ -135,6 +135,63 @@ struct LedgerDev {
     uint32_t last[8];       // last completed reduction
 };
 
+// Arguments of the one-workgroup multigrid kernel (k_mg_block): every level
+// from the first that fits one workgroup's LDS down to the coarsest.
+constexpr int MG_BLOCK_LEVELS = 8;
+template <class S>
+struct MgBlockArgs {
+    int levels;
+    int n[MG_BLOCK_LEVELS];
+    float eps[MG_BLOCK_LEVELS];
+    const uint8_t* fluid[MG_BLOCK_LEVELS];
+    int off[MG_BLOCK_LEVELS + 1];  // cell offsets of each level in LDS
+    S* p0;
+    const S* rhs0;
+    const double* cinv;            // m x m inverse of the coarsest operator (SYM only)
+    const int32_t* ccell;          // unknown -> coarsest cell
+    int cm;                        // unknowns of the coarsest level (0: use sweeps)
+};
+// The arguments and dynamic LDS bytes of k_mg_block over `count` levels.
+template <class S>
+MgBlockArgs<S> mg_block_args(const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
+                             size_t* lds);
+
+// ---- many worlds in one launch ----
+// One workgroup of a batched launch: the world (index into the argument
+// table), the block index the kernel body sees, and that world's grid size
+// (gx | gy << 16).
+struct BatchBlock { uint32_t w, bx, by, gdim; };
+// Per-world arguments of the batched kernels (the single-world kernels'
+// parameters; the step's epoch offset comes with the launch).
+struct BTemperature {
+    Geo g; RiboSimParams P; const uint32_t* ep;
+    const float2* vel; const float* T; const float* Q; float* t_out; float* r_out; uint32_t* eaten; const uint32_t* chem;
+};
+struct BForces { Geo g; RiboSimParams P; const uint32_t* ep; const float* T; float2* forces; };
+// velocity kernels: add_forces (aux = forces), diffuse, advect (decay), vorticity
+struct BVel { Geo g; RiboSimParams P; float decay; const float2* aux; const float2* vin; float2* vout; };
+// divergence (f0 = div, f1 = p0), subtract_gradient (f0 = p), net_flux and
+// enforce_boundaries (f1 = net flux)
+struct BProj { Geo g; const float2* vin; float2* vout; float* f0; float* f1; };
+// thermal_activation, agg, the clear of chem_next, the scatter, the ledger
+struct BChem {
+    Geo g; RiboSimParams P; const uint32_t* ep;
+    uint32_t* chem; uint32_t* next; const float2* vel; const float* temp; uint32_t* agg_e;
+    LedgerDev* led; uint32_t ledger_blocks;
+};
+struct BMgPair { MgLevelDev L, C; };
+enum BatchKernel {
+    BK_TEMPERATURE, BK_FORCES, BK_ADD_FORCES, BK_DIFFUSE, BK_ADVECT, BK_VORTICITY, BK_DIVERGENCE, BK_SUBGRAD,
+    BK_NET_FLUX, BK_BOUNDARIES, BK_THERMAL, BK_AGG, BK_CLEAR_NEXT, BK_SCATTER, BK_LEDGER, BK_LEDGER_FIN,
+    BK_MG_DOWN, BK_MG_UP, BK_MG_BLOCK
+};
+// One batched launch of kernel k over `blocks` workgroups (map: device
+// BatchBlock[blocks], tab: device table of the kernel's argument type).
+void launch_batch(hipStream_t s, BatchKernel k, const void* tab, const BatchBlock* map, uint32_t blocks, uint32_t step,
+                  size_t lds = 0);
+// The grid of kernel k for one world (level_n: the multigrid level's side).
+void batch_grid(BatchKernel k, const Geo& g, uint32_t level_n, uint32_t* gx, uint32_t* gy);
+
 // ---- launchers (kernels.hip) ----
 // terrain-derived data
 void launch_slope(hipStream_t s, const Geo& g, float2* slope);
@@ -246,6 +303,8 @@ void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check);
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n);
+// *p = v on the stream (a value fixed at launch: no host buffer to outlive the call)
+void launch_set_u32(hipStream_t s, uint32_t* p, uint32_t v);
 // The free-matter totals of `cells` cells of a grid into led->last (no
 // check, no epoch).
 void launch_ledger_cells(hipStream_t s, const uint32_t* chem, uint32_t cells, LedgerDev* led);

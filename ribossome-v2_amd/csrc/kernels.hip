@@ -511,7 +511,7 @@ __device__ __forceinline__ float vent_cycle(const RiboSimParams& P, uint32_t x, 
 
 // update_temperature (fluid.wgsl:300-363); the copy back (copy_temperature,
 // :365-372) is a pointer swap on the host, and redox_eaten is cleared here.
-__global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, const uint32_t* ep, const float2* __restrict__ vel,
+__device__ __forceinline__ void b_temperature(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const float2* __restrict__ vel,
                                                      const float* __restrict__ T, const float* __restrict__ Q,
                                                      float* __restrict__ t_out, float* __restrict__ r_out,
                                                      uint32_t* eaten, int eaten_active, const uint32_t* chem) {
@@ -558,13 +558,19 @@ __global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, con
     r *= rb_expf(-rb_max(P.redox_decay, 0.0f) * dt);
     r_out[idx] = rb_clamp(r, 0.0f, REDOX_MAX);
 }
+__global__ __launch_bounds__(256) void k_temperature(Geo g, RiboSimParams P, const uint32_t* ep, const float2* __restrict__ vel,
+                                                     const float* __restrict__ T, const float* __restrict__ Q,
+                                                     float* __restrict__ t_out, float* __restrict__ r_out,
+                                                     uint32_t* eaten, int eaten_active, const uint32_t* chem) {
+    b_temperature(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, ep, vel, T, Q, t_out, r_out, eaten, eaten_active, chem);
+}
 
 // buoyancy (fluid.wgsl:376-389) into the fixed-point force words, then
 // gather_forces (:392-398) and clear_force_vectors (:290-296).  A word holds
 // the agents' fixed-point forces (agent, when non-null) plus the cell's own
 // buoyancy; the sum is an integer sum modulo 2^32, as the reference's atomic
 // adds, so the order of the adds does not matter.
-__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T,
+__device__ __forceinline__ void b_forces(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T,
                                                 float2* forces, uint32_t* agent) {
     FLUID_XY
     if (ep) P.epoch += *ep;
@@ -589,6 +595,10 @@ __global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const ui
     }
     const float2 f = f2((float)qx / FORCE_FP, (float)qy / FORCE_FP);
     forces[idx] = clamp_len(sanitize(f), MAX_FORCE);
+}
+__global__ __launch_bounds__(256) void k_forces(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T,
+                                                float2* forces, uint32_t* agent) {
+    b_forces(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, ep, T, forces, agent);
 }
 
 // smooth_velocity (fluid.wgsl:680-693): the mean of the cell and its four
@@ -622,7 +632,7 @@ __device__ __forceinline__ float2 slope_steer(const Geo& g, const RiboSimParams&
 }
 
 // add_forces (fluid.wgsl:410-432) with rubble_drag (:150-161)
-__global__ __launch_bounds__(256) void k_add_forces(Geo g, RiboSimParams P, const float2* __restrict__ forces,
+__device__ __forceinline__ void b_add_forces(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const float2* __restrict__ forces,
                                                     const float2* __restrict__ vin, float2* __restrict__ vout) {
     FLUID_XY
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
@@ -641,9 +651,13 @@ __global__ __launch_bounds__(256) void k_add_forces(Geo g, RiboSimParams P, cons
     v = reflect(g, idx, v);
     vout[idx] = clamp_len(v, MAX_VEL);
 }
+__global__ __launch_bounds__(256) void k_add_forces(Geo g, RiboSimParams P, const float2* __restrict__ forces,
+                                                    const float2* __restrict__ vin, float2* __restrict__ vout) {
+    b_add_forces(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, forces, vin, vout);
+}
 
 // diffuse_velocity (fluid.wgsl:435-456)
-__global__ __launch_bounds__(256) void k_diffuse(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
+__device__ __forceinline__ void b_diffuse(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
     FLUID_XY
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
     if (x == 0u || x == F - 1u || y == 0u || y == F - 1u) { vout[idx] = vin[idx]; return; }
@@ -656,6 +670,9 @@ __global__ __launch_bounds__(256) void k_diffuse(Geo g, RiboSimParams P, const f
     float2 v = sanitize(add2(v_c, mul2(lap, a)));
     v = reflect(g, idx, v);
     vout[idx] = clamp_len(v, MAX_VEL);
+}
+__global__ __launch_bounds__(256) void k_diffuse(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
+    b_diffuse(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, vin, vout);
 }
 
 // is_solid_at_pos and sample_velocity (fluid.wgsl:226-255)
@@ -680,7 +697,7 @@ __device__ __forceinline__ float2 sample_velocity(const Geo& g, const float2* vi
 
 // advect_velocity (fluid.wgsl:458-477); decay = pow(clamp(fluid_decay,0,1),
 // dt*60), uniform over the step, comes from the host.
-__global__ __launch_bounds__(256) void k_advect(Geo g, RiboSimParams P, float decay, const float2* __restrict__ vin,
+__device__ __forceinline__ void b_advect(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, float decay, const float2* __restrict__ vin,
                                                 float2* __restrict__ vout) {
     FLUID_XY
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
@@ -692,6 +709,10 @@ __global__ __launch_bounds__(256) void k_advect(Geo g, RiboSimParams P, float de
     float2 out = sanitize(mul2(sample_velocity(g, vin, trace), decay));
     out = reflect(g, idx, out);
     vout[idx] = clamp_len(out, MAX_VEL);
+}
+__global__ __launch_bounds__(256) void k_advect(Geo g, RiboSimParams P, float decay, const float2* __restrict__ vin,
+                                                float2* __restrict__ vout) {
+    b_advect(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, decay, vin, vout);
 }
 
 // curl_at (fluid.wgsl:479-489)
@@ -712,7 +733,7 @@ __device__ __forceinline__ float curl_at(const Geo& g, const float2* vin, uint32
 // cells, so the 16 x 16 tile stages those velocities with a 2-cell halo
 // (coordinates clamped into the grid, as curl_at clamps them), then the
 // curls with a 1-cell halo, in LDS; the arithmetic is curl_at's.
-__global__ __launch_bounds__(256) void k_vorticity(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
+__device__ __forceinline__ void b_vorticity(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
     __shared__ float2 rv[20][21];
     __shared__ float cl[18][19];
     const uint32_t F = g.F;
@@ -752,11 +773,14 @@ __global__ __launch_bounds__(256) void k_vorticity(Geo g, RiboSimParams P, const
     v = reflect(g, idx, v);
     vout[idx] = clamp_len(v, MAX_VEL);
 }
+__global__ __launch_bounds__(256) void k_vorticity(Geo g, RiboSimParams P, const float2* __restrict__ vin, float2* __restrict__ vout) {
+    b_vorticity(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, vin, vout);
+}
 
 // compute_divergence (fluid.wgsl:525-551).  Also zeroes the pressure of wall
 // cells, which is what mg_init (multigrid.wgsl:57) does to the level-0
 // multigrid unknowns that alias the pressure field.
-__global__ __launch_bounds__(256) void k_divergence(Geo g, const float2* __restrict__ vin, float* __restrict__ div, float* p0) {
+__device__ __forceinline__ void b_divergence(const uint3 blockIdx, const uint3 gridDim, Geo g, const float2* __restrict__ vin, float* __restrict__ div, float* p0) {
     FLUID_XY
     if (solid(g, idx)) {
         div[idx] = 0.0f;
@@ -775,6 +799,9 @@ __global__ __launch_bounds__(256) void k_divergence(Geo g, const float2* __restr
     const float2 v_l = nbv(xi - 1, yi, true), v_r = nbv(xi + 1, yi, true);
     const float2 v_b = nbv(xi, yi - 1, false), v_t = nbv(xi, yi + 1, false);
     div[idx] = 0.5f * ((v_r.x - v_l.x) + (v_t.y - v_b.y));
+}
+__global__ __launch_bounds__(256) void k_divergence(Geo g, const float2* __restrict__ vin, float* __restrict__ div, float* p0) {
+    b_divergence(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, vin, div, p0);
 }
 
 // jacobi_pressure (fluid.wgsl:553-608): Neumann by mirroring p_c.
@@ -1112,7 +1139,7 @@ __device__ __forceinline__ void bl_prolong_tile(MgTile<H, T>& t, const float* cp
 // Pre-smoothing (2 x red, black) and restriction of one level
 // (src/world/mod.rs:1556-1563; multigrid.wgsl:60-105).
 template <class S, bool SYM, int MG_T>
-__global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
+__device__ __forceinline__ void b_mg_down(const uint3 blockIdx, const uint3 gridDim, MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
     // the bilinear restriction reads residuals one ring beyond the core,
     // so its tiles keep exact values two cells out
@@ -1164,11 +1191,15 @@ __global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, c
         }
     }
 }
+template <class S, bool SYM, int MG_T>
+__global__ __launch_bounds__(256) void k_mg_down(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
+    b_mg_down<S, SYM, MG_T>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), L, C, done);
+}
 
 // Prolongation and post-smoothing (2 x red, black; black, red when SYM) of
 // one level (src/world/mod.rs:1569-1575).
 template <class S, bool SYM, int MG_T>
-__global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
+__device__ __forceinline__ void b_mg_up(const uint3 blockIdx, const uint3 gridDim, MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
     if (done && *done) return;
     constexpr int H = 4;
     __shared__ MgTile<H, MG_T> t;
@@ -1195,6 +1226,10 @@ __global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, con
         if (gx < n && gy < n && t.fl[H + k / MG_T][H + k % MG_T]) stv(L.p, gy * n + gx, t.p[H + k / MG_T][H + k % MG_T]);
     }
 }
+template <class S, bool SYM, int MG_T>
+__global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, const uint32_t* done) {
+    b_mg_up<S, SYM, MG_T>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), L, C, done);
+}
 
 // The coarse end of the V-cycle in one workgroup: every level of `lv` (at
 // most MG_BLOCK_LEVELS, small enough that all of them fit in LDS together)
@@ -1203,20 +1238,6 @@ __global__ __launch_bounds__(256) void k_mg_up(MgLevelT<S> L, MgLevelT<S> C, con
 // inverse of its operator -- then prolongation and post-smoothing back up;
 // the top level's unknowns are read from (the symmetric cycle starts them
 // from 0) and written back to HBM.
-constexpr int MG_BLOCK_LEVELS = 8;
-template <class S>
-struct MgBlockArgs {
-    int levels;
-    int n[MG_BLOCK_LEVELS];
-    float eps[MG_BLOCK_LEVELS];
-    const uint8_t* fluid[MG_BLOCK_LEVELS];
-    int off[MG_BLOCK_LEVELS + 1];  // cell offsets of each level in LDS
-    S* p0;
-    const S* rhs0;
-    const double* cinv;            // m x m inverse of the coarsest operator (SYM only)
-    const int32_t* ccell;          // unknown -> coarsest cell
-    int cm;                        // unknowns of the coarsest level (0: use sweeps)
-};
 
 // The workgroup is 32 x 32 threads; level loops stride rows by 32 and, in a
 // half-sweep, cells of one colour by 64 along the row, so no index needs a
@@ -1444,7 +1465,7 @@ __device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, floa
 }
 
 template <class S, bool SYM>
-__global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint32_t* done) {
+__device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gridDim, const MgBlockArgs<S>& a, const uint32_t* done) {
     if (done && *done) return;
     extern __shared__ float sm[];
     const Team<32, 32> blk((int)threadIdx.x);
@@ -1487,6 +1508,10 @@ __global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint3
             const int i = y * n0 + x;
             if (FL[i]) stv(a.p0, i, P[i]);
         }
+}
+template <class S, bool SYM>
+__global__ __launch_bounds__(1024) void k_mg_block(MgBlockArgs<S> a, const uint32_t* done) {
+    b_mg_block<S, SYM>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), a, done);
 }
 
 // ---- tolerance mode: preconditioned conjugate gradients ------------------------
@@ -2145,7 +2170,7 @@ __global__ __launch_bounds__(256) void k_vc_fin(PcgState* st, uint32_t nb, float
 }
 
 // subtract_gradient (fluid.wgsl:610-630)
-__global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
+__device__ __forceinline__ void b_subgrad(const uint3 blockIdx, const uint3 gridDim, Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
                                                  const float* __restrict__ P) {
     FLUID_XY
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
@@ -2163,6 +2188,10 @@ __global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict
     v = reflect(g, idx, v);
     vout[idx] = clamp_len(sanitize(v), MAX_VEL);
 }
+__global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
+                                                 const float* __restrict__ P) {
+    b_subgrad(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, vin, vout, P);
+}
 
 // net_flux (fluid.wgsl:640-657): the mean v_y of each row and the mean v_x
 // of each column over fluid cells, summed in the reference's order (k = 0,
@@ -2173,7 +2202,7 @@ __global__ __launch_bounds__(256) void k_subgrad(Geo g, const float2* __restrict
 // skip) and count the fluid cells of their stretch; then NF_C lanes walk
 // their chains with one dependent add per cell.
 constexpr uint32_t NF_C = 8, NF_K = 1024, NF_T = 1024, NF_PER = NF_C * NF_K / NF_T;
-__global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
+__device__ __forceinline__ void b_net_flux(const uint3 blockIdx, const uint3 gridDim, Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
     __shared__ float buf[NF_K][NF_C];
     __shared__ uint32_t cnt[NF_T / NF_C][NF_C];
     const uint32_t F = g.F;
@@ -2219,9 +2248,12 @@ __global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restri
     // the reference counts with n += 1.0 in binary32: exact below 2^24
     if (t < NF_C && c0 + t < F) nf[(cols ? F : 0u) + c0 + t] = n > 0u ? sum / (float)n : 0.0f;
 }
+__global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restrict__ vin, float* __restrict__ nf) {
+    b_net_flux(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, vin, nf);
+}
 
 // enforce_boundaries (fluid.wgsl:663-677)
-__global__ __launch_bounds__(256) void k_boundaries(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
+__device__ __forceinline__ void b_boundaries(const uint3 blockIdx, const uint3 gridDim, Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
                                                     const float* __restrict__ nf) {
     FLUID_XY
     if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
@@ -2232,11 +2264,15 @@ __global__ __launch_bounds__(256) void k_boundaries(Geo g, const float2* __restr
     v = reflect(g, idx, v);
     vout[idx] = clamp_len(sanitize(v), MAX_VEL);
 }
+__global__ __launch_bounds__(256) void k_boundaries(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
+                                                    const float* __restrict__ nf) {
+    b_boundaries(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, vin, vout, nf);
+}
 
 // thermal_activation (transport.wgsl:419-438) with chem_activate_one
 // (chem.wgsl:90-110).  Each thread owns its cell's four words, so the
 // compare-and-swap of the reference reduces to a plain read-modify-write.
-__global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T, uint32_t* chem) {
+__device__ __forceinline__ void b_thermal(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T, uint32_t* chem) {
     if (ep) P.epoch += *ep;
     const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     const uint32_t G = g.G, F = g.F;
@@ -2257,6 +2293,9 @@ __global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const u
         }
     }
     if (changed) *reinterpret_cast<uint4*>(chem + 4u * idx) = make_uint4(w[0], w[1], w[2], w[3]);
+}
+__global__ __launch_bounds__(256) void k_thermal(Geo g, RiboSimParams P, const uint32_t* ep, const float* __restrict__ T, uint32_t* chem) {
+    b_thermal(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, ep, T, chem);
 }
 
 // ---- exchanges with the agents (chem.wgsl:40-98) ---------------------------------------
@@ -2706,7 +2745,7 @@ __global__ __launch_bounds__(64) void k_grain_fall(uint32_t G, uint32_t F, RiboS
 // agg_count + agg_neighbours (transport.wgsl:61-87) in one pass: activated
 // counts of the tile and its 1-cell ring go to LDS, then each cell sums its
 // 3x3 block; the scatter only ever reads agg_nb + agg_act, stored as one word.
-__global__ __launch_bounds__(256) void k_agg(Geo g, const uint32_t* __restrict__ chem, uint32_t* __restrict__ agg_e) {
+__device__ __forceinline__ void b_agg(const uint3 blockIdx, const uint3 gridDim, Geo g, const uint32_t* __restrict__ chem, uint32_t* __restrict__ agg_e) {
     __shared__ uint32_t act[18][18];
     const uint32_t G = g.G;
     const int32_t bx = (int32_t)(blockIdx.x * 16), by = (int32_t)(blockIdx.y * 16);
@@ -2729,21 +2768,28 @@ __global__ __launch_bounds__(256) void k_agg(Geo g, const uint32_t* __restrict__
         for (uint32_t dx = 0; dx < 3; ++dx) s += act[threadIdx.y + dy][threadIdx.x + dx];
     agg_e[y * G + x] = s;
 }
+__global__ __launch_bounds__(256) void k_agg(Geo g, const uint32_t* __restrict__ chem, uint32_t* __restrict__ agg_e) {
+    b_agg(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, chem, agg_e);
+}
 
 __global__ void k_epoch_add(uint32_t* ep, uint32_t n) { *ep += n; }
+__global__ void k_set_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 template <class S> __global__ __launch_bounds__(256) void k_zero(S* p, uint32_t n) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i < n) stv(p, (int)i, 0.0f);
 }
 
-__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+__device__ __forceinline__ void b_fill_u32(const uint3 blockIdx, const uint3 gridDim, uint32_t* p, size_t n, uint32_t v) {
     size_t i = (size_t)blockIdx.x * 1024u + threadIdx.x * 4u;
     if (i + 3 < n) {
         *reinterpret_cast<uint4*>(p + i) = make_uint4(v, v, v, v);
     } else {
         for (; i < n; ++i) p[i] = v;
     }
+}
+__global__ __launch_bounds__(256) void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+    b_fill_u32(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), p, n, v);
 }
 
 // fluid_velocity_at_world (fluid.wgsl:260-273)
@@ -2809,7 +2855,7 @@ constexpr uint32_t LCG_A = 1664525u;  // the PCG4D input LCG multiplier
 __device__ __forceinline__ uint32_t movers_of(uint32_t w) {
     return rb_minu((w & CHEM_STATE_MASK) + (w >> 16), MAX_MOVERS_PER_CH);
 }
-__global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
+__device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
     __shared__ float4 s_w[4 * 256];        // hop weights of active movers, [channel][thread]
@@ -3139,13 +3185,18 @@ __global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const u
         atomicAdd(next + ((uint32_t)gy * G + (uint32_t)gx) * 4u + (k & 3u), v);
     }
 }
+__global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
+                                                 uint32_t* __restrict__ next, const float2* __restrict__ vel,
+                                                 const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
+    b_scatter(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, ep, chem, next, vel, temp, agg_e);
+}
 
 // ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state
 // (wrapping, as the reference's atomics).  k_ledger leaves one partial per
 // block; k_ledger_fin (one block, the next launch) sums them in a fixed
 // order, publishes the totals and, when asked, checks the per-channel
 // act + spent totals against the armed values.
-__global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led) {
+__device__ __forceinline__ void b_ledger(const uint3 blockIdx, const uint3 gridDim, const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led) {
     __shared__ uint32_t sh[4][8];
     uint32_t a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < cells; i += gridDim.x * 256) {
@@ -3163,8 +3214,11 @@ __global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ che
     __syncthreads();
     if (threadIdx.x < 8) led->part[blockIdx.x * 8 + threadIdx.x] = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
 }
+__global__ __launch_bounds__(256) void k_ledger(const uint32_t* __restrict__ chem, uint32_t cells, LedgerDev* led) {
+    b_ledger(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), chem, cells, led);
+}
 
-__global__ __launch_bounds__(256) void k_ledger_fin(LedgerDev* led, uint32_t nb, uint32_t epoch, const uint32_t* ep, int check) {
+__device__ __forceinline__ void b_ledger_fin(const uint3 blockIdx, const uint3 gridDim, LedgerDev* led, uint32_t nb, uint32_t epoch, const uint32_t* ep, int check) {
     if (ep) epoch += *ep;
     __shared__ uint32_t sh[4][8];
     uint32_t t8[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -3193,6 +3247,116 @@ __global__ __launch_bounds__(256) void k_ledger_fin(LedgerDev* led, uint32_t nb,
         }
     }
 }
+__global__ __launch_bounds__(256) void k_ledger_fin(LedgerDev* led, uint32_t nb, uint32_t epoch, const uint32_t* ep, int check) {
+    b_ledger_fin(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), led, nb, epoch, ep, check);
+}
+
+// ---- many worlds in one launch ---------------------------------------------------
+// A batched launch runs one kernel over the blocks of several worlds: its
+// block map gives, per workgroup, the world (an index into the kernel's
+// argument table) and the block index and grid size the kernel body sees
+// for that world.  The bodies are the single-world kernels' own, so every
+// world computes exactly what it computes alone.  The map entry and the
+// world index are uniform over the workgroup (scalar loads).
+#define BATCH_BLOCK(tab)                                                                                   \
+    const BatchBlock bb = map[::blockIdx.x];                                                               \
+    const uint32_t bw = __builtin_amdgcn_readfirstlane(bb.w);                                              \
+    const uint32_t bgd = __builtin_amdgcn_readfirstlane(bb.gdim);                                          \
+    const uint3 bid = make_uint3(__builtin_amdgcn_readfirstlane(bb.bx), __builtin_amdgcn_readfirstlane(bb.by), 0u); \
+    const uint3 gdim = make_uint3(bgd & 0xFFFFu, bgd >> 16, 1u);                                          \
+    const auto& a = tab[bw];
+
+__global__ __launch_bounds__(256) void kb_temperature(const BTemperature* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                      uint32_t step) {
+    BATCH_BLOCK(tab)
+    RiboSimParams P = a.P;
+    P.epoch = step;
+    b_temperature(bid, gdim, a.g, P, a.ep, a.vel, a.T, a.Q, a.t_out, a.r_out, a.eaten, 0, a.chem);
+}
+__global__ __launch_bounds__(256) void kb_forces(const BForces* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                 uint32_t step) {
+    BATCH_BLOCK(tab)
+    RiboSimParams P = a.P;
+    P.epoch = step;
+    b_forces(bid, gdim, a.g, P, a.ep, a.T, a.forces, nullptr);
+}
+__global__ __launch_bounds__(256) void kb_add_forces(const BVel* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_add_forces(bid, gdim, a.g, a.P, a.aux, a.vin, a.vout);
+}
+__global__ __launch_bounds__(256) void kb_diffuse(const BVel* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_diffuse(bid, gdim, a.g, a.P, a.vin, a.vout);
+}
+__global__ __launch_bounds__(256) void kb_advect(const BVel* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_advect(bid, gdim, a.g, a.P, a.decay, a.vin, a.vout);
+}
+__global__ __launch_bounds__(256) void kb_vorticity(const BVel* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_vorticity(bid, gdim, a.g, a.P, a.vin, a.vout);
+}
+__global__ __launch_bounds__(256) void kb_divergence(const BProj* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_divergence(bid, gdim, a.g, a.vin, a.f0, a.f1);
+}
+__global__ __launch_bounds__(256) void kb_subgrad(const BProj* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_subgrad(bid, gdim, a.g, a.vin, a.vout, a.f0);
+}
+__global__ __launch_bounds__(NF_T) void kb_net_flux(const BProj* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_net_flux(bid, gdim, a.g, a.vin, a.f1);
+}
+__global__ __launch_bounds__(256) void kb_boundaries(const BProj* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_boundaries(bid, gdim, a.g, a.vin, a.vout, a.f1);
+}
+__global__ __launch_bounds__(256) void kb_thermal(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                  uint32_t step) {
+    BATCH_BLOCK(tab)
+    RiboSimParams P = a.P;
+    P.epoch = step;
+    b_thermal(bid, gdim, a.g, P, a.ep, a.temp, a.chem);
+}
+__global__ __launch_bounds__(256) void kb_agg(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_agg(bid, gdim, a.g, a.chem, a.agg_e);
+}
+__global__ __launch_bounds__(256) void kb_clear_next(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_fill_u32(bid, gdim, a.next, (size_t)4 * a.g.G * a.g.G, 0u);
+}
+__global__ __launch_bounds__(256) void kb_scatter(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                  uint32_t step) {
+    BATCH_BLOCK(tab)
+    RiboSimParams P = a.P;
+    P.epoch = step;
+    b_scatter(bid, gdim, a.g, P, a.ep, a.chem, a.next, a.vel, a.temp, a.agg_e);
+}
+__global__ __launch_bounds__(256) void kb_ledger(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_ledger(bid, gdim, a.chem, a.g.G * a.g.G, a.led);
+}
+__global__ __launch_bounds__(256) void kb_ledger_fin(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                     uint32_t step) {
+    BATCH_BLOCK(tab)
+    b_ledger_fin(bid, gdim, a.led, a.ledger_blocks, step, a.ep, 1);
+}
+__global__ __launch_bounds__(256) void kb_mg_down(const BMgPair* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_mg_down<float, false, 16>(bid, gdim, a.L, a.C, nullptr);
+}
+__global__ __launch_bounds__(256) void kb_mg_up(const BMgPair* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_mg_up<float, false, 16>(bid, gdim, a.L, a.C, nullptr);
+}
+__global__ __launch_bounds__(1024) void kb_mg_block(const MgBlockArgs<float>* __restrict__ tab,
+                                                    const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_mg_block<float, false>(bid, gdim, a, nullptr);
+}
+#undef BATCH_BLOCK
 
 inline dim3 grid2(uint32_t n) { return dim3((n + 15) / 16, (n + 15) / 16); }
 const dim3 B16(16, 16);
@@ -3297,8 +3461,8 @@ template <class S> void launch_zero_level(hipStream_t s, const MgLevelT<S>& L) {
     k_zero<S><<<(n + 255) / 256, 256, 0, s>>>(L.p, n);
 }
 template <class S>
-void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
-                     int cm, const uint32_t* done) {
+MgBlockArgs<S> mg_block_args(const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
+                             size_t* lds) {
     MgBlockArgs<S> a{};
     a.levels = count;
     int off = 0;
@@ -3315,10 +3479,18 @@ void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, 
     a.cinv = cinv;
     a.ccell = ccell;
     a.cm = cm;
-    const size_t lds = mg_block_lds_bytes_total((size_t)off, (size_t)lv[0].n * lv[0].n);
+    *lds = mg_block_lds_bytes_total((size_t)off, (size_t)lv[0].n * lv[0].n);
+    return a;
+}
+template <class S>
+void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
+                     int cm, const uint32_t* done) {
+    size_t lds = 0;
+    const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds);
     if (sym) k_mg_block<S, true><<<1, 1024, lds, s>>>(a, done);
     else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }
+template MgBlockArgs<float> mg_block_args<float>(const MgLevelT<float>*, int, const double*, const int32_t*, int, size_t*);
 inline uint32_t pcg_blocks(uint32_t n) { return ((n + 15) / 16) * ((n + 15) / 16); }
 inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
@@ -3440,7 +3612,47 @@ void launch_ledger_cells(hipStream_t s, const uint32_t* chem, uint32_t cells, Le
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n) {
     k_epoch_add<<<1, 1, 0, s>>>(ep, n);
 }
+void launch_set_u32(hipStream_t s, uint32_t* p, uint32_t v) { k_set_u32<<<1, 1, 0, s>>>(p, v); }
 
+
+// ---- batched launches (see BatchBlock) ----
+void launch_batch(hipStream_t s, BatchKernel k, const void* tab, const BatchBlock* map, uint32_t blocks, uint32_t step,
+                  size_t lds) {
+    if (blocks == 0) return;
+    switch (k) {
+        case BK_TEMPERATURE: kb_temperature<<<blocks, B16, 0, s>>>((const BTemperature*)tab, map, step); break;
+        case BK_FORCES: kb_forces<<<blocks, B16, 0, s>>>((const BForces*)tab, map, step); break;
+        case BK_ADD_FORCES: kb_add_forces<<<blocks, B16, 0, s>>>((const BVel*)tab, map); break;
+        case BK_DIFFUSE: kb_diffuse<<<blocks, B16, 0, s>>>((const BVel*)tab, map); break;
+        case BK_ADVECT: kb_advect<<<blocks, B16, 0, s>>>((const BVel*)tab, map); break;
+        case BK_VORTICITY: kb_vorticity<<<blocks, B16, 0, s>>>((const BVel*)tab, map); break;
+        case BK_DIVERGENCE: kb_divergence<<<blocks, B16, 0, s>>>((const BProj*)tab, map); break;
+        case BK_SUBGRAD: kb_subgrad<<<blocks, B16, 0, s>>>((const BProj*)tab, map); break;
+        case BK_NET_FLUX: kb_net_flux<<<blocks, NF_T, 0, s>>>((const BProj*)tab, map); break;
+        case BK_BOUNDARIES: kb_boundaries<<<blocks, B16, 0, s>>>((const BProj*)tab, map); break;
+        case BK_THERMAL: kb_thermal<<<blocks, B16, 0, s>>>((const BChem*)tab, map, step); break;
+        case BK_AGG: kb_agg<<<blocks, B16, 0, s>>>((const BChem*)tab, map); break;
+        case BK_CLEAR_NEXT: kb_clear_next<<<blocks, 256, 0, s>>>((const BChem*)tab, map); break;
+        case BK_SCATTER: kb_scatter<<<blocks, B16, 0, s>>>((const BChem*)tab, map, step); break;
+        case BK_LEDGER: kb_ledger<<<blocks, 256, 0, s>>>((const BChem*)tab, map); break;
+        case BK_LEDGER_FIN: kb_ledger_fin<<<blocks, 256, 0, s>>>((const BChem*)tab, map, step); break;
+        case BK_MG_DOWN: kb_mg_down<<<blocks, 256, 0, s>>>((const BMgPair*)tab, map); break;
+        case BK_MG_UP: kb_mg_up<<<blocks, 256, 0, s>>>((const BMgPair*)tab, map); break;
+        case BK_MG_BLOCK: kb_mg_block<<<blocks, 1024, lds, s>>>((const MgBlockArgs<float>*)tab, map); break;
+    }
+}
+void batch_grid(BatchKernel k, const Geo& g, uint32_t level_n, uint32_t* gx, uint32_t* gy) {
+    *gy = 1;
+    switch (k) {
+        case BK_THERMAL: case BK_AGG: case BK_SCATTER: *gx = *gy = (g.G + 15) / 16; break;
+        case BK_NET_FLUX: *gx = 2 * ((g.F + NF_C - 1) / NF_C); break;
+        case BK_CLEAR_NEXT: *gx = blocks1((size_t)4 * g.G * g.G, 1024); break;
+        case BK_LEDGER: *gx = rb_minu(blocks1((size_t)g.G * g.G, 256), LEDGER_MAX_BLOCKS); break;
+        case BK_LEDGER_FIN: case BK_MG_BLOCK: *gx = 1; break;
+        case BK_MG_DOWN: case BK_MG_UP: *gx = *gy = (level_n + 15) / 16; break;
+        default: *gx = *gy = (g.F + 15) / 16; break;  // fluid kernels: 16 x 16 tiles
+    }
+}
 
 // ---- scratch of the ledger that allocates nothing (ribo_ledger_noalloc) ----
 // One ledger's per-block partials and totals (sizeof(LedgerDev), ~33 KB),

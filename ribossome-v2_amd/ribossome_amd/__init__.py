@@ -344,6 +344,12 @@ def lib() -> ctypes.CDLL:
         "ribo_lz4_compress": (ctypes.c_int64, [P, SZ, P, SZ]),
         "ribo_lz4_decompress": (ctypes.c_int64, [P, SZ, P, SZ]),
         "ribo_world_device_bytes": (I, [P, ctypes.POINTER(SZ)]),
+        "ribo_batch_create": (I, [ctypes.POINTER(P), U32, ctypes.POINTER(P)]),
+        "ribo_batch_destroy": (I, [P]),
+        "ribo_batch_encode_steps": (I, [P, U32]),
+        "ribo_batch_run": (I, [P, U32]),
+        "ribo_batch_sync": (I, [P]),
+        "ribo_batch_launches": (I, [P, ctypes.POINTER(ctypes.c_uint64)]),
         "ribo_server_create": (I, [ctypes.c_uint64, U32, U32, ctypes.POINTER(P)]),
         "ribo_server_destroy": (I, [P]),
         "ribo_server_submit": (I, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
@@ -741,6 +747,43 @@ class SceneStatus(ctypes.Structure):
     _fields_ = [("state", ctypes.c_int32), ("error", ctypes.c_int32), ("steps_done", ctypes.c_uint32),
                 ("_pad", ctypes.c_uint32), ("t_submit", ctypes.c_double), ("t_admit", ctypes.c_double),
                 ("t_done", ctypes.c_double), ("footprint", ctypes.c_uint64)]
+
+
+class Batch:
+    """Worlds stepped together with one launch per kernel (ribo_batch_*).
+    The worlds stay the caller's: they are read with World.download after
+    the batch's steps, and must not be closed while the batch exists."""
+
+    def __init__(self, worlds):
+        self._worlds = list(worlds)
+        arr = (ctypes.c_void_p * len(self._worlds))(*[w._h.value for w in self._worlds])
+        self._h = ctypes.c_void_p()
+        _check(lib().ribo_batch_create(arr, len(self._worlds), ctypes.byref(self._h)), "batch_create")
+
+    def close(self):
+        if self._h:
+            lib().ribo_batch_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def encode_steps(self, steps: int) -> int:
+        return _check(lib().ribo_batch_encode_steps(self._h, steps), "batch_encode_steps")
+
+    def run(self, steps: int):
+        _check(lib().ribo_batch_run(self._h, steps), "batch_run")
+
+    def sync(self):
+        _check(lib().ribo_batch_sync(self._h), "batch_sync")
+
+    def launches(self) -> int:
+        n = ctypes.c_uint64()
+        _check(lib().ribo_batch_launches(self._h, ctypes.byref(n)), "batch_launches")
+        return n.value
 
 
 SCENE_QUEUED, SCENE_RUNNING, SCENE_DONE, SCENE_FAILED = range(4)

@@ -147,3 +147,14 @@ def test_server_host_logic_without_device():
     srv.release(t)
     assert srv.stats() == (0, 0, 0, 0)
     srv.close()
+
+
+def test_batch_argument_errors_need_no_gpu():
+    L = ra.lib()
+    out = ctypes.c_void_p()
+    assert L.ribo_batch_create(None, 0, ctypes.byref(out)) == -1
+    assert b"empty" in L.ribo_last_error()
+    assert L.ribo_batch_destroy(None) == -1
+    assert L.ribo_batch_encode_steps(None, 4) == -1
+    n = ctypes.c_uint64()
+    assert L.ribo_batch_launches(None, ctypes.byref(n)) == -1

@@ -49,7 +49,10 @@ BYTES_PER_UNIT = {
     "subtract_gradient": (24, "F"),
     "net_flux": (12, "F"),
     "enforce_boundaries": (20, "F"),
-    "thermal_activation": (17, "G"),
+    # temperature 1 B per grid cell (4 B per fluid cell, read once); a hot
+    # cell (temperature above TEMP_ACT_THRESHOLD) also reads and writes its
+    # 16 B chem word: kernel_bytes adds 32 B per hot cell
+    "thermal_activation": (1, "G"),
     "agg": (20, "G"),
     "clear_next": (16, "G"),
     # chem 16 + gamma 4 + aggregation 4 + light 1 + velocity 2 + temperature 1
@@ -99,9 +102,21 @@ def block_start(sizes):
     return b
 
 
-def kernel_bytes(name, cfg, sizes, mode):
+# transport.wgsl:49: thermal activation runs in cells hotter than this
+TEMP_ACT_THRESHOLD = 2.0
+
+
+def hot_grid_cells(w, cfg):
+    """Grid cells whose fluid cell is above TEMP_ACT_THRESHOLD (the cells
+    thermal_activation reads and writes the chem of)."""
+    t = w.download(ra.TEMP)
+    return int(np.count_nonzero(t > TEMP_ACT_THRESHOLD)) * (cfg.cells() // cfg.fluid_cells())
+
+
+def kernel_bytes(name, cfg, sizes, mode, hot=0):
     """Algorithmic bytes of one launch (the mean over one V-cycle's launches
-    for the per-level multigrid kernels)."""
+    for the per-level multigrid kernels); `hot` is the hot grid cells of
+    thermal_activation."""
     S = 2 if MODES[mode]["precision"] == 1 else 4
     tol = MODES[mode]["mg_tol"] > 0
     b = max(block_start(sizes), 1) if tol else block_start(sizes)
@@ -109,7 +124,7 @@ def kernel_bytes(name, cfg, sizes, mode):
     F, G = cfg.fluid_cells(), cfg.cells()
     if name in BYTES_PER_UNIT:
         per, unit = BYTES_PER_UNIT[name]
-        return per * (G if unit == "G" else F)
+        return per * (G if unit == "G" else F) + (32 * hot if name == "thermal_activation" else 0)
     if name == "mg_down":  # fluid 1, p and rhs read, pre-smoothed p written; coarse rhs, p
         levels = sizes[lo:b]
         return sum((1 + 3 * S + S / 2) * n * n for n in levels) / max(len(levels), 1)
@@ -144,10 +159,6 @@ MODES = {
     # the plain FP32 path as the benchmark configuration states it: the
     # reference V-cycle repeated until the relative residual reaches 1e-5
     "vcycle": dict(mg_tol=1e-5, precision=0, stationary=1),
-    # the same, stopped at the true residual the conjugate-gradient modes end
-    # at (they stop on the recursive residual; in binary32 the recomputed one
-    # ends at 3-4e-5 on this world)
-    "vcycle_matched": dict(mg_tol=4e-5, precision=0, stationary=1),
 }
 
 
@@ -433,34 +444,83 @@ def default_profile_measure(cfg, inputs, args):
                        "fluid every 2nd step, transport every 3rd, 1 V-cycle, day cycle from epoch 2000"}
 
 
+def cpu_model():
+    """The host CPU's model name (Linux /proc/cpuinfo), or None."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def oracle_rate(oc, cfg, seed, params, settings, uploads, steps, threads, warm=1):
+    """Cell-steps/s of the CPU oracle on `threads` threads: `warm` untimed
+    steps, then `steps` timed ones.  Returns (rate, seconds)."""
+    oc.lib().oracle_set_threads(threads)
+    ow = oc.OracleWorld(cfg, seed)
+    ow.params = params
+    ow.set_settings(settings)
+    for which, a in uploads:
+        ow.upload(which, a)
+    if warm:
+        ow.encode_steps(warm)
+    t0 = time.perf_counter()
+    ow.run(steps)
+    dt = time.perf_counter() - t0
+    ow.close()
+    return cfg.cells() * steps / dt, dt
+
+
 def cpu_baseline(cfg, inputs, args):
-    """The CPU oracle (C++ restatement of the reference WGSL, OpenMP over all
-    threads it is given) on a bounded sample of the same workload: the same
-    2048^2 world, `--cpu-steps` steps from the same inputs."""
+    """The CPU oracle (C++ restatement of the reference WGSL, OpenMP) on the
+    host's cores: a bounded sample of the same 2048^2 workload on every
+    thread it is given (the value) and on one thread, and the 256^2 plain
+    method (configuration 1: ~10 % rock, Poisson(4), 2 V-cycles a step to a
+    16^2 coarsest level, 100 steps) in full, on every thread and on one."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_ctypes as oc
 
     gamma, vel, chem, heat = inputs
+    all_threads = oc.lib().oracle_get_threads()
     ow = oc.OracleWorld(cfg, SEED)
     p, s = configure(ow.params, ra.WorldSettings.default(), args, args.cpu_mode)
-    ow.params = p
-    ow.set_settings(s)
-    ow.upload(ra.GAMMA, gamma)
-    ow.upload(ra.VELOCITY, vel)
-    ow.upload(ra.CHEM, chem)
-    ow.upload(ra.HEAT_SRC, heat)
-    ow.encode_steps(1)  # first-touch warm-up
-    t0 = time.perf_counter()
-    ow.run(args.cpu_steps)
-    dt = time.perf_counter() - t0
-    cores = oc.lib().oracle_get_threads()
+    ow.close()
+    uploads = ((ra.GAMMA, gamma), (ra.VELOCITY, vel), (ra.CHEM, chem), (ra.HEAT_SRC, heat))
+    rate, dt = oracle_rate(oc, cfg, SEED, p, s, uploads, args.cpu_steps, all_threads)
+    rate1, dt1 = oracle_rate(oc, cfg, SEED, p, s, uploads, 1, 1)
+    # configuration 1 in full
+    c1 = ra.WorldConfig(256, 128)
+    g1 = ra.gen_gamma(c1, 1, 0.10, False)
+    v1 = ra.gen_velocity(c1, 1, 0.5 / 0.017, g1)
+    m1 = ra.gen_chem(c1, 1, 4.0, g1)
+    p1 = ra.SimParams.default(1)
+    p1.transport_every = 1
+    p1.uv_strength = 0.0
+    p1.sun_heat = 0.0
+    s1 = ra.WorldSettings.default()
+    s1.fluid_substep = 1
+    s1.mg_cycles = 2
+    s1.mg_min_size = 16
+    s1.ledger_check = 1
+    up1 = ((ra.GAMMA, g1), (ra.VELOCITY, v1), (ra.CHEM, m1))
+    r1, t1 = oracle_rate(oc, c1, 1, p1, s1, up1, 100, all_threads, warm=0)
+    r11, t11 = oracle_rate(oc, c1, 1, p1, s1, up1, 100, 1, warm=0)
+    oc.lib().oracle_set_threads(all_threads)
     return {
-        "value": cfg.cells() * args.cpu_steps / dt,
+        "value": rate,
         "unit": "cell-steps/s",
-        "cores": int(cores),
+        "cores": int(all_threads),
         "kind": "port",
-        "sample": f"{args.cpu_steps} steps of the same 2048^2 world after 1 warm-up step, "
-                  f"C++ oracle, {cores} OpenMP threads, {dt:.2f} s",
+        "sample": f"{args.cpu_steps} steps of the same 2048^2 world ({args.cpu_mode} mode) after 1 warm-up step, "
+                  f"C++ oracle, {all_threads} OpenMP threads, {dt:.2f} s",
+        "cpu_model": cpu_model(),
+        "one_thread": {"value": rate1, "sample": f"1 step of the same world after 1 warm-up step, {dt1:.2f} s"},
+        "config1_plain_256": {"value": r1, "one_thread_value": r11, "steps": 100,
+                              "seconds": t1, "one_thread_seconds": t11,
+                              "sample": "256^2 grid / 128^2 fluid, ~10 % rock, Poisson(4), 16-mode velocity at "
+                                        "0.5 fluid cells per step, 2 V-cycles per step to 16^2, ledger every step"},
     }
 
 
@@ -519,14 +579,38 @@ def pmc_traffic(kernel, mode):
     return kib * 1024.0 / calls if calls else None
 
 
-def step_model_bytes(cfg, sizes, mode, res):
+IDLE_LAUNCH_US = 0.7  # an early-exit solver launch (kernel trace: 0.64-0.76 us)
+SOLVER_KERNELS = ("cg_down0", "cg_up0", "mg_block", "mg_down", "mg_up")
+
+
+def working_launches(times, sizes, mode, iterations):
+    """{kernel: (ms, launches)} over working launches: every launch of the
+    per-step kernels; for the solver's per-iteration kernels, `iterations`
+    (summed over the solves of the pass) times the launches per iteration,
+    with the early-exit launches' time taken off."""
+    tol = MODES[mode]["mg_tol"] > 0
+    out = {}
+    for k, (ms, n) in times.items():
+        if tol and k in SOLVER_KERNELS:
+            b = max(block_start(sizes), 1)
+            per_it = (b - 1) if k in ("mg_down", "mg_up") else 1
+            busy = min(n, iterations * per_it)
+            if busy <= 0:
+                continue
+            ms = max(ms - (n - busy) * IDLE_LAUNCH_US / 1000.0, 0.0)
+            n = busy
+        out[k] = (ms, n)
+    return out
+
+
+def step_model_bytes(cfg, sizes, mode, res, hot=0):
     """Algorithmic bytes of one world step: the per-step kernels once, the
     solver's kernels once per V-cycle it ran."""
     F, G = cfg.fluid_cells(), cfg.cells()
     once = ["update_temperature", "buoyancy_gather", "add_forces", "diffuse_velocity", "advect_velocity",
             "vorticity", "divergence", "subtract_gradient", "net_flux", "enforce_boundaries", "thermal_activation",
             "agg", "clear_next", "transport_scatter", "ledger"]
-    total = sum(kernel_bytes(k, cfg, sizes, mode) for k in once)
+    total = sum(kernel_bytes(k, cfg, sizes, mode, hot) for k in once)
     tol = MODES[mode]["mg_tol"] > 0
     b = max(block_start(sizes), 1) if tol else block_start(sizes)
     nl = b - (1 if tol else 0)
@@ -708,6 +792,11 @@ def serving(args):
                        "scheduler_threads": args.serve_threads or "library default"}}
 
 
+def note(msg):
+    """A progress line on stderr as each stage of the run ends."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -769,6 +858,7 @@ def main():
         solves, iters, missed = w.timed_solver_stats
         res = w.timed_residuals
         tol_mode = MODES[mode]["mg_tol"] > 0
+        note(f"{mode}: {1000.0 * dt / args.steps:.3f} ms/step")
         results[mode] = {"value": cfg.cells() * args.steps / dt, "ms_per_step": 1000.0 * dt / args.steps,
                          "last_solve_iterations": it, "last_true_rel_residual": rel,
                          # over every timed solve: the refined iterate's true
@@ -790,14 +880,23 @@ def main():
     # Per-kernel HIP-event timing pass (events on the world's stream).
     w.set_timing(True)
     w.reset_kernel_times()
+    s0 = w.solver_stats()
     w.run(args.profile_steps)
+    s1 = w.solver_stats()
     times = w.kernel_times()
     w.set_timing(False)
     sizes = [lv[0] for lv in w.mg_levels()]
-    dom = max(times, key=lambda k: times[k][0])
-    tot_ms, launches = times[dom]
-    avg_s = tot_ms / launches / 1000.0
-    per_launch = kernel_bytes(dom, cfg, sizes, head)
+    hot = hot_grid_cells(w, cfg)
+    # The dominant kernel by working launches: the solver's per-iteration
+    # kernels also take early-exit launches in the iteration slots a solve
+    # did not use (IDLE_LAUNCH_US each); only the iterations the solves ran
+    # count as working launches of those kernels.
+    # a solve applies the preconditioner once before its first iteration
+    work = working_launches(times, sizes, head, (s1[1] - s0[1]) + (s1[0] - s0[0]))
+    dom = max(work, key=lambda k: work[k][0])
+    work_ms, launches = work[dom]
+    avg_s = work_ms / launches / 1000.0
+    per_launch = kernel_bytes(dom, cfg, sizes, head, hot)
     achieved = per_launch / avg_s / 1e9 if per_launch else None
     traffic = pmc_traffic(dom, head)
     roofline = {
@@ -811,11 +910,12 @@ def main():
         "avg_launch_us": avg_s * 1e6,
         "launches": launches,
         "algorithmic_bytes_per_launch": per_launch,
+        "rule": "largest total time over working launches in an eager HIP-event pass of --profile-steps steps",
     }
     # the transport scatter, the largest single kernel of every step, beside it
     sc_ms, sc_n = times.get("transport_scatter", (0.0, 0))
     sc_s = sc_ms / max(sc_n, 1) / 1000.0
-    sc_bytes = kernel_bytes("transport_scatter", cfg, sizes, head)
+    sc_bytes = kernel_bytes("transport_scatter", cfg, sizes, head, hot)
     roofline_scatter = {"kernel": "transport_scatter", "bound": "hbm", "achieved": sc_bytes / sc_s / 1e9 if sc_s else None,
                         "peak": 8000.0, "unit": "GB/s", "frac": sc_bytes / sc_s / 8e12 if sc_s else None,
                         "traffic": pmc_traffic("transport_scatter", head), "avg_launch_us": sc_s * 1e6,
@@ -831,23 +931,29 @@ def main():
                          "frac": valu / sc_s / VALU_PEAK_PER_S, "instructions_per_launch": valu,
                          "avg_launch_us": sc_s * 1e6}
     kernels = {k: [round(v[0] / max(v[1], 1) * 1000.0, 2), v[1]] for k, v in sorted(times.items(), key=lambda kv: -kv[1][0])}
-    step_bytes = {m: step_model_bytes(cfg, sizes, m, results[m]) for m in results}
+    step_bytes = {m: step_model_bytes(cfg, sizes, m, results[m], hot) for m in results}
     hbm = {m: {"algorithmic_bytes_per_step": step_bytes[m],
                "GB_per_s": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 1e9,
                "frac_of_8TBps": step_bytes[m] / (results[m]["ms_per_step"] / 1000.0) / 8e12} for m in results}
 
+    note("kernel timing pass done")
     light = None if args.no_light else light_measure(cfg, inputs, args)
+    note("light done")
     terrain = terrain_measure(cfg, inputs, args) if args.terrain_steps > 0 else None
+    note("terrain done")
     agents = None if args.no_agents else agents_measure(cfg, inputs, args)
+    note("agents done")
     default_profile = default_profile_measure(cfg, inputs, args) if args.default_steps > 0 else None
+    note("default profile done")
     plain = {}
     if args.vcycle_steps > 0:
         # the stationary V-cycle, on a bounded number of steps (each step runs
         # up to --vcycle-max-cycles V-cycles)
         sub = argparse.Namespace(**vars(args))
         sub.steps, sub.warmup = args.vcycle_steps, 16
-        for mode in ("vcycle", "vcycle_matched"):
+        for mode in ("vcycle",):
             dt_v, wv = timed_run(cfg, inputs, sub, mode)
+            note(f"{mode}: {1000.0 * dt_v / sub.steps:.3f} ms/step")
             sv, iv, mv = wv.timed_solver_stats
             it_v, rel_v = wv.mg_stats()
             plain[mode] = {"value": cfg.cells() * sub.steps / dt_v, "ms_per_step": 1000.0 * dt_v / sub.steps,
@@ -858,6 +964,7 @@ def main():
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(cfg, inputs, args)
+        note("cpu baseline done")
     f64 = None
     if not args.no_cpu and not args.no_fp64 and "fp32" in results and "mixed" in results:
         f64 = fp64_error(cfg, inputs, args)
@@ -889,8 +996,8 @@ def main():
         "modes": results,
         "mixed_vs_fp32": (results["mixed"]["value"] / results["fp32"]["value"]) if "mixed" in results and "fp32" in results else None,
         "plain_vcycle": plain or None,
-        # ratios against the stationary V-cycle at the configuration's 1e-5 and
-        # at the true residual the conjugate-gradient modes end at
+        # ratios against the stationary V-cycle at the configuration's 1e-5
+        # (every mode stops on a true relative residual <= 1e-5)
         "mixed_vs_plain_vcycle": {m: results["mixed"]["value"] / v["value"] for m, v in plain.items()}
         if plain and "mixed" in results else None,
         "fp32_vs_plain_vcycle": {m: results["fp32"]["value"] / v["value"] for m, v in plain.items()}

@@ -119,3 +119,14 @@ def test_refit_random_terrains_exact_and_within_capacity(seed):
     out = ra.refit_matter_to_terrain(full, g2)
     assert np.array_equal(out, so.refit_matter_to_terrain(full, g2))
     assert ra.Ledger.from_cells(out) == ra.Ledger.from_cells(full)
+
+
+def test_splitmix64_published_vectors():
+    """The generator under seed_cells and fit_matter_to_terrain is Vigna's
+    splitmix64 (src/world/mod.rs:1842-1855): its published outputs for
+    seeds 0 and 1234567."""
+    r = so.SplitMix(0)
+    assert [r.next() for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    r = so.SplitMix(1234567)
+    assert [r.next() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
+                                            4593380528125082431, 16408922859458223821]

@@ -167,6 +167,8 @@ def configure(p, s, args, mode="fp32"):
     p.uv_strength = 0.0
     p.sun_heat = 0.0
     s.fluid_substep = 1
+    s.terrain_enabled = 0  # the benchmark's profile: frozen terrain and light field
+    s.light_mode = 0
     s.mg_cycles = args.mg_cycles
     s.mg_max_cycles = args.mg_max_cycles
     s.mg_tol = MODES[mode]["mg_tol"]
@@ -501,6 +503,8 @@ def cpu_baseline(cfg, inputs, args):
     p1.sun_heat = 0.0
     s1 = ra.WorldSettings.default()
     s1.fluid_substep = 1
+    s1.terrain_enabled = 0
+    s1.light_mode = 0
     s1.mg_cycles = 2
     s1.mg_min_size = 16
     s1.ledger_check = 1

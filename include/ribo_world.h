@@ -198,8 +198,10 @@ typedef struct RiboMgLevel {
  * reference's concurrent moves (each pass decides on the counts at its
  * start, then moves the grains, then evicts the monomers that no longer fit,
  * cell colour (x mod 3, y mod 3) after colour); with 0 the terrain is a
- * frozen input.  Moving terrain is refused with the tolerance mode (its
- * exact coarse solve is built per terrain).  Light follows light_mode (an extension): 0 keeps the light
+ * frozen input.  With the tolerance mode the coarsest level's exact inverse
+ * follows the moving terrain (rebuilt on the device when the coarsest fluid
+ * mask changes; that level must have at most 256 cells).  Light follows
+ * light_mode (an extension): 0 keeps the light
  * field a frozen input (RIBO_BUF_LIGHT), 1 runs the reference's light
  * schedule with light_rows_per_step and light_interval.  contact_enabled is
  * accepted and ignored (agents are not part of the path). */
@@ -213,7 +215,7 @@ typedef struct RiboWorldSettings {
                                      (at most 16; reference default 1); 0 = the
                                      full sweep every light_interval steps */
     uint32_t light_interval;      /* reference default 10 */
-    uint32_t terrain_enabled;     /* grain dynamics; reference default 1, here 0 */
+    uint32_t terrain_enabled;     /* grain dynamics; reference default 1 */
     uint32_t contact_enabled;     /* ignored */
     /* --- extensions (not in the reference) --- */
     uint32_t mg_min_size;         /* coarsest level bound, reference MG_MIN_SIZE = 8 */
@@ -222,21 +224,21 @@ typedef struct RiboWorldSettings {
                                      gradients preconditioned by a symmetric
                                      V-cycle until ||r||2/||rhs||2 <= mg_tol over
                                      fluid cells of level 0 */
-    uint32_t mg_max_cycles;       /* iteration cap of the tolerance mode.  Each
-                                     ribo_world_encode_steps call records, per
-                                     solve, the most iterations any of the last
-                                     64 solves took plus max(3, 1/5 of it),
-                                     rounded up to even, at most the cap; a solve
+    uint32_t mg_max_cycles;       /* iteration cap of the tolerance mode.  Before
+                                     every 8 steps it records, per solve, the
+                                     most iterations any of the last 32 solves
+                                     took plus max(4, a third of it), rounded up
+                                     to a multiple of 4, at most the cap; a solve
                                      that runs out is counted (see
-                                     ribo_world_solver_stats) and the next call
-                                     records twice the maximum + 4. */
+                                     ribo_world_solver_stats) and the next 8
+                                     steps record twice as many. */
     uint32_t precision;           /* RIBO_PRECISION_FP32 (drop-in) or _MIXED */
     uint32_t ledger_check;        /* 1: exact per-channel totals checked on the
                                      device after every transport pass */
     uint32_t no_graphs;           /* 1: record every step eagerly instead of
                                      replaying captured HIP graphs */
-    uint32_t light_mode;          /* 0: the light field is a frozen input (the
-                                     default); 1: the reference's light step
+    uint32_t light_mode;          /* 0: the light field is a frozen input; 1 (the
+                                     default): the reference's light step
                                      before the fluid of every step
                                      (src/world/mod.rs:1506-1525): the full sweep
                                      when the light is stale (after an upload of

@@ -1769,6 +1769,9 @@ template <class R> void fluid_step(OracleWorld& w, Fluid<R>& fl, const RiboSimPa
             w.last_rel_res = rel;
             w.last_rounded_rel = rel;
         } else if (st.mg_tol > 0.0f) {
+            // the iterate starts from mg_init's pressure: zero at the walls
+            // (a cell the moving terrain has just made solid drops its value)
+            for (uint32_t i = 0; i < L0.n * L0.n; ++i) fl.p_a[i] = fl.mg_p[i];
             pcg_solve<R>(w, fl, mg, lv);
             for (uint32_t i = 0; i < L0.n * L0.n; ++i) fl.mg_p[i] = fl.p_a[i];
         } else {
@@ -2176,7 +2179,7 @@ OracleWorld* oracle_create(const RiboWorldConfig* cfg, uint32_t seed, int fp64) 
     RiboWorldSettings s;
     memset(&s, 0, sizeof(s));
     s.fluid_enabled = 1; s.fluid_substep = 2; s.multigrid = 1; s.mg_cycles = 1; s.jacobi_iters = 128;
-    s.light_rows_per_step = 1; s.light_interval = 10; s.terrain_enabled = 0; s.contact_enabled = 1;
+    s.light_rows_per_step = 1; s.light_interval = 10; s.terrain_enabled = 1; s.contact_enabled = 1; s.light_mode = 1;
     s.mg_min_size = 8; s.mg_tol = 0.0f; s.mg_max_cycles = 1; s.precision = 0; s.ledger_check = 0;
     w->st = s;
     size_t G = (size_t)w->G * w->G, F = (size_t)w->F * w->F;
@@ -2197,7 +2200,6 @@ void oracle_set_params(OracleWorld* w, const RiboSimParams* p) { w->params = *p;
 void oracle_get_params(const OracleWorld* w, RiboSimParams* p) { *p = w->params; }
 
 int oracle_set_settings(OracleWorld* w, const RiboWorldSettings* s) {
-    if (s->terrain_enabled && s->multigrid && s->mg_tol > 0.0f) return RIBO_ERR_UNSUPPORTED;
     if (s->mg_min_size < 1 || s->light_mode > 1u) return RIBO_ERR_ARG;
     w->st = *s;
     w->f32.hist = w->f64.hist = 0;  // new settings start the pressure history again

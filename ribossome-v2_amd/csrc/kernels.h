@@ -150,11 +150,19 @@ struct MgBlockArgs {
     const double* cinv;            // m x m inverse of the coarsest operator (SYM only)
     const int32_t* ccell;          // unknown -> coarsest cell
     int cm;                        // unknowns of the coarsest level (0: use sweeps)
+    const int* cm_dev;             // when set, cm is read from here (the inverse is rebuilt on the device)
 };
 // The arguments and dynamic LDS bytes of k_mg_block over `count` levels.
 template <class S>
 MgBlockArgs<S> mg_block_args(const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
-                             size_t* lds);
+                             size_t* lds, const int* cm_dev = nullptr);
+// Rebuilds the coarsest level's exact inverse on the device when its fluid
+// mask (n x n <= 1024 cells, at most `cap` of them fluid) differs from the
+// snapshot the inverse was built for: the host's Gauss-Jordan elimination
+// (build_coarse_inverse), operation for operation.  `work` holds cap x cap
+// doubles; cinv, ccell and cm are written as the host writes them.
+void launch_coarse_refresh(hipStream_t s, uint32_t n, float eps, const uint8_t* fluid, uint8_t* snap, uint32_t cap,
+                           double* work, double* cinv, int32_t* ccell, int* cm);
 
 // ---- many worlds in one launch ----
 // One workgroup of a batched launch: the world (index into the argument
@@ -239,7 +247,7 @@ template <class S> void launch_zero_level(hipStream_t s, const MgLevelT<S>& L);
 constexpr uint32_t MG_SYM_COARSE_MAX = 1024;
 size_t mg_block_lds_bytes(const uint32_t* n, int count);
 template <class S> void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv,
-                                        const int32_t* ccell, int cm, const uint32_t* done);
+                                        const int32_t* ccell, int cm, const uint32_t* done, const int* cm_dev = nullptr);
 // preconditioned conjugate gradients
 // r = b - A x and the solve's initial state (zeroes the other vectors); when
 // V.xold is set and the last two solves' pressures are on hand, the start

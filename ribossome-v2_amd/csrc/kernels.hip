@@ -1397,19 +1397,19 @@ __device__ void blk_up(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P
 constexpr int MG_CINV_REG = 64;
 template <bool SYM, int TW, int TH, class S>
 __device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, float* P, const float* RHS, const uint8_t* FL,
-                           float* bv, const double (&ci)[MG_CINV_REG / 4]) {
+                           float* bv, const double (&ci)[MG_CINV_REG / 4], const int cm) {
     const int L = a.levels;
     float* p = P + a.off[L - 1];
     const float* rhs = RHS + a.off[L - 1];
     const uint8_t* fl = FL + a.off[L - 1];
     const int t = tm.ty * TW + tm.tx;
-    if (SYM && a.cm > 0) {
+    if (SYM && cm > 0) {
         // exact solve: p = A^-1 rhs over the fluid cells, binary64.  The
         // right-hand side is staged compactly; four lanes share a row (the
         // inverse is stored by columns, so a row's quarter reads words that
         // neighbouring rows read too) and add their partial sums in a fixed
         // order.
-        const int m = a.cm;
+        const int m = cm;
         for (int j = t; j < m; j += TW * TH) bv[j] = rhs[a.ccell[j]];
         tm.sync();
         const int q = t & 3;
@@ -1490,10 +1490,12 @@ __device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gri
         for (int y = blk.ty; y < n; y += 32)
             for (int x = blk.tx; x < n; x += 32) FL[a.off[l] + y * n + x] = a.fluid[l][y * n + x];
     }
-    // this lane's quarter row of the coarsest inverse (see blk_coarse)
+    // the coarsest level's unknowns: from the device when the inverse is
+    // rebuilt there; this lane's quarter row of its inverse (see blk_coarse)
+    const int cm = (SYM && a.cm_dev) ? (int)__builtin_amdgcn_readfirstlane((uint32_t)*a.cm_dev) : a.cm;
     double ci[MG_CINV_REG / 4];
     {
-        const int m = a.cm, t = (int)threadIdx.x, q = t & 3, r0 = t >> 2;
+        const int m = cm, t = (int)threadIdx.x, q = t & 3, r0 = t >> 2;
 #pragma unroll
         for (int k = 0; k < MG_CINV_REG / 4; ++k)
             ci[k] = (SYM && m > 0 && m <= MG_CINV_REG && r0 < m && q + 4 * k < m) ? a.cinv[(size_t)(q + 4 * k) * m + r0] : 0.0;
@@ -1501,7 +1503,7 @@ __device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gri
     __syncthreads();
     const int L = a.levels;
     for (int l = 0; l + 1 < L; ++l) blk_down<SYM>(blk, a, P, RHS, FL, RN, l);
-    blk_coarse<SYM>(blk, a, P, RHS, FL, BV, ci);
+    blk_coarse<SYM>(blk, a, P, RHS, FL, BV, ci, cm);
     for (int l = L - 2; l >= 0; --l) blk_up<SYM>(blk, a, P, RHS, FL, l);
     for (int y = blk.ty; y < n0; y += 32)
         for (int x = blk.tx; x < n0; x += 32) {
@@ -3358,6 +3360,145 @@ __global__ __launch_bounds__(1024) void kb_mg_block(const MgBlockArgs<float>* __
 }
 #undef BATCH_BLOCK
 
+// ---- the coarsest inverse under moving terrain (tolerance mode) --------------------------
+// One workgroup of 1024 threads, one coarsest cell per thread (n^2 <= 1024).
+// Returns at once while the coarsest fluid mask equals `snap` (the mask the
+// inverse was built for); otherwise rebuilds the inverse exactly as the host
+// build does: the unknowns are the fluid cells in index order; row r of A has
+// 1 at each fluid 4-neighbour (left, right, down, up) and -(k + eps) on the
+// diagonal; Gauss-Jordan with partial pivoting (the first row of largest
+// |a[r][c]|), the pivot row divided by its pivot, every other row with a
+// non-zero factor f updated as a[r][j] - f * a[c][j]; the inverse stored by
+// columns (cinv[j * m + r]).  Each element sees the host's operations in the
+// host's order, so the inverse is bit-identical to the host's.  More than
+// `cap` fluid cells, or none, leaves cm = 0 (the coarsest level is smoothed).
+__global__ __launch_bounds__(1024) void k_coarse_refresh(uint32_t n, float eps, const uint8_t* __restrict__ fl,
+                                                         uint8_t* snap, uint32_t cap, double* a, double* inv,
+                                                         double* cinv, int32_t* ccell, int* cm) {
+    __shared__ int s_changed, s_piv;
+    __shared__ int32_t s_idx[1024];
+    __shared__ uint32_t s_wsum[16];
+    __shared__ double s_bv[16];
+    __shared__ int s_bi[16];
+    __shared__ double s_d;
+    __shared__ double s_f[1024];
+    const int t = (int)threadIdx.x;
+    const int nn = (int)(n * n);
+    const uint8_t cur = (t < nn && fl[t]) ? 1u : 0u;
+    if (t == 0) s_changed = 0;
+    __syncthreads();
+    if (t < nn && snap[t] != cur) atomicOr(&s_changed, 1);
+    __syncthreads();
+    if (!s_changed) return;
+    // unknown index of each cell: exclusive scan of the fluid flags
+    const uint64_t bal = __ballot(cur != 0);
+    const int lane = t & 63, wv = t >> 6;
+    const uint32_t below = (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+    if (lane == 0) s_wsum[wv] = (uint32_t)__popcll(bal);
+    __syncthreads();
+    uint32_t base = 0, m = 0;
+    for (int k = 0; k < 16; ++k) {
+        base += k < wv ? s_wsum[k] : 0u;
+        m += s_wsum[k];
+    }
+    s_idx[t] = cur ? (int32_t)(base + below) : -1;
+    __syncthreads();
+    if (t < nn) snap[t] = cur;
+    if (m == 0u || m > cap) {
+        if (t == 0) *cm = 0;
+        return;
+    }
+    const int M = (int)m;
+    if (cur) ccell[s_idx[t]] = t;
+    for (int k = t; k < M * M; k += 1024) {
+        a[k] = 0.0;
+        inv[k] = 0.0;
+    }
+    __syncthreads();
+    if (cur) {
+        const int r = s_idx[t], x = t % (int)n, y = t / (int)n;
+        const int nb[4] = {x > 0 ? s_idx[t - 1] : -1, x + 1 < (int)n ? s_idx[t + 1] : -1,
+                           y > 0 ? s_idx[t - (int)n] : -1, y + 1 < (int)n ? s_idx[t + (int)n] : -1};
+        int k = 0;
+        for (int d = 0; d < 4; ++d)
+            if (nb[d] >= 0) {
+                a[(size_t)r * M + nb[d]] = 1.0;
+                ++k;
+            }
+        a[(size_t)r * M + r] = -((double)k + (double)eps);
+        inv[(size_t)r * M + r] = 1.0;
+    }
+    __syncthreads();
+    for (int c = 0; c < M; ++c) {
+        // pivot: the first row r >= c of largest |a[r][c]|
+        double v = -1.0;
+        int vi = M;
+        if (t >= c && t < M) {
+            v = fabs(a[(size_t)t * M + c]);
+            vi = t;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_down(v, off, 64);
+            const int oi = __shfl_down(vi, off, 64);
+            if (ov > v || (ov == v && oi < vi)) {
+                v = ov;
+                vi = oi;
+            }
+        }
+        if (lane == 0) {
+            s_bv[wv] = v;
+            s_bi[wv] = vi;
+        }
+        __syncthreads();
+        if (t == 0) {
+            double bv = s_bv[0];
+            int bi = s_bi[0];
+            for (int k = 1; k < 16; ++k)
+                if (s_bv[k] > bv || (s_bv[k] == bv && s_bi[k] < bi)) {
+                    bv = s_bv[k];
+                    bi = s_bi[k];
+                }
+            s_piv = bi;
+        }
+        __syncthreads();
+        const int piv = s_piv;
+        if (piv != c)
+            for (int j = t; j < M; j += 1024) {
+                const double x0 = a[(size_t)c * M + j], x1 = a[(size_t)piv * M + j];
+                a[(size_t)c * M + j] = x1;
+                a[(size_t)piv * M + j] = x0;
+                const double y0 = inv[(size_t)c * M + j], y1 = inv[(size_t)piv * M + j];
+                inv[(size_t)c * M + j] = y1;
+                inv[(size_t)piv * M + j] = y0;
+            }
+        __syncthreads();
+        if (t == 0) s_d = a[(size_t)c * M + c];
+        __syncthreads();
+        const double dg = s_d;
+        for (int j = t; j < M; j += 1024) {
+            a[(size_t)c * M + j] = a[(size_t)c * M + j] / dg;
+            inv[(size_t)c * M + j] = inv[(size_t)c * M + j] / dg;
+        }
+        __syncthreads();
+        // every row's factor before any row changes
+        for (int r = t; r < M; r += 1024) s_f[r] = a[(size_t)r * M + c];
+        __syncthreads();
+        for (int k = t; k < M * M; k += 1024) {
+            const int r = k / M, jj = k - r * M;
+            const double fr = s_f[r];
+            if (r == c || fr == 0.0) continue;
+            a[k] = a[k] - fr * a[(size_t)c * M + jj];
+            inv[k] = inv[k] - fr * inv[(size_t)c * M + jj];
+        }
+        __syncthreads();
+    }
+    for (int k = t; k < M * M; k += 1024) {
+        const int r = k / M, jj = k - r * M;
+        cinv[(size_t)jj * M + r] = inv[k];
+    }
+    if (t == 0) *cm = M;
+}
+
 inline dim3 grid2(uint32_t n) { return dim3((n + 15) / 16, (n + 15) / 16); }
 const dim3 B16(16, 16);
 inline uint32_t blocks1(size_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
@@ -3462,7 +3603,7 @@ template <class S> void launch_zero_level(hipStream_t s, const MgLevelT<S>& L) {
 }
 template <class S>
 MgBlockArgs<S> mg_block_args(const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
-                             size_t* lds) {
+                             size_t* lds, const int* cm_dev) {
     MgBlockArgs<S> a{};
     a.levels = count;
     int off = 0;
@@ -3479,18 +3620,20 @@ MgBlockArgs<S> mg_block_args(const MgLevelT<S>* lv, int count, const double* cin
     a.cinv = cinv;
     a.ccell = ccell;
     a.cm = cm;
+    a.cm_dev = cm_dev;
     *lds = mg_block_lds_bytes_total((size_t)off, (size_t)lv[0].n * lv[0].n);
     return a;
 }
 template <class S>
 void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, const double* cinv, const int32_t* ccell,
-                     int cm, const uint32_t* done) {
+                     int cm, const uint32_t* done, const int* cm_dev) {
     size_t lds = 0;
-    const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds);
+    const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds, cm_dev);
     if (sym) k_mg_block<S, true><<<1, 1024, lds, s>>>(a, done);
     else k_mg_block<S, false><<<1, 1024, lds, s>>>(a, done);
 }
-template MgBlockArgs<float> mg_block_args<float>(const MgLevelT<float>*, int, const double*, const int32_t*, int, size_t*);
+template MgBlockArgs<float> mg_block_args<float>(const MgLevelT<float>*, int, const double*, const int32_t*, int, size_t*,
+                                                  const int*);
 inline uint32_t pcg_blocks(uint32_t n) { return ((n + 15) / 16) * ((n + 15) / 16); }
 inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
@@ -3525,7 +3668,7 @@ template <class S> void launch_cg_final(hipStream_t s, const CgVec<S>& V, PcgSta
     template void launch_mg_up<S>(hipStream_t, const MgLevelT<S>&, const MgLevelT<S>&, bool, const uint32_t*, int); \
     template void launch_zero_level<S>(hipStream_t, const MgLevelT<S>&);                                            \
     template void launch_mg_block<S>(hipStream_t, const MgLevelT<S>*, int, bool, const double*, const int32_t*, int, \
-                                     const uint32_t*);                                                              \
+                                     const uint32_t*, const int*);                                                  \
     template void launch_cg_init<S>(hipStream_t, const CgVec<S>&, PcgState*, float, uint32_t);                     \
     template void launch_cg_down0<S>(hipStream_t, const CgVec<S>&, const MgLevelT<S>&, PcgState*, int, int, int,    \
                                      float, uint32_t);                                                             \
@@ -3608,6 +3751,10 @@ void launch_ledger_cells(hipStream_t s, const uint32_t* chem, uint32_t cells, Le
     const uint32_t nb = rb_minu(blocks1(cells, 256), LEDGER_MAX_BLOCKS);
     k_ledger<<<nb, 256, 0, s>>>(chem, cells, led);
     k_ledger_fin<<<1, 256, 0, s>>>(led, nb, 0u, nullptr, 0);
+}
+void launch_coarse_refresh(hipStream_t s, uint32_t n, float eps, const uint8_t* fluid, uint8_t* snap, uint32_t cap,
+                           double* work, double* cinv, int32_t* ccell, int* cm) {
+    k_coarse_refresh<<<1, 1024, 0, s>>>(n, eps, fluid, snap, cap, work, work + (size_t)cap * cap, cinv, ccell, cm);
 }
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n) {
     k_epoch_add<<<1, 1, 0, s>>>(ep, n);

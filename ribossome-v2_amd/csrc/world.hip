@@ -274,12 +274,51 @@ int rebuild_terrain(RiboWorld* w) {
     return RIBO_OK;
 }
 
+// Whether the coarsest inverse follows a moving terrain: the conjugate-
+// gradient tolerance mode with terrain_enabled.
+bool coarse_moving(const RiboWorld* w) {
+    return w->st.terrain_enabled && w->st.multigrid && w->st.mg_tol > 0.0f && !w->st.mg_stationary;
+}
+
 // Inverse of the coarsest level's eps-anchored operator over its fluid
 // cells, in binary64 (Gauss-Jordan with partial pivoting on the host), for
 // the exact coarse solve of the tolerance mode.  Skipped above 1024 unknowns.
+// Under a moving terrain the buffers take every cell of the coarsest level
+// (at most MOVING_COARSE_MAX), with the mask the inverse was built for and
+// its unknown count on the device: k_coarse_refresh rebuilds the inverse
+// there, in the same operations, whenever a fluid step changes that mask.
+constexpr uint32_t MOVING_COARSE_MAX = 256;
 int build_coarse_inverse(RiboWorld* w) {
     const RiboMgLevel& C = w->levels[w->sym_coarse];
     const uint32_t n = C.n;
+    AllocCount count(&w->cinv_bytes);
+    dfree(w->cinv);
+    dfree(w->ccell);
+    dfree(w->csnap);
+    dfree(w->cwork);
+    dfree(w->cm_dev);
+    w->cm = 0;
+    w->cmoving = coarse_moving(w);
+    w->ccap = 0;
+    if (w->cmoving) {
+        // buffers for every cell of the coarsest level and an invalid mask
+        // snapshot: the first fluid step's k_coarse_refresh builds the inverse
+        const uint32_t cap = n * n;
+        if (cap > MOVING_COARSE_MAX)
+            return fail(RIBO_ERR_UNSUPPORTED, "moving terrain with the tolerance mode needs a coarsest multigrid level of "
+                                              "at most 256 cells (lower mg_min_size)");
+        int rc;
+        if ((rc = dalloc(w->stream, &w->cinv, (size_t)cap * cap)) || (rc = dalloc(w->stream, &w->ccell, cap)) ||
+            (rc = dalloc(w->stream, &w->csnap, cap)) || (rc = dalloc(w->stream, &w->cwork, 2 * (size_t)cap * cap)) ||
+            (rc = dalloc(w->stream, &w->cm_dev, 1)))
+            return rc;
+        w->ccap = cap;
+        HIPCHK(hipMemsetAsync(w->csnap, 0xFF, cap, w->stream));
+        HIPCHK(hipMemsetAsync(w->cm_dev, 0, sizeof(int), w->stream));
+        HIPCHK(hipStreamSynchronize(w->stream));
+        w->cinv_valid = true;
+        return RIBO_OK;
+    }
     std::vector<uint8_t> fl((size_t)n * n);
     HIPCHK(hipMemcpyAsync(fl.data(), w->mg_fluid + C.off, fl.size(), hipMemcpyDeviceToHost, w->stream));
     HIPCHK(hipStreamSynchronize(w->stream));
@@ -287,10 +326,6 @@ int build_coarse_inverse(RiboWorld* w) {
     for (uint32_t i = 0; i < n * n; ++i)
         if (fl[i]) { idx[i] = (int32_t)cell.size(); cell.push_back((int32_t)i); }
     const size_t m = cell.size();
-    AllocCount count(&w->cinv_bytes);
-    dfree(w->cinv);
-    dfree(w->ccell);
-    w->cm = 0;
     if (m == 0 || m > 1024) {
         w->cinv_valid = true;
         return RIBO_OK;
@@ -373,7 +408,10 @@ template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevel
         const size_t b = std::min(std::max<size_t>(w->mg_block_start, 1), lc);
         for (size_t l = 1; l < b; ++l)
             run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
-        run(w, "mg_block", [&] { launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done); });
+        run(w, "mg_block", [&] {
+            launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done,
+                               w->cmoving ? w->cm_dev : nullptr);
+        });
         for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
         return;
     }
@@ -481,7 +519,16 @@ void fluid_terrain(RiboWorld* w) {
 
 void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1527-1588
     hipStream_t s = w->stream;
-    if (w->st.terrain_enabled) fluid_terrain(w);
+    if (w->st.terrain_enabled) {
+        fluid_terrain(w);
+        if (w->cmoving) {  // the coarsest inverse follows the coarsest fluid mask
+            const RiboMgLevel& C = w->levels[w->sym_coarse];
+            run(w, "coarse_refresh", [&] {
+                launch_coarse_refresh(s, C.n, C.eps, w->mg_fluid + C.off, w->csnap, w->ccap, w->cwork, w->cinv, w->ccell,
+                                      w->cm_dev);
+            });
+        }
+    }
     Geo g = w->geo();
     const int tc = w->tcur;
     run(w, "update_temperature", [&] {
@@ -636,8 +683,6 @@ int reset_pressure_history(RiboWorld* w) {
 
 int validate_settings(const RiboWorldSettings* s) {
     if (s->terrain_enabled > 1u) return fail(RIBO_ERR_ARG, "terrain_enabled must be 0 or 1");
-    if (s->terrain_enabled && s->multigrid && s->mg_tol > 0.0f)
-        return fail(RIBO_ERR_UNSUPPORTED, "moving terrain (terrain_enabled) is not supported with the tolerance mode (mg_tol > 0)");
     if (s->light_mode > 1u) return fail(RIBO_ERR_ARG, "light_mode must be 0 (frozen) or 1 (the reference's light schedule)");
     if (s->precision != RIBO_PRECISION_FP32 && s->precision != RIBO_PRECISION_MIXED)
         return fail(RIBO_ERR_ARG, "unknown precision");
@@ -680,8 +725,9 @@ void ribo_world_settings_default(RiboWorldSettings* s) {  // src/world/mod.rs:12
     s->jacobi_iters = 128;
     s->light_interval = 10;
     s->light_rows_per_step = 1;
-    s->terrain_enabled = 0;
+    s->terrain_enabled = 1;
     s->contact_enabled = 1;
+    s->light_mode = 1;  // the reference runs its light step every step
     s->mg_min_size = 8;
     s->mg_tol = 0.0f;
     s->mg_max_cycles = 1;
@@ -732,6 +778,8 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     if (st && st->multigrid && st->mg_tol > 0.0f) {  // the exact coarse solve: at most every cell of the coarsest level
         const size_t m = (size_t)lv.back().n * lv.back().n;
         if (m <= 1024) b += a(m * m, 8) + a(m, 4);
+        // under a moving terrain: the mask snapshot, the elimination work and the device count
+        if (st->terrain_enabled && !st->mg_stationary && m <= MOVING_COARSE_MAX) b += a(m, 1) + a(2 * m * m, 8) + a(1, 4);
     }
     return b;
 }
@@ -833,7 +881,7 @@ int ribo_world_destroy(RiboWorld* w) {
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
     dfree(w->pcg_r1); dfree(w->pcg_s0); dfree(w->pcg_s1); dfree(w->pcg_pd); dfree(w->pcg_w); dfree(w->pcg_xlo);
     dfree(w->pcg_z); dfree(w->pcg_r16); dfree(w->pcg_z16); dfree(w->mg_p16);
-    dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
+    dfree(w->mg_rhs16); dfree(w->cinv); dfree(w->ccell); dfree(w->csnap); dfree(w->cwork); dfree(w->cm_dev); dfree(w->ledger); dfree(w->scratch_u32); dfree(w->epoch_dev);
     dfree(w->sun_tab);
     clear_graphs(w);
     if (w->pcg_host) (void)hipHostFree(w->pcg_host);
@@ -1014,6 +1062,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
     steps = std::min(steps, RIBO_MAX_STEPS_PER_CALL);
     if (steps == 0) return 0;
     if (w->st.multigrid && w->st.mg_tol > 0.0f) {
+        if (w->cinv_valid && w->cmoving != coarse_moving(w)) w->cinv_valid = false;
         if (!w->cinv_valid) {
             clear_graphs(w);
             rc = build_coarse_inverse(w);

@@ -110,6 +110,13 @@ struct RiboWorld {
     int32_t* ccell = nullptr;
     int cm = 0;
     bool cinv_valid = false;
+    // moving terrain in the tolerance mode: the coarsest inverse is rebuilt
+    // on the device whenever the coarsest fluid mask changes
+    bool cmoving = false;
+    uint32_t ccap = 0;       // capacity (coarsest cells) of cinv / ccell
+    uint8_t* csnap = nullptr;  // the mask the inverse was built for
+    double* cwork = nullptr;   // 2 ccap^2 doubles of elimination work
+    int* cm_dev = nullptr;     // the inverse's unknowns
     uint32_t* pcg_host = nullptr;   // host-mapped: last iterations, misses, solves
     uint32_t pcg_slots = 0;         // iteration slots recorded per solve (0: not set)
     uint32_t missed_seen = 0;

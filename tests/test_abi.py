@@ -44,7 +44,9 @@ def test_struct_layouts_match_reference():
     assert np.float32(p.vent_cycle_amp) == np.float32(0.91) and p.vent_cycle_period == 7000.0
     s = ra.WorldSettings.default()
     assert (s.fluid_enabled, s.fluid_substep, s.multigrid, s.mg_cycles, s.jacobi_iters) == (1, 2, 1, 1, 128)
-    assert s.mg_min_size == 8 and s.terrain_enabled == 0
+    # the reference's WorldSettings::default() (src/world/mod.rs:121-134):
+    # terrain on, and its light step runs every step (light_mode 1)
+    assert s.mg_min_size == 8 and s.terrain_enabled == 1 and s.light_mode == 1
 
 
 def test_buffer_sizes():
@@ -118,7 +120,11 @@ def test_footprint_model():
     assert 60 * 2048 * 2048 < big < 120 * 2048 * 2048
     t = ra.WorldSettings.default()
     t.mg_tol = 1e-5
+    t.terrain_enabled = 0
     assert ra.world_footprint(ra.WorldConfig(256, 128), t) == small + 64 * 64 * 8 + 64 * 4
+    # under a moving terrain: the mask snapshot, the elimination work and the count
+    t.terrain_enabled = 1
+    assert ra.world_footprint(ra.WorldConfig(256, 128), t) == small + 64 * 64 * 8 + 64 * 4 + 64 + 2 * 64 * 64 * 8 + 4
 
 
 def test_server_host_logic_without_device():

@@ -46,6 +46,8 @@ def batched_profile(seed: int):
     p.sun_heat = 0.0
     s = ra.WorldSettings.default()
     s.fluid_substep = 1
+    s.terrain_enabled = 0
+    s.light_mode = 0
     s.mg_cycles = 1
     s.mg_tol = 0.0
     s.ledger_check = 1

@@ -31,6 +31,17 @@ def test_footprint_matches_allocations(tol):
         else:
             assert have == fp
         w.close()
+        if tol:
+            # under a moving terrain every buffer of the coarse solve takes its capacity
+            s.terrain_enabled = 1
+            w = ra.World(cfg, 3)
+            load(w, inp, parity_params(3), s)
+            w.encode_steps(1)
+            w.sync()
+            have, fp = w.device_bytes(), ra.world_footprint(cfg, s)
+            print(f"{cfg.grid_size}^2 moving terrain: world holds {have} B, footprint {fp} B", flush=True)
+            assert have == fp
+            w.close()
 
 
 SCENES = [  # (grid side, seed, steps, tolerance-mode precision or None for the reference schedule)

@@ -72,7 +72,61 @@ def test_terrain_conserves_at_size():
     assert ra.Ledger.from_cells(gw.download(ra.CHEM)).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
 
 
-def test_moving_terrain_refused_with_tolerance_mode():
-    w = ra.World(ra.WorldConfig(64, 32), 1)
-    with pytest.raises(ra.RiboError):
-        w.settings = parity_settings(terrain_enabled=1, mg_tol=1e-5)
+STILL = dict(sediment_settle=0.0, sediment_compaction=0.0, rock_crumble=0.0, sediment_transport=0.0)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_device_built_coarse_inverse_matches_host(precision):
+    """Under a moving terrain the tolerance mode builds the coarsest level's
+    inverse on the device (and again whenever the coarsest fluid mask
+    changes).  On a terrain of whole rock that nothing moves, the world with
+    the terrain on runs the device-built inverse and the world with it off
+    the host-built one: the same pressures, bit for bit."""
+    print(f"case device coarse inverse precision={precision}", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 51, rock=0.12, rubble=False, vent_rows=4)
+    p = parity_params(51, **STILL)
+    out = []
+    for terrain in (1, 0):
+        w = ra.World(cfg, 51)
+        load(w, inp, p, parity_settings(terrain_enabled=terrain, mg_tol=1e-5, mg_max_cycles=40, precision=precision))
+        w.run(12)
+        it, rel = w.mg_stats()
+        print(f"terrain {terrain}: {it} iterations, true rel {rel:.3e}", flush=True)
+        assert rel <= 1e-5
+        out.append({k: w.download(k) for k in (ra.GAMMA, ra.PRESSURE, ra.VELOCITY, ra.CHEM)})
+        w.close()
+    assert np.array_equal(out[0][ra.GAMMA], inp[ra.GAMMA])  # nothing moved
+    for k in (ra.PRESSURE, ra.VELOCITY, ra.CHEM):
+        assert bit_diff(out[0][k], out[1][k]) == 0, ra.BUFFER_NAMES[k]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_tolerance_mode_with_moving_terrain(precision):
+    """The tolerance mode with the grains moving: every solve reaches the
+    tolerance on its true residual, the fields agree with the oracle's to
+    the tolerance-mode bounds, and grains and monomers are conserved."""
+    print(f"case moving terrain tolerance precision={precision}", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 53, rock=0.12, rubble=True, vent_rows=4, vcells_per_step=2.0)
+    p = parity_params(53, **ACTIVE)
+    s = parity_settings(terrain_enabled=1, mg_tol=1e-5, mg_max_cycles=60, precision=precision, ledger_check=1)
+    gw, ow = ra.World(cfg, 53), oc.OracleWorld(cfg, 53)
+    load(gw, inp, p, s)
+    load(ow, inp, p, s)
+    for k in range(6):
+        gw.encode_steps(1)
+        ow.encode_steps(1)
+        (cyc, rel), (ocyc, orel) = gw.mg_stats(), ow.mg_stats()
+        print(f"step {k}: GPU {cyc} iterations, true rel {rel:.3e}; oracle {ocyc}, {orel:.3e}", flush=True)
+        assert rel <= 1e-5 and orel <= 1e-5
+        assert abs(cyc - ocyc) <= max(3, ocyc // 5)
+    for which, tol in ((ra.VELOCITY, 1e-4), (ra.PRESSURE, 1e-3)):
+        a, b = gw.download(which).astype(np.float64), ow.download(which).astype(np.float64)
+        e = float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+        print(f"{ra.BUFFER_NAMES[which]} rel L2 vs oracle {e:.2e}", flush=True)
+        assert e < tol
+    g = gw.download(ra.GAMMA)
+    assert int(g.sum()) == int(inp[ra.GAMMA].sum()) and int(np.count_nonzero(g != inp[ra.GAMMA])) > 0
+    assert gw.ledger_status() is None
+    assert ra.Ledger.from_cells(gw.download(ra.CHEM)).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()

@@ -132,6 +132,8 @@ def test_full_physics_conserves_each_channel_exactly():
     w.params = p
     s = ra.WorldSettings.default()
     s.fluid_substep = 1
+    s.terrain_enabled = 0
+    s.light_mode = 0
     w.set_settings(s)
     w.upload(ra.GAMMA, gamma)
     chem = ra.gen_chem(cfg, 5, 4.0, gamma)

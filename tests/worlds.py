@@ -22,8 +22,12 @@ def parity_params(seed: int = 1, **over) -> ra.SimParams:
 
 
 def parity_settings(**over) -> ra.WorldSettings:
+    """The parity profile: reference defaults with the fluid every step, a
+    frozen terrain and a frozen (uploaded) light field."""
     s = ra.WorldSettings.default()
     s.fluid_substep = 1
+    s.terrain_enabled = 0
+    s.light_mode = 0
     for k, v in over.items():
         setattr(s, k, v)
     return s

@@ -14,15 +14,21 @@
  * thread, src/world/mod.rs:1406-1632); a batch is the one-launch form of a
  * loop of World::encode_steps over independent worlds.
  *
- * The profile a batch runs: the reference V-cycle (mg_tol 0), no light step
- * (light_mode 0), frozen terrain, no pending agent words, and no solar
- * heating under a moving sun.  Every world must share fluid_substep,
- * transport_every, mg_cycles, ledger_check, the epoch and the ping-pong
- * state (fresh worlds, or worlds stepped the same way); parameters such as
- * the seed, dt and the sizes may differ.  While a world is in a batch its
- * parameters and settings must not change (checked at every call).  The
- * worlds stay owned by the caller; after a batch's steps their buffers are
- * read with ribo_world_download as usual.
+ * The profile a batch runs: the multigrid pressure solve, either the
+ * reference V-cycle (mg_tol 0) or the tolerance mode's conjugate gradients
+ * (mg_tol > 0, one precision per batch; each world keeps its own iteration
+ * slots, chosen every 8 steps as when it steps alone), with a frozen
+ * terrain and light field, no pending agent words and no solar heating
+ * under a moving sun.  Every world must share fluid_substep,
+ * transport_every, the solver (mg_cycles, or the tolerance mode's
+ * precision), ledger_check and the device; parameters such as the seed, dt,
+ * the tolerance and the sizes may differ.  Worlds keep their own epochs and
+ * may join at different epochs when they are at the same phase of the
+ * fluid / transport schedule (any epochs with both every step).  While a
+ * world is in a batch its parameters, settings and terrain must not change,
+ * and it may step on its own only as the others do (all checked at every
+ * call).  The worlds stay owned by the caller; after a batch's steps their
+ * buffers are read with ribo_world_download as usual.
  */
 #ifndef RIBO_BATCH_H
 #define RIBO_BATCH_H

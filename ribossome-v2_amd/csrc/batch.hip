@@ -1,16 +1,23 @@
 // batch.hip - many worlds stepped together with one launch per kernel.
 //
 // A step of the batch records the world step's kernels once each over the
-// blocks of every world (launch_batch): a block map, built once, tells each
-// workgroup its world and block, and an argument table per kernel (one per
-// ping-pong state of the chem and temperature buffers) holds each world's
-// pointers and parameters.  The kernel bodies are the single-world kernels'
-// own, so a world in a batch computes what it computes alone.
+// blocks of every world (launch_batch, launch_batch_tol): a block map, built
+// once, tells each workgroup its world and block, and an argument table per
+// kernel holds each world's pointers and parameters.  The kernel bodies are
+// the single-world kernels' own, so a world in a batch computes what it
+// computes alone.
 //
 // The schedule is World::encode_steps' (src/world/mod.rs:1406-1632) for the
-// profile the batch supports: fluid every fluid_substep epochs (the
-// reference V-cycle, mg_cycles times), transport every transport_every
-// epochs with the ledger check, no light step, frozen terrain.
+// profile a batch supports: fluid every fluid_substep epochs, transport every
+// transport_every epochs with the ledger check, no light step, frozen
+// terrain.  The pressure solve is the reference V-cycle (mg_cycles of them)
+// or the tolerance mode's conjugate gradients (one precision per batch).
+//
+// Worlds keep their own epochs (a device word each) and their own buffer
+// parities: the tables are built relative to each world's parity at
+// creation, and every world must have stepped the same way since (checked
+// at every call).  Worlds at different epochs step together when they share
+// the schedule's phase (always, with fluid and transport every step).
 
 #include <hip/hip_runtime.h>
 #include <string.h>
@@ -41,6 +48,11 @@ struct Shape {
     uint32_t blocks = 0;
 };
 
+uint32_t substep_of(const RiboWorld* w) { return std::min(std::max(w->st.fluid_substep, 1u), 4u); }
+uint32_t te_of(const RiboWorld* w) { return std::max(w->params.transport_every, 1u); }
+bool tol_of(const RiboWorld* w) { return w->st.mg_tol > 0.0f; }
+uint32_t gcd_u(uint32_t a, uint32_t b) { return b ? gcd_u(b, a % b) : a; }
+
 }  // namespace
 
 struct RiboBatch {
@@ -48,27 +60,38 @@ struct RiboBatch {
     hipStream_t stream = nullptr;
     std::vector<hipEvent_t> world_done;  // one per world: its queued work
     hipEvent_t batch_done = nullptr;
-    uint32_t* ep = nullptr;              // the device epoch base of every world (they step together)
+    hipEvent_t staged = nullptr;         // the last upload from the pinned words has run
+    uint32_t* words = nullptr;           // device: each world's epoch base, then its iteration slots
+    uint32_t* host = nullptr;            // pinned staging of the same words
     char* dev = nullptr;                 // maps and tables, one allocation
-    size_t dev_bytes = 0;
     uint64_t launches = 0;
-    uint32_t substep = 1, te = 1, cycles = 1, depth = 0;
-    bool ledger = false, agg = false;
-    size_t block_lds = 0;
-    // shapes
+    uint32_t substep = 1, te = 1, period = 1, cycles = 1, depth = 0;
+    bool ledger = false, agg = false, tol = false, mixed = false;
+    size_t block_lds = 0, tol_lds = 0;
+    std::vector<int> cur0, tcur0;        // each world's parity when the tables were built
+    std::vector<const double*> cinv0;    // each world's coarsest inverse the tables point at
+    // shapes (the reference V-cycle and the kernels every profile shares)
     Shape fluid, grid, netflux, clear, ledger_blocks, one;
-    std::vector<Shape> mg;               // per multigrid depth d: worlds with more than d tile levels
-    // tables (device pointers): [tcur] or [cur][tcur] or per depth
-    const BTemperature* temperature[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};  // [cur][tcur]
+    std::vector<Shape> mg;               // per depth d: worlds with more than d tile levels
+    // shapes of the tolerance mode
+    Shape cg16, cgshift, cgtile;
+    std::vector<Shape> mgt;              // per depth d: worlds whose level d + 1 is a tile level
+    // tables; [c][t] = chem and temperature parities relative to each world's own
+    const BTemperature* temperature[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     const BForces* forces[2] = {nullptr, nullptr};
     const BVel *add_forces = nullptr, *diffuse = nullptr, *advect = nullptr, *vorticity = nullptr;
     const BProj *divergence = nullptr, *subgrad = nullptr, *netflux_t = nullptr, *boundaries = nullptr;
     const BChem* chem[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     std::vector<const BMgPair*> mgpair;
     const MgBlockArgs<float>* mgblock = nullptr;
+    const void* cg = nullptr;            // BCg<S>[count]
+    std::vector<const void*> mgpair_t;   // BMgPairT<S>[count] per depth
+    const void* mgblock_t = nullptr;     // BMgBlock<S>[count]
     // what each world looked like at creation (parameters and settings must not change)
     std::vector<RiboSimParams> params0;
     std::vector<RiboWorldSettings> settings0;
+    uint32_t* ep() const { return words; }
+    uint32_t* nit() const { return words + worlds.size(); }
 };
 
 namespace {
@@ -76,8 +99,9 @@ namespace {
 int check_world(const RiboWorld* w, const RiboWorld* first) {
     const RiboWorldSettings& s = w->st;
     const RiboSimParams& p = w->params;
-    if (!s.fluid_enabled || !s.multigrid || s.mg_tol > 0.0f)
-        return set_error(RIBO_ERR_UNSUPPORTED, "a batch runs the reference V-cycle: fluid and multigrid on, mg_tol 0");
+    if (!s.fluid_enabled || !s.multigrid || s.mg_stationary)
+        return set_error(RIBO_ERR_UNSUPPORTED, "a batch runs the multigrid pressure solve: fluid and multigrid on, "
+                                               "the reference V-cycle or the conjugate-gradient tolerance mode");
     if (s.terrain_enabled || s.light_mode)
         return set_error(RIBO_ERR_UNSUPPORTED, "a batch runs a frozen terrain and light field (terrain_enabled 0, light_mode 0)");
     if (p.sun_heat > 0.0f && p.day_period >= 1.0f)
@@ -88,13 +112,12 @@ int check_world(const RiboWorld* w, const RiboWorld* first) {
         return set_error(RIBO_ERR_UNSUPPORTED, "a world whose coarse levels do not fit one workgroup cannot join a batch");
     if (!first) return RIBO_OK;
     const RiboWorldSettings& f = first->st;
-    auto clampss = [](uint32_t v) { return std::min(std::max(v, 1u), 4u); };
-    if (clampss(s.fluid_substep) != clampss(f.fluid_substep) ||
-        std::max(p.transport_every, 1u) != std::max(first->params.transport_every, 1u) ||
-        std::max(s.mg_cycles, 1u) != std::max(f.mg_cycles, 1u) || (s.ledger_check != 0) != (f.ledger_check != 0) ||
-        p.epoch != first->params.epoch || w->cur != first->cur || w->tcur != first->tcur || w->device != first->device)
-        return set_error(RIBO_ERR_ARG, "the worlds of a batch must share fluid_substep, transport_every, mg_cycles, "
-                                       "ledger_check, the epoch, the ping-pong state and the device");
+    const bool same = substep_of(w) == substep_of(first) && te_of(w) == te_of(first) && tol_of(w) == tol_of(first) &&
+                      (s.ledger_check != 0) == (f.ledger_check != 0) && w->device == first->device &&
+                      (tol_of(w) ? s.precision == f.precision : std::max(s.mg_cycles, 1u) == std::max(f.mg_cycles, 1u));
+    if (!same)
+        return set_error(RIBO_ERR_ARG, "the worlds of a batch must share fluid_substep, transport_every, the solver "
+                                       "(mg_cycles, or the tolerance mode's precision), ledger_check and the device");
     return RIBO_OK;
 }
 
@@ -103,13 +126,145 @@ RiboSimParams step_params(const RiboWorld* w, uint32_t substep) {
     RiboSimParams P = w->params;
     P.fluid_dt = P.dt * (float)substep;
     P.fluid_enabled = 1u;
-    P.epoch = 0u;  // each launch adds its step; the device base adds the chunk's first epoch
+    P.epoch = 0u;  // each launch adds its step; the world's device word adds the chunk's first epoch
     P.sun_now = P.uv_strength > 0.0f ? (P.day_period < 1.0f ? 1.0f : 0.0f) : 0.0f;
     P.sun_slope = 0.0f;
     return P;
 }
 
-template <class T> size_t align_up(size_t n) { return (n + alignof(T) + 255) & ~(size_t)255; }
+// ---- the tolerance mode ----
+template <class S> const std::vector<MgLevelT<S>>& levels_of(const RiboWorld* w);
+template <> const std::vector<MgLevelT<float>>& levels_of<float>(const RiboWorld* w) { return w->mgz; }
+template <> const std::vector<MgLevelT<bf16>>& levels_of<bf16>(const RiboWorld* w) { return w->mgh; }
+// The conjugate-gradient vectors of one world, as pcg_solve forms them.
+template <class S> CgVec<S> cg_vec(RiboWorld* w) {
+    const std::vector<MgLevelT<S>>& lv = levels_of<S>(w);
+    CgVec<S> V{};
+    V.n = w->F;
+    V.eps = w->levels[0].eps;
+    V.fl = w->mg_fluid;
+    V.x = w->p_a;
+    V.xlo = w->pcg_xlo;
+    V.xold = w->p_old;
+    V.b = w->div;
+    V.r[0] = w->pcg_r;
+    V.r[1] = w->pcg_r1;
+    V.s[0] = w->pcg_s0;
+    V.s[1] = w->pcg_s1;
+    V.pd = w->pcg_pd;
+    V.w = w->pcg_w;
+    V.u = lv[0].p;  // the preconditioner's level 0: z (binary32) or z16
+    V.zpre = lv[0].pp;
+    V.rs = sizeof(S) == 2 ? lv[0].rhs : nullptr;
+    return V;
+}
+// First level the symmetric V-cycle solves inside one workgroup
+// (vcycle_sym_inner); its tile levels are 1 .. sym_block - 1.
+size_t sym_block(const RiboWorld* w) { return std::min(std::max<size_t>(w->mg_block_start, 1), w->sym_coarse); }
+
+template <class S> struct TolTables {
+    std::vector<BCg<S>> cg;
+    std::vector<BMgPairT<S>> pair;  // depth-major
+    std::vector<BMgBlock<S>> block;
+};
+
+template <class S>
+void build_tol(RiboBatch* b, std::vector<BatchBlock>& blocks, std::vector<std::pair<Shape*, size_t>>& shape_at,
+               TolTables<S>& T) {
+    const uint32_t count = (uint32_t)b->worlds.size();
+    auto add_shape = [&](Shape* sh, BatchTolKernel k, int depth) {
+        const size_t first = blocks.size();
+        for (uint32_t wi = 0; wi < count; ++wi) {
+            const RiboWorld* w = b->worlds[wi];
+            uint32_t n = 0, gx = 0, gy = 0;
+            if (depth >= 0) {
+                const size_t l = (size_t)depth + 1;
+                if (l >= sym_block(w)) continue;
+                n = levels_of<S>(w)[l].n;
+            }
+            batch_tol_grid(k, w->F, n, &gx, &gy);
+            for (uint32_t by = 0; by < gy; ++by)
+                for (uint32_t bx = 0; bx < gx; ++bx) blocks.push_back(BatchBlock{wi, bx, by, gx | (gy << 16)});
+        }
+        sh->blocks = (uint32_t)(blocks.size() - first);
+        shape_at.push_back({sh, first});
+    };
+    uint32_t depth = 0;
+    for (RiboWorld* w : b->worlds) depth = std::max<uint32_t>(depth, (uint32_t)sym_block(w) - 1u);
+    add_shape(&b->cg16, BT_CG_INIT, -1);
+    add_shape(&b->cgshift, BT_SHIFT, -1);
+    add_shape(&b->cgtile, BT_CG_DOWN0, -1);
+    b->mgt.assign(depth, Shape{});
+    for (uint32_t d = 0; d < depth; ++d) add_shape(&b->mgt[d], BT_MG_DOWN, (int)d);
+    T.cg.resize(count);
+    T.pair.resize((size_t)depth * count);
+    T.block.resize(count);
+    for (uint32_t wi = 0; wi < count; ++wi) {
+        RiboWorld* w = b->worlds[wi];
+        const std::vector<MgLevelT<S>>& lv = levels_of<S>(w);
+        uint32_t tx = 0, ty = 0;
+        batch_tol_grid(BT_CG_DOWN0, w->F, 0, &tx, &ty);
+        BCg<S>& c = T.cg[wi];
+        c.V = cg_vec<S>(w);
+        c.C = lv[1];
+        c.st = w->pcg;
+        c.tol = w->st.mg_tol;
+        c.maxit = std::max(w->st.mg_max_cycles, 1u);
+        c.nit = b->nit() + wi;
+        c.nt2 = tx * ty;
+        c.nb16 = ((w->F + 15u) / 16u) * ((w->F + 15u) / 16u);
+        const size_t sb = sym_block(w);
+        for (size_t d = 0; d + 1 < sb; ++d) T.pair[d * count + wi] = BMgPairT<S>{lv[d + 1], lv[d + 2], &w->pcg->done};
+        size_t lds = 0;
+        T.block[wi].a = mg_block_args<S>(lv.data() + sb, (int)(w->sym_coarse - sb + 1), w->cinv, w->ccell, w->cm, &lds,
+                                         w->cmoving ? w->cm_dev : nullptr);
+        T.block[wi].done = &w->pcg->done;
+        b->tol_lds = std::max(b->tol_lds, lds);
+    }
+}
+
+// One solve of the tolerance mode over the batch: pcg_solve's launches for
+// V-cycles j = 0 .. J, J the most slots any world has for this chunk.
+template <class S> uint64_t record_solve(RiboBatch* b, uint32_t J) {
+    hipStream_t s = b->stream;
+    uint64_t n = 0;
+    auto T = [&](BatchTolKernel k, const void* tab, const Shape& sh, uint32_t j = 0, int stage = 0, int rd = 0,
+                 size_t lds = 0) {
+        launch_batch_tol<S>(s, k, tab, sh.map, sh.blocks, j, stage, rd, lds);
+        ++n;
+    };
+    T(BT_CG_INIT, b->cg, b->cg16);
+    T(BT_FIN_INIT, b->cg, b->one);
+    T(BT_SHIFT, b->cg, b->cgshift);
+    const uint32_t D = (uint32_t)b->mgt.size();
+    for (uint32_t j = 0; j <= J; ++j) {
+        const int stage = j == 0 ? 0 : (j == 1 ? 1 : 2);
+        const int rd = j == 0 ? 1 : (int)(j & 1u);
+        T(BT_CG_DOWN0, b->cg, b->cgtile, j, stage, rd);
+        for (uint32_t d = 0; d < D; ++d) T(BT_MG_DOWN, b->mgpair_t[d], b->mgt[d]);
+        T(BT_MG_BLOCK, b->mgblock_t, b->one, 0, 0, 0, b->tol_lds);
+        for (uint32_t d = D; d-- > 0;) T(BT_MG_UP, b->mgpair_t[d], b->mgt[d]);
+        T(BT_CG_UP0, b->cg, b->cgtile, j);
+    }
+    T(BT_CG_FINAL, b->cg, b->cg16);
+    T(BT_FIN_FINAL, b->cg, b->one);
+    return n;
+}
+
+// The relative parities (chem, temperature) the batch's worlds are at, or
+// -1 when they disagree.
+int relative_parity(const RiboBatch* b, int* c, int* t) {
+    for (size_t i = 0; i < b->worlds.size(); ++i) {
+        const int ci = b->worlds[i]->cur ^ b->cur0[i], ti = b->worlds[i]->tcur ^ b->tcur0[i];
+        if (i == 0) {
+            *c = ci;
+            *t = ti;
+        } else if (ci != *c || ti != *t) {
+            return -1;
+        }
+    }
+    return 0;
+}
 
 }  // namespace
 
@@ -129,15 +284,35 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
     RiboBatch* b = new RiboBatch();
     b->worlds.assign(worlds, worlds + count);
     const RiboWorld* w0 = worlds[0];
-    b->substep = std::min(std::max(w0->st.fluid_substep, 1u), 4u);
-    b->te = std::max(w0->params.transport_every, 1u);
+    b->substep = substep_of(w0);
+    b->te = te_of(w0);
+    b->period = b->substep / gcd_u(b->substep, b->te) * b->te;
     b->cycles = std::max(w0->st.mg_cycles, 1u);
     b->ledger = w0->st.ledger_check != 0;
+    b->tol = tol_of(w0);
+    b->mixed = b->tol && w0->st.precision == RIBO_PRECISION_MIXED;
     for (RiboWorld* w : b->worlds) {
         b->agg = b->agg || w->params.aggregation > 0.0f;
-        b->depth = std::max<uint32_t>(b->depth, (uint32_t)w->mg_block_start);
+        if (!b->tol) b->depth = std::max<uint32_t>(b->depth, (uint32_t)w->mg_block_start);
         b->params0.push_back(w->params);
         b->settings0.push_back(w->st);
+        b->cur0.push_back(w->cur);
+        b->tcur0.push_back(w->tcur);
+    }
+    int rc = RIBO_OK;
+    // the tolerance mode's coarsest inverses, built as a world's own first solve builds them
+    for (size_t i = 0; b->tol && i < b->worlds.size() && !rc; ++i) rc = batch_prepare_solve(b->worlds[i]);
+    for (RiboWorld* w : b->worlds) b->cinv0.push_back(w->cinv);
+    hipError_t he = hipSuccess;
+    if (!rc) {
+        std::lock_guard<std::recursive_mutex> lk(alloc_mutex());
+        he = hipMalloc((void**)&b->words, 2 * (size_t)count * sizeof(uint32_t));
+        if (he == hipSuccess) he = hipHostMalloc((void**)&b->host, 2 * (size_t)count * sizeof(uint32_t));
+        if (he != hipSuccess) rc = set_error(RIBO_ERR_OOM, std::string("batch words: ") + hipGetErrorString(he));
+    }
+    if (rc) {
+        ribo_batch_destroy(b);
+        return rc;
     }
     // ---- block maps (world-major) ----
     std::vector<BatchBlock> blocks;
@@ -166,118 +341,129 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
     add_shape(&b->one, BK_MG_BLOCK, -1);
     b->mg.resize(b->depth);
     for (uint32_t d = 0; d < b->depth; ++d) add_shape(&b->mg[d], BK_MG_DOWN, (int)d);
+    TolTables<float> tf;
+    TolTables<bf16> th;
+    if (b->tol && b->mixed) build_tol<bf16>(b, blocks, shape_at, th);
+    else if (b->tol) build_tol<float>(b, blocks, shape_at, tf);
     // ---- argument tables ----
-    std::vector<BTemperature> t_temp(4 * count);  // [cur][tcur]
-    std::vector<BForces> t_forces(2 * count);
-    std::vector<BVel> t_vel(4 * count);    // add_forces, diffuse, advect, vorticity
-    std::vector<BProj> t_proj(4 * count);  // divergence, subgrad, net_flux, boundaries
-    std::vector<BChem> t_chem(4 * count);  // [cur][tcur]
+    std::vector<BTemperature> t_temp(4 * (size_t)count);
+    std::vector<BForces> t_forces(2 * (size_t)count);
+    std::vector<BVel> t_vel(4 * (size_t)count);    // add_forces, diffuse, advect, vorticity
+    std::vector<BProj> t_proj(4 * (size_t)count);  // divergence, subgrad, net_flux, boundaries
+    std::vector<BChem> t_chem(4 * (size_t)count);
     std::vector<BMgPair> t_mg((size_t)b->depth * count);
-    std::vector<MgBlockArgs<float>> t_block(count);
+    std::vector<MgBlockArgs<float>> t_block(b->tol ? 0 : count);
     for (uint32_t wi = 0; wi < count; ++wi) {
         RiboWorld* w = b->worlds[wi];
         Geo g = w->geo();
         g.sun = nullptr;  // the sun of every step is step_params' (no table: a moving sun is refused)
         const RiboSimParams P = step_params(w, b->substep);
+        const uint32_t* ep = b->ep() + wi;
         for (int t = 0; t < 2; ++t) {
-            for (int c = 0; c < 2; ++c)
+            const int tw = b->tcur0[wi] ^ t;
+            for (int c = 0; c < 2; ++c) {
+                const int cw = b->cur0[wi] ^ c;
                 t_temp[(size_t)(c * 2 + t) * count + wi] = BTemperature{
-                    g, P, b->ep, w->vel_a, w->temp[t], w->redox[t], w->temp[1 - t], w->redox[1 - t], w->eaten, w->chem[c]};
-            t_forces[t * count + wi] = BForces{g, P, b->ep, w->temp[t], w->forces};
+                    g, P, ep, w->vel_a, w->temp[tw], w->redox[tw], w->temp[1 - tw], w->redox[1 - tw], w->eaten, w->chem[cw]};
+                t_chem[(size_t)(c * 2 + t) * count + wi] = BChem{
+                    g, P, ep, w->chem[cw], w->chem[1 - cw], w->vel_a, w->temp[tw], w->agg_e, w->ledger,
+                    std::min<uint32_t>((w->G * w->G + 255u) / 256u, LEDGER_MAX_BLOCKS)};
+            }
+            t_forces[t * (size_t)count + wi] = BForces{g, P, ep, w->temp[tw], w->forces};
         }
         const float decay = step_decay_factor(P.fluid_decay, P.fluid_dt);
-        t_vel[0 * count + wi] = BVel{g, P, decay, w->forces, w->vel_a, w->vel_b};
-        t_vel[1 * count + wi] = BVel{g, P, decay, nullptr, w->vel_b, w->vel_a};
-        t_vel[2 * count + wi] = BVel{g, P, decay, nullptr, w->vel_a, w->vel_b};
-        t_vel[3 * count + wi] = BVel{g, P, decay, nullptr, w->vel_b, w->vel_a};
-        t_proj[0 * count + wi] = BProj{g, w->vel_a, nullptr, w->div, w->p_a};
-        t_proj[1 * count + wi] = BProj{g, w->vel_a, w->vel_b, w->p_a, nullptr};
-        t_proj[2 * count + wi] = BProj{g, w->vel_b, nullptr, nullptr, w->net_flux};
-        t_proj[3 * count + wi] = BProj{g, w->vel_b, w->vel_a, nullptr, w->net_flux};
-        const uint32_t cells = w->G * w->G;
-        const uint32_t lb = std::min<uint32_t>((cells + 255u) / 256u, LEDGER_MAX_BLOCKS);
-        for (int c = 0; c < 2; ++c)
-            for (int t = 0; t < 2; ++t)
-                t_chem[(size_t)(c * 2 + t) * count + wi] =
-                    BChem{g, P, b->ep, w->chem[c], w->chem[1 - c], w->vel_a, w->temp[t], w->agg_e, w->ledger, lb};
+        t_vel[0 * (size_t)count + wi] = BVel{g, P, decay, w->forces, w->vel_a, w->vel_b};
+        t_vel[1 * (size_t)count + wi] = BVel{g, P, decay, nullptr, w->vel_b, w->vel_a};
+        t_vel[2 * (size_t)count + wi] = BVel{g, P, decay, nullptr, w->vel_a, w->vel_b};
+        t_vel[3 * (size_t)count + wi] = BVel{g, P, decay, nullptr, w->vel_b, w->vel_a};
+        t_proj[0 * (size_t)count + wi] = BProj{g, w->vel_a, nullptr, w->div, w->p_a};
+        t_proj[1 * (size_t)count + wi] = BProj{g, w->vel_a, w->vel_b, w->p_a, nullptr};
+        t_proj[2 * (size_t)count + wi] = BProj{g, w->vel_b, nullptr, nullptr, w->net_flux};
+        t_proj[3 * (size_t)count + wi] = BProj{g, w->vel_b, w->vel_a, nullptr, w->net_flux};
         for (uint32_t d = 0; d < b->depth; ++d)
             if (d < w->mg_block_start) t_mg[(size_t)d * count + wi] = BMgPair{w->mg[d], w->mg[d + 1]};
-        size_t lds = 0;
-        const size_t bs = w->mg_block_start;
-        t_block[wi] = mg_block_args<float>(w->mg.data() + bs, (int)(w->mg.size() - bs), nullptr, nullptr, 0, &lds);
-        b->block_lds = std::max(b->block_lds, lds);
-    }
-    // ---- one device allocation: the epoch word, maps, tables ----
-    size_t bytes = 256;
-    auto reserve = [&](size_t n) {
-        const size_t at = bytes;
-        bytes += (n + 255) & ~(size_t)255;
-        return at;
-    };
-    const size_t o_map = reserve(blocks.size() * sizeof(BatchBlock));
-    const size_t o_temp = reserve(t_temp.size() * sizeof(BTemperature));
-    const size_t o_forces = reserve(t_forces.size() * sizeof(BForces));
-    const size_t o_vel = reserve(t_vel.size() * sizeof(BVel));
-    const size_t o_proj = reserve(t_proj.size() * sizeof(BProj));
-    const size_t o_chem = reserve(t_chem.size() * sizeof(BChem));
-    const size_t o_mg = reserve(t_mg.size() * sizeof(BMgPair));
-    const size_t o_block = reserve(t_block.size() * sizeof(MgBlockArgs<float>));
-    {
-        std::lock_guard<std::recursive_mutex> lk(alloc_mutex());
-        hipError_t e = hipMalloc((void**)&b->dev, bytes);
-        if (e != hipSuccess) {
-            delete b;
-            return set_error(RIBO_ERR_OOM, std::string("hipMalloc: ") + hipGetErrorString(e));
+        if (!b->tol) {
+            size_t lds = 0;
+            const size_t bs = w->mg_block_start;
+            t_block[wi] = mg_block_args<float>(w->mg.data() + bs, (int)(w->mg.size() - bs), nullptr, nullptr, 0, &lds);
+            b->block_lds = std::max(b->block_lds, lds);
         }
     }
-    b->dev_bytes = bytes;
-    b->ep = (uint32_t*)b->dev;
-    // the tables hold the epoch pointer: fill it now that it is known
-    for (auto& a : t_temp) a.ep = b->ep;
-    for (auto& a : t_forces) a.ep = b->ep;
-    for (auto& a : t_chem) a.ep = b->ep;
-    int rc = RIBO_OK;
-    hipError_t he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    auto up = [&](size_t off, const void* src, size_t n) {
-        if (he == hipSuccess && n) he = hipMemcpyAsync(b->dev + off, src, n, hipMemcpyHostToDevice, b->stream);
+    // ---- one device allocation: maps and tables ----
+    size_t bytes = 0;
+    struct Up { size_t at; const void* src; size_t n; };
+    std::vector<Up> ups;
+    auto put = [&](const void* src, size_t n) {
+        const size_t at = bytes;
+        bytes += (n + 255) & ~(size_t)255;
+        ups.push_back(Up{at, src, n});
+        return at;
     };
-    up(o_map, blocks.data(), blocks.size() * sizeof(BatchBlock));
-    up(o_temp, t_temp.data(), t_temp.size() * sizeof(BTemperature));
-    up(o_forces, t_forces.data(), t_forces.size() * sizeof(BForces));
-    up(o_vel, t_vel.data(), t_vel.size() * sizeof(BVel));
-    up(o_proj, t_proj.data(), t_proj.size() * sizeof(BProj));
-    up(o_chem, t_chem.data(), t_chem.size() * sizeof(BChem));
-    up(o_mg, t_mg.data(), t_mg.size() * sizeof(BMgPair));
-    up(o_block, t_block.data(), t_block.size() * sizeof(MgBlockArgs<float>));
+    const size_t o_map = put(blocks.data(), blocks.size() * sizeof(BatchBlock));
+    const size_t o_temp = put(t_temp.data(), t_temp.size() * sizeof(BTemperature));
+    const size_t o_forces = put(t_forces.data(), t_forces.size() * sizeof(BForces));
+    const size_t o_vel = put(t_vel.data(), t_vel.size() * sizeof(BVel));
+    const size_t o_proj = put(t_proj.data(), t_proj.size() * sizeof(BProj));
+    const size_t o_chem = put(t_chem.data(), t_chem.size() * sizeof(BChem));
+    const size_t o_mg = put(t_mg.data(), t_mg.size() * sizeof(BMgPair));
+    const size_t o_block = put(t_block.data(), t_block.size() * sizeof(MgBlockArgs<float>));
+    size_t o_cg = 0, o_pair = 0, o_tblock = 0, pair_size = 0;
+    if (b->tol && b->mixed) {
+        o_cg = put(th.cg.data(), th.cg.size() * sizeof(BCg<bf16>));
+        o_pair = put(th.pair.data(), th.pair.size() * sizeof(BMgPairT<bf16>));
+        o_tblock = put(th.block.data(), th.block.size() * sizeof(BMgBlock<bf16>));
+        pair_size = sizeof(BMgPairT<bf16>);
+    } else if (b->tol) {
+        o_cg = put(tf.cg.data(), tf.cg.size() * sizeof(BCg<float>));
+        o_pair = put(tf.pair.data(), tf.pair.size() * sizeof(BMgPairT<float>));
+        o_tblock = put(tf.block.data(), tf.block.size() * sizeof(BMgBlock<float>));
+        pair_size = sizeof(BMgPairT<float>);
+    }
+    {
+        std::lock_guard<std::recursive_mutex> lk(alloc_mutex());
+        he = hipMalloc((void**)&b->dev, std::max<size_t>(bytes, 256));
+    }
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    for (const Up& u : ups)
+        if (he == hipSuccess && u.n) he = hipMemcpyAsync(b->dev + u.at, u.src, u.n, hipMemcpyHostToDevice, b->stream);
     if (he == hipSuccess) he = hipStreamSynchronize(b->stream);
     if (he == hipSuccess) he = hipEventCreateWithFlags(&b->batch_done, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
+    if (he == hipSuccess) he = hipEventRecord(b->staged, b->stream);
     b->world_done.assign(count, nullptr);
-    for (uint32_t i = 0; i < count && he == hipSuccess; ++i) he = hipEventCreateWithFlags(&b->world_done[i], hipEventDisableTiming);
-    if (he != hipSuccess) rc = set_error(RIBO_ERR_HIP, std::string("batch set-up: ") + hipGetErrorString(he));
-    if (rc) {
+    for (uint32_t i = 0; i < count && he == hipSuccess; ++i)
+        he = hipEventCreateWithFlags(&b->world_done[i], hipEventDisableTiming);
+    if (he != hipSuccess) {
+        rc = set_error(RIBO_ERR_HIP, std::string("batch set-up: ") + hipGetErrorString(he));
         ribo_batch_destroy(b);
         return rc;
     }
     const BatchBlock* map = (const BatchBlock*)(b->dev + o_map);
     for (auto& sa : shape_at) sa.first->map = map + sa.second;
     for (int t = 0; t < 2; ++t) {
-        for (int c = 0; c < 2; ++c) b->temperature[c][t] = (const BTemperature*)(b->dev + o_temp) + (size_t)(c * 2 + t) * count;
-        b->forces[t] = (const BForces*)(b->dev + o_forces) + t * count;
+        for (int c = 0; c < 2; ++c) {
+            b->temperature[c][t] = (const BTemperature*)(b->dev + o_temp) + (size_t)(c * 2 + t) * count;
+            b->chem[c][t] = (const BChem*)(b->dev + o_chem) + (size_t)(c * 2 + t) * count;
+        }
+        b->forces[t] = (const BForces*)(b->dev + o_forces) + (size_t)t * count;
     }
     const BVel* tv = (const BVel*)(b->dev + o_vel);
     b->add_forces = tv;
     b->diffuse = tv + count;
-    b->advect = tv + 2 * count;
-    b->vorticity = tv + 3 * count;
+    b->advect = tv + 2 * (size_t)count;
+    b->vorticity = tv + 3 * (size_t)count;
     const BProj* tp = (const BProj*)(b->dev + o_proj);
     b->divergence = tp;
     b->subgrad = tp + count;
-    b->netflux_t = tp + 2 * count;
-    b->boundaries = tp + 3 * count;
-    for (int c = 0; c < 2; ++c)
-        for (int t = 0; t < 2; ++t) b->chem[c][t] = (const BChem*)(b->dev + o_chem) + (size_t)(c * 2 + t) * count;
+    b->netflux_t = tp + 2 * (size_t)count;
+    b->boundaries = tp + 3 * (size_t)count;
     for (uint32_t d = 0; d < b->depth; ++d) b->mgpair.push_back((const BMgPair*)(b->dev + o_mg) + (size_t)d * count);
     b->mgblock = (const MgBlockArgs<float>*)(b->dev + o_block);
+    if (b->tol) {
+        b->cg = b->dev + o_cg;
+        for (size_t d = 0; d < b->mgt.size(); ++d) b->mgpair_t.push_back(b->dev + o_pair + d * count * pair_size);
+        b->mgblock_t = b->dev + o_tblock;
+    }
     *out = b;
     return RIBO_OK;
 }
@@ -288,9 +474,12 @@ int ribo_batch_destroy(RiboBatch* b) {
     for (hipEvent_t e : b->world_done)
         if (e) (void)hipEventDestroy(e);
     if (b->batch_done) (void)hipEventDestroy(b->batch_done);
+    if (b->staged) (void)hipEventDestroy(b->staged);
     {
         std::lock_guard<std::recursive_mutex> lk(alloc_mutex());
         if (b->dev) (void)hipFree(b->dev);
+        if (b->words) (void)hipFree(b->words);
+        if (b->host) (void)hipHostFree(b->host);
     }
     if (b->stream) (void)hipStreamDestroy(b->stream);
     delete b;
@@ -312,51 +501,86 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
             return set_error(RIBO_ERR_ARG, "a world's parameters or settings changed while it is in a batch");
         int rc = check_world(w, i ? b->worlds[0] : nullptr);
         if (rc) return rc;
+        if (w->params.epoch % b->period != b->worlds[0]->params.epoch % b->period)
+            return set_error(RIBO_ERR_ARG, "the worlds of a batch must be at the same phase of the fluid and transport "
+                                           "schedule (epoch modulo its period)");
     }
+    int rc_c = 0, rc_t = 0;
+    if (relative_parity(b, &rc_c, &rc_t) != 0)
+        return set_error(RIBO_ERR_ARG, "a world was stepped on its own by a different count of fluid or transport "
+                                       "steps than the others since the batch was built; build a new batch");
+    int cur = rc_c, tc = rc_t;
     hipStream_t s = b->stream;
     // after every world's queued work
     for (size_t i = 0; i < n; ++i) {
         BCHK(hipEventRecord(b->world_done[i], b->worlds[i]->stream));
         BCHK(hipStreamWaitEvent(s, b->world_done[i], 0));
     }
-    RiboWorld* w0 = b->worlds[0];
-    const uint32_t epoch0 = w0->params.epoch;
-    launch_set_u32(s, b->ep, epoch0);
-    int cur = w0->cur, tc = w0->tcur;
-    uint64_t launched = 1;
+    const uint32_t e0 = b->worlds[0]->params.epoch;
+    uint64_t launched = 0;
     auto L = [&](BatchKernel k, const void* tab, const Shape& sh, uint32_t step, size_t lds = 0) {
         launch_batch(s, k, tab, sh.map, sh.blocks, step, lds);
         ++launched;
     };
-    for (uint32_t i = 0; i < steps; ++i) {
-        const uint32_t e = epoch0 + i;
-        if (e % b->substep == 0) {  // fluid_step (src/world/mod.rs:1527-1588)
-            L(BK_TEMPERATURE, b->temperature[cur][tc], b->fluid, i);
-            tc = 1 - tc;  // copy_temperature
-            L(BK_FORCES, b->forces[tc], b->fluid, i);
-            L(BK_ADD_FORCES, b->add_forces, b->fluid, i);
-            L(BK_DIFFUSE, b->diffuse, b->fluid, i);
-            L(BK_ADVECT, b->advect, b->fluid, i);
-            L(BK_VORTICITY, b->vorticity, b->fluid, i);
-            L(BK_DIVERGENCE, b->divergence, b->fluid, i);
-            for (uint32_t c = 0; c < b->cycles; ++c) {  // the V-cycle (:1555-1575)
-                for (uint32_t d = 0; d < b->depth; ++d) L(BK_MG_DOWN, b->mgpair[d], b->mg[d], i);
-                L(BK_MG_BLOCK, b->mgblock, b->one, i, b->block_lds);
-                for (uint32_t d = b->depth; d-- > 0;) L(BK_MG_UP, b->mgpair[d], b->mg[d], i);
+    const uint32_t chunk = b->tol ? TOL_CHUNK : RIBO_MAX_STEPS_PER_CALL;
+    for (uint32_t first = 0; first < steps; first += chunk) {
+        const uint32_t m = std::min(chunk, steps - first);
+        // the words of this chunk: each world's epoch base and, in the
+        // tolerance mode, its iteration slots (chosen after the previous
+        // chunk's solves finished, as a world stepped alone chooses them)
+        uint32_t J = 0;
+        if (b->tol) {
+            BCHK(hipStreamSynchronize(s));
+            for (size_t i = 0; i < n; ++i) {
+                RiboWorld* w = b->worlds[i];
+                const int rc = batch_prepare_solve(w);
+                if (rc) return rc;
+                if (w->cinv != b->cinv0[i])
+                    return set_error(RIBO_ERR_ARG, "a world's terrain was replaced while it is in a batch; build a new batch");
+                const uint32_t slots = std::max(std::min(w->pcg_slots, std::max(w->st.mg_max_cycles, 1u)), 1u);
+                b->host[n + i] = slots;
+                J = std::max(J, slots);
             }
-            L(BK_SUBGRAD, b->subgrad, b->fluid, i);
-            L(BK_NET_FLUX, b->netflux_t, b->netflux, i);
-            L(BK_BOUNDARIES, b->boundaries, b->fluid, i);
-            L(BK_THERMAL, b->chem[cur][tc], b->grid, i);
         }
-        if (e % b->te == 0) {  // transport (:1594-1601)
-            if (b->agg) L(BK_AGG, b->chem[cur][tc], b->grid, i);
-            L(BK_CLEAR_NEXT, b->chem[cur][tc], b->clear, i);
-            L(BK_SCATTER, b->chem[cur][tc], b->grid, i);
-            cur = 1 - cur;  // transport_commit
-            if (b->ledger) {
-                L(BK_LEDGER, b->chem[cur][tc], b->ledger_blocks, i);
-                L(BK_LEDGER_FIN, b->chem[cur][tc], b->one, i);
+        BCHK(hipEventSynchronize(b->staged));  // the pinned words are free again
+        for (size_t i = 0; i < n; ++i) b->host[i] = b->worlds[i]->params.epoch + first;
+        BCHK(hipMemcpyAsync(b->words, b->host, (b->tol ? 2 : 1) * n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+        BCHK(hipEventRecord(b->staged, s));
+        ++launched;
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t e = e0 + first + i;
+            if (e % b->substep == 0) {  // fluid_step (src/world/mod.rs:1527-1588)
+                L(BK_TEMPERATURE, b->temperature[cur][tc], b->fluid, i);
+                tc = 1 - tc;  // copy_temperature
+                L(BK_FORCES, b->forces[tc], b->fluid, i);
+                L(BK_ADD_FORCES, b->add_forces, b->fluid, i);
+                L(BK_DIFFUSE, b->diffuse, b->fluid, i);
+                L(BK_ADVECT, b->advect, b->fluid, i);
+                L(BK_VORTICITY, b->vorticity, b->fluid, i);
+                L(BK_DIVERGENCE, b->divergence, b->fluid, i);
+                if (b->tol) {  // the tolerance mode (pcg_solve)
+                    launched += b->mixed ? record_solve<bf16>(b, J) : record_solve<float>(b, J);
+                } else {
+                    for (uint32_t c = 0; c < b->cycles; ++c) {  // the V-cycle (:1555-1575)
+                        for (uint32_t d = 0; d < b->depth; ++d) L(BK_MG_DOWN, b->mgpair[d], b->mg[d], i);
+                        L(BK_MG_BLOCK, b->mgblock, b->one, i, b->block_lds);
+                        for (uint32_t d = b->depth; d-- > 0;) L(BK_MG_UP, b->mgpair[d], b->mg[d], i);
+                    }
+                }
+                L(BK_SUBGRAD, b->subgrad, b->fluid, i);
+                L(BK_NET_FLUX, b->netflux_t, b->netflux, i);
+                L(BK_BOUNDARIES, b->boundaries, b->fluid, i);
+                L(BK_THERMAL, b->chem[cur][tc], b->grid, i);
+            }
+            if (e % b->te == 0) {  // transport (:1594-1601)
+                if (b->agg) L(BK_AGG, b->chem[cur][tc], b->grid, i);
+                L(BK_CLEAR_NEXT, b->chem[cur][tc], b->clear, i);
+                L(BK_SCATTER, b->chem[cur][tc], b->grid, i);
+                cur = 1 - cur;  // transport_commit
+                if (b->ledger) {
+                    L(BK_LEDGER, b->chem[cur][tc], b->ledger_blocks, i);
+                    L(BK_LEDGER_FIN, b->chem[cur][tc], b->one, i);
+                }
             }
         }
     }
@@ -364,13 +588,14 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
     b->launches += launched;
     // work queued on a world's stream from now on runs after the batch's steps
     BCHK(hipEventRecord(b->batch_done, s));
-    for (RiboWorld* w : b->worlds) {
+    for (size_t i = 0; i < n; ++i) {
+        RiboWorld* w = b->worlds[i];
         BCHK(hipStreamWaitEvent(w->stream, b->batch_done, 0));
         w->params.epoch += steps;
         w->params.fluid_dt = w->params.dt * (float)b->substep;
         w->params.fluid_enabled = 1u;
-        w->cur = cur;
-        w->tcur = tc;
+        w->cur = b->cur0[i] ^ cur;
+        w->tcur = b->tcur0[i] ^ tc;
         w->epoch_dev_known = false;  // the world's own epoch word did not follow
         w->fluid_was_on = true;
     }

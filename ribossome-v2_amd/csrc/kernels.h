@@ -197,6 +197,34 @@ enum BatchKernel {
 // BatchBlock[blocks], tab: device table of the kernel's argument type).
 void launch_batch(hipStream_t s, BatchKernel k, const void* tab, const BatchBlock* map, uint32_t blocks, uint32_t step,
                   size_t lds = 0);
+
+// The tolerance mode's solve over a batch (preconditioner storage S): per
+// world its conjugate-gradient vectors, level 1 of its preconditioner, its
+// solver state, tolerance and cap, and its iteration slots for this solve
+// (nit: the world's V-cycles are j = 0 .. nit; a batch launch for a larger
+// j leaves it alone, as the single world records no such V-cycle).
+template <class S> struct BCg {
+    CgVec<S> V;
+    MgLevelT<S> C;
+    PcgState* st;
+    float tol;
+    uint32_t maxit;
+    const uint32_t* nit;
+    uint32_t nt2;   // tiles of the level-0 V-cycle kernels (their partial sums)
+    uint32_t nb16;  // 16 x 16 blocks of the init and final kernels
+};
+template <class S> struct BMgPairT { MgLevelT<S> L, C; const uint32_t* done; };
+template <class S> struct BMgBlock { MgBlockArgs<S> a; const uint32_t* done; };
+enum BatchTolKernel {
+    BT_CG_INIT, BT_FIN_INIT, BT_SHIFT, BT_CG_DOWN0, BT_MG_DOWN, BT_MG_BLOCK, BT_MG_UP, BT_CG_UP0, BT_CG_FINAL, BT_FIN_FINAL
+};
+// j: the V-cycle (CG_DOWN0, CG_UP0), stage and rd as launch_cg_down0 takes them.
+template <class S>
+void launch_batch_tol(hipStream_t s, BatchTolKernel k, const void* tab, const BatchBlock* map, uint32_t blocks, uint32_t j,
+                      int stage, int rd, size_t lds = 0);
+// The grid of a tolerance-mode kernel for one world of fluid side n
+// (level_n: the multigrid level's side for MG_DOWN / MG_UP).
+void batch_tol_grid(BatchTolKernel k, uint32_t n, uint32_t level_n, uint32_t* gx, uint32_t* gy);
 // The grid of kernel k for one world (level_n: the multigrid level's side).
 void batch_grid(BatchKernel k, const Geo& g, uint32_t level_n, uint32_t* gx, uint32_t* gy);
 

@@ -1599,7 +1599,7 @@ __device__ __forceinline__ float round_s(float v, const bf16*) { return rb_as_f3
 // (k_pcg_fin), so no block signals or waits on another and the sums do not
 // depend on block scheduling.
 template <int N>
-__device__ __forceinline__ void block_partial(const double (&v)[N], double* part) {
+__device__ __forceinline__ void block_partial(const double (&v)[N], double* part, uint32_t blk) {
     __shared__ double sh[N][16];
     const int tid = threadIdx.y * blockDim.x + threadIdx.x;
     const int lane = tid & 63, wid = tid >> 6, nw = (blockDim.x * blockDim.y) >> 6;
@@ -1616,7 +1616,7 @@ __device__ __forceinline__ void block_partial(const double (&v)[N], double* part
     if (tid < N) {
         double t = 0.0;
         for (int w = 0; w < nw; ++w) t += sh[tid][w];
-        part[(size_t)N * (blockIdx.y * gridDim.x + blockIdx.x) + tid] = t;
+        part[(size_t)N * blk + tid] = t;
     }
 }
 
@@ -1721,7 +1721,7 @@ __device__ inline PcgScal pcg_advance(const PcgState* st, const PcgScal& a, cons
 // slot rd, shares the result through LDS and (workgroup 0) writes it to
 // slot 1 - rd and publishes done and near.
 template <int NT>
-__device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit) {
+__device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit, bool writer) {
     __shared__ PcgScal sh_sc;
     double t[6];
     sum_partials_wg<6, NT>(st->part, nb, t);
@@ -1729,7 +1729,7 @@ __device__ PcgScal pcg_step_wg(PcgState* st, int rd, uint32_t nb, float tol, uin
     if (tid == 0) {
         const PcgScal b = pcg_advance(st, st->sc[rd], t, tol, maxit);
         sh_sc = b;
-        if (blockIdx.x == 0 && blockIdx.y == 0) {
+        if (writer) {
             st->sc[1 - rd] = b;
             st->done = b.done;
             st->near = b.near;
@@ -1768,7 +1768,7 @@ __device__ __forceinline__ double apply_A64_diff(const uint8_t* fl, const float*
 // the start x + theta (x - xold) with theta = (r, q) / (q, q) has the least
 // residual on that line (theta = 1 is the linear extrapolation in time).
 template <class S>
-__global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
+__device__ __forceinline__ void b_cg_init(const uint3 blockIdx, const uint3 gridDim, const CgVec<S>& V, PcgState* st) {
     PCG_XY
     const bool extrap = V.xold != nullptr && st->hist >= 2u;
     double bb = 0.0, rr = 0.0, rq = 0.0, qq = 0.0;
@@ -1790,13 +1790,17 @@ __global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
         V.w[i] = 0.0f;
         V.xlo[i] = 0.0f;
     }
-    block_partial<4>({bb, rr, rq, qq}, st->part);
+    block_partial<4>({bb, rr, rq, qq}, st->part, blockIdx.y * gridDim.x + blockIdx.x);
+}
+template <class S>
+__global__ __launch_bounds__(256) void k_cg_init(CgVec<S> V, PcgState* st) {
+    b_cg_init<S>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), V, st);
 }
 
 // The start on the line: x += theta (x - xold), r -= theta q in fluid cells;
 // xold takes the old x; the search direction (q until now) starts at 0.
-__global__ __launch_bounds__(256) void k_pcg_shift(uint32_t cells, const uint8_t* __restrict__ fl, float* x, float* xlo,
-                                                   float* xold, float* r, float* pd, const PcgState* st) {
+__device__ __forceinline__ void b_pcg_shift(const uint3 blockIdx, uint32_t cells, const uint8_t* __restrict__ fl, float* x,
+                                            float* xlo, float* xold, float* r, float* pd, const PcgState* st) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
     if (i >= cells) return;
     const float th = (float)st->theta;
@@ -1810,6 +1814,10 @@ __global__ __launch_bounds__(256) void k_pcg_shift(uint32_t cells, const uint8_t
     }
     xold[i] = a;
     pd[i] = 0.0f;
+}
+__global__ __launch_bounds__(256) void k_pcg_shift(uint32_t cells, const uint8_t* __restrict__ fl, float* x, float* xlo,
+                                                   float* xold, float* r, float* pd, const PcgState* st) {
+    b_pcg_shift(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), cells, fl, x, xlo, xold, r, pd, st);
 }
 
 // Level-0 tiles of the tolerance mode: 64 x 64 cells per 1024-thread
@@ -1826,8 +1834,9 @@ static_assert((CG_T / 2) * (CG_T / 2) <= CG_NT, "one thread per coarse cell of a
 // restriction (or, for bfloat16 storage, half the sum over fluid children),
 // as k_mg_down.
 template <class S>
-__global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int stage, int rd,
-                                                   uint32_t nb, float tol, uint32_t maxit) {
+__device__ __forceinline__ void b_cg_down0(const uint3 blockIdx, const uint3 gridDim, const CgVec<S>& V,
+                                           const MgLevelT<S>& C, PcgState* st, int par, int stage, int rd, uint32_t nb,
+                                           float tol, uint32_t maxit) {
     // stage 0: the first V-cycle (the initial state in slot rd); later
     // stages first apply the previous V-cycle's inner products to the
     // scalars.  A restart (the recursive residual reached tol, the true one
@@ -1838,7 +1847,7 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
         if (!first && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) st->sc[1 - rd] = st->sc[rd];
         return;
     }
-    const PcgScal sc = first ? st->sc[rd] : pcg_step_wg<CG_NT>(st, rd, nb, tol, maxit);
+    const PcgScal sc = first ? st->sc[rd] : pcg_step_wg<CG_NT>(st, rd, nb, tol, maxit, blockIdx.x == 0 && blockIdx.y == 0);
     if (sc.done) return;
     const bool rst = !first && sc.restart != 0u;
     const bool upd = !first && !rst;
@@ -1949,6 +1958,12 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
         }
     }
 }
+template <class S>
+__global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par, int stage, int rd,
+                                                   uint32_t nb, float tol, uint32_t maxit) {
+    b_cg_down0<S>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), V, C,
+                  st, par, stage, rd, nb, tol, maxit);
+}
 
 // Level 0 of the V-cycle, upward: the bilinear prolongation added to the
 // pre-smoothed u written by k_cg_down0, and the symmetric
@@ -1957,7 +1972,8 @@ __global__ __launch_bounds__(CG_NT) void k_cg_down0(CgVec<S> V, MgLevelT<S> C, P
 // (from u as stored in S) and the block's partial sums of (r, u), (w, u),
 // (u, s), (p, w) and (r, r).
 template <class S>
-__global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par) {
+__device__ __forceinline__ void b_cg_up0(const uint3 blockIdx, const uint3 gridDim, const CgVec<S>& V, const MgLevelT<S>& C,
+                                         PcgState* st, int par) {
     if (st->done) return;
     const bool tr = st->near != 0u;  // also form ||b - A x||^2 (binary64)
     constexpr int H = 5, T = CG_T, NT = CG_NT;
@@ -2036,15 +2052,20 @@ __global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, Pcg
             tt += rt * rt;
         }
     }
-    block_partial<6>({gam, del, e, f, rr, tt}, st->part);
+    block_partial<6>({gam, del, e, f, rr, tt}, st->part, blockIdx.y * gridDim.x + blockIdx.x);
+}
+template <class S>
+__global__ __launch_bounds__(CG_NT) void k_cg_up0(CgVec<S> V, MgLevelT<S> C, PcgState* st, int par) {
+    b_cg_up0<S>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), V, C, st,
+                par);
 }
 
 // ||b - A x||^2 over fluid cells, for the reported residuals.
 template <class S>
-__global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit,
-                                                  uint32_t off) {
+__device__ __forceinline__ void b_cg_final(const uint3 blockIdx, const uint3 gridDim, const CgVec<S>& V, PcgState* st, int rd,
+                                           uint32_t nb, float tol, uint32_t maxit, uint32_t off) {
     // workgroup 0 applies the last V-cycle's inner products (slot rd to 1 - rd)
-    if (blockIdx.x == 0 && blockIdx.y == 0) (void)pcg_step_wg<256>(st, rd, nb, tol, maxit);
+    if (blockIdx.x == 0 && blockIdx.y == 0) (void)pcg_step_wg<256>(st, rd, nb, tol, maxit, true);
     PCG_XY
     // the refined iterate's residual and the stored pressure's, binary64
     double rr = 0.0, rh = 0.0;
@@ -2054,7 +2075,13 @@ __global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int 
         rr = r * r;
         rh = q * q;
     }
-    block_partial<2>({rr, rh}, st->part + off);  // after the partials read above
+    block_partial<2>({rr, rh}, st->part + off, blockIdx.y * gridDim.x + blockIdx.x);  // after the partials read above
+}
+template <class S>
+__global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int rd, uint32_t nb, float tol, uint32_t maxit,
+                                                  uint32_t off) {
+    b_cg_final<S>(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), V, st,
+                  rd, nb, tol, maxit, off);
 }
 
 // The solve's scalar state from the partials of the kernel before: one
@@ -2062,7 +2089,7 @@ __global__ __launch_bounds__(256) void k_cg_final(CgVec<S> V, PcgState* st, int 
 // k_cg_final (the statistics of the solve whose last state is in slot rd).
 enum PcgFin { FIN_INIT, FIN_FINAL };
 template <int W>
-__global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit, int rd, uint32_t off) {
+__device__ __forceinline__ void b_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit, int rd, uint32_t off) {
     constexpr int N = W == FIN_INIT ? 4 : 2;
     double t[N];
     sum_partials<N>(st->part + off, nb, t);
@@ -2110,6 +2137,10 @@ __global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, floa
         }
     }
 }
+template <int W>
+__global__ __launch_bounds__(256) void k_pcg_fin(PcgState* st, uint32_t nb, float tol, uint32_t maxit, int rd, uint32_t off) {
+    b_pcg_fin<W>(st, nb, tol, maxit, rd, off);
+}
 
 // Stationary tolerance mode: the reference V-cycle repeated until the
 // relative residual reaches the tolerance.  k_vc_resid leaves per-block
@@ -2133,7 +2164,7 @@ __global__ __launch_bounds__(256) void k_vc_resid(uint32_t n, float eps, const u
         bb = (double)b[i] * (double)b[i];
         rr = r * r;
     }
-    block_partial<2>({bb, rr}, st->part);
+    block_partial<2>({bb, rr}, st->part, blockIdx.y * gridDim.x + blockIdx.x);
 }
 enum VcFin { VC_BEGIN, VC_CHECK, VC_END };
 template <int W>
@@ -3358,6 +3389,64 @@ __global__ __launch_bounds__(1024) void kb_mg_block(const MgBlockArgs<float>* __
     BATCH_BLOCK(tab)
     b_mg_block<float, false>(bid, gdim, a, nullptr);
 }
+// tolerance mode
+template <class S>
+__global__ __launch_bounds__(256) void kb_cg_init(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_cg_init<S>(bid, gdim, a.V, a.st);
+}
+template <class S>
+__global__ __launch_bounds__(256) void kb_fin_init(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_pcg_fin<FIN_INIT>(a.st, a.nb16, a.tol, a.maxit, 0, 0u);
+}
+template <class S>
+__global__ __launch_bounds__(256) void kb_shift(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    if (a.V.xold) b_pcg_shift(bid, a.V.n * a.V.n, a.V.fl, a.V.x, a.V.xlo, a.V.xold, a.V.r[0], a.V.pd, a.st);
+}
+template <class S>
+__global__ __launch_bounds__(CG_NT) void kb_cg_down0(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                    uint32_t j, int stage, int rd) {
+    BATCH_BLOCK(tab)
+    if (j > *a.nit) return;
+    b_cg_down0<S>(bid, gdim, a.V, a.C, a.st, (int)(j & 1u), stage, rd, a.nt2, a.tol, a.maxit);
+}
+template <class S>
+__global__ __launch_bounds__(256) void kb_mg_down_sym(const BMgPairT<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_mg_down<S, true, 16>(bid, gdim, a.L, a.C, a.done);
+}
+template <class S>
+__global__ __launch_bounds__(256) void kb_mg_up_sym(const BMgPairT<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_mg_up<S, true, 16>(bid, gdim, a.L, a.C, a.done);
+}
+template <class S>
+__global__ __launch_bounds__(1024) void kb_mg_block_sym(const BMgBlock<S>* __restrict__ tab,
+                                                        const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    b_mg_block<S, true>(bid, gdim, a.a, a.done);
+}
+template <class S>
+__global__ __launch_bounds__(CG_NT) void kb_cg_up0(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map,
+                                                  uint32_t j) {
+    BATCH_BLOCK(tab)
+    if (j > *a.nit) return;
+    b_cg_up0<S>(bid, gdim, a.V, a.C, a.st, (int)(j & 1u));
+}
+template <class S>
+__global__ __launch_bounds__(256) void kb_cg_final(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    const int rd = (int)((*a.nit - 1u) & 1u);
+    b_cg_final<S>(bid, gdim, a.V, a.st, rd, a.nt2, a.tol, a.maxit, 6u * a.nt2);
+}
+template <class S>
+__global__ __launch_bounds__(256) void kb_fin_final(const BCg<S>* __restrict__ tab, const BatchBlock* __restrict__ map) {
+    BATCH_BLOCK(tab)
+    const int rd = (int)((*a.nit - 1u) & 1u);
+    b_pcg_fin<FIN_FINAL>(a.st, a.nb16, 0.0f, 0u, 1 - rd, 6u * a.nt2);
+}
 #undef BATCH_BLOCK
 
 // ---- the coarsest inverse under moving terrain (tolerance mode) --------------------------
@@ -3634,6 +3723,8 @@ void launch_mg_block(hipStream_t s, const MgLevelT<S>* lv, int count, bool sym, 
 }
 template MgBlockArgs<float> mg_block_args<float>(const MgLevelT<float>*, int, const double*, const int32_t*, int, size_t*,
                                                   const int*);
+template MgBlockArgs<bf16> mg_block_args<bf16>(const MgLevelT<bf16>*, int, const double*, const int32_t*, int, size_t*,
+                                                const int*);
 inline uint32_t pcg_blocks(uint32_t n) { return ((n + 15) / 16) * ((n + 15) / 16); }
 inline uint32_t cg_tiles(uint32_t n) { return (n + CG_T - 1) / CG_T; }
 template <class S> void launch_cg_init(hipStream_t s, const CgVec<S>& V, PcgState* st, float tol, uint32_t maxit) {
@@ -3786,6 +3877,38 @@ void launch_batch(hipStream_t s, BatchKernel k, const void* tab, const BatchBloc
         case BK_MG_DOWN: kb_mg_down<<<blocks, 256, 0, s>>>((const BMgPair*)tab, map); break;
         case BK_MG_UP: kb_mg_up<<<blocks, 256, 0, s>>>((const BMgPair*)tab, map); break;
         case BK_MG_BLOCK: kb_mg_block<<<blocks, 1024, lds, s>>>((const MgBlockArgs<float>*)tab, map); break;
+    }
+}
+template <class S>
+void launch_batch_tol(hipStream_t s, BatchTolKernel k, const void* tab, const BatchBlock* map, uint32_t blocks, uint32_t j,
+                      int stage, int rd, size_t lds) {
+    if (blocks == 0) return;
+    const BCg<S>* cg = (const BCg<S>*)tab;
+    switch (k) {
+        case BT_CG_INIT: kb_cg_init<S><<<blocks, B16, 0, s>>>(cg, map); break;
+        case BT_FIN_INIT: kb_fin_init<S><<<blocks, 256, 0, s>>>(cg, map); break;
+        case BT_SHIFT: kb_shift<S><<<blocks, 256, 0, s>>>(cg, map); break;
+        case BT_CG_DOWN0: kb_cg_down0<S><<<blocks, CG_NT, 0, s>>>(cg, map, j, stage, rd & 1); break;
+        case BT_MG_DOWN: kb_mg_down_sym<S><<<blocks, 256, 0, s>>>((const BMgPairT<S>*)tab, map); break;
+        case BT_MG_BLOCK: kb_mg_block_sym<S><<<blocks, 1024, lds, s>>>((const BMgBlock<S>*)tab, map); break;
+        case BT_MG_UP: kb_mg_up_sym<S><<<blocks, 256, 0, s>>>((const BMgPairT<S>*)tab, map); break;
+        case BT_CG_UP0: kb_cg_up0<S><<<blocks, CG_NT, 0, s>>>(cg, map, j); break;
+        case BT_CG_FINAL: kb_cg_final<S><<<blocks, B16, 0, s>>>(cg, map); break;
+        case BT_FIN_FINAL: kb_fin_final<S><<<blocks, 256, 0, s>>>(cg, map); break;
+    }
+}
+template void launch_batch_tol<float>(hipStream_t, BatchTolKernel, const void*, const BatchBlock*, uint32_t, uint32_t, int,
+                                      int, size_t);
+template void launch_batch_tol<bf16>(hipStream_t, BatchTolKernel, const void*, const BatchBlock*, uint32_t, uint32_t, int,
+                                     int, size_t);
+void batch_tol_grid(BatchTolKernel k, uint32_t n, uint32_t level_n, uint32_t* gx, uint32_t* gy) {
+    *gy = 1;
+    switch (k) {
+        case BT_CG_INIT: case BT_CG_FINAL: *gx = *gy = (n + 15) / 16; break;
+        case BT_FIN_INIT: case BT_FIN_FINAL: case BT_MG_BLOCK: *gx = 1; break;
+        case BT_SHIFT: *gx = blocks1((size_t)n * n, 256); break;
+        case BT_CG_DOWN0: case BT_CG_UP0: *gx = *gy = cg_tiles(n); break;
+        case BT_MG_DOWN: case BT_MG_UP: *gx = *gy = (level_n + 15) / 16; break;
     }
 }
 void batch_grid(BatchKernel k, const Geo& g, uint32_t level_n, uint32_t* gx, uint32_t* gy) {

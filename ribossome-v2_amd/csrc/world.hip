@@ -1028,7 +1028,7 @@ int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes)
 // slots (counted on the device) gives the next chunk twice the maximum + 4;
 // until 8 solves have run with the current settings every solve gets the
 // full cap.
-constexpr uint32_t SLOT_WINDOW = 32u, TOL_CHUNK = 8u;
+constexpr uint32_t SLOT_WINDOW = 32u;
 static_assert(SLOT_WINDOW <= PCG_HOST_RING_LEN, "the window lies in the host ring");
 static int choose_slots(RiboWorld* w) {
     const uint32_t maxit = std::max(w->st.mg_max_cycles, 1u);
@@ -1475,3 +1475,18 @@ int ribo_world_solver_residuals(RiboWorld* w, RiboSolverResiduals* out, int rese
 }
 
 }  // extern "C"
+
+namespace ribo {
+// The tolerance mode's set-up of a world before a chunk of batched steps:
+// its coarsest inverse (built when missing, as ribo_world_encode_steps
+// builds it) and its iteration slots for the chunk's solves.
+int batch_prepare_solve(RiboWorld* w) {
+    if (w->cinv_valid && w->cmoving != coarse_moving(w)) w->cinv_valid = false;
+    if (!w->cinv_valid) {
+        clear_graphs(w);
+        const int rc = build_coarse_inverse(w);
+        if (rc) return rc;
+    }
+    return choose_slots(w);
+}
+}  // namespace ribo

@@ -27,6 +27,10 @@ int set_error(int code, const std::string& msg);
 // pow(clamp(fluid_decay, 0, 1), dt * 60) of advect_velocity (fluid.wgsl:473),
 // evaluated on the host in binary64 once per step.
 float step_decay_factor(float decay, float fluid_dt);
+// Steps between two choices of the tolerance mode's iteration slots (a
+// graph holds one such chunk).
+constexpr uint32_t TOL_CHUNK = 8u;
+
 // Serialises device allocation and release with graph capture across threads.
 std::recursive_mutex& alloc_mutex();
 }  // namespace ribo
@@ -158,5 +162,11 @@ struct RiboWorld {
         return g;
     }
 };
+
+namespace ribo {
+// The tolerance mode's per-chunk set-up of a world stepped in a batch (its
+// coarsest inverse and iteration slots; world.hip).
+int batch_prepare_solve(RiboWorld* w);
+}  // namespace ribo
 
 #endif

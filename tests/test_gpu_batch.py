@@ -145,6 +145,75 @@ def test_batch_schedule_and_handoff_to_single_steps():
         w.close()
 
 
+def tolerance_profile(seed: int, precision: int):
+    p, s = batched_profile(seed)
+    s.mg_tol = 1e-5
+    s.mg_max_cycles = 60
+    s.precision = precision
+    return p, s
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_batch_tolerance_mode_matches_each_world_alone(precision):
+    """The tolerance mode's solve over a batch: each world takes its own
+    iteration slots (chosen every 8 steps as it would alone) and the batch
+    launches up to the largest; every field of every world is bit-identical
+    to the world stepped alone."""
+    steps = 20
+    print(f"case batch tolerance precision={precision}", flush=True)
+    sides = [128, 192, 256, 384, 128, 192, 256, 512]
+    bw, al, cfgs = [], [], []
+    for i, side in enumerate(sides):
+        cfg, inp = batched_inputs(300 + i, side)
+        p, s = tolerance_profile(300 + i, precision)
+        bw.append(make_world(cfg, 300 + i, inp, p, s))
+        al.append(make_world(cfg, 300 + i, inp, p, s))
+        cfgs.append(cfg)
+    b = ra.Batch(bw)
+    assert b.encode_steps(steps) == steps
+    b.sync()
+    for a in al:
+        a.run(steps)
+        a.sync()
+    for i, (w, a, cfg) in enumerate(zip(bw, al, cfgs)):
+        (it, rel), (ait, arel) = w.mg_stats(), a.mg_stats()
+        print(f"world {i} (side {cfg.grid_size}): batch {it} iterations rel {rel:.3e}; alone {ait} rel {arel:.3e}",
+              flush=True)
+        assert rel <= 1e-5 and w.ledger_status() is None
+        compare(f"world {i}", w, a, cfg)
+    b.close()
+    for w in bw + al:
+        w.close()
+
+
+def test_batch_worlds_at_different_epochs():
+    """Worlds admitted at different times step together (fluid and
+    transport every step): a world that has run 5 steps alone joins a fresh
+    one; each ends bit-identical to itself stepped alone with the same calls."""
+    print("case batch different epochs", flush=True)
+    cfg_a, inp_a = batched_inputs(401, 192)
+    cfg_b, inp_b = batched_inputs(402, 256)
+    pa, sa = tolerance_profile(401, 1)
+    pb, sb = tolerance_profile(402, 1)
+    a, a2 = make_world(cfg_a, 401, inp_a, pa, sa), make_world(cfg_a, 401, inp_a, pa, sa)
+    c, c2 = make_world(cfg_b, 402, inp_b, pb, sb), make_world(cfg_b, 402, inp_b, pb, sb)
+    a.run(5)
+    a2.run(5)
+    b = ra.Batch([a, c])
+    assert b.encode_steps(11) == 11
+    b.sync()
+    a2.run(11)
+    c2.run(11)
+    for w in (a2, c2):
+        w.sync()
+    assert a.params.epoch == 16 and c.params.epoch == 11
+    compare("joined late", a, a2, cfg_a)
+    compare("fresh", c, c2, cfg_b)
+    b.close()
+    for w in (a, a2, c, c2):
+        w.close()
+
+
 def test_batch_refuses_worlds_outside_its_profile():
     print("case batch_refusals", flush=True)
     cfg, inp = batched_inputs(7, 128)
@@ -153,6 +222,8 @@ def test_batch_refuses_worlds_outside_its_profile():
     tol = make_world(cfg, 7, inp, p, s)
     st = tol.settings
     st.mg_tol = 1e-5
+    st.mg_stationary = 1
+    st.mg_max_cycles = 40
     tol.settings = st
     sub = make_world(cfg, 7, inp, p, s)
     st = sub.settings
@@ -163,7 +234,7 @@ def test_batch_refuses_worlds_outside_its_profile():
     ps.sun_heat = 1.0
     ps.day_period = 8000.0
     sun.params = ps
-    for bad, what in ((tol, "tolerance mode"), (sun, "moving sun")):
+    for bad, what in ((tol, "stationary tolerance mode"), (sun, "moving sun")):
         with pytest.raises(ra.RiboError):
             ra.Batch([ok, bad])
         print(f"refused: {what}", flush=True)

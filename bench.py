@@ -731,10 +731,10 @@ def serving_scenes(args):
     return scenes
 
 
-def serve(scenes, budget, max_running=0, arrivals=None, threads=0):
+def serve(scenes, budget, max_running=0, arrivals=None, threads=0, batching=True):
     """Runs the scenes through the library's server; arrivals (seconds from
     the start) or all at once.  Returns (span, latencies, cell-steps)."""
-    srv = ra.Server(budget, max_running, threads)
+    srv = ra.Server(budget, max_running, threads, batching)
     n = len(scenes)
     at = arrivals if arrivals is not None else np.zeros(n)
     tickets, nxt = [], 0
@@ -768,13 +768,16 @@ def serve(scenes, budget, max_running=0, arrivals=None, threads=0):
 
 def serving(args):
     """The serving configuration: the same scene set (1) one scene at a time
-    (the plain per-scene path), (2) all submitted at once to the server
-    (continuous batching: every scene resident and stepped on its own
-    stream as soon as the budget admits it), (3) Poisson arrivals at --qps.
-    Latency = finish - submission, from the server's clock."""
+    on its own stream (the plain per-scene path), (2) all submitted at once,
+    every scene resident on its own stream as soon as the budget admits it,
+    (3) all at once with batching (the scenes stepped together, one launch
+    per kernel, the batch rebuilt as scenes join and finish), (4) Poisson
+    arrivals at --qps with batching.  Latency = finish - submission, from
+    the server's clock."""
     scenes = serving_scenes(args)
     budget = int(args.hbm_budget_gb * (1 << 30))
-    plain_span, _, work = serve(scenes, budget, max_running=1, threads=1)
+    plain_span, _, work = serve(scenes, budget, max_running=1, threads=1, batching=False)
+    streams_span, _, _ = serve(scenes, budget, threads=args.serve_threads, batching=False)
     batch_span, _, _ = serve(scenes, budget, threads=args.serve_threads)
     rng = np.random.default_rng(5001)
     arrivals = np.cumsum(rng.exponential(1.0 / args.qps, len(scenes)))
@@ -787,11 +790,13 @@ def serving(args):
             "throughput_cell_steps_per_s": work / span,
             "saturated_cell_steps_per_s": batch, "plain_per_scene_cell_steps_per_s": plain,
             "saturated_vs_plain": batch / plain,
+            "saturated_streams_cell_steps_per_s": work / streams_span,
+            "batched_vs_streams": streams_span / batch_span,
             "config": {"workload": f"{len(scenes)} scenes, sides log-uniform in [128, 1024], U(50, "
                                    f"{args.serve_steps_max}) steps each, mixed-precision tolerance-mode solver to 1e-5; "
                                    f"Poisson arrivals at {args.qps} per s; HBM budget {args.hbm_budget_gb} GiB",
                        "scenes": len(scenes), "qps": args.qps, "span_s": span, "plain_span_s": plain_span,
-                       "saturated_span_s": batch_span,
+                       "saturated_span_s": batch_span, "saturated_streams_span_s": streams_span,
                        "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default"),
                        "scheduler_threads": args.serve_threads or "library default"}}
 

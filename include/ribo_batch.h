@@ -56,6 +56,9 @@ RIBO_API int ribo_batch_sync(RiboBatch* b);
 /* Kernel launches recorded so far (the world-by-world path would record
  * about `count` times as many). */
 RIBO_API int ribo_batch_launches(const RiboBatch* b, uint64_t* launches);
+/* 1 when the world is inside the profile a batch runs, 0 when it is not
+ * (ribo_last_error says why), negative on a bad argument. */
+RIBO_API int ribo_batch_accepts(const RiboWorld* w);
 
 #ifdef __cplusplus
 }

@@ -6,11 +6,20 @@
  * stream drains, so the work of scenes that arrive while others run
  * overlaps on the device.
  *
+ * By default the server batches: scenes that fit the batch profile
+ * (ribo_batch_accepts) and share its schedule are stepped together by one
+ * scheduler thread, one launch per kernel for all of them (ribo_batch_*),
+ * the batch rebuilt as scenes join and finish; the others keep their own
+ * streams.  ribo_server_set_batching(s, 0) steps every scene on its own
+ * stream.
+ *
  * The reference has no counterpart: it steps one world from its main
  * thread (src/main.rs, src/world/mod.rs:1406-1632).  A scene here is what a
  * reference caller would build with World::new + uploads + encode_steps in
- * a loop; each scene's results are bit-identical to running its world alone
- * with the same chunking (min(64, steps left) per call).
+ * a loop; each scene's results are bit-identical to running its world
+ * alone, as long as no tolerance-mode solve runs out of its iteration slots
+ * (ribo_world_solver_stats counts those; the slots are chosen from the
+ * world's own history either way).
  *
  * Threading: one caller thread at a time per server; the scheduler threads
  * are internal (each world is stepped by one of them only).  A scene's
@@ -62,6 +71,9 @@ typedef struct RiboSceneStatus {
 RIBO_API int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, uint32_t threads, RiboServer** out);
 /* Stops the scheduler and frees every scene. */
 RIBO_API int ribo_server_destroy(RiboServer* s);
+/* on = 1 (the default): batch the scenes that fit a batch; 0: one stream
+ * per scene.  Applies to scenes admitted afterwards. */
+RIBO_API int ribo_server_set_batching(RiboServer* s, int on);
 /* Queues a scene (first in, first admitted).  A scene larger than the whole
  * budget is refused with RIBO_ERR_OOM. */
 RIBO_API int ribo_server_submit(RiboServer* s, const RiboScene* scene, uint64_t* ticket);

@@ -619,6 +619,11 @@ int ribo_batch_sync(RiboBatch* b) {
     return RIBO_OK;
 }
 
+int ribo_batch_accepts(const RiboWorld* w) {
+    if (!w) return set_error(RIBO_ERR_ARG, "null world");
+    return check_world(w, nullptr) == RIBO_OK ? 1 : 0;
+}
+
 int ribo_batch_launches(const RiboBatch* b, uint64_t* launches) {
     if (!b || !launches) return set_error(RIBO_ERR_ARG, "null argument");
     *launches = b->launches;

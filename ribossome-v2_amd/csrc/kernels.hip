@@ -3297,7 +3297,9 @@ __global__ __launch_bounds__(256) void k_ledger_fin(LedgerDev* led, uint32_t nb,
     const uint32_t bgd = __builtin_amdgcn_readfirstlane(bb.gdim);                                          \
     const uint3 bid = make_uint3(__builtin_amdgcn_readfirstlane(bb.bx), __builtin_amdgcn_readfirstlane(bb.by), 0u); \
     const uint3 gdim = make_uint3(bgd & 0xFFFFu, bgd >> 16, 1u);                                          \
-    const auto& a = tab[bw];
+    const auto& a = tab[bw];                                                                               \
+    (void)bid;                                                                                             \
+    (void)gdim;
 
 __global__ __launch_bounds__(256) void kb_temperature(const BTemperature* __restrict__ tab, const BatchBlock* __restrict__ map,
                                                       uint32_t step) {

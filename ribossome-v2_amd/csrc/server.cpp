@@ -1,9 +1,13 @@
 // server.cpp - the serving form: scenes admitted by a device-memory budget,
-// each world resident on its own HIP stream, scheduler threads feeding every
-// admitted world with chunks of up to 64 steps as its stream drains.
+// their worlds resident in HBM.  Scenes that fit a batch and share its
+// schedule are stepped together, one launch per kernel for all of them
+// (ribo_batch_*), by scheduler thread 0, the batch rebuilt when a scene joins
+// or finishes; the others are fed chunks of up to 64 steps on their own
+// streams as the streams drain.
 //
-// Host code only: it drives worlds through the C ABI of world.hip, so a
-// scene's work is exactly what ribo_world_encode_steps records for it alone.
+// Host code only: it drives worlds through the C ABI of world.hip and
+// batch.hip, whose kernels compute for a scene in a batch what they compute
+// for it alone.
 
 #include <stdint.h>
 #include <string.h>
@@ -12,12 +16,15 @@
 #include <chrono>
 #include <condition_variable>
 #include <deque>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <string>
 #include <thread>
 #include <unordered_map>
 #include <vector>
 
+#include "../../include/ribo_batch.h"
 #include "../../include/ribo_server.h"
 
 namespace {
@@ -58,6 +65,7 @@ struct RiboServer {
     std::vector<std::vector<Scene*>> inbox;  // admitted scenes not yet picked up, per thread
     std::vector<uint32_t> load;              // scenes each thread is running or about to start
     std::chrono::steady_clock::time_point t0;
+    bool batching = true;                    // batchable scenes go to thread 0, stepped in batches
 };
 
 namespace {
@@ -101,21 +109,58 @@ void admit(RiboServer* s) {
         sc->charged = true;
         ++s->admitted;
         sc->st.state = RIBO_SCENE_RUNNING;  // from here on only its thread touches its world
-        const size_t k = (size_t)(std::min_element(s->load.begin(), s->load.end()) - s->load.begin());
+        const size_t k = s->batching ? 0 : (size_t)(std::min_element(s->load.begin(), s->load.end()) - s->load.begin());
         ++s->load[k];
         s->inbox[k].push_back(sc);
     }
 }
 
+// Scenes stepped together: the same schedule (fluid_substep, transport_every
+// and the phase of the epoch in it) and solver (tolerance mode, precision,
+// V-cycles), and the same ledger setting.
+using GroupKey = std::tuple<uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t, uint32_t>;
+struct Group {
+    std::vector<Scene*> scenes;  // scenes with steps left
+    RiboBatch* batch = nullptr;
+    bool dirty = true;           // the batch does not hold exactly `scenes`
+};
+
+GroupKey group_key(const Scene* sc) {
+    const RiboWorldSettings& st = sc->req.settings;
+    const RiboSimParams& p = sc->req.params;
+    const uint32_t sub = std::min(std::max(st.fluid_substep, 1u), 4u), te = std::max(p.transport_every, 1u);
+    uint32_t a = sub, b = te;
+    while (b) {
+        const uint32_t t = a % b;
+        a = b;
+        b = t;
+    }
+    const uint32_t period = sub / a * te;
+    const uint32_t done = sc->req.steps - sc->left;
+    const bool tol = st.mg_tol > 0.0f;
+    return GroupKey{sub, te, (p.epoch + done) % period, tol ? 1u : 0u, tol ? st.precision : 0u,
+                    tol ? 0u : std::max(st.mg_cycles, 1u), st.ledger_check ? 1u : 0u};
+}
+
+void fail_scene(RiboServer* s, Scene* sc, int rc, size_t me) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    sc->st.state = RIBO_SCENE_FAILED;
+    sc->st.error = rc;
+    sc->error = ribo_last_error();
+    --s->admitted;  // its memory stays charged until release
+    --s->load[me];
+}
+
 void scheduler(RiboServer* s, size_t me) {
-    std::vector<Scene*> running, starting;
+    std::vector<Scene*> running, starting, draining;
+    std::map<GroupKey, Group> groups;
     for (;;) {
         {
             std::unique_lock<std::mutex> lk(s->mu);
             if (s->stop) break;
             admit(s);
             starting.swap(s->inbox[me]);
-            if (starting.empty() && running.empty()) {
+            if (starting.empty() && running.empty() && groups.empty() && draining.empty()) {
                 s->cv.wait_for(lk, std::chrono::milliseconds(2));
                 continue;
             }
@@ -133,11 +178,92 @@ void scheduler(RiboServer* s, size_t me) {
                 --s->load[me];
             } else {
                 sc->st.t_admit = seconds_since(s->t0);
-                running.push_back(sc);
+                if (s->batching && sc->left > 0 && ribo_batch_accepts(sc->world) == 1) {
+                    Group& g = groups[group_key(sc)];
+                    g.scenes.push_back(sc);
+                    g.dirty = true;
+                } else {
+                    running.push_back(sc);
+                }
             }
         }
         starting.clear();
         bool fed = false;
+        // the batches: every scene of a group steps as many steps as the
+        // group's scene with the fewest left (at most 64)
+        for (auto it = groups.begin(); it != groups.end();) {
+            Group& g = it->second;
+            if (g.dirty && g.batch) {
+                ribo_batch_destroy(g.batch);
+                g.batch = nullptr;
+            }
+            if (g.scenes.empty()) {
+                it = groups.erase(it);
+                continue;
+            }
+            if (!g.batch) {
+                std::vector<RiboWorld*> ws;
+                for (Scene* sc : g.scenes) ws.push_back(sc->world);
+                if (ribo_batch_create(ws.data(), (uint32_t)ws.size(), &g.batch) != RIBO_OK) {
+                    // not batchable together after all: their own streams
+                    g.batch = nullptr;
+                    for (Scene* sc : g.scenes) running.push_back(sc);
+                    it = groups.erase(it);
+                    continue;
+                }
+                g.dirty = false;
+            }
+            uint32_t chunk = RIBO_MAX_STEPS_PER_CALL;
+            for (Scene* sc : g.scenes) chunk = std::min(chunk, sc->left);
+            const int n = ribo_batch_encode_steps(g.batch, chunk);
+            if (n <= 0) {
+                const int rc = n < 0 ? n : RIBO_ERR_ARG;
+                for (Scene* sc : g.scenes) fail_scene(s, sc, rc, me);
+                ribo_batch_destroy(g.batch);
+                it = groups.erase(it);
+                s->cv.notify_all();
+                continue;
+            }
+            fed = true;
+            std::vector<Scene*> keep;
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                for (Scene* sc : g.scenes) {
+                    sc->left -= (uint32_t)n;
+                    sc->st.steps_done += (uint32_t)n;
+                    if (sc->left > 0) keep.push_back(sc);
+                    else draining.push_back(sc);
+                }
+            }
+            if (keep.size() != g.scenes.size()) g.dirty = true;
+            g.scenes.swap(keep);
+            ++it;
+        }
+        // scenes whose last batched steps are queued: done once they ran
+        for (size_t i = 0; i < draining.size();) {
+            Scene* sc = draining[i];
+            const int idle = ribo_world_idle(sc->world);
+            if (idle == 0) {
+                ++i;
+                continue;
+            }
+            {
+                std::lock_guard<std::mutex> lk(s->mu);
+                if (idle < 0) {
+                    sc->st.state = RIBO_SCENE_FAILED;
+                    sc->st.error = idle;
+                    sc->error = ribo_last_error();
+                } else {
+                    sc->st.state = RIBO_SCENE_DONE;
+                    sc->st.t_done = seconds_since(s->t0);
+                }
+                --s->admitted;
+                --s->load[me];
+            }
+            s->cv.notify_all();
+            draining.erase(draining.begin() + (ptrdiff_t)i);
+            fed = true;
+        }
         for (size_t i = 0; i < running.size();) {
             Scene* sc = running[i];
             const int idle = ribo_world_idle(sc->world);
@@ -178,6 +304,8 @@ void scheduler(RiboServer* s, size_t me) {
         }
         if (!fed) std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
+    for (auto& kv : groups)
+        if (kv.second.batch) ribo_batch_destroy(kv.second.batch);
 }
 
 }  // namespace
@@ -196,6 +324,13 @@ int ribo_server_create(uint64_t hbm_budget, uint32_t max_running, uint32_t threa
     s->load.assign(threads, 0u);
     for (uint32_t k = 0; k < threads; ++k) s->threads.emplace_back(scheduler, s, (size_t)k);
     *out = s;
+    return RIBO_OK;
+}
+
+int ribo_server_set_batching(RiboServer* s, int on) {
+    if (!s || (on != 0 && on != 1)) return RIBO_ERR_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batching = on != 0;
     return RIBO_OK;
 }
 

@@ -350,6 +350,8 @@ def lib() -> ctypes.CDLL:
         "ribo_batch_run": (I, [P, U32]),
         "ribo_batch_sync": (I, [P]),
         "ribo_batch_launches": (I, [P, ctypes.POINTER(ctypes.c_uint64)]),
+        "ribo_batch_accepts": (I, [P]),
+        "ribo_server_set_batching": (I, [P, I]),
         "ribo_server_create": (I, [ctypes.c_uint64, U32, U32, ctypes.POINTER(P)]),
         "ribo_server_destroy": (I, [P]),
         "ribo_server_submit": (I, [P, P, ctypes.POINTER(ctypes.c_uint64)]),
@@ -785,19 +787,25 @@ class Batch:
         _check(lib().ribo_batch_launches(self._h, ctypes.byref(n)), "batch_launches")
         return n.value
 
+    @staticmethod
+    def accepts(world: "World") -> bool:
+        return _check(lib().ribo_batch_accepts(world._h), "batch_accepts") == 1
+
 
 SCENE_QUEUED, SCENE_RUNNING, SCENE_DONE, SCENE_FAILED = range(4)
 
 
 class Server:
     """The serving form (ribo_server_*): scenes admitted by an HBM budget,
-    resident on their own streams, stepped by the library's scheduler
-    threads."""
+    resident in HBM; with batching (the default) the scenes that fit a
+    batch are stepped together, one launch per kernel, the others on their
+    own streams by the library's scheduler threads."""
 
-    def __init__(self, hbm_budget: int, max_running: int = 0, threads: int = 0):
+    def __init__(self, hbm_budget: int, max_running: int = 0, threads: int = 0, batching: bool = True):
         self._h = ctypes.c_void_p()
         _check(lib().ribo_server_create(int(hbm_budget), int(max_running), int(threads), ctypes.byref(self._h)),
                "server_create")
+        _check(lib().ribo_server_set_batching(self._h, 1 if batching else 0), "server_set_batching")
 
     def close(self):
         if self._h:

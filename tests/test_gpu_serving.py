@@ -79,22 +79,26 @@ def run_server(srv, reqs):
 
 def test_scenes_identical_alone_and_in_the_mix():
     """Each scene's buffers after its steps are bit-identical whether it ran
-    alone (one scene admitted at a time) or overlapped with five others of
-    other sizes and solver modes, and equal to a plain World run with the
-    same chunking."""
+    alone (one scene admitted at a time, on its own stream) or overlapped
+    with five others of other sizes and solver modes -- on their own streams,
+    or batched (three batches: the reference V-cycle and the tolerance mode
+    in each precision, rebuilt as scenes finish) -- and equal to a plain
+    World run with the same chunking."""
     print("case serving alone vs mix", flush=True)
     reqs = [scene_request(*sc) for sc in SCENES]
     budget = 64 << 30
-    alone, peak1 = run_server(ra.Server(budget, max_running=1, threads=1), reqs)
-    mixed, peakn = run_server(ra.Server(budget, threads=4), reqs)
-    print(f"peak running: alone {peak1}, mix {peakn}", flush=True)
-    assert peak1 == 1 and peakn >= 3
-    for k, (a, b) in enumerate(zip(alone, mixed)):
-        assert np.array_equal(a[ra.CHEM], b[ra.CHEM]), f"scene {k}: chem differs"
-        for which in (ra.VELOCITY, ra.PRESSURE, ra.TEMP):
-            assert bit_diff(a[which], b[which]) == 0, f"scene {k}: {ra.BUFFER_NAMES[which]} differs"
-        cfg, p, s, inp = reqs[k]
-        assert ra.Ledger.from_cells(a[ra.CHEM]).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
+    alone, peak1 = run_server(ra.Server(budget, max_running=1, threads=1, batching=False), reqs)
+    streams, peaks = run_server(ra.Server(budget, threads=4, batching=False), reqs)
+    batched, peakb = run_server(ra.Server(budget, threads=4), reqs)
+    print(f"peak running: alone {peak1}, streams {peaks}, batched {peakb}", flush=True)
+    assert peak1 == 1 and peaks >= 3 and peakb >= 3
+    for mixed, how in ((streams, "streams"), (batched, "batched")):
+        for k, (a, b) in enumerate(zip(alone, mixed)):
+            assert np.array_equal(a[ra.CHEM], b[ra.CHEM]), f"scene {k} ({how}): chem differs"
+            for which in (ra.VELOCITY, ra.PRESSURE, ra.TEMP):
+                assert bit_diff(a[which], b[which]) == 0, f"scene {k} ({how}): {ra.BUFFER_NAMES[which]} differs"
+            cfg, p, s, inp = reqs[k]
+            assert ra.Ledger.from_cells(b[ra.CHEM]).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
     cfg, p, s, inp = reqs[2]
     w = ra.World(cfg, SCENES[2][1])
     load(w, inp, p, s)

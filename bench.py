@@ -832,6 +832,10 @@ def main():
     ap.add_argument("--config", default="world", choices=["world", "batched", "serving"],
                     help="world: the 2048^2 headline; batched: many independent worlds; serving: Poisson scene arrivals")
     ap.add_argument("--worlds", type=int, default=64)
+    ap.add_argument("--line-batched-steps", type=int, default=100,
+                    help="steps of the shortened batched configuration in the headline line (0 skips it)")
+    ap.add_argument("--line-serving-scenes", type=int, default=16,
+                    help="scenes of the shortened serving configuration in the headline line (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--scenes", type=int, default=24)
     ap.add_argument("--qps", type=float, default=1.0)
@@ -954,6 +958,29 @@ def main():
     note("agents done")
     default_profile = default_profile_measure(cfg, inputs, args) if args.default_steps > 0 else None
     note("default profile done")
+    batched_rec = serving_rec = None
+    if args.line_batched_steps > 0:
+        # the batched configuration, shortened: 64 worlds, one launch per
+        # kernel against one stream per world
+        sub = argparse.Namespace(**vars(args))
+        sub.worlds, sub.batch_steps = 64, args.line_batched_steps
+        r = batched(sub)
+        batched_rec = {"value": r["value"], "unit": r["unit"], "ms_per_step": r["ms_per_step"], "steps": r["steps"],
+                       "per_stream": r["per_stream"], "one_launch_vs_per_stream": r["one_launch_vs_per_stream"],
+                       "launches_per_step": r["launches_per_step"], "workload": r["config"]["workload"],
+                       "grid_cells": r["config"]["grid_cells"]}
+        note("batched done")
+    if args.line_serving_scenes > 0:
+        # the serving configuration, shortened: fewer scenes of at most 1000 steps
+        sub = argparse.Namespace(**vars(args))
+        sub.scenes, sub.serve_steps_max, sub.qps = args.line_serving_scenes, 1000, 0.5
+        r = serving(sub)
+        serving_rec = {k: r[k] for k in ("p99", "throughput_cell_steps_per_s", "saturated_cell_steps_per_s",
+                                          "plain_per_scene_cell_steps_per_s", "saturated_vs_plain",
+                                          "saturated_streams_cell_steps_per_s", "batched_vs_streams")}
+        serving_rec["p50"] = r["value"]
+        serving_rec["workload"] = r["config"]["workload"]
+        note("serving done")
     plain = {}
     if args.vcycle_steps > 0:
         # the stationary V-cycle, on a bounded number of steps (each step runs
@@ -1020,6 +1047,8 @@ def main():
         "terrain": terrain,
         "agents": agents,
         "reference_default_profile": default_profile,
+        "batched": batched_rec,
+        "serving": serving_rec,
         "cpu_baseline": cpu,
         "kernel_avg_us": kernels,
     }

@@ -53,6 +53,9 @@ RIBO_API int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps);
 /* ribo_batch_encode_steps in chunks of RIBO_MAX_STEPS_PER_CALL. */
 RIBO_API int ribo_batch_run(RiboBatch* b, uint32_t steps);
 RIBO_API int ribo_batch_sync(RiboBatch* b);
+/* 1 when every step recorded so far has finished, 0 while work runs,
+ * negative on error. */
+RIBO_API int ribo_batch_idle(RiboBatch* b);
 /* Kernel launches recorded so far (the world-by-world path would record
  * about `count` times as many). */
 RIBO_API int ribo_batch_launches(const RiboBatch* b, uint64_t* launches);

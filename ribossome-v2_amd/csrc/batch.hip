@@ -87,6 +87,17 @@ struct RiboBatch {
     const void* cg = nullptr;            // BCg<S>[count]
     std::vector<const void*> mgpair_t;   // BMgPairT<S>[count] per depth
     const void* mgblock_t = nullptr;     // BMgBlock<S>[count]
+    // captured chunks: replayed when the same chunk comes again
+    struct GraphEntry {
+        uint32_t m, J, phase;
+        int cur, tc, cur_end, tc_end;
+        uint64_t launches;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    struct ChunkKey { uint32_t m, J, phase; int cur, tc; };
+    std::vector<ChunkKey> seen;  // chunks recorded once, eagerly: the second time they are captured
+    bool use_graphs = true;
     // what each world looked like at creation (parameters and settings must not change)
     std::vector<RiboSimParams> params0;
     std::vector<RiboWorldSettings> settings0;
@@ -291,6 +302,7 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
     b->ledger = w0->st.ledger_check != 0;
     b->tol = tol_of(w0);
     b->mixed = b->tol && w0->st.precision == RIBO_PRECISION_MIXED;
+    for (const RiboWorld* w : b->worlds) b->use_graphs = b->use_graphs && w->st.no_graphs == 0;
     for (RiboWorld* w : b->worlds) {
         b->agg = b->agg || w->params.aggregation > 0.0f;
         if (!b->tol) b->depth = std::max<uint32_t>(b->depth, (uint32_t)w->mg_block_start);
@@ -471,6 +483,7 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
 int ribo_batch_destroy(RiboBatch* b) {
     if (!b) return set_error(RIBO_ERR_ARG, "null batch");
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
     for (hipEvent_t e : b->world_done)
         if (e) (void)hipEventDestroy(e);
     if (b->batch_done) (void)hipEventDestroy(b->batch_done);
@@ -547,6 +560,42 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
         BCHK(hipMemcpyAsync(b->words, b->host, (b->tol ? 2 : 1) * n * sizeof(uint32_t), hipMemcpyHostToDevice, s));
         BCHK(hipEventRecord(b->staged, s));
         ++launched;
+        // the chunk's launches: replayed from a graph captured the first time
+        // the same chunk (length, slots, parities, schedule phase) comes
+        const uint32_t phase = (e0 + first) % b->period;
+        RiboBatch::GraphEntry* ge = nullptr;
+        if (b->use_graphs)
+            for (auto& g : b->graphs)
+                if (g.m == m && g.J == J && g.phase == phase && g.cur == cur && g.tc == tc) ge = &g;
+        if (ge) {
+            BCHK(hipGraphLaunch(ge->exec, s));
+            launched += ge->launches;
+            cur = ge->cur_end;
+            tc = ge->tc_end;
+            continue;
+        }
+        const int cur_start = cur, tc_start = tc;
+        const uint64_t before = launched;
+        // a chunk is captured the second time it comes (capture and
+        // instantiation cost more than one eager recording)
+        bool capture = false;
+        if (b->use_graphs) {
+            for (const auto& k : b->seen)
+                if (k.m == m && k.J == J && k.phase == phase && k.cur == cur && k.tc == tc) capture = true;
+            if (!capture) {
+                if (b->seen.size() >= 256) b->seen.clear();
+                b->seen.push_back(RiboBatch::ChunkKey{m, J, phase, cur, tc});
+            }
+        }
+        std::unique_lock<std::recursive_mutex> cap_lock(alloc_mutex(), std::defer_lock);
+        if (capture) {
+            cap_lock.lock();
+            if (b->graphs.size() >= 64) {
+                for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+                b->graphs.clear();
+            }
+            BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        }
         for (uint32_t i = 0; i < m; ++i) {
             const uint32_t e = e0 + first + i;
             if (e % b->substep == 0) {  // fluid_step (src/world/mod.rs:1527-1588)
@@ -582,6 +631,18 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
                     L(BK_LEDGER_FIN, b->chem[cur][tc], b->one, i);
                 }
             }
+        }
+        if (capture) {
+            hipGraph_t graph = nullptr;
+            const hipError_t ce = hipStreamEndCapture(s, &graph);
+            if (ce != hipSuccess) return set_error(RIBO_ERR_HIP, std::string("batch capture: ") + hipGetErrorString(ce));
+            RiboBatch::GraphEntry g{m, J, phase, cur_start, tc_start, cur, tc, launched - before, nullptr};
+            const hipError_t ie = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) return set_error(RIBO_ERR_HIP, std::string("batch graph: ") + hipGetErrorString(ie));
+            cap_lock.unlock();
+            b->graphs.push_back(g);
+            BCHK(hipGraphLaunch(g.exec, s));
         }
     }
     BCHK(hipGetLastError());
@@ -622,6 +683,14 @@ int ribo_batch_sync(RiboBatch* b) {
 int ribo_batch_accepts(const RiboWorld* w) {
     if (!w) return set_error(RIBO_ERR_ARG, "null world");
     return check_world(w, nullptr) == RIBO_OK ? 1 : 0;
+}
+
+int ribo_batch_idle(RiboBatch* b) {
+    if (!b) return set_error(RIBO_ERR_ARG, "null batch");
+    const hipError_t e = hipStreamQuery(b->stream);
+    if (e == hipSuccess) return 1;
+    if (e == hipErrorNotReady) return 0;
+    return set_error(RIBO_ERR_HIP, std::string("batch stream: ") + hipGetErrorString(e));
 }
 
 int ribo_batch_launches(const RiboBatch* b, uint64_t* launches) {

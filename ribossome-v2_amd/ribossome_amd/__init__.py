@@ -349,6 +349,7 @@ def lib() -> ctypes.CDLL:
         "ribo_batch_encode_steps": (I, [P, U32]),
         "ribo_batch_run": (I, [P, U32]),
         "ribo_batch_sync": (I, [P]),
+        "ribo_batch_idle": (I, [P]),
         "ribo_batch_launches": (I, [P, ctypes.POINTER(ctypes.c_uint64)]),
         "ribo_batch_accepts": (I, [P]),
         "ribo_server_set_batching": (I, [P, I]),
@@ -781,6 +782,9 @@ class Batch:
 
     def sync(self):
         _check(lib().ribo_batch_sync(self._h), "batch_sync")
+
+    def idle(self) -> bool:
+        return _check(lib().ribo_batch_idle(self._h), "batch_idle") == 1
 
     def launches(self) -> int:
         n = ctypes.c_uint64()

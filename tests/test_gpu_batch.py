@@ -214,6 +214,37 @@ def test_batch_worlds_at_different_epochs():
         w.close()
 
 
+@pytest.mark.parametrize("tol", [False, True])
+def test_batch_graph_replays_match_eager(tol):
+    """A batch captures each chunk (its length, slots, parities and schedule
+    phase) as a graph the first time and replays it after: over several
+    calls with host work in between, every world matches the same batch
+    recorded eagerly (worlds with no_graphs)."""
+    print(f"case batch graphs tol={tol}", flush=True)
+    sides = [128, 192, 256, 128, 384]
+    runs = []
+    for no_graphs in (0, 1):
+        ws = []
+        for i, side in enumerate(sides):
+            cfg, inp = batched_inputs(500 + i, side)
+            p, s = tolerance_profile(500 + i, 1) if tol else batched_profile(500 + i)
+            s.no_graphs = no_graphs
+            ws.append((make_world(cfg, 500 + i, inp, p, s), cfg))
+        b = ra.Batch([w for w, _ in ws])
+        for _ in range(3):
+            assert b.encode_steps(16) == 16
+            b.sync()
+            _ = ws[0][0].download(ra.PRESSURE)  # host work between replays
+        runs.append((b, ws))
+    for (wa, cfg), (wb, _) in zip(runs[0][1], runs[1][1]):
+        compare("graphs vs eager", wa, wb, cfg)
+        assert wa.ledger_status() is None
+    for b, ws in runs:
+        b.close()
+        for w, _ in ws:
+            w.close()
+
+
 def test_batch_refuses_worlds_outside_its_profile():
     print("case batch_refusals", flush=True)
     cfg, inp = batched_inputs(7, 128)

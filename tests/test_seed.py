@@ -130,3 +130,41 @@ def test_splitmix64_published_vectors():
     r = so.SplitMix(1234567)
     assert [r.next() for _ in range(5)] == [6457827717110365317, 3203168211198807973, 9817491932198370423,
                                             4593380528125082431, 16408922859458223821]
+
+
+def _lattice_point_value(seed: int, pts) -> float:
+    """fbm_field at a pixel whose coordinate is a lattice point of every
+    octave: the amplitude-weighted lattice values (amplitudes 0.5 x 0.65^o,
+    six octaves), normalised by the amplitudes' sum; each lattice value is a
+    splitmix64 draw keyed by octave and lattice coordinates
+    (src/world/mod.rs:2046-2078)."""
+    M = (1 << 64) - 1
+    num, norm, amp = 0.0, 0.0, 0.5
+    for o in range(6):
+        ix, iy = pts(o)
+        r = so.SplitMix(seed ^ ((o * 0x9E3779B97F4A7C15) & M) ^ ((ix * 0xD6E8FEB86659FD93) & M)
+                        ^ ((iy * 0xCA5A826395121157) & M))
+        num += amp * float(r.f32())
+        norm += amp
+        amp *= 0.65
+    return num / norm
+
+
+def test_fbm_field_at_lattice_points():
+    """At x = 0 every octave samples its lattice point (0, 0); at x = m/4
+    octave o samples (2^o, 0), and at y = m/4 (0, 2^o): the field there is
+    the weighted mean of those lattice draws, which pins the octave count,
+    the persistence, the normalisation and each hash term."""
+    m, seed = 32, 0x1234567
+    f = so.fbm_field(m, seed)
+    assert abs(float(f[0]) - _lattice_point_value(seed, lambda o: (0, 0))) < 1e-6
+    assert abs(float(f[m // 4]) - _lattice_point_value(seed, lambda o: (1 << o, 0))) < 1e-6
+    assert abs(float(f[(m // 4) * m]) - _lattice_point_value(seed, lambda o: (0, 1 << o))) < 1e-6
+    assert abs(float(f[(m // 4) * m + m // 4]) - _lattice_point_value(seed, lambda o: (1 << o, 1 << o))) < 1e-6
+
+
+def test_seed_cells_density_zero_is_empty():
+    """With density 0 the expected count is 0 and the stochastic rounding
+    never adds a monomer."""
+    assert not so.seed_cells(32, 5, 0.0, 1.0).any()
+    assert not ra.seed_cells(ra.WorldConfig(32, 16), 5, 0.0, 1.0).any()

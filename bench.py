@@ -580,7 +580,7 @@ def pmc_traffic(kernel, mode):
         if "<" in cols[0] and name.startswith("k_mg_") and want not in cols[0]:
             continue
         try:
-            k, n = float(cols[2]) + float(cols[3]), int(cols[4])
+            k, n = float(cols[2]) + float(cols[3]), int(cols[4].split()[0])  # "kept of all" dispatches
         except ValueError:
             continue
         kib += k * n

@@ -841,8 +841,8 @@ def main():
     ap.add_argument("--worlds", type=int, default=64)
     ap.add_argument("--line-batched-steps", type=int, default=100,
                     help="steps of the shortened batched configuration in the headline line (0 skips it)")
-    ap.add_argument("--line-serving-scenes", type=int, default=16,
-                    help="scenes of the shortened serving configuration in the headline line (0 skips it)")
+    ap.add_argument("--line-serving-scenes", type=int, default=24,
+                    help="scenes of the serving configuration in the headline line (0 skips it)")
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--scenes", type=int, default=24)
     ap.add_argument("--qps", type=float, default=1.0)
@@ -978,9 +978,10 @@ def main():
                        "grid_cells": r["config"]["grid_cells"]}
         note("batched done")
     if args.line_serving_scenes > 0:
-        # the serving configuration, shortened: fewer scenes of at most 1000 steps
+        # the serving configuration as --config serving runs it (its scene
+        # count, step range and arrival rate)
         sub = argparse.Namespace(**vars(args))
-        sub.scenes, sub.serve_steps_max, sub.qps = args.line_serving_scenes, 1000, 0.5
+        sub.scenes = args.line_serving_scenes
         r = serving(sub)
         serving_rec = {k: r[k] for k in ("p99", "throughput_cell_steps_per_s", "saturated_cell_steps_per_s",
                                           "plain_per_scene_cell_steps_per_s", "saturated_vs_plain",

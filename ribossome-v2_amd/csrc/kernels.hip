@@ -108,37 +108,6 @@ __device__ __forceinline__ F4 rng_f4(uint32_t key, uint32_t epoch, uint32_t stre
     return u4_to_f4(pcg4d({key, epoch, stream, seed}));
 }
 
-// PCG4D for a run of streams s0, s0+1, ... with (key, epoch, seed) fixed:
-// the LCG images of key, epoch and seed and the first cross-lane update of x
-// do not depend on the stream, so they are computed once and each draw costs
-// 7 of the 12 32-bit multiplies.  Bit-identical to pcg4d({key, epoch, s, seed}).
-struct PcgRun {
-    uint32_t y0, w0, x1, x1s, z0;  // z0: LCG image of the current stream
-    __device__ __forceinline__ PcgRun(uint32_t key, uint32_t epoch, uint32_t stream0, uint32_t seed) {
-        const uint32_t x0 = key * 1664525u + 1013904223u;
-        y0 = epoch * 1664525u + 1013904223u;
-        w0 = seed * 1664525u + 1013904223u;
-        x1 = x0 + y0 * w0;
-        x1s = x1 ^ (x1 >> 16);
-        z0 = stream0 * 1664525u + 1013904223u;
-    }
-    __device__ __forceinline__ U4 at_current() const {
-        uint32_t y1 = y0 + z0 * x1;
-        uint32_t z1 = z0 + x1 * y1;
-        uint32_t w1 = w0 + y1 * z1;
-        y1 ^= y1 >> 16;
-        z1 ^= z1 >> 16;
-        w1 ^= w1 >> 16;
-        U4 r;
-        r.x = x1s + y1 * w1;
-        r.y = y1 + z1 * r.x;
-        r.z = z1 + r.x * r.y;
-        r.w = w1 + r.y * r.z;
-        return r;
-    }
-    __device__ __forceinline__ void advance() { z0 += 1664525u; }
-};
-
 // ---- small vector helpers -------------------------------------------------------
 __device__ __forceinline__ float2 f2(float x, float y) { return make_float2(x, y); }
 __device__ __forceinline__ float2 add2(float2 a, float2 b) { return f2(a.x + b.x, a.y + b.y); }
@@ -2866,37 +2835,68 @@ __device__ float2 smooth_start(const RiboSimParams& P, uint32_t x0, uint32_t y0,
     return f2(1.0f - rb_abs(2.0f * vx - 1.0f), 1.0f - rb_abs(2.0f * vy - 1.0f));
 }
 
-// transport_scatter (transport.wgsl:119-406).  One thread per grid cell of a
-// 16 x 16 tile.  The tile's cells are dealt to threads in order of their
-// mover count (a 64-bucket counting sort in LDS), so the 64 threads of a
-// wave carry similar loads, and each thread runs every mover of its cell,
-// channel after channel, in one loop: a wave costs about its largest cell
-// rather than the sum over channels of each channel's largest cell.  Each
-// mover adds its unit to its destination at once: within SC_R cells of the
-// tile (all destinations unless the flow moves more than ~2 cells per step)
-// into an LDS window that reaches HBM as one coalesced atomic add per
-// non-empty (cell, channel) at the end; farther destinations add to HBM
-// directly.  The blocked test (rock, or more monomers than capacity) of the
-// 16 destinations a cell's movers can reach is evaluated once per cell.
-// Integer sums do not depend on order, so the result is the reference's
-// exactly.
+// transport_scatter (transport.wgsl:119-406) over a 16 x 16 tile of grid
+// cells per workgroup, in three phases.
+//  1. One thread per cell forms what the cell's movers share: the advection
+//     displacement, the hop, settle and binding probabilities, the pressure
+//     weights and Kawasaki acceptances of the four hop directions, and the
+//     blocked and in-window masks of the 16 destination bins.  They go to an
+//     LDS record per cell.
+//  2. The tile's (cell, channel) runs that have movers are sorted by mover
+//     count (a counting sort in LDS, with the runs of cells under the rubble
+//     jitter in a second class) and dealt to the threads in that order, 256
+//     at a time, so the 64 runs of a wave have about the same length and
+//     seldom mix jitter and plain runs.
+//  3. A thread takes each run dealt to it: the photoactivation and decay of
+//     its channel, the cohesion weights of its active movers, then its
+//     movers.  Mover k of channel c draws PCG4D at key idx * 4 + c and stream
+//     S_MOVE + k (S_DISPERSE + k for the jitter).  With key, epoch and seed
+//     fixed, the first half of the hash is a polynomial in k (its y and z
+//     words affine, its w word quadratic), so a run steps it by additions and
+//     a draw costs the four multiplies of the second half.
+// Each mover adds its unit to its destination at once: within SC_R cells of
+// the tile into an LDS window that reaches HBM as one coalesced atomic add
+// per non-empty (cell, channel) at the end; farther destinations add to HBM
+// directly.  Integer sums do not depend on order, so the result is the
+// reference's exactly.
 constexpr int SC_T = 16, SC_R = 3, SC_W = SC_T + 2 * SC_R;
-constexpr int SC_TS = 24;            // LDS row stride of the window flags (bytes, a multiple of 4)
-constexpr uint32_t SC_BUCKETS = 64;  // sort key: movers / 2, capped; one bucket per lane of wave 0
+constexpr int SC_TS = 24;                   // LDS row stride of the window flags (bytes, a multiple of 4)
+constexpr uint32_t SC_MB = 64;              // mover-count buckets per class
+constexpr uint32_t SC_BUCKETS = 2 * SC_MB;  // plain runs, then runs under the rubble jitter
 static_assert(SMOOTH_CELL == (uint32_t)SC_T, "the smooth_start lattice is staged per tile");
-constexpr uint32_t LCG_A = 1664525u;  // the PCG4D input LCG multiplier
+constexpr uint32_t LCG_A = 1664525u, LCG_C = 1013904223u;  // the PCG4D input LCG
+constexpr float SC_U24 = 1.0f / 16777216.0f;
 __device__ __forceinline__ uint32_t movers_of(uint32_t w) {
     return rb_minu((w & CHEM_STATE_MASK) + (w >> 16), MAX_MOVERS_PER_CH);
 }
+// For r = (u >> 8) * 2^-24 and p in [0, 1]: r < p exactly when
+// (u >> 8) < ceil(p * 2^24); p * 2^24 is exact.
+__device__ __forceinline__ uint32_t u24_threshold(float p) { return (uint32_t)__builtin_ceilf(p * 16777216.0f); }
+
+// What the movers of one cell share (written in phase 1, read in phase 3).
+// 80 bytes, so that five workgroups fit in a CU's LDS.  A hop's Kawasaki
+// acceptance is exp(-k_e |drop|) for active movers when the drop is
+// positive and for spent ones when it is negative, else 1: one value per
+// hop and two sign bits.
+struct ScCell {
+    float4 a;   // displacement x, y; smooth_start x, y
+    float4 b;   // p_bound, |displacement|, light at the cell, hop-or-settle threshold | jitter << 31 (bits)
+    float4 pr;  // pressure weights of the hops +x, -x, +y, -y
+    float4 ac;  // exp(-k_e |drop|) of each hop (1 without aggregation)
+    uint4 d;    // base_x, base_y, direct | blocked << 16, hop threshold | positive drops << 24 | negative drops << 28
+};
+constexpr uint32_t SC_JITTER_BIT = 1u << 31;  // the thresholds are at most 2^24
+
 __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
-    __shared__ float4 s_w[4 * 256];        // hop weights of active movers, [channel][thread]
+    __shared__ ScCell s_cell[256];
     __shared__ uint4 s_acc4[SC_W * SC_W];  // [window cell][channel]
     __shared__ uint32_t s_hist[SC_BUCKETS];
+    __shared__ uint32_t s_nruns;
+    __shared__ uint16_t s_run[1024];              // sorted runs: cell * 4 + channel
     __shared__ uint32_t s_tf4[SC_W * SC_TS / 4];  // window flags, rows of SC_TS bytes: 1 rock, 2 over capacity
     __shared__ float2 s_lat[9];
-    __shared__ uint8_t s_cell[256];  // tile cell of each thread
     uint32_t* const s_acc = reinterpret_cast<uint32_t*>(s_acc4);
     uint8_t* const s_tf = reinterpret_cast<uint8_t*>(s_tf4);
     if (ep) P.epoch += *ep;
@@ -2918,33 +2918,10 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
     }
     if (tid < SC_BUCKETS) s_hist[tid] = 0u;
     if (tid < 9u) s_lat[tid] = lattice_rand(P, blockIdx.x + tid % 3u, blockIdx.y + tid / 3u);
-    uint32_t key = 0u;
-    {
-        const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
-        if (x < G && y < G) {
-            const uint4 w = *reinterpret_cast<const uint4*>(chem + 4u * (y * G + x));
-            key = movers_of(w.x) + movers_of(w.y) + movers_of(w.z) + movers_of(w.w);
-        }
-    }
-    const uint32_t bucket = rb_minu(key >> 1, SC_BUCKETS - 1u);
     __syncthreads();
-    const uint32_t rank = atomicAdd(&s_hist[bucket], 1u);
-    __syncthreads();
-    if (tid < SC_BUCKETS) {  // exclusive scan of the bucket counts across wave 0
-        const uint32_t h = s_hist[tid];
-        uint32_t v = h;
-        for (uint32_t off = 1; off < SC_BUCKETS; off <<= 1) {
-            const uint32_t u = __shfl_up(v, off, 64);
-            if (tid >= off) v += u;
-        }
-        s_hist[tid] = v - h;
-    }
-    __syncthreads();
-    s_cell[s_hist[bucket] + rank] = (uint8_t)tid;
-    __syncthreads();
-    const uint32_t cell = s_cell[tid];
 
-    const uint32_t x = blockIdx.x * 16 + (cell & 15u), y = blockIdx.y * 16 + (cell >> 4);
+    // ---- phase 1: the cell's shared quantities ----
+    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
     const uint32_t idx = y * G + x;
     uint32_t src_total = 0u;
     uint4 cw = make_uint4(0u, 0u, 0u, 0u);
@@ -2953,6 +2930,7 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         src_total = (cw.x & CHEM_STATE_MASK) + (cw.x >> 16) + (cw.y & CHEM_STATE_MASK) + (cw.y >> 16) +
                     (cw.z & CHEM_STATE_MASK) + (cw.z >> 16) + (cw.w & CHEM_STATE_MASK) + (cw.w >> 16);
     }
+    bool jitter_on = false;
     if (src_total != 0u) {
         const float cell_w = (float)g.wupc;
         const uint32_t g_src = g.gamma[idx];
@@ -3011,7 +2989,7 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
 
         float p_bound = 0.0f;
         float ac0 = 1.0f, ac1 = 1.0f, ac2 = 1.0f, ac3 = 1.0f;
-        float as0 = 1.0f, as1 = 1.0f, as2 = 1.0f, as3 = 1.0f;
+        uint32_t signs = 0u;
         if (P.aggregation > 0.0f && src_total <= cap_self) {  // transport.wgsl:228-260
             const float e0 = (float)agg_e[idx];
             const float e1 = x + 1u < G ? (float)agg_e[idx + 1u] : 0.0f;
@@ -3024,20 +3002,20 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             const float k_e = P.aggregation * room / t_rel;
             // en - e0 is exactly -(e0 - en), so one exp of -k_e |e0 - en|
             // serves the side that is positive
-            auto acc_of = [&](float en, float& acc, float& accs) {
+            auto acc_of = [&](float en, float& acc, uint32_t bit) {
                 const float drop = e0 - en;
-                const float v = rb_expf(-k_e * rb_abs(drop));
-                acc = drop > 0.0f ? v : acc;
-                accs = drop < 0.0f ? v : accs;
+                acc = rb_expf(-k_e * rb_abs(drop));
+                signs |= (drop > 0.0f ? 1u : 0u) << bit;
+                signs |= (drop < 0.0f ? 1u : 0u) << (bit + 4u);
             };
-            acc_of(e1, ac0, as0);
-            acc_of(e2, ac1, as1);
-            acc_of(e3, ac2, as2);
-            acc_of(e4, ac3, as3);
+            acc_of(e1, ac0, 0u);
+            acc_of(e2, ac1, 1u);
+            acc_of(e3, ac2, 2u);
+            acc_of(e4, ac3, 3u);
             p_bound = 1.0f - rb_expf(-k_e * rb_min(rb_max(e0 - 1.0f, 0.0f), 48.0f));
         }
         const float dlen = len2(disp);
-        const bool jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
+        jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
         // smooth_start is a function of the cell alone; every bound mover of
         // the cell starts there (transport.wgsl:95-117,290-296)
         float2 bound_start = f2(0.0f, 0.0f);
@@ -3093,105 +3071,186 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
                 direct |= (uint32_t)(blk || fast) << b;
             }
         }
+        // hop: r < p_diff; settle: r < p_diff + p_settle (transport.wgsl:337-357)
+        ScCell& R = s_cell[tid];
+        R.a = make_float4(disp.x, disp.y, bound_start.x, bound_start.y);
+        R.b = make_float4(p_bound, dlen, light_t, rb_as_f32(u24_threshold(p_diff + p_settle) | (jitter_on ? SC_JITTER_BIT : 0u)));
+        R.pr = make_float4(pr0, pr1, pr2, pr3);
+        R.ac = make_float4(ac0, ac1, ac2, ac3);
+        R.d = make_uint4((uint32_t)base_x, (uint32_t)base_y, direct | (blocked << 16),
+                         u24_threshold(p_diff) | (signs << 24));
+    }
 
-        // Per channel: photoactivation and decay, then the movers' range in
-        // the cell's loop.  Mover k of channel c draws PCG4D at key
-        // idx * 4 + c and stream S_MOVE + k (S_DISPERSE + k for the jitter):
-        // the LCG images of key and stream are affine in c and k.
-        const PcgRun base(idx * 4u, P.epoch, S_MOVE, P.seed);
-        uint32_t mend[4], aend[4], zb[4];
-        uint32_t t0 = 0u;
-        for (uint32_t ch = 0; ch < 4; ++ch) {
-            const uint32_t slot = idx * 4u + ch;
-            const uint32_t v_ch = ch == 0 ? cw.x : (ch == 1 ? cw.y : (ch == 2 ? cw.z : cw.w));
-            uint32_t act_n = v_ch & CHEM_STATE_MASK;
-            uint32_t spent_n = v_ch >> 16;
-            uint32_t movers = 0u, act_mv = 0u;
-            if (act_n + spent_n != 0u) {
-                if (spent_n > 0u) {
-                    const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
-                    const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
-                                          (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
-                                           rb_max(P.reactivation_rate, 0.0f));
-                    uint32_t na = rb_f2u(rb_floor(exp_act));
-                    const float frac = exp_act - (float)na;
-                    if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
-                    na = rb_minu(na, rb_minu(spent_n, 8u));
-                    act_n += na;
-                    spent_n -= na;
-                }
-                if (act_n > 0u) {
-                    const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
-                    const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
-                    if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
-                        act_n -= 1u;
-                        spent_n += 1u;
-                    }
-                }
-                float ch0 = 1.0f, ch1 = 1.0f, ch2 = 1.0f, ch3 = 1.0f;
-                if (act_n > 0u && P.cohesion > 0.0f) {
-                    auto ac = [&](uint32_t c) { return (float)rb_minu(chem[c * 4u + ch] & CHEM_STATE_MASK, 8u); };
-                    if (x + 1u < G) ch0 = 1.0f + P.cohesion * ac(idx + 1u);
-                    if (x > 0u) ch1 = 1.0f + P.cohesion * ac(idx - 1u);
-                    if (y + 1u < G) ch2 = 1.0f + P.cohesion * ac(idx + G);
-                    if (y > 0u) ch3 = 1.0f + P.cohesion * ac(idx - G);
-                }
-                s_w[ch * 256u + tid] = make_float4(pr0 * ch0, pr1 * ch1, pr2 * ch2, pr3 * ch3);
-                const uint32_t n = act_n + spent_n;
-                movers = rb_minu(n, MAX_MOVERS_PER_CH);
-                act_mv = rb_minu(act_n, movers);
-                const uint32_t act_stay = act_n - act_mv;
-                const uint32_t stay = act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
-                if (stay > 0u) atomicAdd(s_acc + lself * 4 + ch, stay);
-            }
-            aend[ch] = t0 + act_mv;
-            zb[ch] = base.z0 - t0 * LCG_A;
-            t0 += movers;
-            mend[ch] = t0;
+    // ---- phase 2: sort the runs by mover count ----
+    uint32_t rk0 = 0u, rk1 = 0u, rk2 = 0u, rk3 = 0u;
+    const uint32_t cls = jitter_on ? SC_MB : 0u;
+    const uint32_t m0 = movers_of(cw.x), m1 = movers_of(cw.y), m2 = movers_of(cw.z), m3 = movers_of(cw.w);
+    const uint32_t b0 = cls + rb_minu(m0, SC_MB - 1u), b1 = cls + rb_minu(m1, SC_MB - 1u);
+    const uint32_t b2 = cls + rb_minu(m2, SC_MB - 1u), b3 = cls + rb_minu(m3, SC_MB - 1u);
+    if (m0) rk0 = atomicAdd(&s_hist[b0], 1u);
+    if (m1) rk1 = atomicAdd(&s_hist[b1], 1u);
+    if (m2) rk2 = atomicAdd(&s_hist[b2], 1u);
+    if (m3) rk3 = atomicAdd(&s_hist[b3], 1u);
+    __syncthreads();
+    if (tid < 64) {  // exclusive scan of the bucket counts; lane l holds buckets 2l and 2l + 1
+        const uint32_t h0 = s_hist[2 * tid], h1 = s_hist[2 * tid + 1];
+        uint32_t v = h0 + h1;
+        for (uint32_t off = 1; off < 64; off <<= 1) {
+            const uint32_t u = __shfl_up(v, off, 64);
+            if (tid >= off) v += u;
         }
+        const uint32_t ex = v - (h0 + h1);
+        s_hist[2 * tid] = ex;
+        s_hist[2 * tid + 1] = ex + h0;
+        if (tid == 63) s_nruns = v;
+    }
+    __syncthreads();
+    if (m0) s_run[s_hist[b0] + rk0] = (uint16_t)(tid * 4u + 0u);
+    if (m1) s_run[s_hist[b1] + rk1] = (uint16_t)(tid * 4u + 1u);
+    if (m2) s_run[s_hist[b2] + rk2] = (uint16_t)(tid * 4u + 2u);
+    if (m3) s_run[s_hist[b3] + rk3] = (uint16_t)(tid * 4u + 3u);
+    __syncthreads();
 
-        for (uint32_t t = 0; t < t0; ++t) {
-            const bool c1 = t >= mend[0], c2 = t >= mend[1], c3 = t >= mend[2];
-            const uint32_t ch = c3 ? 3u : (c2 ? 2u : (c1 ? 1u : 0u));
-            const bool is_act = t < (c3 ? aend[3] : (c2 ? aend[2] : (c1 ? aend[1] : aend[0])));
-            PcgRun mv = base;
-            mv.x1 = base.x1 + ch * LCG_A;
-            mv.x1s = mv.x1 ^ (mv.x1 >> 16);
-            mv.z0 = (c3 ? zb[3] : (c2 ? zb[2] : (c1 ? zb[1] : zb[0]))) + __umul24(t, LCG_A);
-            const F4 rf = u4_to_f4(mv.at_current());
-            const bool bound = is_act && p_bound > 0.0f && rb_fract(rf.z * 7.31f + rf.w * 3.17f) < p_bound;
-            float px = bound ? bound_start.x : rf.x;
-            float py = bound ? bound_start.y : rf.y;
-            px += disp.x;
-            py += disp.y;
-            if (jitter_on) {
-                PcgRun dsp = mv;
-                dsp.z0 = mv.z0 + (S_DISPERSE - S_MOVE) * LCG_A;
-                const F4 rj = u4_to_f4(dsp.at_current());
-                const float jx = (rj.x * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
-                const float jy = (rj.y * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+    // ---- phase 3: the runs ----
+    const uint32_t nruns = s_nruns;
+    const float tn = (float)rb_maxu(P.transport_every, 1u);
+    // PCG4D words that do not depend on key or stream
+    const uint32_t y0 = P.epoch * LCG_A + LCG_C, w0 = P.seed * LCG_A + LCG_C, y0w0 = y0 * w0;
+    constexpr uint32_t Z0 = S_MOVE * LCG_A + LCG_C, ZJ0 = S_DISPERSE * LCG_A + LCG_C;
+    for (uint32_t pos = tid; pos < nruns; pos += 256) {
+        const uint32_t run = s_run[pos];
+        const uint32_t cell = run >> 2, ch = run & 3u;
+        const uint32_t cx = blockIdx.x * 16 + (cell & 15u), cy = blockIdx.y * 16 + (cell >> 4);
+        const uint32_t cidx = cy * G + cx;
+        const ScCell& R = s_cell[cell];
+        const float4 ra = R.a, rb = R.b, rp = R.pr, rv = R.ac;
+        const uint4 rd = R.d;
+        const float light_t = rb.z;
+        const float pr0 = rp.x, pr1 = rp.y, pr2 = rp.z, pr3 = rp.w;
+        const int32_t base_x = (int32_t)rd.x, base_y = (int32_t)rd.y;
+        const int32_t lself = (int32_t)((cell >> 4) + SC_R) * SC_W + (int32_t)((cell & 15u) + SC_R);
+        const int32_t lbase = rb_addi(rb_addi(lself, (int32_t)((uint32_t)base_y * (uint32_t)SC_W)), base_x);
+        const uint32_t direct = rd.z & 0xFFFFu, blocked = rd.z >> 16;
+        const uint32_t t_hop = rd.w & 0xFFFFFFu, t_set = rb_as_u32(rb.w) & ~SC_JITTER_BIT;
+        const bool jit = (rb_as_u32(rb.w) & SC_JITTER_BIT) != 0u;
+        const uint32_t sg = rd.w >> 24;
+        const float4 rac = make_float4((sg & 1u) ? rv.x : 1.0f, (sg & 2u) ? rv.y : 1.0f, (sg & 4u) ? rv.z : 1.0f, (sg & 8u) ? rv.w : 1.0f);
+        const float4 ras = make_float4((sg & 16u) ? rv.x : 1.0f, (sg & 32u) ? rv.y : 1.0f, (sg & 64u) ? rv.z : 1.0f, (sg & 128u) ? rv.w : 1.0f);
+
+        // photoactivation and decay of the channel (transport.wgsl:269-294)
+        const uint32_t slot = cidx * 4u + ch;
+        const uint32_t v_ch = chem[slot];
+        uint32_t act_n = v_ch & CHEM_STATE_MASK;
+        uint32_t spent_n = v_ch >> 16;
+        if (spent_n > 0u) {
+            const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
+            const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
+                                  (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
+                                   rb_max(P.reactivation_rate, 0.0f));
+            uint32_t na = rb_f2u(rb_floor(exp_act));
+            const float frac = exp_act - (float)na;
+            if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
+            na = rb_minu(na, rb_minu(spent_n, 8u));
+            act_n += na;
+            spent_n -= na;
+        }
+        if (act_n > 0u) {
+            const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
+            const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
+            if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
+                act_n -= 1u;
+                spent_n += 1u;
+            }
+        }
+        // cohesion of active movers (transport.wgsl:296-304)
+        float ch0 = 1.0f, ch1 = 1.0f, ch2 = 1.0f, ch3 = 1.0f;
+        if (act_n > 0u && P.cohesion > 0.0f) {
+            auto ac = [&](uint32_t c) { return (float)rb_minu(chem[c * 4u + ch] & CHEM_STATE_MASK, 8u); };
+            if (cx + 1u < G) ch0 = 1.0f + P.cohesion * ac(cidx + 1u);
+            if (cx > 0u) ch1 = 1.0f + P.cohesion * ac(cidx - 1u);
+            if (cy + 1u < G) ch2 = 1.0f + P.cohesion * ac(cidx + G);
+            if (cy > 0u) ch3 = 1.0f + P.cohesion * ac(cidx - G);
+        }
+        const float wa0 = pr0 * ch0, wa1 = pr1 * ch1, wa2 = pr2 * ch2, wa3 = pr3 * ch3;
+        const float wsum_a = wa0 + wa1 + wa2 + wa3, wsum_s = pr0 + pr1 + pr2 + pr3;
+        const uint32_t n = act_n + spent_n;
+        const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
+        const uint32_t act_mv = rb_minu(act_n, movers);
+        const uint32_t act_stay = act_n - act_mv;
+        const uint32_t stay = act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
+        if (stay > 0u) atomicAdd(s_acc + lself * 4 + (int)ch, stay);
+
+        // the run's PCG4D polynomials in k: y = Y + k dY, z = Z + k dZ,
+        // w = W + k D0 + k (k - 1) / 2 DD (mod 2^32)
+        const uint32_t x1 = slot * LCG_A + LCG_C + y0w0;
+        const uint32_t x1s = x1 ^ (x1 >> 16);
+        const uint32_t dY = LCG_A * x1;
+        const uint32_t dZ = LCG_A + x1 * dY;
+        const uint32_t dYZ = dY * dZ, DD = 2u * dYZ;
+        uint32_t Y = y0 + Z0 * x1;
+        uint32_t Z = Z0 + x1 * Y;
+        uint32_t W = w0 + Y * Z;
+        uint32_t D = Y * dZ + Z * dY + dYZ;
+        uint32_t Yj = 0u, Zj = 0u, Wj = 0u, Dj = 0u;
+        if (jit) {
+            Yj = y0 + ZJ0 * x1;
+            Zj = ZJ0 + x1 * Yj;
+            Wj = w0 + Yj * Zj;
+            Dj = Yj * dZ + Zj * dY + dYZ;
+        }
+        for (uint32_t k = 0; k < movers; ++k) {
+            uint32_t y1 = Y, z1 = Z, w1 = W;
+            Y += dY;
+            Z += dZ;
+            W += D;
+            D += DD;
+            y1 ^= y1 >> 16;
+            z1 ^= z1 >> 16;
+            w1 ^= w1 >> 16;
+            const uint32_t rx = x1s + y1 * w1;
+            const uint32_t ry = y1 + z1 * rx;
+            const uint32_t rz = z1 + rx * ry;
+            const uint32_t rw = w1 + ry * rz;
+            const float fx = (float)(rx >> 8) * SC_U24, fy = (float)(ry >> 8) * SC_U24, fw = (float)(rw >> 8) * SC_U24;
+            const uint32_t kz = rz >> 8;
+            const bool is_act = k < act_mv;
+            const bool bound = is_act && rb.x > 0.0f && rb_fract((float)kz * SC_U24 * 7.31f + fw * 3.17f) < rb.x;
+            float px = bound ? ra.z : fx;
+            float py = bound ? ra.w : fy;
+            px += ra.x;
+            py += ra.y;
+            if (jit) {
+                uint32_t yj = Yj, zj = Zj, wj = Wj;
+                Yj += dY;
+                Zj += dZ;
+                Wj += Dj;
+                Dj += DD;
+                yj ^= yj >> 16;
+                zj ^= zj >> 16;
+                wj ^= wj >> 16;
+                const uint32_t jxu = x1s + yj * wj;
+                const uint32_t jyu = yj + zj * jxu;
+                const float jx = ((float)(jxu >> 8) * SC_U24 * 2.0f - 1.0f) * rb.y * RUBBLE_DISPERSION;
+                const float jy = ((float)(jyu >> 8) * SC_U24 * 2.0f - 1.0f) * rb.y * RUBBLE_DISPERSION;
                 px += rb_clamp(jx, -0.9f, 0.9f);
                 py += rb_clamp(jy, -0.9f, 0.9f);
             }
-            // the hop (transport.wgsl:300-330) as selects: direction d is
+            // the hop (transport.wgsl:337-357) as selects: direction d is
             // the first of u < w0, u - w0 < w1, u - w0 - w1 < w2 to hold
             // (else 3); a position steps by +-1 or by 0, and x + 0 differs
             // from x only for x = -0, which floors to the same bin
-            const float r = rf.z;
-            const bool hop = r < p_diff;
-            const float4 wa = s_w[ch * 256u + tid];
-            const float w0 = is_act ? wa.x : pr0, w1 = is_act ? wa.y : pr1;
-            const float w2 = is_act ? wa.z : pr2, w3 = is_act ? wa.w : pr3;
-            const float u0 = rf.w * (w0 + w1 + w2 + w3);
-            const float u1 = u0 - w0, u2 = u1 - w1;
-            const bool d0 = u0 < w0, d1 = u1 < w1, d2 = u2 < w2;
-            const float a0 = is_act ? ac0 : as0, a1 = is_act ? ac1 : as1;
-            const float a2 = is_act ? ac2 : as2, a3 = is_act ? ac3 : as3;
+            const bool hop = kz < t_hop;
+            const float w0s = is_act ? wa0 : pr0, w1s = is_act ? wa1 : pr1, w2s = is_act ? wa2 : pr2;
+            const float u0 = fw * (is_act ? wsum_a : wsum_s);
+            const float u1 = u0 - w0s, u2 = u1 - w1s;
+            const bool d0 = u0 < w0s, d1 = u1 < w1s, d2 = u2 < w2s;
+            const float a0 = is_act ? rac.x : ras.x, a1 = is_act ? rac.y : ras.y;
+            const float a2 = is_act ? rac.z : ras.z, a3 = is_act ? rac.w : ras.w;
             const float acc_d = d0 ? a0 : (d1 ? a1 : (d2 ? a2 : a3));
-            const bool moves = hop && !(rb_fract(rf.w * 13.73f + rf.x * 5.31f) >= acc_d);
+            const bool moves = hop && !(rb_fract(fw * 13.73f + fx * 5.31f) >= acc_d);
             const bool vert = !d0 && !d1;
             px += (moves && !vert) ? (d0 ? 1.0f : -1.0f) : 0.0f;
-            py += (moves && vert) ? (d2 ? 1.0f : -1.0f) : ((!hop && r < p_diff + p_settle) ? -1.0f : 0.0f);
+            py += (moves && vert) ? (d2 ? 1.0f : -1.0f) : ((!hop && kz < t_set) ? -1.0f : 0.0f);
             const int32_t ox = rb_clampi(rb_addi(rb_f2i(rb_floor(px)), -base_x), 0, 3);
             const int32_t oy = rb_clampi(rb_addi(rb_f2i(rb_floor(py)), -base_y), 0, 3);
             const uint32_t bi = (uint32_t)oy * 4u + (uint32_t)ox;
@@ -3200,8 +3259,8 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
                 const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, oy * SC_W + ox);
                 atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
             } else {
-                const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)x, base_x), ox), 0, (int32_t)G - 1);
-                const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)y, base_y), oy), 0, (int32_t)G - 1);
+                const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)cx, base_x), ox), 0, (int32_t)G - 1);
+                const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)cy, base_y), oy), 0, (int32_t)G - 1);
                 const int lx = tx - wx0, ly = ty - wy0;
                 if (lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W) atomicAdd(s_acc + (ly * SC_W + lx) * 4 + (int)ch, unit);
                 else atomicAdd(next + ((uint32_t)ty * G + (uint32_t)tx) * 4u + ch, unit);

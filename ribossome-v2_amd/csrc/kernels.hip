@@ -2866,6 +2866,9 @@ constexpr uint32_t SC_BUCKETS = 2 * SC_MB;  // plain runs, then runs under the r
 static_assert(SMOOTH_CELL == (uint32_t)SC_T, "the smooth_start lattice is staged per tile");
 constexpr uint32_t LCG_A = 1664525u, LCG_C = 1013904223u;  // the PCG4D input LCG
 constexpr float SC_U24 = 1.0f / 16777216.0f;
+// A draw's word u gives r = (u >> 8) 2^-24 exactly, so r * c rounds as
+// (float)(u >> 8) * (c 2^-24) with c 2^-24 exact, and r + d as one fma.
+constexpr float SC_C731 = 7.31f * SC_U24, SC_C317 = 3.17f * SC_U24, SC_C1373 = 13.73f * SC_U24, SC_C531 = 5.31f * SC_U24;
 __device__ __forceinline__ uint32_t movers_of(uint32_t w) {
     return rb_minu((w & CHEM_STATE_MASK) + (w >> 16), MAX_MOVERS_PER_CH);
 }
@@ -3173,6 +3176,16 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         }
         const float wa0 = pr0 * ch0, wa1 = pr1 * ch1, wa2 = pr2 * ch2, wa3 = pr3 * ch3;
         const float wsum_a = wa0 + wa1 + wa2 + wa3, wsum_s = pr0 + pr1 + pr2 + pr3;
+        // the sums are at least 0.2 (or inf, or NaN), so scaling them by
+        // 2^-24 is exact
+        const float wsum_a24 = wsum_a * SC_U24, wsum_s24 = wsum_s * SC_U24;
+        const float bsdx = ra.z + ra.x, bsdy = ra.w + ra.y;  // a bound mover's start plus the displacement
+        // With |displacement| < 2^22 a position stays below 2^22 + 3 in
+        // magnitude, so floor(p) - base is an exact float and the bin needs
+        // no saturating conversion; the wave takes that form when all its
+        // runs allow it.
+        const bool fast_bins = __all(rb_abs(ra.x) < 4194304.0f && rb_abs(ra.y) < 4194304.0f);
+        const float bxf = (float)base_x, byf = (float)base_y;
         const uint32_t n = act_n + spent_n;
         const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
         const uint32_t act_mv = rb_minu(act_n, movers);
@@ -3211,14 +3224,12 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             const uint32_t ry = y1 + z1 * rx;
             const uint32_t rz = z1 + rx * ry;
             const uint32_t rw = w1 + ry * rz;
-            const float fx = (float)(rx >> 8) * SC_U24, fy = (float)(ry >> 8) * SC_U24, fw = (float)(rw >> 8) * SC_U24;
+            const float kx = (float)(rx >> 8), ky = (float)(ry >> 8), kw = (float)(rw >> 8);
             const uint32_t kz = rz >> 8;
             const bool is_act = k < act_mv;
-            const bool bound = is_act && rb.x > 0.0f && rb_fract((float)kz * SC_U24 * 7.31f + fw * 3.17f) < rb.x;
-            float px = bound ? ra.z : fx;
-            float py = bound ? ra.w : fy;
-            px += ra.x;
-            py += ra.y;
+            const bool bound = is_act && rb.x > 0.0f && rb_fract((float)kz * SC_C731 + kw * SC_C317) < rb.x;
+            float px = bound ? bsdx : rb_fma(kx, SC_U24, ra.x);
+            float py = bound ? bsdy : rb_fma(ky, SC_U24, ra.y);
             if (jit) {
                 uint32_t yj = Yj, zj = Zj, wj = Wj;
                 Yj += dY;
@@ -3241,22 +3252,28 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             // from x only for x = -0, which floors to the same bin
             const bool hop = kz < t_hop;
             const float w0s = is_act ? wa0 : pr0, w1s = is_act ? wa1 : pr1, w2s = is_act ? wa2 : pr2;
-            const float u0 = fw * (is_act ? wsum_a : wsum_s);
+            const float u0 = kw * (is_act ? wsum_a24 : wsum_s24);
             const float u1 = u0 - w0s, u2 = u1 - w1s;
             const bool d0 = u0 < w0s, d1 = u1 < w1s, d2 = u2 < w2s;
             const float a0 = is_act ? rac.x : ras.x, a1 = is_act ? rac.y : ras.y;
             const float a2 = is_act ? rac.z : ras.z, a3 = is_act ? rac.w : ras.w;
             const float acc_d = d0 ? a0 : (d1 ? a1 : (d2 ? a2 : a3));
-            const bool moves = hop && !(rb_fract(fw * 13.73f + fx * 5.31f) >= acc_d);
+            const bool moves = hop && !(rb_fract(kw * SC_C1373 + kx * SC_C531) >= acc_d);
             const bool vert = !d0 && !d1;
             px += (moves && !vert) ? (d0 ? 1.0f : -1.0f) : 0.0f;
             py += (moves && vert) ? (d2 ? 1.0f : -1.0f) : ((!hop && kz < t_set) ? -1.0f : 0.0f);
-            const int32_t ox = rb_clampi(rb_addi(rb_f2i(rb_floor(px)), -base_x), 0, 3);
-            const int32_t oy = rb_clampi(rb_addi(rb_f2i(rb_floor(py)), -base_y), 0, 3);
+            int32_t ox, oy;
+            if (fast_bins) {
+                ox = (int32_t)__builtin_amdgcn_fmed3f(rb_floor(px) - bxf, 0.0f, 3.0f);
+                oy = (int32_t)__builtin_amdgcn_fmed3f(rb_floor(py) - byf, 0.0f, 3.0f);
+            } else {
+                ox = rb_clampi(rb_addi(rb_f2i(rb_floor(px)), -base_x), 0, 3);
+                oy = rb_clampi(rb_addi(rb_f2i(rb_floor(py)), -base_y), 0, 3);
+            }
             const uint32_t bi = (uint32_t)oy * 4u + (uint32_t)ox;
             const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
             if ((direct >> bi) & 1u) {
-                const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, oy * SC_W + ox);
+                const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
                 atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
             } else {
                 const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)cx, base_x), ox), 0, (int32_t)G - 1);

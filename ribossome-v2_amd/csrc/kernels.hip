@@ -2877,18 +2877,20 @@ __device__ __forceinline__ uint32_t movers_of(uint32_t w) {
 __device__ __forceinline__ uint32_t u24_threshold(float p) { return (uint32_t)__builtin_ceilf(p * 16777216.0f); }
 
 // What the movers of one cell share (written in phase 1, read in phase 3).
-// 80 bytes, so that five workgroups fit in a CU's LDS.  A hop's Kawasaki
-// acceptance is exp(-k_e |drop|) for active movers when the drop is
-// positive and for spent ones when it is negative, else 1: one value per
-// hop and two sign bits.
+// 64 bytes, so that six workgroups fit in a CU's LDS (a run forms the bin
+// base, |displacement| and the light itself).  A hop's Kawasaki acceptance
+// is exp(-k_e |drop|) for active movers when the drop is positive and for
+// spent ones when it is negative, else 1: one value per hop and two sign
+// bits.
 struct ScCell {
     float4 a;   // displacement x, y; smooth_start x, y
-    float4 b;   // p_bound, |displacement|, light at the cell, hop-or-settle threshold | jitter << 31 (bits)
+    uint4 b;    // p_bound (bits), hop-or-settle threshold | jitter << 31, direct | blocked << 16,
+                // hop threshold | positive drops << 24 | negative drops << 28
     float4 pr;  // pressure weights of the hops +x, -x, +y, -y
     float4 ac;  // exp(-k_e |drop|) of each hop (1 without aggregation)
-    uint4 d;    // base_x, base_y, direct | blocked << 16, hop threshold | positive drops << 24 | negative drops << 28
 };
 constexpr uint32_t SC_JITTER_BIT = 1u << 31;  // the thresholds are at most 2^24
+constexpr uint32_t SC_TF_WORDS = SC_W * SC_TS / 4;
 
 __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
@@ -2897,9 +2899,14 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
     __shared__ uint4 s_acc4[SC_W * SC_W];  // [window cell][channel]
     __shared__ uint32_t s_hist[SC_BUCKETS];
     __shared__ uint32_t s_nruns;
-    __shared__ uint16_t s_run[1024];              // sorted runs: cell * 4 + channel
-    __shared__ uint32_t s_tf4[SC_W * SC_TS / 4];  // window flags, rows of SC_TS bytes: 1 rock, 2 over capacity
-    __shared__ float2 s_lat[9];
+    // phase 1 reads the window flags (rows of SC_TS bytes: 1 rock, 2 over
+    // capacity) and the smooth_start lattice; phases 2 and 3 reuse the words
+    // for the sorted runs (cell * 4 + channel)
+    __shared__ uint32_t s_u[512];
+    static_assert(SC_TF_WORDS + 18 <= 512 && SC_TF_WORDS % 2 == 0, "flags and lattice fit the run list's words");
+    uint32_t* const s_tf4 = s_u;
+    float2* const s_lat = reinterpret_cast<float2*>(s_u + SC_TF_WORDS);
+    uint16_t* const s_run = reinterpret_cast<uint16_t*>(s_u);
     uint32_t* const s_acc = reinterpret_cast<uint32_t*>(s_acc4);
     uint8_t* const s_tf = reinterpret_cast<uint8_t*>(s_tf4);
     if (ep) P.epoch += *ep;
@@ -2987,8 +2994,6 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             pr3 = press_of(y > 0u, idx - G, pr3);
         }
         p_diff = rb_clamp(p_diff + PRESSURE_HOP_P * P.monomer_pressure * max_drop * tn, 0.0f, 0.5f);
-        const uint32_t lxi = rb_minu(x, G - 1u) / LIGHT_DIV, lyi = rb_minu(y, G - 1u) / LIGHT_DIV;
-        const float light_t = g.light[lyi * g.Lsz + lxi];
 
         float p_bound = 0.0f;
         float ac0 = 1.0f, ac1 = 1.0f, ac2 = 1.0f, ac3 = 1.0f;
@@ -3017,7 +3022,6 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             acc_of(e4, ac3, 3u);
             p_bound = 1.0f - rb_expf(-k_e * rb_min(rb_max(e0 - 1.0f, 0.0f), 48.0f));
         }
-        const float dlen = len2(disp);
         jitter_on = g_src > 0u && (disp.x != 0.0f || disp.y != 0.0f);
         // smooth_start is a function of the cell alone; every bound mover of
         // the cell starts there (transport.wgsl:95-117,290-296)
@@ -3077,11 +3081,10 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         // hop: r < p_diff; settle: r < p_diff + p_settle (transport.wgsl:337-357)
         ScCell& R = s_cell[tid];
         R.a = make_float4(disp.x, disp.y, bound_start.x, bound_start.y);
-        R.b = make_float4(p_bound, dlen, light_t, rb_as_f32(u24_threshold(p_diff + p_settle) | (jitter_on ? SC_JITTER_BIT : 0u)));
+        R.b = make_uint4(rb_as_u32(p_bound), u24_threshold(p_diff + p_settle) | (jitter_on ? SC_JITTER_BIT : 0u),
+                         direct | (blocked << 16), u24_threshold(p_diff) | (signs << 24));
         R.pr = make_float4(pr0, pr1, pr2, pr3);
         R.ac = make_float4(ac0, ac1, ac2, ac3);
-        R.d = make_uint4((uint32_t)base_x, (uint32_t)base_y, direct | (blocked << 16),
-                         u24_threshold(p_diff) | (signs << 24));
     }
 
     // ---- phase 2: sort the runs by mover count ----
@@ -3126,16 +3129,17 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         const uint32_t cx = blockIdx.x * 16 + (cell & 15u), cy = blockIdx.y * 16 + (cell >> 4);
         const uint32_t cidx = cy * G + cx;
         const ScCell& R = s_cell[cell];
-        const float4 ra = R.a, rb = R.b, rp = R.pr, rv = R.ac;
-        const uint4 rd = R.d;
-        const float light_t = rb.z;
+        const float4 ra = R.a, rp = R.pr, rv = R.ac;
+        const uint4 rd = R.b;
+        const float p_bound = rb_as_f32(rd.x);
         const float pr0 = rp.x, pr1 = rp.y, pr2 = rp.z, pr3 = rp.w;
-        const int32_t base_x = (int32_t)rd.x, base_y = (int32_t)rd.y;
+        const int32_t base_x = rb_addi(rb_f2i(rb_floor(ra.x)), -1);
+        const int32_t base_y = rb_addi(rb_f2i(rb_floor(ra.y)), -1);
         const int32_t lself = (int32_t)((cell >> 4) + SC_R) * SC_W + (int32_t)((cell & 15u) + SC_R);
         const int32_t lbase = rb_addi(rb_addi(lself, (int32_t)((uint32_t)base_y * (uint32_t)SC_W)), base_x);
         const uint32_t direct = rd.z & 0xFFFFu, blocked = rd.z >> 16;
-        const uint32_t t_hop = rd.w & 0xFFFFFFu, t_set = rb_as_u32(rb.w) & ~SC_JITTER_BIT;
-        const bool jit = (rb_as_u32(rb.w) & SC_JITTER_BIT) != 0u;
+        const uint32_t t_hop = rd.w & 0xFFFFFFu, t_set = rd.y & ~SC_JITTER_BIT;
+        const bool jit = (rd.y & SC_JITTER_BIT) != 0u;
         const uint32_t sg = rd.w >> 24;
         const float4 rac = make_float4((sg & 1u) ? rv.x : 1.0f, (sg & 2u) ? rv.y : 1.0f, (sg & 4u) ? rv.z : 1.0f, (sg & 8u) ? rv.w : 1.0f);
         const float4 ras = make_float4((sg & 16u) ? rv.x : 1.0f, (sg & 32u) ? rv.y : 1.0f, (sg & 64u) ? rv.z : 1.0f, (sg & 128u) ? rv.w : 1.0f);
@@ -3146,6 +3150,8 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         uint32_t act_n = v_ch & CHEM_STATE_MASK;
         uint32_t spent_n = v_ch >> 16;
         if (spent_n > 0u) {
+            const uint32_t lxi = rb_minu(cx, G - 1u) / LIGHT_DIV, lyi = rb_minu(cy, G - 1u) / LIGHT_DIV;
+            const float light_t = g.light[lyi * g.Lsz + lxi];
             const float sens = 1.0f + CHEM_SENSITIZE * (float)rb_minu(act_n, 8u);
             const float exp_act = (float)rb_minu(spent_n, 8u) * tn *
                                   (LIGHT_ACT_P * rb_max(P.uv_strength, 0.0f) * rb_max(P.direct_photoactivation, 0.0f) * light_t * sens +
@@ -3205,7 +3211,9 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         uint32_t W = w0 + Y * Z;
         uint32_t D = Y * dZ + Z * dY + dYZ;
         uint32_t Yj = 0u, Zj = 0u, Wj = 0u, Dj = 0u;
+        float dlen = 0.0f;
         if (jit) {
+            dlen = len2(f2(ra.x, ra.y));
             Yj = y0 + ZJ0 * x1;
             Zj = ZJ0 + x1 * Yj;
             Wj = w0 + Yj * Zj;
@@ -3227,7 +3235,7 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             const float kx = (float)(rx >> 8), ky = (float)(ry >> 8), kw = (float)(rw >> 8);
             const uint32_t kz = rz >> 8;
             const bool is_act = k < act_mv;
-            const bool bound = is_act && rb.x > 0.0f && rb_fract((float)kz * SC_C731 + kw * SC_C317) < rb.x;
+            const bool bound = is_act && p_bound > 0.0f && rb_fract((float)kz * SC_C731 + kw * SC_C317) < p_bound;
             float px = bound ? bsdx : rb_fma(kx, SC_U24, ra.x);
             float py = bound ? bsdy : rb_fma(ky, SC_U24, ra.y);
             if (jit) {
@@ -3241,8 +3249,8 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
                 wj ^= wj >> 16;
                 const uint32_t jxu = x1s + yj * wj;
                 const uint32_t jyu = yj + zj * jxu;
-                const float jx = ((float)(jxu >> 8) * SC_U24 * 2.0f - 1.0f) * rb.y * RUBBLE_DISPERSION;
-                const float jy = ((float)(jyu >> 8) * SC_U24 * 2.0f - 1.0f) * rb.y * RUBBLE_DISPERSION;
+                const float jx = ((float)(jxu >> 8) * SC_U24 * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
+                const float jy = ((float)(jyu >> 8) * SC_U24 * 2.0f - 1.0f) * dlen * RUBBLE_DISPERSION;
                 px += rb_clamp(jx, -0.9f, 0.9f);
                 py += rb_clamp(jy, -0.9f, 0.9f);
             }

@@ -544,6 +544,7 @@ HIP_NAMES = {
 }
 PMC_SUMMARY = os.path.join(ROOT, "profiles", "world_step_2048_hbm_counters.md")
 SQ_SUMMARY = os.path.join(ROOT, "profiles", "transport_scatter_2048_sq_counters.md")
+STEP_SUMMARY = os.path.join(ROOT, "profiles", "world_step_2048_fp32_per_step.md")
 # one wave-level VALU instruction per CU per cycle: 256 CUs at 2.4 GHz
 VALU_PEAK_PER_S = 256 * 2.4e9
 
@@ -560,6 +561,25 @@ def sq_valu_per_launch(kernel="k_scatter"):
                 return float(cols[3])
             except ValueError:
                 return None
+    return None
+
+
+def trace_dominant():
+    """The timing name of the kernel with the most kernel time per step of the
+    headline mode in the committed kernel trace (the first row of its
+    table), or None.  HIP events around each eager launch add a few
+    microseconds to every launch, which ranks the solver's short, numerous
+    launches above the kernel trace's ranking; the trace decides."""
+    if not os.path.exists(STEP_SUMMARY):
+        return None
+    rows = 0
+    for line in open(STEP_SUMMARY):
+        cols = [c.strip().strip("`") for c in line.strip().strip("|").split("|")]
+        if not line.startswith("|") or len(cols) < 6:
+            continue
+        rows += 1
+        if rows == 3:  # header and separator first
+            return cols[0]
     return None
 
 
@@ -588,15 +608,16 @@ def pmc_traffic(kernel, mode):
     return kib * 1024.0 / calls if calls else None
 
 
-IDLE_LAUNCH_US = 0.7  # an early-exit solver launch (kernel trace: 0.64-0.76 us)
 SOLVER_KERNELS = ("cg_down0", "cg_up0", "mg_block", "mg_down", "mg_up")
 
 
-def working_launches(times, sizes, mode, iterations):
+def working_launches(times, sizes, mode, iterations, launch_ms=None):
     """{kernel: (ms, launches)} over working launches: every launch of the
     per-step kernels; for the solver's per-iteration kernels, `iterations`
     (summed over the solves of the pass) times the launches per iteration,
-    with the early-exit launches' time taken off."""
+    taken as the longest of the kernel's launches (`launch_ms(name)` gives
+    every launch's milliseconds): a launch in an iteration slot the solve did
+    not use returns at once and is the shortest."""
     tol = MODES[mode]["mg_tol"] > 0
     out = {}
     for k, (ms, n) in times.items():
@@ -606,7 +627,9 @@ def working_launches(times, sizes, mode, iterations):
             busy = min(n, iterations * per_it)
             if busy <= 0:
                 continue
-            ms = max(ms - (n - busy) * IDLE_LAUNCH_US / 1000.0, 0.0)
+            each = np.sort(launch_ms(k))[::-1] if launch_ms else None
+            if each is not None and len(each) == n:
+                ms = float(each[:busy].sum())
             n = busy
         out[k] = (ms, n)
     return out
@@ -811,6 +834,61 @@ def note(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
+def device_ceiling(nbytes=1 << 30, reps=20):
+    """The device's name, CU count, peak engine clock and LDS per workgroup
+    (hipDeviceGetAttribute), and a measured HBM copy ceiling: a 1 GiB
+    device-to-device hipMemcpyAsync (a copy kernel) timed with HIP events
+    over `reps` copies, counting the bytes read and written.  Measurement
+    plumbing only; the world step never calls it."""
+    import ctypes
+
+    hip = ctypes.CDLL("libamdhip64.so")
+    c_int, c_void_p, c_size_t = ctypes.c_int, ctypes.c_void_p, ctypes.c_size_t
+
+    def chk(rc, what):
+        if rc != 0:
+            raise RuntimeError(f"{what}: HIP error {rc}")
+
+    def attr(a):
+        v = c_int(0)
+        chk(hip.hipDeviceGetAttribute(ctypes.byref(v), c_int(a), c_int(0)), "hipDeviceGetAttribute")
+        return v.value
+
+    # hipDeviceAttribute_t values of ROCm 7.2's hip_runtime_api.h:
+    # ClockRate 5 (kHz), MultiprocessorCount 63, MaxSharedMemoryPerBlock 74
+    name = ctypes.create_string_buffer(256)
+    chk(hip.hipDeviceGetName(name, c_int(256), c_int(0)), "hipDeviceGetName")
+    info = {"name": name.value.decode(errors="replace"), "cus": attr(63), "clock_mhz": attr(5) / 1000.0,
+            "lds_per_workgroup_bytes": attr(74)}
+    a, b = c_void_p(), c_void_p()
+    chk(hip.hipMalloc(ctypes.byref(a), c_size_t(nbytes)), "hipMalloc")
+    chk(hip.hipMalloc(ctypes.byref(b), c_size_t(nbytes)), "hipMalloc")
+    e0, e1 = c_void_p(), c_void_p()
+    chk(hip.hipEventCreate(ctypes.byref(e0)), "hipEventCreate")
+    chk(hip.hipEventCreate(ctypes.byref(e1)), "hipEventCreate")
+    d2d = 3  # hipMemcpyDeviceToDevice
+    try:
+        chk(hip.hipMemset(a, c_int(1), c_size_t(nbytes)), "hipMemset")
+        for _ in range(3):
+            chk(hip.hipMemcpyAsync(b, a, c_size_t(nbytes), c_int(d2d), c_void_p(0)), "hipMemcpyAsync")
+        chk(hip.hipEventRecord(e0, c_void_p(0)), "hipEventRecord")
+        for _ in range(reps):
+            chk(hip.hipMemcpyAsync(b, a, c_size_t(nbytes), c_int(d2d), c_void_p(0)), "hipMemcpyAsync")
+        chk(hip.hipEventRecord(e1, c_void_p(0)), "hipEventRecord")
+        chk(hip.hipEventSynchronize(e1), "hipEventSynchronize")
+        ms = ctypes.c_float(0.0)
+        chk(hip.hipEventElapsedTime(ctypes.byref(ms), e0, e1), "hipEventElapsedTime")
+        info["copy_GBps"] = 2.0 * nbytes * reps / (ms.value / 1000.0) / 1e9
+        info["copy_frac_of_8TBps"] = info["copy_GBps"] / 8000.0
+        info["copy_sample"] = f"{reps} device-to-device copies of 1 GiB (read + write bytes), HIP events"
+    finally:
+        hip.hipEventDestroy(e0)
+        hip.hipEventDestroy(e1)
+        hip.hipFree(a)
+        hip.hipFree(b)
+    return info
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -829,6 +907,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the binary64-oracle error sample")
     ap.add_argument("--no-light", action="store_true", help="skip the light-step kernel timing")
     ap.add_argument("--no-agents", action="store_true", help="skip the agents' coupling-point timing")
+    ap.add_argument("--no-device", action="store_true", help="skip the device properties and the copy-ceiling measurement")
     ap.add_argument("--terrain-steps", type=int, default=64,
                     help="steps of the moving-terrain measurement (0 skips it)")
     ap.add_argument("--default-steps", type=int, default=256,
@@ -909,11 +988,14 @@ def main():
     hot = hot_grid_cells(w, cfg)
     # The dominant kernel by working launches: the solver's per-iteration
     # kernels also take early-exit launches in the iteration slots a solve
-    # did not use (IDLE_LAUNCH_US each); only the iterations the solves ran
-    # count as working launches of those kernels.
-    # a solve applies the preconditioner once before its first iteration
-    work = working_launches(times, sizes, head, (s1[1] - s0[1]) + (s1[0] - s0[0]))
-    dom = max(work, key=lambda k: work[k][0])
+    # did not use; only as many launches as the solves ran iterations (a
+    # solve applies the preconditioner once before its first iteration) count
+    # as working launches of those kernels, the longest ones.
+    work = working_launches(times, sizes, head, (s1[1] - s0[1]) + (s1[0] - s0[0]), w.kernel_launch_ms)
+    dom_events = max(work, key=lambda k: work[k][0])
+    dom = trace_dominant()
+    if dom not in work:
+        dom = dom_events
     work_ms, launches = work[dom]
     avg_s = work_ms / launches / 1000.0
     per_launch = kernel_bytes(dom, cfg, sizes, head, hot)
@@ -930,7 +1012,10 @@ def main():
         "avg_launch_us": avg_s * 1e6,
         "launches": launches,
         "algorithmic_bytes_per_launch": per_launch,
-        "rule": "largest total time over working launches in an eager HIP-event pass of --profile-steps steps",
+        "rule": "the most kernel time per step of the headline mode in the committed kernel trace "
+                "(profiles/world_step_2048_fp32_per_step.md); its average over working launches in an eager "
+                "HIP-event pass of --profile-steps steps",
+        "dominant_by_events": dom_events,
     }
     # the transport scatter, the largest single kernel of every step, beside it
     sc_ms, sc_n = times.get("transport_scatter", (0.0, 0))
@@ -1022,6 +1107,12 @@ def main():
     if not args.no_cpu and not args.no_fp64 and "fp32" in results and "mixed" in results:
         f64 = fp64_error(cfg, inputs, args)
 
+    device = None
+    if not args.no_device:
+        try:
+            device = device_ceiling()
+        except (OSError, RuntimeError, AttributeError) as e:
+            device = {"error": str(e)}
     out = {
         "metric": METRIC,
         "value": value,
@@ -1059,6 +1150,7 @@ def main():
         if "fp32" in plain and "vcycle" in plain else None,
         "error_vs_fp64": f64,
         "roofline": roofline,
+        "device": device,
         "roofline_transport_scatter": roofline_scatter,
         "roofline_transport_scatter_valu": roofline_valu,
         "hbm_step_model": hbm,

@@ -457,6 +457,10 @@ RIBO_API int ribo_world_set_timing(RiboWorld* w, int enable);
  * milliseconds, launch counts.  *count returns the number of kernels. */
 RIBO_API int ribo_world_kernel_times(RiboWorld* w, const char** names, double* total_ms,
                             uint64_t* launches, uint32_t cap, uint32_t* count);
+/* Every launch's milliseconds of kernel `name` since the last reset, in
+ * launch order (up to cap of them into ms; *count returns how many there
+ * are, 0 for an unknown name). */
+RIBO_API int ribo_world_kernel_launch_ms(RiboWorld* w, const char* name, float* ms, uint32_t cap, uint32_t* count);
 RIBO_API int ribo_world_reset_kernel_times(RiboWorld* w);
 /* Last multigrid statistics of the world: V-cycles run in the last fluid
  * step and the final relative residual (tolerance mode only, else -1). */

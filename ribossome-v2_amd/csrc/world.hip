@@ -141,6 +141,7 @@ void resolve_timers(RiboWorld* w) {
         if (hipEventSynchronize(p.b) == hipSuccess && hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
             w->timers[p.timer].total_ms += ms;
             w->timers[p.timer].launches += 1;
+            w->timers[p.timer].each.push_back(ms);
         }
         w->event_pool.push_back(p.a);
         w->event_pool.push_back(p.b);
@@ -1366,6 +1367,18 @@ int ribo_world_kernel_times(RiboWorld* w, const char** names, double* total_ms, 
         if (names) names[i] = w->timers[i].name;
         if (total_ms) total_ms[i] = w->timers[i].total_ms;
         if (launches) launches[i] = w->timers[i].launches;
+    }
+    return RIBO_OK;
+}
+int ribo_world_kernel_launch_ms(RiboWorld* w, const char* name, float* ms, uint32_t cap, uint32_t* count) {
+    if (!w || !name || !count) return fail(RIBO_ERR_ARG, "null argument");
+    HIPCHK(hipStreamSynchronize(w->stream));
+    resolve_timers(w);
+    *count = 0;
+    for (const Timer& t : w->timers) {
+        if (strcmp(t.name, name) != 0) continue;
+        *count = (uint32_t)t.each.size();
+        for (uint32_t i = 0; ms && i < cap && i < *count; ++i) ms[i] = t.each[i];
     }
     return RIBO_OK;
 }

@@ -19,6 +19,7 @@ struct Timer {
     const char* name;
     double total_ms = 0;
     uint64_t launches = 0;
+    std::vector<float> each;  // every launch's milliseconds since the last reset
 };
 
 

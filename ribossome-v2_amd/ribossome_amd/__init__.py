@@ -309,6 +309,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_ledger_status": (I, [P, ctypes.POINTER(U32)]),
         "ribo_world_set_timing": (I, [P, I]),
         "ribo_world_kernel_times": (I, [P, P, P, P, U32, ctypes.POINTER(U32)]),
+        "ribo_world_kernel_launch_ms": (I, [P, ctypes.c_char_p, P, U32, ctypes.POINTER(U32)]),
         "ribo_world_reset_kernel_times": (I, [P]),
         "ribo_world_mg_stats": (I, [P, ctypes.POINTER(U32), ctypes.POINTER(ctypes.c_double)]),
         "ribo_world_solver_stats": (I, [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
@@ -576,6 +577,16 @@ class World:
         n = ctypes.c_uint32()
         _check(lib().ribo_world_kernel_times(self._h, names, ms, cnt, cap, ctypes.byref(n)), "kernel_times")
         return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, cap))}
+
+    def kernel_launch_ms(self, name: str) -> np.ndarray:
+        """Every launch's milliseconds of kernel `name` since the last reset."""
+        n = ctypes.c_uint32()
+        _check(lib().ribo_world_kernel_launch_ms(self._h, name.encode(), None, 0, ctypes.byref(n)), "kernel_launch_ms")
+        out = np.zeros(n.value, np.float32)
+        if n.value:
+            _check(lib().ribo_world_kernel_launch_ms(self._h, name.encode(), out.ctypes.data, n.value, ctypes.byref(n)),
+                   "kernel_launch_ms")
+        return out
 
     def reset_kernel_times(self):
         _check(lib().ribo_world_reset_kernel_times(self._h), "reset_kernel_times")

@@ -127,6 +127,40 @@ def test_transport_only_bit_exact():
     assert ra.Ledger.from_cells(a).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
 
 
+def test_transport_extreme_displacements_bit_exact():
+    """Transport under a frozen velocity with fast, huge and non-finite cells:
+    displacements of tens of cells leave the LDS window, ones above 2^22
+    cells take the saturating float-to-int bins, and NaN or infinite
+    velocities are cleared by the sampler's sanitising, all as in the
+    oracle."""
+    print("case transport_extreme", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 23, rock=0.1, rubble=True, vcells_per_step=1.0)
+    vel = inp[ra.VELOCITY].reshape(-1, 2).copy()
+    rng = np.random.default_rng(23)
+    n = vel.shape[0]
+    fast = rng.choice(n, n // 8, replace=False)
+    vel[fast] *= rng.uniform(20.0, 60.0, (fast.size, 1)).astype(np.float32)
+    huge = rng.choice(n, n // 40, replace=False)
+    vel[huge] = (rng.choice([-1.0, 1.0], (huge.size, 2)) * 10.0 ** rng.uniform(7.0, 19.5, (huge.size, 2))).astype(np.float32)
+    bad = rng.choice(n, 12, replace=False)
+    vel[bad[:4], 0] = np.nan
+    vel[bad[4:8], 1] = np.inf
+    vel[bad[8:], 0] = -np.inf
+    inp[ra.VELOCITY] = vel.reshape(-1)
+    p = parity_params(23, epoch=1, diffusion=10.0, settle=1.0, monomer_pressure=0.3, cohesion=0.2)
+    s = parity_settings(fluid_substep=4)
+    gw, ow = ra.World(cfg, 23), oc.OracleWorld(cfg, 23)
+    load(gw, inp, p, s)
+    load(ow, inp, p, s)
+    for k in range(3):
+        gw.encode_steps(1)
+        ow.encode_steps(1)
+        a, b = gw.download(ra.CHEM), ow.download(ra.CHEM)
+        assert np.array_equal(a, b), f"step {k}: {np.count_nonzero(a != b)} chem words differ"
+    assert ra.Ledger.from_cells(a).channels() == ra.Ledger.from_cells(inp[ra.CHEM]).channels()
+
+
 def test_conservation_and_gpu_ledger():
     """tests/conservation.rs:30-69 on the HIP world: 600 steps with strong
     diffusion, settle, thermal activation and aggregation; exact per-channel

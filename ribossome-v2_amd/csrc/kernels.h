@@ -303,7 +303,7 @@ void launch_vc_step(hipStream_t s, int phase, uint32_t n, float eps, const uint8
                     PcgState* st, float tol, uint32_t maxit);
 void launch_subgrad(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* p);
 void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf);
-void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf);
+void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf, float2* vcopy = nullptr);
 void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
                     uint32_t* chem);
 

@@ -2255,20 +2255,27 @@ __global__ __launch_bounds__(NF_T) void k_net_flux(Geo g, const float2* __restri
 }
 
 // enforce_boundaries (fluid.wgsl:663-677)
+// vcopy (optional) receives the same velocity: the copy that the transport
+// of a pipelined step reads while the next fluid step rewrites vout.
 __device__ __forceinline__ void b_boundaries(const uint3 blockIdx, const uint3 gridDim, Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
-                                                    const float* __restrict__ nf) {
+                                                    const float* __restrict__ nf, float2* __restrict__ vcopy) {
     FLUID_XY
-    if (solid(g, idx)) { vout[idx] = f2(0.0f, 0.0f); return; }
-    float2 v = sanitize(vin[idx]);
-    v = sub2(v, f2(nf[F + x], nf[y]));
-    if (x == 0u || x == F - 1u) v.x = 0.0f;
-    if (y == 0u || y == F - 1u) v.y = 0.0f;
-    v = reflect(g, idx, v);
-    vout[idx] = clamp_len(sanitize(v), MAX_VEL);
+    float2 v = f2(0.0f, 0.0f);
+    if (!solid(g, idx)) {
+        v = sanitize(vin[idx]);
+        v = sub2(v, f2(nf[F + x], nf[y]));
+        if (x == 0u || x == F - 1u) v.x = 0.0f;
+        if (y == 0u || y == F - 1u) v.y = 0.0f;
+        v = reflect(g, idx, v);
+        v = clamp_len(sanitize(v), MAX_VEL);
+    }
+    vout[idx] = v;
+    if (vcopy) vcopy[idx] = v;
 }
 __global__ __launch_bounds__(256) void k_boundaries(Geo g, const float2* __restrict__ vin, float2* __restrict__ vout,
-                                                    const float* __restrict__ nf) {
-    b_boundaries(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, vin, vout, nf);
+                                                    const float* __restrict__ nf, float2* __restrict__ vcopy) {
+    b_boundaries(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, vin, vout, nf,
+                 vcopy);
 }
 
 // thermal_activation (transport.wgsl:419-438) with chem_activate_one
@@ -3429,7 +3436,7 @@ __global__ __launch_bounds__(NF_T) void kb_net_flux(const BProj* __restrict__ ta
 }
 __global__ __launch_bounds__(256) void kb_boundaries(const BProj* __restrict__ tab, const BatchBlock* __restrict__ map) {
     BATCH_BLOCK(tab)
-    b_boundaries(bid, gdim, a.g, a.vin, a.vout, a.f1);
+    b_boundaries(bid, gdim, a.g, a.vin, a.vout, a.f1, nullptr);
 }
 __global__ __launch_bounds__(256) void kb_thermal(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
                                                   uint32_t step) {
@@ -3861,8 +3868,8 @@ void launch_net_flux(hipStream_t s, const Geo& g, const float2* vin, float* nf) 
     uint32_t nb = (g.F + NF_C - 1) / NF_C;
     k_net_flux<<<2 * nb, NF_T, 0, s>>>(g, vin, nf);
 }
-void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf) {
-    k_boundaries<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, nf);
+void launch_boundaries(hipStream_t s, const Geo& g, const float2* vin, float2* vout, const float* nf, float2* vcopy) {
+    k_boundaries<<<grid2(g.F), B16, 0, s>>>(g, vin, vout, nf, vcopy);
 }
 void launch_thermal(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const float* temp,
                     uint32_t* chem) {

@@ -532,6 +532,9 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     }
     Geo g = w->geo();
     const int tc = w->tcur;
+    // pipelined: update_temperature rewrites temp[1 - tc], which the
+    // transport before last read
+    if (w->pipe && w->pipe_temp_read[1 - tc]) (void)hipStreamWaitEvent(s, w->pe_temp[1 - tc], 0);
     run(w, "update_temperature", [&] {
         launch_temperature(s, g, P, ep, w->vel_a, w->temp[tc], w->redox[tc], w->temp[1 - tc], w->redox[1 - tc], w->eaten,
                            w->eaten_active ? 1 : 0, w->chem[w->cur]);
@@ -566,18 +569,38 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     }
     run(w, "subtract_gradient", [&] { launch_subgrad(s, g, w->vel_a, w->vel_b, w->p_a); });
     run(w, "net_flux", [&] { launch_net_flux(s, g, w->vel_b, w->net_flux); });
-    run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux); });
-    run(w, "thermal_activation", [&] { launch_thermal(s, g, P, ep, T, w->chem[w->cur]); });
+    if (!w->pipe) {
+        run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux); });
+        run(w, "thermal_activation", [&] { launch_thermal(s, g, P, ep, T, w->chem[w->cur]); });
+        return;
+    }
+    // pipelined: vel_s waits for the last transport's reads; the thermal
+    // activation follows on the transport stream
+    if (w->pipe_vel_read) (void)hipStreamWaitEvent(s, w->pe_vel, 0);
+    run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux, w->vel_s); });
+    (void)hipEventRecord(w->pe_fluid, s);
+    hipStream_t ts = w->tstream;
+    (void)hipStreamWaitEvent(ts, w->pe_fluid, 0);
+    run(w, "thermal_activation", [&] { launch_thermal(ts, g, P, ep, T, w->chem[w->cur]); });
+    (void)hipEventRecord(w->pe_temp[w->tcur], ts);
+    w->pipe_temp_read[w->tcur] = true;
 }
 
 void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1594-1601
-    hipStream_t s = w->stream;
+    hipStream_t s = w->pipe ? w->tstream : w->stream;
     Geo g = w->geo();
     uint32_t* cur = w->chem[w->cur];
     uint32_t* nxt = w->chem[1 - w->cur];
     if (P.aggregation > 0.0f) run(w, "agg", [&] { launch_agg(s, g, cur, w->agg_e); });
     run(w, "clear_next", [&] { launch_fill_u32(s, nxt, (size_t)4 * w->G * w->G, 0u); });
-    run(w, "transport_scatter", [&] { launch_scatter(s, g, P, ep, cur, nxt, w->vel_a, w->temp[w->tcur], w->agg_e); });
+    const float2* vel = w->pipe ? w->vel_s : w->vel_a;
+    run(w, "transport_scatter", [&] { launch_scatter(s, g, P, ep, cur, nxt, vel, w->temp[w->tcur], w->agg_e); });
+    if (w->pipe) {
+        (void)hipEventRecord(w->pe_temp[w->tcur], s);
+        (void)hipEventRecord(w->pe_vel, s);
+        w->pipe_temp_read[w->tcur] = true;
+        w->pipe_vel_read = true;
+    }
     w->cur = 1 - w->cur;  // transport_commit
     if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, w->chem[w->cur], w->ledger, P.epoch, ep, 1); });
 }
@@ -611,6 +634,11 @@ void light_step(RiboWorld* w, const RiboSimParams& P, uint32_t epoch, const uint
 // params.epoch + first + i, passed as is (eager) or as i on top of the
 // device epoch base `ep` (graph capture).
 void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep, const uint32_t* ep) {
+    if (w->pipe) {  // the transport stream starts after everything before
+        (void)hipEventRecord(w->pe_fork, w->stream);
+        (void)hipStreamWaitEvent(w->tstream, w->pe_fork, 0);
+        w->pipe_temp_read[0] = w->pipe_temp_read[1] = w->pipe_vel_read = false;
+    }
     for (uint32_t i = 0; i < count; ++i) {
         const uint32_t epoch = w->params.epoch + first + i;
         RiboSimParams P = w->params;
@@ -622,6 +650,24 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
         if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
         if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep);
     }
+    if (w->pipe) {  // ... and everything after waits for it
+        (void)hipEventRecord(w->pe_join, w->tstream);
+        (void)hipStreamWaitEvent(w->stream, w->pe_join, 0);
+    }
+}
+
+// Steps pipeline when nothing of a fluid step reads what the transport
+// writes: the light frozen (the light step reads the monomers), no solar
+// heating (update_temperature's absorbed light reads them), the terrain
+// still (grain moves rewrite monomers and walls); and not while kernels are
+// timed one by one.  The tolerance modes do not pipeline: their solver is a
+// chain of short, latency-bound launches, and the scatter beside it takes
+// every CU's LDS and waves and delays the chain by more than it hides (a
+// binary32 step measured 2.72 ms pipelined against 2.45 ms in one stream;
+// the reference schedule 0.68 against 0.75 ms).
+bool pipeline_ok(const RiboWorld* w) {
+    return !w->timing && !w->st.light_mode && !w->st.terrain_enabled && !(w->params.sun_heat > 0.0f) &&
+           !(w->st.multigrid && w->st.mg_tol > 0.0f);
 }
 
 // Graph replay is used when nothing in the recorded work depends on host
@@ -767,7 +813,7 @@ size_t ribo_world_footprint(const RiboWorldConfig* cfg, const RiboWorldSettings*
     const size_t G = (size_t)cfg->grid_size * cfg->grid_size, F = (size_t)cfg->fluid_size * cfg->fluid_size;
     const size_t Ls = cfg->grid_size / LIGHT_DIV, nb = (cfg->fluid_size + 15) / 16;
     size_t b = 2 * a(4 * G, 4) + a(G, 4) + 2 * a(2 * Ls * Ls, 4) + a(G, 8) + a(G, 4);  // chem x2, gamma, light x2, slope, agg
-    b += 2 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x2, p x3, div, temp/redox x2, eaten, forces, heat
+    b += 3 * a(F, 8) + 4 * a(F, 4) + 4 * a(F, 4) + a(F, 4) + 2 * a(F, 8);  // velocity x3, p x3, div, temp/redox x2, eaten, forces, heat
     b += a(2 * F, 4) + 2 * a(G, 1);  // the agents' force words, the grain move codes and masks
     b += a(2 * (size_t)cfg->fluid_size, 4) + a(F, 1) + a(F, 8) + 2 * a(F, 4) + a(F, 8);  // net flux, terrain-derived
     b += a(1, sizeof(PcgState)) + a(5 * nb * nb, 8) + 8 * a(F, 4) + 2 * a(F, 2);  // tolerance-mode vectors
@@ -824,15 +870,19 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     size_t G = (size_t)w->G * w->G, F = (size_t)w->F * w->F;
     int rc;
     hipError_t he = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
+    if (he == hipSuccess) he = hipStreamCreateWithFlags(&w->tstream, hipStreamNonBlocking);
+    for (hipEvent_t* e : {&w->pe_fork, &w->pe_join, &w->pe_fluid, &w->pe_temp[0], &w->pe_temp[1], &w->pe_vel})
+        if (he == hipSuccess) he = hipEventCreateWithFlags(e, hipEventDisableTiming);
     if (he != hipSuccess) {
-        delete w;
+        ribo_world_destroy(w);
         return fail(RIBO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     AllocCount count(&w->base_bytes);
     if ((rc = dalloc(w->stream, &w->chem[0], 4 * G)) || (rc = dalloc(w->stream, &w->chem[1], 4 * G)) || (rc = dalloc(w->stream, &w->gamma, G)) ||
         (rc = dalloc(w->stream, &w->light[0], 2 * (size_t)w->Lsz * w->Lsz)) ||
         (rc = dalloc(w->stream, &w->light[1], 2 * (size_t)w->Lsz * w->Lsz)) || (rc = dalloc(w->stream, &w->slope, G)) || (rc = dalloc(w->stream, &w->agg_e, G)) ||
-        (rc = dalloc(w->stream, &w->vel_a, F)) || (rc = dalloc(w->stream, &w->vel_b, F)) || (rc = dalloc(w->stream, &w->p_a, F)) || (rc = dalloc(w->stream, &w->p_b, F)) ||
+        (rc = dalloc(w->stream, &w->vel_a, F)) || (rc = dalloc(w->stream, &w->vel_b, F)) || (rc = dalloc(w->stream, &w->vel_s, F)) ||
+        (rc = dalloc(w->stream, &w->p_a, F)) || (rc = dalloc(w->stream, &w->p_b, F)) ||
         (rc = dalloc(w->stream, &w->p_old, F)) ||
         (rc = dalloc(w->stream, &w->div, F)) || (rc = dalloc(w->stream, &w->temp[0], F)) || (rc = dalloc(w->stream, &w->temp[1], F)) ||
         (rc = dalloc(w->stream, &w->redox[0], F)) || (rc = dalloc(w->stream, &w->redox[1], F)) || (rc = dalloc(w->stream, &w->eaten, F)) ||
@@ -873,10 +923,13 @@ int ribo_world_destroy(RiboWorld* w) {
     if (!w) return fail(RIBO_ERR_ARG, "null world");
     std::lock_guard<std::recursive_mutex> lk(device_mutex());
     if (w->stream) (void)hipStreamSynchronize(w->stream);
+    if (w->tstream) (void)hipStreamSynchronize(w->tstream);
     resolve_timers(w);
     for (auto e : w->event_pool) (void)hipEventDestroy(e);
+    for (hipEvent_t e : {w->pe_fork, w->pe_join, w->pe_fluid, w->pe_temp[0], w->pe_temp[1], w->pe_vel})
+        if (e) (void)hipEventDestroy(e);
     dfree(w->chem[0]); dfree(w->chem[1]); dfree(w->gamma); dfree(w->light[0]); dfree(w->light[1]); dfree(w->slope); dfree(w->agg_e);
-    dfree(w->vel_a); dfree(w->vel_b); dfree(w->p_a); dfree(w->p_b); dfree(w->p_old); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
+    dfree(w->vel_a); dfree(w->vel_b); dfree(w->vel_s); dfree(w->p_a); dfree(w->p_b); dfree(w->p_old); dfree(w->div); dfree(w->temp[0]); dfree(w->temp[1]);
     dfree(w->redox[0]); dfree(w->redox[1]); dfree(w->eaten); dfree(w->forces); dfree(w->agent_force); dfree(w->terrain_code); dfree(w->terrain_inc); dfree(w->heat); dfree(w->net_flux);
     dfree(w->cflags); dfree(w->normal); dfree(w->fill); dfree(w->dragg); dfree(w->slope_f);
     dfree(w->mg_p); dfree(w->mg_pp); dfree(w->mg_pp16); dfree(w->mg_rhs); dfree(w->mg_fluid); dfree(w->pcg); dfree(w->pcg_part); dfree(w->pcg_r);
@@ -887,6 +940,7 @@ int ribo_world_destroy(RiboWorld* w) {
     clear_graphs(w);
     if (w->pcg_host) (void)hipHostFree(w->pcg_host);
     if (w->stream) (void)hipStreamDestroy(w->stream);
+    if (w->tstream) (void)hipStreamDestroy(w->tstream);
     delete w;
     return RIBO_OK;
 }
@@ -1083,6 +1137,9 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
     w->fluid_was_on = st.fluid_enabled != 0;
     w->params.fluid_dt = w->params.dt * (float)substep;
     w->params.fluid_enabled = st.fluid_enabled ? 1u : 0u;
+    w->pipe = pipeline_ok(w);
+    if (w->pipe)  // the transport's copy of the velocity, until a fluid step writes it
+        HIPCHK(hipMemcpyAsync(w->vel_s, w->vel_a, sizeof(float2) * (size_t)w->F * w->F, hipMemcpyDeviceToDevice, w->stream));
     if (graph_eligible(w)) {
         // any change of the by-value kernel arguments invalidates the graphs
         RiboSimParams a = w->params, b = w->graph_params;

@@ -47,6 +47,20 @@ struct RiboWorld {
     bool fluid_was_on = true;  // src/world/mod.rs:929
     int device = 0;
     hipStream_t stream = nullptr;
+    // Pipelined steps (nothing of the fluid step reads the monomers: the
+    // light frozen, no solar heating, the terrain still): the transport
+    // chain (thermal activation, aggregation, scatter, ledger) runs on
+    // tstream beside the next fluid step on `stream`.  It reads the step's
+    // velocity from vel_s, which enforce_boundaries writes beside vel_a.
+    // Events order the streams: pe_fluid (a fluid step done), pe_temp[b]
+    // and pe_vel (the transport's last reads of temp[b] and vel_s, which the
+    // fluid step after next rewrites), pe_fork and pe_join (a recording's
+    // start and end).  A recording starts and ends joined to `stream`.
+    hipStream_t tstream = nullptr;
+    float2* vel_s = nullptr;
+    hipEvent_t pe_fork = nullptr, pe_join = nullptr, pe_fluid = nullptr, pe_temp[2] = {nullptr, nullptr}, pe_vel = nullptr;
+    bool pipe = false;
+    bool pipe_temp_read[2] = {false, false}, pipe_vel_read = false;
 
     // world-resolution buffers (bindings.wgsl group 1)
     uint32_t* chem[2] = {nullptr, nullptr};  // chem_grid / chem_next, ping-pong

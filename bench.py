@@ -681,6 +681,7 @@ def batched(args):
         w.upload(ra.VELOCITY, vel)
         w.upload(ra.CHEM, chem)
         w.settings = s
+        w.set_pipeline(False)  # one stream per world
         w.autotune()
         worlds.append((w, cfg, ra.Ledger.from_cells(chem)))
     steps = args.batch_steps

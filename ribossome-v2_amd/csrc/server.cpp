@@ -75,7 +75,9 @@ int start_scene(Scene* sc) {
     RiboWorld* w = nullptr;
     int rc = ribo_world_create(&sc->req.cfg, sc->req.seed, &w);
     if (rc) return rc;
-    if ((rc = ribo_world_set_params(w, &sc->req.params)) || (rc = ribo_world_set_settings(w, &sc->req.settings))) {
+    // the server steps many scenes at once: one stream per scene
+    if ((rc = ribo_world_set_pipeline(w, 0)) || (rc = ribo_world_set_params(w, &sc->req.params)) ||
+        (rc = ribo_world_set_settings(w, &sc->req.settings))) {
         ribo_world_destroy(w);
         return rc;
     }

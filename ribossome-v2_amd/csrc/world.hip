@@ -666,8 +666,19 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
 // binary32 step measured 2.72 ms pipelined against 2.45 ms in one stream;
 // the reference schedule 0.68 against 0.75 ms).
 bool pipeline_ok(const RiboWorld* w) {
-    return !w->timing && !w->st.light_mode && !w->st.terrain_enabled && !(w->params.sun_heat > 0.0f) &&
+    return w->pipe_allowed && !w->timing && !w->st.light_mode && !w->st.terrain_enabled && !(w->params.sun_heat > 0.0f) &&
            !(w->st.multigrid && w->st.mg_tol > 0.0f);
+}
+
+// The transport stream and the events, made when a world first pipelines
+// (worlds that never do, such as a server's, keep one stream).
+int ensure_pipeline(RiboWorld* w) {
+    if (w->tstream) return RIBO_OK;
+    hipError_t he = hipStreamCreateWithFlags(&w->tstream, hipStreamNonBlocking);
+    for (hipEvent_t* e : {&w->pe_fork, &w->pe_join, &w->pe_fluid, &w->pe_temp[0], &w->pe_temp[1], &w->pe_vel})
+        if (he == hipSuccess) he = hipEventCreateWithFlags(e, hipEventDisableTiming);
+    if (he != hipSuccess) return fail(RIBO_ERR_HIP, std::string("pipeline stream: ") + hipGetErrorString(he));
+    return RIBO_OK;
 }
 
 // Graph replay is used when nothing in the recorded work depends on host
@@ -870,11 +881,8 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     size_t G = (size_t)w->G * w->G, F = (size_t)w->F * w->F;
     int rc;
     hipError_t he = hipStreamCreateWithFlags(&w->stream, hipStreamNonBlocking);
-    if (he == hipSuccess) he = hipStreamCreateWithFlags(&w->tstream, hipStreamNonBlocking);
-    for (hipEvent_t* e : {&w->pe_fork, &w->pe_join, &w->pe_fluid, &w->pe_temp[0], &w->pe_temp[1], &w->pe_vel})
-        if (he == hipSuccess) he = hipEventCreateWithFlags(e, hipEventDisableTiming);
     if (he != hipSuccess) {
-        ribo_world_destroy(w);
+        delete w;
         return fail(RIBO_ERR_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(he));
     }
     AllocCount count(&w->base_bytes);
@@ -1138,6 +1146,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
     w->params.fluid_dt = w->params.dt * (float)substep;
     w->params.fluid_enabled = st.fluid_enabled ? 1u : 0u;
     w->pipe = pipeline_ok(w);
+    if (w->pipe && (rc = ensure_pipeline(w))) return rc;
     if (w->pipe)  // the transport's copy of the velocity, until a fluid step writes it
         HIPCHK(hipMemcpyAsync(w->vel_s, w->vel_a, sizeof(float2) * (size_t)w->F * w->F, hipMemcpyDeviceToDevice, w->stream));
     if (graph_eligible(w)) {
@@ -1409,6 +1418,11 @@ int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch) {
     return RIBO_OK;
 }
 
+int ribo_world_set_pipeline(RiboWorld* w, int enable) {
+    if (!w) return fail(RIBO_ERR_ARG, "null world");
+    w->pipe_allowed = enable != 0;
+    return RIBO_OK;
+}
 int ribo_world_set_timing(RiboWorld* w, int enable) {
     if (!w) return fail(RIBO_ERR_ARG, "null world");
     w->timing = enable != 0;

@@ -59,7 +59,7 @@ struct RiboWorld {
     hipStream_t tstream = nullptr;
     float2* vel_s = nullptr;
     hipEvent_t pe_fork = nullptr, pe_join = nullptr, pe_fluid = nullptr, pe_temp[2] = {nullptr, nullptr}, pe_vel = nullptr;
-    bool pipe = false;
+    bool pipe = false, pipe_allowed = true;
     bool pipe_temp_read[2] = {false, false}, pipe_vel_read = false;
 
     // world-resolution buffers (bindings.wgsl group 1)

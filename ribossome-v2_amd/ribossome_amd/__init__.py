@@ -308,6 +308,7 @@ def lib() -> ctypes.CDLL:
         "ribo_world_chem_ops": (I, [P, P, U32, P]),
         "ribo_world_ledger_status": (I, [P, ctypes.POINTER(U32)]),
         "ribo_world_set_timing": (I, [P, I]),
+        "ribo_world_set_pipeline": (I, [P, I]),
         "ribo_world_kernel_times": (I, [P, P, P, P, U32, ctypes.POINTER(U32)]),
         "ribo_world_kernel_launch_ms": (I, [P, ctypes.c_char_p, P, U32, ctypes.POINTER(U32)]),
         "ribo_world_reset_kernel_times": (I, [P]),
@@ -577,6 +578,11 @@ class World:
         n = ctypes.c_uint32()
         _check(lib().ribo_world_kernel_times(self._h, names, ms, cnt, cap, ctypes.byref(n)), "kernel_times")
         return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, cap))}
+
+    def set_pipeline(self, enable: bool):
+        """Pipeline the transport chain beside the next fluid step where the
+        step allows it (on by default)."""
+        _check(lib().ribo_world_set_pipeline(self._h, 1 if enable else 0), "set_pipeline")
 
     def kernel_launch_ms(self, name: str) -> np.ndarray:
         """Every launch's milliseconds of kernel `name` since the last reset."""

@@ -67,6 +67,12 @@ struct RiboBatch {
     uint64_t launches = 0;
     uint32_t substep = 1, te = 1, period = 1, cycles = 1, depth = 0;
     bool ledger = false, agg = false, tol = false, mixed = false;
+    // Pipelined batches (the reference V-cycle, no solar heating): the
+    // transport chain runs on tstream beside the next fluid step, ordered by
+    // events as a world stepped alone orders them (world_state.h).
+    bool pipe = false;
+    hipStream_t tstream = nullptr;
+    hipEvent_t pe_fork = nullptr, pe_join = nullptr, pe_fluid = nullptr, pe_temp[2] = {nullptr, nullptr}, pe_vel = nullptr;
     size_t block_lds = 0, tol_lds = 0;
     std::vector<int> cur0, tcur0;        // each world's parity when the tables were built
     std::vector<const double*> cinv0;    // each world's coarsest inverse the tables point at
@@ -302,6 +308,8 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
     b->ledger = w0->st.ledger_check != 0;
     b->tol = tol_of(w0);
     b->mixed = b->tol && w0->st.precision == RIBO_PRECISION_MIXED;
+    b->pipe = !b->tol;
+    for (const RiboWorld* w : b->worlds) b->pipe = b->pipe && !(w->params.sun_heat > 0.0f);
     for (const RiboWorld* w : b->worlds) b->use_graphs = b->use_graphs && w->st.no_graphs == 0;
     for (RiboWorld* w : b->worlds) {
         b->agg = b->agg || w->params.aggregation > 0.0f;
@@ -378,7 +386,7 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
                 t_temp[(size_t)(c * 2 + t) * count + wi] = BTemperature{
                     g, P, ep, w->vel_a, w->temp[tw], w->redox[tw], w->temp[1 - tw], w->redox[1 - tw], w->eaten, w->chem[cw]};
                 t_chem[(size_t)(c * 2 + t) * count + wi] = BChem{
-                    g, P, ep, w->chem[cw], w->chem[1 - cw], w->vel_a, w->temp[tw], w->agg_e, w->ledger,
+                    g, P, ep, w->chem[cw], w->chem[1 - cw], b->pipe ? w->vel_s : w->vel_a, w->temp[tw], w->agg_e, w->ledger,
                     std::min<uint32_t>((w->G * w->G + 255u) / 256u, LEDGER_MAX_BLOCKS)};
             }
             t_forces[t * (size_t)count + wi] = BForces{g, P, ep, w->temp[tw], w->forces};
@@ -391,7 +399,7 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
         t_proj[0 * (size_t)count + wi] = BProj{g, w->vel_a, nullptr, w->div, w->p_a};
         t_proj[1 * (size_t)count + wi] = BProj{g, w->vel_a, w->vel_b, w->p_a, nullptr};
         t_proj[2 * (size_t)count + wi] = BProj{g, w->vel_b, nullptr, nullptr, w->net_flux};
-        t_proj[3 * (size_t)count + wi] = BProj{g, w->vel_b, w->vel_a, nullptr, w->net_flux};
+        t_proj[3 * (size_t)count + wi] = BProj{g, w->vel_b, w->vel_a, nullptr, w->net_flux, b->pipe ? w->vel_s : nullptr};
         for (uint32_t d = 0; d < b->depth; ++d)
             if (d < w->mg_block_start) t_mg[(size_t)d * count + wi] = BMgPair{w->mg[d], w->mg[d + 1]};
         if (!b->tol) {
@@ -442,6 +450,10 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
     if (he == hipSuccess) he = hipEventCreateWithFlags(&b->batch_done, hipEventDisableTiming);
     if (he == hipSuccess) he = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
     if (he == hipSuccess) he = hipEventRecord(b->staged, b->stream);
+    if (he == hipSuccess && b->pipe) he = hipStreamCreateWithFlags(&b->tstream, hipStreamNonBlocking);
+    if (b->pipe)
+        for (hipEvent_t* e : {&b->pe_fork, &b->pe_join, &b->pe_fluid, &b->pe_temp[0], &b->pe_temp[1], &b->pe_vel})
+            if (he == hipSuccess) he = hipEventCreateWithFlags(e, hipEventDisableTiming);
     b->world_done.assign(count, nullptr);
     for (uint32_t i = 0; i < count && he == hipSuccess; ++i)
         he = hipEventCreateWithFlags(&b->world_done[i], hipEventDisableTiming);
@@ -483,7 +495,10 @@ int ribo_batch_create(RiboWorld* const* worlds, uint32_t count, RiboBatch** out)
 int ribo_batch_destroy(RiboBatch* b) {
     if (!b) return set_error(RIBO_ERR_ARG, "null batch");
     if (b->stream) (void)hipStreamSynchronize(b->stream);
+    if (b->tstream) (void)hipStreamSynchronize(b->tstream);
     for (auto& g : b->graphs) (void)hipGraphExecDestroy(g.exec);
+    for (hipEvent_t e : {b->pe_fork, b->pe_join, b->pe_fluid, b->pe_temp[0], b->pe_temp[1], b->pe_vel})
+        if (e) (void)hipEventDestroy(e);
     for (hipEvent_t e : b->world_done)
         if (e) (void)hipEventDestroy(e);
     if (b->batch_done) (void)hipEventDestroy(b->batch_done);
@@ -495,6 +510,7 @@ int ribo_batch_destroy(RiboBatch* b) {
         if (b->host) (void)hipHostFree(b->host);
     }
     if (b->stream) (void)hipStreamDestroy(b->stream);
+    if (b->tstream) (void)hipStreamDestroy(b->tstream);
     delete b;
     return RIBO_OK;
 }
@@ -531,10 +547,17 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
     }
     const uint32_t e0 = b->worlds[0]->params.epoch;
     uint64_t launched = 0;
+    hipStream_t ls = s;  // the stream the next launches go to
     auto L = [&](BatchKernel k, const void* tab, const Shape& sh, uint32_t step, size_t lds = 0) {
-        launch_batch(s, k, tab, sh.map, sh.blocks, step, lds);
+        launch_batch(ls, k, tab, sh.map, sh.blocks, step, lds);
         ++launched;
     };
+    if (b->pipe)  // the transport's copy of each world's velocity, until a fluid step writes it
+        for (size_t i = 0; i < n; ++i) {
+            const RiboWorld* w = b->worlds[i];
+            BCHK(hipMemcpyAsync(w->vel_s, w->vel_a, sizeof(float2) * (size_t)w->F * w->F, hipMemcpyDeviceToDevice, s));
+        }
+    bool temp_read[2] = {false, false}, vel_read = false;
     const uint32_t chunk = b->tol ? TOL_CHUNK : RIBO_MAX_STEPS_PER_CALL;
     for (uint32_t first = 0; first < steps; first += chunk) {
         const uint32_t m = std::min(chunk, steps - first);
@@ -596,9 +619,17 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
             }
             BCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
         }
+        if (b->pipe) {  // the transport stream starts after everything before
+            BCHK(hipEventRecord(b->pe_fork, s));
+            BCHK(hipStreamWaitEvent(b->tstream, b->pe_fork, 0));
+            temp_read[0] = temp_read[1] = vel_read = false;
+        }
         for (uint32_t i = 0; i < m; ++i) {
             const uint32_t e = e0 + first + i;
             if (e % b->substep == 0) {  // fluid_step (src/world/mod.rs:1527-1588)
+                ls = s;
+                // the temperature buffer rewritten now was read by the transport before last
+                if (b->pipe && temp_read[1 - tc]) BCHK(hipStreamWaitEvent(s, b->pe_temp[1 - tc], 0));
                 L(BK_TEMPERATURE, b->temperature[cur][tc], b->fluid, i);
                 tc = 1 - tc;  // copy_temperature
                 L(BK_FORCES, b->forces[tc], b->fluid, i);
@@ -618,19 +649,40 @@ int ribo_batch_encode_steps(RiboBatch* b, uint32_t steps) {
                 }
                 L(BK_SUBGRAD, b->subgrad, b->fluid, i);
                 L(BK_NET_FLUX, b->netflux_t, b->netflux, i);
+                if (b->pipe && vel_read) BCHK(hipStreamWaitEvent(s, b->pe_vel, 0));  // the velocity copy's last reader
                 L(BK_BOUNDARIES, b->boundaries, b->fluid, i);
+                if (b->pipe) {  // the thermal activation follows on the transport stream
+                    BCHK(hipEventRecord(b->pe_fluid, s));
+                    BCHK(hipStreamWaitEvent(b->tstream, b->pe_fluid, 0));
+                    ls = b->tstream;
+                }
                 L(BK_THERMAL, b->chem[cur][tc], b->grid, i);
+                if (b->pipe) {
+                    BCHK(hipEventRecord(b->pe_temp[tc], b->tstream));
+                    temp_read[tc] = true;
+                }
             }
             if (e % b->te == 0) {  // transport (:1594-1601)
+                ls = b->pipe ? b->tstream : s;
                 if (b->agg) L(BK_AGG, b->chem[cur][tc], b->grid, i);
                 L(BK_CLEAR_NEXT, b->chem[cur][tc], b->clear, i);
                 L(BK_SCATTER, b->chem[cur][tc], b->grid, i);
+                if (b->pipe) {
+                    BCHK(hipEventRecord(b->pe_temp[tc], b->tstream));
+                    BCHK(hipEventRecord(b->pe_vel, b->tstream));
+                    temp_read[tc] = vel_read = true;
+                }
                 cur = 1 - cur;  // transport_commit
                 if (b->ledger) {
                     L(BK_LEDGER, b->chem[cur][tc], b->ledger_blocks, i);
                     L(BK_LEDGER_FIN, b->chem[cur][tc], b->one, i);
                 }
             }
+        }
+        ls = s;
+        if (b->pipe) {  // ... and everything after waits for it
+            BCHK(hipEventRecord(b->pe_join, b->tstream));
+            BCHK(hipStreamWaitEvent(s, b->pe_join, 0));
         }
         if (capture) {
             hipGraph_t graph = nullptr;

@@ -180,7 +180,7 @@ struct BForces { Geo g; RiboSimParams P; const uint32_t* ep; const float* T; flo
 struct BVel { Geo g; RiboSimParams P; float decay; const float2* aux; const float2* vin; float2* vout; };
 // divergence (f0 = div, f1 = p0), subtract_gradient (f0 = p), net_flux and
 // enforce_boundaries (f1 = net flux)
-struct BProj { Geo g; const float2* vin; float2* vout; float* f0; float* f1; };
+struct BProj { Geo g; const float2* vin; float2* vout; float* f0; float* f1; float2* vcopy; };  // vcopy: enforce_boundaries' copy
 // thermal_activation, agg, the clear of chem_next, the scatter, the ledger
 struct BChem {
     Geo g; RiboSimParams P; const uint32_t* ep;

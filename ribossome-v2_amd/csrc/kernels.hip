@@ -3436,7 +3436,7 @@ __global__ __launch_bounds__(NF_T) void kb_net_flux(const BProj* __restrict__ ta
 }
 __global__ __launch_bounds__(256) void kb_boundaries(const BProj* __restrict__ tab, const BatchBlock* __restrict__ map) {
     BATCH_BLOCK(tab)
-    b_boundaries(bid, gdim, a.g, a.vin, a.vout, a.f1, nullptr);
+    b_boundaries(bid, gdim, a.g, a.vin, a.vout, a.f1, a.vcopy);
 }
 __global__ __launch_bounds__(256) void kb_thermal(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
                                                   uint32_t step) {

@@ -2883,6 +2883,18 @@ __device__ __forceinline__ uint32_t movers_of(uint32_t w) {
 // (u >> 8) < ceil(p * 2^24); p * 2^24 is exact.
 __device__ __forceinline__ uint32_t u24_threshold(float p) { return (uint32_t)__builtin_ceilf(p * 16777216.0f); }
 
+// rng_f4(key, epoch, stream, seed).x from the key's x1 = LCG(key) +
+// LCG(epoch) LCG(seed) and x1s = x1 ^ (x1 >> 16), with y0 = LCG(epoch),
+// w0 = LCG(seed), z0 = LCG(stream): the x word needs y, z and w of the
+// first round and y, w of the second.
+__device__ __forceinline__ float pcg_x_of(uint32_t x1, uint32_t x1s, uint32_t y0, uint32_t w0, uint32_t z0) {
+    const uint32_t y1 = y0 + z0 * x1;
+    const uint32_t z1 = z0 + x1 * y1;
+    const uint32_t w1 = w0 + y1 * z1;
+    const uint32_t rx = x1s + (y1 ^ (y1 >> 16)) * (w1 ^ (w1 >> 16));
+    return (float)(rx >> 8) * SC_U24;
+}
+
 // What the movers of one cell share (written in phase 1, read in phase 3).
 // 64 bytes, so that six workgroups fit in a CU's LDS (a run forms the bin
 // base, |displacement| and the light itself).  A hop's Kawasaki acceptance
@@ -3151,8 +3163,11 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         const float4 rac = make_float4((sg & 1u) ? rv.x : 1.0f, (sg & 2u) ? rv.y : 1.0f, (sg & 4u) ? rv.z : 1.0f, (sg & 8u) ? rv.w : 1.0f);
         const float4 ras = make_float4((sg & 16u) ? rv.x : 1.0f, (sg & 32u) ? rv.y : 1.0f, (sg & 64u) ? rv.z : 1.0f, (sg & 128u) ? rv.w : 1.0f);
 
-        // photoactivation and decay of the channel (transport.wgsl:269-294)
+        // photoactivation and decay of the channel (transport.wgsl:269-294);
+        // every draw of the run shares the key's first PCG4D words
         const uint32_t slot = cidx * 4u + ch;
+        const uint32_t x1 = slot * LCG_A + LCG_C + y0w0;
+        const uint32_t x1s = x1 ^ (x1 >> 16);
         const uint32_t v_ch = chem[slot];
         uint32_t act_n = v_ch & CHEM_STATE_MASK;
         uint32_t spent_n = v_ch >> 16;
@@ -3165,15 +3180,16 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
                                    rb_max(P.reactivation_rate, 0.0f));
             uint32_t na = rb_f2u(rb_floor(exp_act));
             const float frac = exp_act - (float)na;
-            if (frac > 0.0f && rng_f4(slot, P.epoch, S_PHOTO, P.seed).x < frac) na += 1u;
+            if (frac > 0.0f && pcg_x_of(x1, x1s, y0, w0, S_PHOTO * LCG_A + LCG_C) < frac) na += 1u;
             na = rb_minu(na, rb_minu(spent_n, 8u));
             act_n += na;
             spent_n -= na;
         }
-        if (act_n > 0u) {
+        // (a decay rate of 0 or below makes exp_dec 0: no draw, no change)
+        if (act_n > 0u && P.activation_decay > 0.0f) {
             const float shield = 1.0f / (1.0f + CHEM_SHIELD * (float)(rb_minu(act_n, 8u) - 1u));
             const float exp_dec = (float)rb_minu(act_n, 8u) * rb_max(P.activation_decay, 0.0f) * shield * tn;
-            if (exp_dec > 0.0f && rng_f4(slot, P.epoch, S_DECAY, P.seed).x < exp_dec) {
+            if (exp_dec > 0.0f && pcg_x_of(x1, x1s, y0, w0, S_DECAY * LCG_A + LCG_C) < exp_dec) {
                 act_n -= 1u;
                 spent_n += 1u;
             }
@@ -3208,8 +3224,6 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
 
         // the run's PCG4D polynomials in k: y = Y + k dY, z = Z + k dZ,
         // w = W + k D0 + k (k - 1) / 2 DD (mod 2^32)
-        const uint32_t x1 = slot * LCG_A + LCG_C + y0w0;
-        const uint32_t x1s = x1 ^ (x1 >> 16);
         const uint32_t dY = LCG_A * x1;
         const uint32_t dZ = LCG_A + x1 * dY;
         const uint32_t dYZ = dY * dZ, DD = 2u * dYZ;

@@ -355,6 +355,43 @@ def test_graph_replays_match_eager_recording(tol):
     assert worlds[0].ledger_status() is None and worlds[1].ledger_status() is None
 
 
+@pytest.mark.parametrize("schedule", [(1, 1), (2, 3)])
+def test_pipelined_steps_match_one_stream(schedule):
+    """The reference schedule with the transport chain on a second stream
+    beside the next fluid step (the default) gives the same bits as one
+    stream, through graph capture and replay and eager recording, with steps
+    that run only the fluid or only the transport (substep 2, transport
+    every 3), and with velocity uploads between chunks."""
+    substep, every = schedule
+    print(f"case pipeline substep={substep} transport_every={every}", flush=True)
+    cfg = ra.WorldConfig(128, 64)
+    inp = make_inputs(cfg, 19, rock=0.12, rubble=True, vent_rows=4, vcells_per_step=1.2)
+    worlds = []
+    for pipeline, no_graphs in ((False, 0), (True, 0), (True, 1)):
+        w = ra.World(cfg, 19)
+        load(w, inp, parity_params(19, transport_every=every, thermal_activation=1.0, diffusion=5.0),
+             parity_settings(fluid_substep=substep, no_graphs=no_graphs, ledger_check=1))
+        w.set_pipeline(pipeline)
+        worlds.append(w)
+    rng = np.random.default_rng(19)
+    for k, chunk in enumerate((7, 64, 13, 64, 5)):
+        if k == 2:  # an upload between chunks: the transport's velocity copy follows it
+            vel = (inp[ra.VELOCITY] * rng.uniform(0.5, 1.5)).astype(np.float32)
+            for w in worlds:
+                w.upload(ra.VELOCITY, vel)
+        for w in worlds:
+            assert w.encode_steps(chunk) == chunk
+        ref = worlds[0]
+        ref.sync()
+        for w in worlds[1:]:
+            w.sync()
+            assert np.array_equal(ref.download(ra.CHEM), w.download(ra.CHEM)), f"chunk {k}: chem differs"
+            for which in FIELDS:
+                assert bit_diff(ref.download(which), w.download(which)) == 0, f"chunk {k}: {ra.BUFFER_NAMES[which]}"
+    for w in worlds:
+        assert w.ledger_status() is None
+
+
 def test_reference_default_profile_graphs_match_eager():
     """The reference's default profile (day cycle, solar heating, light
     propagation, moving terrain, substep 2, transport every 3) replays as

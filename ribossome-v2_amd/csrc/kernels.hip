@@ -3214,6 +3214,9 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         // no saturating conversion; the wave takes that form when all its
         // runs allow it.
         const bool fast_bins = __all(rb_abs(ra.x) < 4194304.0f && rb_abs(ra.y) < 4194304.0f);
+        // every bin of every run of the wave lies in the LDS window (the
+        // common case away from the grid's edge and from fast flow)
+        const bool all_direct = __all(direct == 0xFFFFu);
         const float bxf = (float)base_x, byf = (float)base_y;
         const uint32_t n = act_n + spent_n;
         const uint32_t movers = rb_minu(n, MAX_MOVERS_PER_CH);
@@ -3301,7 +3304,10 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             }
             const uint32_t bi = (uint32_t)oy * 4u + (uint32_t)ox;
             const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
-            if ((direct >> bi) & 1u) {
+            if (all_direct) {
+                const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
+                atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
+            } else if ((direct >> bi) & 1u) {
                 const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
                 atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
             } else {

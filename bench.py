@@ -191,6 +191,8 @@ def build_world(cfg, inputs, args, mode="fp32"):
     w.upload(ra.CHEM, chem)
     w.upload(ra.HEAT_SRC, heat)
     w.settings = s
+    if getattr(args, "no_overlap", False):
+        w.set_pipeline(False)  # every kernel on its own, as the HIP-event pass times them
     if not getattr(args, "no_autotune", False):
         w.autotune()
     return w
@@ -909,6 +911,9 @@ def main():
     ap.add_argument("--no-light", action="store_true", help="skip the light-step kernel timing")
     ap.add_argument("--no-agents", action="store_true", help="skip the agents' coupling-point timing")
     ap.add_argument("--no-device", action="store_true", help="skip the device properties and the copy-ceiling measurement")
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="run each step's transport after its fluid, not beside the next step's (kernel traces of the "
+                         "separate kernels)")
     ap.add_argument("--terrain-steps", type=int, default=64,
                     help="steps of the moving-terrain measurement (0 skips it)")
     ap.add_argument("--default-steps", type=int, default=256,
@@ -1013,9 +1018,9 @@ def main():
         "avg_launch_us": avg_s * 1e6,
         "launches": launches,
         "algorithmic_bytes_per_launch": per_launch,
-        "rule": "the most kernel time per step of the headline mode in the committed kernel trace "
-                "(profiles/world_step_2048_fp32_per_step.md); its average over working launches in an eager "
-                "HIP-event pass of --profile-steps steps",
+        "rule": "the most kernel time per step of the headline mode in the committed kernel trace with the "
+                "kernels separate (profiles/world_step_2048_fp32_per_step.md, --no-overlap); its average over "
+                "working launches in an eager HIP-event pass of --profile-steps steps (kernels timed one by one)",
         "dominant_by_events": dom_events,
     }
     # the transport scatter, the largest single kernel of every step, beside it
@@ -1138,6 +1143,10 @@ def main():
             "mode": head,
             "mg_tol": MODES[head]["mg_tol"],
             "settle_steps": args.settle_steps,
+            "overlap": "off (--no-overlap)" if args.no_overlap else
+                       "each step's transport beside the next step's fluid: the scatter's tiles inside the "
+                       "conjugate-gradient V-cycles' one-workgroup launches (fp32, mixed), a second stream "
+                       "(reference)",
         },
         "modes": results,
         "mixed_vs_fp32": (results["mixed"]["value"] / results["fp32"]["value"]) if "mixed" in results and "fp32" in results else None,

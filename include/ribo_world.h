@@ -451,10 +451,13 @@ RIBO_API int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch);
 /* ---- measurement (KernelTiming / profile_kernels.rs) -------------------- */
 
 /* Steps whose fluid part reads nothing the transport writes (the light
- * frozen, no solar heating, the terrain still) and that run the reference's
- * fixed V-cycles record the transport chain on a second stream of the
- * world, beside the next fluid step, ordered by events (on by default; the
- * results are the same).  Many worlds stepping at once already fill the
+ * frozen, no solar heating, the terrain still) overlap a step's transport
+ * with the next step's fluid (on by default; the results are the same):
+ * with the reference's fixed V-cycles the transport chain runs on a second
+ * stream of the world, ordered by events; in the conjugate-gradient
+ * tolerance mode (one fluid step and one transport per step) the scatter's
+ * tiles run inside the next solve's V-cycles, beside the coarse levels'
+ * one-workgroup kernel.  Many worlds stepping at once already fill the
  * device: a caller that steps them on their own streams turns it off. */
 RIBO_API int ribo_world_set_pipeline(RiboWorld* w, int enable);
 /* Enables per-kernel HIP-event timing on the world's stream (off by default;

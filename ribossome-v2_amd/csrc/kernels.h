@@ -336,6 +336,21 @@ void launch_agg(hipStream_t s, const Geo& g, const uint32_t* chem, uint32_t* agg
 void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const uint32_t* chem,
                     uint32_t* next, const float2* vel, const float* temp, const uint32_t* agg_e);
+// The scatter's operands, for the launch that runs it beside the symmetric
+// V-cycle's one-workgroup levels: workgroup 0 is launch_mg_block's (sym),
+// the others take SC_FUSE_TILES of the scatter's 16 x 16 tiles each, tiles
+// [t0, t1) of the scatter_tiles(G) in row-major order.  Fits when the
+// V-cycle's LDS does not exceed the tiles'.
+struct ScatterArgs {
+    Geo g; RiboSimParams P; const uint32_t* ep;
+    const uint32_t* chem; uint32_t* next; const float2* vel; const float* temp; const uint32_t* agg_e;
+};
+constexpr uint32_t SC_FUSE_TILES = 4;
+uint32_t scatter_tiles(uint32_t G);
+template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count);
+template <class S>
+void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
+                             const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1);
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check);
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n);

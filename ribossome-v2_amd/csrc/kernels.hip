@@ -1434,9 +1434,11 @@ __device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, floa
 }
 
 template <class S, bool SYM>
-__device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gridDim, const MgBlockArgs<S>& a, const uint32_t* done) {
+__device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gridDim, const MgBlockArgs<S>& a, const uint32_t* done,
+                                           float* lds = nullptr) {
     if (done && *done) return;
-    extern __shared__ float sm[];
+    extern __shared__ float dyn_sm[];
+    float* const sm = lds ? lds : dyn_sm;  // the launch's dynamic LDS, or the caller's
     const Team<32, 32> blk((int)threadIdx.x);
     const int total = a.off[a.levels];
     float* P = sm;
@@ -2911,25 +2913,43 @@ struct ScCell {
 constexpr uint32_t SC_JITTER_BIT = 1u << 31;  // the thresholds are at most 2^24
 constexpr uint32_t SC_TF_WORDS = SC_W * SC_TS / 4;
 
-__device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 gridDim, Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
-                                                 uint32_t* __restrict__ next, const float2* __restrict__ vel,
-                                                 const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
-    __shared__ ScCell s_cell[256];
-    __shared__ uint4 s_acc4[SC_W * SC_W];  // [window cell][channel]
-    __shared__ uint32_t s_hist[SC_BUCKETS];
-    __shared__ uint32_t s_nruns;
+// atomicAdd on a word of LDS, kept an LDS atomic whatever the compiler
+// infers of the pointer (a generic atomic would take a flat one).
+__device__ __forceinline__ uint32_t lds_add(uint32_t* p, uint32_t v) {
+    return __atomic_fetch_add((__attribute__((address_space(3))) uint32_t*)p, v, __ATOMIC_RELAXED);
+}
+
+// The LDS of one tile (26.7 KB).
+struct ScLds {
+    ScCell cell[256];
+    uint4 acc4[SC_W * SC_W];  // [window cell][channel]
     // phase 1 reads the window flags (rows of SC_TS bytes: 1 rock, 2 over
     // capacity) and the smooth_start lattice; phases 2 and 3 reuse the words
     // for the sorted runs (cell * 4 + channel)
-    __shared__ uint32_t s_u[512];
-    static_assert(SC_TF_WORDS + 18 <= 512 && SC_TF_WORDS % 2 == 0, "flags and lattice fit the run list's words");
+    uint32_t u[512];
+    uint32_t hist[SC_BUCKETS];
+    uint32_t nruns;
+};
+static_assert(SC_TF_WORDS + 18 <= 512 && SC_TF_WORDS % 2 == 0, "flags and lattice fit the run list's words");
+
+// The tile at blockIdx (x, y) by 256 threads (thx, thy in 0..15) that share
+// L and every barrier of the workgroup.
+__device__ __forceinline__ void b_scatter(ScLds& L, const uint32_t thx, const uint32_t thy, const uint3 blockIdx, Geo g,
+                                          RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
+                                          uint32_t* __restrict__ next, const float2* __restrict__ vel,
+                                          const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
+    ScCell* const s_cell = L.cell;
+    uint4* const s_acc4 = L.acc4;
+    uint32_t* const s_hist = L.hist;
+    uint32_t& s_nruns = L.nruns;
+    uint32_t* const s_u = L.u;
     uint32_t* const s_tf4 = s_u;
     float2* const s_lat = reinterpret_cast<float2*>(s_u + SC_TF_WORDS);
     uint16_t* const s_run = reinterpret_cast<uint16_t*>(s_u);
     uint32_t* const s_acc = reinterpret_cast<uint32_t*>(s_acc4);
     uint8_t* const s_tf = reinterpret_cast<uint8_t*>(s_tf4);
     if (ep) P.epoch += *ep;
-    const uint32_t tid = threadIdx.y * 16 + threadIdx.x;
+    const uint32_t tid = thy * 16 + thx;
     const uint32_t G = g.G, F = g.F;
     const int wx0 = (int)(blockIdx.x * SC_T) - SC_R, wy0 = (int)(blockIdx.y * SC_T) - SC_R;
     for (uint32_t k = tid; k < SC_W * SC_W; k += 256) s_acc4[k] = make_uint4(0u, 0u, 0u, 0u);
@@ -2950,7 +2970,7 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
     __syncthreads();
 
     // ---- phase 1: the cell's shared quantities ----
-    const uint32_t x = blockIdx.x * 16 + threadIdx.x, y = blockIdx.y * 16 + threadIdx.y;
+    const uint32_t x = blockIdx.x * 16 + thx, y = blockIdx.y * 16 + thy;
     const uint32_t idx = y * G + x;
     uint32_t src_total = 0u;
     uint4 cw = make_uint4(0u, 0u, 0u, 0u);
@@ -3112,10 +3132,10 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
     const uint32_t m0 = movers_of(cw.x), m1 = movers_of(cw.y), m2 = movers_of(cw.z), m3 = movers_of(cw.w);
     const uint32_t b0 = cls + rb_minu(m0, SC_MB - 1u), b1 = cls + rb_minu(m1, SC_MB - 1u);
     const uint32_t b2 = cls + rb_minu(m2, SC_MB - 1u), b3 = cls + rb_minu(m3, SC_MB - 1u);
-    if (m0) rk0 = atomicAdd(&s_hist[b0], 1u);
-    if (m1) rk1 = atomicAdd(&s_hist[b1], 1u);
-    if (m2) rk2 = atomicAdd(&s_hist[b2], 1u);
-    if (m3) rk3 = atomicAdd(&s_hist[b3], 1u);
+    if (m0) rk0 = lds_add(&s_hist[b0], 1u);
+    if (m1) rk1 = lds_add(&s_hist[b1], 1u);
+    if (m2) rk2 = lds_add(&s_hist[b2], 1u);
+    if (m3) rk3 = lds_add(&s_hist[b3], 1u);
     __syncthreads();
     if (tid < 64) {  // exclusive scan of the bucket counts; lane l holds buckets 2l and 2l + 1
         const uint32_t h0 = s_hist[2 * tid], h1 = s_hist[2 * tid + 1];
@@ -3223,7 +3243,7 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         const uint32_t act_mv = rb_minu(act_n, movers);
         const uint32_t act_stay = act_n - act_mv;
         const uint32_t stay = act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
-        if (stay > 0u) atomicAdd(s_acc + lself * 4 + (int)ch, stay);
+        if (stay > 0u) lds_add(s_acc + lself * 4 + (int)ch, stay);
 
         // the run's PCG4D polynomials in k: y = Y + k dY, z = Z + k dZ,
         // w = W + k D0 + k (k - 1) / 2 DD (mod 2^32)
@@ -3306,15 +3326,15 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
             const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
             if (all_direct) {
                 const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
-                atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
+                lds_add(s_acc + tgt * 4 + (int)ch, unit);
             } else if ((direct >> bi) & 1u) {
                 const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
-                atomicAdd(s_acc + tgt * 4 + (int)ch, unit);
+                lds_add(s_acc + tgt * 4 + (int)ch, unit);
             } else {
                 const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)cx, base_x), ox), 0, (int32_t)G - 1);
                 const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)cy, base_y), oy), 0, (int32_t)G - 1);
                 const int lx = tx - wx0, ly = ty - wy0;
-                if (lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W) atomicAdd(s_acc + (ly * SC_W + lx) * 4 + (int)ch, unit);
+                if (lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W) lds_add(s_acc + (ly * SC_W + lx) * 4 + (int)ch, unit);
                 else atomicAdd(next + ((uint32_t)ty * G + (uint32_t)tx) * 4u + ch, unit);
             }
         }
@@ -3329,10 +3349,36 @@ __device__ __forceinline__ void b_scatter(const uint3 blockIdx, const uint3 grid
         atomicAdd(next + ((uint32_t)gy * G + (uint32_t)gx) * 4u + (k & 3u), v);
     }
 }
-__global__ __launch_bounds__(256) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
+__global__ __launch_bounds__(256, 6) void k_scatter(Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
-    b_scatter(make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), make_uint3(::gridDim.x, ::gridDim.y, ::gridDim.z), g, P, ep, chem, next, vel, temp, agg_e);
+    __shared__ ScLds lds;
+    b_scatter(lds, threadIdx.x, threadIdx.y, make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), g, P, ep, chem, next, vel, temp,
+              agg_e);
+}
+
+// The transport scatter beside the solver's one-workgroup coarse levels, in
+// one launch: workgroup 0 runs the symmetric V-cycle's k_mg_block, which
+// leaves every other CU idle; every other workgroup runs SC_FUSE_TILES
+// scatter tiles of 256 threads each, tiles t0 + (blockIdx.x - 1)
+// SC_FUSE_TILES + (threadIdx.x >> 8) up to t1 (a sub-tile past t1 takes a
+// tile past the grid's last row: no cells, only the barriers).  Neither part
+// reads what the other writes; the scatter runs whether or not the solve is
+// done.  The V-cycle's LDS overlays the tiles' (it is at most
+// SC_FUSE_TILES sizeof(ScLds) bytes).
+template <class S>
+__global__ __launch_bounds__(1024) void k_mg_block_scatter(MgBlockArgs<S> a, const uint32_t* done, ScatterArgs sc, uint32_t t0,
+                                                           uint32_t t1) {
+    __shared__ ScLds lds[SC_FUSE_TILES];
+    if (blockIdx.x == 0) {
+        b_mg_block<S, true>(make_uint3(0u, 0u, 0u), make_uint3(1u, 1u, 1u), a, done, reinterpret_cast<float*>(lds));
+        return;
+    }
+    const uint32_t sub = threadIdx.x >> 8, t = threadIdx.x & 255u;
+    const uint32_t tiles_x = (sc.g.G + 15u) / 16u;
+    const uint32_t tile = t0 + (blockIdx.x - 1u) * SC_FUSE_TILES + sub;
+    const uint3 tb = tile < t1 ? make_uint3(tile % tiles_x, tile / tiles_x, 0u) : make_uint3(0u, tiles_x, 0u);
+    b_scatter(lds[sub], t & 15u, t >> 4, tb, sc.g, sc.P, sc.ep, sc.chem, sc.next, sc.vel, sc.temp, sc.agg_e);
 }
 
 // ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state
@@ -3473,12 +3519,13 @@ __global__ __launch_bounds__(256) void kb_clear_next(const BChem* __restrict__ t
     BATCH_BLOCK(tab)
     b_fill_u32(bid, gdim, a.next, (size_t)4 * a.g.G * a.g.G, 0u);
 }
-__global__ __launch_bounds__(256) void kb_scatter(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
+__global__ __launch_bounds__(256, 6) void kb_scatter(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map,
                                                   uint32_t step) {
     BATCH_BLOCK(tab)
     RiboSimParams P = a.P;
     P.epoch = step;
-    b_scatter(bid, gdim, a.g, P, a.ep, a.chem, a.next, a.vel, a.temp, a.agg_e);
+    __shared__ ScLds lds;
+    b_scatter(lds, threadIdx.x, threadIdx.y, bid, a.g, P, a.ep, a.chem, a.next, a.vel, a.temp, a.agg_e);
 }
 __global__ __launch_bounds__(256) void kb_ledger(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map) {
     BATCH_BLOCK(tab)
@@ -3944,6 +3991,26 @@ void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
                     uint32_t* next, const float2* vel, const float* temp, const uint32_t* agg_e) {
     k_scatter<<<grid2(g.G), B16, 0, s>>>(g, P, ep, chem, next, vel, temp, agg_e);
 }
+uint32_t scatter_tiles(uint32_t G) { return ((G + 15u) / 16u) * ((G + 15u) / 16u); }
+template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count) {
+    size_t lds = 0;
+    (void)mg_block_args<S>(lv, count, nullptr, nullptr, 0, &lds, nullptr);
+    return lds <= SC_FUSE_TILES * sizeof(ScLds);
+}
+template <class S>
+void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
+                             const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1) {
+    size_t lds = 0;
+    const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds, cm_dev);
+    const uint32_t wgs = 1u + (t1 - t0 + SC_FUSE_TILES - 1u) / SC_FUSE_TILES;
+    k_mg_block_scatter<S><<<wgs, 1024, 0, s>>>(a, done, sc, t0, t1);
+}
+template bool mg_block_scatter_fits<float>(const MgLevelT<float>*, int);
+template bool mg_block_scatter_fits<bf16>(const MgLevelT<bf16>*, int);
+template void launch_mg_block_scatter<float>(hipStream_t, const MgLevelT<float>*, int, const double*, const int32_t*, int,
+                                             const uint32_t*, const int*, const ScatterArgs&, uint32_t, uint32_t);
+template void launch_mg_block_scatter<bf16>(hipStream_t, const MgLevelT<bf16>*, int, const double*, const int32_t*, int,
+                                            const uint32_t*, const int*, const ScatterArgs&, uint32_t, uint32_t);
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check) {
     uint32_t cells = g.G * g.G;

@@ -402,16 +402,38 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
 // conjugate-gradient kernels.  (Stopping at 16 x 16 with a 256-unknown
 // inverse was measured: the iterations stayed the same at 1024^2 and the
 // one-workgroup inverse product was slower than the levels it replaced.)
-template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done) {
+// Tiles [t0, t1) of the pending scatter that V-cycle j of n carries: the
+// tiles in as few slices as fit the CUs beside k_mg_block (one workgroup
+// each, SC_FUSE_TILES tiles a workgroup), and in every V-cycle when there
+// are fewer V-cycles than that.
+void fused_slice(const RiboWorld* w, uint32_t j, uint32_t n, uint32_t* t0, uint32_t* t1) {
+    const uint32_t tiles = scatter_tiles(w->G);
+    const uint32_t per_launch = (w->cu_count - 1u) * SC_FUSE_TILES;
+    const uint32_t slices = std::max(std::min((tiles + per_launch - 1u) / per_launch, n), 1u);
+    const uint32_t per = (tiles + slices - 1u) / slices;
+    *t0 = std::min(j * per, tiles);
+    *t1 = std::min(*t0 + per, tiles);
+}
+
+template <class S>
+void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done, uint32_t j = 0, uint32_t n = 0) {
     const size_t L = lv.size(), lc = w->sym_coarse;
     hipStream_t s = w->stream;
     if (w->mg_block_start < L) {
         const size_t b = std::min(std::max<size_t>(w->mg_block_start, 1), lc);
         for (size_t l = 1; l < b; ++l)
             run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
+        uint32_t t0 = 0, t1 = 0;
+        if (w->pend.on) fused_slice(w, j, n, &t0, &t1);
         run(w, "mg_block", [&] {
-            launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done,
-                               w->cmoving ? w->cm_dev : nullptr);
+            const int* cmd = w->cmoving ? w->cm_dev : nullptr;
+            if (t1 > t0) {
+                const RiboWorld::PendingScatter& q = w->pend;
+                const ScatterArgs sa{w->geo(), q.P, q.ep, q.chem, q.next, w->vel_s, q.temp, w->agg_e};
+                launch_mg_block_scatter<S>(s, lv.data() + b, (int)(lc - b + 1), w->cinv, w->ccell, w->cm, done, cmd, sa, t0, t1);
+            } else {
+                launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done, cmd);
+            }
         });
         for (size_t l = b; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
         return;
@@ -422,6 +444,18 @@ template <class S> void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevel
         run(w, "mg_down", [&] { launch_mg_down<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
     run(w, "mg_zero", [&] { launch_zero_level<S>(s, lv[L - 1]); });
     for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
+}
+
+// The end of a deferred scatter: its tiles ran in the V-cycles (ran), or it
+// runs now on its own; then the ledger of its output.
+void finish_scatter(RiboWorld* w, bool ran) {
+    RiboWorld::PendingScatter& q = w->pend;
+    if (!q.on) return;
+    hipStream_t s = w->stream;
+    const Geo g = w->geo();
+    if (!ran) run(w, "transport_scatter", [&] { launch_scatter(s, g, q.P, q.ep, q.chem, q.next, w->vel_s, q.temp, w->agg_e); });
+    q.on = false;
+    if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, q.next, w->ledger, q.P.epoch, q.ep, 1); });
 }
 
 // Tolerance mode: preconditioned conjugate gradients (Chronopoulos-Gear
@@ -463,10 +497,11 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
         const int stage = j == 0 ? 0 : (j == 1 ? 1 : 2);
         const int rd = j == 0 ? 1 : (int)(j & 1u);
         run(w, "cg_down0", [&] { launch_cg_down0<S>(s, V, lv[1], w->pcg, (int)(j & 1u), stage, rd, tol, maxit); });
-        vcycle_sym_inner<S>(w, lv, done);
+        vcycle_sym_inner<S>(w, lv, done, j, n_it + 1u);
         run(w, "cg_up0", [&] { launch_cg_up0<S>(s, V, lv[1], w->pcg, (int)(j & 1u)); });
     }
     run(w, "pcg_final", [&] { launch_cg_final<S>(s, V, w->pcg, (int)((n_it - 1u) & 1u), tol, maxit); });
+    finish_scatter(w, w->mg_block_start < lv.size());
 }
 
 // Stationary tolerance mode: the reference V-cycle (as vcycle() records it)
@@ -570,7 +605,7 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     run(w, "subtract_gradient", [&] { launch_subgrad(s, g, w->vel_a, w->vel_b, w->p_a); });
     run(w, "net_flux", [&] { launch_net_flux(s, g, w->vel_b, w->net_flux); });
     if (!w->pipe) {
-        run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux); });
+        run(w, "enforce_boundaries", [&] { launch_boundaries(s, g, w->vel_b, w->vel_a, w->net_flux, w->fuse ? w->vel_s : nullptr); });
         run(w, "thermal_activation", [&] { launch_thermal(s, g, P, ep, T, w->chem[w->cur]); });
         return;
     }
@@ -586,13 +621,24 @@ void fluid_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // 
     w->pipe_temp_read[w->tcur] = true;
 }
 
-void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep) {  // src/world/mod.rs:1594-1601
+void transport_step(RiboWorld* w, const RiboSimParams& P, const uint32_t* ep, bool defer) {  // src/world/mod.rs:1594-1601
     hipStream_t s = w->pipe ? w->tstream : w->stream;
     Geo g = w->geo();
     uint32_t* cur = w->chem[w->cur];
     uint32_t* nxt = w->chem[1 - w->cur];
     if (P.aggregation > 0.0f) run(w, "agg", [&] { launch_agg(s, g, cur, w->agg_e); });
     run(w, "clear_next", [&] { launch_fill_u32(s, nxt, (size_t)4 * w->G * w->G, 0u); });
+    if (defer) {  // the scatter rides on the next solve (finish_scatter adds the ledger)
+        RiboWorld::PendingScatter& q = w->pend;
+        q.on = true;
+        q.P = P;
+        q.ep = ep;
+        q.chem = cur;
+        q.next = nxt;
+        q.temp = w->temp[w->tcur];
+        w->cur = 1 - w->cur;  // transport_commit
+        return;
+    }
     const float2* vel = w->pipe ? w->vel_s : w->vel_a;
     run(w, "transport_scatter", [&] { launch_scatter(s, g, P, ep, cur, nxt, vel, w->temp[w->tcur], w->agg_e); });
     if (w->pipe) {
@@ -648,8 +694,9 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
         if (w->st.terrain_enabled) terrain_step(w, P, ep);
         light_step(w, P, epoch, ep);
         if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
-        if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep);
+        if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep, w->fuse && i + 1 < count);
     }
+    finish_scatter(w, false);
     if (w->pipe) {  // ... and everything after waits for it
         (void)hipEventRecord(w->pe_join, w->tstream);
         (void)hipStreamWaitEvent(w->stream, w->pe_join, 0);
@@ -668,6 +715,22 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
 bool pipeline_ok(const RiboWorld* w) {
     return w->pipe_allowed && !w->timing && !w->st.light_mode && !w->st.terrain_enabled && !(w->params.sun_heat > 0.0f) &&
            !(w->st.multigrid && w->st.mg_tol > 0.0f);
+}
+
+// The scatter rides on the next step's solve (RiboWorld::fuse) under the
+// conditions of pipeline_ok, in the conjugate-gradient tolerance mode with a
+// one-workgroup V-cycle kernel whose LDS fits the scatter tiles', when every
+// step has one fluid step and one transport.
+bool fuse_ok(const RiboWorld* w, uint32_t substep) {
+    if (!w->pipe_allowed || w->timing || w->st.light_mode || w->st.terrain_enabled || w->params.sun_heat > 0.0f) return false;
+    if (!w->st.fluid_enabled || !w->st.multigrid || !(w->st.mg_tol > 0.0f) || w->st.mg_stationary) return false;
+    if (substep != 1u || std::max(w->params.transport_every, 1u) != 1u) return false;
+    const size_t L = w->mgz.size();
+    if (w->mg_block_start >= L || L < 2) return false;
+    const size_t b = std::min(std::max<size_t>(w->mg_block_start, 1), w->sym_coarse);
+    const int count = (int)(w->sym_coarse - b + 1);
+    return w->st.precision == RIBO_PRECISION_MIXED ? mg_block_scatter_fits<bf16>(w->mgh.data() + b, count)
+                                                   : mg_block_scatter_fits<float>(w->mgz.data() + b, count);
 }
 
 // The transport stream and the events, made when a world first pipelines
@@ -875,6 +938,7 @@ int ribo_world_create(const RiboWorldConfig* cfg, uint32_t seed, RiboWorld** out
     // k_mg_block takes its LDS dynamically, up to the workgroup maximum
     // (160 KB on gfx950)
     w->lds_limit = (size_t)prop.sharedMemPerBlock;
+    w->cu_count = (uint32_t)std::max(prop.multiProcessorCount, 2);
     ribo_sim_params_fill_default(&w->params);
     w->params.seed = seed;
     ribo_world_settings_default(&w->st);
@@ -1146,6 +1210,7 @@ int ribo_world_encode_steps(RiboWorld* w, uint32_t steps) {  // src/world/mod.rs
     w->params.fluid_dt = w->params.dt * (float)substep;
     w->params.fluid_enabled = st.fluid_enabled ? 1u : 0u;
     w->pipe = pipeline_ok(w);
+    w->fuse = !w->pipe && fuse_ok(w, substep);
     if (w->pipe && (rc = ensure_pipeline(w))) return rc;
     if (w->pipe)  // the transport's copy of the velocity, until a fluid step writes it
         HIPCHK(hipMemcpyAsync(w->vel_s, w->vel_a, sizeof(float2) * (size_t)w->F * w->F, hipMemcpyDeviceToDevice, w->stream));
@@ -1420,6 +1485,7 @@ int ribo_world_ledger_status(RiboWorld* w, uint32_t* epoch) {
 
 int ribo_world_set_pipeline(RiboWorld* w, int enable) {
     if (!w) return fail(RIBO_ERR_ARG, "null world");
+    if (w->pipe_allowed != (enable != 0)) clear_graphs(w);  // recordings differ
     w->pipe_allowed = enable != 0;
     return RIBO_OK;
 }

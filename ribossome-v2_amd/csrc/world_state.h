@@ -61,6 +61,24 @@ struct RiboWorld {
     hipEvent_t pe_fork = nullptr, pe_join = nullptr, pe_fluid = nullptr, pe_temp[2] = {nullptr, nullptr}, pe_vel = nullptr;
     bool pipe = false, pipe_allowed = true;
     bool pipe_temp_read[2] = {false, false}, pipe_vel_read = false;
+    // Tolerance mode under the same conditions (and one fluid step and one
+    // transport per step): a step's scatter runs inside the next step's
+    // solve, its tiles spread over the V-cycles' k_mg_block launches, which
+    // otherwise keep one CU busy (launch_mg_block_scatter).  pend holds the
+    // deferred scatter from its transport step to the next solve; it reads
+    // the step's velocity from vel_s and writes chem[cur] (already
+    // committed), which nothing reads before the solve ends.  A recording
+    // runs what is still pending at its end.
+    bool fuse = false;
+    struct PendingScatter {
+        bool on = false;
+        RiboSimParams P{};
+        const uint32_t* ep = nullptr;
+        const uint32_t* chem = nullptr;
+        uint32_t* next = nullptr;
+        const float* temp = nullptr;
+    } pend;
+    uint32_t cu_count = 0;
 
     // world-resolution buffers (bindings.wgsl group 1)
     uint32_t* chem[2] = {nullptr, nullptr};  // chem_grid / chem_next, ping-pong

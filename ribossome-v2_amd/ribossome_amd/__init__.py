@@ -580,8 +580,10 @@ class World:
         return {names[i].decode(): (ms[i], int(cnt[i])) for i in range(min(n.value, cap))}
 
     def set_pipeline(self, enable: bool):
-        """Pipeline the transport chain beside the next fluid step where the
-        step allows it (on by default)."""
+        """Overlap each step's transport with the next step's fluid where the
+        step allows it (on by default): a second stream in the reference
+        schedule, the scatter inside the next solve's V-cycles in the
+        tolerance mode."""
         _check(lib().ribo_world_set_pipeline(self._h, 1 if enable else 0), "set_pipeline")
 
     def kernel_launch_ms(self, name: str) -> np.ndarray:

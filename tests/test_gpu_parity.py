@@ -392,6 +392,43 @@ def test_pipelined_steps_match_one_stream(schedule):
         assert w.ledger_status() is None
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_fused_scatter_steps_match_one_stream(precision):
+    """The tolerance mode with each step's transport scatter run inside the
+    next step's V-cycles, beside the one-workgroup coarse levels (the
+    default), gives the same bits as the scatter on its own after each step,
+    through graph capture and replay and eager recording, with chunks that
+    end mid-graph, a 512^2 grid whose tiles take two V-cycles, and a velocity
+    upload between chunks; the ledger is checked every step."""
+    print(f"case fused scatter precision={precision}", flush=True)
+    cfg = ra.WorldConfig(512, 256)
+    inp = make_inputs(cfg, 23, rock=0.12, rubble=True, vent_rows=4, vcells_per_step=1.2)
+    worlds = []
+    for fused, no_graphs in ((False, 0), (True, 0), (True, 1)):
+        w = ra.World(cfg, 23)
+        load(w, inp, parity_params(23, thermal_activation=1.0, diffusion=5.0),
+             parity_settings(mg_tol=1e-5, mg_max_cycles=60, precision=precision, no_graphs=no_graphs, ledger_check=1))
+        w.set_pipeline(fused)
+        worlds.append(w)
+    rng = np.random.default_rng(23)
+    for k, chunk in enumerate((7, 12, 9, 1, 16)):
+        if k == 2:  # an upload between chunks
+            vel = (inp[ra.VELOCITY] * rng.uniform(0.5, 1.5)).astype(np.float32)
+            for w in worlds:
+                w.upload(ra.VELOCITY, vel)
+        for w in worlds:
+            assert w.encode_steps(chunk) == chunk
+        ref = worlds[0]
+        ref.sync()
+        for w in worlds[1:]:
+            w.sync()
+            assert np.array_equal(ref.download(ra.CHEM), w.download(ra.CHEM)), f"chunk {k}: chem differs"
+            for which in FIELDS:
+                assert bit_diff(ref.download(which), w.download(which)) == 0, f"chunk {k}: {ra.BUFFER_NAMES[which]}"
+    for w in worlds:
+        assert w.ledger_status() is None
+
+
 def test_reference_default_profile_graphs_match_eager():
     """The reference's default profile (day cycle, solar heating, light
     propagation, moving terrain, substep 2, transport every 3) replays as

@@ -2929,19 +2929,24 @@ struct ScLds {
     uint32_t u[512];
     uint32_t hist[SC_BUCKETS];
     uint32_t nruns;
+    uint32_t org;  // the tile: blockIdx.x | blockIdx.y << 16
 };
 static_assert(SC_TF_WORDS + 18 <= 512 && SC_TF_WORDS % 2 == 0, "flags and lattice fit the run list's words");
 
 // The tile at blockIdx (x, y) by 256 threads (thx, thy in 0..15) that share
-// L and every barrier of the workgroup.
-__device__ __forceinline__ void b_scatter(ScLds& L, const uint32_t thx, const uint32_t thy, const uint3 blockIdx, Geo g,
-                                          RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
+// Ls[sub] and every barrier of the workgroup.  NS tiles of one workgroup
+// (sub = 0 .. NS - 1) sort their runs together and share them out over all
+// NS x 256 threads, so that no tile's threads wait at the barrier after the
+// runs for another tile's longer ones.
+template <int NS>
+__device__ __forceinline__ void b_scatter(ScLds* Ls, const uint32_t sub, const uint32_t thx, const uint32_t thy, const uint3 blockIdx,
+                                          Geo g, RiboSimParams P, const uint32_t* ep, const uint32_t* __restrict__ chem,
                                           uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                           const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
+    ScLds& L = Ls[sub];
     ScCell* const s_cell = L.cell;
     uint4* const s_acc4 = L.acc4;
     uint32_t* const s_hist = L.hist;
-    uint32_t& s_nruns = L.nruns;
     uint32_t* const s_u = L.u;
     uint32_t* const s_tf4 = s_u;
     float2* const s_lat = reinterpret_cast<float2*>(s_u + SC_TF_WORDS);
@@ -2966,6 +2971,7 @@ __device__ __forceinline__ void b_scatter(ScLds& L, const uint32_t thx, const ui
         s_tf[ky * SC_TS + kx] = f;
     }
     if (tid < SC_BUCKETS) s_hist[tid] = 0u;
+    if (NS > 1 && tid == 0) L.org = blockIdx.x | (blockIdx.y << 16);
     if (tid < 9u) s_lat[tid] = lattice_rand(P, blockIdx.x + tid % 3u, blockIdx.y + tid / 3u);
     __syncthreads();
 
@@ -3137,37 +3143,62 @@ __device__ __forceinline__ void b_scatter(ScLds& L, const uint32_t thx, const ui
     if (m2) rk2 = lds_add(&s_hist[b2], 1u);
     if (m3) rk3 = lds_add(&s_hist[b3], 1u);
     __syncthreads();
-    if (tid < 64) {  // exclusive scan of the bucket counts; lane l holds buckets 2l and 2l + 1
-        const uint32_t h0 = s_hist[2 * tid], h1 = s_hist[2 * tid + 1];
-        uint32_t v = h0 + h1;
+    if (sub == 0 && tid < 64) {  // exclusive scan of the bucket counts; lane l holds buckets 2l and 2l + 1
+        uint32_t h0[NS], h1[NS], t0 = 0u, t1 = 0u;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            h0[k] = Ls[k].hist[2 * tid];
+            h1[k] = Ls[k].hist[2 * tid + 1];
+            t0 += h0[k];
+            t1 += h1[k];
+        }
+        uint32_t v = t0 + t1;
         for (uint32_t off = 1; off < 64; off <<= 1) {
             const uint32_t u = __shfl_up(v, off, 64);
             if (tid >= off) v += u;
         }
-        const uint32_t ex = v - (h0 + h1);
-        s_hist[2 * tid] = ex;
-        s_hist[2 * tid + 1] = ex + h0;
-        if (tid == 63) s_nruns = v;
+        // a bucket's runs: tile 0's, then tile 1's, ...
+        uint32_t o0 = v - (t0 + t1), o1 = o0 + t0;
+#pragma unroll
+        for (int k = 0; k < NS; ++k) {
+            Ls[k].hist[2 * tid] = o0;
+            Ls[k].hist[2 * tid + 1] = o1;
+            o0 += h0[k];
+            o1 += h1[k];
+        }
+        if (tid == 63) Ls[0].nruns = v;
     }
     __syncthreads();
-    if (m0) s_run[s_hist[b0] + rk0] = (uint16_t)(tid * 4u + 0u);
-    if (m1) s_run[s_hist[b1] + rk1] = (uint16_t)(tid * 4u + 1u);
-    if (m2) s_run[s_hist[b2] + rk2] = (uint16_t)(tid * 4u + 2u);
-    if (m3) s_run[s_hist[b3] + rk3] = (uint16_t)(tid * 4u + 3u);
+    // run list entries: tile << 10 | cell * 4 + channel; entry p lies in
+    // tile p >> 10's words
+    auto put_run = [&](uint32_t p, uint32_t c) {
+        const uint16_t e = (uint16_t)((NS > 1 ? sub << 10 : 0u) | (tid * 4u + c));
+        if (NS == 1) s_run[p] = e;
+        else reinterpret_cast<uint16_t*>(Ls[p >> 10].u)[p & 1023u] = e;
+    };
+    if (m0) put_run(s_hist[b0] + rk0, 0u);
+    if (m1) put_run(s_hist[b1] + rk1, 1u);
+    if (m2) put_run(s_hist[b2] + rk2, 2u);
+    if (m3) put_run(s_hist[b3] + rk3, 3u);
     __syncthreads();
 
     // ---- phase 3: the runs ----
-    const uint32_t nruns = s_nruns;
+    const uint32_t nruns = Ls[0].nruns;
     const float tn = (float)rb_maxu(P.transport_every, 1u);
     // PCG4D words that do not depend on key or stream
     const uint32_t y0 = P.epoch * LCG_A + LCG_C, w0 = P.seed * LCG_A + LCG_C, y0w0 = y0 * w0;
     constexpr uint32_t Z0 = S_MOVE * LCG_A + LCG_C, ZJ0 = S_DISPERSE * LCG_A + LCG_C;
-    for (uint32_t pos = tid; pos < nruns; pos += 256) {
-        const uint32_t run = s_run[pos];
-        const uint32_t cell = run >> 2, ch = run & 3u;
-        const uint32_t cx = blockIdx.x * 16 + (cell & 15u), cy = blockIdx.y * 16 + (cell >> 4);
+    for (uint32_t pos = sub * 256u + tid; pos < nruns; pos += 256u * NS) {
+        const uint32_t e = NS == 1 ? s_run[pos] : reinterpret_cast<const uint16_t*>(Ls[pos >> 10].u)[pos & 1023u];
+        const uint32_t run = e & 1023u, cell = run >> 2, ch = run & 3u;
+        ScLds& Lr = Ls[NS == 1 ? 0u : e >> 10];  // the run's tile
+        uint32_t* const r_acc = reinterpret_cast<uint32_t*>(Lr.acc4);
+        const uint32_t org = NS == 1 ? 0u : Lr.org;
+        const uint32_t rbx = NS == 1 ? blockIdx.x : (org & 0xFFFFu), rby = NS == 1 ? blockIdx.y : (org >> 16);
+        const int rwx0 = (int)(rbx * SC_T) - SC_R, rwy0 = (int)(rby * SC_T) - SC_R;
+        const uint32_t cx = rbx * 16 + (cell & 15u), cy = rby * 16 + (cell >> 4);
         const uint32_t cidx = cy * G + cx;
-        const ScCell& R = s_cell[cell];
+        const ScCell& R = Lr.cell[cell];
         const float4 ra = R.a, rp = R.pr, rv = R.ac;
         const uint4 rd = R.b;
         const float p_bound = rb_as_f32(rd.x);
@@ -3243,7 +3274,7 @@ __device__ __forceinline__ void b_scatter(ScLds& L, const uint32_t thx, const ui
         const uint32_t act_mv = rb_minu(act_n, movers);
         const uint32_t act_stay = act_n - act_mv;
         const uint32_t stay = act_stay + (n - movers - act_stay) * CHEM_SPENT_ONE;
-        if (stay > 0u) lds_add(s_acc + lself * 4 + (int)ch, stay);
+        if (stay > 0u) lds_add(r_acc + lself * 4 + (int)ch, stay);
 
         // the run's PCG4D polynomials in k: y = Y + k dY, z = Z + k dZ,
         // w = W + k D0 + k (k - 1) / 2 DD (mod 2^32)
@@ -3326,15 +3357,15 @@ __device__ __forceinline__ void b_scatter(ScLds& L, const uint32_t thx, const ui
             const uint32_t unit = is_act ? 1u : CHEM_SPENT_ONE;
             if (all_direct) {
                 const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
-                lds_add(s_acc + tgt * 4 + (int)ch, unit);
+                lds_add(r_acc + tgt * 4 + (int)ch, unit);
             } else if ((direct >> bi) & 1u) {
                 const int32_t tgt = ((blocked >> bi) & 1u) ? lself : rb_addi(lbase, (int32_t)__umul24((uint32_t)oy, SC_W) + ox);
-                lds_add(s_acc + tgt * 4 + (int)ch, unit);
+                lds_add(r_acc + tgt * 4 + (int)ch, unit);
             } else {
                 const int32_t tx = rb_clampi(rb_addi(rb_addi((int32_t)cx, base_x), ox), 0, (int32_t)G - 1);
                 const int32_t ty = rb_clampi(rb_addi(rb_addi((int32_t)cy, base_y), oy), 0, (int32_t)G - 1);
-                const int lx = tx - wx0, ly = ty - wy0;
-                if (lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W) lds_add(s_acc + (ly * SC_W + lx) * 4 + (int)ch, unit);
+                const int lx = tx - rwx0, ly = ty - rwy0;
+                if (lx >= 0 && ly >= 0 && lx < SC_W && ly < SC_W) lds_add(r_acc + (ly * SC_W + lx) * 4 + (int)ch, unit);
                 else atomicAdd(next + ((uint32_t)ty * G + (uint32_t)tx) * 4u + ch, unit);
             }
         }
@@ -3353,8 +3384,8 @@ __global__ __launch_bounds__(256, 6) void k_scatter(Geo g, RiboSimParams P, cons
                                                  uint32_t* __restrict__ next, const float2* __restrict__ vel,
                                                  const float* __restrict__ temp, const uint32_t* __restrict__ agg_e) {
     __shared__ ScLds lds;
-    b_scatter(lds, threadIdx.x, threadIdx.y, make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), g, P, ep, chem, next, vel, temp,
-              agg_e);
+    b_scatter<1>(&lds, 0u, threadIdx.x, threadIdx.y, make_uint3(::blockIdx.x, ::blockIdx.y, ::blockIdx.z), g, P, ep, chem, next, vel,
+                 temp, agg_e);
 }
 
 // The transport scatter beside the solver's one-workgroup coarse levels, in
@@ -3378,7 +3409,7 @@ __global__ __launch_bounds__(1024) void k_mg_block_scatter(MgBlockArgs<S> a, con
     const uint32_t tiles_x = (sc.g.G + 15u) / 16u;
     const uint32_t tile = t0 + (blockIdx.x - 1u) * SC_FUSE_TILES + sub;
     const uint3 tb = tile < t1 ? make_uint3(tile % tiles_x, tile / tiles_x, 0u) : make_uint3(0u, tiles_x, 0u);
-    b_scatter(lds[sub], t & 15u, t >> 4, tb, sc.g, sc.P, sc.ep, sc.chem, sc.next, sc.vel, sc.temp, sc.agg_e);
+    b_scatter<SC_FUSE_TILES>(lds, sub, t & 15u, t >> 4, tb, sc.g, sc.P, sc.ep, sc.chem, sc.next, sc.vel, sc.temp, sc.agg_e);
 }
 
 // ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state
@@ -3525,7 +3556,7 @@ __global__ __launch_bounds__(256, 6) void kb_scatter(const BChem* __restrict__ t
     RiboSimParams P = a.P;
     P.epoch = step;
     __shared__ ScLds lds;
-    b_scatter(lds, threadIdx.x, threadIdx.y, bid, a.g, P, a.ep, a.chem, a.next, a.vel, a.temp, a.agg_e);
+    b_scatter<1>(&lds, 0u, threadIdx.x, threadIdx.y, bid, a.g, P, a.ep, a.chem, a.next, a.vel, a.temp, a.agg_e);
 }
 __global__ __launch_bounds__(256) void kb_ledger(const BChem* __restrict__ tab, const BatchBlock* __restrict__ map) {
     BATCH_BLOCK(tab)

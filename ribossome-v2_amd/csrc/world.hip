@@ -1143,8 +1143,8 @@ int ribo_world_download(RiboWorld* w, RiboBuffer which, void* dst, size_t bytes)
 // Iteration slots per solve for the next graph chunk of the tolerance mode
 // (read after draining the stream, so the choice depends only on the
 // sequence of calls): the most iterations any of the last SLOT_WINDOW
-// solves took, plus max(4, 1/3 of it) (max(8, 1/2 of it) for the stationary
-// V-cycle, whose counts move more), rounded up to a multiple of 4 (few
+// solves took, plus max(3, 1/4 of it) (max(8, 1/2 of it) for the stationary
+// V-cycle, whose counts move more), rounded up to an even count (fewer
 // distinct graphs).  The choice is made again before every chunk of at most
 // TOL_CHUNK steps, so it is never older than that.  The window spans two
 // periods of the buoyancy noise (a new bucket every 16 epochs, fluid.wgsl:384,
@@ -1171,9 +1171,9 @@ static int choose_slots(RiboWorld* w) {
             most = std::max(most, __atomic_load_n(&w->pcg_host[PCG_HOST_RING + (solves - 1u - k) % PCG_HOST_RING_LEN],
                                                   __ATOMIC_RELAXED));
         if (known >= 8u) {
-            const uint32_t margin = w->st.mg_stationary ? std::max(8u, most / 2u) : std::max(4u, most / 3u);
+            const uint32_t margin = w->st.mg_stationary ? std::max(8u, most / 2u) : std::max(3u, most / 4u);
             const uint32_t want = missed != w->missed_seen ? 2u * most + 4u : most + margin;
-            slots = std::min(maxit, (want + 3u) & ~3u);
+            slots = std::min(maxit, (want + 1u) & ~1u);
         }
         w->solves_seen = solves;
     }

@@ -348,6 +348,8 @@ struct ScatterArgs {
 constexpr uint32_t SC_FUSE_TILES = 4;
 uint32_t scatter_tiles(uint32_t G);
 template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count);
+// Tiles [t0, t1) of the scatter alone, one 256-thread workgroup a tile.
+void launch_scatter_range(hipStream_t s, const ScatterArgs& a, uint32_t t0, uint32_t t1);
 template <class S>
 void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
                              const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1);

@@ -3388,6 +3388,14 @@ __global__ __launch_bounds__(256, 6) void k_scatter(Geo g, RiboSimParams P, cons
                  temp, agg_e);
 }
 
+// Tiles t0 + blockIdx.x of the scatter (row-major over the grid's tiles).
+__global__ __launch_bounds__(256, 6) void k_scatter_range(ScatterArgs a, uint32_t t0) {
+    __shared__ ScLds lds;
+    const uint32_t tiles_x = (a.g.G + 15u) / 16u, t = t0 + blockIdx.x;
+    b_scatter<1>(&lds, 0u, threadIdx.x, threadIdx.y, make_uint3(t % tiles_x, t / tiles_x, 0u), a.g, a.P, a.ep, a.chem, a.next, a.vel,
+                 a.temp, a.agg_e);
+}
+
 // The transport scatter beside the solver's one-workgroup coarse levels, in
 // one launch: workgroup 0 runs the symmetric V-cycle's k_mg_block, which
 // leaves every other CU idle; every other workgroup runs SC_FUSE_TILES
@@ -4035,6 +4043,9 @@ void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, co
     const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds, cm_dev);
     const uint32_t wgs = 1u + (t1 - t0 + SC_FUSE_TILES - 1u) / SC_FUSE_TILES;
     k_mg_block_scatter<S><<<wgs, 1024, 0, s>>>(a, done, sc, t0, t1);
+}
+void launch_scatter_range(hipStream_t s, const ScatterArgs& a, uint32_t t0, uint32_t t1) {
+    if (t1 > t0) k_scatter_range<<<t1 - t0, B16, 0, s>>>(a, t0);
 }
 template bool mg_block_scatter_fits<float>(const MgLevelT<float>*, int);
 template bool mg_block_scatter_fits<bf16>(const MgLevelT<bf16>*, int);

@@ -397,24 +397,36 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
         run(w, "mg_up", [&] { launch_mg_up<float>(s, w->mg[l], w->mg[l + 1], false, done, w->mg_tile[l]); });
 }
 
+// Tiles [t0, t1) of the pending scatter that V-cycle j of n carries: the
+// tiles spread evenly over as few slices as fit the CUs beside k_mg_block
+// (one workgroup each, SC_FUSE_TILES tiles a workgroup, so that every
+// workgroup of a launch runs at once).  With fewer V-cycles than that, each
+// takes a full slice and the tiles from fused_rest(n) on run on their own
+// after the solve.
+uint32_t fused_per(const RiboWorld* w, uint32_t n, uint32_t* slices) {
+    const uint32_t tiles = scatter_tiles(w->G);
+    const uint32_t cap = (w->cu_count - 1u) * SC_FUSE_TILES;
+    const uint32_t need = (tiles + cap - 1u) / cap;
+    *slices = std::max(std::min(need, n), 1u);
+    return *slices == need ? (tiles + need - 1u) / need : cap;
+}
+void fused_slice(const RiboWorld* w, uint32_t j, uint32_t n, uint32_t* t0, uint32_t* t1) {
+    uint32_t slices = 0;
+    const uint32_t per = fused_per(w, n, &slices), tiles = scatter_tiles(w->G);
+    *t0 = j < slices ? std::min(j * per, tiles) : tiles;
+    *t1 = j < slices ? std::min(*t0 + per, tiles) : tiles;
+}
+uint32_t fused_rest(const RiboWorld* w, uint32_t n) {
+    uint32_t slices = 0;
+    const uint32_t per = fused_per(w, n, &slices);
+    return std::min(slices * per, scatter_tiles(w->G));
+}
+
 // Levels 1 .. sym_coarse (the coarsest, solved exactly with its dense
 // inverse) of the symmetric V-cycle; level 0 is fused into the
 // conjugate-gradient kernels.  (Stopping at 16 x 16 with a 256-unknown
 // inverse was measured: the iterations stayed the same at 1024^2 and the
 // one-workgroup inverse product was slower than the levels it replaced.)
-// Tiles [t0, t1) of the pending scatter that V-cycle j of n carries: the
-// tiles in as few slices as fit the CUs beside k_mg_block (one workgroup
-// each, SC_FUSE_TILES tiles a workgroup), and in every V-cycle when there
-// are fewer V-cycles than that.
-void fused_slice(const RiboWorld* w, uint32_t j, uint32_t n, uint32_t* t0, uint32_t* t1) {
-    const uint32_t tiles = scatter_tiles(w->G);
-    const uint32_t per_launch = (w->cu_count - 1u) * SC_FUSE_TILES;
-    const uint32_t slices = std::max(std::min((tiles + per_launch - 1u) / per_launch, n), 1u);
-    const uint32_t per = (tiles + slices - 1u) / slices;
-    *t0 = std::min(j * per, tiles);
-    *t1 = std::min(*t0 + per, tiles);
-}
-
 template <class S>
 void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const uint32_t* done, uint32_t j = 0, uint32_t n = 0) {
     const size_t L = lv.size(), lc = w->sym_coarse;
@@ -446,14 +458,20 @@ void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const ui
     for (size_t l = L - 1; l-- > 1;) run(w, "mg_up", [&] { launch_mg_up<S>(s, lv[l], lv[l + 1], true, done, w->mg_tile[l]); });
 }
 
-// The end of a deferred scatter: its tiles ran in the V-cycles (ran), or it
-// runs now on its own; then the ledger of its output.
-void finish_scatter(RiboWorld* w, bool ran) {
+// The end of a deferred scatter: its tiles below `rest` ran in the
+// V-cycles, the others run now on their own (all of them when rest is 0);
+// then the ledger of its output.
+void finish_scatter(RiboWorld* w, uint32_t rest) {
     RiboWorld::PendingScatter& q = w->pend;
     if (!q.on) return;
     hipStream_t s = w->stream;
     const Geo g = w->geo();
-    if (!ran) run(w, "transport_scatter", [&] { launch_scatter(s, g, q.P, q.ep, q.chem, q.next, w->vel_s, q.temp, w->agg_e); });
+    const uint32_t tiles = scatter_tiles(w->G);
+    if (rest == 0u) run(w, "transport_scatter", [&] { launch_scatter(s, g, q.P, q.ep, q.chem, q.next, w->vel_s, q.temp, w->agg_e); });
+    else if (rest < tiles)
+        run(w, "transport_scatter", [&] {
+            launch_scatter_range(s, ScatterArgs{g, q.P, q.ep, q.chem, q.next, w->vel_s, q.temp, w->agg_e}, rest, tiles);
+        });
     q.on = false;
     if (w->st.ledger_check) run(w, "ledger", [&] { launch_ledger(s, g, q.next, w->ledger, q.P.epoch, q.ep, 1); });
 }
@@ -501,7 +519,7 @@ template <class S> void pcg_solve(RiboWorld* w, const std::vector<MgLevelT<S>>& 
         run(w, "cg_up0", [&] { launch_cg_up0<S>(s, V, lv[1], w->pcg, (int)(j & 1u)); });
     }
     run(w, "pcg_final", [&] { launch_cg_final<S>(s, V, w->pcg, (int)((n_it - 1u) & 1u), tol, maxit); });
-    finish_scatter(w, w->mg_block_start < lv.size());
+    finish_scatter(w, w->mg_block_start < lv.size() ? fused_rest(w, n_it + 1u) : 0u);
 }
 
 // Stationary tolerance mode: the reference V-cycle (as vcycle() records it)
@@ -696,7 +714,7 @@ void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep
         if (w->st.fluid_enabled && epoch % substep == 0) fluid_step(w, P, ep);
         if (epoch % std::max(P.transport_every, 1u) == 0) transport_step(w, P, ep, w->fuse && i + 1 < count);
     }
-    finish_scatter(w, false);
+    finish_scatter(w, 0u);
     if (w->pipe) {  // ... and everything after waits for it
         (void)hipEventRecord(w->pe_join, w->tstream);
         (void)hipStreamWaitEvent(w->stream, w->pe_join, 0);

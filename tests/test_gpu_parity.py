@@ -392,22 +392,24 @@ def test_pipelined_steps_match_one_stream(schedule):
         assert w.ledger_status() is None
 
 
-@pytest.mark.parametrize("precision", [0, 1])
-def test_fused_scatter_steps_match_one_stream(precision):
+@pytest.mark.parametrize("precision,grid,cycles", [(0, 512, 60), (1, 512, 60), (0, 1024, 3)])
+def test_fused_scatter_steps_match_one_stream(precision, grid, cycles):
     """The tolerance mode with each step's transport scatter run inside the
     next step's V-cycles, beside the one-workgroup coarse levels (the
     default), gives the same bits as the scatter on its own after each step,
     through graph capture and replay and eager recording, with chunks that
-    end mid-graph, a 512^2 grid whose tiles take two V-cycles, and a velocity
-    upload between chunks; the ledger is checked every step."""
-    print(f"case fused scatter precision={precision}", flush=True)
-    cfg = ra.WorldConfig(512, 256)
+    end mid-graph, a 512^2 grid whose tiles take two V-cycles, a 1024^2 grid
+    whose tiles need five V-cycles but get four (the rest run after the
+    solve), and a velocity upload between chunks; the ledger is checked
+    every step."""
+    print(f"case fused scatter precision={precision} grid={grid} cycles={cycles}", flush=True)
+    cfg = ra.WorldConfig(grid, grid // 2)
     inp = make_inputs(cfg, 23, rock=0.12, rubble=True, vent_rows=4, vcells_per_step=1.2)
     worlds = []
     for fused, no_graphs in ((False, 0), (True, 0), (True, 1)):
         w = ra.World(cfg, 23)
         load(w, inp, parity_params(23, thermal_activation=1.0, diffusion=5.0),
-             parity_settings(mg_tol=1e-5, mg_max_cycles=60, precision=precision, no_graphs=no_graphs, ledger_check=1))
+             parity_settings(mg_tol=1e-5, mg_max_cycles=cycles, precision=precision, no_graphs=no_graphs, ledger_check=1))
         w.set_pipeline(fused)
         worlds.append(w)
     rng = np.random.default_rng(23)

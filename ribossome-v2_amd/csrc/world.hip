@@ -698,6 +698,7 @@ void light_step(RiboWorld* w, const RiboSimParams& P, uint32_t epoch, const uint
 // params.epoch + first + i, passed as is (eager) or as i on top of the
 // device epoch base `ep` (graph capture).
 void record_steps(RiboWorld* w, uint32_t first, uint32_t count, uint32_t substep, const uint32_t* ep) {
+    w->pend.on = false;  // a recording cut short by an error leaves nothing pending for the next
     if (w->pipe) {  // the transport stream starts after everything before
         (void)hipEventRecord(w->pe_fork, w->stream);
         (void)hipStreamWaitEvent(w->tstream, w->pe_fork, 0);

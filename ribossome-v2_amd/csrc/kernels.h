@@ -338,14 +338,18 @@ void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const u
                     uint32_t* next, const float2* vel, const float* temp, const uint32_t* agg_e);
 // The scatter's operands, for the launch that runs it beside the symmetric
 // V-cycle's one-workgroup levels: workgroup 0 is launch_mg_block's (sym),
-// the others take SC_FUSE_TILES of the scatter's 16 x 16 tiles each, tiles
+// the others take FuseCfg<S>::tiles of the scatter's 16 x 16 tiles each, tiles
 // [t0, t1) of the scatter_tiles(G) in row-major order.  Fits when the
 // V-cycle's LDS does not exceed the tiles'.
 struct ScatterArgs {
     Geo g; RiboSimParams P; const uint32_t* ep;
     const uint32_t* chem; uint32_t* next; const float2* vel; const float* temp; const uint32_t* agg_e;
 };
-constexpr uint32_t SC_FUSE_TILES = 4;
+// Tiles of a workgroup and workgroups a CU holds at once: binary32 levels
+// take 3 tiles (768 threads, 80 KB of LDS, two workgroups per CU); the
+// bfloat16 ones, whose V-cycle needs more registers, 4 (1024 threads, one).
+template <class S> struct FuseCfg { static constexpr uint32_t tiles = 3, per_cu = 2; };
+template <> struct FuseCfg<bf16> { static constexpr uint32_t tiles = 4, per_cu = 1; };
 uint32_t scatter_tiles(uint32_t G);
 template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count);
 // Tiles [t0, t1) of the scatter alone, one 256-thread workgroup a tile.

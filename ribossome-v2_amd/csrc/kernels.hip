@@ -1433,13 +1433,14 @@ __device__ void blk_coarse(const Team<TW, TH>& tm, const MgBlockArgs<S>& a, floa
     }
 }
 
-template <class S, bool SYM>
+// TH: rows of the 32-wide team (the workgroup has 32 TH threads).
+template <class S, bool SYM, int TH = 32>
 __device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gridDim, const MgBlockArgs<S>& a, const uint32_t* done,
                                            float* lds = nullptr) {
     if (done && *done) return;
     extern __shared__ float dyn_sm[];
     float* const sm = lds ? lds : dyn_sm;  // the launch's dynamic LDS, or the caller's
-    const Team<32, 32> blk((int)threadIdx.x);
+    const Team<32, TH> blk((int)threadIdx.x);
     const int total = a.off[a.levels];
     float* P = sm;
     float* RHS = sm + total;
@@ -1447,7 +1448,7 @@ __device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gri
     float* BV = reinterpret_cast<float*>(reinterpret_cast<uint8_t*>(sm) + ((9 * total + 3) & ~3));
     float* RN = BV + MG_SYM_COARSE_MAX;  // res / W of a fine level (SYM), n0^2 floats
     const int n0 = a.n[0];
-    for (int y = blk.ty; y < n0; y += 32)
+    for (int y = blk.ty; y < n0; y += TH)
         for (int x = blk.tx; x < n0; x += 32) {
             const int i = y * n0 + x;
             const uint8_t f = a.fluid[0][i];
@@ -1458,7 +1459,7 @@ __device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gri
         }
     for (int l = 1; l < a.levels; ++l) {
         const int n = a.n[l];
-        for (int y = blk.ty; y < n; y += 32)
+        for (int y = blk.ty; y < n; y += TH)
             for (int x = blk.tx; x < n; x += 32) FL[a.off[l] + y * n + x] = a.fluid[l][y * n + x];
     }
     // the coarsest level's unknowns: from the device when the inverse is
@@ -1476,7 +1477,7 @@ __device__ __forceinline__ void b_mg_block(const uint3 blockIdx, const uint3 gri
     for (int l = 0; l + 1 < L; ++l) blk_down<SYM>(blk, a, P, RHS, FL, RN, l);
     blk_coarse<SYM>(blk, a, P, RHS, FL, BV, ci, cm);
     for (int l = L - 2; l >= 0; --l) blk_up<SYM>(blk, a, P, RHS, FL, l);
-    for (int y = blk.ty; y < n0; y += 32)
+    for (int y = blk.ty; y < n0; y += TH)
         for (int x = blk.tx; x < n0; x += 32) {
             const int i = y * n0 + x;
             if (FL[i]) stv(a.p0, i, P[i]);
@@ -3398,26 +3399,26 @@ __global__ __launch_bounds__(256, 6) void k_scatter_range(ScatterArgs a, uint32_
 
 // The transport scatter beside the solver's one-workgroup coarse levels, in
 // one launch: workgroup 0 runs the symmetric V-cycle's k_mg_block, which
-// leaves every other CU idle; every other workgroup runs SC_FUSE_TILES
-// scatter tiles of 256 threads each, tiles t0 + (blockIdx.x - 1)
-// SC_FUSE_TILES + (threadIdx.x >> 8) up to t1 (a sub-tile past t1 takes a
-// tile past the grid's last row: no cells, only the barriers).  Neither part
-// reads what the other writes; the scatter runs whether or not the solve is
-// done.  The V-cycle's LDS overlays the tiles' (it is at most
-// SC_FUSE_TILES sizeof(ScLds) bytes).
+// leaves every other CU idle; every other workgroup runs NT = FuseCfg<S>::tiles
+// scatter tiles of 256 threads each, tiles t0 + (blockIdx.x - 1) NT +
+// (threadIdx.x >> 8) up to t1 (a sub-tile past t1 takes a tile past the
+// grid's last row: no cells, only the barriers).  Neither part reads what
+// the other writes; the scatter runs whether or not the solve is done.  The
+// V-cycle's LDS overlays the tiles' (it is at most NT sizeof(ScLds) bytes).
 template <class S>
-__global__ __launch_bounds__(1024) void k_mg_block_scatter(MgBlockArgs<S> a, const uint32_t* done, ScatterArgs sc, uint32_t t0,
-                                                           uint32_t t1) {
-    __shared__ ScLds lds[SC_FUSE_TILES];
+__global__ __launch_bounds__(FuseCfg<S>::tiles * 256, FuseCfg<S>::tiles * FuseCfg<S>::per_cu) void k_mg_block_scatter(
+    MgBlockArgs<S> a, const uint32_t* done, ScatterArgs sc, uint32_t t0, uint32_t t1) {
+    constexpr uint32_t NT = FuseCfg<S>::tiles;
+    __shared__ ScLds lds[NT];
     if (blockIdx.x == 0) {
-        b_mg_block<S, true>(make_uint3(0u, 0u, 0u), make_uint3(1u, 1u, 1u), a, done, reinterpret_cast<float*>(lds));
+        b_mg_block<S, true, NT * 8>(make_uint3(0u, 0u, 0u), make_uint3(1u, 1u, 1u), a, done, reinterpret_cast<float*>(lds));
         return;
     }
     const uint32_t sub = threadIdx.x >> 8, t = threadIdx.x & 255u;
     const uint32_t tiles_x = (sc.g.G + 15u) / 16u;
-    const uint32_t tile = t0 + (blockIdx.x - 1u) * SC_FUSE_TILES + sub;
+    const uint32_t tile = t0 + (blockIdx.x - 1u) * NT + sub;
     const uint3 tb = tile < t1 ? make_uint3(tile % tiles_x, tile / tiles_x, 0u) : make_uint3(0u, tiles_x, 0u);
-    b_scatter<SC_FUSE_TILES>(lds, sub, t & 15u, t >> 4, tb, sc.g, sc.P, sc.ep, sc.chem, sc.next, sc.vel, sc.temp, sc.agg_e);
+    b_scatter<NT>(lds, sub, t & 15u, t >> 4, tb, sc.g, sc.P, sc.ep, sc.chem, sc.next, sc.vel, sc.temp, sc.agg_e);
 }
 
 // ledger_reduce (ledger.wgsl:5-27): exact u32 sums per channel and state
@@ -4034,15 +4035,16 @@ uint32_t scatter_tiles(uint32_t G) { return ((G + 15u) / 16u) * ((G + 15u) / 16u
 template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count) {
     size_t lds = 0;
     (void)mg_block_args<S>(lv, count, nullptr, nullptr, 0, &lds, nullptr);
-    return lds <= SC_FUSE_TILES * sizeof(ScLds);
+    return lds <= FuseCfg<S>::tiles * sizeof(ScLds);
 }
 template <class S>
 void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
                              const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1) {
     size_t lds = 0;
     const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds, cm_dev);
-    const uint32_t wgs = 1u + (t1 - t0 + SC_FUSE_TILES - 1u) / SC_FUSE_TILES;
-    k_mg_block_scatter<S><<<wgs, 1024, 0, s>>>(a, done, sc, t0, t1);
+    constexpr uint32_t NT = FuseCfg<S>::tiles;
+    const uint32_t wgs = 1u + (t1 - t0 + NT - 1u) / NT;
+    k_mg_block_scatter<S><<<wgs, NT * 256, 0, s>>>(a, done, sc, t0, t1);
 }
 void launch_scatter_range(hipStream_t s, const ScatterArgs& a, uint32_t t0, uint32_t t1) {
     if (t1 > t0) k_scatter_range<<<t1 - t0, B16, 0, s>>>(a, t0);

@@ -399,13 +399,15 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
 
 // Tiles [t0, t1) of the pending scatter that V-cycle j of n carries: the
 // tiles spread evenly over as few slices as fit the CUs beside k_mg_block
-// (one workgroup each, SC_FUSE_TILES tiles a workgroup, so that every
-// workgroup of a launch runs at once).  With fewer V-cycles than that, each
+// (FuseCfg: tiles a workgroup, workgroups a CU, so that every workgroup of a
+// launch runs at once).  With fewer V-cycles than that, each
 // takes a full slice and the tiles from fused_rest(n) on run on their own
 // after the solve.
 uint32_t fused_per(const RiboWorld* w, uint32_t n, uint32_t* slices) {
     const uint32_t tiles = scatter_tiles(w->G);
-    const uint32_t cap = (w->cu_count - 1u) * SC_FUSE_TILES;
+    const bool half = w->st.precision == RIBO_PRECISION_MIXED;
+    const uint32_t per_cu = half ? FuseCfg<bf16>::per_cu : FuseCfg<float>::per_cu;
+    const uint32_t cap = (per_cu * w->cu_count - 1u) * (half ? FuseCfg<bf16>::tiles : FuseCfg<float>::tiles);
     const uint32_t need = (tiles + cap - 1u) / cap;
     *slices = std::max(std::min(need, n), 1u);
     return *slices == need ? (tiles + need - 1u) / need : cap;

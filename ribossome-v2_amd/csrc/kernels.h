@@ -356,7 +356,8 @@ template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count);
 void launch_scatter_range(hipStream_t s, const ScatterArgs& a, uint32_t t0, uint32_t t1);
 template <class S>
 void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
-                             const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1);
+                             const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1,
+                             uint32_t cus);
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check);
 void launch_epoch_add(hipStream_t s, uint32_t* ep, uint32_t n);

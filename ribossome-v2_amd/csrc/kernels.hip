@@ -3398,25 +3398,25 @@ __global__ __launch_bounds__(256, 6) void k_scatter_range(ScatterArgs a, uint32_
 }
 
 // The transport scatter beside the solver's one-workgroup coarse levels, in
-// one launch: workgroup 0 runs the symmetric V-cycle's k_mg_block, which
-// leaves every other CU idle; every other workgroup runs NT = FuseCfg<S>::tiles
-// scatter tiles of 256 threads each, tiles t0 + (blockIdx.x - 1) NT +
-// (threadIdx.x >> 8) up to t1 (a sub-tile past t1 takes a tile past the
+// one launch: workgroup mgw runs the symmetric V-cycle's k_mg_block, which
+// leaves every other CU idle; every other workgroup (w-th in order) runs
+// NT = FuseCfg<S>::tiles scatter tiles of 256 threads each, tiles t0 + w NT
+// + (threadIdx.x >> 8) up to t1 (a sub-tile past t1 takes a tile past the
 // grid's last row: no cells, only the barriers).  Neither part reads what
 // the other writes; the scatter runs whether or not the solve is done.  The
 // V-cycle's LDS overlays the tiles' (it is at most NT sizeof(ScLds) bytes).
 template <class S>
 __global__ __launch_bounds__(FuseCfg<S>::tiles * 256, FuseCfg<S>::tiles * FuseCfg<S>::per_cu) void k_mg_block_scatter(
-    MgBlockArgs<S> a, const uint32_t* done, ScatterArgs sc, uint32_t t0, uint32_t t1) {
+    MgBlockArgs<S> a, const uint32_t* done, ScatterArgs sc, uint32_t t0, uint32_t t1, uint32_t mgw) {
     constexpr uint32_t NT = FuseCfg<S>::tiles;
     __shared__ ScLds lds[NT];
-    if (blockIdx.x == 0) {
+    if (blockIdx.x == mgw) {
         b_mg_block<S, true, NT * 8>(make_uint3(0u, 0u, 0u), make_uint3(1u, 1u, 1u), a, done, reinterpret_cast<float*>(lds));
         return;
     }
     const uint32_t sub = threadIdx.x >> 8, t = threadIdx.x & 255u;
     const uint32_t tiles_x = (sc.g.G + 15u) / 16u;
-    const uint32_t tile = t0 + (blockIdx.x - 1u) * NT + sub;
+    const uint32_t tile = t0 + (blockIdx.x < mgw ? blockIdx.x : blockIdx.x - 1u) * NT + sub;
     const uint3 tb = tile < t1 ? make_uint3(tile % tiles_x, tile / tiles_x, 0u) : make_uint3(0u, tiles_x, 0u);
     b_scatter<NT>(lds, sub, t & 15u, t >> 4, tb, sc.g, sc.P, sc.ep, sc.chem, sc.next, sc.vel, sc.temp, sc.agg_e);
 }
@@ -4039,12 +4039,17 @@ template <class S> bool mg_block_scatter_fits(const MgLevelT<S>* lv, int count) 
 }
 template <class S>
 void launch_mg_block_scatter(hipStream_t s, const MgLevelT<S>* lv, int count, const double* cinv, const int32_t* ccell, int cm,
-                             const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1) {
+                             const uint32_t* done, const int* cm_dev, const ScatterArgs& sc, uint32_t t0, uint32_t t1,
+                             uint32_t cus) {
     size_t lds = 0;
     const MgBlockArgs<S> a = mg_block_args<S>(lv, count, cinv, ccell, cm, &lds, cm_dev);
     constexpr uint32_t NT = FuseCfg<S>::tiles;
     const uint32_t wgs = 1u + (t1 - t0 + NT - 1u) / NT;
-    k_mg_block_scatter<S><<<wgs, NT * 256, 0, s>>>(a, done, sc, t0, t1);
+    // with two workgroups to a CU, the V-cycle's is the last of the first
+    // pass over the CUs, so that (with no more than 2 cus - 1 workgroups)
+    // its CU takes no second one
+    const uint32_t mgw = FuseCfg<S>::per_cu > 1 ? std::min(wgs, cus) - 1u : 0u;
+    k_mg_block_scatter<S><<<wgs, NT * 256, 0, s>>>(a, done, sc, t0, t1, mgw);
 }
 void launch_scatter_range(hipStream_t s, const ScatterArgs& a, uint32_t t0, uint32_t t1) {
     if (t1 > t0) k_scatter_range<<<t1 - t0, B16, 0, s>>>(a, t0);
@@ -4052,9 +4057,9 @@ void launch_scatter_range(hipStream_t s, const ScatterArgs& a, uint32_t t0, uint
 template bool mg_block_scatter_fits<float>(const MgLevelT<float>*, int);
 template bool mg_block_scatter_fits<bf16>(const MgLevelT<bf16>*, int);
 template void launch_mg_block_scatter<float>(hipStream_t, const MgLevelT<float>*, int, const double*, const int32_t*, int,
-                                             const uint32_t*, const int*, const ScatterArgs&, uint32_t, uint32_t);
+                                             const uint32_t*, const int*, const ScatterArgs&, uint32_t, uint32_t, uint32_t);
 template void launch_mg_block_scatter<bf16>(hipStream_t, const MgLevelT<bf16>*, int, const double*, const int32_t*, int,
-                                            const uint32_t*, const int*, const ScatterArgs&, uint32_t, uint32_t);
+                                            const uint32_t*, const int*, const ScatterArgs&, uint32_t, uint32_t, uint32_t);
 void launch_ledger(hipStream_t s, const Geo& g, const uint32_t* chem, LedgerDev* led, uint32_t epoch, const uint32_t* ep,
                    int check) {
     uint32_t cells = g.G * g.G;

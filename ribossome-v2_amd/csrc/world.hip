@@ -399,15 +399,16 @@ void vcycle(RiboWorld* w, const uint32_t* done) {
 
 // Tiles [t0, t1) of the pending scatter that V-cycle j of n carries: the
 // tiles spread evenly over as few slices as fit the CUs beside k_mg_block
-// (FuseCfg: tiles a workgroup, workgroups a CU, so that every workgroup of a
-// launch runs at once).  With fewer V-cycles than that, each
+// (FuseCfg: tiles a workgroup, workgroups a CU; every workgroup of a launch
+// runs at once, and the CU of k_mg_block's holds no scatter workgroup beside
+// it).  With fewer V-cycles than that, each
 // takes a full slice and the tiles from fused_rest(n) on run on their own
 // after the solve.
 uint32_t fused_per(const RiboWorld* w, uint32_t n, uint32_t* slices) {
     const uint32_t tiles = scatter_tiles(w->G);
     const bool half = w->st.precision == RIBO_PRECISION_MIXED;
     const uint32_t per_cu = half ? FuseCfg<bf16>::per_cu : FuseCfg<float>::per_cu;
-    const uint32_t cap = (per_cu * w->cu_count - 1u) * (half ? FuseCfg<bf16>::tiles : FuseCfg<float>::tiles);
+    const uint32_t cap = (per_cu * w->cu_count - per_cu) * (half ? FuseCfg<bf16>::tiles : FuseCfg<float>::tiles);
     const uint32_t need = (tiles + cap - 1u) / cap;
     *slices = std::max(std::min(need, n), 1u);
     return *slices == need ? (tiles + need - 1u) / need : cap;
@@ -444,7 +445,8 @@ void vcycle_sym_inner(RiboWorld* w, const std::vector<MgLevelT<S>>& lv, const ui
             if (t1 > t0) {
                 const RiboWorld::PendingScatter& q = w->pend;
                 const ScatterArgs sa{w->geo(), q.P, q.ep, q.chem, q.next, w->vel_s, q.temp, w->agg_e};
-                launch_mg_block_scatter<S>(s, lv.data() + b, (int)(lc - b + 1), w->cinv, w->ccell, w->cm, done, cmd, sa, t0, t1);
+                launch_mg_block_scatter<S>(s, lv.data() + b, (int)(lc - b + 1), w->cinv, w->ccell, w->cm, done, cmd, sa, t0, t1,
+                                           w->cu_count);
             } else {
                 launch_mg_block<S>(s, lv.data() + b, (int)(lc - b + 1), true, w->cinv, w->ccell, w->cm, done, cmd);
             }

@@ -337,10 +337,11 @@ void launch_fill_u32(hipStream_t s, uint32_t* p, size_t n, uint32_t v);
 void launch_scatter(hipStream_t s, const Geo& g, const RiboSimParams& P, const uint32_t* ep, const uint32_t* chem,
                     uint32_t* next, const float2* vel, const float* temp, const uint32_t* agg_e);
 // The scatter's operands, for the launch that runs it beside the symmetric
-// V-cycle's one-workgroup levels: workgroup 0 is launch_mg_block's (sym),
-// the others take FuseCfg<S>::tiles of the scatter's 16 x 16 tiles each, tiles
-// [t0, t1) of the scatter_tiles(G) in row-major order.  Fits when the
-// V-cycle's LDS does not exceed the tiles'.
+// V-cycle's one-workgroup levels: one workgroup is launch_mg_block's (sym;
+// with two workgroups to a CU, the last of the first pass over the device's
+// `cus` CUs), the others take FuseCfg<S>::tiles of the scatter's 16 x 16
+// tiles each, tiles [t0, t1) of the scatter_tiles(G) in row-major order.
+// Fits when the V-cycle's LDS does not exceed the tiles'.
 struct ScatterArgs {
     Geo g; RiboSimParams P; const uint32_t* ep;
     const uint32_t* chem; uint32_t* next; const float2* vel; const float* temp; const uint32_t* agg_e;
